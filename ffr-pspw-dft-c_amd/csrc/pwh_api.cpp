@@ -1,0 +1,985 @@
+// pwh_api.cpp -- host orchestration and the C ABI of libpwh.
+//
+// One library stream carries every launch.  A pwh_system holds, in HBM, the sphere
+// maps, the phases/projectors in both FP64 and FP32, the local potential, and
+// resident work buffers that grow on demand; nothing is allocated per call once a
+// system has seen its largest block.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/pwh.h"
+#include "pwh_blas.h"
+#include "pwh_internal.h"
+
+using namespace pwh;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(PWH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+#define RBCHK(expr)                                                                     \
+  do {                                                                                  \
+    rocblas_status s_ = (expr);                                                         \
+    if (s_ != rocblas_status_success)                                                   \
+      return fail(PWH_ERR_HIP, std::string(#expr) + ": rocblas status " + std::to_string((int)s_)); \
+  } while (0)
+#define CHK(expr)                 \
+  do {                            \
+    int r_ = (expr);              \
+    if (r_ != PWH_OK) return r_;  \
+  } while (0)
+
+enum TimerId { TM_ZINV, TM_PLANE, TM_ZFWD, TM_PROJ, TM_EXPAND, TM_HPSI, TM_COUNT };
+const char* kTimerNames[TM_COUNT] = {"zinv", "plane", "zfwd", "proj", "expand", "hpsi"};
+
+struct Ctx {
+  bool ready = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle rb = nullptr;
+  bool timing = false;
+  struct Rec { int id; hipEvent_t a, b; };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  double total_ms[TM_COUNT] = {0};
+  int count[TM_COUNT] = {0};
+  // workspace for operations without a system (ortho_qr)
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+};
+Ctx g;
+
+hipEvent_t take_event() {
+  if (!g.pool.empty()) {
+    hipEvent_t e = g.pool.back();
+    g.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+
+// Records HIP events around a launch on the library stream when timing is enabled.
+struct Timed {
+  int id;
+  hipEvent_t a = nullptr;
+  explicit Timed(int i) : id(i) {
+    if (g.timing) {
+      a = take_event();
+      hipEventRecord(a, g.stream);
+    }
+  }
+  ~Timed() {
+    if (g.timing && a) {
+      hipEvent_t b = take_event();
+      hipEventRecord(b, g.stream);
+      g.pending.push_back({id, a, b});
+    }
+  }
+};
+
+void resolve_timers() {
+  for (auto& r : g.pending) {
+    hipEventSynchronize(r.b);
+    float ms = 0;
+    hipEventElapsedTime(&ms, r.a, r.b);
+    g.total_ms[r.id] += ms;
+    g.count[r.id] += 1;
+    g.pool.push_back(r.a);
+    g.pool.push_back(r.b);
+  }
+  g.pending.clear();
+}
+
+int ensure_init() {
+  if (g.ready) return PWH_OK;
+  return pwh_init(0);
+}
+
+// Device buffer that grows on demand.  Growing waits for the library stream first so
+// no queued kernel still reads the old allocation.
+template <typename T>
+struct Buf {
+  T* p = nullptr;
+  size_t cap = 0;
+  Buf() = default;
+  Buf(const Buf&) = delete;
+  Buf& operator=(const Buf&) = delete;
+  ~Buf() { release(); }
+  int ensure(size_t n) {
+    if (n <= cap) return PWH_OK;
+    if (p) {
+      if (g.stream) hipStreamSynchronize(g.stream);
+      hipFree(p);
+    }
+    p = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1));
+    if (e != hipSuccess) return fail(PWH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    hipMemsetAsync(p, 0, sizeof(T) * std::max<size_t>(n, 1), g.stream);
+    cap = n;
+    return PWH_OK;
+  }
+  void release() {
+    if (p) {
+      if (g.stream) hipStreamSynchronize(g.stream);
+      hipFree(p);
+    }
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+template <typename T>
+int upload(Buf<T>& b, const T* h, size_t n) {
+  CHK(b.ensure(n));
+  HIPCHK(hipMemcpyAsync(b.p, h, sizeof(T) * n, hipMemcpyHostToDevice, g.stream));
+  return PWH_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// The system
+// ---------------------------------------------------------------------------
+struct pwh_system {
+  int nr[3] = {0, 0, 0};
+  int nnr = 0, ngw = 0, ngw_pad = 0;
+  double tpiba2 = 0;
+  SphereMaps maps{};
+  std::vector<int> perm;
+  Buf<int> d_perm, d_colstart, d_kcol, d_kz, d_colxy, d_xa;
+  int nsp = 0, nat = 0;
+  std::vector<int> na, lm, aoff;
+  std::vector<double> wl;   // [nsp*4]
+  // constants, both precisions
+  Buf<cx<double>> E64;
+  Buf<double> P64, kin64, wl64, V64;
+  Buf<cx<float>> E32;
+  Buf<float> P32, kin32, wl32, V32;
+  bool vset = false;
+  Buf<cx<double>> tw64[3];
+  Buf<cx<float>> tw32[3];
+  // FP64 work
+  Buf<cx<double>> X, HX, A64, B64, C64, W64, Hnl64, proj64, Q64, part64, stage, S, Vl;
+  Buf<double> evals, rwork, dres;
+  Buf<rocblas_int> info;
+  // FP32 work
+  Buf<cx<float>> X32, Y32, Y1_32, W32, Hnl32, proj32, Q32, part32;
+  int resident_nb = 0;
+};
+
+namespace {
+
+template <typename T> struct Sel;
+template <> struct Sel<double> {
+  static cx<double>* E(pwh_system* s) { return s->E64.p; }
+  static double* P(pwh_system* s) { return s->P64.p; }
+  static double* kin(pwh_system* s) { return s->kin64.p; }
+  static double* wl(pwh_system* s) { return s->wl64.p; }
+  static double* V(pwh_system* s) { return s->V64.p; }
+  static void* tw(pwh_system* s, int d) { return s->tw64[d].p; }
+  static Buf<cx<double>>& W(pwh_system* s) { return s->W64; }
+  static Buf<cx<double>>& Hnl(pwh_system* s) { return s->Hnl64; }
+  static Buf<cx<double>>& proj(pwh_system* s) { return s->proj64; }
+  static Buf<cx<double>>& Q(pwh_system* s) { return s->Q64; }
+  static Buf<cx<double>>& part(pwh_system* s) { return s->part64; }
+};
+template <> struct Sel<float> {
+  static cx<float>* E(pwh_system* s) { return s->E32.p; }
+  static float* P(pwh_system* s) { return s->P32.p; }
+  static float* kin(pwh_system* s) { return s->kin32.p; }
+  static float* wl(pwh_system* s) { return s->wl32.p; }
+  static float* V(pwh_system* s) { return s->V32.p; }
+  static void* tw(pwh_system* s, int d) { return s->tw32[d].p; }
+  static Buf<cx<float>>& W(pwh_system* s) { return s->W32; }
+  static Buf<cx<float>>& Hnl(pwh_system* s) { return s->Hnl32; }
+  static Buf<cx<float>>& proj(pwh_system* s) { return s->proj32; }
+  static Buf<cx<float>>& Q(pwh_system* s) { return s->Q32; }
+  static Buf<cx<float>>& part(pwh_system* s) { return s->part32; }
+};
+
+// out = alpha * H X + beta * X + gamma * Z  on a block of nb bands in sphere order.
+template <typename T>
+int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double beta,
+         double gamma, const cx<T>* Z, int mode) {
+  if (!s->vset) return fail(PWH_ERR_ARG, "local potential not set (pwh_set_vloc)");
+  Timed th(TM_HPSI);
+  hipStream_t st = g.stream;
+  const int ld = s->ngw_pad;
+  CHK(Sel<T>::W(s).ensure((size_t)nb * s->nr[2] * s->maps.ncol_pad));
+  bool have_nl = false;
+  for (int sp = 0; sp < s->nsp; ++sp) {
+    if (s->na[sp] == 0 || s->lm[sp] == 0) continue;
+    const int M = s->na[sp] * s->lm[sp];
+    CHK(Sel<T>::Hnl(s).ensure((size_t)nb * ld));
+    CHK(Sel<T>::proj(s).ensure((size_t)M * nb));
+    CHK(Sel<T>::Q(s).ensure((size_t)M * nb));
+    const size_t pe = proj_part_elems(M, nb, s->ngw);
+    CHK(Sel<T>::part(s).ensure(pe));
+    BetaGen bg{Sel<T>::E(s), Sel<T>::P(s) + (size_t)sp * 4 * ld, s->aoff[sp], s->na[sp], s->lm[sp], ld};
+    {
+      Timed t(TM_PROJ);
+      HIPCHK(gemm_proj<T>(bg, X, ld, nb, s->ngw, Sel<T>::proj(s).p, Sel<T>::part(s).p, pe, mode, st));
+    }
+    HIPCHK(scale_proj<T>(Sel<T>::proj(s).p, Sel<T>::wl(s) + 4 * sp, s->na[sp], s->lm[sp], nb,
+                         Sel<T>::Q(s).p, st));
+    {
+      Timed t(TM_EXPAND);
+      HIPCHK(gemm_expand<T>(bg, Sel<T>::Q(s).p, nb, s->ngw, Sel<T>::Hnl(s).p, ld, have_nl, mode, st));
+    }
+    have_nl = true;
+  }
+  {
+    Timed t(TM_ZINV);
+    HIPCHK(launch_zinv<T>(s->maps, Sel<T>::tw(s, 2), X, ld, Sel<T>::W(s).p, nb, st));
+  }
+  {
+    Timed t(TM_PLANE);
+    HIPCHK(launch_plane<T>(s->maps, Sel<T>::tw(s, 0), Sel<T>::tw(s, 1), Sel<T>::V(s), Sel<T>::W(s).p,
+                           nb, st));
+  }
+  {
+    Timed t(TM_ZFWD);
+    HIPCHK(launch_zfwd_epilogue<T>(s->maps, Sel<T>::tw(s, 2), Sel<T>::W(s).p, Sel<T>::kin(s), X,
+                                   have_nl ? Sel<T>::Hnl(s).p : nullptr, Z, out, ld, nb,
+                                   (T)(1.0 / s->nnr), (T)alpha, (T)beta, (T)gamma, st));
+  }
+  return PWH_OK;
+}
+
+// chebyfilt.cpp:6-74 on device buffers b0 (= X in), b1, b2 (work); *result -> filtered block
+template <typename T>
+int chebyfilt_dev(pwh_system* s, cx<T>* b0, cx<T>* b1, cx<T>* b2, int nb, int degree, double lb,
+                  double ub, int mode, cx<T>** result) {
+  const double e = (ub - lb) / 2.0;
+  const double c = (ub + lb) / 2.0;
+  double sigma = e / (lb - ub);
+  const double sigma1 = sigma;
+  cx<T>* X = b0;
+  cx<T>* Y = b1;
+  cx<T>* Y1 = b2;
+  CHK(hpsi<T>(s, X, Y, nb, sigma1 / e, -c * sigma1 / e, 0.0, nullptr, mode));
+  for (int id = 2; id <= degree; ++id) {
+    const double sigma2 = 1.0 / (2.0 / sigma1 - sigma);
+    CHK(hpsi<T>(s, Y, Y1, nb, 2.0 * sigma2 / e, -c * 2.0 * sigma2 / e, -sigma * sigma2, X, mode));
+    cx<T>* t = X;
+    X = Y;
+    Y = Y1;
+    Y1 = t;
+    sigma = sigma2;
+  }
+  *result = Y;
+  return PWH_OK;
+}
+
+int check_ld(pwh_system* s, int nb) {
+  if (!s) return fail(PWH_ERR_ARG, "null system");
+  if (nb < 1) return fail(PWH_ERR_ARG, "nstates < 1");
+  return PWH_OK;
+}
+
+// X (ref order, host, [ngw x nb]) -> s->X (sphere order, FP64)
+int load_block(pwh_system* s, const double* Xh, int nb) {
+  const size_t nref = (size_t)s->ngw * nb;
+  CHK(upload(s->stage, reinterpret_cast<const cx<double>*>(Xh), nref));
+  CHK(s->X.ensure((size_t)nb * s->ngw_pad));
+  HIPCHK(permute_in_f64(s->stage.p, s->ngw, s->d_perm.p, s->ngw, s->ngw_pad, s->X.p, nb, g.stream));
+  return PWH_OK;
+}
+
+int store_block(pwh_system* s, const cx<double>* Xd, double* Xh, int nb) {
+  const size_t nref = (size_t)s->ngw * nb;
+  CHK(s->stage.ensure(nref));
+  HIPCHK(permute_out_f64(Xd, s->ngw_pad, s->d_perm.p, s->ngw, s->stage.p, s->ngw, nb, g.stream));
+  HIPCHK(hipMemcpyAsync(Xh, s->stage.p, sizeof(cx<double>) * nref, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return PWH_OK;
+}
+
+// H X in the given precision for the FP64 sphere-order block X -> out (FP64)
+int apply_prec(pwh_system* s, const cx<double>* X, cx<double>* out, int nb, int precision) {
+  if (precision == PWH_FP64) return hpsi<double>(s, X, out, nb, 1.0, 0.0, 0.0, nullptr, PREC_FP64);
+  const size_t n = (size_t)nb * s->ngw_pad;
+  CHK(s->X32.ensure(n));
+  CHK(s->Y32.ensure(n));
+  HIPCHK((convert_cx<double, float>(X, s->X32.p, n, g.stream)));
+  CHK(hpsi<float>(s, s->X32.p, s->Y32.p, nb, 1.0, 0.0, 0.0, nullptr, precision));
+  HIPCHK((convert_cx<float, double>(s->Y32.p, out, n, g.stream)));
+  return PWH_OK;
+}
+
+// Chebyshev filter of the FP64 sphere-order block X in place.
+int filter_prec(pwh_system* s, cx<double>* X, int nb, int degree, double lb, double ub,
+                int precision) {
+  const size_t n = (size_t)nb * s->ngw_pad;
+  if (degree < 1) return PWH_OK;
+  if (precision == PWH_FP64) {
+    CHK(s->A64.ensure(n));
+    CHK(s->B64.ensure(n));
+    cx<double>* res = nullptr;
+    CHK(chebyfilt_dev<double>(s, X, s->A64.p, s->B64.p, nb, degree, lb, ub, PREC_FP64, &res));
+    if (res != X) HIPCHK(hipMemcpyAsync(X, res, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, g.stream));
+    return PWH_OK;
+  }
+  CHK(s->X32.ensure(n));
+  CHK(s->Y32.ensure(n));
+  CHK(s->Y1_32.ensure(n));
+  HIPCHK((convert_cx<double, float>(X, s->X32.p, n, g.stream)));
+  cx<float>* res = nullptr;
+  CHK(chebyfilt_dev<float>(s, s->X32.p, s->Y32.p, s->Y1_32.p, nb, degree, lb, ub, precision, &res));
+  HIPCHK((convert_cx<float, double>(res, X, n, g.stream)));
+  return PWH_OK;
+}
+
+// CholeskyQR2 (FP64) of the nb columns of X [nb][ld] (nrows valid rows), using tmp [nb][ld].
+int cholqr2(cx<double>* X, int ld, int nrows, int nb, cx<double>* tmp, cx<double>* S,
+            cx<double>* part, size_t part_elems, rocblas_int* dinfo) {
+  for (int pass = 0; pass < 2; ++pass) {
+    cx<double>* src = (pass == 0) ? X : tmp;
+    cx<double>* dst = (pass == 0) ? tmp : X;
+    HIPCHK(gemm_cn_f64(src, ld, nb, src, ld, nb, nrows, S, nb, part, part_elems, g.stream));
+    RBCHK(rocsolver_zpotrf(g.rb, rocblas_fill_upper, nb, reinterpret_cast<rocblas_double_complex*>(S),
+                           nb, dinfo));
+    rocblas_int hinfo = 0;
+    HIPCHK(hipMemcpyAsync(&hinfo, dinfo, sizeof(hinfo), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (hinfo != 0)
+      return fail(PWH_ERR_LAPACK, "CholeskyQR: overlap matrix not positive definite (zpotrf info " +
+                                      std::to_string(hinfo) + ")");
+    RBCHK(rocsolver_ztrtri(g.rb, rocblas_fill_upper, rocblas_diagonal_non_unit, nb,
+                           reinterpret_cast<rocblas_double_complex*>(S), nb, dinfo));
+    HIPCHK(zero_lower(S, nb, nb, g.stream));
+    HIPCHK(gemm_nn_f64(src, ld, nb, S, nb, nb, nrows, dst, ld, g.stream));
+  }
+  return PWH_OK;
+}
+
+int sys_cholqr2(pwh_system* s, cx<double>* X, int nb) {
+  const size_t n = (size_t)nb * s->ngw_pad;
+  CHK(s->A64.ensure(n));
+  CHK(s->S.ensure((size_t)nb * nb));
+  const size_t pe = proj_part_elems(nb, nb, s->ngw);
+  CHK(s->part64.ensure(pe));
+  CHK(s->info.ensure(1));
+  return cholqr2(X, s->ngw_pad, s->ngw, nb, s->A64.p, s->S.p, s->part64.p, pe, s->info.p);
+}
+
+// Rayleigh-Ritz (ChebySCFLoop.cpp:292-306) on the orthonormal block X: X <- X Q, HX <- (H X) Q,
+// evals (device) ascending.
+int rayleigh_ritz_dev(pwh_system* s, cx<double>* X, int nb) {
+  const int ld = s->ngw_pad;
+  const size_t n = (size_t)nb * ld;
+  CHK(s->HX.ensure(n));
+  CHK(s->B64.ensure(n));
+  CHK(s->S.ensure((size_t)nb * nb));
+  CHK(s->evals.ensure(nb));
+  CHK(s->rwork.ensure(nb));
+  CHK(s->info.ensure(1));
+  const size_t pe = proj_part_elems(nb, nb, s->ngw);
+  CHK(s->part64.ensure(pe));
+  CHK(hpsi<double>(s, X, s->HX.p, nb, 1.0, 0.0, 0.0, nullptr, PREC_FP64));
+  HIPCHK(gemm_cn_f64(X, ld, nb, s->HX.p, ld, nb, s->ngw, s->S.p, nb, s->part64.p, pe, g.stream));
+  HIPCHK(symmetrize(s->S.p, nb, nb, g.stream));
+  RBCHK(rocsolver_zheevd(g.rb, rocblas_evect_original, rocblas_fill_upper, nb,
+                         reinterpret_cast<rocblas_double_complex*>(s->S.p), nb, s->evals.p,
+                         s->rwork.p, s->info.p));
+  rocblas_int hinfo = 0;
+  HIPCHK(hipMemcpyAsync(&hinfo, s->info.p, sizeof(hinfo), hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  if (hinfo != 0) return fail(PWH_ERR_LAPACK, "zheevd info " + std::to_string(hinfo));
+  HIPCHK(gemm_nn_f64(X, ld, nb, s->S.p, nb, nb, s->ngw, s->B64.p, ld, g.stream));
+  HIPCHK(hipMemcpyAsync(X, s->B64.p, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, g.stream));
+  HIPCHK(gemm_nn_f64(s->HX.p, ld, nb, s->S.p, nb, nb, s->ngw, s->B64.p, ld, g.stream));
+  HIPCHK(hipMemcpyAsync(s->HX.p, s->B64.p, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, g.stream));
+  return PWH_OK;
+}
+
+// Residual report (ChebySCFLoop.cpp:415-433) for X: HX, G = sym(X^H HX), R = HX - X G.
+int residuals_dev(pwh_system* s, const cx<double>* X, int nb, double* res_host) {
+  const int ld = s->ngw_pad;
+  const size_t n = (size_t)nb * ld;
+  CHK(s->HX.ensure(n));
+  CHK(s->B64.ensure(n));
+  CHK(s->S.ensure((size_t)nb * nb));
+  CHK(s->dres.ensure(nb));
+  const size_t pe = proj_part_elems(nb, nb, s->ngw);
+  CHK(s->part64.ensure(pe));
+  CHK(hpsi<double>(s, X, s->HX.p, nb, 1.0, 0.0, 0.0, nullptr, PREC_FP64));
+  HIPCHK(gemm_cn_f64(X, ld, nb, s->HX.p, ld, nb, s->ngw, s->S.p, nb, s->part64.p, pe, g.stream));
+  HIPCHK(symmetrize(s->S.p, nb, nb, g.stream));
+  HIPCHK(gemm_nn_f64(X, ld, nb, s->S.p, nb, nb, s->ngw, s->B64.p, ld, g.stream));
+  HIPCHK(resid_norms(s->HX.p, s->B64.p, nullptr, ld, s->ngw, nb, s->dres.p, g.stream));
+  HIPCHK(hipMemcpyAsync(res_host, s->dres.p, sizeof(double) * nb, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return PWH_OK;
+}
+
+// lanczos.cpp:8-101 on the sphere-order vector v0 (device, normalised in place).
+// T (host, ldT x k column-major), fnorm2 = ||f||^2, f (device) the final residual.
+int lanczos_dev(pwh_system* s, cx<double>* v0, int k, double* T, int ldT, cx<double>* f,
+                double* fnorm2) {
+  const int ld = s->ngw_pad, n = s->ngw;
+  CHK(s->Vl.ensure((size_t)k * ld));
+  CHK(s->C64.ensure((size_t)ld));          // Hv
+  CHK(s->S.ensure((size_t)2 * k + 2));     // h, s, coefficients
+  cx<double>* V = s->Vl.p;
+  cx<double>* Hv = s->C64.p;
+  cx<double>* dh = s->S.p;
+  cx<double>* dcoef = s->S.p + k + 1;
+  std::vector<cx<double>> h(k), sv(k), coef(k);
+  for (int i = 0; i < ldT * k; ++i) T[i] = 0.0;
+  auto dot = [&](const cx<double>* x, const cx<double>* y, cx<double>* out) -> int {
+    HIPCHK(multi_dotc(x, ld, 1, y, n, dh, g.stream));
+    HIPCHK(hipMemcpyAsync(out, dh, sizeof(cx<double>), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return PWH_OK;
+  };
+  cx<double> tz;
+  CHK(dot(v0, v0, &tz));
+  double beta = 1.0 / std::sqrt(tz.x);
+  HIPCHK(axpby(v0, beta, nullptr, 0.0, v0, n, g.stream));
+  HIPCHK(hipMemcpyAsync(V, v0, sizeof(cx<double>) * ld, hipMemcpyDeviceToDevice, g.stream));
+  CHK(hpsi<double>(s, V, Hv, 1, 1.0, 0.0, 0.0, nullptr, PREC_FP64));
+  CHK(dot(V, Hv, &tz));
+  h[0] = tz;
+  T[0] = tz.x;
+  HIPCHK(axpby(Hv, 1.0, nullptr, 0.0, f, n, g.stream));
+  coef[0] = mk<double>(-h[0].x, -h[0].y);
+  HIPCHK(hipMemcpyAsync(dcoef, coef.data(), sizeof(cx<double>), hipMemcpyHostToDevice, g.stream));
+  HIPCHK(multi_axpy(V, ld, 1, dcoef, f, n, g.stream));
+  CHK(dot(V, f, &sv[0]));
+  h[0] = h[0] + sv[0];
+  coef[0] = mk<double>(-sv[0].x, -sv[0].y);
+  HIPCHK(hipMemcpyAsync(dcoef, coef.data(), sizeof(cx<double>), hipMemcpyHostToDevice, g.stream));
+  HIPCHK(multi_axpy(V, ld, 1, dcoef, f, n, g.stream));
+  for (int j = 2; j <= k; ++j) {
+    CHK(dot(f, f, &tz));
+    T[(j - 1) + (size_t)ldT * (j - 2)] = std::sqrt(tz.x);
+    beta = 1.0 / std::sqrt(tz.x);
+    cx<double>* vj = V + (size_t)(j - 1) * ld;
+    HIPCHK(axpby(f, beta, nullptr, 0.0, vj, n, g.stream));
+    CHK(hpsi<double>(s, vj, Hv, 1, 1.0, 0.0, 0.0, nullptr, PREC_FP64));
+    HIPCHK(axpby(Hv, 1.0, nullptr, 0.0, f, n, g.stream));
+    HIPCHK(multi_dotc(V, ld, j, Hv, n, dh, g.stream));
+    HIPCHK(hipMemcpyAsync(h.data(), dh, sizeof(cx<double>) * j, hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    for (int i = 0; i < j; ++i) coef[i] = mk<double>(-h[i].x, -h[i].y);
+    HIPCHK(hipMemcpyAsync(dcoef, coef.data(), sizeof(cx<double>) * j, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(multi_axpy(V, ld, j, dcoef, f, n, g.stream));
+    HIPCHK(multi_dotc(V, ld, j, f, n, dh, g.stream));
+    HIPCHK(hipMemcpyAsync(sv.data(), dh, sizeof(cx<double>) * j, hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    for (int i = 0; i < j; ++i) {
+      h[i] = h[i] + sv[i];
+      coef[i] = mk<double>(-sv[i].x, -sv[i].y);
+    }
+    HIPCHK(hipMemcpyAsync(dcoef, coef.data(), sizeof(cx<double>) * j, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(multi_axpy(V, ld, j, dcoef, f, n, g.stream));
+    for (int i = 0; i < j; ++i) T[i + (size_t)ldT * (j - 1)] = h[i].x;
+  }
+  CHK(dot(f, f, &tz));
+  *fnorm2 = tz.x;
+  return PWH_OK;
+}
+
+// Eigenvalues of the symmetric matrix held in the upper triangle of T (k x k, ld ldT),
+// dsyev_('N','U') of ChebySCFLoop.cpp:189, on the GPU.
+int sym_eigvals_upper(const double* T, int ldT, int k, std::vector<double>& w) {
+  Buf<double> dA, dD, dE;
+  Buf<rocblas_int> di;
+  std::vector<double> A((size_t)k * k);
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < k; ++i) A[i + (size_t)k * j] = T[i + (size_t)ldT * j];
+  CHK(upload(dA, A.data(), A.size()));
+  CHK(dD.ensure(k));
+  CHK(dE.ensure(k));
+  CHK(di.ensure(1));
+  RBCHK(rocsolver_dsyevd(g.rb, rocblas_evect_none, rocblas_fill_upper, k, dA.p, k, dD.p, dE.p, di.p));
+  w.resize(k);
+  rocblas_int hinfo = 0;
+  HIPCHK(hipMemcpyAsync(w.data(), dD.p, sizeof(double) * k, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipMemcpyAsync(&hinfo, di.p, sizeof(hinfo), hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  dA.release(); dD.release(); dE.release(); di.release();
+  if (hinfo != 0) return fail(PWH_ERR_LAPACK, "dsyevd info " + std::to_string(hinfo));
+  return PWH_OK;
+}
+
+// induced_norm_mat, ChebySCFLoop.cpp:7-34 (squared 2-norm of A * 1/sqrt(N)).
+double induced_norm_sq(const double* A, int ldA, int N) {
+  const double d = 1.0 / std::sqrt((double)N);
+  double res = 0;
+  for (int i = 0; i < N; ++i) {
+    double v = 0;
+    for (int j = 0; j < N; ++j) v += A[i + (size_t)ldA * j] * d;
+    res += v * v;
+  }
+  return res;
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+std::vector<cx<double>> twiddles(int n) {
+  std::vector<cx<double>> t(n);
+  for (int m = 0; m < n; ++m) {
+    const double a = 2.0 * M_PI * (double)m / (double)n;
+    t[m] = mk<double>(std::cos(a), std::sin(a));
+  }
+  return t;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* pwh_last_error(void) { return g_err.c_str(); }
+
+int pwh_device_count(int* count) {
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (count) *count = n;
+  return PWH_OK;
+}
+
+int pwh_init(int device) {
+  if (g.ready && g.device == device) return PWH_OK;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(PWH_ERR_HIP, "no HIP device available");
+  if (device < 0 || device >= n) return fail(PWH_ERR_ARG, "device index out of range");
+  HIPCHK(hipSetDevice(device));
+  if (!g.stream) HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+  if (!g.rb) {
+    RBCHK(rocblas_create_handle(&g.rb));
+    RBCHK(rocblas_set_stream(g.rb, g.stream));
+  }
+  g.device = device;
+  g.ready = true;
+  return PWH_OK;
+}
+
+int pwh_selftest_mfma(int kind, double* max_err) {
+  CHK(ensure_init());
+  if (kind < 0 || kind > 2 || !max_err) return fail(PWH_ERR_ARG, "kind must be 0, 1 or 2");
+  HIPCHK(mfma_selftest(kind, max_err));
+  return PWH_OK;
+}
+
+int pwh_grid_supported(int nr1, int nr2, int nr3) {
+  return plane_supported(nr1, nr2) && fft_size_supported(nr3) ? 1 : 0;
+}
+
+int pwh_system_create(const pwh_system_desc* d, pwh_system** out) {
+  if (!d || !out) return fail(PWH_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (d->nr1 < 1 || d->nr2 < 1 || d->nr3 < 1 || d->ngw < 1 || d->ngwx < d->ngw || !d->n123 ||
+      !d->xkg || d->nsp < 0 || d->nax < 0 || d->nlmax < 1)
+    return fail(PWH_ERR_ARG, "invalid system description");
+  if (!pwh_grid_supported(d->nr1, d->nr2, d->nr3))
+    return fail(PWH_ERR_UNSUPPORTED, "no compiled FFT plan for grid " + std::to_string(d->nr1) + "x" +
+                                         std::to_string(d->nr2) + "x" + std::to_string(d->nr3));
+  CHK(ensure_init());
+  auto s = new pwh_system();
+  auto bail = [&](int code) { pwh_system_destroy(s); return code; };
+  s->nr[0] = d->nr1; s->nr[1] = d->nr2; s->nr[2] = d->nr3;
+  s->nnr = d->nr1 * d->nr2 * d->nr3;
+  s->ngw = d->ngw;
+  s->ngw_pad = (d->ngw + 63) / 64 * 64;
+  s->tpiba2 = d->tpiba2;
+
+  // species / projectors: lm_end = L_MAX^2 - 2 L_LOC + 1 (ApplyHam_block.cpp:69-73)
+  s->nsp = d->nsp;
+  s->na.resize(d->nsp); s->lm.resize(d->nsp); s->aoff.resize(d->nsp); s->wl.assign(4 * d->nsp, 0.0);
+  for (int sp = 0; sp < d->nsp; ++sp) {
+    const int lm = d->l_max[sp] * d->l_max[sp] - 2 * d->l_loc[sp] + 1;
+    if (lm > 4) { delete s; return fail(PWH_ERR_ARG, "lm_end > 4 is not implemented"); }
+    if (d->na[sp] > d->nax) { delete s; return fail(PWH_ERR_ARG, "na > nax"); }
+    s->lm[sp] = std::max(lm, 0);
+    s->na[sp] = d->na[sp];
+    s->aoff[sp] = s->nat;
+    s->nat += d->na[sp];
+    for (int l = 0; l < s->lm[sp]; ++l) s->wl[4 * sp + l] = d->wnl[sp + d->nsp * l];
+  }
+
+  // sphere packing: group plane waves by column (i1,i2), then by i3
+  const int nr1 = d->nr1, nr2 = d->nr2, ngw = d->ngw;
+  std::vector<long long> key(ngw);
+  for (int ig = 0; ig < ngw; ++ig) {
+    const unsigned n = d->n123[ig];
+    if (n < 1 || (int)n > s->nnr) { delete s; return fail(PWH_ERR_ARG, "N123 entry out of range"); }
+    const int idx = (int)n - 1;
+    const int i1 = idx % nr1, i2 = (idx / nr1) % nr2, i3 = idx / (nr1 * nr2);
+    key[ig] = ((long long)(i1 + nr1 * i2) << 20) | i3;
+  }
+  s->perm.resize(ngw);
+  for (int i = 0; i < ngw; ++i) s->perm[i] = i;
+  std::stable_sort(s->perm.begin(), s->perm.end(), [&](int a, int b) { return key[a] < key[b]; });
+  for (int k = 1; k < ngw; ++k)
+    if (key[s->perm[k]] == key[s->perm[k - 1]]) { delete s; return fail(PWH_ERR_ARG, "duplicate N123 entries"); }
+  std::vector<int> colstart, kcol(ngw), kz(ngw), colxy, xa;
+  std::vector<char> xseen(nr1, 0);
+  for (int k = 0; k < ngw; ++k) {
+    const long long kk = key[s->perm[k]];
+    const int xy = (int)(kk >> 20);
+    if (k == 0 || xy != (int)(key[s->perm[k - 1]] >> 20)) {
+      colstart.push_back(k);
+      colxy.push_back(xy);
+      if (!xseen[xy % nr1]) { xseen[xy % nr1] = 1; }
+    }
+    kcol[k] = (int)colxy.size() - 1;
+    kz[k] = (int)(kk & ((1 << 20) - 1));
+  }
+  for (int x = 0; x < nr1; ++x) if (xseen[x]) xa.push_back(x);
+  const int ncol = (int)colxy.size();
+  const int ncol_pad = (ncol + 15) / 16 * 16;
+  while ((int)colstart.size() < ncol_pad + 1) colstart.push_back(ngw);
+  int rc;
+  if ((rc = upload(s->d_perm, s->perm.data(), ngw)) ||
+      (rc = upload(s->d_colstart, colstart.data(), colstart.size())) ||
+      (rc = upload(s->d_kcol, kcol.data(), ngw)) || (rc = upload(s->d_kz, kz.data(), ngw)) ||
+      (rc = upload(s->d_colxy, colxy.data(), ncol)) || (rc = upload(s->d_xa, xa.data(), xa.size())))
+    return bail(rc);
+  s->maps.nr1 = nr1; s->maps.nr2 = nr2; s->maps.nr3 = d->nr3; s->maps.nnr = s->nnr;
+  s->maps.ngw = ngw; s->maps.ngw_pad = s->ngw_pad;
+  s->maps.ncol = ncol; s->maps.ncol_pad = ncol_pad; s->maps.nxa = (int)xa.size();
+  s->maps.colstart = s->d_colstart.p; s->maps.kcol = s->d_kcol.p; s->maps.kz = s->d_kz.p;
+  s->maps.colxy = s->d_colxy.p; s->maps.xa = s->d_xa.p;
+
+  // constants in sphere order
+  const int ld = s->ngw_pad;
+  std::vector<cx<double>> E((size_t)std::max(s->nat, 1) * ld, mk<double>(0, 0));
+  std::vector<double> P((size_t)std::max(s->nsp, 1) * 4 * ld, 0.0), kin(ld, 0.0);
+  const cx<double>* eigr = reinterpret_cast<const cx<double>*>(d->eigr);
+  for (int sp = 0; sp < s->nsp; ++sp) {
+    for (int ia = 0; ia < s->na[sp]; ++ia)
+      for (int k = 0; k < ngw; ++k)
+        E[(size_t)(s->aoff[sp] + ia) * ld + k] = eigr[s->perm[k] + (size_t)d->ngwx * (ia + (size_t)d->nax * sp)];
+    for (int l = 0; l < s->lm[sp]; ++l)
+      for (int k = 0; k < ngw; ++k)
+        P[((size_t)sp * 4 + l) * ld + k] = d->pkg_a[l + (size_t)d->nlmax * (s->perm[k] + (size_t)d->ngwx * sp)];
+  }
+  for (int k = 0; k < ngw; ++k) kin[k] = 0.5 * d->xkg[s->perm[k]] * d->tpiba2;
+  std::vector<cx<float>> E32(E.size());
+  std::vector<float> P32(P.size()), kin32(kin.size()), wl32(s->wl.size());
+  for (size_t i = 0; i < E.size(); ++i) E32[i] = mk<float>((float)E[i].x, (float)E[i].y);
+  for (size_t i = 0; i < P.size(); ++i) P32[i] = (float)P[i];
+  for (size_t i = 0; i < kin.size(); ++i) kin32[i] = (float)kin[i];
+  for (size_t i = 0; i < s->wl.size(); ++i) wl32[i] = (float)s->wl[i];
+  std::vector<double> wlv = s->wl;
+  if (wlv.empty()) wlv.assign(4, 0.0), wl32.assign(4, 0.f);
+  if ((rc = upload(s->E64, E.data(), E.size())) || (rc = upload(s->P64, P.data(), P.size())) ||
+      (rc = upload(s->kin64, kin.data(), kin.size())) || (rc = upload(s->wl64, wlv.data(), wlv.size())) ||
+      (rc = upload(s->E32, E32.data(), E32.size())) || (rc = upload(s->P32, P32.data(), P32.size())) ||
+      (rc = upload(s->kin32, kin32.data(), kin32.size())) || (rc = upload(s->wl32, wl32.data(), wl32.size())))
+    return bail(rc);
+  for (int dd = 0; dd < 3; ++dd) {
+    auto t = twiddles(s->nr[dd]);
+    std::vector<cx<float>> t32(t.size());
+    for (size_t i = 0; i < t.size(); ++i) t32[i] = mk<float>((float)t[i].x, (float)t[i].y);
+    if ((rc = upload(s->tw64[dd], t.data(), t.size())) || (rc = upload(s->tw32[dd], t32.data(), t32.size())))
+      return bail(rc);
+  }
+  hipError_t e = hipStreamSynchronize(g.stream);
+  if (e != hipSuccess) { pwh_system_destroy(s); return fail(PWH_ERR_HIP, hipGetErrorString(e)); }
+  *out = s;
+  return PWH_OK;
+}
+
+int pwh_system_destroy(pwh_system* s) {
+  if (!s) return PWH_OK;
+  if (g.stream) hipStreamSynchronize(g.stream);
+  for (auto* b : {&s->d_perm, &s->d_colstart, &s->d_kcol, &s->d_kz, &s->d_colxy, &s->d_xa}) b->release();
+  for (auto* b : {&s->E64, &s->X, &s->HX, &s->A64, &s->B64, &s->C64, &s->W64, &s->Hnl64, &s->proj64,
+                  &s->Q64, &s->part64, &s->stage, &s->S, &s->Vl})
+    b->release();
+  for (auto* b : {&s->P64, &s->kin64, &s->wl64, &s->V64, &s->evals, &s->rwork, &s->dres}) b->release();
+  for (auto* b : {&s->E32, &s->X32, &s->Y32, &s->Y1_32, &s->W32, &s->Hnl32, &s->proj32, &s->Q32, &s->part32})
+    b->release();
+  for (auto* b : {&s->P32, &s->kin32, &s->wl32, &s->V32}) b->release();
+  for (int d = 0; d < 3; ++d) { s->tw64[d].release(); s->tw32[d].release(); }
+  s->info.release();
+  delete s;
+  return PWH_OK;
+}
+
+int pwh_system_info(const pwh_system* s, int* ngw, int* ncol, int* nxa, size_t* bytes) {
+  if (!s) return fail(PWH_ERR_ARG, "null system");
+  if (ngw) *ngw = s->ngw;
+  if (ncol) *ncol = s->maps.ncol;
+  if (nxa) *nxa = s->maps.nxa;
+  if (bytes) {
+    size_t b = 0;
+    auto add = [&](size_t cap, size_t el) { b += cap * el; };
+    add(s->E64.cap, 16); add(s->E32.cap, 8); add(s->P64.cap, 8); add(s->P32.cap, 4);
+    add(s->X.cap, 16); add(s->HX.cap, 16); add(s->A64.cap, 16); add(s->B64.cap, 16);
+    add(s->W64.cap, 16); add(s->W32.cap, 8); add(s->Hnl64.cap, 16); add(s->Hnl32.cap, 8);
+    add(s->X32.cap, 8); add(s->Y32.cap, 8); add(s->Y1_32.cap, 8); add(s->part64.cap, 16);
+    add(s->part32.cap, 8); add(s->V64.cap, 8); add(s->V32.cap, 4); add(s->Vl.cap, 16);
+    *bytes = b;
+  }
+  return PWH_OK;
+}
+
+int pwh_set_vloc(pwh_system* s, const double* rho, int nnr) {
+  if (!s || !rho) return fail(PWH_ERR_ARG, "null argument");
+  if (nnr != s->nnr)
+    return fail(PWH_ERR_ARG, "nnr != NNR : " + std::to_string(nnr) + " " + std::to_string(s->nnr));
+  CHK(upload(s->V64, rho, nnr));
+  std::vector<float> v32(rho, rho + nnr);
+  CHK(upload(s->V32, v32.data(), v32.size()));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  s->vset = true;
+  return PWH_OK;
+}
+
+int pwh_apply_ham_block(pwh_system* s, const double* vin, double* vout, int ngw_ik, int nstates,
+                        int precision) {
+  CHK(check_ld(s, nstates));
+  if (!vin || !vout) return fail(PWH_ERR_ARG, "null vector");
+  if (ngw_ik != s->ngw) return fail(PWH_ERR_ARG, "NGW_ik does not match the system");
+  if (precision < PWH_FP64 || precision > PWH_BF16X3) return fail(PWH_ERR_ARG, "bad precision");
+  CHK(load_block(s, vin, nstates));
+  CHK(s->HX.ensure((size_t)nstates * s->ngw_pad));
+  CHK(apply_prec(s, s->X.p, s->HX.p, nstates, precision));
+  return store_block(s, s->HX.p, vout, nstates);
+}
+
+int pwh_chebyfilt(pwh_system* s, double* X, int nstates, int degree, double lb, double ub,
+                  int precision) {
+  CHK(check_ld(s, nstates));
+  if (!X) return fail(PWH_ERR_ARG, "null X");
+  if (precision < PWH_FP64 || precision > PWH_BF16X3) return fail(PWH_ERR_ARG, "bad precision");
+  if (!(ub > lb)) return fail(PWH_ERR_ARG, "chebyfilt needs ub > lb");
+  CHK(load_block(s, X, nstates));
+  CHK(filter_prec(s, s->X.p, nstates, degree, lb, ub, precision));
+  return store_block(s, s->X.p, X, nstates);
+}
+
+int pwh_ortho_qr(double* X, int nbasis, int nstates) {
+  if (!X || nbasis < 1 || nstates < 1 || nstates > nbasis) return fail(PWH_ERR_ARG, "bad ortho_qr arguments");
+  CHK(ensure_init());
+  const size_t n = (size_t)nbasis * nstates;
+  Buf<cx<double>> dX, dT, dS, dP;
+  Buf<rocblas_int> di;
+  const size_t pe = proj_part_elems(nstates, nstates, nbasis);
+  int rc;
+  if ((rc = upload(dX, reinterpret_cast<const cx<double>*>(X), n)) || (rc = dT.ensure(n)) ||
+      (rc = dS.ensure((size_t)nstates * nstates)) || (rc = dP.ensure(pe)) || (rc = di.ensure(1))) {
+    dX.release(); dT.release(); dS.release(); dP.release(); di.release();
+    return rc;
+  }
+  rc = cholqr2(dX.p, nbasis, nbasis, nstates, dT.p, dS.p, dP.p, pe, di.p);
+  if (rc == PWH_OK) {
+    hipError_t e = hipMemcpyAsync(X, dX.p, sizeof(cx<double>) * n, hipMemcpyDeviceToHost, g.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+    if (e != hipSuccess) rc = fail(PWH_ERR_HIP, hipGetErrorString(e));
+  }
+  hipStreamSynchronize(g.stream);
+  dX.release(); dT.release(); dS.release(); dP.release(); di.release();
+  return rc;
+}
+
+int pwh_lanczos(pwh_system* s, double* v0, int k, double* T, int ldT, double* f) {
+  CHK(check_ld(s, 1));
+  if (!v0 || !T || !f || k < 1 || ldT < k) return fail(PWH_ERR_ARG, "bad lanczos arguments");
+  CHK(load_block(s, v0, 1));
+  CHK(s->HX.ensure((size_t)s->ngw_pad));
+  double fn2 = 0;
+  CHK(lanczos_dev(s, s->X.p, k, T, ldT, s->HX.p, &fn2));
+  CHK(store_block(s, s->HX.p, f, 1));
+  return store_block(s, s->X.p, v0, 1);
+}
+
+int pwh_rayleigh_ritz(pwh_system* s, double* X, int nstates, double* eig) {
+  CHK(check_ld(s, nstates));
+  if (!X || !eig) return fail(PWH_ERR_ARG, "null argument");
+  CHK(load_block(s, X, nstates));
+  CHK(rayleigh_ritz_dev(s, s->X.p, nstates));
+  HIPCHK(hipMemcpyAsync(eig, s->evals.p, sizeof(double) * nstates, hipMemcpyDeviceToHost, g.stream));
+  return store_block(s, s->X.p, X, nstates);
+}
+
+int pwh_residual_norms(pwh_system* s, const double* X, int nstates, double* res) {
+  CHK(check_ld(s, nstates));
+  if (!X || !res) return fail(PWH_ERR_ARG, "null argument");
+  CHK(load_block(s, X, nstates));
+  return residuals_dev(s, s->X.p, nstates, res);
+}
+
+int pwh_chfsi(pwh_system* s, double* X, int nstates, double* v0, pwh_chfsi_opts* o, double* eig,
+              double* res) {
+  CHK(check_ld(s, nstates));
+  if (!X || !o || !eig || !res) return fail(PWH_ERR_ARG, "null argument");
+  if (o->degree < 1 || o->niter < 0) return fail(PWH_ERR_ARG, "bad degree / niter");
+  if (o->lanczos_k > 0 && !v0) return fail(PWH_ERR_ARG, "Lanczos needs v0");
+  const int nb = nstates, NX = nstates - 2;
+  const double t0 = now_s();
+  o->t_filter_s = o->t_ortho_s = o->t_rr_s = o->t_lanczos_s = 0;
+  o->band_applies = 0;
+  double lb = o->lb, ub = o->ub;
+  std::vector<double> T;
+  int kref = std::max(2 * NX, 10);
+  // Lanczos bounds (ChebySCFLoop.cpp:115,176-196) on a separate vector slot
+  Buf<cx<double>> dv0, df;
+  auto run_lanczos = [&](int k, int ldT, double* fn2) -> int {
+    T.assign((size_t)ldT * k, 0.0);
+    CHK(dv0.ensure(s->ngw_pad));
+    CHK(df.ensure(s->ngw_pad));
+    CHK(upload(s->stage, reinterpret_cast<const cx<double>*>(v0), s->ngw));
+    HIPCHK(permute_in_f64(s->stage.p, s->ngw, s->d_perm.p, s->ngw, s->ngw_pad, dv0.p, 1, g.stream));
+    const double ta = now_s();
+    CHK(lanczos_dev(s, dv0.p, k, T.data(), ldT, df.p, fn2));
+    o->t_lanczos_s += now_s() - ta;
+    o->band_applies += k;
+    return PWH_OK;
+  };
+  int rc = PWH_OK;
+  if (o->lanczos_k > 0) {
+    const int k = (o->bounds_mode == 1) ? kref : o->lanczos_k;
+    double fn2 = 0;
+    if ((rc = run_lanczos(k, k, &fn2))) goto done;
+    std::vector<double> w;
+    if ((rc = sym_eigvals_upper(T.data(), k, k, w))) goto done;
+    if (o->bounds_mode == 1) {
+      if (k < 2 * NX) { rc = fail(PWH_ERR_ARG, "reference bounds need k >= 2 NX"); goto done; }
+      lb = w[NX + 1];
+      ub = w[2 * NX - 1];
+    } else {
+      ub = w[k - 1] + std::sqrt(fn2);
+      const double frac = std::min(0.5, std::max(0.02, std::pow((double)nb / s->ngw, 2.0 / 3.0)));
+      lb = w[0] + frac * (ub - w[0]);
+    }
+  }
+  if ((rc = load_block(s, X, nb))) goto done;
+  for (int it = 0; it < o->niter; ++it) {
+    double ta = now_s();
+    if ((rc = filter_prec(s, s->X.p, nb, o->degree, lb, ub, o->precision))) goto done;
+    HIPCHK(hipStreamSynchronize(g.stream));
+    o->t_filter_s += now_s() - ta;
+    o->band_applies += (long long)o->degree * nb;
+    ta = now_s();
+    if ((rc = sys_cholqr2(s, s->X.p, nb))) goto done;
+    HIPCHK(hipStreamSynchronize(g.stream));
+    o->t_ortho_s += now_s() - ta;
+    ta = now_s();
+    if ((rc = rayleigh_ritz_dev(s, s->X.p, nb))) goto done;
+    HIPCHK(hipMemcpyAsync(eig, s->evals.p, sizeof(double) * nb, hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    o->t_rr_s += now_s() - ta;
+    o->band_applies += nb;
+    // bounds update (ChebySCFLoop.cpp:355-368)
+    lb = eig[nb - 1];
+    if (o->bounds_mode == 1) {
+      const int kk = (kref > 10) ? 10 : kref;
+      double fn2 = 0;
+      if ((rc = run_lanczos(kk, kref, &fn2))) goto done;
+      ub = induced_norm_sq(T.data(), kref, kk) + std::sqrt(fn2);
+    }
+  }
+  // residual report of the final block
+  if ((rc = residuals_dev(s, s->X.p, nb, res))) goto done;
+  if ((rc = store_block(s, s->X.p, X, nb))) goto done;
+  o->lb = lb;
+  o->ub = ub;
+  o->t_total_s = now_s() - t0;
+done:
+  dv0.release();
+  df.release();
+  return rc;
+}
+
+int pwh_resident_set(pwh_system* s, const double* X, int nstates) {
+  CHK(check_ld(s, nstates));
+  CHK(load_block(s, X, nstates));
+  const size_t n = (size_t)nstates * s->ngw_pad;
+  CHK(s->X32.ensure(n));
+  HIPCHK((convert_cx<double, float>(s->X.p, s->X32.p, n, g.stream)));
+  s->resident_nb = nstates;
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return PWH_OK;
+}
+
+int pwh_resident_get(pwh_system* s, double* X, int nstates) {
+  CHK(check_ld(s, nstates));
+  if (nstates > s->resident_nb) return fail(PWH_ERR_ARG, "more states than resident");
+  return store_block(s, s->X.p, X, nstates);
+}
+
+int pwh_resident_apply(pwh_system* s, int nstates, int precision) {
+  CHK(check_ld(s, nstates));
+  if (nstates > s->resident_nb) return fail(PWH_ERR_ARG, "more states than resident");
+  const size_t n = (size_t)nstates * s->ngw_pad;
+  if (precision == PWH_FP64) {
+    CHK(s->HX.ensure(n));
+    return hpsi<double>(s, s->X.p, s->HX.p, nstates, 1.0, 0.0, 0.0, nullptr, PREC_FP64);
+  }
+  // FP32 modes use the FP32 copy made by pwh_resident_set
+  CHK(s->Y32.ensure(n));
+  return hpsi<float>(s, s->X32.p, s->Y32.p, nstates, 1.0, 0.0, 0.0, nullptr, precision);
+}
+
+int pwh_resident_chebyfilt(pwh_system* s, int nstates, int degree, double lb, double ub,
+                           int precision) {
+  CHK(check_ld(s, nstates));
+  if (nstates > s->resident_nb) return fail(PWH_ERR_ARG, "more states than resident");
+  if (!(ub > lb)) return fail(PWH_ERR_ARG, "chebyfilt needs ub > lb");
+  return filter_prec(s, s->X.p, nstates, degree, lb, ub, precision);
+}
+
+int pwh_synchronize(void) {
+  CHK(ensure_init());
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return PWH_OK;
+}
+
+int pwh_timing_enable(int on) {
+  CHK(ensure_init());
+  g.timing = on != 0;
+  return PWH_OK;
+}
+
+int pwh_timing_reset(void) {
+  resolve_timers();
+  for (int i = 0; i < TM_COUNT; ++i) { g.total_ms[i] = 0; g.count[i] = 0; }
+  return PWH_OK;
+}
+
+int pwh_timing_get(const char* name, double* total_ms, int* count) {
+  resolve_timers();
+  for (int i = 0; i < TM_COUNT; ++i)
+    if (name && std::strcmp(name, kTimerNames[i]) == 0) {
+      if (total_ms) *total_ms = g.total_ms[i];
+      if (count) *count = g.count[i];
+      return PWH_OK;
+    }
+  return fail(PWH_ERR_ARG, "unknown timer name");
+}
+
+}  // extern "C"

@@ -1,0 +1,383 @@
+// pwh_fft.hip -- the local part of H|psi> on gfx950: a sphere-pruned 3D FFT pair with
+// the local potential fused into the real-space pass and the kinetic term (plus the
+// nonlocal result and the Chebyshev three-term update) fused into the last pass.
+//
+// Replaces, for a whole block of bands at once, the per-band sequence of
+// ApplyHam_block.cpp:33-61: scatter through N123, fft_fftw3(inverse), *rho,
+// fft_fftw3(forward, 1/N), gather + 0.5*XKG*TPIBA2.
+//
+// Three passes per block of nb bands:
+//   zinv   (grid ncol_pad/CT x nb): gather each column's sphere entries, inverse
+//          FFT along i3 in LDS, write W[b][i3][col].
+//   plane  (grid nr3 x nb): one (i3, band) plane per workgroup, fully in LDS:
+//          inverse FFT along i2 on the active i1 columns only, inverse FFT along i1,
+//          multiply by V, forward FFT along i1, forward FFT along i2 on the active
+//          columns, keep the sphere columns.
+//   zfwd   (grid ncol_pad/CT x nb): forward FFT along i3, keep the sphere entries,
+//          scale 1/nnr, add kinetic * X, add the nonlocal result, apply
+//          out = alpha*H + beta*X + gamma*Z.
+// Transforms are Stockham autosort in LDS, radix 8/4/2/3/5, with each stage loading
+// its butterflies into registers before a barrier so a single LDS buffer suffices.
+#include "pwh_internal.h"
+
+namespace pwh {
+
+constexpr int pick_radix(int rem) {
+  return (rem % 8 == 0 && rem != 16) ? 8
+       : (rem % 4 == 0)              ? 4
+       : (rem % 2 == 0)              ? 2
+       : (rem % 3 == 0)              ? 3
+       : (rem % 5 == 0)              ? 5
+                                     : rem;
+}
+
+// ---------------------------------------------------------------------------
+// Small DFTs, w = exp(SIGN * 2 pi i / R)
+// ---------------------------------------------------------------------------
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft4(cx<T>& a0, cx<T>& a1, cx<T>& a2, cx<T>& a3) {
+  cx<T> t0 = a0 + a2, t1 = a0 - a2, t2 = a1 + a3, t3 = mul_si<SIGN>(a1 - a3);
+  a0 = t0 + t2; a2 = t0 - t2; a1 = t1 + t3; a3 = t1 - t3;
+}
+
+template <typename T, int R, int SIGN>
+__device__ __forceinline__ void dft(cx<T>* v) {
+  if constexpr (R == 2) {
+    cx<T> a = v[0], b = v[1];
+    v[0] = a + b; v[1] = a - b;
+  } else if constexpr (R == 4) {
+    dft4<T, SIGN>(v[0], v[1], v[2], v[3]);
+  } else if constexpr (R == 8) {
+    cx<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    cx<T> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<T, SIGN>(e0, e1, e2, e3);
+    dft4<T, SIGN>(o0, o1, o2, o3);
+    const T c = T(0.70710678118654752440084436210485);
+    // o1 *= w, o2 *= w^2, o3 *= w^3 with w = (1 + SIGN i)/sqrt2
+    cx<T> w1 = mk<T>(c * (o1.x - SIGN * o1.y), c * (o1.y + SIGN * o1.x));
+    cx<T> w2 = mul_si<SIGN>(o2);
+    cx<T> w3 = mk<T>(c * (-o3.x - SIGN * o3.y), c * (-o3.y + SIGN * o3.x));
+    v[0] = e0 + o0; v[4] = e0 - o0;
+    v[1] = e1 + w1; v[5] = e1 - w1;
+    v[2] = e2 + w2; v[6] = e2 - w2;
+    v[3] = e3 + w3; v[7] = e3 - w3;
+  } else if constexpr (R == 3) {
+    const T h = T(0.86602540378443864676372317075294);
+    cx<T> s = v[1] + v[2], d = v[1] - v[2];
+    cx<T> t = v[0] - T(0.5) * s;
+    cx<T> u = mul_si<SIGN>(h * d);
+    v[0] = v[0] + s; v[1] = t + u; v[2] = t - u;
+  } else if constexpr (R == 5) {
+    const T c1 = T(0.30901699437494742410229341718282);    // cos(2pi/5)
+    const T c2 = T(-0.80901699437494742410229341718282);   // cos(4pi/5)
+    const T s1 = T(0.95105651629515357211643933337938);    // sin(2pi/5)
+    const T s2 = T(0.58778525229247312916870595463907);    // sin(4pi/5)
+    cx<T> a0 = v[0];
+    cx<T> b1 = v[1] + v[4], b2 = v[2] + v[3], d1 = v[1] - v[4], d2 = v[2] - v[3];
+    cx<T> r1 = a0 + c1 * b1 + c2 * b2;
+    cx<T> r2 = a0 + c2 * b1 + c1 * b2;
+    cx<T> i1 = mul_si<SIGN>(s1 * d1 + s2 * d2);
+    cx<T> i2 = mul_si<SIGN>(s2 * d1 - s1 * d2);
+    v[0] = a0 + b1 + b2;
+    v[1] = r1 + i1; v[4] = r1 - i1;
+    v[2] = r2 + i2; v[3] = r2 - i2;
+  } else {
+    static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 8, "unsupported radix");
+  }
+}
+
+// One Stockham stage over `count` transforms of length N held in LDS.  Element e of
+// transform t lives at buf[base(t) + e*es].  TFAST: consecutive threads take
+// consecutive transforms (columns), else consecutive butterflies (rows).
+template <typename T, int N, int R, int NS, int SIGN, int NT, int MAXCOUNT, bool TFAST, typename BaseF>
+__device__ __forceinline__ void lds_stage(cx<T>* buf, int count, int es, BaseF base,
+                                          const cx<T>* tw, int tid) {
+  constexpr int NB = N / R;
+  constexpr int MAXB = (MAXCOUNT * NB + NT - 1) / NT;
+  const int total = count * NB;
+  cx<T> v[MAXB][R];
+  int off[MAXB], jv[MAXB];
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int idx = tid + q * NT;
+    jv[q] = -1;
+    off[q] = 0;
+    if (idx < total) {
+      int t, j;
+      if constexpr (TFAST) {
+        j = idx / count;
+        t = idx - j * count;
+      } else {
+        t = idx / NB;
+        j = idx - t * NB;
+      }
+      const int b = base(t);
+      off[q] = b;
+      jv[q] = j;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = buf[b + (j + r * NB) * es];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    if (jv[q] >= 0) {
+      const int j = jv[q];
+      const int k = j % NS;
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          cx<T> w = tw[k * r * (N / (NS * R))];
+          if constexpr (SIGN < 0) w.y = -w.y;
+          v[q][r] = v[q][r] * w;
+        }
+      }
+      dft<T, R, SIGN>(v[q]);
+      const int o = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[off[q] + (o + r * NS) * es] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int N, int SIGN, int NT, int MAXCOUNT, bool TFAST, int NS = 1, typename BaseF>
+__device__ __forceinline__ void lds_fft(cx<T>* buf, int count, int es, BaseF base, const cx<T>* tw,
+                                        int tid) {
+  if constexpr (NS < N) {
+    constexpr int R = pick_radix(N / NS);
+    lds_stage<T, N, R, NS, SIGN, NT, MAXCOUNT, TFAST>(buf, count, es, base, tw, tid);
+    lds_fft<T, N, SIGN, NT, MAXCOUNT, TFAST, NS * R>(buf, count, es, base, tw, tid);
+  }
+}
+
+constexpr bool smooth235(int n) {
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  while (n % 5 == 0) n /= 5;
+  return n == 1;
+}
+
+// ---------------------------------------------------------------------------
+// Pass 1: inverse transform along i3 of the sphere columns
+// ---------------------------------------------------------------------------
+constexpr int kColTile = 16;
+constexpr int kColThreads = 128;
+constexpr int kPlaneThreads = 256;
+
+template <typename T, int NZ>
+__global__ __launch_bounds__(kColThreads) void k_zinv(SphereMaps m, const cx<T>* __restrict__ tw3g,
+                                                      const cx<T>* __restrict__ X, int ldx,
+                                                      cx<T>* __restrict__ W) {
+  constexpr int CT = kColTile, NT = kColThreads;
+  __shared__ cx<T> tile[NZ * CT];
+  __shared__ cx<T> tw[NZ];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * CT;
+  const int b = blockIdx.y;
+  for (int i = tid; i < NZ; i += NT) tw[i] = tw3g[i];
+  for (int i = tid; i < NZ * CT; i += NT) tile[i] = mk<T>(0, 0);
+  __syncthreads();
+  const int kb = m.colstart[c0], ke = m.colstart[c0 + CT];
+  const cx<T>* xb = X + (size_t)b * ldx;
+  for (int k = kb + tid; k < ke; k += NT) tile[m.kz[k] * CT + (m.kcol[k] - c0)] = xb[k];
+  __syncthreads();
+  lds_fft<T, NZ, +1, NT, CT, true>(tile, CT, CT, [](int t) { return t; }, tw, tid);
+  cx<T>* wb = W + (size_t)b * NZ * m.ncol_pad + c0;
+  for (int i = tid; i < NZ * CT; i += NT) {
+    const int z = i / CT, c = i - z * CT;
+    wb[(size_t)z * m.ncol_pad + c] = tile[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pass 2: (i1,i2) planes: inverse y, inverse x, *V, forward x, forward y
+// ---------------------------------------------------------------------------
+template <typename T, int N1, int N2>
+constexpr size_t plane_smem() {
+  return sizeof(cx<T>) * (size_t)(N1 * N2 + N1 + N2) + sizeof(int) * N1;
+}
+
+template <typename T, int N1, int N2>
+__global__ __launch_bounds__(kPlaneThreads) void k_plane(SphereMaps m, const cx<T>* __restrict__ tw1g,
+                                                         const cx<T>* __restrict__ tw2g,
+                                                         const T* __restrict__ V,
+                                                         cx<T>* __restrict__ W) {
+  constexpr int NT = kPlaneThreads;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
+  cx<T>* tw1 = tile + N1 * N2;
+  cx<T>* tw2 = tw1 + N1;
+  int* xa = reinterpret_cast<int*>(tw2 + N2);
+  const int tid = threadIdx.x;
+  const int z = blockIdx.x, b = blockIdx.y;
+  const int nxa = m.nxa;
+  for (int i = tid; i < N1; i += NT) tw1[i] = tw1g[i];
+  for (int i = tid; i < N2; i += NT) tw2[i] = tw2g[i];
+  for (int i = tid; i < nxa; i += NT) xa[i] = m.xa[i];
+  for (int i = tid; i < N1 * N2; i += NT) tile[i] = mk<T>(0, 0);
+  __syncthreads();
+  cx<T>* wrow = W + ((size_t)b * m.nr3 + z) * m.ncol_pad;
+  for (int c = tid; c < m.ncol; c += NT) tile[m.colxy[c]] = wrow[c];
+  __syncthreads();
+  lds_fft<T, N2, +1, NT, N1, true>(tile, nxa, N1, [&](int t) { return xa[t]; }, tw2, tid);
+  lds_fft<T, N1, +1, NT, N2, false>(tile, N2, 1, [](int t) { return t * N1; }, tw1, tid);
+  const T* vz = V + (size_t)z * N1 * N2;
+  for (int i = tid; i < N1 * N2; i += NT) {
+    const T v = vz[i];
+    tile[i] = v * tile[i];
+  }
+  __syncthreads();
+  lds_fft<T, N1, -1, NT, N2, false>(tile, N2, 1, [](int t) { return t * N1; }, tw1, tid);
+  lds_fft<T, N2, -1, NT, N1, true>(tile, nxa, N1, [&](int t) { return xa[t]; }, tw2, tid);
+  for (int c = tid; c < m.ncol; c += NT) wrow[c] = tile[m.colxy[c]];
+}
+
+// ---------------------------------------------------------------------------
+// Pass 3: forward transform along i3, gather the sphere, fused epilogue
+// ---------------------------------------------------------------------------
+template <typename T, int NZ>
+__global__ __launch_bounds__(kColThreads) void k_zfwd(SphereMaps m, const cx<T>* __restrict__ tw3g,
+                                                      const cx<T>* __restrict__ W,
+                                                      const T* __restrict__ kin,
+                                                      const cx<T>* __restrict__ X,
+                                                      const cx<T>* __restrict__ Hnl,
+                                                      const cx<T>* __restrict__ Z,
+                                                      cx<T>* __restrict__ out, int ld, T inv_nnr,
+                                                      T alpha, T beta, T gamma) {
+  constexpr int CT = kColTile, NT = kColThreads;
+  __shared__ cx<T> tile[NZ * CT];
+  __shared__ cx<T> tw[NZ];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * CT;
+  const int b = blockIdx.y;
+  for (int i = tid; i < NZ; i += NT) tw[i] = tw3g[i];
+  const cx<T>* wb = W + (size_t)b * NZ * m.ncol_pad + c0;
+  for (int i = tid; i < NZ * CT; i += NT) {
+    const int z = i / CT, c = i - z * CT;
+    tile[i] = wb[(size_t)z * m.ncol_pad + c];
+  }
+  __syncthreads();
+  lds_fft<T, NZ, -1, NT, CT, true>(tile, CT, CT, [](int t) { return t; }, tw, tid);
+  const int kb = m.colstart[c0], ke = m.colstart[c0 + CT];
+  const size_t ob = (size_t)b * ld;
+  for (int k = kb + tid; k < ke; k += NT) {
+    const cx<T> x = X[ob + k];
+    cx<T> h = inv_nnr * tile[m.kz[k] * CT + (m.kcol[k] - c0)];
+    h = h + kin[k] * x;
+    if (Hnl) h = h + Hnl[ob + k];
+    cx<T> o = alpha * h;
+    if (beta != T(0)) o = o + beta * x;
+    if (Z) o = o + gamma * Z[ob + k];
+    out[ob + k] = o;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch
+// ---------------------------------------------------------------------------
+#define PWH_FFT_SIZES(X) \
+  X(12) X(16) X(18) X(20) X(24) X(30) X(32) X(36) X(40) X(48) X(50) X(54) X(60) X(64) \
+  X(72) X(80) X(90) X(96) X(100) X(108) X(120) X(128)
+
+bool fft_size_supported(int n) {
+  switch (n) {
+#define PWH_CASE(N) case N: return true;
+    PWH_FFT_SIZES(PWH_CASE)
+#undef PWH_CASE
+    default: return false;
+  }
+}
+
+bool plane_supported(int n1, int n2) { return n1 == n2 && fft_size_supported(n1); }
+
+template <typename T, int N>
+static hipError_t launch_plane_n(const SphereMaps& m, const void* tw1, const void* tw2, const T* V,
+                                 cx<T>* W, int nb, hipStream_t s) {
+  constexpr size_t smem = plane_smem<T, N, N>();
+  if constexpr (smem > 160 * 1024) {
+    return hipErrorInvalidValue;
+  } else {
+    static bool attr_set = false;
+    if (!attr_set) {
+      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane<T, N, N>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+      if (e != hipSuccess) return e;
+      attr_set = true;
+    }
+    dim3 grid(m.nr3, nb);
+    hipLaunchKernelGGL((k_plane<T, N, N>), grid, dim3(kPlaneThreads), smem, s, m,
+                       static_cast<const cx<T>*>(tw1), static_cast<const cx<T>*>(tw2), V, W);
+    return hipGetLastError();
+  }
+}
+
+template <typename T>
+hipError_t launch_zinv(const SphereMaps& m, const void* tw3, const cx<T>* X, int ldx, cx<T>* W,
+                       int nb, hipStream_t s) {
+  if (!fft_size_supported(m.nr3) || m.ncol_pad % kColTile) return hipErrorInvalidValue;
+  dim3 cgrid(m.ncol_pad / kColTile, nb);
+  switch (m.nr3) {
+#define PWH_CASE(N)                                                                               \
+  case N:                                                                                         \
+    hipLaunchKernelGGL((k_zinv<T, N>), cgrid, dim3(kColThreads), 0, s, m,                         \
+                       static_cast<const cx<T>*>(tw3), X, ldx, W);                                \
+    return hipGetLastError();
+    PWH_FFT_SIZES(PWH_CASE)
+#undef PWH_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename T>
+hipError_t launch_plane(const SphereMaps& m, const void* tw1, const void* tw2, const T* V, cx<T>* W,
+                        int nb, hipStream_t s) {
+  if (!plane_supported(m.nr1, m.nr2)) return hipErrorInvalidValue;
+  switch (m.nr1) {
+#define PWH_CASE(N) \
+  case N: return launch_plane_n<T, N>(m, tw1, tw2, V, W, nb, s);
+    PWH_FFT_SIZES(PWH_CASE)
+#undef PWH_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <typename T>
+hipError_t launch_zfwd_epilogue(const SphereMaps& m, const void* tw3, const cx<T>* W, const T* kin,
+                                const cx<T>* X, const cx<T>* Hnl, const cx<T>* Z, cx<T>* out,
+                                int ld, int nb, T inv_nnr, T alpha, T beta, T gamma,
+                                hipStream_t s) {
+  if (!fft_size_supported(m.nr3) || m.ncol_pad % kColTile) return hipErrorInvalidValue;
+  dim3 cgrid(m.ncol_pad / kColTile, nb);
+  switch (m.nr3) {
+#define PWH_CASE(N)                                                                               \
+  case N:                                                                                         \
+    hipLaunchKernelGGL((k_zfwd<T, N>), cgrid, dim3(kColThreads), 0, s, m,                         \
+                       static_cast<const cx<T>*>(tw3), W, kin, X, Hnl, Z, out, ld, inv_nnr,       \
+                       alpha, beta, gamma);                                                       \
+    return hipGetLastError();
+    PWH_FFT_SIZES(PWH_CASE)
+#undef PWH_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template hipError_t launch_zinv<double>(const SphereMaps&, const void*, const cx<double>*, int,
+                                        cx<double>*, int, hipStream_t);
+template hipError_t launch_zinv<float>(const SphereMaps&, const void*, const cx<float>*, int,
+                                       cx<float>*, int, hipStream_t);
+template hipError_t launch_plane<double>(const SphereMaps&, const void*, const void*, const double*,
+                                         cx<double>*, int, hipStream_t);
+template hipError_t launch_plane<float>(const SphereMaps&, const void*, const void*, const float*,
+                                        cx<float>*, int, hipStream_t);
+template hipError_t launch_zfwd_epilogue<double>(const SphereMaps&, const void*, const cx<double>*,
+                                                 const double*, const cx<double>*,
+                                                 const cx<double>*, const cx<double>*, cx<double>*,
+                                                 int, int, double, double, double, double,
+                                                 hipStream_t);
+template hipError_t launch_zfwd_epilogue<float>(const SphereMaps&, const void*, const cx<float>*,
+                                                const float*, const cx<float>*, const cx<float>*,
+                                                const cx<float>*, cx<float>*, int, int, float,
+                                                float, float, float, hipStream_t);
+
+}  // namespace pwh

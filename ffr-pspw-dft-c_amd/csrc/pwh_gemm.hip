@@ -1,0 +1,485 @@
+// pwh_gemm.hip -- complex GEMMs on the gfx950 matrix cores.
+//
+// Two shapes cover the nonlocal projectors and the subspace matrices:
+//
+//   proj   C[m][n] = sum_k conj(A[m][k]) * B[n][k]      (reduction over plane waves)
+//          - nonlocal:  A = beta[(a,l)][k] = EIGR[a][k]*PKG_A[l][k] generated while the
+//            tile is staged (never stored), B = psi[b][k]  ->  <beta_(a,l)|psi_b>
+//            (ApplyHam_block.cpp:88-117: conj(EIGR)*PKG_A then zdotu with vin)
+//          - subspace:  S = X^H X, T = X^H (HX)  (ChebySCFLoop.cpp:296)
+//          split over K into deterministic partial sums, then a reduction that can
+//          apply the per-projector D = WNL(l) scale.
+//   expand out[n][k] (+)= sum_j A[j][k] * Q[j][n]      (reduction over a small index)
+//          - nonlocal:  A = beta (generated), Q = D <beta|psi>  ->  sum beta D <beta|psi>
+//            (ApplyHam_block.cpp:118-163)
+//          - rotation:  X <- X Q (ChebySCFLoop.cpp:304-306, CholeskyQR's X R^-1)
+//
+// Complex products use four real MFMAs (Re/Im of each operand).  Precisions:
+//   FP64   v_mfma_f64_16x16x4_f64
+//   FP32   v_mfma_f32_16x16x4_f32
+//   BF16X3 v_mfma_f32_16x16x16_bf16 on hi/lo bf16 splits of FP32 operands
+//          (a*b ~ ah*bh + ah*bl + al*bh), FP32 accumulation.
+// Block tile 64x64 complex, 4 waves of 32x32, LDS staging of Re/Im planes.
+#include <cmath>
+#include <type_traits>
+#include <vector>
+
+#include "pwh_blas.h"
+#include "pwh_internal.h"
+
+namespace pwh {
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef short s4v __attribute__((ext_vector_type(4)));
+
+template <typename T> struct AccT;
+template <> struct AccT<double> { using type = d4v; };
+template <> struct AccT<float> { using type = f4v; };
+
+__device__ __forceinline__ d4v mfma4(double a, double b, d4v c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4v mfma4(float a, float b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4v mfma16bf(s4v a, s4v b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// round-to-nearest-even float -> bf16 bits
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float(((unsigned)h) << 16); }
+
+struct Split4 { s4v hi, lo; };
+__device__ __forceinline__ Split4 split4(float a, float b, float c, float d) {
+  Split4 s;
+  unsigned short h0 = f2bf(a), h1 = f2bf(b), h2 = f2bf(c), h3 = f2bf(d);
+  s.hi = s4v{(short)h0, (short)h1, (short)h2, (short)h3};
+  s.lo = s4v{(short)f2bf(a - bf2f(h0)), (short)f2bf(b - bf2f(h1)), (short)f2bf(c - bf2f(h2)),
+             (short)f2bf(d - bf2f(h3))};
+  return s;
+}
+__device__ __forceinline__ s4v neg4(s4v v) {
+  return s4v{(short)(v[0] ^ (short)0x8000), (short)(v[1] ^ (short)0x8000),
+             (short)(v[2] ^ (short)0x8000), (short)(v[3] ^ (short)0x8000)};
+}
+
+constexpr int BT = 64;    // block tile (both output dims)
+constexpr int BKS = 16;   // reduction depth per LDS stage
+constexpr int LPAD = 4;
+constexpr int GT = 256;   // threads per block
+
+// Row loader: either a plain k-contiguous matrix or beta generated from EIGR x PKG_A.
+template <typename T, bool BETA>
+struct RowLoader {
+  const cx<T>* A; int lda;       // plain
+  const cx<T>* E; const T* P;    // beta: E [nat][ld], P [lm][ld]
+  int a0, lm, ld;
+  __device__ __forceinline__ cx<T> operator()(int m, int k) const {
+    if constexpr (BETA) {
+      const int a = m / lm, l = m - a * lm;
+      const cx<T> e = E[(size_t)(a0 + a) * ld + k];
+      const T p = P[(size_t)l * ld + k];
+      return mk<T>(e.x * p, e.y * p);
+    } else {
+      return A[(size_t)m * lda + k];
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// proj: part[s][m][n] = sum_{k in chunk s} conj(A[m][k]) B[n][k]
+// ---------------------------------------------------------------------------
+template <typename T, bool BETA, bool BF16X3>
+__global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const cx<T>* __restrict__ B,
+                                                  int ldb, int M, int N, int K, int kchunk,
+                                                  cx<T>* __restrict__ part) {
+  using acc_t = typename AccT<T>::type;
+  __shared__ T sAr[BKS][BT + LPAD], sAi[BKS][BT + LPAD], sBr[BKS][BT + LPAD], sBi[BKS][BT + LPAD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tiles_n = (N + BT - 1) / BT;
+  const int m0 = (blockIdx.x / tiles_n) * BT, n0 = (blockIdx.x % tiles_n) * BT;
+  const int ks = blockIdx.y * kchunk, ke = min(K, ks + kchunk);
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int li = lane & 15, lq = lane >> 4;
+  acc_t cr[2][2], ci[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) { cr[a][b] = acc_t{0, 0, 0, 0}; ci[a][b] = acc_t{0, 0, 0, 0}; }
+
+  for (int k0 = ks; k0 < ke; k0 += BKS) {
+#pragma unroll
+    for (int e = tid; e < BT * BKS; e += GT) {
+      const int kk = e % BKS, rr = e / BKS;
+      const int k = k0 + kk;
+      cx<T> va = mk<T>(0, 0), vb = mk<T>(0, 0);
+      if (k < ke) {
+        if (m0 + rr < M) va = ld(m0 + rr, k);
+        if (n0 + rr < N) vb = B[(size_t)(n0 + rr) * ldb + k];
+      }
+      sAr[kk][rr] = va.x; sAi[kk][rr] = va.y;
+      sBr[kk][rr] = vb.x; sBi[kk][rr] = vb.y;
+    }
+    __syncthreads();
+    if constexpr (!BF16X3) {
+#pragma unroll
+      for (int kk = 0; kk < BKS; kk += 4) {
+        T ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          ar[t] = sAr[kk + lq][wm + t * 16 + li]; ai[t] = sAi[kk + lq][wm + t * 16 + li];
+          br[t] = sBr[kk + lq][wn + t * 16 + li]; bi[t] = sBi[kk + lq][wn + t * 16 + li];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            cr[a][b] = mfma4(ar[a], br[b], cr[a][b]);
+            cr[a][b] = mfma4(ai[a], bi[b], cr[a][b]);
+            ci[a][b] = mfma4(ar[a], bi[b], ci[a][b]);
+            ci[a][b] = mfma4(-ai[a], br[b], ci[a][b]);
+          }
+      }
+    } else {
+      Split4 ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int c = wm + t * 16 + li, d = wn + t * 16 + li, q = 4 * lq;
+        ar[t] = split4(sAr[q][c], sAr[q + 1][c], sAr[q + 2][c], sAr[q + 3][c]);
+        ai[t] = split4(sAi[q][c], sAi[q + 1][c], sAi[q + 2][c], sAi[q + 3][c]);
+        br[t] = split4(sBr[q][d], sBr[q + 1][d], sBr[q + 2][d], sBr[q + 3][d]);
+        bi[t] = split4(sBi[q][d], sBi[q + 1][d], sBi[q + 2][d], sBi[q + 3][d]);
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const s4v nai_h = neg4(ai[a].hi), nai_l = neg4(ai[a].lo);
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          f4v r = cr[a][b], i = ci[a][b];
+          r = mfma16bf(ar[a].hi, br[b].hi, r); r = mfma16bf(ar[a].hi, br[b].lo, r); r = mfma16bf(ar[a].lo, br[b].hi, r);
+          r = mfma16bf(ai[a].hi, bi[b].hi, r); r = mfma16bf(ai[a].hi, bi[b].lo, r); r = mfma16bf(ai[a].lo, bi[b].hi, r);
+          i = mfma16bf(ar[a].hi, bi[b].hi, i); i = mfma16bf(ar[a].hi, bi[b].lo, i); i = mfma16bf(ar[a].lo, bi[b].hi, i);
+          i = mfma16bf(nai_h, br[b].hi, i); i = mfma16bf(nai_h, br[b].lo, i); i = mfma16bf(nai_l, br[b].hi, i);
+          cr[a][b] = r; ci[a][b] = i;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  cx<T>* P = part + (size_t)blockIdx.y * M * N;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm + a * 16 + 4 * lq + r, n = n0 + wn + b * 16 + li;
+        if (m < M && n < N) P[(size_t)m * N + n] = mk<T>(cr[a][b][r], ci[a][b][r]);
+      }
+}
+
+// C = scale(m) * sum_s part[s][m][n], stored C[m*ldc + n] (row-major) or C[m + n*ldc]
+// (column-major, the LAPACK layout handed to rocSOLVER); scale(m) = wl[m % lm] when wl != null
+template <typename T, typename TO>
+__global__ void k_reduce_parts(const cx<T>* __restrict__ part, int S, int M, int N,
+                               cx<TO>* __restrict__ C, int ldc, const T* __restrict__ wl, int lm,
+                               int colmajor) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= M * N) return;
+  const int m = idx / N, n = idx - m * N;
+  TO sr = 0, si = 0;
+  for (int s = 0; s < S; ++s) {
+    const cx<T> v = part[(size_t)s * M * N + idx];
+    sr += (TO)v.x; si += (TO)v.y;
+  }
+  if (wl) { const TO w = (TO)wl[m % lm]; sr *= w; si *= w; }
+  if (colmajor) C[m + (size_t)n * ldc] = mk<TO>(sr, si);
+  else C[(size_t)m * ldc + n] = mk<TO>(sr, si);
+}
+
+// ---------------------------------------------------------------------------
+// expand: out[n][k] (+)= sum_j A[j][k] Q[j][n]
+// ---------------------------------------------------------------------------
+template <typename T, bool BETA, bool BF16X3, bool QCM>
+__global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const cx<T>* __restrict__ Q,
+                                                    int ldq, int J, int N, int K,
+                                                    cx<T>* __restrict__ out, int ldo, int accumulate) {
+  using acc_t = typename AccT<T>::type;
+  __shared__ T sAr[BKS][BT + LPAD], sAi[BKS][BT + LPAD], sQr[BKS][BT + LPAD], sQi[BKS][BT + LPAD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int k0 = blockIdx.x * BT, n0 = blockIdx.y * BT;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int li = lane & 15, lq = lane >> 4;
+  acc_t cr[2][2], ci[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) { cr[a][b] = acc_t{0, 0, 0, 0}; ci[a][b] = acc_t{0, 0, 0, 0}; }
+
+  for (int j0 = 0; j0 < J; j0 += BKS) {
+#pragma unroll
+    for (int e = tid; e < BT * BKS; e += GT) {
+      const int kk = e % BT, jj = e / BT;
+      const int j = j0 + jj;
+      cx<T> va = mk<T>(0, 0), vq = mk<T>(0, 0);
+      if (j < J) {
+        if (k0 + kk < K) va = ld(j, k0 + kk);
+        if (n0 + kk < N) vq = QCM ? Q[j + (size_t)(n0 + kk) * ldq] : Q[(size_t)j * ldq + n0 + kk];
+      }
+      sAr[jj][kk] = va.x; sAi[jj][kk] = va.y;
+      sQr[jj][kk] = vq.x; sQi[jj][kk] = vq.y;
+    }
+    __syncthreads();
+    if constexpr (!BF16X3) {
+#pragma unroll
+      for (int jj = 0; jj < BKS; jj += 4) {
+        T ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          ar[t] = sAr[jj + lq][wm + t * 16 + li]; ai[t] = sAi[jj + lq][wm + t * 16 + li];
+          br[t] = sQr[jj + lq][wn + t * 16 + li]; bi[t] = sQi[jj + lq][wn + t * 16 + li];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            cr[a][b] = mfma4(ar[a], br[b], cr[a][b]);
+            cr[a][b] = mfma4(-ai[a], bi[b], cr[a][b]);
+            ci[a][b] = mfma4(ar[a], bi[b], ci[a][b]);
+            ci[a][b] = mfma4(ai[a], br[b], ci[a][b]);
+          }
+      }
+    } else {
+      Split4 ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int c = wm + t * 16 + li, d = wn + t * 16 + li, q = 4 * lq;
+        ar[t] = split4(sAr[q][c], sAr[q + 1][c], sAr[q + 2][c], sAr[q + 3][c]);
+        ai[t] = split4(sAi[q][c], sAi[q + 1][c], sAi[q + 2][c], sAi[q + 3][c]);
+        br[t] = split4(sQr[q][d], sQr[q + 1][d], sQr[q + 2][d], sQr[q + 3][d]);
+        bi[t] = split4(sQi[q][d], sQi[q + 1][d], sQi[q + 2][d], sQi[q + 3][d]);
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const s4v nai_h = neg4(ai[a].hi), nai_l = neg4(ai[a].lo);
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          f4v r = cr[a][b], i = ci[a][b];
+          r = mfma16bf(ar[a].hi, br[b].hi, r); r = mfma16bf(ar[a].hi, br[b].lo, r); r = mfma16bf(ar[a].lo, br[b].hi, r);
+          r = mfma16bf(nai_h, bi[b].hi, r); r = mfma16bf(nai_h, bi[b].lo, r); r = mfma16bf(nai_l, bi[b].hi, r);
+          i = mfma16bf(ar[a].hi, bi[b].hi, i); i = mfma16bf(ar[a].hi, bi[b].lo, i); i = mfma16bf(ar[a].lo, bi[b].hi, i);
+          i = mfma16bf(ai[a].hi, br[b].hi, i); i = mfma16bf(ai[a].hi, br[b].lo, i); i = mfma16bf(ai[a].lo, br[b].hi, i);
+          cr[a][b] = r; ci[a][b] = i;
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = k0 + wm + a * 16 + 4 * lq + r, n = n0 + wn + b * 16 + li;
+        if (k < K && n < N) {
+          cx<T> v = mk<T>(cr[a][b][r], ci[a][b][r]);
+          cx<T>* o = out + (size_t)n * ldo + k;
+          if (accumulate) v = v + *o;
+          *o = v;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static int choose_split(int tiles, int K, size_t part_elems, size_t MN) {
+  // aim for ~2 blocks per CU on 256 CUs, at least 128 plane waves per block
+  int s = (512 + tiles - 1) / tiles;
+  int smax = (K + 127) / 128;
+  if (s > smax) s = smax;
+  if (s < 1) s = 1;
+  while (s > 1 && (size_t)s * MN > part_elems) --s;
+  return s;
+}
+
+size_t proj_part_elems(int M, int N, int K) {
+  const int tiles = ((M + BT - 1) / BT) * ((N + BT - 1) / BT);
+  const int s = choose_split(tiles, K, (size_t)-1, (size_t)M * N);
+  return (size_t)s * M * N;
+}
+
+template <typename T, bool BETA>
+static hipError_t run_proj(RowLoader<T, BETA> ld, const cx<T>* B, int ldb, int M, int N, int K,
+                           cx<T>* part, size_t part_elems, bool bf16, hipStream_t s, int* S_out) {
+  const int tiles = ((M + BT - 1) / BT) * ((N + BT - 1) / BT);
+  int S = choose_split(tiles, K, part_elems, (size_t)M * N);
+  int kchunk = (K + S - 1) / S;
+  kchunk = ((kchunk + BKS - 1) / BKS) * BKS;
+  S = (K + kchunk - 1) / kchunk;
+  if ((size_t)S * M * N > part_elems) return hipErrorInvalidValue;
+  dim3 grid(tiles, S);
+  if constexpr (std::is_same<T, float>::value) {
+    if (bf16) {
+      hipLaunchKernelGGL((k_gemm_proj<T, BETA, true>), grid, dim3(GT), 0, s, ld, B, ldb, M, N, K, kchunk, part);
+    } else {
+      hipLaunchKernelGGL((k_gemm_proj<T, BETA, false>), grid, dim3(GT), 0, s, ld, B, ldb, M, N, K, kchunk, part);
+    }
+  } else {
+    hipLaunchKernelGGL((k_gemm_proj<T, BETA, false>), grid, dim3(GT), 0, s, ld, B, ldb, M, N, K, kchunk, part);
+  }
+  *S_out = S;
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t gemm_proj(const BetaGen& bg, const cx<T>* X, int ldx, int nb, int K, cx<T>* C,
+                     cx<T>* part, size_t part_elems, int mode, hipStream_t s) {
+  RowLoader<T, true> ld{};
+  ld.E = static_cast<const cx<T>*>(bg.E);
+  ld.P = static_cast<const T*>(bg.P);
+  ld.a0 = bg.a0; ld.lm = bg.lm; ld.ld = bg.ld;
+  const int M = bg.na * bg.lm;
+  int S = 0;
+  hipError_t e = run_proj<T, true>(ld, X, ldx, M, nb, K, part, part_elems, mode == PREC_BF16X3, s, &S);
+  if (e != hipSuccess) return e;
+  const int tot = M * nb;
+  hipLaunchKernelGGL((k_reduce_parts<T, T>), dim3((tot + 255) / 256), dim3(256), 0, s, part, S, M,
+                     nb, C, nb, static_cast<const T*>(nullptr), bg.lm, 0);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t gemm_expand(const BetaGen& bg, const cx<T>* Q, int nb, int K, cx<T>* out, int ldo,
+                       bool accumulate, int mode, hipStream_t s) {
+  RowLoader<T, true> ld{};
+  ld.E = static_cast<const cx<T>*>(bg.E);
+  ld.P = static_cast<const T*>(bg.P);
+  ld.a0 = bg.a0; ld.lm = bg.lm; ld.ld = bg.ld;
+  const int J = bg.na * bg.lm;
+  dim3 grid((K + BT - 1) / BT, (nb + BT - 1) / BT);
+  if constexpr (std::is_same<T, float>::value) {
+    if (mode == PREC_BF16X3) {
+      hipLaunchKernelGGL((k_gemm_expand<T, true, true, false>), grid, dim3(GT), 0, s, ld, Q, nb, J, nb, K, out, ldo, accumulate ? 1 : 0);
+      return hipGetLastError();
+    }
+  }
+  hipLaunchKernelGGL((k_gemm_expand<T, true, false, false>), grid, dim3(GT), 0, s, ld, Q, nb, J, nb, K, out, ldo, accumulate ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t gemm_cn_f64(const cx<double>* A, int lda, int M, const cx<double>* B, int ldb, int N,
+                       int K, cx<double>* C, int ldc, cx<double>* part, size_t part_elems,
+                       hipStream_t s) {
+  RowLoader<double, false> ld{};
+  ld.A = A; ld.lda = lda;
+  int S = 0;
+  hipError_t e = run_proj<double, false>(ld, B, ldb, M, N, K, part, part_elems, false, s, &S);
+  if (e != hipSuccess) return e;
+  const int tot = M * N;
+  hipLaunchKernelGGL((k_reduce_parts<double, double>), dim3((tot + 255) / 256), dim3(256), 0, s, part,
+                     S, M, N, C, ldc, static_cast<const double*>(nullptr), 1, 1);
+  return hipGetLastError();
+}
+
+hipError_t gemm_nn_f64(const cx<double>* A, int lda, int J, const cx<double>* Q, int ldq, int N,
+                       int K, cx<double>* out, int ldo, hipStream_t s) {
+  RowLoader<double, false> ld{};
+  ld.A = A; ld.lda = lda;
+  dim3 grid((K + BT - 1) / BT, (N + BT - 1) / BT);
+  hipLaunchKernelGGL((k_gemm_expand<double, false, false, true>), grid, dim3(GT), 0, s, ld, Q, ldq, J, N, K,
+                     out, ldo, 0);
+  return hipGetLastError();
+}
+
+// scale_proj: Q[(a,l)][b] = wl[l] * C[(a,l)][b]
+template <typename T>
+__global__ void k_scale_proj(const cx<T>* __restrict__ C, const T* __restrict__ wl, int lm, int rows,
+                             int nb, cx<T>* __restrict__ Q) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * nb) return;
+  const int m = idx / nb;
+  Q[idx] = wl[m % lm] * C[idx];
+}
+
+template <typename T>
+hipError_t scale_proj(const cx<T>* Cin, const T* wl, int na, int lm, int nb, cx<T>* Q, hipStream_t s) {
+  const int tot = na * lm * nb;
+  hipLaunchKernelGGL((k_scale_proj<T>), dim3((tot + 255) / 256), dim3(256), 0, s, Cin, wl, lm, na * lm,
+                     nb, Q);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Self-test of the MFMA operand / result lane layouts assumed above: one wave
+// computes D = A(16x4) B(4x16) (kind 0: f64, 1: f32) or A(16x16) B(16x16)
+// (kind 2: bf16) and the host compares with a plain product.
+// ---------------------------------------------------------------------------
+__global__ void k_mfma_probe(int kind, const double* A, const double* B, double* D) {
+  const int lane = threadIdx.x, li = lane & 15, lq = lane >> 4;
+  if (kind == 0) {
+    d4v c = {0, 0, 0, 0};
+    c = mfma4(A[li * 4 + lq], B[lq * 16 + li], c);
+    for (int r = 0; r < 4; ++r) D[(4 * lq + r) * 16 + li] = c[r];
+  } else if (kind == 1) {
+    f4v c = {0, 0, 0, 0};
+    c = mfma4((float)A[li * 4 + lq], (float)B[lq * 16 + li], c);
+    for (int r = 0; r < 4; ++r) D[(4 * lq + r) * 16 + li] = c[r];
+  } else {
+    f4v c = {0, 0, 0, 0};
+    s4v a, b;
+    for (int v = 0; v < 4; ++v) {
+      a[v] = (short)f2bf((float)A[li * 16 + 4 * lq + v]);
+      b[v] = (short)f2bf((float)B[(4 * lq + v) * 16 + li]);
+    }
+    c = mfma16bf(a, b, c);
+    for (int r = 0; r < 4; ++r) D[(4 * lq + r) * 16 + li] = c[r];
+  }
+}
+
+hipError_t mfma_selftest(int kind, double* max_err) {
+  const int KD = (kind == 2) ? 16 : 4;
+  std::vector<double> hA(16 * KD), hB(KD * 16), hD(256), ref(256, 0.0);
+  for (int i = 0; i < 16 * KD; ++i) hA[i] = (double)((i * 7) % 13) - 6.0;   // small integers: exact
+  for (int i = 0; i < KD * 16; ++i) hB[i] = (double)((i * 5) % 11) - 5.0;
+  for (int i = 0; i < 16; ++i)
+    for (int j = 0; j < 16; ++j)
+      for (int k = 0; k < KD; ++k) ref[i * 16 + j] += hA[i * KD + k] * hB[k * 16 + j];
+  double *dA = nullptr, *dB = nullptr, *dD = nullptr;
+  hipError_t e;
+  if ((e = hipMalloc(&dA, sizeof(double) * hA.size())) != hipSuccess) return e;
+  if ((e = hipMalloc(&dB, sizeof(double) * hB.size())) != hipSuccess) { hipFree(dA); return e; }
+  if ((e = hipMalloc(&dD, sizeof(double) * 256)) != hipSuccess) { hipFree(dA); hipFree(dB); return e; }
+  hipMemcpy(dA, hA.data(), sizeof(double) * hA.size(), hipMemcpyHostToDevice);
+  hipMemcpy(dB, hB.data(), sizeof(double) * hB.size(), hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, 0, kind, dA, dB, dD);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(hD.data(), dD, sizeof(double) * 256, hipMemcpyDeviceToHost);
+  hipFree(dA); hipFree(dB); hipFree(dD);
+  double m = 0;
+  for (int i = 0; i < 256; ++i) m = fmax(m, fabs(hD[i] - ref[i]));
+  *max_err = m;
+  return e;
+}
+
+template hipError_t gemm_proj<double>(const BetaGen&, const cx<double>*, int, int, int, cx<double>*,
+                                      cx<double>*, size_t, int, hipStream_t);
+template hipError_t gemm_proj<float>(const BetaGen&, const cx<float>*, int, int, int, cx<float>*,
+                                     cx<float>*, size_t, int, hipStream_t);
+template hipError_t gemm_expand<double>(const BetaGen&, const cx<double>*, int, int, cx<double>*, int,
+                                        bool, int, hipStream_t);
+template hipError_t gemm_expand<float>(const BetaGen&, const cx<float>*, int, int, cx<float>*, int,
+                                       bool, int, hipStream_t);
+template hipError_t scale_proj<double>(const cx<double>*, const double*, int, int, int, cx<double>*,
+                                       hipStream_t);
+template hipError_t scale_proj<float>(const cx<float>*, const float*, int, int, int, cx<float>*,
+                                      hipStream_t);
+
+}  // namespace pwh

@@ -1,0 +1,109 @@
+// pwh_internal.h -- types shared by the HIP kernels and the host orchestration of
+// libpwh (MI355X / gfx950 Kohn-Sham H|psi> and Chebyshev-filtered subspace iteration).
+//
+// Data layout in HBM (all per system, one k-point):
+//   * "sphere order": the ngw plane waves are held in an internal order that groups
+//     them by FFT column (i1,i2) and, inside a column, by i3.  perm[k] gives the
+//     reference index ig (0-based) of internal slot k.  Every wave-function block is
+//     band-major [nb][ngw_pad] (plane waves contiguous, ld = ngw_pad).
+//   * W: the z-transformed columns between the passes, [nb][nr3][ncol_pad].
+//   * V: the local potential on the grid, [nr3][nr2][nr1] (x fastest), like the
+//     reference's rho argument.
+//   * E: structure-factor phases EIGR, [nat][ngw_pad]; P: PKG_A, [nsp][4][ngw_pad].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace pwh {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cx {
+  T x, y;
+};
+
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> mk(T a, T b) { cx<T> r; r.x = a; r.y = b; return r; }
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator*(cx<T> a, cx<T> b) {
+  return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator*(T s, cx<T> a) { return mk<T>(s * a.x, s * a.y); }
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> conjg(cx<T> a) { return mk<T>(a.x, -a.y); }
+// multiply by (sign * i)
+template <int SIGN, typename T>
+__host__ __device__ __forceinline__ cx<T> mul_si(cx<T> a) {
+  return SIGN > 0 ? mk<T>(-a.y, a.x) : mk<T>(a.y, -a.x);
+}
+
+enum Precision { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2 };
+
+// Device-side description of the sphere <-> grid maps used by the FFT passes.
+struct SphereMaps {
+  int nr1, nr2, nr3, nnr;
+  int ngw, ngw_pad;
+  int ncol, ncol_pad;   // columns (i1,i2) that hold sphere points; padded to the column tile
+  int nxa;              // number of distinct i1 among the columns
+  int* colstart;        // [ncol_pad+1] first internal slot of each column
+  int* kcol;            // [ngw] column of each internal slot
+  int* kz;              // [ngw] i3 of each internal slot
+  int* colxy;           // [ncol] i1 + nr1*i2 of each column
+  int* xa;              // [nxa] active i1 values
+};
+
+// Launchers (pwh_fft.hip)
+template <typename T>
+hipError_t launch_zinv(const SphereMaps& m, const void* tw3, const cx<T>* X, int ldx, cx<T>* W,
+                       int nb, hipStream_t s);
+template <typename T>
+hipError_t launch_plane(const SphereMaps& m, const void* tw1, const void* tw2, const T* V, cx<T>* W,
+                        int nb, hipStream_t s);
+template <typename T>
+hipError_t launch_zfwd_epilogue(const SphereMaps& m, const void* tw3, const cx<T>* W, const T* kin,
+                                const cx<T>* X, const cx<T>* Hnl, const cx<T>* Z, cx<T>* out,
+                                int ld, int nb, T inv_nnr, T alpha, T beta, T gamma,
+                                hipStream_t s);
+bool fft_size_supported(int n);
+bool plane_supported(int n1, int n2);
+
+// GEMM launchers (pwh_gemm.hip)
+struct BetaGen {
+  // beta[(a,l)][k] = E[a][k] * P[l][k] for atoms a0..a0+na-1 and l < lm
+  const void* E;   // cx<T> [nat][ld]
+  const void* P;   // T [lm][ld] (this species)
+  int a0, na, lm, ld;
+};
+template <typename T>
+hipError_t gemm_proj(const BetaGen& bg, const cx<T>* X, int ldx, int nb, int K, cx<T>* C,
+                     cx<T>* part, size_t part_elems, int mode, hipStream_t s);
+template <typename T>
+hipError_t gemm_expand(const BetaGen& bg, const cx<T>* Q, int nb, int K, cx<T>* out, int ldo,
+                       bool accumulate, int mode, hipStream_t s);
+// C[m][n] = sum_k conj(A[m][k]) B[n][k]  (A: [M][lda], B: [N][ldb]); FP64 result
+hipError_t gemm_cn_f64(const cx<double>* A, int lda, int M, const cx<double>* B, int ldb, int N,
+                       int K, cx<double>* C, int ldc, cx<double>* part, size_t part_elems,
+                       hipStream_t s);
+// out[n][k] = sum_j A[j][k] Q[j][n]  (A: [J][lda], Q: [J][ldq] row j, out: [N][ldo])
+hipError_t gemm_nn_f64(const cx<double>* A, int lda, int J, const cx<double>* Q, int ldq, int N,
+                       int K, cx<double>* out, int ldo, hipStream_t s);
+hipError_t mfma_selftest(int kind, double* max_err);
+
+// Elementwise / small kernels (pwh_blas.hip)
+template <typename Tin, typename Tout>
+hipError_t convert_cx(const cx<Tin>* in, cx<Tout>* out, size_t n, hipStream_t s);
+hipError_t permute_in_f64(const cx<double>* ref, int ldr, const int* perm, int ngw, int ngw_pad,
+                          cx<double>* out, int nb, hipStream_t s);
+hipError_t permute_out_f64(const cx<double>* in, int ngw_pad, const int* perm, int ngw,
+                           cx<double>* ref, int ldr, int nb, hipStream_t s);
+template <typename T>
+hipError_t scale_proj(const cx<T>* Cin, const T* wl, int na, int lm, int nb, cx<T>* Q, hipStream_t s);
+
+}  // namespace pwh

@@ -1,0 +1,375 @@
+"""pwh -- Python binding of libpwh, the gfx950 Kohn-Sham H|psi> / ChFSI library.
+
+A thin ctypes layer over the C ABI declared in include/pwh.h.  The names follow the
+reference routines they replace (ApplyHam_block, chebyfilt, ortho_qr, lanczos, the
+Rayleigh-Ritz and residual blocks of ChebySCFLoop).  Wave-function blocks are numpy
+complex128 arrays of shape (nstates, ngw), C-contiguous, which is exactly the
+reference's column-major [ngw x nstates] buffer.
+
+There is no fallback: every compute call goes through libpwh.so on the GPU, and
+importing this module fails if the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libpwh.so")
+SHIM_PATH = os.path.join(PKG_DIR, "libpwh_shim.so")
+REFGLOBALS_PATH = os.path.join(PKG_DIR, "libpwh_refglobals.so")
+
+FP64, FP32, BF16X3 = 0, 1, 2
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
+
+PWH_OK = 0
+PWH_ERR_ARG = -1
+PWH_ERR_HIP = -2
+PWH_ERR_UNSUPPORTED = -3
+PWH_ERR_LAPACK = -4
+
+# Every function declared in include/pwh.h (checked by the CPU test suite).
+EXPORTS = [
+    "pwh_init", "pwh_last_error", "pwh_device_count", "pwh_selftest_mfma", "pwh_grid_supported",
+    "pwh_fft_grid", "pwh_gvec_sizes", "pwh_gvectors", "pwh_phase_factor",
+    "pwh_system_create", "pwh_system_destroy", "pwh_system_info", "pwh_set_vloc",
+    "pwh_apply_ham_block", "pwh_chebyfilt", "pwh_ortho_qr", "pwh_lanczos", "pwh_rayleigh_ritz",
+    "pwh_residual_norms", "pwh_chfsi", "pwh_resident_set", "pwh_resident_get",
+    "pwh_resident_apply", "pwh_resident_chebyfilt", "pwh_synchronize", "pwh_timing_enable",
+    "pwh_timing_reset", "pwh_timing_get",
+]
+
+
+class PwhError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libpwh error {code}: {msg}")
+        self.code = code
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [("nr1", C.c_int), ("nr2", C.c_int), ("nr3", C.c_int), ("ngw", C.c_int),
+                ("ngwx", C.c_int), ("n123", C.c_void_p), ("xkg", C.c_void_p),
+                ("tpiba2", C.c_double), ("nsp", C.c_int), ("nax", C.c_int), ("nlmax", C.c_int),
+                ("na", C.c_void_p), ("l_max", C.c_void_p), ("l_loc", C.c_void_p),
+                ("wnl", C.c_void_p), ("eigr", C.c_void_p), ("pkg_a", C.c_void_p)]
+
+
+class ChfsiOpts(C.Structure):
+    _fields_ = [("degree", C.c_int), ("niter", C.c_int), ("precision", C.c_int),
+                ("lanczos_k", C.c_int), ("bounds_mode", C.c_int), ("lb", C.c_double),
+                ("ub", C.c_double), ("t_filter_s", C.c_double), ("t_ortho_s", C.c_double),
+                ("t_rr_s", C.c_double), ("t_lanczos_s", C.c_double), ("t_total_s", C.c_double),
+                ("band_applies", C.c_longlong)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libpwh.so (raises if it has not been built: there is no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                          f"(make -C {PKG_DIR})")
+    L = C.CDLL(LIB_PATH)
+    L.pwh_last_error.restype = C.c_char_p
+    vp = C.c_void_p
+    proto = {
+        "pwh_init": [C.c_int],
+        "pwh_device_count": [C.POINTER(C.c_int)],
+        "pwh_selftest_mfma": [C.c_int, C.POINTER(C.c_double)],
+        "pwh_grid_supported": [C.c_int, C.c_int, C.c_int],
+        "pwh_fft_grid": [_dp, C.c_double, _ip],
+        "pwh_gvec_sizes": [_dp, C.c_double, _ip, _dp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+        "pwh_gvectors": [_dp, C.c_double, _ip, _dp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
+                         _ip, _ip, _ip, _dp, _up, _dp],
+        "pwh_phase_factor": [_dp, _ip, C.c_int, _ip, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip,
+                             C.c_int, _ip, vp],
+        "pwh_system_create": [C.POINTER(SystemDesc), C.POINTER(vp)],
+        "pwh_system_destroy": [vp],
+        "pwh_system_info": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                            C.POINTER(C.c_size_t)],
+        "pwh_set_vloc": [vp, _dp, C.c_int],
+        "pwh_apply_ham_block": [vp, vp, vp, C.c_int, C.c_int, C.c_int],
+        "pwh_chebyfilt": [vp, vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int],
+        "pwh_ortho_qr": [vp, C.c_int, C.c_int],
+        "pwh_lanczos": [vp, vp, C.c_int, _dp, C.c_int, vp],
+        "pwh_rayleigh_ritz": [vp, vp, C.c_int, _dp],
+        "pwh_residual_norms": [vp, vp, C.c_int, _dp],
+        "pwh_chfsi": [vp, vp, C.c_int, vp, C.POINTER(ChfsiOpts), _dp, _dp],
+        "pwh_resident_set": [vp, vp, C.c_int],
+        "pwh_resident_get": [vp, vp, C.c_int],
+        "pwh_resident_apply": [vp, C.c_int, C.c_int],
+        "pwh_resident_chebyfilt": [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int],
+        "pwh_synchronize": [],
+        "pwh_timing_enable": [C.c_int],
+        "pwh_timing_reset": [],
+        "pwh_timing_get": [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int)],
+    }
+    for name, args in proto.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != PWH_OK:
+        raise PwhError(rc, lib().pwh_last_error().decode())
+
+
+def _zptr(a):
+    assert a.dtype == np.complex128 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _block(a):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a if a.ndim == 2 else a.reshape(1, -1)
+
+
+def _prec(p):
+    return PRECISIONS[p] if isinstance(p, str) else int(p)
+
+
+# --------------------------------------------------------------------------
+# library / setup
+# --------------------------------------------------------------------------
+
+def init(device=0):
+    _check(lib().pwh_init(device))
+
+
+def device_count():
+    n = C.c_int()
+    rc = lib().pwh_device_count(C.byref(n))
+    return n.value if rc == PWH_OK else 0
+
+
+def selftest_mfma(kind):
+    err = C.c_double()
+    _check(lib().pwh_selftest_mfma(kind, C.byref(err)))
+    return err.value
+
+
+def grid_supported(nr):
+    return bool(lib().pwh_grid_supported(int(nr[0]), int(nr[1]), int(nr[2])))
+
+
+def _lat(A):
+    return np.ascontiguousarray(np.asarray(A, np.float64).reshape(9))
+
+
+def fft_grid(A, ecut):
+    """Setup.cpp:20-34."""
+    nr = np.zeros(3, np.int32)
+    _check(lib().pwh_fft_grid(_lat(A), float(ecut), nr))
+    return tuple(int(x) for x in nr)
+
+
+def gvectors(A, ecut, nr, xk=None):
+    """GVectorsGen0/GVectorsGen/GkVectorsGen: the reference's G maps as its globals hold them."""
+    xk = np.zeros((1, 3)) if xk is None else np.asarray(xk, np.float64).reshape(-1, 3)
+    nkpt = xk.shape[0]
+    xkf = np.ascontiguousarray(xk.reshape(-1))
+    nr = np.asarray(nr, np.int32)
+    L = lib()
+    ngx, ngwx = C.c_int(), C.c_int()
+    _check(L.pwh_gvec_sizes(_lat(A), float(ecut), nr, xkf, nkpt, C.byref(ngx), C.byref(ngwx)))
+    ngx, ngwx = ngx.value, ngwx.value
+    ng = C.c_int()
+    in123 = np.zeros(3 * ngx, np.int32)
+    ngw = np.zeros(nkpt, np.int32)
+    igk = np.zeros(ngwx * nkpt, np.int32)
+    xkg = np.zeros(ngwx * nkpt)
+    n123 = np.zeros(ngwx * nkpt, np.uint32)
+    ggk = np.zeros(3 * ngwx * nkpt)
+    _check(L.pwh_gvectors(_lat(A), float(ecut), nr, xkf, nkpt, ngx, ngwx, C.byref(ng), in123, ngw,
+                          igk, xkg, n123, ggk))
+    return dict(ngx=ngx, ngwx=ngwx, ng=ng.value, in123=in123, ngw=ngw, igk=igk, xkg=xkg,
+                n123=n123, ggk=ggk, nkpt=nkpt)
+
+
+def phase_factor(A, nr, na, nax, tau, gv):
+    """PhaseFactor.cpp:5-80 -> EIGR [ngwx*nax*nsp*nkpt] complex128."""
+    na = np.asarray(na, np.int32)
+    eigr = np.zeros(gv["ngwx"] * nax * len(na) * gv["nkpt"], np.complex128)
+    _check(lib().pwh_phase_factor(_lat(A), np.asarray(nr, np.int32), len(na), na, nax,
+                                  np.ascontiguousarray(np.asarray(tau, np.float64).reshape(-1)),
+                                  gv["nkpt"], gv["ngw"], gv["ngwx"], gv["igk"], gv["ngx"],
+                                  gv["in123"], _zptr(eigr)))
+    return eigr
+
+
+def ortho_qr(X):
+    """ortho_qr.cpp:3-34 on X (nstates, nbasis); returns the orthonormalised copy."""
+    X = np.array(_block(X), order="C")
+    _check(lib().pwh_ortho_qr(_zptr(X), X.shape[1], X.shape[0]))
+    return X
+
+
+def synchronize():
+    _check(lib().pwh_synchronize())
+
+
+def timing_enable(on=True):
+    _check(lib().pwh_timing_enable(1 if on else 0))
+
+
+def timing_reset():
+    _check(lib().pwh_timing_reset())
+
+
+def timing_get(name):
+    ms, n = C.c_double(), C.c_int()
+    _check(lib().pwh_timing_get(name.encode(), C.byref(ms), C.byref(n)))
+    return ms.value, n.value
+
+
+# --------------------------------------------------------------------------
+# the device-resident system
+# --------------------------------------------------------------------------
+
+@dataclass
+class Globals:
+    """The reference globals read by ApplyHam_block for one k-point."""
+    nr: tuple
+    ngw: int
+    ngwx: int
+    n123: np.ndarray    # uint32 [ngw] (or [ngwx]) 1-based
+    xkg: np.ndarray     # float64 [ngw]
+    tpiba2: float
+    na: np.ndarray      # int32 [nsp]
+    nax: int
+    l_max: np.ndarray   # int32 [nsp]
+    l_loc: np.ndarray   # int32 [nsp]
+    nlmax: int
+    wnl: np.ndarray     # float64 [nsp*nlmax]
+    eigr: np.ndarray    # complex128 [ngwx*nax*nsp]
+    pkg_a: np.ndarray   # float64 [nlmax*ngwx*nsp]
+
+
+class System:
+    """A pwh_system: maps, phases and projectors resident in HBM for one k-point."""
+
+    def __init__(self, g: Globals):
+        self._keep = [np.ascontiguousarray(g.n123, dtype=np.uint32),
+                      np.ascontiguousarray(g.xkg, dtype=np.float64),
+                      np.ascontiguousarray(g.na, dtype=np.int32),
+                      np.ascontiguousarray(g.l_max, dtype=np.int32),
+                      np.ascontiguousarray(g.l_loc, dtype=np.int32),
+                      np.ascontiguousarray(g.wnl, dtype=np.float64),
+                      np.ascontiguousarray(g.eigr, dtype=np.complex128),
+                      np.ascontiguousarray(g.pkg_a, dtype=np.float64)]
+        n123, xkg, na, lmx, llc, wnl, eigr, pkg = self._keep
+        d = SystemDesc(int(g.nr[0]), int(g.nr[1]), int(g.nr[2]), int(g.ngw), int(g.ngwx),
+                       n123.ctypes.data, xkg.ctypes.data, float(g.tpiba2), int(len(na)),
+                       int(g.nax), int(g.nlmax), na.ctypes.data, lmx.ctypes.data,
+                       llc.ctypes.data, wnl.ctypes.data, eigr.ctypes.data, pkg.ctypes.data)
+        h = C.c_void_p()
+        _check(lib().pwh_system_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self.ngw = int(g.ngw)
+        self.nnr = int(g.nr[0] * g.nr[1] * g.nr[2])
+        self._keep = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pwh_system_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        ngw, ncol, nxa, nbytes = C.c_int(), C.c_int(), C.c_int(), C.c_size_t()
+        _check(lib().pwh_system_info(self.h, C.byref(ngw), C.byref(ncol), C.byref(nxa),
+                                     C.byref(nbytes)))
+        return dict(ngw=ngw.value, ncol=ncol.value, nxa=nxa.value, device_bytes=nbytes.value)
+
+    def set_vloc(self, rho):
+        rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
+        _check(lib().pwh_set_vloc(self.h, rho, len(rho)))
+
+    def apply_ham_block(self, vin, precision=FP64):
+        """ApplyHam_block.cpp:5-172: returns H vin for vin (nstates, ngw)."""
+        vin = _block(vin)
+        vout = np.empty_like(vin)
+        _check(lib().pwh_apply_ham_block(self.h, _zptr(vin), _zptr(vout), vin.shape[1],
+                                         vin.shape[0], _prec(precision)))
+        return vout
+
+    def chebyfilt(self, X, degree, lb, ub, precision=FP64):
+        """chebyfilt.cpp:6-74 on a copy of X (nstates, ngw)."""
+        X = np.array(_block(X), order="C")
+        _check(lib().pwh_chebyfilt(self.h, _zptr(X), X.shape[0], int(degree), float(lb), float(ub),
+                                   _prec(precision)))
+        return X
+
+    def lanczos(self, v0, k):
+        """lanczos.cpp:8-101.  Returns (T[k,k] as the reference fills it, f, v0 normalised)."""
+        v0 = np.array(np.asarray(v0, np.complex128).reshape(-1), order="C")
+        T = np.zeros(k * k)
+        f = np.zeros(self.ngw, np.complex128)
+        _check(lib().pwh_lanczos(self.h, _zptr(v0), int(k), T, int(k), _zptr(f)))
+        return T.reshape(k, k).T.copy(), f, v0
+
+    def rayleigh_ritz(self, X):
+        """ChebySCFLoop.cpp:292-311: returns (eigenvalues, rotated X)."""
+        X = np.array(_block(X), order="C")
+        w = np.zeros(X.shape[0])
+        _check(lib().pwh_rayleigh_ritz(self.h, _zptr(X), X.shape[0], w))
+        return w, X
+
+    def residual_norms(self, X):
+        """ChebySCFLoop.cpp:415-433."""
+        X = _block(X)
+        r = np.zeros(X.shape[0])
+        _check(lib().pwh_residual_norms(self.h, _zptr(X), X.shape[0], r))
+        return r
+
+    def chfsi(self, X, degree=10, niter=10, precision=FP64, lanczos_k=20, bounds_mode=0,
+              v0=None, lb=0.0, ub=0.0):
+        """ChFSI outer loop (eigen part of ChebySCFLoop.cpp:271-311)."""
+        X = np.array(_block(X), order="C")
+        if v0 is None:
+            v0 = np.ones(self.ngw, np.complex128)
+        v0 = np.array(np.asarray(v0, np.complex128).reshape(-1), order="C")
+        o = ChfsiOpts(int(degree), int(niter), _prec(precision), int(lanczos_k), int(bounds_mode),
+                      float(lb), float(ub), 0, 0, 0, 0, 0, 0)
+        eig = np.zeros(X.shape[0])
+        res = np.zeros(X.shape[0])
+        _check(lib().pwh_chfsi(self.h, _zptr(X), X.shape[0], _zptr(v0), C.byref(o), eig, res))
+        stats = {f: getattr(o, f) for f, _ in ChfsiOpts._fields_}
+        return eig, X, res, stats
+
+    # device-resident entry points
+    def resident_set(self, X):
+        X = _block(X)
+        _check(lib().pwh_resident_set(self.h, _zptr(X), X.shape[0]))
+
+    def resident_get(self, nstates):
+        X = np.zeros((nstates, self.ngw), np.complex128)
+        _check(lib().pwh_resident_get(self.h, _zptr(X), nstates))
+        return X
+
+    def resident_apply(self, nstates, precision=FP64):
+        _check(lib().pwh_resident_apply(self.h, int(nstates), _prec(precision)))
+
+    def resident_chebyfilt(self, nstates, degree, lb, ub, precision=FP64):
+        _check(lib().pwh_resident_chebyfilt(self.h, int(nstates), int(degree), float(lb),
+                                            float(ub), _prec(precision)))

@@ -1,0 +1,136 @@
+"""Synthetic Si-diamond systems for the benchmark and parity configurations.
+
+Each system is what the reference's Setup/PrepareSCF would leave in its globals for a
+simple-cubic cell of Si atoms on a diamond lattice at the Gamma point, with two
+synthetic ingredients in place of pseudopotential files:
+
+  * local potential  V(r) = sum_atoms sum_images -2 exp(-|r - R|^2 / 2)  Ha
+    on the grid r = (i1/NR1, i2/NR2, i3/NR3) * a (the rho argument of ApplyHam_block);
+  * 4 projectors per atom (s and p, i.e. L_MAX = 3, L_LOC = 3 -> lm_end = 4),
+    PKG_A(l, G) = exp(-(0.8 |G|)^2 / 2) * {1, Gx/|G|, Gy/|G|, Gz/|G|}, |G| in 1/bohr,
+    p components zero at G = 0 (the unnormalised real-harmonic convention of
+    NLFormFactor.cpp:56-86), WNL = (+1, -0.5, -0.5, -0.5) Ha.
+
+The G maps and phases come from libpwh's host setup (pwh_gvectors / pwh_phase_factor,
+bit-exact with GVectorsGen / GkVectorsGen / PhaseFactor).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import Globals, gvectors, phase_factor
+
+# 8-atom cubic diamond cell, fractional coordinates in the order of works/Si8/INPUT:25-32
+DIAMOND8 = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0],
+                     [.25, .25, .25], [.75, .25, .75], [.25, .75, .75], [.75, .75, .25]])
+
+WNL_SP = (1.0, -0.5, -0.5, -0.5)
+PROJ_WIDTH = 0.8     # bohr
+V_DEPTH = -2.0       # Ha
+V_WIDTH = 1.0        # bohr
+
+# name -> (a [bohr], ecut [Ry], cells per edge, FFT grid, bands)
+CONFIGS = {
+    "si8": (10.26, 15.0, 1, (32, 32, 32), 16),
+    "si64": (20.52, 20.0, 2, (64, 64, 64), 128),
+    "si216": (30.8, 20.0, 3, (96, 96, 96), 432),
+}
+
+
+def diamond_fractional(ncell):
+    """Fractional positions of an ncell^3 supercell of the 8-atom diamond cell."""
+    out = []
+    for i in range(ncell):
+        for j in range(ncell):
+            for k in range(ncell):
+                out.append((DIAMOND8 + np.array([i, j, k])) / ncell)
+    return np.concatenate(out, axis=0)
+
+
+def gaussian_vloc(a, nr, tau):
+    """V on the grid, x fastest (flat [nr1*nr2*nr3]); cubic cell of edge a, images +-1."""
+    axes = [np.arange(n) / n * a for n in nr]
+    V = np.zeros((nr[2], nr[1], nr[0]))
+    shifts = np.array([-1.0, 0.0, 1.0]) * a
+    for R in tau:
+        g = []
+        for d in range(3):
+            x = axes[d][:, None] - R[d] + shifts[None, :]
+            g.append(np.exp(-x * x / (2.0 * V_WIDTH ** 2)).sum(axis=1))
+        V += g[2][:, None, None] * g[1][None, :, None] * g[0][None, None, :]
+    return (V_DEPTH * V).reshape(-1)
+
+
+@dataclass
+class SynthSystem:
+    name: str
+    a: float
+    ecut: float
+    nr: tuple
+    nbands: int
+    A: np.ndarray
+    tau: np.ndarray       # (natoms, 3) bohr
+    gv: dict
+    eigr: np.ndarray
+    pkg_a: np.ndarray
+    wnl: np.ndarray
+    vloc: np.ndarray
+    tpiba2: float
+
+    @property
+    def ngw(self):
+        return int(self.gv["ngw"][0])
+
+    @property
+    def natoms(self):
+        return int(self.tau.shape[0])
+
+    def globals(self) -> Globals:
+        ngw = self.ngw
+        return Globals(nr=self.nr, ngw=ngw, ngwx=self.gv["ngwx"], n123=self.gv["n123"][:ngw],
+                       xkg=self.gv["xkg"][:ngw], tpiba2=self.tpiba2,
+                       na=np.array([self.natoms], np.int32), nax=self.natoms,
+                       l_max=np.array([3], np.int32), l_loc=np.array([3], np.int32), nlmax=4,
+                       wnl=self.wnl, eigr=self.eigr, pkg_a=self.pkg_a)
+
+    def psi(self, nbands=None, seed=0):
+        """complex128 i.i.d. N(0,1) coefficients from the seed, QR-orthonormalised (nb, ngw)."""
+        nb = self.nbands if nbands is None else nbands
+        rng = np.random.default_rng(seed)
+        X = rng.standard_normal((nb, self.ngw)) + 1j * rng.standard_normal((nb, self.ngw))
+        Q, _ = np.linalg.qr(X.T, mode="reduced")
+        return np.ascontiguousarray(Q.T)
+
+
+def make_system(a, ecut, ncell, nr, nbands, name="custom", natoms=None):
+    A = np.diag([a, a, a]).astype(np.float64)
+    frac = diamond_fractional(ncell)
+    if natoms is not None:
+        frac = frac[:natoms]
+    tau = frac * a
+    gv = gvectors(A, ecut, nr)
+    nat = tau.shape[0]
+    eigr = phase_factor(A, nr, [nat], nat, tau.reshape(1, nat, 3), gv)
+    ngw, ngwx = int(gv["ngw"][0]), gv["ngwx"]
+    tpiba = 2.0 * np.pi / a
+    xkg = gv["xkg"][:ngw]
+    t = np.sqrt(xkg)
+    q = np.exp(-((PROJ_WIDTH * t * tpiba) ** 2) / 2.0)
+    ggk = gv["ggk"][:3 * ngw].reshape(ngw, 3)
+    pkg = np.zeros((ngwx, 4))
+    pkg[:ngw, 0] = q
+    nz = t >= 1.0e-4
+    for d in range(3):
+        pkg[:ngw, 1 + d][nz] = q[nz] * ggk[nz, d] / t[nz]
+    pkg_a = np.ascontiguousarray(pkg.reshape(-1))     # l fastest: [l + 4*ig]
+    wnl = np.array(WNL_SP, np.float64)                 # WNL(1,l) at 0 + 1*l
+    vloc = gaussian_vloc(a, nr, tau)
+    return SynthSystem(name=name, a=a, ecut=ecut, nr=tuple(nr), nbands=nbands, A=A, tau=tau, gv=gv,
+                       eigr=eigr, pkg_a=pkg_a, wnl=wnl, vloc=vloc, tpiba2=tpiba * tpiba)
+
+
+def config(name):
+    a, ecut, ncell, nr, nb = CONFIGS[name]
+    return make_system(a, ecut, ncell, nr, nb, name=name)

@@ -1,0 +1,15 @@
+// pwh_refglobals.cpp -- definitions of the reference globals read by pwh_shim.cpp, with
+// the names and types of global_variables.cpp:23-132, for programs (the tests) that
+// link the shim without the reference's own global_variables.cpp.  A reference build
+// links its global_variables.cpp instead of this file.
+typedef double _Complex zcomplex;
+
+int NR1 = 0, NR2 = 0, NR3 = 0, NNR = 0;
+int NGWX = 0, NX = 0, NSP = 0, NAX = 0, NLMAX = 0, NKPT = 1;
+int *NGW = 0, *NA = 0, *L_MAX = 0, *L_LOC = 0;
+unsigned int *N123 = 0;
+double *XKG = 0;
+double TPIBA2 = 0.0;
+double *WNL = 0;
+double *PKG_A = 0;
+zcomplex *EIGR = 0;

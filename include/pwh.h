@@ -1,0 +1,149 @@
+/*
+ * pwh.h -- C ABI of libpwh: MI355X (gfx950) Kohn-Sham H|psi> application and
+ * Chebyshev-filtered subspace iteration, a drop-in for the hot path of
+ * f-fathurrahman/ffr-pspw-dft-c.
+ *
+ * Conventions (identical to the reference's globals and arguments):
+ *   - complex arrays are interleaved (re, im) doubles, i.e. C99 `double complex`;
+ *   - wave-function blocks are column-major [ngw x nstates] with ld = ngw (the
+ *     reference's vin/vout/X, ApplyHam_block.cpp:34-61);
+ *   - N123 is 1-based, grid index i1 + NR1*(i2 + NR2*i3) (GkVectorsGen.cpp:30);
+ *   - EIGR(ig, ia, is) at ig + ld*(ia + NAX*is); PKG_A(l, ig, is) at
+ *     l + NLMAX*(ig + ld*is); WNL(is, l) at is + NSP*l (common_pspw.h:17-20).
+ * All pointers are host pointers unless the name says "device".  Every call returns
+ * PWH_OK (0) or a negative PWH_ERR_* code; pwh_last_error() describes the failure.
+ * The library is not reentrant: one caller thread, one HIP stream (the reference's
+ * own threading model, common_pspw.h globals).
+ */
+#ifndef PWH_H
+#define PWH_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PWH_OK 0
+#define PWH_ERR_ARG -1          /* invalid argument (the reference aborts: nnr != NNR, lm_end > 4) */
+#define PWH_ERR_HIP -2          /* HIP runtime failure (no device, launch failure, OOM) */
+#define PWH_ERR_UNSUPPORTED -3  /* grid size without a compiled FFT plan */
+#define PWH_ERR_LAPACK -4       /* factorisation failure (Cholesky / eigensolver info != 0) */
+
+#define PWH_FP64 0    /* FP64 FFT, FP64 MFMA GEMMs */
+#define PWH_FP32 1    /* FP32 FFT, FP32 MFMA GEMMs */
+#define PWH_BF16X3 2  /* FP32 FFT, BF16 hi/lo split GEMMs with FP32 accumulation */
+
+typedef struct pwh_system pwh_system;
+
+/* ---- library ------------------------------------------------------------ */
+/* Select the HIP device and create the library stream (idempotent). */
+int pwh_init(int device);
+const char *pwh_last_error(void);
+int pwh_device_count(int *count);
+/* Self-test of the matrix-core lane layouts: kind 0 f64, 1 f32, 2 bf16. */
+int pwh_selftest_mfma(int kind, double *max_err);
+/* Whether the FFT passes have a compiled plan for this grid. */
+int pwh_grid_supported(int nr1, int nr2, int nr3);
+
+/* ---- setup: producers of the reference's hot-path globals (host, FP64) ------ */
+/* Fourier grid from the cutoff: Setup.cpp:20-34.  A = A1,A2,A3 (bohr), ecut in Ry. */
+int pwh_fft_grid(const double A[9], double ecut, int nr[3]);
+/* NGX, NGWX: GVectorsGen0.cpp:5-63 + Setup.cpp:63-66. xk: XK(1:3,1:nkpt) in 2pi/a. */
+int pwh_gvec_sizes(const double A[9], double ecut, const int nr[3], const double *xk, int nkpt,
+                   int *ngx, int *ngwx);
+/* G and G+k maps: GVectorsGen.cpp:5-137 (with sort1.cpp:6-55) and GkVectorsGen.cpp:5-38.
+ * in123 [3*ngx] (IN1,IN2,IN3), ngw [nkpt], igk/xkg/n123 [ngwx*nkpt], ggk [3*ngwx*nkpt]. */
+int pwh_gvectors(const double A[9], double ecut, const int nr[3], const double *xk, int nkpt,
+                 int ngx, int ngwx, int *ng, int *in123, int *ngw, int *igk, double *xkg,
+                 unsigned int *n123, double *ggk);
+/* EIGR = exp(-i G.tau): PhaseFactor.cpp:5-80.  tau = TAU(1:3,1:nax,1:nsp) bohr.
+ * eigr [ngwx*nax*nsp*nkpt]. */
+int pwh_phase_factor(const double A[9], const int nr[3], int nsp, const int *na, int nax,
+                     const double *tau, int nkpt, const int *ngw, int ngwx, const int *igk,
+                     int ngx, const int *in123, double *eigr /* complex */);
+
+/* ---- the device-resident system (one k-point) ------------------------------ */
+typedef struct {
+  int nr1, nr2, nr3;          /* NR1..NR3 */
+  int ngw;                    /* NGW(ik) */
+  int ngwx;                   /* NGWX: leading dimension of eigr/pkg_a */
+  const unsigned int *n123;   /* N123(:,ik) [ngw], 1-based */
+  const double *xkg;          /* XKG(:,ik) [ngw], |k+G|^2 in (2pi/a)^2 */
+  double tpiba2;              /* TPIBA2 */
+  int nsp, nax, nlmax;        /* NSP, NAX, NLMAX */
+  const int *na;              /* NA [nsp] */
+  const int *l_max;           /* L_MAX [nsp] */
+  const int *l_loc;           /* L_LOC [nsp] */
+  const double *wnl;          /* WNL [nsp*nlmax] */
+  const double *eigr;         /* EIGR(:,:,:,ik) [ngwx*nax*nsp] complex */
+  const double *pkg_a;        /* PKG_A(:,:,:,ik) [nlmax*ngwx*nsp] */
+} pwh_system_desc;
+
+/* Uploads the maps, phases and projectors; replaces the reference's per-call reads of
+ * the globals N123, XKG, EIGR, PKG_A, WNL (ApplyHam_block.cpp:34-163). */
+int pwh_system_create(const pwh_system_desc *desc, pwh_system **sys);
+int pwh_system_destroy(pwh_system *sys);
+int pwh_system_info(const pwh_system *sys, int *ngw, int *ncol, int *nxa, size_t *device_bytes);
+/* The rho argument of ApplyHam_block (V_eff on the grid, EvalEnergy.cpp:128-131).
+ * nnr != NR1*NR2*NR3 -> PWH_ERR_ARG (ApplyHam_block.cpp:18-21). */
+int pwh_set_vloc(pwh_system *sys, const double *rho, int nnr);
+
+/* ---- the hot path (host pointers, reference order) ------------------------- */
+/* ApplyHam_block.cpp:5-172 (ApplyHam.cpp:5-142 for nstates = 1): vout = H vin. */
+int pwh_apply_ham_block(pwh_system *sys, const double *vin, double *vout, int ngw_ik, int nstates,
+                        int precision);
+/* chebyfilt.cpp:6-74: degree-m scaled Chebyshev filter on X [ngw x nstates], in place. */
+int pwh_chebyfilt(pwh_system *sys, double *X, int nstates, int degree, double lb, double ub,
+                  int precision);
+/* ortho_qr.cpp:3-34: orthonormalise the columns of X [nbasis x nstates] in place (FP64
+ * CholeskyQR2 on the GPU; columns span the same space as Householder QR, column phases
+ * follow the Cholesky convention R_jj > 0). */
+int pwh_ortho_qr(double *X, int nbasis, int nstates);
+/* lanczos.cpp:8-101: k-step Lanczos (FP64) from v0 [ngw] (normalised in place);
+ * T [ldT x k] column-major as the reference fills it, f [ngw] the final residual. */
+int pwh_lanczos(pwh_system *sys, double *v0, int k, double *T, int ldT, double *f);
+/* Rayleigh-Ritz block, ChebySCFLoop.cpp:289-311 (+ symmetrize lobpcg_cpu.cpp:15-27,
+ * eig_zheevd.cpp:5-34): X orthonormal in, X <- X Q out, eig [nstates] ascending. */
+int pwh_rayleigh_ritz(pwh_system *sys, double *X, int nstates, double *eig);
+/* Residual report, ChebySCFLoop.cpp:415-433: res[j] = || H x_j - X (X^H H X)_sym e_j ||. */
+int pwh_residual_norms(pwh_system *sys, const double *X, int nstates, double *res);
+
+/* ---- ChFSI outer loop (eigen part of ChebySCFLoop.cpp:271-311) -------------- */
+typedef struct {
+  int degree;            /* CHEBYPOL_DEGREE */
+  int niter;             /* outer iterations */
+  int precision;         /* filter precision (PWH_FP64 / PWH_FP32 / PWH_BF16X3) */
+  int lanczos_k;         /* steps of the FP64 Lanczos bound estimate */
+  int bounds_mode;       /* 0: ub = max Ritz + ||f|| once, lb = largest RR value after each RR
+                            1: the reference's rules (ChebySCFLoop.cpp:176-196,355-368) */
+  double lb, ub;         /* in: used when lanczos_k == 0; out: final bounds */
+  double t_filter_s, t_ortho_s, t_rr_s, t_lanczos_s, t_total_s;   /* out: wall times */
+  long long band_applies;                                          /* out */
+} pwh_chfsi_opts;
+/* X [ngw x nstates] start vectors in, Ritz vectors out; eig/res [nstates]. v0 [ngw]: the
+ * Lanczos start vector (ChebySCFLoop.cpp:102-107), normalised in place. */
+int pwh_chfsi(pwh_system *sys, double *X, int nstates, double *v0, pwh_chfsi_opts *opts,
+              double *eig, double *res);
+
+/* ---- device-resident entry points (inputs already in HBM; used for timing) --- */
+/* Copy a block into / out of the system's resident FP64 wave-function buffer. */
+int pwh_resident_set(pwh_system *sys, const double *X, int nstates);
+int pwh_resident_get(pwh_system *sys, double *X, int nstates);
+/* One H|psi> over the resident block (nb bands) in the given precision. */
+int pwh_resident_apply(pwh_system *sys, int nstates, int precision);
+/* Chebyshev filter of the resident block (FP32 modes convert in and out once). */
+int pwh_resident_chebyfilt(pwh_system *sys, int nstates, int degree, double lb, double ub,
+                           int precision);
+int pwh_synchronize(void);
+
+/* ---- per-kernel timing with HIP events on the library stream ---------------- */
+int pwh_timing_enable(int on);
+int pwh_timing_reset(void);
+/* name: "zinv", "plane", "zfwd", "proj", "expand", "hpsi". total_ms / count of launches. */
+int pwh_timing_get(const char *name, double *total_ms, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PWH_H */

@@ -1,0 +1,344 @@
+"""CPU FP64 oracle for the reference's H|psi> / ChFSI path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product library never
+imports it.
+
+Two layers:
+  * liboracle.so (oracle.c): the C restatement of G-vector generation, phases,
+    the 3D FFT conventions, ApplyHam_block, chebyfilt and lanczos.
+  * numpy (LAPACK) restatements of the dense steps: ortho_qr (zgeqrf+zungqr,
+    ortho_qr.cpp:19-26), eig_zheevd ('V','U', eig_zheevd.cpp:22-25), symmetrize
+    (lobpcg_cpu.cpp:15-27), the Rayleigh-Ritz block (ChebySCFLoop.cpp:289-311),
+    the residual report (ChebySCFLoop.cpp:415-433) and the bound logic
+    (ChebySCFLoop.cpp:7-34,176-196,355-368).
+
+Optionally, oracle/_ref/libref.so (built from the reference's own sources by
+`make -C oracle ref`) is wrapped by `Reference` for cross-checks.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libref.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_zp = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
+
+
+def build():
+    """Compile liboracle.so (gcc) and, when the reference tree is present, oracle/_ref."""
+    subprocess.check_call(["make", "-s", "-C", HERE, "liboracle.so"])
+    if os.path.isdir("/root/reference"):
+        subprocess.check_call(["make", "-s", "-C", HERE, "ref"])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            subprocess.check_call(["make", "-s", "-C", HERE, "liboracle.so"])
+        L = C.CDLL(LIB_PATH)
+        L.orc_fft_grid.argtypes = [_dp, C.c_double, _ip]
+        L.orc_setup_scalars.argtypes = [_dp, C.c_double, _dp]
+        L.orc_gvec_sizes.argtypes = [_dp, C.c_double, _ip, _dp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_sort1.argtypes = [C.c_int, _dp, _ip]
+        L.orc_gvectors.restype = C.c_int
+        L.orc_gvectors.argtypes = [_dp, C.c_double, _ip, _dp, C.c_int, C.c_int, C.c_int,
+                                   C.POINTER(C.c_int), _ip, _ip, _ip, _dp, _up, _dp]
+        L.orc_phase_factor.argtypes = [_dp, _ip, C.c_int, _ip, C.c_int, _dp, C.c_int, _ip, C.c_int,
+                                       _ip, C.c_int, _ip, _zp]
+        L.orc_fft3d.argtypes = [_zp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_apply_ham_block.restype = C.c_int
+        L.orc_apply_ham_block.argtypes = [C.c_void_p, _zp, _zp, C.c_int, C.c_int, _dp, C.c_int, C.c_int]
+        L.orc_apply_ham_block_par.restype = C.c_int
+        L.orc_apply_ham_block_par.argtypes = [C.c_void_p, _zp, _zp, C.c_int, C.c_int, _dp, C.c_int,
+                                              C.c_int, C.c_int]
+        L.orc_chebyfilt.restype = C.c_int
+        L.orc_chebyfilt.argtypes = [C.c_void_p, C.c_int, _zp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                    C.c_double, _dp, C.c_int]
+        L.orc_lanczos.restype = C.c_int
+        L.orc_lanczos.argtypes = [C.c_void_p, C.c_int, _zp, C.c_int, C.c_int, _dp, C.c_int, _zp, _dp,
+                                  C.c_int, C.c_void_p]
+        L.orc_induced_norm_mat.restype = C.c_double
+        L.orc_induced_norm_mat.argtypes = [_dp, C.c_int, C.c_int]
+        _lib = L
+    return _lib
+
+
+# --------------------------------------------------------------------------
+# Setup restatements
+# --------------------------------------------------------------------------
+
+def lattice(A) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(A, dtype=np.float64).reshape(9))
+
+
+def fft_grid(A, ecut):
+    nr = np.zeros(3, np.int32)
+    lib().orc_fft_grid(lattice(A), float(ecut), nr)
+    return tuple(int(x) for x in nr)
+
+
+def setup_scalars(A, ecut):
+    out = np.zeros(6)
+    lib().orc_setup_scalars(lattice(A), float(ecut), out)
+    return dict(alat=out[0], omega=out[1], tpiba=out[2], tpiba2=out[3], gcut=out[4], gcutw=out[5])
+
+
+def sort1(arr):
+    arr = np.ascontiguousarray(arr, dtype=np.float64)
+    idx = np.zeros(len(arr), np.int32)
+    lib().orc_sort1(len(arr), arr, idx)
+    return idx
+
+
+def gvectors(A, ecut, nr, xk=None):
+    """Returns the G / G+k maps exactly as the reference's globals hold them."""
+    xk = np.zeros((1, 3)) if xk is None else np.asarray(xk, np.float64).reshape(-1, 3)
+    nkpt = xk.shape[0]
+    xkf = np.ascontiguousarray(xk.reshape(-1))
+    nr = np.asarray(nr, np.int32)
+    ngx, ngwx = C.c_int(), C.c_int()
+    L = lib()
+    L.orc_gvec_sizes(lattice(A), float(ecut), nr, xkf, nkpt, C.byref(ngx), C.byref(ngwx))
+    ngx, ngwx = ngx.value, ngwx.value
+    ng = C.c_int()
+    in123 = np.zeros(3 * ngx, np.int32)
+    ngw = np.zeros(nkpt, np.int32)
+    igk = np.zeros(ngwx * nkpt, np.int32)
+    xkg = np.zeros(ngwx * nkpt)
+    n123 = np.zeros(ngwx * nkpt, np.uint32)
+    ggk = np.zeros(3 * ngwx * nkpt)
+    rc = L.orc_gvectors(lattice(A), float(ecut), nr, xkf, nkpt, ngx, ngwx, C.byref(ng), in123, ngw,
+                        igk, xkg, n123, ggk)
+    if rc != 0:
+        raise RuntimeError("orc_gvectors: reference abort condition")
+    return dict(ngx=ngx, ngwx=ngwx, ng=ng.value, in123=in123, ngw=ngw, igk=igk, xkg=xkg,
+                n123=n123, ggk=ggk, nkpt=nkpt)
+
+
+def phase_factor(A, nr, na, nax, tau, gv):
+    """EIGR [ngwx*nax*nsp*nkpt] (PhaseFactor.cpp:5-80). tau: (nsp, nax, 3) bohr."""
+    na = np.asarray(na, np.int32)
+    nsp = len(na)
+    tau = np.ascontiguousarray(np.asarray(tau, np.float64).reshape(-1))
+    eigr = np.zeros(gv["ngwx"] * nax * nsp * gv["nkpt"], np.complex128)
+    lib().orc_phase_factor(lattice(A), np.asarray(nr, np.int32), nsp, na, nax, tau, gv["nkpt"],
+                           gv["ngw"], gv["ngwx"], gv["igk"], gv["ngx"], gv["in123"], eigr)
+    return eigr
+
+
+def fft3d(data, nr, inverse):
+    """fft_fftw3 conventions: data flat [nr1*nr2*nr3], x fastest."""
+    d = np.ascontiguousarray(np.array(data, dtype=np.complex128).reshape(-1))
+    lib().orc_fft3d(d, int(nr[0]), int(nr[1]), int(nr[2]), 1 if inverse else 0)
+    return d
+
+
+# --------------------------------------------------------------------------
+# The reference's globals for H|psi>
+# --------------------------------------------------------------------------
+
+class _OrcSys(C.Structure):
+    _fields_ = [("nr1", C.c_int), ("nr2", C.c_int), ("nr3", C.c_int), ("nnr", C.c_int),
+                ("ngwx", C.c_int),
+                ("n123", C.c_void_p), ("xkg", C.c_void_p), ("tpiba2", C.c_double),
+                ("nsp", C.c_int), ("nax", C.c_int), ("nlmax", C.c_int),
+                ("na", C.c_void_p), ("l_max", C.c_void_p), ("l_loc", C.c_void_p),
+                ("wnl", C.c_void_p), ("eigr", C.c_void_p), ("pkg_a", C.c_void_p)]
+
+
+@dataclass
+class Globals:
+    """The reference globals ApplyHam_block reads (common_pspw.h:43-148)."""
+    nr: tuple
+    ngwx: int
+    n123: np.ndarray     # uint32 [ngwx*nkpt], 1-based
+    xkg: np.ndarray      # f64 [ngwx*nkpt]
+    tpiba2: float
+    na: np.ndarray       # int32 [nsp]
+    nax: int
+    l_max: np.ndarray    # int32 [nsp]
+    l_loc: np.ndarray    # int32 [nsp]
+    nlmax: int
+    wnl: np.ndarray      # f64 [nsp*nlmax], WNL(is,l) at is + nsp*l
+    eigr: np.ndarray     # c128 [ngwx*nax*nsp*nkpt]
+    pkg_a: np.ndarray    # f64 [nlmax*ngwx*nsp*nkpt]
+
+    def cstruct(self):
+        self._keep = [np.ascontiguousarray(a) for a in (self.n123, self.xkg, self.na, self.l_max,
+                                                        self.l_loc, self.wnl, self.eigr, self.pkg_a)]
+        n123, xkg, na, lmx, llc, wnl, eigr, pkg = self._keep
+        s = _OrcSys(int(self.nr[0]), int(self.nr[1]), int(self.nr[2]),
+                    int(self.nr[0] * self.nr[1] * self.nr[2]), int(self.ngwx),
+                    n123.ctypes.data, xkg.ctypes.data, float(self.tpiba2),
+                    int(len(self.na)), int(self.nax), int(self.nlmax),
+                    na.ctypes.data, lmx.ctypes.data, llc.ctypes.data, wnl.ctypes.data,
+                    eigr.ctypes.data, pkg.ctypes.data)
+        self._s = s
+        return C.byref(s)
+
+    @property
+    def nnr(self):
+        return int(self.nr[0] * self.nr[1] * self.nr[2])
+
+
+def apply_ham_block(g: Globals, vin, rho, ngw, ik=1, nthreads=1):
+    """ApplyHam_block.cpp:5-172.  vin: (nstates, ngw) rows = bands (column-major ld=ngw)."""
+    vin = np.ascontiguousarray(vin, dtype=np.complex128)
+    nst = vin.shape[0] if vin.ndim == 2 else 1
+    vout = np.zeros_like(vin)
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    if nthreads > 1:
+        rc = lib().orc_apply_ham_block_par(g.cstruct(), vin.reshape(-1), vout.reshape(-1), ngw, ik,
+                                           rho, len(rho), nst, nthreads)
+    else:
+        rc = lib().orc_apply_ham_block(g.cstruct(), vin.reshape(-1), vout.reshape(-1), ngw, ik,
+                                       rho, len(rho), nst)
+    if rc != 0:
+        raise RuntimeError("ApplyHam_block: reference abort condition (nnr != NNR or lm_end > 4)")
+    return vout
+
+
+def chebyfilt(g: Globals, X, degree, lb, ub, rho, ngw, ik=1):
+    """chebyfilt.cpp:6-74 on X (nstates, ngw); returns the filtered copy."""
+    X = np.array(X, dtype=np.complex128, order="C")
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    rc = lib().orc_chebyfilt(g.cstruct(), ik, X.reshape(-1), ngw, X.shape[0], degree, lb, ub, rho,
+                             len(rho))
+    if rc != 0:
+        raise RuntimeError("chebyfilt: reference abort condition")
+    return X
+
+
+def lanczos(g: Globals, v0, k, rho, ngw, ik=1, want_v=False):
+    """lanczos.cpp:8-101. Returns (T[k,k] as the reference fills it, f, v0_normalised[, V])."""
+    v0 = np.array(v0, dtype=np.complex128)
+    T = np.zeros(k * k)
+    f = np.zeros(ngw, np.complex128)
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    V = np.zeros((k, ngw), np.complex128) if want_v else None
+    rc = lib().orc_lanczos(g.cstruct(), ik, v0, ngw, k, T, k, f, rho, len(rho),
+                           V.ctypes.data if want_v else None)
+    if rc != 0:
+        raise RuntimeError("lanczos: reference abort condition")
+    T = T.reshape(k, k).T.copy()   # column-major -> T[i, j]
+    return (T, f, v0, V) if want_v else (T, f, v0)
+
+
+def induced_norm_mat(T, N):
+    """ChebySCFLoop.cpp:7-34 (squared norm, ldA = T.shape[0])."""
+    Tf = np.asfortranarray(T, dtype=np.float64).reshape(-1, order="F")
+    return lib().orc_induced_norm_mat(np.ascontiguousarray(Tf), T.shape[0], N)
+
+
+# --------------------------------------------------------------------------
+# Dense FP64 steps (numpy -> LAPACK)
+# --------------------------------------------------------------------------
+
+def ortho_qr(X):
+    """ortho_qr.cpp:3-34: Householder QR (zgeqrf) and explicit Q (zungqr).
+    X: (nstates, nbasis) rows = columns of the reference's matrix."""
+    Q, _ = np.linalg.qr(np.asarray(X).T, mode="reduced")
+    return np.ascontiguousarray(Q.T)
+
+
+def symmetrize(T):
+    """lobpcg_cpu.cpp:15-27 -> (T + T^H)/2 on the stored matrix."""
+    return 0.5 * (T + T.conj().T)
+
+
+def eig_zheevd(T):
+    """eig_zheevd.cpp:5-34: jobz='V', uplo='U'."""
+    return np.linalg.eigh(T, UPLO="U")
+
+
+def rayleigh_ritz(X, HX):
+    """ChebySCFLoop.cpp:289-311 given orthonormal X and HX (rows = bands):
+    T = X^H HX, symmetrize, zheevd, X <- X T.  Returns (eigenvalues, rotated X)."""
+    Xm, HXm = X.T, HX.T
+    T = symmetrize(Xm.conj().T @ HXm)
+    w, Q = eig_zheevd(T)
+    return w, np.ascontiguousarray((Xm @ Q).T)
+
+
+def residual_norms(X, HX):
+    """ChebySCFLoop.cpp:415-433: R = HX - X (X^H HX)_sym; per-band 2-norms."""
+    Xm, HXm = X.T, HX.T
+    G = symmetrize(Xm.conj().T @ HXm)
+    R = HXm - Xm @ G
+    return np.sqrt(np.real(np.sum(R.conj() * R, axis=0)))
+
+
+class Reference:
+    """ctypes wrapper of oracle/_ref/libref.so (the reference's own compiled setup code)."""
+
+    def __init__(self, path=REF_PATH):
+        L = C.CDLL(path)
+        L.ref_setup.restype = C.c_int
+        L.ref_setup.argtypes = [_dp, C.c_double, _ip, C.c_int, _ip, C.c_int, _dp, C.c_int, _dp]
+        L.ref_sizes.argtypes = [_ip]
+        L.ref_scalars.argtypes = [_dp]
+        L.ref_copy.argtypes = [_ip, _ip, _dp, _up, _dp, _zp]
+        L.ref_sort1.argtypes = [C.c_int, _dp, _ip]
+        L.ref_ortho_qr.argtypes = [_zp, C.c_int, C.c_int]
+        L.ref_eig_zheevd.argtypes = [_zp, C.c_int, _dp, C.c_int]
+        self.L = L
+
+    @staticmethod
+    def available(path=REF_PATH):
+        return os.path.exists(path)
+
+    def setup(self, A, ecut, nr, na, nax, tau, xk=None):
+        xk = np.zeros((1, 3)) if xk is None else np.asarray(xk, np.float64).reshape(-1, 3)
+        nkpt = xk.shape[0]
+        na = np.asarray(na, np.int32)
+        self.L.ref_setup(lattice(A), float(ecut), np.asarray(nr, np.int32), len(na), na, nax,
+                         np.ascontiguousarray(np.asarray(tau, np.float64).reshape(-1)), nkpt,
+                         np.ascontiguousarray(xk.reshape(-1)))
+        sz = np.zeros(3 + nkpt, np.int32)
+        self.L.ref_sizes(sz)
+        ngx, ngwx, ng = int(sz[0]), int(sz[1]), int(sz[2])
+        in123 = np.zeros(3 * ngx, np.int32)
+        igk = np.zeros(ngwx * nkpt, np.int32)
+        xkg = np.zeros(ngwx * nkpt)
+        n123 = np.zeros(ngwx * nkpt, np.uint32)
+        ggk = np.zeros(3 * ngwx * nkpt)
+        eigr = np.zeros(ngwx * nax * len(na) * nkpt, np.complex128)
+        self.L.ref_copy(in123, igk, xkg, n123, ggk, eigr)
+        sc = np.zeros(15)
+        self.L.ref_scalars(sc)
+        return dict(ngx=ngx, ngwx=ngwx, ng=ng, ngw=sz[3:].copy(), in123=in123, igk=igk, xkg=xkg,
+                    n123=n123, ggk=ggk, eigr=eigr, scalars=sc, nkpt=nkpt)
+
+    def sort1(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.float64)
+        idx = np.zeros(len(arr), np.int32)
+        self.L.ref_sort1(len(arr), arr, idx)
+        return idx
+
+    def ortho_qr(self, X):
+        """X rows = bands -> column-major (nbasis x nstates) buffer is X itself."""
+        Xc = np.array(X, dtype=np.complex128, order="C")
+        self.L.ref_ortho_qr(Xc.reshape(-1), Xc.shape[1], Xc.shape[0])
+        return Xc
+
+    def eig_zheevd(self, T):
+        n = T.shape[0]
+        A = np.array(np.asarray(T, np.complex128).T, order="C", copy=True)   # column-major buffer
+        w = np.zeros(n)
+        self.L.ref_eig_zheevd(A.reshape(-1), n, w, n)
+        return w, A.T.copy()

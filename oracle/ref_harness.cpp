@@ -1,0 +1,114 @@
+// ref_harness.cpp -- drives the reference's own compiled setup routines
+// (GVectorsGen0, GVectorsGen, sort1, GkVectorsGen, PhaseFactor) and its LAPACK
+// wrappers (ortho_qr, eig_zheevd) so the oracle can be checked against them.
+//
+// TEST INFRASTRUCTURE ONLY (built into oracle/_ref/, never shipped).
+//
+// Setup() itself (Setup.cpp:5-145) also reads pseudopotential files and input
+// files, so this harness sets the reference's globals directly and repeats the
+// scalar lines of Setup.cpp:39-66 before calling the reference routines in the
+// order Setup.cpp:128,139 and PrepareSCF.cpp:12 call them.
+#include "common_pspw.h"
+
+extern "C" {
+
+static int g_ready = 0;
+
+static void free_all()
+{
+  free(NGW); NGW = NULL;
+  free(G); G = NULL; free(GG); GG = NULL;
+  free(N1); N1 = NULL; free(N2); N2 = NULL; free(N3); N3 = NULL;
+  free(IN1); IN1 = NULL; free(IN2); IN2 = NULL; free(IN3); IN3 = NULL;
+  free(IGK); IGK = NULL; free(XKG); XKG = NULL; free(N123); N123 = NULL; free(GGK); GGK = NULL;
+  free(EI1); EI1 = NULL; free(EI2); EI2 = NULL; free(EI3); EI3 = NULL; free(EIGR); EIGR = NULL;
+  free(NA); NA = NULL; free(TAU); TAU = NULL; free(XK); XK = NULL; free(WKPT); WKPT = NULL;
+  g_ready = 0;
+}
+
+// A: A1,A2,A3 (bohr), ecut (Ry), nr: FFT grid, tau: TAU(1:3,1:nax,1:nsp) bohr,
+// xk: XK(1:3,1:nkpt) in 2pi/a.
+int ref_setup(const double *A, double ecut, const int *nr, int nsp, const int *na, int nax,
+              const double *tau, int nkpt, const double *xk)
+{
+  if (g_ready) free_all();
+  for (int i = 0; i < 3; i++) { A1[i] = A[i]; A2[i] = A[3+i]; A3[i] = A[6+i]; }
+  ECUT = ecut;
+  NR1 = nr[0]; NR2 = nr[1]; NR3 = nr[2]; NNR = NR1*NR2*NR3; USER_FFT_GRID = true;
+  NSP = nsp; NAX = nax;
+  NA = (int*)malloc(nsp*sizeof(int));
+  for (int i = 0; i < nsp; i++) NA[i] = na[i];
+  TAU = (double*)malloc(3*nax*nsp*sizeof(double));
+  for (int i = 0; i < 3*nax*nsp; i++) TAU[i] = tau[i];
+  NKPT = nkpt;
+  XK = (double*)malloc(3*nkpt*sizeof(double));
+  WKPT = (double*)malloc(nkpt*sizeof(double));
+  for (int i = 0; i < 3*nkpt; i++) XK[i] = xk[i];
+  for (int i = 0; i < nkpt; i++) WKPT[i] = 1.0/nkpt;
+
+  // Setup.cpp:39-66
+  ALAT = A1[0];
+  OMEGA = determinant3x3(A1, A2, A3);
+  CalcRecipLatt();
+  NGW = (int*)malloc(NKPT*sizeof(int));
+  TPIBA = 2.*M_PI/ALAT;
+  TPIBA2 = TPIBA*TPIBA;
+  GCUT = 4.*ECUT/TPIBA/TPIBA;
+  GCUTW = GCUT/4.0;
+  GVectorsGen0();
+  NGWX = NGX/8 + 8;
+  NGX = NGWX*8 + 8;
+  NGWX = NGWX + 1;
+
+  // AllocateArrays.cpp:13-38 (the arrays these routines touch)
+  G = (double*)calloc(NGX, sizeof(double));
+  GG = (double*)calloc((NGX+1)*3, sizeof(double));
+  N1 = (int*)calloc(NGX, sizeof(int)); N2 = (int*)calloc(NGX, sizeof(int)); N3 = (int*)calloc(NGX, sizeof(int));
+  IN1 = (int*)calloc(NGX, sizeof(int)); IN2 = (int*)calloc(NGX, sizeof(int)); IN3 = (int*)calloc(NGX, sizeof(int));
+  IGK = (int*)calloc(NGWX*NKPT, sizeof(int));
+  XKG = (double*)calloc(NGWX*NKPT, sizeof(double));
+  N123 = (unsigned int*)calloc(NGWX*NKPT, sizeof(unsigned int));
+  GGK = (double*)calloc(3*NGWX*NKPT, sizeof(double));
+  EI1 = (double complex*)calloc((NR1+1)*NAX*NSP, sizeof(double complex));
+  EI2 = (double complex*)calloc((NR2+1)*NAX*NSP, sizeof(double complex));
+  EI3 = (double complex*)calloc((NR3+1)*NAX*NSP, sizeof(double complex));
+  EIGR = (double complex*)calloc(NGWX*NAX*NSP*NKPT, sizeof(double complex));
+
+  GVectorsGen();
+  GkVectorsGen();
+  PhaseFactor();
+  g_ready = 1;
+  return 0;
+}
+
+// out: NGX, NGWX, NG, NGW(1..nkpt)
+void ref_sizes(int *out)
+{
+  out[0] = NGX; out[1] = NGWX; out[2] = NG;
+  for (int ik = 0; ik < NKPT; ik++) out[3+ik] = NGW[ik];
+}
+
+void ref_scalars(double *out)
+{
+  out[0] = ALAT; out[1] = OMEGA; out[2] = TPIBA; out[3] = TPIBA2; out[4] = GCUT; out[5] = GCUTW;
+  for (int i = 0; i < 3; i++) { out[6+i] = B1[i]; out[9+i] = B2[i]; out[12+i] = B3[i]; }
+}
+
+// in123 [3*NGX], igk/xkg/n123 [NGWX*NKPT], ggk [3*NGWX*NKPT], eigr [NGWX*NAX*NSP*NKPT]
+void ref_copy(int *in123, int *igk, double *xkg, unsigned int *n123, double *ggk, double complex *eigr)
+{
+  for (int i = 0; i < NGX; i++) { in123[i] = IN1[i]; in123[NGX+i] = IN2[i]; in123[2*NGX+i] = IN3[i]; }
+  for (int i = 0; i < NGWX*NKPT; i++) { igk[i] = IGK[i]; xkg[i] = XKG[i]; n123[i] = N123[i]; }
+  for (int i = 0; i < 3*NGWX*NKPT; i++) ggk[i] = GGK[i];
+  for (int i = 0; i < NGWX*NAX*NSP*NKPT; i++) eigr[i] = EIGR[i];
+}
+
+void ref_sort1(int n, double *arr, int *index) { sort1(n, arr, index); }
+
+void ref_ortho_qr(double complex *X, int nbasis, int nstates) { ortho_qr(X, nbasis, nstates); }
+
+void ref_eig_zheevd(double complex *A, int ldA, double *lambda, int N) { eig_zheevd(A, ldA, lambda, N); }
+
+void ref_free() { free_all(); }
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+"""CPU checks of the boundary: libpwh.so loads and exports every entry point declared in
+include/pwh.h, the shim exports the reference's mangled C++ symbols, and the library's
+host setup (G maps, phases) equals the reference bit for bit (no GPU compute here)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import PKG, ROOT
+
+HEADER = os.path.join(ROOT, "include", "pwh.h")
+
+# Itanium-mangled names of the reference's hot-path routines (common_pspw.h:176-178,187,222)
+REF_SYMBOLS = ["_Z14ApplyHam_blockPCdS0_iiPdii", "_Z8ApplyHamPCdS0_iiPdi",
+               "_Z9chebyfiltiPCdiddPdi", "_Z8ortho_qrPCdii"]
+
+
+def declared_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(pwh_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    import pwh
+    L = pwh.lib()
+    names = declared_functions()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(L, n), n
+    assert sorted(pwh.EXPORTS) == names
+
+
+def test_shim_exports_reference_signatures():
+    out = subprocess.run(["nm", "-D", "--defined-only", os.path.join(PKG, "libpwh_shim.so")],
+                         capture_output=True, text=True, check=True).stdout
+    for sym in REF_SYMBOLS:
+        assert re.search(r"\sT\s" + re.escape(sym) + r"$", out, flags=re.M), sym
+
+
+def test_shim_loads_with_reference_globals():
+    ctypes.CDLL(os.path.join(PKG, "libpwh_refglobals.so"), mode=ctypes.RTLD_GLOBAL)
+    shim = ctypes.CDLL(os.path.join(PKG, "libpwh_shim.so"))
+    for sym in REF_SYMBOLS:
+        assert hasattr(shim, sym)
+
+
+@pytest.mark.parametrize("name", ["si8_8ry", "si8_15ry", "si8_mixed"])
+def test_host_setup_equals_reference_fixtures(name):
+    import pwh
+    z = np.load(os.path.join(ROOT, "tests", "golden", f"ref_maps_{name}.npz"))
+    A, nr = z["A"], tuple(int(x) for x in z["nr"])
+    gv = pwh.gvectors(A, float(z["ecut"]), nr)
+    n = int(z["ngw"])
+    assert gv["ngw"][0] == n
+    assert np.array_equal(gv["n123"][:n], z["n123"])
+    assert np.array_equal(gv["igk"][:n], z["igk"])
+    assert np.array_equal(gv["xkg"][:n], z["xkg"])
+    tau = z["tau"]
+    nat = tau.shape[0]
+    e = pwh.phase_factor(A, nr, [nat], nat, tau[None], gv).reshape(nat, gv["ngwx"])[:, :n]
+    assert np.array_equal(e, z["eigr"])
+
+
+@pytest.mark.parametrize("a,ecut,ncell,nr", [(10.266, 8.0, 1, (20, 20, 20)), (20.52, 20.0, 2, (64, 64, 64)),
+                                             (30.8, 20.0, 3, (96, 96, 96)), (10.0, 12.0, 1, (24, 30, 36)),
+                                             (9.0, 16.0, 1, (28, 28, 28))])
+def test_host_setup_equals_oracle(oracle, a, ecut, ncell, nr):
+    import pwh
+    from pwh import synth
+    A = np.diag([a, a, a])
+    tau = synth.diamond_fractional(ncell) * a
+    nat = len(tau)
+    g1 = pwh.gvectors(A, ecut, nr)
+    g2 = oracle.gvectors(A, ecut, nr)
+    for k in ("in123", "igk", "xkg", "n123", "ggk", "ngw"):
+        assert np.array_equal(g1[k], g2[k]), k
+    assert (g1["ng"], g1["ngx"], g1["ngwx"]) == (g2["ng"], g2["ngx"], g2["ngwx"])
+    e1 = pwh.phase_factor(A, nr, [nat], nat, tau[None], g1)
+    e2 = oracle.phase_factor(A, nr, [nat], nat, tau[None], g2)
+    assert np.array_equal(e1, e2)
+    assert pwh.fft_grid(A, ecut) == oracle.fft_grid(A, ecut)
+
+
+def test_config_sizes():
+    """Exact plane-wave counts of the benchmark cells."""
+    import pwh
+    for a, ecut, nr, ngw in [(10.26, 15.0, (32,) * 3, 1021), (20.52, 20.0, (64,) * 3, 13133),
+                             (30.8, 20.0, (96,) * 3, 43867)]:
+        assert pwh.gvectors(np.diag([a] * 3), ecut, nr)["ngw"][0] == ngw
+
+
+def test_grid_support_table():
+    import pwh
+    for n in (24, 30, 32, 36, 40, 48, 50, 54, 60, 64, 72, 80, 90, 96):
+        assert pwh.grid_supported((n, n, n))
+    assert not pwh.grid_supported((22, 22, 22))        # prime factor 11
+    assert not pwh.grid_supported((32, 30, 32))        # non-square (i1,i2) planes
+
+
+def test_no_device_is_a_clean_error():
+    import pwh
+    if pwh.device_count() > 0:
+        pytest.skip("a GPU is present")
+    with pytest.raises(pwh.PwhError) as e:
+        pwh.init(0)
+    assert e.value.code == pwh.PWH_ERR_HIP

@@ -1,0 +1,112 @@
+"""GPU parity of chebyfilt, lanczos, ortho_qr, Rayleigh-Ritz, the residual report and
+the ChFSI outer loop against the FP64 oracle (and a dense diagonalisation at Si8).
+
+Tolerances: FP64 filter 1e-11 relative; Lanczos T 1e-10 absolute (Ha); ortho_qr
+Q^H Q = I to 1e-13 and the same column space as LAPACK QR to 1e-12; FP64 ChFSI
+eigenvalues within 1e-8 Ha of the dense eigensolution; mixed-precision ChFSI
+eigenvalues within 1e-8 Ha of the FP64 ChFSI run.
+"""
+import numpy as np
+import pytest
+
+from conftest import oracle_globals, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def test_chebyfilt_fp64_matches_oracle(si8, gpu_si8, oracle):
+    X = si8.psi(16, seed=11)
+    ref = oracle.chebyfilt(oracle_globals(si8), X, 6, 0.3, 9.0, si8.vloc, si8.ngw)
+    out = gpu_si8.chebyfilt(X, 6, 0.3, 9.0, "fp64")
+    assert relerr(out, ref) < 1e-11
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16x3", 1e-4)])
+def test_chebyfilt_mixed_error(si8, gpu_si8, oracle, prec, tol):
+    X = si8.psi(16, seed=12)
+    ref = oracle.chebyfilt(oracle_globals(si8), X, 6, 0.3, 9.0, si8.vloc, si8.ngw)
+    out = gpu_si8.chebyfilt(X, 6, 0.3, 9.0, prec)
+    assert relerr(out, ref) < tol
+
+
+def test_chebyfilt_degree_one(si8, gpu_si8, oracle):
+    X = si8.psi(3, seed=13)
+    ref = oracle.chebyfilt(oracle_globals(si8), X, 1, -1.0, 5.0, si8.vloc, si8.ngw)
+    assert relerr(gpu_si8.chebyfilt(X, 1, -1.0, 5.0), ref) < 1e-12
+
+
+def test_lanczos_matches_oracle(si8, gpu_si8, oracle):
+    rng = np.random.default_rng(14)
+    v0 = rng.random(si8.ngw) + 1j * rng.random(si8.ngw)
+    Tr, fr, vr = oracle.lanczos(oracle_globals(si8), v0, 12, si8.vloc, si8.ngw)
+    T, f, v = gpu_si8.lanczos(v0, 12)
+    assert np.abs(T - Tr).max() < 1e-10
+    assert relerr(f, fr) < 1e-8
+    assert relerr(v, vr) < 1e-14
+
+
+def test_ortho_qr_column_space(si8):
+    import pwh
+    rng = np.random.default_rng(15)
+    X = rng.standard_normal((24, 500)) + 1j * rng.standard_normal((24, 500))
+    Q = pwh.ortho_qr(X)
+    G = Q.conj() @ Q.T
+    assert np.abs(G - np.eye(24)).max() < 1e-13
+    Qr, R = np.linalg.qr(X.T)
+    P1 = Q.T @ Q.conj()
+    P2 = Qr @ Qr.conj().T
+    assert np.abs(P1 - P2).max() < 1e-12
+    # R = Q^H X is upper triangular with a positive real diagonal (Cholesky convention)
+    Rq = Q.conj() @ X.T
+    assert np.abs(np.tril(Rq, -1)).max() < 1e-10 * np.abs(Rq).max()
+    assert np.all(np.real(np.diag(Rq)) > 0)
+
+
+def test_ortho_qr_rank_deficient_reports_error():
+    import pwh
+    X = np.ones((3, 50), np.complex128)
+    with pytest.raises(pwh.PwhError) as e:
+        pwh.ortho_qr(X)
+    assert e.value.code == pwh.PWH_ERR_LAPACK
+
+
+def test_rayleigh_ritz_and_residuals(si8, gpu_si8, oracle):
+    X = si8.psi(16, seed=16)
+    HX = oracle.apply_ham_block(oracle_globals(si8), X, si8.vloc, si8.ngw)
+    wr, _ = oracle.rayleigh_ritz(X, HX)
+    w, Xr = gpu_si8.rayleigh_ritz(X)
+    assert np.abs(w - wr).max() < 1e-11
+    res_ref = oracle.residual_norms(X, HX)
+    assert np.abs(gpu_si8.residual_norms(X) - res_ref).max() < 1e-11
+    # after RR the rotated block diagonalises H within the subspace
+    HXr = gpu_si8.apply_ham_block(Xr)
+    T = Xr.conj() @ HXr.T
+    assert np.abs(T - np.diag(w)).max() < 1e-11
+
+
+@pytest.fixture(scope="module")
+def si8_dense_eigs(si8, oracle):
+    """Exact spectrum: dense H from the oracle applied to unit vectors, numpy eigh."""
+    n = si8.ngw
+    H = oracle.apply_ham_block(oracle_globals(si8), np.eye(n, dtype=np.complex128), si8.vloc, n,
+                               nthreads=8)
+    H = H.T
+    return np.linalg.eigvalsh(0.5 * (H + H.conj().T))
+
+
+def test_chfsi_fp64_converges_to_dense_spectrum(si8, gpu_si8, si8_dense_eigs):
+    X0 = si8.psi(16, seed=17)
+    v0 = np.ones(si8.ngw, np.complex128)
+    eig, X, res, st = gpu_si8.chfsi(X0, degree=10, niter=30, precision="fp64", lanczos_k=20, v0=v0)
+    nx = 14
+    assert np.abs(eig[:nx] - si8_dense_eigs[:nx]).max() < 1e-8
+    assert res[:nx].max() < 1e-4
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_chfsi_mixed_matches_fp64(si8, gpu_si8, prec):
+    X0 = si8.psi(16, seed=18)
+    v0 = np.ones(si8.ngw, np.complex128)
+    e64, _, _, _ = gpu_si8.chfsi(X0, degree=10, niter=30, precision="fp64", lanczos_k=20, v0=v0)
+    e32, _, _, _ = gpu_si8.chfsi(X0, degree=10, niter=30, precision=prec, lanczos_k=20, v0=v0)
+    assert np.abs(e32[:14] - e64[:14]).max() < 1e-8
