@@ -1,0 +1,231 @@
+"""Benchmark: H|psi> band-applies/s on the Si64 cell (64^3 grid, 13133 plane waves,
+128 bands, 256 projectors), mixed precision vs FP64, plus ChFSI s/iter.
+
+A step is one H|psi> over the whole 128-band block, resident in HBM, in the mixed path
+(FP32 FFT + FP32 MFMA GEMMs).  The FP64 path and the BF16x3 path are timed the same
+way; ChFSI (degree 10, 10 outer iterations, 20-step FP64 Lanczos bounds, FP64
+CholeskyQR2 and Rayleigh-Ritz on the GPU) is timed once per filter precision.
+
+Prints ONE JSON line.  --gpus above 1 exits with a message: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "ffr-pspw-dft-c_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(s, info, nb, prec_bytes):
+    """Algorithmic HBM bytes per launch of each pass (compulsory traffic: inputs read
+    once, outputs written once).  cx = bytes per complex, r = bytes per real."""
+    cx = 2 * prec_bytes
+    r = prec_bytes
+    nr1, nr2, nr3 = s.nr
+    ngw = s.ngw
+    ncol = info["ncol"]
+    nnr = nr1 * nr2 * nr3
+    return {
+        # zinv: read the block's sphere coefficients, write the z-transformed columns
+        "zinv": nb * (ngw * cx + nr3 * ncol * cx),
+        # plane: read and write every column of every plane, read V once
+        "plane": nb * (2 * nr3 * ncol * cx) + nnr * r,
+        # zfwd: read the columns, read X and the nonlocal result, write H X
+        "zfwd": nb * (nr3 * ncol * cx + 3 * ngw * cx),
+    }
+
+
+def gemm_flops(s, nb):
+    nproj = 4 * s.natoms
+    return 8.0 * nproj * s.ngw * nb      # one complex GEMM: 8 flops per complex MAC
+
+
+def time_steps(fn, steps, warmup):
+    import pwh
+    for _ in range(warmup):
+        fn()
+    pwh.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    pwh.synchronize()
+    return time.perf_counter() - t0
+
+
+def kernel_profile(sysm, nb, prec, reps):
+    """Average per-launch durations from HIP events on the library stream."""
+    import pwh
+    pwh.synchronize()
+    pwh.timing_reset()
+    pwh.timing_enable(True)
+    for _ in range(reps):
+        sysm.resident_apply(nb, prec)
+    pwh.synchronize()
+    out = {}
+    for k in ("zinv", "plane", "zfwd", "proj", "expand", "hpsi"):
+        ms, n = pwh.timing_get(k)
+        out[k] = ms / max(n, 1)
+    pwh.timing_enable(False)
+    pwh.timing_reset()
+    return out
+
+
+def cpu_baseline(s, psi, seconds_target=15.0):
+    """The oracle's FP64 ApplyHam_block (CPU restatement) on a bounded band sample."""
+    from oracle import oracle as O
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    g = s.globals()
+    og = O.Globals(nr=s.nr, ngwx=g.ngwx, n123=s.gv["n123"], xkg=s.gv["xkg"], tpiba2=g.tpiba2,
+                   na=g.na, nax=g.nax, l_max=g.l_max, l_loc=g.l_loc, nlmax=g.nlmax, wnl=g.wnl,
+                   eigr=g.eigr, pkg_a=g.pkg_a)
+    t0 = time.perf_counter()
+    O.apply_ham_block(og, psi[:1], s.vloc, s.ngw)
+    t1 = time.perf_counter() - t0
+    nband = int(max(1, min(psi.shape[0], seconds_target / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    O.apply_ham_block(og, psi[:nband], s.vloc, s.ngw)
+    dt = time.perf_counter() - t0
+    return {"value": nband / dt, "unit": "band-applies/s", "cores": 1, "kind": "port",
+            "sample": f"{nband} bands of the Si64 block through the oracle's FP64 ApplyHam_block "
+                      f"(C restatement, own mixed-radix FFT, per-band loop as ApplyHam_block.cpp:28-167), "
+                      f"1 thread, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="si64")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-chfsi", action="store_true", help="skip the ChFSI timing")
+    ap.add_argument("--only", default="", help="fp64|fp32|bf16x3: time one path only (profiling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(multi-GPU sharding is described in DESIGN.md, not implemented).", file=sys.stderr)
+        sys.exit(2)
+
+    import pwh
+    from pwh import synth
+
+    s = synth.config(args.config)
+    nb = s.nbands
+    log(f"bench: {args.config} grid {s.nr} ngw {s.ngw} bands {nb} projectors {4 * s.natoms}")
+    pwh.init(0)
+    sysm = pwh.System(s.globals())
+    sysm.set_vloc(s.vloc)
+    psi = s.psi(nb, seed=0)
+    sysm.resident_set(psi)
+    info = sysm.info()
+
+    paths = ["fp64", "fp32", "bf16x3"] if not args.only else [args.only]
+    rates, ms_step = {}, {}
+    for p in paths:
+        dt = time_steps(lambda: sysm.resident_apply(nb, p), args.steps, args.warmup)
+        rates[p] = nb * args.steps / dt
+        ms_step[p] = 1e3 * dt / args.steps
+        log(f"bench: {p}: {ms_step[p]:.3f} ms per {nb}-band H|psi>, {rates[p]:.0f} band-applies/s")
+
+    main_path = "fp32" if "fp32" in rates else paths[0]
+    prof = kernel_profile(sysm, nb, main_path, reps=max(5, min(args.steps, 20)))
+    log("bench: kernel avg ms " + json.dumps({k: round(v, 4) for k, v in prof.items()}))
+    pbytes = 8 if main_path == "fp64" else 4
+    kb = kernel_bytes(s, info, nb, pbytes)
+    # the dominant kernel of the step
+    candidates = {k: prof[k] for k in ("zinv", "plane", "zfwd", "proj", "expand")}
+    dom = max(candidates, key=candidates.get)
+    if dom in kb:
+        achieved = kb[dom] / (prof[dom] * 1e-3) / 1e9
+        roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes": int(kb[dom]), "avg_ms": round(prof[dom], 5)}
+    else:
+        fl = gemm_flops(s, nb)
+        peak = FP64_PEAK_TFLOPS if main_path == "fp64" else FP32_PEAK_TFLOPS
+        achieved = fl / (prof[dom] * 1e-3) / 1e12
+        roof = {"kernel": dom, "bound": "mfma", "achieved": round(achieved, 2), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                "algorithmic_flops": fl, "avg_ms": round(prof[dom], 5)}
+    per_kernel = {}
+    for k, v in prof.items():
+        e = {"avg_ms": round(v, 5)}
+        if k in kb:
+            e["GB/s"] = round(kb[k] / (v * 1e-3) / 1e9, 1)
+        if k in ("proj", "expand"):
+            e["TFLOP/s"] = round(gemm_flops(s, nb) / (v * 1e-3) / 1e12, 2)
+        per_kernel[k] = e
+
+    chfsi = {}
+    if not args.no_chfsi and not args.only:
+        X0 = s.psi(nb, seed=1)
+        v0 = np.ones(s.ngw, np.complex128)
+        ref_eig = None
+        for p in ("fp64", "fp32", "bf16x3"):
+            eig, _, res, st = sysm.chfsi(X0, degree=10, niter=10, precision=p, lanczos_k=20, v0=v0)
+            nx = nb - 2
+            d = {"s_per_iter": round((st["t_total_s"] - st["t_lanczos_s"]) / 10, 5),
+                 "filter_s_per_iter": round(st["t_filter_s"] / 10, 5),
+                 "ortho_s_per_iter": round(st["t_ortho_s"] / 10, 5),
+                 "rr_s_per_iter": round(st["t_rr_s"] / 10, 5),
+                 "lanczos_s": round(st["t_lanczos_s"], 5),
+                 "max_residual": float(res[:nx].max())}
+            if p == "fp64":
+                ref_eig = eig
+            else:
+                d["max_abs_eig_err_vs_fp64_Ha"] = float(np.abs(eig[:nx] - ref_eig[:nx]).max())
+            chfsi[p] = d
+            log(f"bench: chfsi {p}: {json.dumps(d)}")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(s, psi)
+        log(f"bench: cpu baseline {cpu['value']:.2f} band-applies/s ({cpu['sample']})")
+
+    value = rates[main_path]
+    line = {
+        "metric": "H|psi> band-applies/s, mixed precision (FP32 FFT + FP32 MFMA), Si64 cell",
+        "value": round(value, 1),
+        "unit": "band-applies/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step[main_path], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if main_path != "fp64" else "fp64",
+        "data": "synthetic (seeded Si64 diamond cell, Gaussian V_loc and s+p projectors, random orthonormal psi)",
+        "config": {"workload": f"{args.config}: simple cubic a={s.a} bohr, {s.natoms} atoms, ecut {s.ecut} Ry, "
+                               f"grid {s.nr[0]}x{s.nr[1]}x{s.nr[2]}, {s.ngw} plane waves, {nb} bands, "
+                               f"{4 * s.natoms} projectors; step = one H|psi> over the {nb}-band block",
+                   "path": main_path, "global_batch": nb, "parallelism": "single GPU"},
+        "band_applies_per_s": {k: round(v, 1) for k, v in rates.items()},
+        "ms_per_block": {k: round(v, 4) for k, v in ms_step.items()},
+        "mixed_vs_fp64": round(rates.get("fp32", 0) / rates["fp64"], 3) if "fp64" in rates and "fp32" in rates else None,
+        "roofline": roof,
+        "kernels": per_kernel,
+        "chfsi": chfsi or None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    sysm.close()
+
+
+if __name__ == "__main__":
+    main()

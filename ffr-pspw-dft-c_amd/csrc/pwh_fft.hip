@@ -86,22 +86,35 @@ __device__ __forceinline__ void dft(cx<T>* v) {
   }
 }
 
+// Stage hooks: which input elements of the first stage are live (the others are known
+// zeros and are not read), which outputs of the last stage are stored, and a pointwise
+// operation applied to the last stage's outputs in registers.
+struct NoHooks {
+  __device__ __forceinline__ bool live_in(int, int) const { return true; }
+  __device__ __forceinline__ bool live_out(int, int) const { return true; }
+  template <typename V>
+  __device__ __forceinline__ void post(int, int, V&) const {}
+};
+
 // One Stockham stage over `count` transforms of length N held in LDS.  Element e of
 // transform t lives at buf[base(t) + e*es].  TFAST: consecutive threads take
 // consecutive transforms (columns), else consecutive butterflies (rows).
-template <typename T, int N, int R, int NS, int SIGN, int NT, int MAXCOUNT, bool TFAST, typename BaseF>
+template <typename T, int N, int R, int NS, int SIGN, int NT, int MAXCOUNT, bool TFAST,
+          typename BaseF, typename Hooks>
 __device__ __forceinline__ void lds_stage(cx<T>* buf, int count, int es, BaseF base,
-                                          const cx<T>* tw, int tid) {
+                                          const cx<T>* tw, int tid, const Hooks& hk) {
   constexpr int NB = N / R;
   constexpr int MAXB = (MAXCOUNT * NB + NT - 1) / NT;
+  constexpr bool FIRST = (NS == 1), LAST = (NS * R == N);
   const int total = count * NB;
   cx<T> v[MAXB][R];
-  int off[MAXB], jv[MAXB];
+  int off[MAXB], jv[MAXB], tv[MAXB];
 #pragma unroll
   for (int q = 0; q < MAXB; ++q) {
     const int idx = tid + q * NT;
     jv[q] = -1;
     off[q] = 0;
+    tv[q] = 0;
     if (idx < total) {
       int t, j;
       if constexpr (TFAST) {
@@ -114,8 +127,12 @@ __device__ __forceinline__ void lds_stage(cx<T>* buf, int count, int es, BaseF b
       const int b = base(t);
       off[q] = b;
       jv[q] = j;
+      tv[q] = t;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = buf[b + (j + r * NB) * es];
+      for (int r = 0; r < R; ++r) {
+        if (!FIRST || hk.live_in(t, j + r * NB)) v[q][r] = buf[b + (j + r * NB) * es];
+        else v[q][r] = mk<T>(0, 0);
+      }
     }
   }
   __syncthreads();
@@ -135,21 +152,52 @@ __device__ __forceinline__ void lds_stage(cx<T>* buf, int count, int es, BaseF b
       dft<T, R, SIGN>(v[q]);
       const int o = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[off[q] + (o + r * NS) * es] = v[q][r];
+      for (int r = 0; r < R; ++r) {
+        const int e = o + r * NS;
+        if constexpr (LAST) {
+          if (!hk.live_out(tv[q], e)) continue;
+          hk.post(tv[q], e, v[q][r]);
+        }
+        buf[off[q] + e * es] = v[q][r];
+      }
     }
   }
   __syncthreads();
 }
 
-template <typename T, int N, int SIGN, int NT, int MAXCOUNT, bool TFAST, int NS = 1, typename BaseF>
+template <typename T, int N, int SIGN, int NT, int MAXCOUNT, bool TFAST, int NS = 1, typename BaseF,
+          typename Hooks = NoHooks>
 __device__ __forceinline__ void lds_fft(cx<T>* buf, int count, int es, BaseF base, const cx<T>* tw,
-                                        int tid) {
+                                        int tid, const Hooks& hk = Hooks()) {
   if constexpr (NS < N) {
     constexpr int R = pick_radix(N / NS);
-    lds_stage<T, N, R, NS, SIGN, NT, MAXCOUNT, TFAST>(buf, count, es, base, tw, tid);
-    lds_fft<T, N, SIGN, NT, MAXCOUNT, TFAST, NS * R>(buf, count, es, base, tw, tid);
+    lds_stage<T, N, R, NS, SIGN, NT, MAXCOUNT, TFAST>(buf, count, es, base, tw, tid, hk);
+    lds_fft<T, N, SIGN, NT, MAXCOUNT, TFAST, NS * R>(buf, count, es, base, tw, tid, hk);
   }
 }
+
+// x-direction hooks of the plane pass: inactive i1 columns are zero on input of the
+// inverse transform; its outputs are multiplied by V; the forward transform stores only
+// the active columns.
+template <typename T>
+struct XInvHooks {
+  const unsigned char* act;  // LDS [N1]
+  const T* vrow;             // V of this plane, [N2][N1]
+  int n1;
+  __device__ __forceinline__ bool live_in(int, int e) const { return act[e] != 0; }
+  __device__ __forceinline__ bool live_out(int, int) const { return true; }
+  __device__ __forceinline__ void post(int t, int e, cx<T>& v) const {
+    const T s = vrow[t * n1 + e];
+    v = s * v;
+  }
+};
+struct XFwdHooks {
+  const unsigned char* act;
+  __device__ __forceinline__ bool live_in(int, int) const { return true; }
+  __device__ __forceinline__ bool live_out(int, int e) const { return act[e] != 0; }
+  template <typename V>
+  __device__ __forceinline__ void post(int, int, V&) const {}
+};
 
 constexpr bool smooth235(int n) {
   while (n % 2 == 0) n /= 2;
@@ -195,7 +243,7 @@ __global__ __launch_bounds__(kColThreads) void k_zinv(SphereMaps m, const cx<T>*
 // ---------------------------------------------------------------------------
 template <typename T, int N1, int N2>
 constexpr size_t plane_smem() {
-  return sizeof(cx<T>) * (size_t)(N1 * N2 + N1 + N2) + sizeof(int) * N1;
+  return sizeof(cx<T>) * (size_t)(N1 * N2 + N1 + N2) + sizeof(int) * N1 + N1;
 }
 
 template <typename T, int N1, int N2>
@@ -209,26 +257,28 @@ __global__ __launch_bounds__(kPlaneThreads) void k_plane(SphereMaps m, const cx<
   cx<T>* tw1 = tile + N1 * N2;
   cx<T>* tw2 = tw1 + N1;
   int* xa = reinterpret_cast<int*>(tw2 + N2);
+  unsigned char* xact = reinterpret_cast<unsigned char*>(xa + N1);
   const int tid = threadIdx.x;
   const int z = blockIdx.x, b = blockIdx.y;
   const int nxa = m.nxa;
-  for (int i = tid; i < N1; i += NT) tw1[i] = tw1g[i];
+  for (int i = tid; i < N1; i += NT) { tw1[i] = tw1g[i]; xact[i] = 0; }
   for (int i = tid; i < N2; i += NT) tw2[i] = tw2g[i];
-  for (int i = tid; i < nxa; i += NT) xa[i] = m.xa[i];
-  for (int i = tid; i < N1 * N2; i += NT) tile[i] = mk<T>(0, 0);
+  __syncthreads();
+  for (int i = tid; i < nxa; i += NT) { const int x = m.xa[i]; xa[i] = x; xact[x] = 1; }
+  // only the active columns need zeros: the x transforms never read the others
+  for (int i = tid; i < nxa * N2; i += NT) {
+    const int y = i / nxa, c = i - y * nxa;
+    tile[y * N1 + m.xa[c]] = mk<T>(0, 0);
+  }
   __syncthreads();
   cx<T>* wrow = W + ((size_t)b * m.nr3 + z) * m.ncol_pad;
   for (int c = tid; c < m.ncol; c += NT) tile[m.colxy[c]] = wrow[c];
   __syncthreads();
   lds_fft<T, N2, +1, NT, N1, true>(tile, nxa, N1, [&](int t) { return xa[t]; }, tw2, tid);
-  lds_fft<T, N1, +1, NT, N2, false>(tile, N2, 1, [](int t) { return t * N1; }, tw1, tid);
-  const T* vz = V + (size_t)z * N1 * N2;
-  for (int i = tid; i < N1 * N2; i += NT) {
-    const T v = vz[i];
-    tile[i] = v * tile[i];
-  }
-  __syncthreads();
-  lds_fft<T, N1, -1, NT, N2, false>(tile, N2, 1, [](int t) { return t * N1; }, tw1, tid);
+  const XInvHooks<T> hinv{xact, V + (size_t)z * N1 * N2, N1};
+  lds_fft<T, N1, +1, NT, N2, false>(tile, N2, 1, [](int t) { return t * N1; }, tw1, tid, hinv);
+  const XFwdHooks hfwd{xact};
+  lds_fft<T, N1, -1, NT, N2, false>(tile, N2, 1, [](int t) { return t * N1; }, tw1, tid, hfwd);
   lds_fft<T, N2, -1, NT, N1, true>(tile, nxa, N1, [&](int t) { return xa[t]; }, tw2, tid);
   for (int c = tid; c < m.ncol; c += NT) wrow[c] = tile[m.colxy[c]];
 }

@@ -47,6 +47,14 @@ __device__ __forceinline__ f4v mfma16bf(s4v a, s4v b, f4v c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// Row of the 16x16 result held in accumulator register r of a lane in row-group lq
+// (lq = lane / 16; the column is lane % 16).  The f32 and bf16 instructions give each
+// lane four consecutive rows; the f64 instruction interleaves them (register r holds
+// row 4r + lq).
+template <typename T> __device__ __forceinline__ int acc_row(int lq, int r);
+template <> __device__ __forceinline__ int acc_row<float>(int lq, int r) { return 4 * lq + r; }
+template <> __device__ __forceinline__ int acc_row<double>(int lq, int r) { return 4 * r + lq; }
+
 // round-to-nearest-even float -> bf16 bits
 __device__ __forceinline__ unsigned short f2bf(float f) {
   unsigned u = __float_as_uint(f);
@@ -113,20 +121,34 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
 #pragma unroll
     for (int b = 0; b < 2; ++b) { cr[a][b] = acc_t{0, 0, 0, 0}; ci[a][b] = acc_t{0, 0, 0, 0}; }
 
-  for (int k0 = ks; k0 < ke; k0 += BKS) {
+  // register prefetch of the next K tile overlaps its global loads with the MFMAs
+  constexpr int PER = BT * BKS / GT;
+  cx<T> pa[PER], pb[PER];
+  auto fetch = [&](int k0) {
 #pragma unroll
-    for (int e = tid; e < BT * BKS; e += GT) {
+    for (int q = 0; q < PER; ++q) {
+      const int e = tid + q * GT;
       const int kk = e % BKS, rr = e / BKS;
       const int k = k0 + kk;
-      cx<T> va = mk<T>(0, 0), vb = mk<T>(0, 0);
+      pa[q] = mk<T>(0, 0);
+      pb[q] = mk<T>(0, 0);
       if (k < ke) {
-        if (m0 + rr < M) va = ld(m0 + rr, k);
-        if (n0 + rr < N) vb = B[(size_t)(n0 + rr) * ldb + k];
+        if (m0 + rr < M) pa[q] = ld(m0 + rr, k);
+        if (n0 + rr < N) pb[q] = B[(size_t)(n0 + rr) * ldb + k];
       }
-      sAr[kk][rr] = va.x; sAi[kk][rr] = va.y;
-      sBr[kk][rr] = vb.x; sBi[kk][rr] = vb.y;
+    }
+  };
+  fetch(ks);
+  for (int k0 = ks; k0 < ke; k0 += BKS) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = tid + q * GT;
+      const int kk = e % BKS, rr = e / BKS;
+      sAr[kk][rr] = pa[q].x; sAi[kk][rr] = pa[q].y;
+      sBr[kk][rr] = pb[q].x; sBi[kk][rr] = pb[q].y;
     }
     __syncthreads();
+    if (k0 + BKS < ke) fetch(k0 + BKS);
     if constexpr (!BF16X3) {
 #pragma unroll
       for (int kk = 0; kk < BKS; kk += 4) {
@@ -179,7 +201,7 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm + a * 16 + 4 * lq + r, n = n0 + wn + b * 16 + li;
+        const int m = m0 + wm + a * 16 + acc_row<T>(lq, r), n = n0 + wn + b * 16 + li;
         if (m < M && n < N) P[(size_t)m * N + n] = mk<T>(cr[a][b][r], ci[a][b][r]);
       }
 }
@@ -222,20 +244,33 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
 #pragma unroll
     for (int b = 0; b < 2; ++b) { cr[a][b] = acc_t{0, 0, 0, 0}; ci[a][b] = acc_t{0, 0, 0, 0}; }
 
-  for (int j0 = 0; j0 < J; j0 += BKS) {
+  constexpr int PER = BT * BKS / GT;
+  cx<T> pa[PER], pq[PER];
+  auto fetch = [&](int j0) {
 #pragma unroll
-    for (int e = tid; e < BT * BKS; e += GT) {
+    for (int q = 0; q < PER; ++q) {
+      const int e = tid + q * GT;
       const int kk = e % BT, jj = e / BT;
       const int j = j0 + jj;
-      cx<T> va = mk<T>(0, 0), vq = mk<T>(0, 0);
+      pa[q] = mk<T>(0, 0);
+      pq[q] = mk<T>(0, 0);
       if (j < J) {
-        if (k0 + kk < K) va = ld(j, k0 + kk);
-        if (n0 + kk < N) vq = QCM ? Q[j + (size_t)(n0 + kk) * ldq] : Q[(size_t)j * ldq + n0 + kk];
+        if (k0 + kk < K) pa[q] = ld(j, k0 + kk);
+        if (n0 + kk < N) pq[q] = QCM ? Q[j + (size_t)(n0 + kk) * ldq] : Q[(size_t)j * ldq + n0 + kk];
       }
-      sAr[jj][kk] = va.x; sAi[jj][kk] = va.y;
-      sQr[jj][kk] = vq.x; sQi[jj][kk] = vq.y;
+    }
+  };
+  fetch(0);
+  for (int j0 = 0; j0 < J; j0 += BKS) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = tid + q * GT;
+      const int kk = e % BT, jj = e / BT;
+      sAr[jj][kk] = pa[q].x; sAi[jj][kk] = pa[q].y;
+      sQr[jj][kk] = pq[q].x; sQi[jj][kk] = pq[q].y;
     }
     __syncthreads();
+    if (j0 + BKS < J) fetch(j0 + BKS);
     if constexpr (!BF16X3) {
 #pragma unroll
       for (int jj = 0; jj < BKS; jj += 4) {
@@ -287,7 +322,7 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int k = k0 + wm + a * 16 + 4 * lq + r, n = n0 + wn + b * 16 + li;
+        const int k = k0 + wm + a * 16 + acc_row<T>(lq, r), n = n0 + wn + b * 16 + li;
         if (k < K && n < N) {
           cx<T> v = mk<T>(cr[a][b][r], ci[a][b][r]);
           cx<T>* o = out + (size_t)n * ldo + k;
@@ -427,11 +462,11 @@ __global__ void k_mfma_probe(int kind, const double* A, const double* B, double*
   if (kind == 0) {
     d4v c = {0, 0, 0, 0};
     c = mfma4(A[li * 4 + lq], B[lq * 16 + li], c);
-    for (int r = 0; r < 4; ++r) D[(4 * lq + r) * 16 + li] = c[r];
+    for (int r = 0; r < 4; ++r) D[acc_row<double>(lq, r) * 16 + li] = c[r];
   } else if (kind == 1) {
     f4v c = {0, 0, 0, 0};
     c = mfma4((float)A[li * 4 + lq], (float)B[lq * 16 + li], c);
-    for (int r = 0; r < 4; ++r) D[(4 * lq + r) * 16 + li] = c[r];
+    for (int r = 0; r < 4; ++r) D[acc_row<float>(lq, r) * 16 + li] = c[r];
   } else {
     f4v c = {0, 0, 0, 0};
     s4v a, b;
@@ -440,7 +475,7 @@ __global__ void k_mfma_probe(int kind, const double* A, const double* B, double*
       b[v] = (short)f2bf((float)B[(4 * lq + v) * 16 + li]);
     }
     c = mfma16bf(a, b, c);
-    for (int r = 0; r < 4; ++r) D[(4 * lq + r) * 16 + li] = c[r];
+    for (int r = 0; r < 4; ++r) D[acc_row<float>(lq, r) * 16 + li] = c[r];
   }
 }
 

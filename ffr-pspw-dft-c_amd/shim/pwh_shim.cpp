@@ -45,6 +45,7 @@ ShimState g_shim = {0, Key()};
 
 void die(const char *where) {
   printf("%s: %s\n", where, pwh_last_error());
+  fflush(stdout);
   abort();
 }
 
@@ -89,6 +90,7 @@ pwh_system *system_for(int ik) {
 void check_nnr(int nnr) {
   if (nnr != NNR) {
     printf("nnr != NNR : %d %d\n", nnr, NNR);
+    fflush(stdout);
     abort();
   }
 }

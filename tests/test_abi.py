@@ -108,3 +108,21 @@ def test_no_device_is_a_clean_error():
     with pytest.raises(pwh.PwhError) as e:
         pwh.init(0)
     assert e.value.code == pwh.PWH_ERR_HIP
+
+
+def test_shim_aborts_on_nnr_mismatch_like_the_reference():
+    """ApplyHam_block.cpp:18-21: the shim prints the same message and aborts before any
+    device work (run in a child process)."""
+    import subprocess
+    import sys
+    code = (
+        "import ctypes as C, os, numpy as np\n"
+        f"G = C.CDLL(os.path.join({PKG!r}, 'libpwh_refglobals.so'), mode=C.RTLD_GLOBAL)\n"
+        f"S = C.CDLL(os.path.join({PKG!r}, 'libpwh_shim.so'))\n"
+        "C.c_int.in_dll(G, 'NNR').value = 1000\n"
+        "f = getattr(S, '_Z14ApplyHam_blockPCdS0_iiPdii')\n"
+        "f.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int]\n"
+        "f(None, None, 1, 1, None, 999, 1)\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "nnr != NNR : 999 1000" in r.stdout
