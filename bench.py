@@ -106,6 +106,79 @@ def cpu_baseline(s, psi, seconds_target=15.0):
                       f"1 thread, {dt:.1f} s"}
 
 
+def measured_traffic(kernel, path):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by the plain-copy calibration)."""
+    f = os.path.join(ROOT, "profiles", "pmc_traffic_si64.md")
+    if not os.path.exists(f):
+        return None
+    for line in open(f):
+        parts = line.strip().strip("|").split("|")
+        parts = [p.strip() for p in parts]
+        if len(parts) >= 6 and parts[0] == kernel and parts[1] == path:
+            try:
+                return int(float(parts[5]))
+            except ValueError:
+                return None
+    return None
+
+
+def c3_batch(steps, degree=10):
+    """24 uneven cells filtered together (one captured graph) vs one at a time."""
+    import pwh
+    from pwh import synth
+    cells = synth.uneven_batch(count=24, seed=2026)
+    systems = []
+    for c in cells:
+        x = pwh.System(c.globals())
+        x.set_vloc(c.vloc)
+        systems.append(x)
+    B = pwh.Batch(systems, [c.nbands for c in cells])
+    for i, c in enumerate(cells):
+        B.set(i, c.psi(c.nbands, seed=i))
+    lb = np.zeros(len(cells))
+    ub = np.full(len(cells), 60.0)
+    nbt = sum(c.nbands for c in cells)
+    out = {"workload": f"{len(cells)} cells, grids {sorted({c.nr[0] for c in cells})}, "
+                       f"NGW {min(c.ngw for c in cells)}-{max(c.ngw for c in cells)}, "
+                       f"bands {min(c.nbands for c in cells)}-{max(c.nbands for c in cells)} "
+                       f"(total {nbt}), FP32 filter degree {degree}"}
+    for mode, name in ((0, "one_at_a_time"), (1, "batched_graph")):
+        B.chebyfilt(degree, lb, ub, "fp32", mode)
+        pwh.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            B.chebyfilt(degree, lb, ub, "fp32", mode)
+        pwh.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        out[name] = {"ms_per_filter": round(dt * 1e3, 3), "band_applies_per_s": round(degree * nbt / dt, 1)}
+    out["speedup"] = round(out["one_at_a_time"]["ms_per_filter"] / out["batched_graph"]["ms_per_filter"], 3)
+    B.close()
+    for x in systems:
+        x.close()
+    return out
+
+
+def c4_rates(steps):
+    """Si216 cell (96^3 grid, 43867 plane waves, 432 bands, 864 projectors)."""
+    import pwh
+    from pwh import synth
+    s = synth.config("si216")
+    x = pwh.System(s.globals())
+    x.set_vloc(s.vloc)
+    rng = np.random.default_rng(5)
+    psi = rng.standard_normal((s.nbands, s.ngw)) + 1j * rng.standard_normal((s.nbands, s.ngw))
+    psi /= np.linalg.norm(psi, axis=1, keepdims=True)
+    x.resident_set(psi)
+    out = {"workload": f"si216: a={s.a} bohr, {s.natoms} atoms, grid {s.nr[0]}^3, {s.ngw} PW, "
+                       f"{s.nbands} bands, {4 * s.natoms} projectors"}
+    for p in ("fp64", "fp32", "bf16x3"):
+        dt = time_steps(lambda: x.resident_apply(s.nbands, p), steps, 1)
+        out[p] = {"ms_per_block": round(1e3 * dt / steps, 3), "band_applies_per_s": round(s.nbands * steps / dt, 1)}
+    x.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,7 +188,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-chfsi", action="store_true", help="skip the ChFSI timing")
     ap.add_argument("--only", default="", help="fp64|fp32|bf16x3: time one path only (profiling)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the C3 batch and C4 sections")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="run only a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calib:
+        import torch
+        a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes_per_launch_read": 1 << 30, "write": 1 << 30}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(multi-GPU sharding is described in DESIGN.md, not implemented).", file=sys.stderr)
@@ -153,7 +238,8 @@ def main():
     if dom in kb:
         achieved = kb[dom] / (prof[dom] * 1e-3) / 1e9
         roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(dom, main_path),
                 "algorithmic_bytes": int(kb[dom]), "avg_ms": round(prof[dom], 5)}
     else:
         fl = gemm_flops(s, nb)
@@ -176,6 +262,21 @@ def main():
         X0 = s.psi(nb, seed=1)
         v0 = np.ones(s.ngw, np.complex128)
         ref_eig = None
+        # one untimed iteration per path loads the solver kernels (first-call cost)
+        for p in ("fp64", "fp32", "bf16x3"):
+            sysm.chfsi(X0, degree=2, niter=1, precision=p, lanczos_k=4, v0=v0)
+        # the 128x128 FP64 Rayleigh-Ritz eigensolve: rocSOLVER D&C, Jacobi, Jacobi-D&C
+        rr_ms = {}
+        for solver in ("evd", "evj", "evdj"):
+            os.environ["PWH_EIGSOLVER"] = solver
+            sysm.chfsi(X0, degree=2, niter=1, precision="fp32", lanczos_k=4, v0=v0)
+            _, _, _, st = sysm.chfsi(X0, degree=2, niter=3, precision="fp32", lanczos_k=4, v0=v0)
+            rr_ms[solver] = round(1e3 * st["t_rr_s"] / 3, 3)
+        best = min(rr_ms, key=rr_ms.get)
+        os.environ["PWH_EIGSOLVER"] = best
+        chfsi["rr_ms_per_iter_by_eigensolver"] = rr_ms
+        chfsi["eigensolver"] = best
+        log(f"bench: RR ms/iter by eigensolver {rr_ms}, using {best}")
         for p in ("fp64", "fp32", "bf16x3"):
             eig, _, res, st = sysm.chfsi(X0, degree=10, niter=10, precision=p, lanczos_k=20, v0=v0)
             nx = nb - 2
@@ -191,6 +292,13 @@ def main():
                 d["max_abs_eig_err_vs_fp64_Ha"] = float(np.abs(eig[:nx] - ref_eig[:nx]).max())
             chfsi[p] = d
             log(f"bench: chfsi {p}: {json.dumps(d)}")
+
+    extra = {}
+    if not args.no_extra and not args.only:
+        extra["c3_batched_uneven"] = c3_batch(steps=5)
+        log("bench: c3 " + json.dumps(extra["c3_batched_uneven"]))
+        extra["c4_si216"] = c4_rates(steps=5)
+        log("bench: c4 " + json.dumps(extra["c4_si216"]))
 
     cpu = None
     if not args.no_cpu:
@@ -221,6 +329,8 @@ def main():
         "roofline": roof,
         "kernels": per_kernel,
         "chfsi": chfsi or None,
+        "c3_batched_uneven": extra.get("c3_batched_uneven"),
+        "c4_si216": extra.get("c4_si216"),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -83,17 +84,18 @@ hipEvent_t take_event() {
 // Records HIP events around a launch on the library stream when timing is enabled.
 struct Timed {
   int id;
+  hipStream_t st;
   hipEvent_t a = nullptr;
-  explicit Timed(int i) : id(i) {
+  Timed(int i, hipStream_t s) : id(i), st(s) {
     if (g.timing) {
       a = take_event();
-      hipEventRecord(a, g.stream);
+      hipEventRecord(a, st);
     }
   }
   ~Timed() {
     if (g.timing && a) {
       hipEvent_t b = take_event();
-      hipEventRecord(b, g.stream);
+      hipEventRecord(b, st);
       g.pending.push_back({id, a, b});
     }
   }
@@ -221,12 +223,28 @@ template <> struct Sel<float> {
 };
 
 // out = alpha * H X + beta * X + gamma * Z  on a block of nb bands in sphere order.
+// Ensures every work buffer hpsi<T> needs for nb bands (so a later call allocates nothing).
+template <typename T>
+int hpsi_prepare(pwh_system* s, int nb) {
+  const int ld = s->ngw_pad;
+  CHK(Sel<T>::W(s).ensure((size_t)nb * s->nr[2] * s->maps.ncol_pad));
+  for (int sp = 0; sp < s->nsp; ++sp) {
+    if (s->na[sp] == 0 || s->lm[sp] == 0) continue;
+    const int M = s->na[sp] * s->lm[sp];
+    CHK(Sel<T>::Hnl(s).ensure((size_t)nb * ld));
+    CHK(Sel<T>::proj(s).ensure((size_t)M * nb));
+    CHK(Sel<T>::Q(s).ensure((size_t)M * nb));
+    CHK(Sel<T>::part(s).ensure(proj_part_elems(M, nb, s->ngw)));
+  }
+  return PWH_OK;
+}
+
 template <typename T>
 int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double beta,
-         double gamma, const cx<T>* Z, int mode) {
+         double gamma, const cx<T>* Z, int mode, hipStream_t st = nullptr) {
   if (!s->vset) return fail(PWH_ERR_ARG, "local potential not set (pwh_set_vloc)");
-  Timed th(TM_HPSI);
-  hipStream_t st = g.stream;
+  if (!st) st = g.stream;
+  Timed th(TM_HPSI, st);
   const int ld = s->ngw_pad;
   CHK(Sel<T>::W(s).ensure((size_t)nb * s->nr[2] * s->maps.ncol_pad));
   bool have_nl = false;
@@ -240,28 +258,28 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
     CHK(Sel<T>::part(s).ensure(pe));
     BetaGen bg{Sel<T>::E(s), Sel<T>::P(s) + (size_t)sp * 4 * ld, s->aoff[sp], s->na[sp], s->lm[sp], ld};
     {
-      Timed t(TM_PROJ);
+      Timed t(TM_PROJ, st);
       HIPCHK(gemm_proj<T>(bg, X, ld, nb, s->ngw, Sel<T>::proj(s).p, Sel<T>::part(s).p, pe, mode, st));
     }
     HIPCHK(scale_proj<T>(Sel<T>::proj(s).p, Sel<T>::wl(s) + 4 * sp, s->na[sp], s->lm[sp], nb,
                          Sel<T>::Q(s).p, st));
     {
-      Timed t(TM_EXPAND);
+      Timed t(TM_EXPAND, st);
       HIPCHK(gemm_expand<T>(bg, Sel<T>::Q(s).p, nb, s->ngw, Sel<T>::Hnl(s).p, ld, have_nl, mode, st));
     }
     have_nl = true;
   }
   {
-    Timed t(TM_ZINV);
+    Timed t(TM_ZINV, st);
     HIPCHK(launch_zinv<T>(s->maps, Sel<T>::tw(s, 2), X, ld, Sel<T>::W(s).p, nb, st));
   }
   {
-    Timed t(TM_PLANE);
+    Timed t(TM_PLANE, st);
     HIPCHK(launch_plane<T>(s->maps, Sel<T>::tw(s, 0), Sel<T>::tw(s, 1), Sel<T>::V(s), Sel<T>::W(s).p,
                            nb, st));
   }
   {
-    Timed t(TM_ZFWD);
+    Timed t(TM_ZFWD, st);
     HIPCHK(launch_zfwd_epilogue<T>(s->maps, Sel<T>::tw(s, 2), Sel<T>::W(s).p, Sel<T>::kin(s), X,
                                    have_nl ? Sel<T>::Hnl(s).p : nullptr, Z, out, ld, nb,
                                    (T)(1.0 / s->nnr), (T)alpha, (T)beta, (T)gamma, st));
@@ -272,7 +290,7 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
 // chebyfilt.cpp:6-74 on device buffers b0 (= X in), b1, b2 (work); *result -> filtered block
 template <typename T>
 int chebyfilt_dev(pwh_system* s, cx<T>* b0, cx<T>* b1, cx<T>* b2, int nb, int degree, double lb,
-                  double ub, int mode, cx<T>** result) {
+                  double ub, int mode, cx<T>** result, hipStream_t st = nullptr) {
   const double e = (ub - lb) / 2.0;
   const double c = (ub + lb) / 2.0;
   double sigma = e / (lb - ub);
@@ -280,10 +298,10 @@ int chebyfilt_dev(pwh_system* s, cx<T>* b0, cx<T>* b1, cx<T>* b2, int nb, int de
   cx<T>* X = b0;
   cx<T>* Y = b1;
   cx<T>* Y1 = b2;
-  CHK(hpsi<T>(s, X, Y, nb, sigma1 / e, -c * sigma1 / e, 0.0, nullptr, mode));
+  CHK(hpsi<T>(s, X, Y, nb, sigma1 / e, -c * sigma1 / e, 0.0, nullptr, mode, st));
   for (int id = 2; id <= degree; ++id) {
     const double sigma2 = 1.0 / (2.0 / sigma1 - sigma);
-    CHK(hpsi<T>(s, Y, Y1, nb, 2.0 * sigma2 / e, -c * 2.0 * sigma2 / e, -sigma * sigma2, X, mode));
+    CHK(hpsi<T>(s, Y, Y1, nb, 2.0 * sigma2 / e, -c * 2.0 * sigma2 / e, -sigma * sigma2, X, mode, st));
     cx<T>* t = X;
     X = Y;
     Y = Y1;
@@ -353,6 +371,41 @@ int filter_prec(pwh_system* s, cx<double>* X, int nb, int degree, double lb, dou
   return PWH_OK;
 }
 
+// Ensures the buffers of filter_on_stream for nb bands in the given precision.
+int filter_prepare(pwh_system* s, int nb, int precision) {
+  const size_t n = (size_t)nb * s->ngw_pad;
+  CHK(s->X.ensure(n));
+  if (precision == PWH_FP64) {
+    CHK(s->A64.ensure(n));
+    CHK(s->B64.ensure(n));
+    return hpsi_prepare<double>(s, nb);
+  }
+  CHK(s->X32.ensure(n));
+  CHK(s->Y32.ensure(n));
+  CHK(s->Y1_32.ensure(n));
+  return hpsi_prepare<float>(s, nb);
+}
+
+// Chebyshev filter of the resident FP64 block s->X on stream st; buffers must be
+// prepared (filter_prepare), so this issues kernels and device copies only and can be
+// captured into a graph.
+int filter_on_stream(pwh_system* s, int nb, int degree, double lb, double ub, int precision,
+                     hipStream_t st) {
+  const size_t n = (size_t)nb * s->ngw_pad;
+  if (precision == PWH_FP64) {
+    cx<double>* res = nullptr;
+    CHK(chebyfilt_dev<double>(s, s->X.p, s->A64.p, s->B64.p, nb, degree, lb, ub, PREC_FP64, &res, st));
+    if (res != s->X.p)
+      HIPCHK(hipMemcpyAsync(s->X.p, res, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, st));
+    return PWH_OK;
+  }
+  HIPCHK((convert_cx<double, float>(s->X.p, s->X32.p, n, st)));
+  cx<float>* res = nullptr;
+  CHK(chebyfilt_dev<float>(s, s->X32.p, s->Y32.p, s->Y1_32.p, nb, degree, lb, ub, precision, &res, st));
+  HIPCHK((convert_cx<float, double>(res, s->X.p, n, st)));
+  return PWH_OK;
+}
+
 // CholeskyQR2 (FP64) of the nb columns of X [nb][ld] (nrows valid rows), using tmp [nb][ld].
 int cholqr2(cx<double>* X, int ld, int nrows, int nb, cx<double>* tmp, cx<double>* S,
             cx<double>* part, size_t part_elems, rocblas_int* dinfo) {
@@ -386,6 +439,24 @@ int sys_cholqr2(pwh_system* s, cx<double>* X, int nb) {
   return cholqr2(X, s->ngw_pad, s->ngw, nb, s->A64.p, s->S.p, s->part64.p, pe, s->info.p);
 }
 
+// FP64 Hermitian eigensolve on the GPU, eig_zheevd.cpp:5-34 ('V', 'U'): A (n x n,
+// column-major) is replaced by its eigenvectors, w gets the eigenvalues ascending.
+// PWH_EIGSOLVER selects rocSOLVER's divide and conquer (evd, default), Jacobi (evj) or
+// Jacobi with divide and conquer (evdj).  work: n doubles + 2 ints of device scratch.
+int hermitian_eig(cx<double>* A, int n, double* w, double* work, rocblas_int* info) {
+  const char* e = getenv("PWH_EIGSOLVER");
+  auto* a = reinterpret_cast<rocblas_double_complex*>(A);
+  if (e && !strcmp(e, "evj")) {
+    RBCHK(rocsolver_zheevj(g.rb, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, n,
+                           a, n, 0.0, work, 100, info + 1, w, info));
+  } else if (e && !strcmp(e, "evdj")) {
+    RBCHK(rocsolver_zheevdj(g.rb, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, info));
+  } else {
+    RBCHK(rocsolver_zheevd(g.rb, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, work, info));
+  }
+  return PWH_OK;
+}
+
 // Rayleigh-Ritz (ChebySCFLoop.cpp:292-306) on the orthonormal block X: X <- X Q, HX <- (H X) Q,
 // evals (device) ascending.
 int rayleigh_ritz_dev(pwh_system* s, cx<double>* X, int nb) {
@@ -396,15 +467,13 @@ int rayleigh_ritz_dev(pwh_system* s, cx<double>* X, int nb) {
   CHK(s->S.ensure((size_t)nb * nb));
   CHK(s->evals.ensure(nb));
   CHK(s->rwork.ensure(nb));
-  CHK(s->info.ensure(1));
+  CHK(s->info.ensure(2));
   const size_t pe = proj_part_elems(nb, nb, s->ngw);
   CHK(s->part64.ensure(pe));
   CHK(hpsi<double>(s, X, s->HX.p, nb, 1.0, 0.0, 0.0, nullptr, PREC_FP64));
   HIPCHK(gemm_cn_f64(X, ld, nb, s->HX.p, ld, nb, s->ngw, s->S.p, nb, s->part64.p, pe, g.stream));
   HIPCHK(symmetrize(s->S.p, nb, nb, g.stream));
-  RBCHK(rocsolver_zheevd(g.rb, rocblas_evect_original, rocblas_fill_upper, nb,
-                         reinterpret_cast<rocblas_double_complex*>(s->S.p), nb, s->evals.p,
-                         s->rwork.p, s->info.p));
+  CHK(hermitian_eig(s->S.p, nb, s->evals.p, s->rwork.p, s->info.p));
   rocblas_int hinfo = 0;
   HIPCHK(hipMemcpyAsync(&hinfo, s->info.p, sizeof(hinfo), hipMemcpyDeviceToHost, g.stream));
   HIPCHK(hipStreamSynchronize(g.stream));
@@ -951,6 +1020,125 @@ int pwh_resident_chebyfilt(pwh_system* s, int nstates, int degree, double lb, do
   if (nstates > s->resident_nb) return fail(PWH_ERR_ARG, "more states than resident");
   if (!(ub > lb)) return fail(PWH_ERR_ARG, "chebyfilt needs ub > lb");
   return filter_prec(s, s->X.p, nstates, degree, lb, ub, precision);
+}
+
+}  // extern "C"
+
+struct pwh_batch {
+  std::vector<pwh_system*> sys;
+  std::vector<int> nb;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> done;
+  hipEvent_t fork = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int g_degree = -1, g_prec = -1;
+  std::vector<double> g_lb, g_ub;
+};
+
+namespace {
+void batch_drop_graph(pwh_batch* b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->g_degree = -1;
+}
+}  // namespace
+
+extern "C" {
+
+int pwh_batch_create(pwh_system* const* systems, const int* nstates, int nsys, pwh_batch** out) {
+  if (!systems || !nstates || nsys < 1 || !out) return fail(PWH_ERR_ARG, "bad batch arguments");
+  CHK(ensure_init());
+  auto b = new pwh_batch();
+  for (int i = 0; i < nsys; ++i) {
+    if (!systems[i] || nstates[i] < 1) { delete b; return fail(PWH_ERR_ARG, "bad system in batch"); }
+    b->sys.push_back(systems[i]);
+    b->nb.push_back(nstates[i]);
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      pwh_batch_destroy(b);
+      return fail(PWH_ERR_HIP, "batch stream/event creation failed");
+    }
+    b->streams.push_back(st);
+    b->done.push_back(ev);
+  }
+  HIPCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+  *out = b;
+  return PWH_OK;
+}
+
+int pwh_batch_destroy(pwh_batch* b) {
+  if (!b) return PWH_OK;
+  if (g.stream) hipStreamSynchronize(g.stream);
+  batch_drop_graph(b);
+  for (auto st : b->streams) { hipStreamSynchronize(st); hipStreamDestroy(st); }
+  for (auto ev : b->done) hipEventDestroy(ev);
+  if (b->fork) hipEventDestroy(b->fork);
+  delete b;
+  return PWH_OK;
+}
+
+int pwh_batch_set(pwh_batch* b, int i, const double* X) {
+  if (!b || i < 0 || i >= (int)b->sys.size() || !X) return fail(PWH_ERR_ARG, "bad batch index");
+  CHK(load_block(b->sys[i], X, b->nb[i]));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return PWH_OK;
+}
+
+int pwh_batch_get(pwh_batch* b, int i, double* X) {
+  if (!b || i < 0 || i >= (int)b->sys.size() || !X) return fail(PWH_ERR_ARG, "bad batch index");
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return store_block(b->sys[i], b->sys[i]->X.p, X, b->nb[i]);
+}
+
+int pwh_batch_chebyfilt(pwh_batch* b, int degree, const double* lb, const double* ub, int precision,
+                        int mode) {
+  if (!b || !lb || !ub || degree < 1) return fail(PWH_ERR_ARG, "bad batch filter arguments");
+  if (precision < PWH_FP64 || precision > PWH_BF16X3) return fail(PWH_ERR_ARG, "bad precision");
+  const int n = (int)b->sys.size();
+  for (int i = 0; i < n; ++i) {
+    if (!(ub[i] > lb[i])) return fail(PWH_ERR_ARG, "chebyfilt needs ub > lb");
+    CHK(filter_prepare(b->sys[i], b->nb[i], precision));
+  }
+  HIPCHK(hipStreamSynchronize(g.stream));
+  if (mode == 0) {
+    for (int i = 0; i < n; ++i)
+      CHK(filter_on_stream(b->sys[i], b->nb[i], degree, lb[i], ub[i], precision, g.stream));
+    return PWH_OK;
+  }
+  const bool same = b->exec && b->g_degree == degree && b->g_prec == precision &&
+                    std::equal(lb, lb + n, b->g_lb.begin()) && std::equal(ub, ub + n, b->g_ub.begin());
+  if (!same) {
+    batch_drop_graph(b);
+    const bool timing = g.timing;
+    g.timing = false;
+    int rc = PWH_OK;
+    HIPCHK(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+    hipEventRecord(b->fork, g.stream);
+    for (int i = 0; i < n && rc == PWH_OK; ++i) {
+      hipStreamWaitEvent(b->streams[i], b->fork, 0);
+      rc = filter_on_stream(b->sys[i], b->nb[i], degree, lb[i], ub[i], precision, b->streams[i]);
+      hipEventRecord(b->done[i], b->streams[i]);
+      hipStreamWaitEvent(g.stream, b->done[i], 0);
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(g.stream, &graph);
+    g.timing = timing;
+    if (rc != PWH_OK) { if (graph) hipGraphDestroy(graph); return rc; }
+    if (e != hipSuccess) return fail(PWH_ERR_HIP, std::string("stream capture: ") + hipGetErrorString(e));
+    b->graph = graph;
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->g_degree = degree;
+    b->g_prec = precision;
+    b->g_lb.assign(lb, lb + n);
+    b->g_ub.assign(ub, ub + n);
+  }
+  HIPCHK(hipGraphLaunch(b->exec, g.stream));
+  return PWH_OK;
 }
 
 int pwh_synchronize(void) {

@@ -241,9 +241,15 @@ __global__ __launch_bounds__(kColThreads) void k_zinv(SphereMaps m, const cx<T>*
 // ---------------------------------------------------------------------------
 // Pass 2: (i1,i2) planes: inverse y, inverse x, *V, forward x, forward y
 // ---------------------------------------------------------------------------
+// The plane tile keeps rows (fixed i2) at an odd stride S1 = N1 + 1 complex: with
+// transform-fast lane mapping a wave then touches 64 different rows at distinct banks
+// in every stage of the row transforms, and consecutive columns in the column ones.
+template <int N1>
+constexpr int plane_stride() { return N1 + 1; }
+
 template <typename T, int N1, int N2>
 constexpr size_t plane_smem() {
-  return sizeof(cx<T>) * (size_t)(N1 * N2 + N1 + N2) + sizeof(int) * N1 + N1;
+  return sizeof(cx<T>) * (size_t)(N2 * plane_stride<N1>() + N1 + N2) + sizeof(int) * N1 + N1;
 }
 
 template <typename T, int N1, int N2>
@@ -252,9 +258,10 @@ __global__ __launch_bounds__(kPlaneThreads) void k_plane(SphereMaps m, const cx<
                                                          const T* __restrict__ V,
                                                          cx<T>* __restrict__ W) {
   constexpr int NT = kPlaneThreads;
+  constexpr int S1 = plane_stride<N1>();
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
-  cx<T>* tw1 = tile + N1 * N2;
+  cx<T>* tw1 = tile + N2 * S1;
   cx<T>* tw2 = tw1 + N1;
   int* xa = reinterpret_cast<int*>(tw2 + N2);
   unsigned char* xact = reinterpret_cast<unsigned char*>(xa + N1);
@@ -268,19 +275,25 @@ __global__ __launch_bounds__(kPlaneThreads) void k_plane(SphereMaps m, const cx<
   // only the active columns need zeros: the x transforms never read the others
   for (int i = tid; i < nxa * N2; i += NT) {
     const int y = i / nxa, c = i - y * nxa;
-    tile[y * N1 + m.xa[c]] = mk<T>(0, 0);
+    tile[y * S1 + m.xa[c]] = mk<T>(0, 0);
   }
   __syncthreads();
   cx<T>* wrow = W + ((size_t)b * m.nr3 + z) * m.ncol_pad;
-  for (int c = tid; c < m.ncol; c += NT) tile[m.colxy[c]] = wrow[c];
+  for (int c = tid; c < m.ncol; c += NT) {
+    const int xy = m.colxy[c];
+    tile[(xy / N1) * S1 + xy % N1] = wrow[c];
+  }
   __syncthreads();
-  lds_fft<T, N2, +1, NT, N1, true>(tile, nxa, N1, [&](int t) { return xa[t]; }, tw2, tid);
+  lds_fft<T, N2, +1, NT, N1, true>(tile, nxa, S1, [&](int t) { return xa[t]; }, tw2, tid);
   const XInvHooks<T> hinv{xact, V + (size_t)z * N1 * N2, N1};
-  lds_fft<T, N1, +1, NT, N2, false>(tile, N2, 1, [](int t) { return t * N1; }, tw1, tid, hinv);
+  lds_fft<T, N1, +1, NT, N2, true>(tile, N2, 1, [](int t) { return t * S1; }, tw1, tid, hinv);
   const XFwdHooks hfwd{xact};
-  lds_fft<T, N1, -1, NT, N2, false>(tile, N2, 1, [](int t) { return t * N1; }, tw1, tid, hfwd);
-  lds_fft<T, N2, -1, NT, N1, true>(tile, nxa, N1, [&](int t) { return xa[t]; }, tw2, tid);
-  for (int c = tid; c < m.ncol; c += NT) wrow[c] = tile[m.colxy[c]];
+  lds_fft<T, N1, -1, NT, N2, true>(tile, N2, 1, [](int t) { return t * S1; }, tw1, tid, hfwd);
+  lds_fft<T, N2, -1, NT, N1, true>(tile, nxa, S1, [&](int t) { return xa[t]; }, tw2, tid);
+  for (int c = tid; c < m.ncol; c += NT) {
+    const int xy = m.colxy[c];
+    wrow[c] = tile[(xy / N1) * S1 + xy % N1];
+  }
 }
 
 // ---------------------------------------------------------------------------

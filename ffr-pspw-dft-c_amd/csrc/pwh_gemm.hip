@@ -209,20 +209,29 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
 // C = scale(m) * sum_s part[s][m][n], stored C[m*ldc + n] (row-major) or C[m + n*ldc]
 // (column-major, the LAPACK layout handed to rocSOLVER); scale(m) = wl[m % lm] when wl != null
 template <typename T, typename TO>
-__global__ void k_reduce_parts(const cx<T>* __restrict__ part, int S, int M, int N,
-                               cx<TO>* __restrict__ C, int ldc, const T* __restrict__ wl, int lm,
-                               int colmajor) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= M * N) return;
-  const int m = idx / N, n = idx - m * N;
-  TO sr = 0, si = 0;
-  for (int s = 0; s < S; ++s) {
-    const cx<T> v = part[(size_t)s * M * N + idx];
-    sr += (TO)v.x; si += (TO)v.y;
+__global__ __launch_bounds__(256) void k_reduce_parts(const cx<T>* __restrict__ part, int S, int M,
+                                                      int N, cx<TO>* __restrict__ C, int ldc,
+                                                      const T* __restrict__ wl, int lm, int colmajor) {
+  // 64 outputs per block; the 4 waves sum every 4th split, then combine in LDS
+  __shared__ TO sr[4][64], si[4][64];
+  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const int idx = blockIdx.x * 64 + lane;
+  TO ar = 0, ai = 0;
+  if (idx < M * N) {
+    for (int s = grp; s < S; s += 4) {
+      const cx<T> v = part[(size_t)s * M * N + idx];
+      ar += (TO)v.x; ai += (TO)v.y;
+    }
   }
-  if (wl) { const TO w = (TO)wl[m % lm]; sr *= w; si *= w; }
-  if (colmajor) C[m + (size_t)n * ldc] = mk<TO>(sr, si);
-  else C[(size_t)m * ldc + n] = mk<TO>(sr, si);
+  sr[grp][lane] = ar; si[grp][lane] = ai;
+  __syncthreads();
+  if (grp != 0 || idx >= M * N) return;
+  TO tr = (sr[0][lane] + sr[1][lane]) + (sr[2][lane] + sr[3][lane]);
+  TO ti = (si[0][lane] + si[1][lane]) + (si[2][lane] + si[3][lane]);
+  const int m = idx / N, n = idx - m * N;
+  if (wl) { const TO w = (TO)wl[m % lm]; tr *= w; ti *= w; }
+  if (colmajor) C[m + (size_t)n * ldc] = mk<TO>(tr, ti);
+  else C[(size_t)m * ldc + n] = mk<TO>(tr, ti);
 }
 
 // ---------------------------------------------------------------------------
@@ -386,7 +395,7 @@ hipError_t gemm_proj(const BetaGen& bg, const cx<T>* X, int ldx, int nb, int K, 
   hipError_t e = run_proj<T, true>(ld, X, ldx, M, nb, K, part, part_elems, mode == PREC_BF16X3, s, &S);
   if (e != hipSuccess) return e;
   const int tot = M * nb;
-  hipLaunchKernelGGL((k_reduce_parts<T, T>), dim3((tot + 255) / 256), dim3(256), 0, s, part, S, M,
+  hipLaunchKernelGGL((k_reduce_parts<T, T>), dim3((tot + 63) / 64), dim3(256), 0, s, part, S, M,
                      nb, C, nb, static_cast<const T*>(nullptr), bg.lm, 0);
   return hipGetLastError();
 }
@@ -419,7 +428,7 @@ hipError_t gemm_cn_f64(const cx<double>* A, int lda, int M, const cx<double>* B,
   hipError_t e = run_proj<double, false>(ld, B, ldb, M, N, K, part, part_elems, false, s, &S);
   if (e != hipSuccess) return e;
   const int tot = M * N;
-  hipLaunchKernelGGL((k_reduce_parts<double, double>), dim3((tot + 255) / 256), dim3(256), 0, s, part,
+  hipLaunchKernelGGL((k_reduce_parts<double, double>), dim3((tot + 63) / 64), dim3(256), 0, s, part,
                      S, M, N, C, ldc, static_cast<const double*>(nullptr), 1, 1);
   return hipGetLastError();
 }

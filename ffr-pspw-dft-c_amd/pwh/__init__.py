@@ -39,7 +39,8 @@ EXPORTS = [
     "pwh_apply_ham_block", "pwh_chebyfilt", "pwh_ortho_qr", "pwh_lanczos", "pwh_rayleigh_ritz",
     "pwh_residual_norms", "pwh_chfsi", "pwh_resident_set", "pwh_resident_get",
     "pwh_resident_apply", "pwh_resident_chebyfilt", "pwh_synchronize", "pwh_timing_enable",
-    "pwh_timing_reset", "pwh_timing_get",
+    "pwh_timing_reset", "pwh_timing_get", "pwh_batch_create", "pwh_batch_destroy",
+    "pwh_batch_set", "pwh_batch_get", "pwh_batch_chebyfilt",
 ]
 
 
@@ -115,6 +116,11 @@ def lib():
         "pwh_timing_enable": [C.c_int],
         "pwh_timing_reset": [],
         "pwh_timing_get": [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int)],
+        "pwh_batch_create": [C.POINTER(vp), _ip, C.c_int, C.POINTER(vp)],
+        "pwh_batch_destroy": [vp],
+        "pwh_batch_set": [vp, C.c_int, vp],
+        "pwh_batch_get": [vp, C.c_int, vp],
+        "pwh_batch_chebyfilt": [vp, C.c_int, _dp, _dp, C.c_int, C.c_int],
     }
     for name, args in proto.items():
         f = getattr(L, name)
@@ -373,3 +379,44 @@ class System:
     def resident_chebyfilt(self, nstates, degree, lb, ub, precision=FP64):
         _check(lib().pwh_resident_chebyfilt(self.h, int(nstates), int(degree), float(lb),
                                             float(ub), _prec(precision)))
+
+
+class Batch:
+    """Independent systems whose resident blocks are Chebyshev-filtered together
+    (pwh_batch_*): mode 0 one after the other, mode 1 one hipGraph with a branch per
+    system."""
+
+    def __init__(self, systems, nstates):
+        self.systems = list(systems)
+        self.nstates = [int(n) for n in nstates]
+        hs = (C.c_void_p * len(self.systems))(*[x.h.value for x in self.systems])
+        h = C.c_void_p()
+        _check(lib().pwh_batch_create(hs, np.asarray(self.nstates, np.int32), len(self.systems),
+                                      C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pwh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, i, X):
+        X = _block(X)
+        assert X.shape == (self.nstates[i], self.systems[i].ngw)
+        _check(lib().pwh_batch_set(self.h, int(i), _zptr(X)))
+
+    def get(self, i):
+        X = np.zeros((self.nstates[i], self.systems[i].ngw), np.complex128)
+        _check(lib().pwh_batch_get(self.h, int(i), _zptr(X)))
+        return X
+
+    def chebyfilt(self, degree, lb, ub, precision=FP64, mode=1):
+        lb = np.ascontiguousarray(lb, np.float64)
+        ub = np.ascontiguousarray(ub, np.float64)
+        _check(lib().pwh_batch_chebyfilt(self.h, int(degree), lb, ub, _prec(precision), int(mode)))

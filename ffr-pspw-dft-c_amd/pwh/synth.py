@@ -104,12 +104,13 @@ class SynthSystem:
         return np.ascontiguousarray(Q.T)
 
 
-def make_system(a, ecut, ncell, nr, nbands, name="custom", natoms=None):
+def make_system(a, ecut, ncell, nr, nbands, name="custom", natoms=None, frac=None):
     A = np.diag([a, a, a]).astype(np.float64)
-    frac = diamond_fractional(ncell)
-    if natoms is not None:
-        frac = frac[:natoms]
-    tau = frac * a
+    if frac is None:
+        frac = diamond_fractional(ncell)
+        if natoms is not None:
+            frac = frac[:natoms]
+    tau = np.asarray(frac, np.float64) * a
     gv = gvectors(A, ecut, nr)
     nat = tau.shape[0]
     eigr = phase_factor(A, nr, [nat], nat, tau.reshape(1, nat, 3), gv)
@@ -134,3 +135,37 @@ def make_system(a, ecut, ncell, nr, nbands, name="custom", natoms=None):
 def config(name):
     a, ecut, ncell, nr, nb = CONFIGS[name]
     return make_system(a, ecut, ncell, nr, nb, name=name)
+
+
+def smooth_grid(n):
+    """Smallest even 2,3,5-smooth size >= n (the FFT sizes the library compiles)."""
+    def smooth(m):
+        for p in (2, 3, 5):
+            while m % p == 0:
+                m //= p
+        return m == 1
+    n = max(n, 2)
+    while n % 2 or not smooth(n):
+        n += 1
+    return n
+
+
+def uneven_batch(count=24, seed=2026):
+    """Independent cubic cells with a ~ U[8,18] bohr and ecut ~ U[15,20] Ry.
+
+    The grid is the Setup.cpp:20-34 size rounded up to an even 2,3,5-smooth length.
+    The cell holds round(8 (a/10.26)^3) atoms (Si density) at seeded random fractional
+    positions, with 4 projectors per atom and 2*natoms + 4 bands (clipped to 8..96).
+    """
+    from . import fft_grid
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        a = float(rng.uniform(8.0, 18.0))
+        ecut = float(rng.uniform(15.0, 20.0))
+        n = smooth_grid(fft_grid(np.diag([a, a, a]), ecut)[0])
+        nat = max(1, int(round(8.0 * (a / 10.26) ** 3)))
+        frac = rng.random((nat, 3))
+        nb = int(min(96, max(8, 2 * nat + 4)))
+        out.append(make_system(a, ecut, 1, (n, n, n), nb, name=f"cell{i}", frac=frac))
+    return out

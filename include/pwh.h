@@ -137,6 +137,23 @@ int pwh_resident_chebyfilt(pwh_system *sys, int nstates, int degree, double lb, 
                            int precision);
 int pwh_synchronize(void);
 
+/* ---- many uneven systems through one captured filter ------------------------ */
+/* No counterpart in the reference (it runs one system per process): a batch of
+ * independent systems (different grids, plane-wave and band counts) whose resident
+ * blocks are Chebyshev-filtered together.  Every system's chain of passes runs on its
+ * own stream; mode 1 captures all of them, with fork/join edges, into one hipGraph
+ * that is replayed while degree, precision and bounds stay the same. */
+typedef struct pwh_batch pwh_batch;
+int pwh_batch_create(pwh_system *const *systems, const int *nstates, int nsys, pwh_batch **batch);
+int pwh_batch_destroy(pwh_batch *batch);
+/* Upload / download system isys's block [ngw x nstates] (reference order). */
+int pwh_batch_set(pwh_batch *batch, int isys, const double *X);
+int pwh_batch_get(pwh_batch *batch, int isys, double *X);
+/* chebyfilt.cpp:6-74 on every resident block; lb/ub [nsys].  mode 0: systems one after
+ * the other on the library stream; mode 1: one hipGraph with a branch per system. */
+int pwh_batch_chebyfilt(pwh_batch *batch, int degree, const double *lb, const double *ub,
+                        int precision, int mode);
+
 /* ---- per-kernel timing with HIP events on the library stream ---------------- */
 int pwh_timing_enable(int on);
 int pwh_timing_reset(void);

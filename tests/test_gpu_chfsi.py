@@ -110,3 +110,30 @@ def test_chfsi_mixed_matches_fp64(si8, gpu_si8, prec):
     e64, _, _, _ = gpu_si8.chfsi(X0, degree=10, niter=30, precision="fp64", lanczos_k=20, v0=v0)
     e32, _, _, _ = gpu_si8.chfsi(X0, degree=10, niter=30, precision=prec, lanczos_k=20, v0=v0)
     assert np.abs(e32[:14] - e64[:14]).max() < 1e-8
+
+
+def test_reference_bound_rules_match_oracle(si8, gpu_si8, oracle):
+    """bounds_mode 1 = ChebySCFLoop.cpp:176-196: k = max(2 NX, 10) Lanczos steps from v0,
+    dsyev('N','U') of T, lb = Ritz[NX+1], ub = Ritz[2 NX - 1] (NX = nstates - 2)."""
+    nb = 16
+    nx = nb - 2
+    k = max(2 * nx, 10)
+    rng = np.random.default_rng(19)
+    v0 = rng.random(si8.ngw) + 1j * rng.random(si8.ngw)
+    T, f, _ = oracle.lanczos(oracle_globals(si8), v0, k, si8.vloc, si8.ngw)
+    w = np.linalg.eigvalsh(np.triu(T) + np.triu(T, 1).T)
+    X0 = si8.psi(nb, seed=20)
+    _, _, _, st = gpu_si8.chfsi(X0, degree=4, niter=0, precision="fp64", lanczos_k=k,
+                                bounds_mode=1, v0=v0)
+    assert abs(st["lb"] - w[nx + 1]) < 1e-9
+    assert abs(st["ub"] - w[2 * nx - 1]) < 1e-9
+    # one outer iteration then the update rule: lb = largest RR value, ub from a 10-step
+    # Lanczos as induced_norm_mat(T, 10) + ||f|| (ChebySCFLoop.cpp:355-368)
+    eig, _, _, st1 = gpu_si8.chfsi(X0, degree=4, niter=1, precision="fp64", lanczos_k=k,
+                                   bounds_mode=1, v0=v0)
+    T10, f10, _ = oracle.lanczos(oracle_globals(si8), v0, 10, si8.vloc, si8.ngw)
+    Tk = np.zeros((k, k))
+    Tk[:10, :10] = T10
+    ub_ref = oracle.induced_norm_mat(Tk, 10) + np.linalg.norm(f10)
+    assert abs(st1["lb"] - eig[nb - 1]) < 1e-12
+    assert abs(st1["ub"] - ub_ref) < 1e-8 * ub_ref
