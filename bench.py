@@ -265,9 +265,9 @@ def main():
         # one untimed iteration per path loads the solver kernels (first-call cost)
         for p in ("fp64", "fp32", "bf16x3"):
             sysm.chfsi(X0, degree=2, niter=1, precision=p, lanczos_k=4, v0=v0)
-        # the 128x128 FP64 Rayleigh-Ritz eigensolve: rocSOLVER D&C, Jacobi, Jacobi-D&C
+        # the 128x128 FP64 Rayleigh-Ritz eigensolve: rocSOLVER D&C, QR iteration, Jacobi, Jacobi-D&C
         rr_ms = {}
-        for solver in ("evd", "evj", "evdj"):
+        for solver in ("evd", "ev", "evj", "evdj"):
             os.environ["PWH_EIGSOLVER"] = solver
             sysm.chfsi(X0, degree=2, niter=1, precision="fp32", lanczos_k=4, v0=v0)
             _, _, _, st = sysm.chfsi(X0, degree=2, niter=3, precision="fp32", lanczos_k=4, v0=v0)
@@ -292,6 +292,21 @@ def main():
                 d["max_abs_eig_err_vs_fp64_Ha"] = float(np.abs(eig[:nx] - ref_eig[:nx]).max())
             chfsi[p] = d
             log(f"bench: chfsi {p}: {json.dumps(d)}")
+        # the same start converged further (30 outer iterations): eigenvalue error of the
+        # mixed paths against the FP64 run once both have converged
+        conv, ref30 = {}, None
+        for p in ("fp64", "fp32", "bf16x3"):
+            eig, _, res, st = sysm.chfsi(X0, degree=10, niter=30, precision=p, lanczos_k=20, v0=v0)
+            nx = nb - 2
+            d = {"max_residual": float(res[:nx].max()),
+                 "s_per_iter": round((st["t_total_s"] - st["t_lanczos_s"]) / 30, 5)}
+            if p == "fp64":
+                ref30 = eig
+            else:
+                d["max_abs_eig_err_vs_fp64_Ha"] = float(np.abs(eig[:nx] - ref30[:nx]).max())
+            conv[p] = d
+        chfsi["converged_30_iterations"] = conv
+        log(f"bench: chfsi 30 iterations: {json.dumps(conv)}")
 
     extra = {}
     if not args.no_extra and not args.only:

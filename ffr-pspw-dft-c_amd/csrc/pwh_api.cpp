@@ -119,8 +119,9 @@ int ensure_init() {
   return pwh_init(0);
 }
 
-// Device buffer that grows on demand.  Growing waits for the library stream first so
-// no queued kernel still reads the old allocation.
+// Device buffer that grows on demand.  Growing waits for the device first so no queued
+// kernel (on the library stream or a system's second stream) still reads the old
+// allocation.
 template <typename T>
 struct Buf {
   T* p = nullptr;
@@ -132,7 +133,7 @@ struct Buf {
   int ensure(size_t n) {
     if (n <= cap) return PWH_OK;
     if (p) {
-      if (g.stream) hipStreamSynchronize(g.stream);
+      if (g.ready) hipDeviceSynchronize();
       hipFree(p);
     }
     p = nullptr;
@@ -145,7 +146,7 @@ struct Buf {
   }
   void release() {
     if (p) {
-      if (g.stream) hipStreamSynchronize(g.stream);
+      if (g.ready) hipDeviceSynchronize();
       hipFree(p);
     }
     p = nullptr;
@@ -190,6 +191,9 @@ struct pwh_system {
   // FP32 work
   Buf<cx<float>> X32, Y32, Y1_32, W32, Hnl32, proj32, Q32, part32;
   int resident_nb = 0;
+  // second stream for the nonlocal GEMMs, which overlap the FFT passes (fork/join events)
+  hipStream_t nl_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace {
@@ -223,10 +227,32 @@ template <> struct Sel<float> {
 };
 
 // out = alpha * H X + beta * X + gamma * Z  on a block of nb bands in sphere order.
+int overlap_prepare(pwh_system* s) {
+  if (s->nl_stream) return PWH_OK;
+  HIPCHK(hipStreamCreateWithFlags(&s->nl_stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming));
+  return PWH_OK;
+}
+
+// The fork/join costs two event operations per H|psi>; it pays only when the block is
+// large enough that the GEMMs run concurrently with a still-busy FFT pass (Si64: 13133 PW
+// x 128 bands gains 12%; 24 small cells of <= 5449 PW x 76 bands lose 25%).  Branches of
+// a captured batch already run side by side, so they never fork.
+constexpr size_t kOverlapMinElems = size_t(1) << 19;
+bool overlap_enabled(const pwh_system* s, int nb, hipStream_t st) {
+  static const int force = [] {
+    const char* e = getenv("PWH_NO_OVERLAP");
+    return (e && e[0] == '1') ? 0 : 1;
+  }();
+  return force && st == g.stream && (size_t)nb * (size_t)s->ngw >= kOverlapMinElems;
+}
+
 // Ensures every work buffer hpsi<T> needs for nb bands (so a later call allocates nothing).
 template <typename T>
 int hpsi_prepare(pwh_system* s, int nb) {
   const int ld = s->ngw_pad;
+  CHK(overlap_prepare(s));
   CHK(Sel<T>::W(s).ensure((size_t)nb * s->nr[2] * s->maps.ncol_pad));
   for (int sp = 0; sp < s->nsp; ++sp) {
     if (s->na[sp] == 0 || s->lm[sp] == 0) continue;
@@ -247,6 +273,17 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
   Timed th(TM_HPSI, st);
   const int ld = s->ngw_pad;
   CHK(Sel<T>::W(s).ensure((size_t)nb * s->nr[2] * s->maps.ncol_pad));
+  bool any_nl = false;
+  for (int sp = 0; sp < s->nsp; ++sp) any_nl = any_nl || (s->na[sp] > 0 && s->lm[sp] > 0);
+  // the nonlocal GEMMs run on a second stream, overlapping the FFT passes
+  hipStream_t sn = st;
+  const bool overlap = any_nl && overlap_enabled(s, nb, st);
+  if (overlap) {
+    CHK(overlap_prepare(s));
+    HIPCHK(hipEventRecord(s->ev_fork, st));
+    HIPCHK(hipStreamWaitEvent(s->nl_stream, s->ev_fork, 0));
+    sn = s->nl_stream;
+  }
   bool have_nl = false;
   for (int sp = 0; sp < s->nsp; ++sp) {
     if (s->na[sp] == 0 || s->lm[sp] == 0) continue;
@@ -258,17 +295,18 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
     CHK(Sel<T>::part(s).ensure(pe));
     BetaGen bg{Sel<T>::E(s), Sel<T>::P(s) + (size_t)sp * 4 * ld, s->aoff[sp], s->na[sp], s->lm[sp], ld};
     {
-      Timed t(TM_PROJ, st);
-      HIPCHK(gemm_proj<T>(bg, X, ld, nb, s->ngw, Sel<T>::proj(s).p, Sel<T>::part(s).p, pe, mode, st));
+      Timed t(TM_PROJ, sn);
+      HIPCHK(gemm_proj<T>(bg, X, ld, nb, s->ngw, Sel<T>::proj(s).p, Sel<T>::part(s).p, pe, mode, sn));
     }
     HIPCHK(scale_proj<T>(Sel<T>::proj(s).p, Sel<T>::wl(s) + 4 * sp, s->na[sp], s->lm[sp], nb,
-                         Sel<T>::Q(s).p, st));
+                         Sel<T>::Q(s).p, sn));
     {
-      Timed t(TM_EXPAND, st);
-      HIPCHK(gemm_expand<T>(bg, Sel<T>::Q(s).p, nb, s->ngw, Sel<T>::Hnl(s).p, ld, have_nl, mode, st));
+      Timed t(TM_EXPAND, sn);
+      HIPCHK(gemm_expand<T>(bg, Sel<T>::Q(s).p, nb, s->ngw, Sel<T>::Hnl(s).p, ld, have_nl, mode, sn));
     }
     have_nl = true;
   }
+  if (overlap) HIPCHK(hipEventRecord(s->ev_join, sn));
   {
     Timed t(TM_ZINV, st);
     HIPCHK(launch_zinv<T>(s->maps, Sel<T>::tw(s, 2), X, ld, Sel<T>::W(s).p, nb, st));
@@ -278,6 +316,7 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
     HIPCHK(launch_plane<T>(s->maps, Sel<T>::tw(s, 0), Sel<T>::tw(s, 1), Sel<T>::V(s), Sel<T>::W(s).p,
                            nb, st));
   }
+  if (overlap) HIPCHK(hipStreamWaitEvent(st, s->ev_join, 0));
   {
     Timed t(TM_ZFWD, st);
     HIPCHK(launch_zfwd_epilogue<T>(s->maps, Sel<T>::tw(s, 2), Sel<T>::W(s).p, Sel<T>::kin(s), X,
@@ -441,14 +480,16 @@ int sys_cholqr2(pwh_system* s, cx<double>* X, int nb) {
 
 // FP64 Hermitian eigensolve on the GPU, eig_zheevd.cpp:5-34 ('V', 'U'): A (n x n,
 // column-major) is replaced by its eigenvectors, w gets the eigenvalues ascending.
-// PWH_EIGSOLVER selects rocSOLVER's divide and conquer (evd, default), Jacobi (evj) or
-// Jacobi with divide and conquer (evdj).  work: n doubles + 2 ints of device scratch.
+// PWH_EIGSOLVER selects rocSOLVER's divide and conquer (evd, default), QR iteration (ev),
+// Jacobi (evj) or Jacobi with divide and conquer (evdj).  work: n doubles + 2 ints of device scratch.
 int hermitian_eig(cx<double>* A, int n, double* w, double* work, rocblas_int* info) {
   const char* e = getenv("PWH_EIGSOLVER");
   auto* a = reinterpret_cast<rocblas_double_complex*>(A);
   if (e && !strcmp(e, "evj")) {
     RBCHK(rocsolver_zheevj(g.rb, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, n,
                            a, n, 0.0, work, 100, info + 1, w, info));
+  } else if (e && !strcmp(e, "ev")) {
+    RBCHK(rocsolver_zheev(g.rb, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, work, info));
   } else if (e && !strcmp(e, "evdj")) {
     RBCHK(rocsolver_zheevdj(g.rb, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, info));
   } else {
@@ -794,6 +835,9 @@ int pwh_system_destroy(pwh_system* s) {
   for (auto* b : {&s->P32, &s->kin32, &s->wl32, &s->V32}) b->release();
   for (int d = 0; d < 3; ++d) { s->tw64[d].release(); s->tw32[d].release(); }
   s->info.release();
+  if (s->nl_stream) { hipStreamSynchronize(s->nl_stream); hipStreamDestroy(s->nl_stream); }
+  if (s->ev_fork) hipEventDestroy(s->ev_fork);
+  if (s->ev_join) hipEventDestroy(s->ev_join);
   delete s;
   return PWH_OK;
 }

