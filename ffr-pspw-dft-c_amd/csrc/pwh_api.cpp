@@ -238,14 +238,15 @@ int overlap_prepare(pwh_system* s) {
 // The fork/join costs two event operations per H|psi>; it pays only when the block is
 // large enough that the GEMMs run concurrently with a still-busy FFT pass (Si64: 13133 PW
 // x 128 bands gains 12%; 24 small cells of <= 5449 PW x 76 bands lose 25%).  Branches of
-// a captured batch already run side by side, so they never fork.
+// a captured batch already run side by side, so they never fork.  With per-kernel timing
+// on, everything stays on one stream so each event pair brackets one kernel running alone.
 constexpr size_t kOverlapMinElems = size_t(1) << 19;
 bool overlap_enabled(const pwh_system* s, int nb, hipStream_t st) {
   static const int force = [] {
     const char* e = getenv("PWH_NO_OVERLAP");
     return (e && e[0] == '1') ? 0 : 1;
   }();
-  return force && st == g.stream && (size_t)nb * (size_t)s->ngw >= kOverlapMinElems;
+  return force && !g.timing && st == g.stream && (size_t)nb * (size_t)s->ngw >= kOverlapMinElems;
 }
 
 // Ensures every work buffer hpsi<T> needs for nb bands (so a later call allocates nothing).
@@ -559,7 +560,7 @@ int lanczos_dev(pwh_system* s, cx<double>* v0, int k, double* T, int ldT, cx<dou
   cx<double>* dh = s->S.p;
   cx<double>* dcoef = s->S.p + k + 1;
   std::vector<cx<double>> h(k), sv(k), coef(k);
-  for (int i = 0; i < ldT * k; ++i) T[i] = 0.0;
+  for (int i = 0; i < k * k; ++i) T[i] = 0.0;   // the reference clears k*k entries (lanczos.cpp:26)
   auto dot = [&](const cx<double>* x, const cx<double>* y, cx<double>* out) -> int {
     HIPCHK(multi_dotc(x, ld, 1, y, n, dh, g.stream));
     HIPCHK(hipMemcpyAsync(out, dh, sizeof(cx<double>), hipMemcpyDeviceToHost, g.stream));
@@ -651,6 +652,19 @@ double induced_norm_sq(const double* A, int ldA, int N) {
 
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Device buffers of the system-free entry points (dense FFT, eigensolver), kept for the
+// life of the process so repeated calls allocate nothing after the first.
+struct Scratch {
+  Buf<cx<double>> fft, fft_tw[3], eig_a;
+  int fft_n[3] = {0, 0, 0};
+  Buf<double> eig_w, eig_work;
+  Buf<rocblas_int> eig_info;
+};
+Scratch& scratch() {
+  static Scratch* s = new Scratch();   // never destroyed: no HIP calls after runtime teardown
+  return *s;
 }
 
 std::vector<cx<double>> twiddles(int n) {
@@ -779,6 +793,13 @@ int pwh_system_create(const pwh_system_desc* d, pwh_system** out) {
   s->maps.nr1 = nr1; s->maps.nr2 = nr2; s->maps.nr3 = d->nr3; s->maps.nnr = s->nnr;
   s->maps.ngw = ngw; s->maps.ngw_pad = s->ngw_pad;
   s->maps.ncol = ncol; s->maps.ncol_pad = ncol_pad; s->maps.nxa = (int)xa.size();
+  {
+    auto in_band = [](int i, int n) { return i < n / 4 || i >= n - n / 4; };
+    bool band = nr1 == nr2 && nr1 % 4 == 0;
+    for (int c = 0; band && c < ncol; ++c)
+      band = in_band(colxy[c] % nr1, nr1) && in_band(colxy[c] / nr1, nr2);
+    s->maps.band_xy = band ? 1 : 0;
+  }
   s->maps.colstart = s->d_colstart.p; s->maps.kcol = s->d_kcol.p; s->maps.kz = s->d_kz.p;
   s->maps.colxy = s->d_colxy.p; s->maps.xa = s->d_xa.p;
 
@@ -917,6 +938,56 @@ int pwh_ortho_qr(double* X, int nbasis, int nstates) {
   hipStreamSynchronize(g.stream);
   dX.release(); dT.release(); dS.release(); dP.release(); di.release();
   return rc;
+}
+
+int pwh_fft3d(double* data, int nr1, int nr2, int nr3, int inverse) {
+  if (!data || nr1 < 1 || nr2 < 1 || nr3 < 1) return fail(PWH_ERR_ARG, "bad fft3d arguments");
+  const int nr[3] = {nr1, nr2, nr3};
+  for (int d = 0; d < 3; ++d)
+    if (!fft_size_supported(nr[d]))
+      return fail(PWH_ERR_UNSUPPORTED, "no compiled FFT plan for length " + std::to_string(nr[d]));
+  CHK(ensure_init());
+  Scratch& w = scratch();
+  const size_t nnr = (size_t)nr1 * nr2 * nr3;
+  for (int d = 0; d < 3; ++d) {
+    if (w.fft_n[d] != nr[d]) {
+      const auto t = twiddles(nr[d]);
+      CHK(upload(w.fft_tw[d], t.data(), t.size()));
+      w.fft_n[d] = nr[d];
+    }
+  }
+  CHK(upload(w.fft, reinterpret_cast<const cx<double>*>(data), nnr));
+  const int sign = inverse ? +1 : -1;
+  const double scale = inverse ? 1.0 : 1.0 / (double)nnr;
+  // z lines (stride nr1*nr2), y lines (stride nr1), x lines (contiguous, scaled)
+  HIPCHK(launch_fft_lines(w.fft.p, nr3, nr1 * nr2, nr1 * nr2, w.fft_tw[2].p, sign, 1.0, g.stream));
+  HIPCHK(launch_fft_lines(w.fft.p, nr2, nr1, nr1 * nr3, w.fft_tw[1].p, sign, 1.0, g.stream));
+  HIPCHK(launch_fft_lines(w.fft.p, nr1, 1, nr2 * nr3, w.fft_tw[0].p, sign, scale, g.stream));
+  HIPCHK(hipMemcpyAsync(data, w.fft.p, sizeof(cx<double>) * nnr, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return PWH_OK;
+}
+
+int pwh_eig_zheevd(double* A, int ldA, double* lambda, int n) {
+  if (!A || !lambda || n < 1 || ldA < n) return fail(PWH_ERR_ARG, "bad eig_zheevd arguments");
+  CHK(ensure_init());
+  Scratch& w = scratch();
+  CHK(w.eig_a.ensure((size_t)n * n));
+  CHK(w.eig_w.ensure(n));
+  CHK(w.eig_work.ensure(n));
+  CHK(w.eig_info.ensure(2));
+  HIPCHK(hipMemcpy2DAsync(w.eig_a.p, sizeof(cx<double>) * n, A, sizeof(cx<double>) * ldA,
+                          sizeof(cx<double>) * n, n, hipMemcpyHostToDevice, g.stream));
+  CHK(hermitian_eig(w.eig_a.p, n, w.eig_w.p, w.eig_work.p, w.eig_info.p));
+  rocblas_int info = 0;
+  HIPCHK(hipMemcpyAsync(&info, w.eig_info.p, sizeof(info), hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  if (info != 0) return fail(PWH_ERR_LAPACK, "zheevd info " + std::to_string(info));
+  HIPCHK(hipMemcpy2DAsync(A, sizeof(cx<double>) * ldA, w.eig_a.p, sizeof(cx<double>) * n,
+                          sizeof(cx<double>) * n, n, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipMemcpyAsync(lambda, w.eig_w.p, sizeof(double) * n, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return PWH_OK;
 }
 
 int pwh_lanczos(pwh_system* s, double* v0, int k, double* T, int ldT, double* f) {

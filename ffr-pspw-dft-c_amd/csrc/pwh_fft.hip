@@ -297,6 +297,237 @@ __global__ __launch_bounds__(NT) void k_plane(SphereMaps m, const cx<T>* __restr
 }
 
 // ---------------------------------------------------------------------------
+// Pass 2, band-pruned: the sphere's i1 and i2 lie in the band [-n/4, n/4)
+// ---------------------------------------------------------------------------
+// With a first inverse radix R in {4, 8} (n % 4 == 0), butterfly j of the first stage
+// reads elements j + r n/R, which lie in the band exactly for r < R/4 or r >= R - R/4:
+// half of its inputs are known zeros.  The forward transforms run the radices in
+// reverse order, so their last stage has the same R and half of its outputs (those
+// outside the band) are never needed.  The column transforms run over the n/2 band
+// columns (a compile-time count) instead of the active ones.
+
+constexpr int nstages(int n) {
+  int s = 0;
+  for (int rem = n; rem > 1; rem /= pick_radix(rem)) ++s;
+  return s;
+}
+constexpr int radix_nat(int n, int i) {
+  int rem = n;
+  for (int s = 0; s < i; ++s) rem /= pick_radix(rem);
+  return pick_radix(rem);
+}
+constexpr int radix_at(int n, int i, bool rev) { return rev ? radix_nat(n, nstages(n) - 1 - i) : radix_nat(n, i); }
+constexpr int ns_at(int n, int i, bool rev) {
+  int ns = 1;
+  for (int s = 0; s < i; ++s) ns *= radix_at(n, s, rev);
+  return ns;
+}
+constexpr bool band_live(int R, int r) { return r < R / 4 || r >= R - R / 4; }
+
+// DFT of 4 with v[1] = v[2] = 0
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft4_in03(cx<T>* v) {
+  const cx<T> a = v[0], d = v[3], sd = mul_si<SIGN>(d);
+  v[0] = a + d; v[1] = a - sd; v[2] = a - d; v[3] = a + sd;
+}
+// DFT of 8 with v[2..5] = 0 (even/odd split, each half a two-input DFT of 4)
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft8_in0167(cx<T>* v) {
+  const cx<T> e0 = v[0], e3 = v[6], o0 = v[1], o3 = v[7];
+  const cx<T> se = mul_si<SIGN>(e3), so = mul_si<SIGN>(o3);
+  const cx<T> E0 = e0 + e3, E1 = e0 - se, E2 = e0 - e3, E3 = e0 + se;
+  const cx<T> O0 = o0 + o3, O1 = o0 - so, O2 = o0 - o3, O3 = o0 + so;
+  const T c = T(0.70710678118654752440084436210485);
+  const cx<T> w1 = mk<T>(c * (O1.x - SIGN * O1.y), c * (O1.y + SIGN * O1.x));
+  const cx<T> w2 = mul_si<SIGN>(O2);
+  const cx<T> w3 = mk<T>(c * (-O3.x - SIGN * O3.y), c * (-O3.y + SIGN * O3.x));
+  v[0] = E0 + O0; v[4] = E0 - O0;
+  v[1] = E1 + w1; v[5] = E1 - w1;
+  v[2] = E2 + w2; v[6] = E2 - w2;
+  v[3] = E3 + w3; v[7] = E3 - w3;
+}
+// DFT of 4, outputs 0 and 3 only
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft4_out03(cx<T>* v) {
+  const cx<T> a = v[0] + v[2], b = v[1] + v[3], c = v[0] - v[2], d = mul_si<SIGN>(v[1] - v[3]);
+  v[0] = a + b; v[3] = c - d;
+}
+// DFT of 8, outputs 0, 1, 6, 7 only: X[2m] = DFT4(v_n + v_n+4)[m], X[2m+1] =
+// DFT4((v_n - v_n+4) w^n)[m] for m in {0, 3}
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft8_out0167(cx<T>* v) {
+  cx<T> s[4], d[4];
+#pragma unroll
+  for (int n = 0; n < 4; ++n) { s[n] = v[n] + v[n + 4]; d[n] = v[n] - v[n + 4]; }
+  const T c = T(0.70710678118654752440084436210485);
+  d[1] = mk<T>(c * (d[1].x - SIGN * d[1].y), c * (d[1].y + SIGN * d[1].x));
+  d[2] = mul_si<SIGN>(d[2]);
+  d[3] = mk<T>(c * (-d[3].x - SIGN * d[3].y), c * (-d[3].y + SIGN * d[3].x));
+  dft4_out03<T, SIGN>(s);
+  dft4_out03<T, SIGN>(d);
+  v[0] = s[0]; v[6] = s[3]; v[1] = d[0]; v[7] = d[3];
+}
+
+struct NoPost {
+  template <typename V>
+  __device__ __forceinline__ void operator()(int, int, V&) const {}
+};
+
+// One Stockham stage (radix R, NS = product of the earlier radices) over COUNT transforms
+// of length N; element e of transform t lives at buf[base(t) + e*ES].  Consecutive lanes
+// take consecutive transforms.  PR: 0 full, 1 inputs outside the band are zero, 2 only
+// outputs in the band are stored.  When COUNT is a multiple of 64 every wave works on
+// one butterfly index j, so its twiddles are read as scalars from twg.
+template <typename T, int N, int R, int NS, int SIGN, int NT, int COUNT, int ES, int PR, bool LASTST,
+          typename BaseF, typename Post>
+__device__ __forceinline__ void band_stage(cx<T>* buf, BaseF base, const cx<T>* tw,
+                                           const cx<T>* __restrict__ twg, int tid, const Post& post) {
+  constexpr int NB = N / R;
+  constexpr int TOTAL = COUNT * NB;
+  constexpr int MAXB = (TOTAL + NT - 1) / NT;
+  constexpr bool UNIFORM = (COUNT % 64 == 0);
+  static_assert(PR == 0 || R == 4 || R == 8, "band pruning needs a radix-4 or radix-8 stage");
+  cx<T> v[MAXB][R];
+  int tb[MAXB], jv[MAXB];
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    const int idx = tid + q * NT;
+    jv[q] = -1;
+    tb[q] = 0;
+    if (TOTAL % NT == 0 || idx < TOTAL) {
+      int j = idx / COUNT;
+      if constexpr (UNIFORM) j = __builtin_amdgcn_readfirstlane(j);
+      const int t = idx - j * COUNT;
+      const int b = base(t);
+      tb[q] = b;
+      jv[q] = j;
+      const int bo = b + j * ES;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (PR == 1 && !band_live(R, r)) v[q][r] = mk<T>(0, 0);
+        else v[q][r] = buf[bo + r * NB * ES];
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXB; ++q) {
+    if (jv[q] >= 0) {
+      int j = jv[q];
+      if constexpr (UNIFORM) j = __builtin_amdgcn_readfirstlane(j);
+      const int k = j % NS;
+      if constexpr (NS > 1) {
+        const int ks = k * (N / (NS * R));
+        int ti = 0;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          ti += ks;
+          cx<T> w = UNIFORM ? twg[ti] : tw[ti];
+          if constexpr (SIGN < 0) w.y = -w.y;
+          v[q][r] = v[q][r] * w;
+        }
+      }
+      if constexpr (PR == 1 && R == 8) dft8_in0167<T, SIGN>(v[q]);
+      else if constexpr (PR == 1 && R == 4) dft4_in03<T, SIGN>(v[q]);
+      else if constexpr (PR == 2 && R == 8) dft8_out0167<T, SIGN>(v[q]);
+      else if constexpr (PR == 2 && R == 4) dft4_out03<T, SIGN>(v[q]);
+      else dft<T, R, SIGN>(v[q]);
+      const int o = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (PR == 2 && !band_live(R, r)) continue;
+        if constexpr (LASTST) post(q, r, v[q][r]);
+        buf[tb[q] + (o + r * NS) * ES] = v[q][r];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Full transform: stages I..nstages(N)-1.  PIN prunes the first stage's inputs, POUT the
+// last stage's outputs; REV runs the radices in reverse order.
+template <typename T, int N, int SIGN, bool REV, int NT, int COUNT, int ES, bool PIN, bool POUT,
+          int I = 0, typename BaseF, typename Post>
+__device__ __forceinline__ void band_fft(cx<T>* buf, BaseF base, const cx<T>* tw, const cx<T>* __restrict__ twg,
+                                         int tid, const Post& post) {
+  constexpr int NSTG = nstages(N);
+  if constexpr (I < NSTG) {
+    constexpr int R = radix_at(N, I, REV), NS = ns_at(N, I, REV);
+    constexpr int PR = (PIN && I == 0) ? 1 : (POUT && I == NSTG - 1) ? 2 : 0;
+    band_stage<T, N, R, NS, SIGN, NT, COUNT, ES, PR, I == NSTG - 1>(buf, base, tw, twg, tid, post);
+    band_fft<T, N, SIGN, REV, NT, COUNT, ES, PIN, POUT, I + 1>(buf, base, tw, twg, tid, post);
+  }
+}
+
+template <int N>
+constexpr int band_plane_threads() {
+  const int w = N * (N / radix_nat(N, 0));
+  const int r = (w + 63) / 64 * 64;
+  return r < 512 ? r : 512;
+}
+
+template <typename T, int N>
+constexpr size_t band_plane_smem() {
+  return sizeof(cx<T>) * ((size_t)N * (N + 1) + N);
+}
+
+template <typename T, int N, int NT>
+__global__ __launch_bounds__(NT) void k_plane_band(SphereMaps m, const cx<T>* __restrict__ twg,
+                                                   const T* __restrict__ V, cx<T>* __restrict__ W) {
+  constexpr int S1 = N + 1, Q = N / 4, NBC = N / 2;
+  static_assert(N % 4 == 0, "band pruning needs n % 4 == 0");
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
+  cx<T>* tw = tile + N * S1;
+  const int tid = threadIdx.x;
+  const int z = blockIdx.x, b = blockIdx.y;
+
+  // V for the outputs this lane produces in the last inverse row stage, loaded first so
+  // the latency hides behind the column transforms
+  constexpr int RL = radix_at(N, nstages(N) - 1, false), NSL = N / RL;
+  constexpr int TOTL = N * NSL, MBL = (TOTL + NT - 1) / NT;
+  T vp[MBL][RL];
+  const T* Vz = V + (size_t)z * N * N;
+#pragma unroll
+  for (int q = 0; q < MBL; ++q) {
+    const int idx = tid + q * NT;
+    if (TOTL % NT == 0 || idx < TOTL) {
+      const int j = idx / N, t = idx - j * N;
+#pragma unroll
+      for (int r = 0; r < RL; ++r) vp[q][r] = Vz[t * N + j + r * NSL];
+    }
+  }
+
+  auto band_x = [](int t) { return t < Q ? t : t + NBC; };
+  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
+  for (int i = tid; i < NBC * NBC; i += NT) {
+    const int yy = i / NBC, xx = i - yy * NBC;
+    tile[band_x(yy) * S1 + band_x(xx)] = mk<T>(0, 0);
+  }
+  __syncthreads();
+  cx<T>* wrow = W + ((size_t)b * m.nr3 + z) * m.ncol_pad;
+  for (int c = tid; c < m.ncol; c += NT) {
+    const int xy = m.colxy[c];
+    tile[(xy / N) * S1 + xy % N] = wrow[c];
+  }
+  __syncthreads();
+  const NoPost none;
+  auto col_base = [&](int t) { return band_x(t); };
+  auto row_base = [](int t) { return t * S1; };
+  auto vmul = [&](int q, int r, cx<T>& v) { v = vp[q][r] * v; };
+  // inverse along i2 on the band columns, inverse along i1 on every row (times V),
+  // forward along i1 keeping band columns, forward along i2 keeping band rows
+  band_fft<T, N, +1, false, NT, NBC, S1, true, false>(tile, col_base, tw, twg, tid, none);
+  band_fft<T, N, +1, false, NT, N, 1, true, false>(tile, row_base, tw, twg, tid, vmul);
+  band_fft<T, N, -1, true, NT, N, 1, false, true>(tile, row_base, tw, twg, tid, none);
+  band_fft<T, N, -1, true, NT, NBC, S1, false, true>(tile, col_base, tw, twg, tid, none);
+  for (int c = tid; c < m.ncol; c += NT) {
+    const int xy = m.colxy[c];
+    wrow[c] = tile[(xy / N) * S1 + xy % N];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Pass 2 with the 1D transforms in registers: two lanes per transform
 // ---------------------------------------------------------------------------
 // exp(SIGN 2 pi i e / N) for an e that is a compile-time constant after unrolling:
@@ -507,6 +738,43 @@ __global__ __launch_bounds__(kColThreads) void k_zfwd(SphereMaps m, const cx<T>*
 }
 
 // ---------------------------------------------------------------------------
+// Dense 3D transform (fft_fftw3.cpp:13-46): one pass per axis over every line
+// ---------------------------------------------------------------------------
+// The array is [outer][N][inner]; a line is the N elements at stride `inner`.  A block
+// transforms kColTile lines: consecutive lines when inner > 1 (coalesced along inner),
+// or consecutive contiguous lines when inner == 1.  `scale` multiplies the result.
+template <typename T, int N, int SIGN>
+__global__ __launch_bounds__(kColThreads) void k_fft_lines(cx<T>* __restrict__ data, int inner, int nlines,
+                                                           const cx<T>* __restrict__ twg, T scale) {
+  constexpr int CT = kColTile, NT = kColThreads, S = CT + 1;
+  __shared__ cx<T> tile[N * S];
+  __shared__ cx<T> tw[N];
+  const int tid = threadIdx.x;
+  const long long l0 = (long long)blockIdx.x * CT;
+  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
+  auto addr = [&](int t, int e) -> long long {
+    const long long L = l0 + t;
+    if (inner == 1) return L * N + e;
+    const long long o = L / inner, i = L - o * inner;
+    return o * N * inner + i + (long long)e * inner;
+  };
+  for (int idx = tid; idx < N * CT; idx += NT) {
+    int t, e;
+    if (inner == 1) { t = idx / N; e = idx - t * N; }
+    else { e = idx / CT; t = idx - e * CT; }
+    tile[e * S + t] = (l0 + t < nlines) ? data[addr(t, e)] : mk<T>(0, 0);
+  }
+  __syncthreads();
+  lds_fft<T, N, SIGN, NT, CT, true>(tile, CT, S, [](int t) { return t; }, tw, tid);
+  for (int idx = tid; idx < N * CT; idx += NT) {
+    int t, e;
+    if (inner == 1) { t = idx / N; e = idx - t * N; }
+    else { e = idx / CT; t = idx - e * CT; }
+    if (l0 + t < nlines) data[addr(t, e)] = scale * tile[e * S + t];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Dispatch
 // ---------------------------------------------------------------------------
 #define PWH_FFT_SIZES(X) \
@@ -570,6 +838,27 @@ static hipError_t launch_plane_n(const SphereMaps& m, const void* tw1, const voi
       return hipGetLastError();
     }
   }
+  static const bool band_on = [] {
+    const char* e = getenv("PWH_PLANE_BAND");
+    return !(e && e[0] == '0');
+  }();
+  if constexpr (N % 4 == 0 && band_plane_smem<T, N>() <= 160 * 1024) {
+    if (m.band_xy && band_on) {
+      constexpr int NT = band_plane_threads<N>();
+      constexpr size_t bsm = band_plane_smem<T, N>();
+      static bool battr = false;
+      if (!battr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_band<T, N, NT>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bsm);
+        if (e != hipSuccess) return e;
+        battr = true;
+      }
+      dim3 grid(m.nr3, nb);
+      hipLaunchKernelGGL((k_plane_band<T, N, NT>), grid, dim3(NT), bsm, s, m,
+                         static_cast<const cx<T>*>(tw1), V, W);
+      return hipGetLastError();
+    }
+  }
   constexpr size_t smem = plane_smem<T, N, N>();
   if constexpr (smem > 160 * 1024) {
     return hipErrorInvalidValue;
@@ -622,6 +911,26 @@ hipError_t launch_zfwd_epilogue(const SphereMaps& m, const void* tw3, const cx<T
     hipLaunchKernelGGL((k_zfwd<T, N>), cgrid, dim3(kColThreads), 0, s, m,                         \
                        static_cast<const cx<T>*>(tw3), W, kin, X, Hnl, Z, out, ld, inv_nnr,       \
                        alpha, beta, gamma);                                                       \
+    return hipGetLastError();
+    PWH_FFT_SIZES(PWH_CASE)
+#undef PWH_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_fft_lines(cx<double>* data, int n, int inner, int nlines, const cx<double>* tw,
+                            int sign, double scale, hipStream_t s) {
+  if (!fft_size_supported(n) || nlines < 1 || inner < 1) return hipErrorInvalidValue;
+  dim3 grid((nlines + kColTile - 1) / kColTile);
+  switch (n) {
+#define PWH_CASE(N)                                                                                \
+  case N:                                                                                          \
+    if (sign > 0)                                                                                  \
+      hipLaunchKernelGGL((k_fft_lines<double, N, +1>), grid, dim3(kColThreads), 0, s, data, inner, \
+                         nlines, tw, scale);                                                       \
+    else                                                                                           \
+      hipLaunchKernelGGL((k_fft_lines<double, N, -1>), grid, dim3(kColThreads), 0, s, data, inner, \
+                         nlines, tw, scale);                                                       \
     return hipGetLastError();
     PWH_FFT_SIZES(PWH_CASE)
 #undef PWH_CASE

@@ -150,13 +150,16 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
     __syncthreads();
     if (k0 + BKS < ke) fetch(k0 + BKS);
     if constexpr (!BF16X3) {
+      // substep sub reduces over k = 4*lq + sub: lane quarters then read rows 4 apart,
+      // 4*(BT+LPAD) = 16 banks, so the four quarters of a wave never share a bank
 #pragma unroll
-      for (int kk = 0; kk < BKS; kk += 4) {
+      for (int sub = 0; sub < 4; ++sub) {
         T ar[2], ai[2], br[2], bi[2];
+        const int kr = 4 * lq + sub;
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-          ar[t] = sAr[kk + lq][wm + t * 16 + li]; ai[t] = sAi[kk + lq][wm + t * 16 + li];
-          br[t] = sBr[kk + lq][wn + t * 16 + li]; bi[t] = sBi[kk + lq][wn + t * 16 + li];
+          ar[t] = sAr[kr][wm + t * 16 + li]; ai[t] = sAi[kr][wm + t * 16 + li];
+          br[t] = sBr[kr][wn + t * 16 + li]; bi[t] = sBi[kr][wn + t * 16 + li];
         }
         // first halves of all 8 accumulators, then the second halves: consecutive
         // MFMAs never feed each other
@@ -297,13 +300,15 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
     __syncthreads();
     if (j0 + BKS < J) fetch(j0 + BKS);
     if constexpr (!BF16X3) {
+      // reduction index j = 4*lq + sub (conflict-free rows, as in k_gemm_proj)
 #pragma unroll
-      for (int jj = 0; jj < BKS; jj += 4) {
+      for (int sub = 0; sub < 4; ++sub) {
         T ar[2], ai[2], br[2], bi[2];
+        const int jr = 4 * lq + sub;
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-          ar[t] = sAr[jj + lq][wm + t * 16 + li]; ai[t] = sAi[jj + lq][wm + t * 16 + li];
-          br[t] = sQr[jj + lq][wn + t * 16 + li]; bi[t] = sQi[jj + lq][wn + t * 16 + li];
+          ar[t] = sAr[jr][wm + t * 16 + li]; ai[t] = sAi[jr][wm + t * 16 + li];
+          br[t] = sQr[jr][wn + t * 16 + li]; bi[t] = sQi[jr][wn + t * 16 + li];
         }
 #pragma unroll
         for (int a = 0; a < 2; ++a)

@@ -52,6 +52,10 @@ struct SphereMaps {
   int ngw, ngw_pad;
   int ncol, ncol_pad;   // columns (i1,i2) that hold sphere points; padded to the column tile
   int nxa;              // number of distinct i1 among the columns
+  // 1 when nr1 == nr2, nr1 % 4 == 0 and every column has i1, i2 in the band
+  // [0, n/4) U [n - n/4, n), i.e. signed index in [-n/4, n/4): true for grids sized
+  // for the density cutoff (4 ecut), where the sphere radius is below n/4
+  int band_xy;
   int* colstart;        // [ncol_pad+1] first internal slot of each column
   int* kcol;            // [ngw] column of each internal slot
   int* kz;              // [ngw] i3 of each internal slot
@@ -73,6 +77,10 @@ hipError_t launch_zfwd_epilogue(const SphereMaps& m, const void* tw3, const cx<T
                                 hipStream_t s);
 bool fft_size_supported(int n);
 bool plane_supported(int n1, int n2);
+// Dense FP64 transform of length n along an axis of stride `inner` for every line of an
+// [outer][n][inner] array, in place; tw[m] = exp(+2 pi i m / n); sign +1 inverse, -1 forward.
+hipError_t launch_fft_lines(cx<double>* data, int n, int inner, int nlines, const cx<double>* tw,
+                            int sign, double scale, hipStream_t s);
 
 // GEMM launchers (pwh_gemm.hip)
 struct BetaGen {

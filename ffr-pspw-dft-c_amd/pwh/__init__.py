@@ -40,7 +40,7 @@ EXPORTS = [
     "pwh_residual_norms", "pwh_chfsi", "pwh_resident_set", "pwh_resident_get",
     "pwh_resident_apply", "pwh_resident_chebyfilt", "pwh_synchronize", "pwh_timing_enable",
     "pwh_timing_reset", "pwh_timing_get", "pwh_batch_create", "pwh_batch_destroy",
-    "pwh_batch_set", "pwh_batch_get", "pwh_batch_chebyfilt",
+    "pwh_batch_set", "pwh_batch_get", "pwh_batch_chebyfilt", "pwh_fft3d", "pwh_eig_zheevd",
 ]
 
 
@@ -121,6 +121,8 @@ def lib():
         "pwh_batch_set": [vp, C.c_int, vp],
         "pwh_batch_get": [vp, C.c_int, vp],
         "pwh_batch_chebyfilt": [vp, C.c_int, _dp, _dp, C.c_int, C.c_int],
+        "pwh_fft3d": [vp, C.c_int, C.c_int, C.c_int, C.c_int],
+        "pwh_eig_zheevd": [vp, C.c_int, _dp, C.c_int],
     }
     for name, args in proto.items():
         f = getattr(L, name)
@@ -223,6 +225,30 @@ def ortho_qr(X):
     X = np.array(_block(X), order="C")
     _check(lib().pwh_ortho_qr(_zptr(X), X.shape[1], X.shape[0]))
     return X
+
+
+def fft3d(data, inverse):
+    """fft_fftw3.cpp:13-46 on a complex grid shaped (nr3, nr2, nr1) (x fastest); returns a copy.
+
+    inverse: exp(+i), unscaled; forward: exp(-i) times 1/(nr1 nr2 nr3)."""
+    d = np.array(data, dtype=np.complex128, order="C")
+    assert d.ndim == 3
+    nr3, nr2, nr1 = d.shape
+    _check(lib().pwh_fft3d(_zptr(d), nr1, nr2, nr3, 1 if inverse else 0))
+    return d
+
+
+def eig_zheevd(A, ldA=None):
+    """eig_zheevd.cpp:5-34 ('V','U') on the column-major n x n matrix A (numpy (n, n), entry
+    [i, j] = A(i, j)); only the upper triangle is read.  Returns (lambda, eigenvectors)."""
+    A = np.asarray(A, dtype=np.complex128)
+    n = A.shape[0]
+    ld = n if ldA is None else ldA
+    buf = np.zeros((n, ld), np.complex128)      # column j at buf[j, :ld]
+    buf[:, :n] = A.T
+    lam = np.zeros(n)
+    _check(lib().pwh_eig_zheevd(_zptr(buf), ld, lam, n))
+    return lam, np.ascontiguousarray(buf[:, :n].T)
 
 
 def synchronize():

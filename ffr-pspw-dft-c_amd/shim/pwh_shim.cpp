@@ -6,13 +6,20 @@
 //   ApplyHam        ApplyHam.cpp:5-142         -> pwh_apply_ham_block, nstates = 1
 //   chebyfilt       chebyfilt.cpp:6-74         -> pwh_chebyfilt on NX+2 bands
 //   ortho_qr        ortho_qr.cpp:3-34          -> pwh_ortho_qr
+//   lanczos         lanczos.cpp:8-101          -> pwh_lanczos
+//   eig_zheevd      eig_zheevd.cpp:5-34        -> pwh_eig_zheevd
+//   fft_fftw3       fft_fftw3.cpp:13-46        -> pwh_fft3d
 //
-// Link it in place of those four .cpp files.  It reads the reference's globals
+// Link it in place of those .cpp files.  It reads the reference's globals
 // (global_variables.cpp) and keeps one device system per k-point, rebuilt when the
 // global arrays it was built from change.  Error behaviour follows the reference:
 // the conditions it aborts on (nnr != NNR, lm_end > 4) abort here too, and
 // factorisation failures in ortho_qr print and return.  The filter precision of
 // chebyfilt comes from PWH_PRECISION=fp64|fp32|bf16x3 (default fp64).
+//
+// lanczos clears T[0 .. k*k) and fills T as the reference does; eig_zheevd prints
+// "ERROR calling zheevd_ : info = %d" and returns on a failed factorisation, as the
+// reference does after zheevd_.
 //
 // Compiled as gnu++98 so `double _Complex` is the reference's `double complex`.
 #include <cstdio>
@@ -125,4 +132,26 @@ void ortho_qr(zcomplex *X, int nbasis, int nstates) {
     printf("ERROR in ortho_qr : %s\n", pwh_last_error());
     return;
   }
+}
+
+void lanczos(int ik, zcomplex *v0, int k, double *T, int ldT, zcomplex *f, double *rhoe, int nnr) {
+  check_nnr(nnr);
+  pwh_system *s = system_for(ik);
+  if (pwh_set_vloc(s, rhoe, nnr) != PWH_OK) die("lanczos");
+  if (pwh_lanczos(s, reinterpret_cast<double *>(v0), k, T, ldT, reinterpret_cast<double *>(f)) != PWH_OK)
+    die("lanczos");
+}
+
+void eig_zheevd(zcomplex *A, int ldA, double *lambda, int N) {
+  if (pwh_eig_zheevd(reinterpret_cast<double *>(A), ldA, lambda, N) != PWH_OK) {
+    const char *e = pwh_last_error();
+    const char *p = strstr(e, "info ");
+    if (p) printf("ERROR calling zheevd_ : info = %d\n", atoi(p + 5));
+    else die("eig_zheevd");
+  }
+}
+
+void fft_fftw3(zcomplex *data, int NR1_, int NR2_, int NR3_, bool inverse) {
+  if (pwh_fft3d(reinterpret_cast<double *>(data), NR1_, NR2_, NR3_, inverse ? 1 : 0) != PWH_OK)
+    die("fft_fftw3");
 }

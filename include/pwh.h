@@ -109,6 +109,17 @@ int pwh_rayleigh_ritz(pwh_system *sys, double *X, int nstates, double *eig);
 /* Residual report, ChebySCFLoop.cpp:415-433: res[j] = || H x_j - X (X^H H X)_sym e_j ||. */
 int pwh_residual_norms(pwh_system *sys, const double *X, int nstates, double *res);
 
+/* ---- adjacent dense routines (no system needed) ------------------------------ */
+/* fft_fftw3.cpp:13-46: in-place 3D DFT of data [nr3][nr2][nr1] complex (x fastest).
+ * inverse != 0: exp(+i G.r), unscaled (FFTW_BACKWARD); inverse == 0: exp(-i G.r) then
+ * 1/(nr1 nr2 nr3) (FFTW_FORWARD + zdscal).  FP64 on the GPU; each length must have a
+ * compiled plan (2,3,5-smooth, 12..128), else PWH_ERR_UNSUPPORTED. */
+int pwh_fft3d(double *data, int nr1, int nr2, int nr3, int inverse);
+/* eig_zheevd.cpp:5-34: Hermitian eigensolve of A [ldA x n] column-major, upper triangle
+ * referenced ('V','U'); A <- eigenvectors, lambda [n] ascending.  FP64 on the GPU
+ * (rocSOLVER); info != 0 -> PWH_ERR_LAPACK. */
+int pwh_eig_zheevd(double *A, int ldA, double *lambda, int n);
+
 /* ---- ChFSI outer loop (eigen part of ChebySCFLoop.cpp:271-311) -------------- */
 typedef struct {
   int degree;            /* CHEBYPOL_DEGREE */
@@ -155,6 +166,8 @@ int pwh_batch_chebyfilt(pwh_batch *batch, int degree, const double *lb, const do
                         int precision, int mode);
 
 /* ---- per-kernel timing with HIP events on the library stream ---------------- */
+/* While timing is on, H|psi> runs every kernel on the library stream (no overlap of the
+ * nonlocal GEMMs with the FFT passes), so each event pair times one kernel alone. */
 int pwh_timing_enable(int on);
 int pwh_timing_reset(void);
 /* name: "zinv", "plane", "zfwd", "proj", "expand", "hpsi". total_ms / count of launches. */

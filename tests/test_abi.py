@@ -14,8 +14,11 @@ from conftest import PKG, ROOT
 HEADER = os.path.join(ROOT, "include", "pwh.h")
 
 # Itanium-mangled names of the reference's hot-path routines (common_pspw.h:176-178,187,222)
+# and the adjacent ones the shim also replaces (lanczos :185-186, eig_zheevd :217,
+# fft_fftw3 :348), as g++ -std=gnu++98 mangles the reference's declarations
 REF_SYMBOLS = ["_Z14ApplyHam_blockPCdS0_iiPdii", "_Z8ApplyHamPCdS0_iiPdi",
-               "_Z9chebyfiltiPCdiddPdi", "_Z8ortho_qrPCdii"]
+               "_Z9chebyfiltiPCdiddPdi", "_Z8ortho_qrPCdii", "_Z7lanczosiPCdiPdiS0_S1_i",
+               "_Z10eig_zheevdPCdiPdi", "_Z9fft_fftw3PCdiiib"]
 
 
 def declared_functions():

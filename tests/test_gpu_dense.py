@@ -1,0 +1,131 @@
+"""GPU parity of the dense routines next to the hot path that the shim also replaces:
+fft_fftw3 (pwh_fft3d) and eig_zheevd (pwh_eig_zheevd), called through the C ABI and
+through the reference's mangled C++ symbols.
+
+Tolerances: the FFT within 1e-13 (norm-relative) of numpy's FFT and of the oracle's
+mixed-radix DFT, with the fft_fftw3.cpp:27-38 conventions (inverse exp(+i) unscaled,
+forward exp(-i) times 1/N, x fastest); eigenvalues within 1e-12 of the reference's own
+zheevd output (golden fixture) and of numpy, eigenvectors checked through A V = V Lambda
+and V^H V = I to 1e-12 (their phases are a LAPACK implementation choice).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from conftest import PKG, ROOT, relerr
+
+pytestmark = pytest.mark.gpu
+
+SYM_FFT = "_Z9fft_fftw3PCdiiib"
+SYM_EIG = "_Z10eig_zheevdPCdiPdi"
+
+
+def _grid(shape, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.mark.parametrize("nr", [(12, 16, 20), (32, 32, 32), (24, 30, 36), (64, 64, 64), (96, 96, 96),
+                                (128, 20, 12)])
+def test_fft3d_matches_numpy(nr):
+    import pwh
+    pwh.init(0)
+    x = _grid((nr[2], nr[1], nr[0]), seed=sum(nr))
+    n = x.size
+    assert relerr(pwh.fft3d(x, inverse=True), np.fft.ifftn(x) * n) < 1e-13
+    y = pwh.fft3d(x, inverse=False)
+    assert relerr(y, np.fft.fftn(x) / n) < 1e-13
+    assert relerr(pwh.fft3d(y, inverse=True), x) < 1e-13
+
+
+def test_fft3d_matches_oracle(oracle):
+    import pwh
+    nr = (20, 12, 16)
+    x = _grid((nr[2], nr[1], nr[0]), seed=5)
+    for inv in (True, False):
+        ref = oracle.fft3d(x.reshape(-1), nr, inv).reshape(x.shape)
+        assert relerr(pwh.fft3d(x, inverse=inv), ref) < 1e-13
+
+
+def test_fft3d_plane_wave_closed_form():
+    """exp(+i 2pi (g.r)) on the grid: the forward transform is a unit spike at g."""
+    import pwh
+    nr = (24, 30, 36)
+    g = (5, -7, 11)
+    i3, i2, i1 = np.meshgrid(np.arange(nr[2]), np.arange(nr[1]), np.arange(nr[0]), indexing="ij")
+    ph = 2 * np.pi * (g[0] * i1 / nr[0] + g[1] * i2 / nr[1] + g[2] * i3 / nr[2])
+    y = pwh.fft3d(np.exp(1j * ph), inverse=False)
+    spike = np.zeros_like(y)
+    spike[g[2] % nr[2], g[1] % nr[1], g[0] % nr[0]] = 1.0
+    assert np.abs(y - spike).max() < 1e-13
+
+
+def test_fft3d_unsupported_length():
+    import pwh
+    with pytest.raises(pwh.PwhError) as e:
+        pwh.fft3d(np.zeros((22, 22, 22), np.complex128), inverse=True)
+    assert e.value.code == pwh.PWH_ERR_UNSUPPORTED
+
+
+def _check_eig(A, lam, V, tol=1e-12):
+    s = max(1.0, np.abs(A).max())
+    assert np.abs(A @ V - V * lam[None, :]).max() < tol * s * A.shape[0]
+    assert np.abs(V.conj().T @ V - np.eye(A.shape[0])).max() < tol * A.shape[0]
+    assert np.all(np.diff(lam) >= 0)
+
+
+def test_eig_zheevd_reference_fixtures():
+    import pwh
+    pwh.init(0)
+    z = np.load(os.path.join(ROOT, "tests", "golden", "ref_lapack.npz"))
+    for A, w in ((z["T"], z["w"]), (z["A4"], z["w4"])):
+        lam, V = pwh.eig_zheevd(A)
+        assert np.abs(lam - w).max() < 1e-12
+        H = np.triu(A) + np.triu(A, 1).conj().T
+        _check_eig(H, lam, V)
+
+
+@pytest.mark.parametrize("n,ld", [(128, 128), (97, 110), (432, 432)])
+def test_eig_zheevd_random_reads_upper_triangle(n, ld):
+    import pwh
+    rng = np.random.default_rng(n)
+    B = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    H = 0.5 * (B + B.conj().T)
+    A = np.triu(H) + np.tril(rng.standard_normal((n, n)), -1)   # garbage below the diagonal
+    lam, V = pwh.eig_zheevd(A, ldA=ld)
+    assert np.abs(lam - np.linalg.eigvalsh(H)).max() < 1e-12 * np.abs(H).max() * np.sqrt(n)
+    _check_eig(H, lam, V, tol=1e-13)
+
+
+@pytest.fixture(scope="module")
+def shim_lib():
+    C.CDLL(os.path.join(PKG, "libpwh_refglobals.so"), mode=C.RTLD_GLOBAL)
+    S = C.CDLL(os.path.join(PKG, "libpwh_shim.so"))
+    vp, ci = C.c_void_p, C.c_int
+    getattr(S, SYM_FFT).argtypes = [vp, ci, ci, ci, C.c_bool]
+    getattr(S, SYM_EIG).argtypes = [vp, ci, vp, ci]
+    return S
+
+
+def test_shim_fft_fftw3(shim_lib):
+    nr = (32, 24, 20)
+    x = _grid((nr[2], nr[1], nr[0]), seed=8)
+    d = x.copy()
+    getattr(shim_lib, SYM_FFT)(d.ctypes.data, nr[0], nr[1], nr[2], True)
+    assert relerr(d, np.fft.ifftn(x) * x.size) < 1e-13
+    getattr(shim_lib, SYM_FFT)(d.ctypes.data, nr[0], nr[1], nr[2], False)
+    assert relerr(d, x) < 1e-13
+
+
+def test_shim_eig_zheevd(shim_lib):
+    z = np.load(os.path.join(ROOT, "tests", "golden", "ref_lapack.npz"))
+    A = z["T"]
+    n = A.shape[0]
+    buf = np.ascontiguousarray(A.T)          # column-major, ldA = n
+    lam = np.zeros(n)
+    getattr(shim_lib, SYM_EIG)(buf.ctypes.data, n, lam.ctypes.data, n)
+    assert np.abs(lam - z["w"]).max() < 1e-12
+    V = buf.T
+    _check_eig(np.triu(A) + np.triu(A, 1).conj().T, lam, V)

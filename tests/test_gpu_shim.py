@@ -1,7 +1,8 @@
 """The drop-in shim called exactly as the reference's drivers call it: the C++-mangled
-ApplyHam_block / ApplyHam / chebyfilt / ortho_qr, reading the reference globals
-(set here through libpwh_refglobals.so).  Parity with the FP64 oracle: 1e-12 for
-ApplyHam_block and ApplyHam, 1e-11 for chebyfilt."""
+ApplyHam_block / ApplyHam / chebyfilt / ortho_qr / lanczos, reading the reference
+globals (set here through libpwh_refglobals.so).  Parity with the FP64 oracle: 1e-12
+for ApplyHam_block and ApplyHam, 1e-11 for chebyfilt, 1e-10 (absolute, Ha) for the
+Lanczos T and 1e-8 for its residual vector."""
 import ctypes as C
 import os
 
@@ -16,6 +17,7 @@ SYM_BLOCK = "_Z14ApplyHam_blockPCdS0_iiPdii"
 SYM_ONE = "_Z8ApplyHamPCdS0_iiPdi"
 SYM_CHEB = "_Z9chebyfiltiPCdiddPdi"
 SYM_QR = "_Z8ortho_qrPCdii"
+SYM_LANCZOS = "_Z7lanczosiPCdiPdiS0_S1_i"
 
 
 @pytest.fixture(scope="module")
@@ -41,6 +43,7 @@ def shim(si8):
     getattr(S, SYM_ONE).argtypes = [vp, vp, ci, ci, vp, ci]
     getattr(S, SYM_CHEB).argtypes = [ci, vp, ci, cd, cd, vp, ci]
     getattr(S, SYM_QR).argtypes = [vp, ci, ci]
+    getattr(S, SYM_LANCZOS).argtypes = [ci, vp, ci, vp, ci, vp, vp, ci]
     yield S, keep
     del keep
 
@@ -77,3 +80,22 @@ def test_shim_ortho_qr(si8, shim):
     assert np.abs(X.conj() @ X.T - np.eye(16)).max() < 1e-13
     Qr, _ = np.linalg.qr(X0.T)
     assert np.abs(X.T @ X.conj() - Qr @ Qr.conj().T).max() < 1e-12
+
+
+def test_shim_lanczos(si8, shim, oracle):
+    """lanczos.cpp:8-101 through its mangled symbol: v0 normalised in place, T[0..k*k)
+    cleared then filled (upper + subdiagonal), f the final residual."""
+    S, _ = shim
+    rng = np.random.default_rng(34)
+    v0 = rng.random(si8.ngw) + 1j * rng.random(si8.ngw)
+    k = 12
+    Tr, fr, _ = oracle.lanczos(oracle_globals(si8), v0, k, si8.vloc, si8.ngw)
+    v = np.ascontiguousarray(v0.copy())
+    T = np.full(k * k, 7.0)                     # every entry is overwritten or cleared
+    f = np.zeros(si8.ngw, np.complex128)
+    rho = np.ascontiguousarray(si8.vloc)
+    getattr(S, SYM_LANCZOS)(1, v.ctypes.data, k, T.ctypes.data, k, f.ctypes.data, rho.ctypes.data,
+                            rho.size)
+    assert np.abs(T.reshape(k, k).T - Tr).max() < 1e-10
+    assert relerr(f, fr) < 1e-8
+    assert abs(np.linalg.norm(v) - 1.0) < 1e-14
