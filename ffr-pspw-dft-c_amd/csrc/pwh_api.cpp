@@ -51,6 +51,7 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 enum TimerId { TM_ZINV, TM_PLANE, TM_ZFWD, TM_PROJ, TM_EXPAND, TM_HPSI, TM_COUNT };
+constexpr size_t kRocblasWorkspace = size_t(128) << 20;   // bytes
 const char* kTimerNames[TM_COUNT] = {"zinv", "plane", "zfwd", "proj", "expand", "hpsi"};
 
 struct Ctx {
@@ -64,7 +65,7 @@ struct Ctx {
   std::vector<hipEvent_t> pool;
   double total_ms[TM_COUNT] = {0};
   int count[TM_COUNT] = {0};
-  // workspace for operations without a system (ortho_qr)
+  // fixed device workspace handed to the rocBLAS handle
   void* ws = nullptr;
   size_t ws_bytes = 0;
 };
@@ -481,17 +482,44 @@ int sys_cholqr2(pwh_system* s, cx<double>* X, int nb) {
 
 // FP64 Hermitian eigensolve on the GPU, eig_zheevd.cpp:5-34 ('V', 'U'): A (n x n,
 // column-major) is replaced by its eigenvectors, w gets the eigenvalues ascending.
-// PWH_EIGSOLVER selects rocSOLVER's divide and conquer (evd, default), QR iteration (ev),
-// Jacobi (evj) or Jacobi with divide and conquer (evdj).  work: n doubles + 2 ints of device scratch.
-int hermitian_eig(cx<double>* A, int n, double* w, double* work, rocblas_int* info) {
-  const char* e = getenv("PWH_EIGSOLVER");
+// Default: the one-workgroup solver of pwh_eig.hip for n <= 128 (Householder tridiagonal
+// reduction, multisection, inverse iteration, back-transformation), rocSOLVER's divide and
+// conquer above.  PWH_EIGSOLVER=evd|ev|evj|evdj forces rocSOLVER's divide and conquer, QR
+// iteration, Jacobi or Jacobi with divide and conquer.  If the small solver meets a
+// non-finite value, the saved input is solved again by rocSOLVER (zheevd).
+// work: n doubles; info: 2 ints of device scratch.  Synchronises the library stream when
+// the small solver ran (to read its flag).
+int hermitian_eig_impl(cx<double>* A, int n, double* w, double* work, rocblas_int* info) {
+  const int mode = [] {
+    const char* e = getenv("PWH_EIGSOLVER");   // read per call: a caller may switch solvers
+    if (!e || !strcmp(e, "own")) return 0;
+    if (!strcmp(e, "evj")) return 2;
+    if (!strcmp(e, "ev")) return 3;
+    if (!strcmp(e, "evdj")) return 4;
+    return 1;
+  }();
   auto* a = reinterpret_cast<rocblas_double_complex*>(A);
-  if (e && !strcmp(e, "evj")) {
+  if (mode == 0 && n <= kHeevSmallMax) {
+    static Buf<double>* ws = new Buf<double>();      // kept for the life of the process
+    static Buf<cx<double>>* save = new Buf<cx<double>>();
+    CHK(ws->ensure(heev_small_scratch_doubles()));
+    CHK(save->ensure((size_t)n * n));
+    HIPCHK(hipMemcpyAsync(save->p, A, sizeof(cx<double>) * n * n, hipMemcpyDeviceToDevice, g.stream));
+    HIPCHK(launch_heev_small(A, n, n, w, reinterpret_cast<int*>(info), ws->p, g.stream));
+    rocblas_int flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, info, sizeof(flag), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (flag == 0) return PWH_OK;
+    HIPCHK(hipMemcpyAsync(A, save->p, sizeof(cx<double>) * n * n, hipMemcpyDeviceToDevice, g.stream));
+    RBCHK(rocsolver_zheevd(g.rb, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, work, info));
+    return PWH_OK;
+  }
+  if (mode == 2) {
     RBCHK(rocsolver_zheevj(g.rb, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, n,
                            a, n, 0.0, work, 100, info + 1, w, info));
-  } else if (e && !strcmp(e, "ev")) {
+  } else if (mode == 3) {
     RBCHK(rocsolver_zheev(g.rb, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, work, info));
-  } else if (e && !strcmp(e, "evdj")) {
+  } else if (mode == 4) {
     RBCHK(rocsolver_zheevdj(g.rb, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, info));
   } else {
     RBCHK(rocsolver_zheevd(g.rb, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, work, info));
@@ -515,7 +543,7 @@ int rayleigh_ritz_dev(pwh_system* s, cx<double>* X, int nb) {
   CHK(hpsi<double>(s, X, s->HX.p, nb, 1.0, 0.0, 0.0, nullptr, PREC_FP64));
   HIPCHK(gemm_cn_f64(X, ld, nb, s->HX.p, ld, nb, s->ngw, s->S.p, nb, s->part64.p, pe, g.stream));
   HIPCHK(symmetrize(s->S.p, nb, nb, g.stream));
-  CHK(hermitian_eig(s->S.p, nb, s->evals.p, s->rwork.p, s->info.p));
+  CHK(hermitian_eig_impl(s->S.p, nb, s->evals.p, s->rwork.p, s->info.p));
   rocblas_int hinfo = 0;
   HIPCHK(hipMemcpyAsync(&hinfo, s->info.p, sizeof(hinfo), hipMemcpyDeviceToHost, g.stream));
   HIPCHK(hipStreamSynchronize(g.stream));
@@ -703,6 +731,12 @@ int pwh_init(int device) {
   if (!g.rb) {
     RBCHK(rocblas_create_handle(&g.rb));
     RBCHK(rocblas_set_stream(g.rb, g.stream));
+    // one fixed device workspace for rocBLAS/rocSOLVER (zpotrf, ztrtri, zheevd up to
+    // n = 432 need a few MiB): without it the handle re-allocates whenever consecutive
+    // calls ask for different sizes, each time with a device synchronisation
+    HIPCHK(hipMalloc(&g.ws, kRocblasWorkspace));
+    g.ws_bytes = kRocblasWorkspace;
+    RBCHK(rocblas_set_workspace(g.rb, g.ws, g.ws_bytes));
   }
   g.device = device;
   g.ready = true;
@@ -978,7 +1012,7 @@ int pwh_eig_zheevd(double* A, int ldA, double* lambda, int n) {
   CHK(w.eig_info.ensure(2));
   HIPCHK(hipMemcpy2DAsync(w.eig_a.p, sizeof(cx<double>) * n, A, sizeof(cx<double>) * ldA,
                           sizeof(cx<double>) * n, n, hipMemcpyHostToDevice, g.stream));
-  CHK(hermitian_eig(w.eig_a.p, n, w.eig_w.p, w.eig_work.p, w.eig_info.p));
+  CHK(hermitian_eig_impl(w.eig_a.p, n, w.eig_w.p, w.eig_work.p, w.eig_info.p));
   rocblas_int info = 0;
   HIPCHK(hipMemcpyAsync(&info, w.eig_info.p, sizeof(info), hipMemcpyDeviceToHost, g.stream));
   HIPCHK(hipStreamSynchronize(g.stream));

@@ -77,6 +77,12 @@ hipError_t launch_zfwd_epilogue(const SphereMaps& m, const void* tw3, const cx<T
                                 hipStream_t s);
 bool fft_size_supported(int n);
 bool plane_supported(int n1, int n2);
+// One-workgroup FP64 Hermitian eigensolver for n <= 128 (pwh_eig.hip): A (column-major,
+// lda, upper triangle read) -> eigenvectors, w ascending; info[0] != 0 on a non-finite value.
+size_t heev_small_scratch_doubles();
+constexpr int kHeevSmallMax = 128;
+hipError_t launch_heev_small(cx<double>* A, int lda, int n, double* w, int* info, double* scratch,
+                             hipStream_t s);
 // Dense FP64 transform of length n along an axis of stride `inner` for every line of an
 // [outer][n][inner] array, in place; tw[m] = exp(+2 pi i m / n); sign +1 inverse, -1 forward.
 hipError_t launch_fft_lines(cx<double>* data, int n, int inner, int nlines, const cx<double>* tw,

@@ -129,3 +129,42 @@ def test_shim_eig_zheevd(shim_lib):
     assert np.abs(lam - z["w"]).max() < 1e-12
     V = buf.T
     _check_eig(np.triu(A) + np.triu(A, 1).conj().T, lam, V)
+
+
+def _unitary(n, seed):
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    return Q
+
+
+@pytest.mark.parametrize("case", ["4-fold", "40-fold", "near-diagonal", "diagonal", "wide", "n1", "n2",
+                                  "n33", "zero"])
+def test_eig_zheevd_hard_spectra(case):
+    """The one-workgroup solver (n <= 128): exact and near degeneracies (cluster
+    Gram-Schmidt), diagonal input (zero off-diagonals), tiny and odd sizes, a zero matrix,
+    and a spread like the Rayleigh-Ritz matrices (-0.3 .. 60 Ha)."""
+    import pwh
+    n = {"n1": 1, "n2": 2, "n33": 33}.get(case, 128)
+    Q = _unitary(n, 7)
+    rng = np.random.default_rng(3)
+    if case == "4-fold":
+        lam = np.repeat(np.linspace(-1, 2, 32), 4) + 1e-15 * rng.standard_normal(n)
+    elif case == "40-fold":
+        lam = np.sort(np.concatenate([np.full(40, 0.3), np.linspace(1, 5, 88)]))
+    elif case == "wide":
+        lam = np.sort(np.concatenate([np.linspace(-0.3, 0.5, 100), rng.uniform(1, 60, 28)]))
+    elif case == "zero":
+        lam = np.zeros(n)
+    else:
+        lam = np.sort(rng.standard_normal(n))
+    if case == "diagonal":
+        H = np.diag(lam).astype(np.complex128)
+    elif case == "near-diagonal":
+        H = np.diag(np.linspace(-2, 3, n)) + 1e-9 * (Q + Q.conj().T)
+    else:
+        H = (Q * lam) @ Q.conj().T
+    H = 0.5 * (H + H.conj().T)
+    got, V = pwh.eig_zheevd(H)
+    s = max(np.abs(H).max(), 1.0)
+    assert np.abs(got - np.linalg.eigvalsh(H)).max() < 1e-12 * s
+    _check_eig(H, got, V, tol=1e-13)
