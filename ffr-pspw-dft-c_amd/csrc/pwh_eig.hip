@@ -1,23 +1,25 @@
-// pwh_eig.hip -- FP64 Hermitian eigensolver for the Rayleigh-Ritz matrices (n <= 128) in
-// one workgroup: the job of eig_zheevd.cpp:5-34 (zheevd 'V','U') without the hundreds
-// of small launches a library divide-and-conquer makes at this size.
+// pwh_eig.hip -- FP64 Hermitian eigensolver for the Rayleigh-Ritz matrices (n <= 128):
+// the job of eig_zheevd.cpp:5-34 (zheevd 'V','U') in five launches instead of the hundreds
+// of small ones a library divide-and-conquer makes at this size.
 //
-// Three kernels of one 512-lane workgroup each (phases 1, 2-3, 4):
-//   1. Householder reduction to real tridiagonal form, A = Q T Q^H with
-//      Q = H_0 H_1 ... H_{n-2}, H_k = I - tau_k v_k v_k^H (the zhetd2 'L' recurrence:
-//      p = tau A v, w = p - tau/2 (p^H v) v, A -= v w^H + w v^H).  The matrix lives in
-//      registers: lane (tr, tc) = (tid / 32, tid % 32) holds rows tr + 16a and columns
-//      tc + 32b (32 entries), so the row sums of A v are half-wave shuffles.
-//   2. Eigenvalues of T by multisection on Sturm counts: 4 lanes per eigenvalue test 4
-//      points of its bracket per round (the bracket shrinks 5x per round).
-//   3. Eigenvectors of T by inverse iteration (T - lambda I = P L U with partial
-//      pivoting, three solves), all eigenvalues at once; then every cluster (a run of
-//      eigenvalues with gaps below 1e-6 ||T||) is orthonormalised member by member
-//      (modified Gram-Schmidt, two passes, a half-wave per member), and a Loewdin step
-//      Z <- Z - (Z Z^T - I) Z / 2 removes the ~eps ||T|| / gap overlaps that remain.
-//   4. Back-transformation Y = Q Z, registers again: lane (r, c) = (tid % 32, tid / 32)
-//      holds rows r + 32a and columns c + 16b, so a half-wave owns whole columns and
-//      needs no barrier.
+//   1. k_heev_tridiag (1 workgroup): Householder reduction to real tridiagonal form,
+//      A = Q T Q^H, Q = H_0 ... H_{n-2}, H_k = I - tau_k v_k v_k^H (the zhetd2 'L'
+//      recurrence: p = tau A v, w = p - tau/2 (p^H v) v, A -= v w^H + w v^H).  The matrix
+//      lives in registers: lane (tr, tc) = (tid / 32, tid % 32) holds rows tr + 16a and
+//      columns tc + 32b, so the row sums of A v are half-wave shuffles; blocks of the
+//      finished leading rows and columns are skipped.
+//   2. k_heev_bisect (n / 16 workgroups): eigenvalues of T by multisection on Sturm counts,
+//      a half-wave per eigenvalue testing 32 points of its bracket per round (the bracket
+//      shrinks 33x per round).
+//   3. k_heev_invit (n / 16 workgroups): eigenvectors of T by inverse iteration
+//      (T - lambda I = P L U with partial pivoting, three solves), a lane per eigenvalue,
+//      factors in LDS.
+//   4. k_heev_orth (1 workgroup): every multiplet (a run of eigenvalues with gaps below
+//      1e-6 ||T||) orthonormalised member by member (modified Gram-Schmidt, two passes, a
+//      half-wave per member), then a Loewdin step Z <- Z - (Z Z^T - I) Z / 2 removes the
+//      ~eps ||T|| / gap overlaps left between the other vectors.
+//   5. k_heev_backtr (n / 16 workgroups): Y = Q Z, a half-wave per column, Householder
+//      vectors staged in LDS.
 #include <cfloat>
 
 #include "pwh_internal.h"
@@ -29,7 +31,8 @@ constexpr int kEigT = 512;                  // lanes (2 waves per SIMD: up to 25
 constexpr int kEigG = kEigT / 32;           // half-wave groups (16)
 constexpr int kEigRA = kEigN / kEigG;       // row blocks of a lane in phase 1 (8)
 constexpr int kEigCB = kEigN / 32;          // column blocks of a lane in phase 1 (4)
-constexpr int kEigSamples = kEigT / kEigN;  // multisection points per eigenvalue per round (4)
+constexpr int kBisS = 32;                   // multisection points per eigenvalue per round
+constexpr int kInvE = 16;                   // eigenvalues per inverse-iteration workgroup
 
 struct EigScratch {
   double* V;     // [kEigN][kEigN] complex, V[k*kEigN + i] = v_k[i]
@@ -39,6 +42,8 @@ struct EigScratch {
   double* tau;   // [kEigN] complex
   double* d;     // [kEigN] diagonal of T
   double* e;     // [kEigN] off-diagonal of T
+  double* lam;   // [kEigN] eigenvalues of the scaled T
+  double* st;    // [4] scale applied to T, ||T|| (scaled), Gershgorin low, high
 };
 
 __device__ __forceinline__ cx<double> cmul(cx<double> a, cx<double> b) {
@@ -79,6 +84,23 @@ __device__ __forceinline__ cx<double> reduce_scatter8(const cx<double> (&v)[8], 
   row = 4 * (int)h4 + 2 * (int)h3 + (int)h2;
   return t;
 }
+// Cross-lane move of a double through DPP (CTRL: quad_perm / row_ror patterns, every
+// source lane valid).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+// sum over the 32 lanes of a half-wave: quads (xor 1, xor 2), rows of 16 (rotate 4, 8),
+// then the two rows (one permute)
+__device__ __forceinline__ double halfwave_sum_dpp(double v) {
+  v += dpp_f64<0xB1>(v);    // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4E>(v);    // quad_perm [2,3,0,1]
+  v += dpp_f64<0x124>(v);   // row_ror:4
+  v += dpp_f64<0x128>(v);   // row_ror:8
+  return v + __shfl_xor(v, 16);
+}
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
@@ -96,28 +118,16 @@ __device__ __forceinline__ void lds_barrier() {
 // packed offset of v_k[k+1] in the lower triangle of V (v_k has n - 1 - k entries)
 __device__ __forceinline__ int vpack_off(int k, int n) { return k * (n - 1) - (k * (k - 1)) / 2; }
 
-// number of eigenvalues of T (d, e2 = e^2) below x
-__device__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
-  double q = d[0] - x;
-  if (fabs(q) < pivmin) q = -pivmin;
-  int c = q < 0.0 ? 1 : 0;
-  for (int j = 1; j < n; ++j) {
-    q = d[j] - x - e2[j - 1] / q;
-    if (fabs(q) < pivmin) q = -pivmin;
-    c += q < 0.0 ? 1 : 0;
-  }
-  return c;
-}
-
 // Phase 1.  A: n x n column-major (lda), upper triangle read (Hermitian completion).
 __global__ __launch_bounds__(kEigT) void k_heev_tridiag(const cx<double>* __restrict__ A, int lda, int n,
-                                                        EigScratch sc) {
+                                                        int* __restrict__ info, EigScratch sc) {
   __shared__ cx<double> vx[2][kEigN];   // x, then v, of the current step (double-buffered)
   __shared__ cx<double> pv[kEigN];      // p = tau A v
   __shared__ cx<double> gpart[kEigG];   // per-row-group partial sums of p^H v
   __shared__ cx<double> tau_s[kEigN];
   const int tid = threadIdx.x;
   const int tr = tid >> 5, tc = tid & 31, lane = tid & 63;
+  if (tid == 0) info[0] = 0;
 
   // ---- phase 1: load (upper triangle, Hermitian completion), tridiagonalise ----------
   cx<double> a[kEigRA][kEigCB];
@@ -199,8 +209,11 @@ __global__ __launch_bounds__(kEigT) void k_heev_tridiag(const cx<double>* __rest
 #pragma unroll
       for (int ia = 0; ia < kEigRA; ++ia) {
         cx<double> sum = mk<double>(0.0, 0.0);
+        if (kEigG * ia + kEigG - 1 > k) {          // block has rows of the trailing matrix
 #pragma unroll
-        for (int jb = 0; jb < kEigCB; ++jb) sum = sum + cmul(a[ia][jb], vj[jb]);
+          for (int jb = 0; jb < kEigCB; ++jb)
+            if (32 * jb + 31 > k) sum = sum + cmul(a[ia][jb], vj[jb]);
+        }
         rs[ia] = sum;
       }
       int ra;
@@ -231,12 +244,14 @@ __global__ __launch_bounds__(kEigT) void k_heev_tridiag(const cx<double>* __rest
       }
 #pragma unroll
       for (int ia = 0; ia < kEigRA; ++ia) {
+        if (kEigG * ia + kEigG - 1 <= k) continue;  // finished rows (uniform branch)
         const int i = tr + kEigG * ia;
         const bool in = i > k && i < n;
         const cx<double> vi = in ? vx[cur][i] : mk<double>(0.0, 0.0);
         const cx<double> wi = in ? pv[i] + cmul(gam, vi) : mk<double>(0.0, 0.0);
 #pragma unroll
         for (int jb = 0; jb < kEigCB; ++jb) {
+          if (32 * jb + 31 <= k) continue;           // finished columns
           // v_i conj(w_j) + w_i conj(v_j); zero outside the trailing block
           const cx<double> u = cmulc(wj[jb], vi) + cmulc(vj[jb], wi);
           a[ia][jb] = a[ia][jb] - u;
@@ -255,316 +270,335 @@ __global__ __launch_bounds__(kEigT) void k_heev_tridiag(const cx<double>* __rest
   }
 }
 
-// Phases 2 and 3: eigenvalues w (ascending) and eigenvectors Z of T; info[0] = 1 when a
-// non-finite value appeared.
-__global__ __launch_bounds__(kEigT) void k_heev_trieig(int n, double* __restrict__ w, int* __restrict__ info,
-                                                       EigScratch sc) {
-  __shared__ double dv[kEigN], ev[kEigN], e2v[kEigN], lam[kEigN];
-  __shared__ double scal[5];            // gl, gu, pivmin, tnorm, scale applied to T
-  __shared__ int cstart[kEigN];
-  __shared__ int maxpos, bad;
+// 1 / q for the Sturm and factorisation recurrences: hardware reciprocal + one Newton step
+// (relative error ~1e-16; the recurrences are backward stable, so no exact rounding needed)
+__device__ __forceinline__ double frcp(double q) {
+  const double r = __builtin_amdgcn_rcp(q);
+  return r * __builtin_fma(-q, r, 2.0);
+}
+
+// number of eigenvalues of T (d, e2 = e^2) below x
+__device__ __forceinline__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
+  double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  int c = q < 0.0 ? 1 : 0;
+  for (int j = 1; j < n; ++j) {
+    q = d[j] - x - e2[j - 1] * frcp(q);
+    if (fabs(q) < pivmin) q = -pivmin;
+    c += q < 0.0 ? 1 : 0;
+  }
+  return c;
+}
+
+// T from the tridiagonal reduction into LDS, scaled to unit norm when its norm is far from 1
+// (T = 0 gives tnorm = 0); every workgroup computes the same statistics.
+__device__ void load_tridiag(int n, const EigScratch& sc, double* dv, double* ev, double* stat) {
   const int tid = threadIdx.x, lane = tid & 63;
-  if (tid == 0) { maxpos = 0; bad = 0; }
-  if (tid < n) {
-    dv[tid] = sc.d[tid];
-    ev[tid] = tid + 1 < n ? sc.e[tid] : 0.0;
+  for (int t = tid; t < n; t += blockDim.x) {
+    dv[t] = sc.d[t];
+    ev[t] = t + 1 < n ? sc.e[t] : 0.0;
   }
   __syncthreads();
-
-  // ---- phase 2: eigenvalues of T by multisection --------------------------------------
   if (tid < 64) {
-    double gl = DBL_MAX, gu = -DBL_MAX, emax2 = 0.0, dmax = 0.0, emax = 0.0;
+    double dmax = 0.0, emax = 0.0;
+    for (int j = lane; j < n; j += 64) {
+      dmax = fmax(dmax, fabs(dv[j]));
+      emax = fmax(emax, fabs(ev[j]));
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+      dmax = fmax(dmax, __shfl_xor(dmax, m));
+      emax = fmax(emax, __shfl_xor(emax, m));
+    }
+    const double tn0 = dmax + 2.0 * emax;
+    const double sc_ = (tn0 > 0.0 && (tn0 < 1e-100 || tn0 > 1e100)) ? 1.0 / tn0 : 1.0;
+    if (sc_ != 1.0)
+      for (int j = lane; j < n; j += 64) { dv[j] *= sc_; ev[j] *= sc_; }
+    if (lane == 0) { stat[0] = sc_; stat[1] = tn0 * sc_; }
+  }
+  __syncthreads();
+}
+
+// Phase 2: eigenvalue i = blockIdx.x * 16 + (tid / 32) by a half-wave; lam (scaled units)
+__global__ __launch_bounds__(kEigT) void k_heev_bisect(int n, EigScratch sc) {
+  __shared__ double dv[kEigN], ev[kEigN], e2v[kEigN], stat[4];
+  __shared__ double pivmin_s;
+  const int tid = threadIdx.x, lane = tid & 63;
+  load_tridiag(n, sc, dv, ev, stat);
+  if (tid < 64) {
+    double gl = DBL_MAX, gu = -DBL_MAX, emax2 = 0.0;
     for (int j = lane; j < n; j += 64) {
       const double el = j > 0 ? fabs(ev[j - 1]) : 0.0, er = j + 1 < n ? fabs(ev[j]) : 0.0;
       gl = fmin(gl, dv[j] - el - er);
       gu = fmax(gu, dv[j] + el + er);
-      dmax = fmax(dmax, fabs(dv[j]));
-      if (j + 1 < n) { emax2 = fmax(emax2, ev[j] * ev[j]); emax = fmax(emax, er); e2v[j] = ev[j] * ev[j]; }
+      if (j + 1 < n) { emax2 = fmax(emax2, ev[j] * ev[j]); e2v[j] = ev[j] * ev[j]; }
     }
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) {
       gl = fmin(gl, __shfl_xor(gl, m));
       gu = fmax(gu, __shfl_xor(gu, m));
       emax2 = fmax(emax2, __shfl_xor(emax2, m));
-      dmax = fmax(dmax, __shfl_xor(dmax, m));
-      emax = fmax(emax, __shfl_xor(emax, m));
-    }
-    // T = 0: eigenvalues 0, eigenvectors e_i.  Otherwise T is scaled to unit norm when
-    // its norm is far from 1 (overflow-free inverse iteration), and lambda scaled back.
-    const double tn0 = dmax + 2.0 * emax;
-    const double sc_ = (tn0 > 0.0 && (tn0 < 1e-100 || tn0 > 1e100)) ? 1.0 / tn0 : 1.0;
-    if (sc_ != 1.0) {
-      for (int j = lane; j < n; j += 64) {
-        dv[j] *= sc_;
-        if (j + 1 < n) { ev[j] *= sc_; e2v[j] = ev[j] * ev[j]; }
-      }
-      gl *= sc_; gu *= sc_; emax2 *= sc_ * sc_; dmax *= sc_; emax *= sc_;
     }
     if (lane == 0) {
-      scal[4] = sc_;
       const double tn = fmax(fabs(gl), fabs(gu));
       const double pad = 2.0 * DBL_EPSILON * tn * n + DBL_MIN;
-      scal[0] = gl - pad;
-      scal[1] = gu + pad;
-      scal[2] = fmax(emax2, 1.0) * (DBL_MIN / DBL_EPSILON);
-      scal[3] = dmax + 2.0 * emax;
+      stat[2] = gl - pad;
+      stat[3] = gu + pad;
+      pivmin_s = fmax(emax2, 1.0) * (DBL_MIN / DBL_EPSILON);
+      if (blockIdx.x == 0) { sc.st[0] = stat[0]; sc.st[1] = stat[1]; }
     }
   }
   __syncthreads();
-  {
-    const int i = tid / kEigSamples, s = tid % kEigSamples;
-    if (i < n) {
-      double lo = scal[0], hi = scal[1];
-      const double pivmin = scal[2];
-      for (int round = 0; round < 64; ++round) {
-        if (!(hi - lo > 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + DBL_MIN)) break;
-        const double x = lo + (hi - lo) * (double)(s + 1) / (double)(kEigSamples + 1);
-        const int c = sturm_count(dv, e2v, n, x, pivmin);
-        double nlo = c <= i ? x : lo, nhi = c > i ? x : hi;
+  const int i = blockIdx.x * (kEigT / kBisS) + (tid >> 5), s = tid & 31;
+  if (i >= n) return;
+  const double pivmin = pivmin_s;
+  double lo = stat[2], hi = stat[3];
+  for (int round = 0; round < 40; ++round) {
+    if (!(hi - lo > 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + DBL_MIN)) break;
+    const double x = lo + (hi - lo) * (double)(s + 1) / (double)(kBisS + 1);
+    const int c = sturm_count(dv, e2v, n, x, pivmin);
+    double nlo = c <= i ? x : lo, nhi = c > i ? x : hi;
 #pragma unroll
-        for (int m = 1; m < kEigSamples; m <<= 1) {
-          nlo = fmax(nlo, __shfl_xor(nlo, m));
-          nhi = fmin(nhi, __shfl_xor(nhi, m));
-        }
-        if (nlo == lo && nhi == hi) break;
-        lo = nlo;
-        hi = nhi;
-      }
-      if (s == 0) lam[i] = 0.5 * (lo + hi);
+    for (int m = 16; m >= 1; m >>= 1) {
+      nlo = fmax(nlo, __shfl_xor(nlo, m));
+      nhi = fmin(nhi, __shfl_xor(nhi, m));
     }
+    if (nlo == lo && nhi == hi) break;
+    lo = nlo;
+    hi = nhi;
   }
-  __syncthreads();
-
-  // ---- phase 3: eigenvectors of T by inverse iteration --------------------------------
-  const double tnorm = scal[3];
-  // clusters for Gram-Schmidt: gaps below 1e-6 ||T|| (exact and near multiplets); the
-  // overlaps between vectors of wider gaps, at most ~eps ||T|| / gap, are removed by the
-  // Loewdin step 3c
-  const double ortol = 1e-6 * tnorm, pivtol = DBL_EPSILON * tnorm;
-  if (tnorm == 0.0) {
-    // T = 0 (A = 0): identity eigenvectors
-    if (tid < n)
-      for (int k = 0; k < n; ++k) sc.Z[(size_t)tid * kEigN + k] = k == tid ? 1.0 : 0.0;
-    if (tid < n) w[tid] = 0.0;
-    if (tid == 0) info[0] = 0;
-    return;
-  }
-  // 3a: inverse iteration, one lane per eigenvalue, all eigenvalues at once
-  double* u0 = sc.lu;
-  double* u1 = sc.lu + kEigN * kEigN;
-  double* u2 = sc.lu + 2 * kEigN * kEigN;
-  double* ll = sc.lu + 3 * kEigN * kEigN;
-  double* pv_ = sc.lu + 4 * kEigN * kEigN;
-  double* x = sc.x;
-  if (tid < n) {
-    const int i = tid;
-    const double lmb = lam[i];
-    // T - lambda I = P L U, row interchanges where the subdiagonal dominates; the factors
-    // of lane i live at [k][i] (consecutive lanes, consecutive addresses)
-    double cd = dv[0] - lmb, cs = n > 1 ? ev[0] : 0.0;
-    for (int k = 0; k + 1 < n; ++k) {
-      const double sub = ev[k], nd = dv[k + 1] - lmb, ns = k + 2 < n ? ev[k + 1] : 0.0;
-      const size_t o = (size_t)k * kEigN + i;
-      if (fabs(cd) >= fabs(sub)) {
-        const double m = cd != 0.0 ? sub / cd : 0.0;
-        u0[o] = cd; u1[o] = cs; u2[o] = 0.0; ll[o] = m; pv_[o] = 0.0;
-        cd = nd - m * cs;
-        cs = ns;
-      } else {
-        const double m = cd / sub;
-        u0[o] = sub; u1[o] = nd; u2[o] = ns; ll[o] = m; pv_[o] = 1.0;
-        const double t = cs - m * nd;
-        cs = -m * ns;
-        cd = t;
-      }
-    }
-    u0[(size_t)(n - 1) * kEigN + i] = cd;
-    // start vector b_k = cos(k (1 + 0.37 i) + i / 2), then three solves (T - lambda I) x = b / |b|
-    double nrm = 0.0;
-    for (int k = 0; k < n; ++k) {
-      const double b = cos((double)k * (1.0 + 0.37 * i) + 0.5 * i);
-      x[(size_t)k * kEigN + i] = b;
-      nrm += b * b;
-    }
-    double inv = 1.0 / sqrt(nrm);
-    for (int it = 0; it < 3; ++it) {
-      double yk = x[i] * inv;
-#pragma unroll 4
-      for (int k = 0; k + 1 < n; ++k) {
-        const size_t o = (size_t)k * kEigN + i;
-        double yn = x[o + kEigN] * inv;
-        if (pv_[o] != 0.0) { const double t = yk; yk = yn; yn = t; }
-        x[o] = yk;
-        yk = yn - ll[o] * yk;
-      }
-      x[(size_t)(n - 1) * kEigN + i] = yk;
-      double x1 = 0.0, x2 = 0.0;
-      nrm = 0.0;
-#pragma unroll 4
-      for (int k = n - 1; k >= 0; --k) {
-        const size_t o = (size_t)k * kEigN + i;
-        double u = u0[o];
-        if (fabs(u) < pivtol) u = u < 0.0 ? -pivtol : pivtol;
-        double rhs = x[o];
-        if (k + 1 < n) rhs -= u1[o] * x1;
-        if (k + 2 < n) rhs -= u2[o] * x2;
-        const double xv = rhs / u;
-        x[o] = xv;
-        x2 = x1;
-        x1 = xv;
-        nrm += xv * xv;
-      }
-      inv = 1.0 / sqrt(nrm);
-    }
-    // eigenvector i stored contiguously: Z[i * kEigN + k]
-    for (int k = 0; k < n; ++k) sc.Z[(size_t)i * kEigN + k] = x[(size_t)k * kEigN + i] * inv;
-    if (!(nrm > 0.0 && nrm <= DBL_MAX) || !(lmb == lmb)) atomicOr(&bad, 1);
-    // cluster: the run of eigenvalues with consecutive gaps <= ortol that contains i
-    int st = i;
-    while (st > 0 && lam[st] - lam[st - 1] <= ortol) --st;
-    cstart[i] = st;
-    atomicMax(&maxpos, i - st);
-  }
-  __syncthreads();
-  if (tid == 0) e2v[0] = 0.0;   // Sturm counts are done; e2v[0] holds max |E| in step 3c
-  // 3b: orthonormalise every cluster member against the earlier ones (modified
-  // Gram-Schmidt, two passes), one half-wave per member, members of the same rank in
-  // parallel; a lane holds components lane + 32c
-  {
-    const int hw = tid >> 5, hl = tid & 31;
-    const int rounds = maxpos;
-    for (int r = 1; r <= rounds; ++r) {
-      for (int i = hw; i < n; i += kEigG) {
-        const int st = cstart[i];
-        if (i - st != r) continue;
-        double xv[kEigCB];
-#pragma unroll
-        for (int c = 0; c < kEigCB; ++c) {
-          const int k = hl + 32 * c;
-          xv[c] = k < n ? sc.Z[(size_t)i * kEigN + k] : 0.0;
-        }
-        for (int pass = 0; pass < 2; ++pass) {
-          for (int j = st; j < i; ++j) {
-            double zj[kEigCB], dot = 0.0;
-#pragma unroll
-            for (int c = 0; c < kEigCB; ++c) {
-              const int k = hl + 32 * c;
-              zj[c] = k < n ? sc.Z[(size_t)j * kEigN + k] : 0.0;
-              dot += zj[c] * xv[c];
-            }
-#pragma unroll
-            for (int m = 16; m >= 1; m >>= 1) dot += __shfl_xor(dot, m);
-#pragma unroll
-            for (int c = 0; c < kEigCB; ++c) xv[c] -= dot * zj[c];
-          }
-        }
-        double nn = 0.0;
-#pragma unroll
-        for (int c = 0; c < kEigCB; ++c) nn += xv[c] * xv[c];
-#pragma unroll
-        for (int m = 16; m >= 1; m >>= 1) nn += __shfl_xor(nn, m);
-        if (!(nn > 0.0 && nn <= DBL_MAX) && hl == 0) atomicOr(&bad, 1);
-        const double s = 1.0 / sqrt(nn);
-#pragma unroll
-        for (int c = 0; c < kEigCB; ++c) {
-          const int k = hl + 32 * c;
-          if (k < n) sc.Z[(size_t)i * kEigN + k] = xv[c] * s;
-        }
-      }
-      __syncthreads();
-    }
-  }
-
-  // 3c: Loewdin orthonormalisation Z <- Z - 1/2 (Z Z^T - I) Z, repeated while the largest
-  // overlap exceeds 1e-14 (at most 3 times; first order, so each pass squares the error)
-  {
-    extern __shared__ __align__(16) unsigned char tsm[];
-    double* zs = reinterpret_cast<double*>(tsm);           // [kEigN][kEigN + 1]
-    constexpr int ZS = kEigN + 1;
-    double* E = sc.lu;                                     // [kEigN][kEigN] in global scratch
-    const int ti = tid >> 5, tj = tid & 31;
-    for (int pass = 0; pass < 3; ++pass) {
-      for (int t = tid; t < n * n; t += kEigT) {
-        const int i = t / n, k = t - i * n;
-        zs[i * ZS + k] = sc.Z[(size_t)i * kEigN + k];
-      }
-      if (tid == 0) maxpos = 0;
-      __syncthreads();
-      // E[i][j] for rows i = ti + 16a, columns j = tj + 32b
-      double acc[kEigRA][kEigCB];
-#pragma unroll
-      for (int a = 0; a < kEigRA; ++a)
-#pragma unroll
-        for (int b = 0; b < kEigCB; ++b) acc[a][b] = 0.0;
-      for (int k = 0; k < n; ++k) {
-        double zi[kEigRA], zj[kEigCB];
-#pragma unroll
-        for (int a = 0; a < kEigRA; ++a) zi[a] = zs[(ti + kEigG * a) * ZS + k];
-#pragma unroll
-        for (int b = 0; b < kEigCB; ++b) zj[b] = zs[(tj + 32 * b) * ZS + k];
-#pragma unroll
-        for (int a = 0; a < kEigRA; ++a)
-#pragma unroll
-          for (int b = 0; b < kEigCB; ++b) acc[a][b] += zi[a] * zj[b];
-      }
-      double emax = 0.0;
-#pragma unroll
-      for (int a = 0; a < kEigRA; ++a)
-#pragma unroll
-        for (int b = 0; b < kEigCB; ++b) {
-          const int i = ti + kEigG * a, j = tj + 32 * b;
-          if (i < n && j < n) {
-            const double e = acc[a][b] - (i == j ? 1.0 : 0.0);
-            E[(size_t)i * kEigN + j] = e;
-            emax = fmax(emax, fabs(e));
-          }
-        }
-      // max |E| as an integer bit pattern (non-negative doubles order like their bits)
-      const unsigned long long eb = __double_as_longlong(emax);
-      atomicMax(reinterpret_cast<unsigned long long*>(&e2v[0]), eb);
-      __syncthreads();
-      const double emx = __longlong_as_double((long long)*reinterpret_cast<unsigned long long*>(&e2v[0]));
-      if (!(emx > 1e-14)) break;
-      if (!(emx < 0.1)) { if (tid == 0) atomicOr(&bad, 1); break; }
-      // Z'[i][k] = Z[i][k] - 1/2 sum_j E[i][j] Z[j][k], rows i = ti + 16a, components k = tj + 32b
-#pragma unroll
-      for (int a = 0; a < kEigRA; ++a)
-#pragma unroll
-        for (int b = 0; b < kEigCB; ++b) acc[a][b] = 0.0;
-      for (int j = 0; j < n; ++j) {
-        double ej[kEigRA], zk[kEigCB];
-#pragma unroll
-        for (int a = 0; a < kEigRA; ++a) {
-          const int i = ti + kEigG * a;
-          ej[a] = i < n ? E[(size_t)i * kEigN + j] : 0.0;
-        }
-#pragma unroll
-        for (int b = 0; b < kEigCB; ++b) zk[b] = zs[j * ZS + tj + 32 * b];
-#pragma unroll
-        for (int a = 0; a < kEigRA; ++a)
-#pragma unroll
-          for (int b = 0; b < kEigCB; ++b) acc[a][b] += ej[a] * zk[b];
-      }
-#pragma unroll
-      for (int a = 0; a < kEigRA; ++a)
-#pragma unroll
-        for (int b = 0; b < kEigCB; ++b) {
-          const int i = ti + kEigG * a, k = tj + 32 * b;
-          if (i < n && k < n) sc.Z[(size_t)i * kEigN + k] = zs[i * ZS + k] - 0.5 * acc[a][b];
-        }
-      __syncthreads();
-      if (tid == 0) e2v[0] = 0.0;
-      __syncthreads();
-    }
-  }
-
-  if (tid < n) w[tid] = lam[tid] / scal[4];
-  if (tid == 0) info[0] = bad;
+  if (s == 0) sc.lam[i] = 0.5 * (lo + hi);
 }
 
-// Phase 4: A <- Q Z (the eigenvectors of A, column-major, lda).  The Householder vectors
-// are staged in LDS first (packed lower triangle, n (n-1) / 2 complex), so the chain of
-// n - 1 reflections reads no global memory.
+// Phase 3: eigenvector of T for eigenvalue i = blockIdx.x * 16 + lane (lanes 0..15), by
+// inverse iteration with the factors in LDS ([k][lane], conflict-free).  Z[i][k] (global).
+__global__ __launch_bounds__(64) void k_heev_invit(int n, int* __restrict__ info, EigScratch sc) {
+  extern __shared__ __align__(16) double ism[];
+  double* dv = ism;
+  double* ev = dv + kEigN;
+  double* u0 = ev + kEigN;
+  double* u1 = u0 + kEigN * kInvE;
+  double* u2 = u1 + kEigN * kInvE;
+  double* ll = u2 + kEigN * kInvE;
+  double* pw = ll + kEigN * kInvE;
+  double* x = pw + kEigN * kInvE;
+  double* stat = x + kEigN * kInvE;
+  const int tid = threadIdx.x;
+  load_tridiag(n, sc, dv, ev, stat);
+  const int l = tid, i = blockIdx.x * kInvE + l;
+  if (l >= kInvE || i >= n) return;
+  const double tnorm = stat[1];
+  if (tnorm == 0.0) {   // T = 0 (A = 0): eigenvectors e_i
+    for (int k = 0; k < n; ++k) sc.Z[(size_t)i * kEigN + k] = k == i ? 1.0 : 0.0;
+    return;
+  }
+  const double pivtol = DBL_EPSILON * tnorm;
+  const double lmb = sc.lam[i];
+  // T - lambda I = P L U, row interchanges where the subdiagonal dominates
+  double cd = dv[0] - lmb, cs = n > 1 ? ev[0] : 0.0;
+  for (int k = 0; k + 1 < n; ++k) {
+    const double sub = ev[k], nd = dv[k + 1] - lmb, ns = k + 2 < n ? ev[k + 1] : 0.0;
+    const int o = k * kInvE + l;
+    if (fabs(cd) >= fabs(sub)) {
+      const double m = cd != 0.0 ? sub * frcp(cd) : 0.0;
+      u0[o] = cd; u1[o] = cs; u2[o] = 0.0; ll[o] = m; pw[o] = 0.0;
+      cd = nd - m * cs;
+      cs = ns;
+    } else {
+      const double m = cd * frcp(sub);
+      u0[o] = sub; u1[o] = nd; u2[o] = ns; ll[o] = m; pw[o] = 1.0;
+      const double t = cs - m * nd;
+      cs = -m * ns;
+      cd = t;
+    }
+  }
+  u0[(n - 1) * kInvE + l] = cd;
+  // start vector b_k = cos(k (1 + 0.37 i) + i / 2), then three solves (T - lambda I) x = b / |b|
+  double nrm = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const double b = cos((double)k * (1.0 + 0.37 * i) + 0.5 * i);
+    x[k * kInvE + l] = b;
+    nrm += b * b;
+  }
+  double inv = 1.0 / sqrt(nrm);
+  for (int it = 0; it < 3; ++it) {
+    double yk = x[l] * inv;
+    for (int k = 0; k + 1 < n; ++k) {
+      const int o = k * kInvE + l;
+      double yn = x[o + kInvE] * inv;
+      if (pw[o] != 0.0) { const double t = yk; yk = yn; yn = t; }
+      x[o] = yk;
+      yk = yn - ll[o] * yk;
+    }
+    x[(n - 1) * kInvE + l] = yk;
+    double x1 = 0.0, x2 = 0.0;
+    nrm = 0.0;
+    for (int k = n - 1; k >= 0; --k) {
+      const int o = k * kInvE + l;
+      double u = u0[o];
+      if (fabs(u) < pivtol) u = u < 0.0 ? -pivtol : pivtol;
+      double rhs = x[o];
+      if (k + 1 < n) rhs -= u1[o] * x1;
+      if (k + 2 < n) rhs -= u2[o] * x2;
+      const double xv = rhs * frcp(u);
+      x[o] = xv;
+      x2 = x1;
+      x1 = xv;
+      nrm += xv * xv;
+    }
+    inv = 1.0 / sqrt(nrm);
+  }
+  for (int k = 0; k < n; ++k) sc.Z[(size_t)i * kEigN + k] = x[k * kInvE + l] * inv;
+  if (!(nrm > 0.0 && nrm <= DBL_MAX) || !(lmb == lmb)) atomicOr(info, 1);
+}
+
+// Phase 4 (one workgroup): multiplets by modified Gram-Schmidt, then Loewdin steps
+// Z <- Z - 1/2 (Z Z^T - I) Z while the largest overlap exceeds 1e-14 (one step when it is
+// below 1e-7: the step leaves ~E^2); eigenvalues unscaled into w.
+__global__ __launch_bounds__(kEigT) void k_heev_orth(int n, double* __restrict__ w, int* __restrict__ info,
+                                                     EigScratch sc) {
+  extern __shared__ __align__(16) double zs[];     // [kEigN][kEigN + 1]
+  constexpr int ZS = kEigN + 1;
+  __shared__ int cstart[kEigN];
+  __shared__ int maxpos;
+  __shared__ unsigned long long emax_bits;
+  const int tid = threadIdx.x;
+  if (tid == 0) { maxpos = 0; emax_bits = 0ull; }
+  for (int t = tid; t < n * n; t += kEigT) {
+    const int i = t / n, k = t - i * n;
+    zs[i * ZS + k] = sc.Z[(size_t)i * kEigN + k];
+  }
+  const double scale = sc.st[0], tnorm = sc.st[1];
+  const double ortol = 1e-6 * tnorm;
+  __syncthreads();
+  if (tid < n) {
+    int st = tid;
+    while (st > 0 && sc.lam[st] - sc.lam[st - 1] <= ortol) --st;
+    cstart[tid] = st;
+    atomicMax(&maxpos, tid - st);
+    w[tid] = sc.lam[tid] / scale;
+  }
+  __syncthreads();
+  const int hw = tid >> 5, hl = tid & 31;
+  const int rounds = tnorm == 0.0 ? 0 : maxpos;    // T = 0: the e_i are orthonormal already
+  for (int r = 1; r <= rounds; ++r) {
+    for (int i = hw; i < n; i += kEigG) {
+      const int st = cstart[i];
+      if (i - st != r) continue;
+      double xv[kEigCB];
+#pragma unroll
+      for (int c = 0; c < kEigCB; ++c) {
+        const int k = hl + 32 * c;
+        xv[c] = k < n ? zs[i * ZS + k] : 0.0;
+      }
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int j = st; j < i; ++j) {
+          double zj[kEigCB], dot = 0.0;
+#pragma unroll
+          for (int c = 0; c < kEigCB; ++c) {
+            const int k = hl + 32 * c;
+            zj[c] = k < n ? zs[j * ZS + k] : 0.0;
+            dot += zj[c] * xv[c];
+          }
+          dot = halfwave_sum_dpp(dot);
+#pragma unroll
+          for (int c = 0; c < kEigCB; ++c) xv[c] -= dot * zj[c];
+        }
+      }
+      double nn = 0.0;
+#pragma unroll
+      for (int c = 0; c < kEigCB; ++c) nn += xv[c] * xv[c];
+      nn = halfwave_sum_dpp(nn);
+      if (!(nn > 0.0 && nn <= DBL_MAX) && hl == 0) atomicOr(info, 1);
+      const double sn = 1.0 / sqrt(nn);
+#pragma unroll
+      for (int c = 0; c < kEigCB; ++c) {
+        const int k = hl + 32 * c;
+        if (k < n) zs[i * ZS + k] = xv[c] * sn;
+      }
+    }
+    __syncthreads();
+  }
+  // Loewdin: E = Z Z^T - I for rows i = ti + 16a, columns j = tj + 32b
+  double* E = sc.lu;
+  const int ti = tid >> 5, tj = tid & 31;
+  for (int pass = 0; pass < 3; ++pass) {
+    double acc[kEigRA][kEigCB];
+#pragma unroll
+    for (int a = 0; a < kEigRA; ++a)
+#pragma unroll
+      for (int b = 0; b < kEigCB; ++b) acc[a][b] = 0.0;
+    for (int k = 0; k < n; ++k) {
+      double zi[kEigRA], zj[kEigCB];
+#pragma unroll
+      for (int a = 0; a < kEigRA; ++a) zi[a] = zs[(ti + kEigG * a) * ZS + k];
+#pragma unroll
+      for (int b = 0; b < kEigCB; ++b) zj[b] = zs[(tj + 32 * b) * ZS + k];
+#pragma unroll
+      for (int a = 0; a < kEigRA; ++a)
+#pragma unroll
+        for (int b = 0; b < kEigCB; ++b) acc[a][b] += zi[a] * zj[b];
+    }
+    double em = 0.0;
+#pragma unroll
+    for (int a = 0; a < kEigRA; ++a)
+#pragma unroll
+      for (int b = 0; b < kEigCB; ++b) {
+        const int i = ti + kEigG * a, j = tj + 32 * b;
+        if (i < n && j < n) {
+          const double e = acc[a][b] - (i == j ? 1.0 : 0.0);
+          E[(size_t)i * kEigN + j] = e;
+          em = fmax(em, fabs(e));
+        }
+      }
+    // non-negative doubles order like their bit patterns
+    atomicMax(&emax_bits, (unsigned long long)__double_as_longlong(em));
+    __syncthreads();
+    const double emx = __longlong_as_double((long long)emax_bits);
+    if (!(emx > 1e-14)) break;
+    if (!(emx < 0.1)) { if (tid == 0) atomicOr(info, 1); break; }
+    // Z'[i][k] = Z[i][k] - 1/2 sum_j E[i][j] Z[j][k] for rows i = ti + 16a, k = tj + 32b
+#pragma unroll
+    for (int a = 0; a < kEigRA; ++a)
+#pragma unroll
+      for (int b = 0; b < kEigCB; ++b) acc[a][b] = 0.0;
+    for (int j = 0; j < n; ++j) {
+      double ej[kEigRA], zk[kEigCB];
+#pragma unroll
+      for (int a = 0; a < kEigRA; ++a) {
+        const int i = ti + kEigG * a;
+        ej[a] = i < n ? E[(size_t)i * kEigN + j] : 0.0;
+      }
+#pragma unroll
+      for (int b = 0; b < kEigCB; ++b) zk[b] = zs[j * ZS + tj + 32 * b];
+#pragma unroll
+      for (int a = 0; a < kEigRA; ++a)
+#pragma unroll
+        for (int b = 0; b < kEigCB; ++b) acc[a][b] += ej[a] * zk[b];
+    }
+    __syncthreads();    // every lane has read zs before it is overwritten
+#pragma unroll
+    for (int a = 0; a < kEigRA; ++a)
+#pragma unroll
+      for (int b = 0; b < kEigCB; ++b) {
+        const int i = ti + kEigG * a, k = tj + 32 * b;
+        if (i < n && k < n) zs[i * ZS + k] -= 0.5 * acc[a][b];
+      }
+    if (tid == 0) emax_bits = 0ull;
+    __syncthreads();
+    if (emx < 1e-7) break;
+  }
+  for (int t = tid; t < n * n; t += kEigT) {
+    const int i = t / n, k = t - i * n;
+    sc.Z[(size_t)i * kEigN + k] = zs[i * ZS + k];
+  }
+}
+
+// Phase 4: A <- Q Z (the eigenvectors of A, column-major, lda).  Columns are independent:
+// a workgroup takes kBtCols of them, one half-wave per column (lane r holds rows r + 32a).
+// The Householder vectors are staged in LDS first (packed lower triangle, n (n-1) / 2
+// complex), so the chain of n - 1 reflections reads no global memory.
+constexpr int kBtCols = kEigT / 32;
 __global__ __launch_bounds__(kEigT) void k_heev_backtr(cx<double>* __restrict__ A, int lda, int n,
                                                        EigScratch sc) {
   extern __shared__ __align__(16) unsigned char bsm[];
@@ -580,75 +614,77 @@ __global__ __launch_bounds__(kEigT) void k_heev_backtr(cx<double>* __restrict__ 
   }
   for (int k = tid; k + 1 < n; k += kEigT) tp[k] = mk<double>(sc.tau[2 * k], sc.tau[2 * k + 1]);
   __syncthreads();
-  const int rr = tid & 31, cc = tid >> 5;
-  cx<double> y[kEigCB][kEigRA];
+  const int r = tid & 31, j = blockIdx.x * kBtCols + (tid >> 5);
+  if (j >= n) return;
+  cx<double> y[kEigCB];
 #pragma unroll
-  for (int ia = 0; ia < kEigCB; ++ia)
-#pragma unroll
-    for (int jb = 0; jb < kEigRA; ++jb) {
-      const int i = rr + 32 * ia, j = cc + kEigG * jb;
-      y[ia][jb] = mk<double>((i < n && j < n) ? sc.Z[(size_t)j * kEigN + i] : 0.0, 0.0);
-    }
+  for (int a = 0; a < kEigCB; ++a) {
+    const int i = r + 32 * a;
+    y[a] = mk<double>(i < n ? sc.Z[(size_t)j * kEigN + i] : 0.0, 0.0);
+  }
   for (int k = n - 2; k >= 0; --k) {
     const cx<double> tk = tp[k];
     const int o = vpack_off(k, n) - (k + 1);
     cx<double> v[kEigCB];
+    double sr = 0.0, si = 0.0;
 #pragma unroll
-    for (int ia = 0; ia < kEigCB; ++ia) {
-      const int i = rr + 32 * ia;
-      v[ia] = (i > k && i < n) ? vp[o + i] : mk<double>(0.0, 0.0);
+    for (int a = 0; a < kEigCB; ++a) {
+      const int i = r + 32 * a;
+      const bool in = i > k && i < n;
+      const cx<double> t = vp[in ? o + i : 0];
+      v[a] = in ? t : mk<double>(0.0, 0.0);
+      // conj(v) y
+      sr += v[a].x * y[a].x + v[a].y * y[a].y;
+      si += v[a].x * y[a].y - v[a].y * y[a].x;
     }
+    sr = halfwave_sum_dpp(sr);
+    si = halfwave_sum_dpp(si);
+    const cx<double> sv = cmul(tk, mk<double>(sr, si));
 #pragma unroll
-    for (int jb = 0; jb < kEigRA; ++jb) {
-      cx<double> s = mk<double>(0.0, 0.0);
-#pragma unroll
-      for (int ia = 0; ia < kEigCB; ++ia) s = s + cmulc(v[ia], y[ia][jb]);
-      s = cmul(tk, halfwave_sum(s));
-#pragma unroll
-      for (int ia = 0; ia < kEigCB; ++ia) y[ia][jb] = y[ia][jb] - cmul(v[ia], s);
-    }
+    for (int a = 0; a < kEigCB; ++a) y[a] = y[a] - cmul(v[a], sv);
   }
 #pragma unroll
-  for (int ia = 0; ia < kEigCB; ++ia)
-#pragma unroll
-    for (int jb = 0; jb < kEigRA; ++jb) {
-      const int i = rr + 32 * ia, j = cc + kEigG * jb;
-      if (i < n && j < n) A[i + (size_t)j * lda] = y[ia][jb];
-    }
+  for (int a = 0; a < kEigCB; ++a) {
+    const int i = r + 32 * a;
+    if (i < n) A[i + (size_t)j * lda] = y[a];
+  }
 }
 
-size_t heev_small_scratch_doubles() { return (size_t)kEigN * kEigN * (2 + 1 + 5 + 1) + 4 * kEigN; }
+size_t heev_small_scratch_doubles() { return (size_t)kEigN * kEigN * (2 + 1 + 1) + 6 * kEigN + 8; }
 
 hipError_t launch_heev_small(cx<double>* A, int lda, int n, double* w, int* info, double* scratch,
                              hipStream_t s) {
   if (n < 1 || n > kEigN || lda < n) return hipErrorInvalidValue;
   EigScratch sc;
-  sc.V = scratch;
-  sc.Z = scratch + 2 * kEigN * kEigN;
-  sc.lu = sc.Z + kEigN * kEigN;
-  sc.x = sc.lu + 5 * kEigN * kEigN;
-  sc.tau = sc.x + kEigN * kEigN;
+  sc.V = scratch;                               // 2 kEigN^2
+  sc.Z = sc.V + 2 * kEigN * kEigN;              // kEigN^2
+  sc.lu = sc.Z + kEigN * kEigN;                 // kEigN^2 (the Loewdin E)
+  sc.tau = sc.lu + kEigN * kEigN;               // 2 kEigN
   sc.d = sc.tau + 2 * kEigN;
   sc.e = sc.d + kEigN;
-  hipLaunchKernelGGL(k_heev_tridiag, dim3(1), dim3(kEigT), 0, s, A, lda, n, sc);
-  constexpr size_t tsm = sizeof(double) * (size_t)kEigN * (kEigN + 1);
-  static bool tattr = false;
-  if (!tattr) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_heev_trieig),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)tsm);
-    if (e != hipSuccess) return e;
-    tattr = true;
-  }
-  hipLaunchKernelGGL(k_heev_trieig, dim3(1), dim3(kEigT), tsm, s, n, w, info, sc);
+  sc.lam = sc.e + kEigN;
+  sc.st = sc.lam + kEigN;                       // 4 (+ padding)
+  constexpr size_t ism = sizeof(double) * ((size_t)2 * kEigN + 6 * (size_t)kEigN * kInvE + 8);
+  constexpr size_t osm = sizeof(double) * (size_t)kEigN * (kEigN + 1);
   constexpr size_t bsm = sizeof(cx<double>) * ((size_t)kEigN * (kEigN - 1) / 2 + kEigN);
   static bool attr = false;
   if (!attr) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_heev_backtr),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bsm);
-    if (e != hipSuccess) return e;
+    hipError_t e;
+    if ((e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_heev_invit),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)ism)) != hipSuccess ||
+        (e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_heev_orth),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)osm)) != hipSuccess ||
+        (e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_heev_backtr),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bsm)) != hipSuccess)
+      return e;
     attr = true;
   }
-  hipLaunchKernelGGL(k_heev_backtr, dim3(1), dim3(kEigT), bsm, s, A, lda, n, sc);
+  const int per_bis = kEigT / kBisS;
+  hipLaunchKernelGGL(k_heev_tridiag, dim3(1), dim3(kEigT), 0, s, A, lda, n, info, sc);
+  hipLaunchKernelGGL(k_heev_bisect, dim3((n + per_bis - 1) / per_bis), dim3(kEigT), 0, s, n, sc);
+  hipLaunchKernelGGL(k_heev_invit, dim3((n + kInvE - 1) / kInvE), dim3(64), ism, s, n, info, sc);
+  hipLaunchKernelGGL(k_heev_orth, dim3(1), dim3(kEigT), osm, s, n, w, info, sc);
+  hipLaunchKernelGGL(k_heev_backtr, dim3((n + kBtCols - 1) / kBtCols), dim3(kEigT), bsm, s, A, lda, n, sc);
   return hipGetLastError();
 }
 
