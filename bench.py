@@ -265,9 +265,10 @@ def main():
         # one untimed iteration per path loads the solver kernels (first-call cost)
         for p in ("fp64", "fp32", "bf16x3"):
             sysm.chfsi(X0, degree=2, niter=1, precision=p, lanczos_k=4, v0=v0)
-        # the 128x128 FP64 Rayleigh-Ritz eigensolve: rocSOLVER D&C, QR iteration, Jacobi, Jacobi-D&C
+        # the 128x128 FP64 Rayleigh-Ritz eigensolve: the library's own five-launch solver
+        # (default), rocSOLVER D&C, QR iteration, Jacobi, Jacobi-D&C
         rr_ms = {}
-        for solver in ("evd", "ev", "evj", "evdj"):
+        for solver in ("own", "evd", "ev", "evj", "evdj"):
             os.environ["PWH_EIGSOLVER"] = solver
             sysm.chfsi(X0, degree=2, niter=1, precision="fp32", lanczos_k=4, v0=v0)
             _, _, _, st = sysm.chfsi(X0, degree=2, niter=3, precision="fp32", lanczos_k=4, v0=v0)

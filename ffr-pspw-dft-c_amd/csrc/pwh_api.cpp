@@ -1,7 +1,7 @@
 // pwh_api.cpp -- host orchestration and the C ABI of libpwh.
 //
 // One library stream carries every launch.  A pwh_system holds, in HBM, the sphere
-// maps, the phases/projectors in both FP64 and FP32, the local potential, and
+// maps, the projectors beta = EIGR x PKG_A in both FP64 and FP32, the local potential, and
 // resident work buffers that grow on demand; nothing is allocated per call once a
 // system has seen its largest block.
 #include <hip/hip_runtime.h>
@@ -175,13 +175,14 @@ struct pwh_system {
   std::vector<int> perm;
   Buf<int> d_perm, d_colstart, d_kcol, d_kz, d_colxy, d_xa;
   int nsp = 0, nat = 0;
-  std::vector<int> na, lm, aoff;
+  std::vector<int> na, lm, aoff, poff;   // poff: first projector row of each species
   std::vector<double> wl;   // [nsp*4]
   // constants, both precisions
-  Buf<cx<double>> E64;
-  Buf<double> P64, kin64, wl64, V64;
-  Buf<cx<float>> E32;
-  Buf<float> P32, kin32, wl32, V32;
+  // projectors beta[(a,l)][k] = EIGR(k,a) PKG_A(l,k), [nproj][ngw_pad], species sp from row poff[sp]
+  Buf<cx<double>> Bt64;
+  Buf<double> kin64, wl64, V64;
+  Buf<cx<float>> Bt32;
+  Buf<float> kin32, wl32, V32;
   bool vset = false;
   Buf<cx<double>> tw64[3];
   Buf<cx<float>> tw32[3];
@@ -201,8 +202,7 @@ namespace {
 
 template <typename T> struct Sel;
 template <> struct Sel<double> {
-  static cx<double>* E(pwh_system* s) { return s->E64.p; }
-  static double* P(pwh_system* s) { return s->P64.p; }
+  static cx<double>* Bt(pwh_system* s) { return s->Bt64.p; }
   static double* kin(pwh_system* s) { return s->kin64.p; }
   static double* wl(pwh_system* s) { return s->wl64.p; }
   static double* V(pwh_system* s) { return s->V64.p; }
@@ -214,8 +214,7 @@ template <> struct Sel<double> {
   static Buf<cx<double>>& part(pwh_system* s) { return s->part64; }
 };
 template <> struct Sel<float> {
-  static cx<float>* E(pwh_system* s) { return s->E32.p; }
-  static float* P(pwh_system* s) { return s->P32.p; }
+  static cx<float>* Bt(pwh_system* s) { return s->Bt32.p; }
   static float* kin(pwh_system* s) { return s->kin32.p; }
   static float* wl(pwh_system* s) { return s->wl32.p; }
   static float* V(pwh_system* s) { return s->V32.p; }
@@ -295,7 +294,7 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
     CHK(Sel<T>::Q(s).ensure((size_t)M * nb));
     const size_t pe = proj_part_elems(M, nb, s->ngw);
     CHK(Sel<T>::part(s).ensure(pe));
-    BetaGen bg{Sel<T>::E(s), Sel<T>::P(s) + (size_t)sp * 4 * ld, s->aoff[sp], s->na[sp], s->lm[sp], ld};
+    BetaGen bg{Sel<T>::Bt(s) + (size_t)s->poff[sp] * ld, M, s->lm[sp], ld};
     {
       Timed t(TM_PROJ, sn);
       HIPCHK(gemm_proj<T>(bg, X, ld, nb, s->ngw, Sel<T>::proj(s).p, Sel<T>::part(s).p, pe, mode, sn));
@@ -837,31 +836,37 @@ int pwh_system_create(const pwh_system_desc* d, pwh_system** out) {
   s->maps.colstart = s->d_colstart.p; s->maps.kcol = s->d_kcol.p; s->maps.kz = s->d_kz.p;
   s->maps.colxy = s->d_colxy.p; s->maps.xa = s->d_xa.p;
 
-  // constants in sphere order
+  // constants in sphere order; the projectors beta = EIGR x PKG_A (ApplyHam_block.cpp:88-95)
+  // are formed once in FP64 and rounded once for the FP32 copy
   const int ld = s->ngw_pad;
-  std::vector<cx<double>> E((size_t)std::max(s->nat, 1) * ld, mk<double>(0, 0));
-  std::vector<double> P((size_t)std::max(s->nsp, 1) * 4 * ld, 0.0), kin(ld, 0.0);
+  s->poff.assign(s->nsp, 0);
+  int nproj = 0;
+  for (int sp = 0; sp < s->nsp; ++sp) { s->poff[sp] = nproj; nproj += s->na[sp] * s->lm[sp]; }
+  std::vector<cx<double>> Bt((size_t)std::max(nproj, 1) * ld, mk<double>(0, 0));
+  std::vector<double> kin(ld, 0.0);
   const cx<double>* eigr = reinterpret_cast<const cx<double>*>(d->eigr);
-  for (int sp = 0; sp < s->nsp; ++sp) {
+  for (int sp = 0; sp < s->nsp; ++sp)
     for (int ia = 0; ia < s->na[sp]; ++ia)
-      for (int k = 0; k < ngw; ++k)
-        E[(size_t)(s->aoff[sp] + ia) * ld + k] = eigr[s->perm[k] + (size_t)d->ngwx * (ia + (size_t)d->nax * sp)];
-    for (int l = 0; l < s->lm[sp]; ++l)
-      for (int k = 0; k < ngw; ++k)
-        P[((size_t)sp * 4 + l) * ld + k] = d->pkg_a[l + (size_t)d->nlmax * (s->perm[k] + (size_t)d->ngwx * sp)];
-  }
+      for (int l = 0; l < s->lm[sp]; ++l) {
+        cx<double>* row = Bt.data() + (size_t)(s->poff[sp] + ia * s->lm[sp] + l) * ld;
+        for (int k = 0; k < ngw; ++k) {
+          const size_t ig = s->perm[k];
+          const cx<double> e = eigr[ig + (size_t)d->ngwx * (ia + (size_t)d->nax * sp)];
+          const double pk = d->pkg_a[l + (size_t)d->nlmax * (ig + (size_t)d->ngwx * sp)];
+          row[k] = mk<double>(e.x * pk, e.y * pk);
+        }
+      }
   for (int k = 0; k < ngw; ++k) kin[k] = 0.5 * d->xkg[s->perm[k]] * d->tpiba2;
-  std::vector<cx<float>> E32(E.size());
-  std::vector<float> P32(P.size()), kin32(kin.size()), wl32(s->wl.size());
-  for (size_t i = 0; i < E.size(); ++i) E32[i] = mk<float>((float)E[i].x, (float)E[i].y);
-  for (size_t i = 0; i < P.size(); ++i) P32[i] = (float)P[i];
+  std::vector<cx<float>> Bt32(Bt.size());
+  std::vector<float> kin32(kin.size()), wl32(s->wl.size());
+  for (size_t i = 0; i < Bt.size(); ++i) Bt32[i] = mk<float>((float)Bt[i].x, (float)Bt[i].y);
   for (size_t i = 0; i < kin.size(); ++i) kin32[i] = (float)kin[i];
   for (size_t i = 0; i < s->wl.size(); ++i) wl32[i] = (float)s->wl[i];
   std::vector<double> wlv = s->wl;
   if (wlv.empty()) wlv.assign(4, 0.0), wl32.assign(4, 0.f);
-  if ((rc = upload(s->E64, E.data(), E.size())) || (rc = upload(s->P64, P.data(), P.size())) ||
+  if ((rc = upload(s->Bt64, Bt.data(), Bt.size())) ||
       (rc = upload(s->kin64, kin.data(), kin.size())) || (rc = upload(s->wl64, wlv.data(), wlv.size())) ||
-      (rc = upload(s->E32, E32.data(), E32.size())) || (rc = upload(s->P32, P32.data(), P32.size())) ||
+      (rc = upload(s->Bt32, Bt32.data(), Bt32.size())) ||
       (rc = upload(s->kin32, kin32.data(), kin32.size())) || (rc = upload(s->wl32, wl32.data(), wl32.size())))
     return bail(rc);
   for (int dd = 0; dd < 3; ++dd) {
@@ -881,13 +886,13 @@ int pwh_system_destroy(pwh_system* s) {
   if (!s) return PWH_OK;
   if (g.stream) hipStreamSynchronize(g.stream);
   for (auto* b : {&s->d_perm, &s->d_colstart, &s->d_kcol, &s->d_kz, &s->d_colxy, &s->d_xa}) b->release();
-  for (auto* b : {&s->E64, &s->X, &s->HX, &s->A64, &s->B64, &s->C64, &s->W64, &s->Hnl64, &s->proj64,
+  for (auto* b : {&s->Bt64, &s->X, &s->HX, &s->A64, &s->B64, &s->C64, &s->W64, &s->Hnl64, &s->proj64,
                   &s->Q64, &s->part64, &s->stage, &s->S, &s->Vl})
     b->release();
-  for (auto* b : {&s->P64, &s->kin64, &s->wl64, &s->V64, &s->evals, &s->rwork, &s->dres}) b->release();
-  for (auto* b : {&s->E32, &s->X32, &s->Y32, &s->Y1_32, &s->W32, &s->Hnl32, &s->proj32, &s->Q32, &s->part32})
+  for (auto* b : {&s->kin64, &s->wl64, &s->V64, &s->evals, &s->rwork, &s->dres}) b->release();
+  for (auto* b : {&s->Bt32, &s->X32, &s->Y32, &s->Y1_32, &s->W32, &s->Hnl32, &s->proj32, &s->Q32, &s->part32})
     b->release();
-  for (auto* b : {&s->P32, &s->kin32, &s->wl32, &s->V32}) b->release();
+  for (auto* b : {&s->kin32, &s->wl32, &s->V32}) b->release();
   for (int d = 0; d < 3; ++d) { s->tw64[d].release(); s->tw32[d].release(); }
   s->info.release();
   if (s->nl_stream) { hipStreamSynchronize(s->nl_stream); hipStreamDestroy(s->nl_stream); }
@@ -905,7 +910,7 @@ int pwh_system_info(const pwh_system* s, int* ngw, int* ncol, int* nxa, size_t* 
   if (bytes) {
     size_t b = 0;
     auto add = [&](size_t cap, size_t el) { b += cap * el; };
-    add(s->E64.cap, 16); add(s->E32.cap, 8); add(s->P64.cap, 8); add(s->P32.cap, 4);
+    add(s->Bt64.cap, 16); add(s->Bt32.cap, 8);
     add(s->X.cap, 16); add(s->HX.cap, 16); add(s->A64.cap, 16); add(s->B64.cap, 16);
     add(s->W64.cap, 16); add(s->W32.cap, 8); add(s->Hnl64.cap, 16); add(s->Hnl32.cap, 8);
     add(s->X32.cap, 8); add(s->Y32.cap, 8); add(s->Y1_32.cap, 8); add(s->part64.cap, 16);

@@ -3,14 +3,14 @@
 // Two shapes cover the nonlocal projectors and the subspace matrices:
 //
 //   proj   C[m][n] = sum_k conj(A[m][k]) * B[n][k]      (reduction over plane waves)
-//          - nonlocal:  A = beta[(a,l)][k] = EIGR[a][k]*PKG_A[l][k] generated while the
-//            tile is staged (never stored), B = psi[b][k]  ->  <beta_(a,l)|psi_b>
+//          - nonlocal:  A = beta[(a,l)][k] = EIGR[a][k]*PKG_A[l][k] (stored once per
+//            system), B = psi[b][k]  ->  <beta_(a,l)|psi_b>
 //            (ApplyHam_block.cpp:88-117: conj(EIGR)*PKG_A then zdotu with vin)
 //          - subspace:  S = X^H X, T = X^H (HX)  (ChebySCFLoop.cpp:296)
 //          split over K into deterministic partial sums, then a reduction that can
 //          apply the per-projector D = WNL(l) scale.
 //   expand out[n][k] (+)= sum_j A[j][k] * Q[j][n]      (reduction over a small index)
-//          - nonlocal:  A = beta (generated), Q = D <beta|psi>  ->  sum beta D <beta|psi>
+//          - nonlocal:  A = beta (stored), Q = D <beta|psi>  ->  sum beta D <beta|psi>
 //            (ApplyHam_block.cpp:118-163)
 //          - rotation:  X <- X Q (ChebySCFLoop.cpp:304-306, CholeskyQR's X R^-1)
 //
@@ -21,6 +21,7 @@
 //          (a*b ~ ah*bh + ah*bl + al*bh), FP32 accumulation.
 // Block tile 64x64 complex, 4 waves of 32x32, LDS staging of Re/Im planes.
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 #include <vector>
 
@@ -78,37 +79,36 @@ __device__ __forceinline__ s4v neg4(s4v v) {
 }
 
 constexpr int BT = 64;    // block tile (both output dims)
-constexpr int BKS = 16;   // reduction depth per LDS stage
 constexpr int LPAD = 4;
 constexpr int GT = 256;   // threads per block
 
-// Row loader: either a plain k-contiguous matrix or beta generated from EIGR x PKG_A.
+// Row loader of a k-contiguous matrix A[m][k] (lda).  The BETA flag is kept in the kernel
+// signatures for the tile shapes; only the plain form is instantiated.
 template <typename T, bool BETA>
 struct RowLoader {
-  const cx<T>* A; int lda;       // plain
-  const cx<T>* E; const T* P;    // beta: E [nat][ld], P [lm][ld]
-  int a0, lm, ld;
-  __device__ __forceinline__ cx<T> operator()(int m, int k) const {
-    if constexpr (BETA) {
-      const int a = m / lm, l = m - a * lm;
-      const cx<T> e = E[(size_t)(a0 + a) * ld + k];
-      const T p = P[(size_t)l * ld + k];
-      return mk<T>(e.x * p, e.y * p);
-    } else {
-      return A[(size_t)m * lda + k];
-    }
-  }
+  const cx<T>* A; int lda;
+  __device__ __forceinline__ cx<T> operator()(int m, int k) const { return A[(size_t)m * lda + k]; }
 };
 
 // ---------------------------------------------------------------------------
 // proj: part[s][m][n] = sum_{k in chunk s} conj(A[m][k]) B[n][k]
 // ---------------------------------------------------------------------------
-template <typename T, bool BETA, bool BF16X3>
+// LDS of one block: four planes (Re/Im of both operands) of KS rows of BT + LPAD
+template <typename T, int KS>
+constexpr size_t gemm_smem() { return 4 * sizeof(T) * (size_t)KS * (BT + LPAD); }
+
+template <typename T, bool BETA, bool BF16X3, int KS>
 __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const cx<T>* __restrict__ B,
                                                   int ldb, int M, int N, int K, int kchunk,
                                                   cx<T>* __restrict__ part) {
   using acc_t = typename AccT<T>::type;
-  __shared__ T sAr[BKS][BT + LPAD], sAi[BKS][BT + LPAD], sBr[BKS][BT + LPAD], sBi[BKS][BT + LPAD];
+  static_assert(KS % 16 == 0, "reduction depth per stage: multiple of 16");
+  extern __shared__ __align__(16) unsigned char gsm[];
+  typedef T Row[BT + LPAD];
+  Row* sAr = reinterpret_cast<Row*>(gsm);
+  Row* sAi = sAr + KS;
+  Row* sBr = sAi + KS;
+  Row* sBi = sBr + KS;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int tiles_n = (N + BT - 1) / BT;
   const int m0 = (blockIdx.x / tiles_n) * BT, n0 = (blockIdx.x % tiles_n) * BT;
@@ -122,13 +122,17 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
     for (int b = 0; b < 2; ++b) { cr[a][b] = acc_t{0, 0, 0, 0}; ci[a][b] = acc_t{0, 0, 0, 0}; }
 
   // register prefetch of the next K tile overlaps its global loads with the MFMAs
-  constexpr int PER = BT * BKS / GT;
+  // element e of a stage: 16 consecutive k of one row per 16 lanes (row r = (e/16) % 64,
+  // k = 16 (e/1024) + e%16): 128-byte row segments, conflict-free transposed LDS stores
+  constexpr int PER = BT * KS / GT;
   cx<T> pa[PER], pb[PER];
+  auto slot = [](int e, int& kk, int& rr) { kk = (e & 15) + 16 * (e >> 10); rr = (e >> 4) & (BT - 1); };
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
       const int e = tid + q * GT;
-      const int kk = e % BKS, rr = e / BKS;
+      int kk, rr;
+      slot(e, kk, rr);
       const int k = k0 + kk;
       pa[q] = mk<T>(0, 0);
       pb[q] = mk<T>(0, 0);
@@ -139,23 +143,26 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
     }
   };
   fetch(ks);
-  for (int k0 = ks; k0 < ke; k0 += BKS) {
+  for (int k0 = ks; k0 < ke; k0 += KS) {
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
       const int e = tid + q * GT;
-      const int kk = e % BKS, rr = e / BKS;
+      int kk, rr;
+      slot(e, kk, rr);
       sAr[kk][rr] = pa[q].x; sAi[kk][rr] = pa[q].y;
       sBr[kk][rr] = pb[q].x; sBi[kk][rr] = pb[q].y;
     }
     __syncthreads();
-    if (k0 + BKS < ke) fetch(k0 + BKS);
+    if (k0 + KS < ke) fetch(k0 + KS);
+#pragma unroll
+    for (int kc = 0; kc < KS; kc += 16) {
     if constexpr (!BF16X3) {
-      // substep sub reduces over k = 4*lq + sub: lane quarters then read rows 4 apart,
+      // substep sub reduces over k = kc + 4*lq + sub: lane quarters then read rows 4 apart,
       // 4*(BT+LPAD) = 16 banks, so the four quarters of a wave never share a bank
 #pragma unroll
       for (int sub = 0; sub < 4; ++sub) {
         T ar[2], ai[2], br[2], bi[2];
-        const int kr = 4 * lq + sub;
+        const int kr = kc + 4 * lq + sub;
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
           ar[t] = sAr[kr][wm + t * 16 + li]; ai[t] = sAi[kr][wm + t * 16 + li];
@@ -182,7 +189,7 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
       Split4 ar[2], ai[2], br[2], bi[2];
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
-        const int c = wm + t * 16 + li, d = wn + t * 16 + li, q = 4 * lq;
+        const int c = wm + t * 16 + li, d = wn + t * 16 + li, q = kc + 4 * lq;
         ar[t] = split4(sAr[q][c], sAr[q + 1][c], sAr[q + 2][c], sAr[q + 3][c]);
         ai[t] = split4(sAi[q][c], sAi[q + 1][c], sAi[q + 2][c], sAi[q + 3][c]);
         br[t] = split4(sBr[q][d], sBr[q + 1][d], sBr[q + 2][d], sBr[q + 3][d]);
@@ -210,6 +217,7 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
             ci[a][b] = mfma16bf(xb, yb, ci[a][b]);
           }
       }
+    }
     }
     __syncthreads();
   }
@@ -256,12 +264,18 @@ __global__ __launch_bounds__(256) void k_reduce_parts(const cx<T>* __restrict__ 
 // ---------------------------------------------------------------------------
 // expand: out[n][k] (+)= sum_j A[j][k] Q[j][n]
 // ---------------------------------------------------------------------------
-template <typename T, bool BETA, bool BF16X3, bool QCM>
+template <typename T, bool BETA, bool BF16X3, bool QCM, int KS>
 __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const cx<T>* __restrict__ Q,
                                                     int ldq, int J, int N, int K,
                                                     cx<T>* __restrict__ out, int ldo, int accumulate) {
   using acc_t = typename AccT<T>::type;
-  __shared__ T sAr[BKS][BT + LPAD], sAi[BKS][BT + LPAD], sQr[BKS][BT + LPAD], sQi[BKS][BT + LPAD];
+  static_assert(KS % 16 == 0, "reduction depth per stage: multiple of 16");
+  extern __shared__ __align__(16) unsigned char gsm[];
+  typedef T Row[BT + LPAD];
+  Row* sAr = reinterpret_cast<Row*>(gsm);
+  Row* sAi = sAr + KS;
+  Row* sQr = sAi + KS;
+  Row* sQi = sQr + KS;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int k0 = blockIdx.x * BT, n0 = blockIdx.y * BT;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -272,7 +286,7 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
 #pragma unroll
     for (int b = 0; b < 2; ++b) { cr[a][b] = acc_t{0, 0, 0, 0}; ci[a][b] = acc_t{0, 0, 0, 0}; }
 
-  constexpr int PER = BT * BKS / GT;
+  constexpr int PER = BT * KS / GT;
   cx<T> pa[PER], pq[PER];
   auto fetch = [&](int j0) {
 #pragma unroll
@@ -289,7 +303,7 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
     }
   };
   fetch(0);
-  for (int j0 = 0; j0 < J; j0 += BKS) {
+  for (int j0 = 0; j0 < J; j0 += KS) {
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
       const int e = tid + q * GT;
@@ -298,13 +312,15 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
       sQr[jj][kk] = pq[q].x; sQi[jj][kk] = pq[q].y;
     }
     __syncthreads();
-    if (j0 + BKS < J) fetch(j0 + BKS);
+    if (j0 + KS < J) fetch(j0 + KS);
+#pragma unroll
+    for (int kc = 0; kc < KS; kc += 16) {
     if constexpr (!BF16X3) {
-      // reduction index j = 4*lq + sub (conflict-free rows, as in k_gemm_proj)
+      // reduction index j = kc + 4*lq + sub (conflict-free rows, as in k_gemm_proj)
 #pragma unroll
       for (int sub = 0; sub < 4; ++sub) {
         T ar[2], ai[2], br[2], bi[2];
-        const int jr = 4 * lq + sub;
+        const int jr = kc + 4 * lq + sub;
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
           ar[t] = sAr[jr][wm + t * 16 + li]; ai[t] = sAi[jr][wm + t * 16 + li];
@@ -329,7 +345,7 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
       Split4 ar[2], ai[2], br[2], bi[2];
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
-        const int c = wm + t * 16 + li, d = wn + t * 16 + li, q = 4 * lq;
+        const int c = wm + t * 16 + li, d = wn + t * 16 + li, q = kc + 4 * lq;
         ar[t] = split4(sAr[q][c], sAr[q + 1][c], sAr[q + 2][c], sAr[q + 3][c]);
         ai[t] = split4(sAi[q][c], sAi[q + 1][c], sAi[q + 2][c], sAi[q + 3][c]);
         br[t] = split4(sQr[q][d], sQr[q + 1][d], sQr[q + 2][d], sQr[q + 3][d]);
@@ -357,6 +373,7 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
           }
       }
     }
+    }
     __syncthreads();
   }
 #pragma unroll
@@ -378,9 +395,18 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
+static int gemm_target_blocks() {
+  static const int t = [] {
+    const char* e = getenv("PWH_GEMM_BLOCKS");
+    const int v = e ? atoi(e) : 0;
+    return v >= 64 ? v : 512;
+  }();
+  return t;
+}
+
 static int choose_split(int tiles, int K, size_t part_elems, size_t MN) {
   // aim for ~2 blocks per CU on 256 CUs, at least 128 plane waves per block
-  int s = (512 + tiles - 1) / tiles;
+  int s = (gemm_target_blocks() + tiles - 1) / tiles;
   int smax = (K + 127) / 128;
   if (s > smax) s = smax;
   if (s < 1) s = 1;
@@ -394,39 +420,98 @@ size_t proj_part_elems(int M, int N, int K) {
   return (size_t)s * M * N;
 }
 
+// Reduction depth per LDS stage: deeper stages give each wave more MFMA work per barrier
+// and per register prefetch, which has to hide the global load latency.  PWH_GEMM_KS
+// (16, 32, 64; 64 for FP32 only) overrides the default.
+template <typename T>
+static int gemm_ks() {
+  static const int ks = [] {
+    const char* e = getenv("PWH_GEMM_KS");
+    const int v = e ? atoi(e) : 0;
+    if (v == 16 || v == 32 || (v == 64 && sizeof(T) == 4)) return v;
+    return 16;
+  }();
+  return ks;
+}
+
+// dynamic LDS above 64 KiB has to be enabled once per kernel
+template <typename F>
+static hipError_t allow_smem(F* kern, size_t bytes) {
+  if (bytes <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes);
+}
+
+template <typename T, bool BETA, bool BF, int KS>
+static hipError_t launch_proj_ks(dim3 grid, hipStream_t s, RowLoader<T, BETA> ld, const cx<T>* B, int ldb,
+                                 int M, int N, int K, int kchunk, cx<T>* part) {
+  constexpr size_t sm = gemm_smem<T, KS>();
+  static hipError_t attr = allow_smem(&k_gemm_proj<T, BETA, BF, KS>, sm);
+  if (attr != hipSuccess) return attr;
+  hipLaunchKernelGGL((k_gemm_proj<T, BETA, BF, KS>), grid, dim3(GT), sm, s, ld, B, ldb, M, N, K, kchunk, part);
+  return hipGetLastError();
+}
+
+template <typename T, bool BETA, bool BF>
+static hipError_t launch_proj(int ks, dim3 grid, hipStream_t s, RowLoader<T, BETA> ld, const cx<T>* B,
+                              int ldb, int M, int N, int K, int kchunk, cx<T>* part) {
+  if (ks == 32) return launch_proj_ks<T, BETA, BF, 32>(grid, s, ld, B, ldb, M, N, K, kchunk, part);
+  if constexpr (sizeof(T) == 4)
+    if (ks == 64) return launch_proj_ks<T, BETA, BF, 64>(grid, s, ld, B, ldb, M, N, K, kchunk, part);
+  return launch_proj_ks<T, BETA, BF, 16>(grid, s, ld, B, ldb, M, N, K, kchunk, part);
+}
+
+template <typename T, bool BETA, bool BF, bool QCM, int KS>
+static hipError_t launch_expand_ks(dim3 grid, hipStream_t s, RowLoader<T, BETA> ld, const cx<T>* Q, int ldq,
+                                   int J, int N, int K, cx<T>* out, int ldo, int acc) {
+  constexpr size_t sm = gemm_smem<T, KS>();
+  static hipError_t attr = allow_smem(&k_gemm_expand<T, BETA, BF, QCM, KS>, sm);
+  if (attr != hipSuccess) return attr;
+  hipLaunchKernelGGL((k_gemm_expand<T, BETA, BF, QCM, KS>), grid, dim3(GT), sm, s, ld, Q, ldq, J, N, K, out,
+                     ldo, acc);
+  return hipGetLastError();
+}
+
+template <typename T, bool BETA, bool BF, bool QCM>
+static hipError_t launch_expand(int ks, dim3 grid, hipStream_t s, RowLoader<T, BETA> ld, const cx<T>* Q,
+                                int ldq, int J, int N, int K, cx<T>* out, int ldo, int acc) {
+  if (ks == 32) return launch_expand_ks<T, BETA, BF, QCM, 32>(grid, s, ld, Q, ldq, J, N, K, out, ldo, acc);
+  if constexpr (sizeof(T) == 4)
+    if (ks == 64) return launch_expand_ks<T, BETA, BF, QCM, 64>(grid, s, ld, Q, ldq, J, N, K, out, ldo, acc);
+  return launch_expand_ks<T, BETA, BF, QCM, 16>(grid, s, ld, Q, ldq, J, N, K, out, ldo, acc);
+}
+
 template <typename T, bool BETA>
 static hipError_t run_proj(RowLoader<T, BETA> ld, const cx<T>* B, int ldb, int M, int N, int K,
                            cx<T>* part, size_t part_elems, bool bf16, hipStream_t s, int* S_out) {
+  const int ks = gemm_ks<T>();
   const int tiles = ((M + BT - 1) / BT) * ((N + BT - 1) / BT);
   int S = choose_split(tiles, K, part_elems, (size_t)M * N);
   int kchunk = (K + S - 1) / S;
-  kchunk = ((kchunk + BKS - 1) / BKS) * BKS;
+  kchunk = ((kchunk + ks - 1) / ks) * ks;
   S = (K + kchunk - 1) / kchunk;
   if ((size_t)S * M * N > part_elems) return hipErrorInvalidValue;
   dim3 grid(tiles, S);
+  hipError_t e;
   if constexpr (std::is_same<T, float>::value) {
-    if (bf16) {
-      hipLaunchKernelGGL((k_gemm_proj<T, BETA, true>), grid, dim3(GT), 0, s, ld, B, ldb, M, N, K, kchunk, part);
-    } else {
-      hipLaunchKernelGGL((k_gemm_proj<T, BETA, false>), grid, dim3(GT), 0, s, ld, B, ldb, M, N, K, kchunk, part);
-    }
+    e = bf16 ? launch_proj<T, BETA, true>(ks, grid, s, ld, B, ldb, M, N, K, kchunk, part)
+             : launch_proj<T, BETA, false>(ks, grid, s, ld, B, ldb, M, N, K, kchunk, part);
   } else {
-    hipLaunchKernelGGL((k_gemm_proj<T, BETA, false>), grid, dim3(GT), 0, s, ld, B, ldb, M, N, K, kchunk, part);
+    e = launch_proj<T, BETA, false>(ks, grid, s, ld, B, ldb, M, N, K, kchunk, part);
   }
   *S_out = S;
-  return hipGetLastError();
+  return e;
 }
 
 template <typename T>
 hipError_t gemm_proj(const BetaGen& bg, const cx<T>* X, int ldx, int nb, int K, cx<T>* C,
                      cx<T>* part, size_t part_elems, int mode, hipStream_t s) {
-  RowLoader<T, true> ld{};
-  ld.E = static_cast<const cx<T>*>(bg.E);
-  ld.P = static_cast<const T*>(bg.P);
-  ld.a0 = bg.a0; ld.lm = bg.lm; ld.ld = bg.ld;
-  const int M = bg.na * bg.lm;
+  RowLoader<T, false> ld{};
+  ld.A = static_cast<const cx<T>*>(bg.B);
+  ld.lda = bg.ld;
+  const int M = bg.rows;
   int S = 0;
-  hipError_t e = run_proj<T, true>(ld, X, ldx, M, nb, K, part, part_elems, mode == PREC_BF16X3, s, &S);
+  hipError_t e = run_proj<T, false>(ld, X, ldx, M, nb, K, part, part_elems, mode == PREC_BF16X3, s, &S);
   if (e != hipSuccess) return e;
   const int tot = M * nb;
   hipLaunchKernelGGL((k_reduce_parts<T, T>), dim3((tot + 63) / 64), dim3(256), 0, s, part, S, M,
@@ -437,20 +522,17 @@ hipError_t gemm_proj(const BetaGen& bg, const cx<T>* X, int ldx, int nb, int K, 
 template <typename T>
 hipError_t gemm_expand(const BetaGen& bg, const cx<T>* Q, int nb, int K, cx<T>* out, int ldo,
                        bool accumulate, int mode, hipStream_t s) {
-  RowLoader<T, true> ld{};
-  ld.E = static_cast<const cx<T>*>(bg.E);
-  ld.P = static_cast<const T*>(bg.P);
-  ld.a0 = bg.a0; ld.lm = bg.lm; ld.ld = bg.ld;
-  const int J = bg.na * bg.lm;
+  RowLoader<T, false> ld{};
+  ld.A = static_cast<const cx<T>*>(bg.B);
+  ld.lda = bg.ld;
+  const int J = bg.rows;
   dim3 grid((K + BT - 1) / BT, (nb + BT - 1) / BT);
+  const int ks = gemm_ks<T>();
   if constexpr (std::is_same<T, float>::value) {
-    if (mode == PREC_BF16X3) {
-      hipLaunchKernelGGL((k_gemm_expand<T, true, true, false>), grid, dim3(GT), 0, s, ld, Q, nb, J, nb, K, out, ldo, accumulate ? 1 : 0);
-      return hipGetLastError();
-    }
+    if (mode == PREC_BF16X3)
+      return launch_expand<T, false, true, false>(ks, grid, s, ld, Q, nb, J, nb, K, out, ldo, accumulate ? 1 : 0);
   }
-  hipLaunchKernelGGL((k_gemm_expand<T, true, false, false>), grid, dim3(GT), 0, s, ld, Q, nb, J, nb, K, out, ldo, accumulate ? 1 : 0);
-  return hipGetLastError();
+  return launch_expand<T, false, false, false>(ks, grid, s, ld, Q, nb, J, nb, K, out, ldo, accumulate ? 1 : 0);
 }
 
 hipError_t gemm_cn_f64(const cx<double>* A, int lda, int M, const cx<double>* B, int ldb, int N,
@@ -472,9 +554,7 @@ hipError_t gemm_nn_f64(const cx<double>* A, int lda, int J, const cx<double>* Q,
   RowLoader<double, false> ld{};
   ld.A = A; ld.lda = lda;
   dim3 grid((K + BT - 1) / BT, (N + BT - 1) / BT);
-  hipLaunchKernelGGL((k_gemm_expand<double, false, false, true>), grid, dim3(GT), 0, s, ld, Q, ldq, J, N, K,
-                     out, ldo, 0);
-  return hipGetLastError();
+  return launch_expand<double, false, false, true>(gemm_ks<double>(), grid, s, ld, Q, ldq, J, N, K, out, ldo, 0);
 }
 
 // scale_proj: Q[(a,l)][b] = wl[l] * C[(a,l)][b]

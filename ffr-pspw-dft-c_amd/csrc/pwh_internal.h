@@ -9,7 +9,8 @@
 //   * W: the z-transformed columns between the passes, [nb][nr3][ncol_pad].
 //   * V: the local potential on the grid, [nr3][nr2][nr1] (x fastest), like the
 //     reference's rho argument.
-//   * E: structure-factor phases EIGR, [nat][ngw_pad]; P: PKG_A, [nsp][4][ngw_pad].
+//   * beta: the projectors EIGR(G,a) * PKG_A(l,G), [P][ngw_pad] (P = sum over species of
+//     na * lm rows, row (a, l) of a species at a * lm + l), in FP64 and FP32.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstddef>
@@ -90,10 +91,10 @@ hipError_t launch_fft_lines(cx<double>* data, int n, int inner, int nlines, cons
 
 // GEMM launchers (pwh_gemm.hip)
 struct BetaGen {
-  // beta[(a,l)][k] = E[a][k] * P[l][k] for atoms a0..a0+na-1 and l < lm
-  const void* E;   // cx<T> [nat][ld]
-  const void* P;   // T [lm][ld] (this species)
-  int a0, na, lm, ld;
+  // the projectors of one species, beta[(a,l)][k] = EIGR(k,a) * PKG_A(l,k), stored
+  // row-major [rows = na * lm][ld] (row (a, l) at a * lm + l)
+  const void* B;   // cx<T>
+  int rows, lm, ld;
 };
 template <typename T>
 hipError_t gemm_proj(const BetaGen& bg, const cx<T>* X, int ldx, int nb, int K, cx<T>* C,
