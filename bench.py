@@ -50,6 +50,49 @@ def kernel_bytes(s, info, nb, prec_bytes):
     }
 
 
+# real flops of one butterfly of the plane pass's Stockham stages: full DFT of radix R,
+# first stage with the out-of-band inputs known zero ("in"), last stage keeping the band
+# outputs only ("out"); twiddles cost 6 flops per non-trivial complex product
+_DFT_FLOPS = {2: 4, 3: 16, 4: 16, 5: 48, 8: 56}
+_DFT_FLOPS_IN = {4: 8, 8: 40}
+_DFT_FLOPS_OUT = {4: 12, 8: 48}
+
+
+def _radices(n):
+    out, rem = [], n
+    while rem > 1:
+        r = 8 if (rem % 8 == 0 and rem != 16) else 4 if rem % 4 == 0 else 2 if rem % 2 == 0 \
+            else 3 if rem % 3 == 0 else 5
+        out.append(r)
+        rem //= r
+    return out
+
+
+def _transform_flops(n, prune, reverse):
+    rads = _radices(n)[::-1] if reverse else _radices(n)
+    tot, ns = 0, 1
+    for i, r in enumerate(rads):
+        nbf = n // r
+        f = _DFT_FLOPS[r]
+        if prune == "in" and i == 0:
+            f = _DFT_FLOPS_IN.get(r, f)
+        if prune == "out" and i == len(rads) - 1:
+            f = _DFT_FLOPS_OUT.get(r, f)
+        tw = 6 * (r - 1) if ns > 1 else 0
+        tot += nbf * (f + tw)
+        ns *= r
+    return tot
+
+
+def plane_flops(s, nb):
+    """FFT + V arithmetic of one band-pruned plane pass (k_plane_band) over nb bands."""
+    n = s.nr[0]
+    cols = n // 2
+    per_plane = (cols * _transform_flops(n, "in", False) + n * _transform_flops(n, "in", False)
+                 + 2 * n * n + n * _transform_flops(n, "out", True) + cols * _transform_flops(n, "out", True))
+    return per_plane * s.nr[2] * nb
+
+
 def gemm_flops(s, nb):
     nproj = 4 * s.natoms
     return 8.0 * nproj * s.ngw * nb      # one complex GEMM: 8 flops per complex MAC
@@ -241,6 +284,14 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(dom, main_path),
                 "algorithmic_bytes": int(kb[dom]), "avg_ms": round(prof[dom], 5)}
+        if dom == "plane" and info.get("band_xy", 1):
+            # the pass is bound by VALU issue rather than HBM: its FFT arithmetic against
+            # the FP32 vector peak without packed math (157.3 / 2 TFLOP/s)
+            fl = plane_flops(s, nb)
+            vpeak = (FP64_PEAK_TFLOPS if main_path == "fp64" else FP32_PEAK_TFLOPS / 2)
+            tf = fl / (prof[dom] * 1e-3) / 1e12
+            roof["valu"] = {"flops": int(fl), "achieved": round(tf, 2), "peak": vpeak,
+                            "unit": "TFLOP/s", "frac": round(tf / vpeak, 4)}
     else:
         fl = gemm_flops(s, nb)
         peak = FP64_PEAK_TFLOPS if main_path == "fp64" else FP32_PEAK_TFLOPS

@@ -7,7 +7,7 @@
 //   chebyfilt       chebyfilt.cpp:6-74         -> pwh_chebyfilt on NX+2 bands
 //   ortho_qr        ortho_qr.cpp:3-34          -> pwh_ortho_qr
 //   lanczos         lanczos.cpp:8-101          -> pwh_lanczos
-//   eig_zheevd      eig_zheevd.cpp:5-34        -> pwh_eig_zheevd
+//   eig_zheevd      eig_zheevd.cpp:5-69        -> pwh_eig_zheevd (both overloads)
 //   fft_fftw3       fft_fftw3.cpp:13-46        -> pwh_fft3d
 //
 // Link it in place of those .cpp files.  It reads the reference's globals
@@ -149,6 +149,13 @@ void eig_zheevd(zcomplex *A, int ldA, double *lambda, int N) {
     if (p) printf("ERROR calling zheevd_ : info = %d\n", atoi(p + 5));
     else die("eig_zheevd");
   }
+}
+
+// eig_zheevd.cpp:37-69: the overload that keeps only the eigenvectors
+void eig_zheevd(zcomplex *A, int ldA, int N) {
+  double *lambda = static_cast<double *>(malloc(sizeof(double) * (N > 0 ? N : 1)));
+  eig_zheevd(A, ldA, lambda, N);
+  free(lambda);
 }
 
 void fft_fftw3(zcomplex *data, int NR1_, int NR2_, int NR3_, bool inverse) {

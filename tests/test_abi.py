@@ -18,7 +18,7 @@ HEADER = os.path.join(ROOT, "include", "pwh.h")
 # fft_fftw3 :348), as g++ -std=gnu++98 mangles the reference's declarations
 REF_SYMBOLS = ["_Z14ApplyHam_blockPCdS0_iiPdii", "_Z8ApplyHamPCdS0_iiPdi",
                "_Z9chebyfiltiPCdiddPdi", "_Z8ortho_qrPCdii", "_Z7lanczosiPCdiPdiS0_S1_i",
-               "_Z10eig_zheevdPCdiPdi", "_Z9fft_fftw3PCdiiib"]
+               "_Z10eig_zheevdPCdiPdi", "_Z10eig_zheevdPCdii", "_Z9fft_fftw3PCdiiib"]
 
 
 def declared_functions():

@@ -168,3 +168,15 @@ def test_eig_zheevd_hard_spectra(case):
     s = max(np.abs(H).max(), 1.0)
     assert np.abs(got - np.linalg.eigvalsh(H)).max() < 1e-12 * s
     _check_eig(H, got, V, tol=1e-13)
+
+
+def test_shim_eig_zheevd_vectors_only(shim_lib):
+    """eig_zheevd.cpp:37-69: the two-argument overload leaves the eigenvectors in A."""
+    sym = "_Z10eig_zheevdPCdii"
+    getattr(shim_lib, sym).argtypes = [C.c_void_p, C.c_int, C.c_int]
+    z = np.load(os.path.join(ROOT, "tests", "golden", "ref_lapack.npz"))
+    A = z["T"]
+    n = A.shape[0]
+    buf = np.ascontiguousarray(A.T)
+    getattr(shim_lib, sym)(buf.ctypes.data, n, n)
+    _check_eig(np.triu(A) + np.triu(A, 1).conj().T, z["w"], buf.T)
