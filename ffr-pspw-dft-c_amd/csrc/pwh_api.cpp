@@ -189,6 +189,7 @@ struct pwh_system {
   // FP64 work
   Buf<cx<double>> X, HX, A64, B64, C64, W64, Hnl64, proj64, Q64, part64, stage, S, Vl;
   Buf<double> evals, rwork, dres;
+  Buf<double> rho_part, rho, rho_w, rho_q, rho_k;   // density pass (pwh_eval_rho_psi)
   Buf<rocblas_int> info;
   // FP32 work
   Buf<cx<float>> X32, Y32, Y1_32, W32, Hnl32, proj32, Q32, part32;
@@ -889,7 +890,9 @@ int pwh_system_destroy(pwh_system* s) {
   for (auto* b : {&s->Bt64, &s->X, &s->HX, &s->A64, &s->B64, &s->C64, &s->W64, &s->Hnl64, &s->proj64,
                   &s->Q64, &s->part64, &s->stage, &s->S, &s->Vl})
     b->release();
-  for (auto* b : {&s->kin64, &s->wl64, &s->V64, &s->evals, &s->rwork, &s->dres}) b->release();
+  for (auto* b : {&s->kin64, &s->wl64, &s->V64, &s->evals, &s->rwork, &s->dres, &s->rho_part, &s->rho,
+                  &s->rho_w, &s->rho_q, &s->rho_k})
+    b->release();
   for (auto* b : {&s->Bt32, &s->X32, &s->Y32, &s->Y1_32, &s->W32, &s->Hnl32, &s->proj32, &s->Q32, &s->part32})
     b->release();
   for (auto* b : {&s->kin32, &s->wl32, &s->V32}) b->release();
@@ -1026,6 +1029,51 @@ int pwh_eig_zheevd(double* A, int ldA, double* lambda, int n) {
                           sizeof(cx<double>) * n, n, hipMemcpyDeviceToHost, g.stream));
   HIPCHK(hipMemcpyAsync(lambda, w.eig_w.p, sizeof(double) * n, hipMemcpyDeviceToHost, g.stream));
   HIPCHK(hipStreamSynchronize(g.stream));
+  return PWH_OK;
+}
+
+int pwh_eval_rho_psi(pwh_system* s, const double* eigvec, int ld, int nstates, const double* w, double omega,
+                     double* rhoe, double* ekin, double* charge) {
+  CHK(check_ld(s, nstates));
+  if (!eigvec || !w || !rhoe || !ekin || !charge || ld < s->ngw || !(omega > 0.0))
+    return fail(PWH_ERR_ARG, "bad eval_rho_psi arguments");
+  const int nb = nstates, ngw = s->ngw, ldp = s->ngw_pad;
+  const size_t nnr = (size_t)s->nnr;
+  // coefficients in sphere order; the reference's loop stops at ig < NGW_ik, so the last
+  // plane wave contributes to neither the density nor the sums (EvalRhoPsi.cpp:117)
+  CHK(s->stage.ensure((size_t)ngw * nb));
+  HIPCHK(hipMemcpy2DAsync(s->stage.p, sizeof(cx<double>) * ngw, eigvec, sizeof(cx<double>) * ld,
+                          sizeof(cx<double>) * ngw, nb, hipMemcpyHostToDevice, g.stream));
+  CHK(s->X.ensure((size_t)nb * ldp));
+  HIPCHK(permute_in_f64(s->stage.p, ngw, s->d_perm.p, ngw, ldp, s->X.p, nb, g.stream));
+  int last = 0;
+  for (int k = 0; k < ngw; ++k)
+    if (s->perm[k] == ngw - 1) last = k;
+  HIPCHK(hipMemset2DAsync(s->X.p + last, sizeof(cx<double>) * ldp, 0, sizeof(cx<double>), nb, g.stream));
+  CHK(s->rho_q.ensure(nb));
+  CHK(s->rho_k.ensure(nb));
+  HIPCHK(band_sums(s->X.p, ldp, ngw, s->kin64.p, nb, s->rho_q.p, s->rho_k.p, g.stream));
+  // inverse transform and w |psi|^2 summed over bands (FP64)
+  CHK(s->W64.ensure((size_t)nb * s->nr[2] * s->maps.ncol_pad));
+  HIPCHK(launch_zinv<double>(s->maps, s->tw64[2].p, s->X.p, ldp, s->W64.p, nb, g.stream));
+  std::vector<double> wo(nb);
+  for (int b = 0; b < nb; ++b) wo[b] = w[b] / omega;
+  CHK(upload(s->rho_w, wo.data(), wo.size()));
+  const int groups = std::max(1, std::min(nb, 512 / s->nr[2]));
+  CHK(s->rho_part.ensure((size_t)groups * nnr));
+  CHK(s->rho.ensure(nnr));
+  HIPCHK(launch_plane_rho(s->maps, s->tw64[0].p, s->tw64[1].p, s->W64.p, nb, groups, s->rho_w.p,
+                          s->rho_part.p, s->rho.p, g.stream));
+  std::vector<double> rho(nnr), ks(nb);
+  HIPCHK(hipMemcpyAsync(rho.data(), s->rho.p, sizeof(double) * nnr, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipMemcpyAsync(charge, s->rho_q.p, sizeof(double) * nb, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipMemcpyAsync(ks.data(), s->rho_k.p, sizeof(double) * nb, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  for (size_t i = 0; i < nnr; ++i) rhoe[i] += rho[i];
+  // sum_b w_b/2 sum_G XKG |c|^2 TPIBA2 = sum_b w_b sum_G kin |c|^2 (kin = XKG TPIBA2 / 2)
+  double e = 0.0;
+  for (int b = 0; b < nb; ++b) e += w[b] * ks[b];
+  *ekin = e;
   return PWH_OK;
 }
 

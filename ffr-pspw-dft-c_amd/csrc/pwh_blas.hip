@@ -156,6 +156,36 @@ __global__ void k_resid(const cx<double>* __restrict__ H, const cx<double>* __re
   }
   if (t == 0) res[b] = sqrt(sh[0]);
 }
+// per band b: charge[b] = sum_k |X[b][k]|^2, ksum[b] = sum_k kin[k] |X[b][k]|^2 (one block per
+// band, fixed-order tree reduction)
+__global__ __launch_bounds__(256) void k_band_sums(const cx<double>* __restrict__ X, int ld, int n,
+                                                   const double* __restrict__ kin, double* __restrict__ charge,
+                                                   double* __restrict__ ksum) {
+  __shared__ double s1[256], s2[256];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const cx<double>* x = X + (size_t)b * ld;
+  double a = 0.0, c = 0.0;
+  for (int k = t; k < n; k += 256) {
+    const double m = x[k].x * x[k].x + x[k].y * x[k].y;
+    a += m;
+    c += kin[k] * m;
+  }
+  s1[t] = a;
+  s2[t] = c;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) { s1[t] += s1[t + h]; s2[t] += s2[t + h]; }
+    __syncthreads();
+  }
+  if (t == 0) { charge[b] = s1[0]; ksum[b] = s2[0]; }
+}
+
+hipError_t band_sums(const cx<double>* X, int ld, int n, const double* kin, int nb, double* charge,
+                     double* ksum, hipStream_t s) {
+  hipLaunchKernelGGL(k_band_sums, dim3(nb), dim3(256), 0, s, X, ld, n, kin, charge, ksum);
+  return hipGetLastError();
+}
+
 hipError_t resid_norms(const cx<double>* H, const cx<double>* X, const double* lam, int ld, int n,
                        int nb, double* res, hipStream_t s) {
   hipLaunchKernelGGL(k_resid, dim3(nb), dim3(256), 0, s, H, X, lam, ld, n, res);

@@ -528,6 +528,90 @@ __global__ __launch_bounds__(NT) void k_plane_band(SphereMaps m, const cx<T>* __
 }
 
 // ---------------------------------------------------------------------------
+// Density pass (EvalRhoPsi.cpp:138-150): per (i3, band group) plane, inverse FFT along i2
+// on the active columns and along i1, then sum w_b |psi_b(r)|^2 over the group's bands
+// ---------------------------------------------------------------------------
+struct XLiveHooks {
+  const unsigned char* act;
+  __device__ __forceinline__ bool live_in(int, int e) const { return act[e] != 0; }
+  __device__ __forceinline__ bool live_out(int, int) const { return true; }
+  template <typename V>
+  __device__ __forceinline__ void post(int, int, V&) const {}
+};
+
+constexpr int kRhoThreads = 256;
+
+// part[g][i3][i2][i1] = sum over bands b = g, g + groups, ... of wts[b] |psi_b|^2 (FP64)
+template <int N1, int N2>
+__global__ __launch_bounds__(kRhoThreads) void k_plane_rho(SphereMaps m, const cx<double>* __restrict__ tw1g,
+                                                           const cx<double>* __restrict__ tw2g,
+                                                           const cx<double>* __restrict__ W, int nb, int groups,
+                                                           const double* __restrict__ wts,
+                                                           double* __restrict__ part) {
+  constexpr int NT = kRhoThreads;
+  constexpr int S1 = plane_stride<N1>();
+  constexpr int PTS = (N1 * N2 + NT - 1) / NT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<double>* tile = reinterpret_cast<cx<double>*>(smem);
+  cx<double>* tw1 = tile + N2 * S1;
+  cx<double>* tw2 = tw1 + N1;
+  int* xa = reinterpret_cast<int*>(tw2 + N2);
+  unsigned char* xact = reinterpret_cast<unsigned char*>(xa + N1);
+  const int tid = threadIdx.x;
+  const int z = blockIdx.x, g0 = blockIdx.y;
+  const int nxa = m.nxa;
+  for (int i = tid; i < N1; i += NT) { tw1[i] = tw1g[i]; xact[i] = 0; }
+  for (int i = tid; i < N2; i += NT) tw2[i] = tw2g[i];
+  __syncthreads();
+  for (int i = tid; i < nxa; i += NT) { const int x = m.xa[i]; xa[i] = x; xact[x] = 1; }
+  double acc[PTS];
+#pragma unroll
+  for (int q = 0; q < PTS; ++q) acc[q] = 0.0;
+  const XLiveHooks hk{xact};
+  for (int b = g0; b < nb; b += groups) {
+    __syncthreads();
+    for (int i = tid; i < nxa * N2; i += NT) {
+      const int y = i / nxa, c = i - y * nxa;
+      tile[y * S1 + m.xa[c]] = mk<double>(0, 0);
+    }
+    __syncthreads();
+    const cx<double>* wrow = W + ((size_t)b * m.nr3 + z) * m.ncol_pad;
+    for (int c = tid; c < m.ncol; c += NT) {
+      const int xy = m.colxy[c];
+      tile[(xy / N1) * S1 + xy % N1] = wrow[c];
+    }
+    __syncthreads();
+    lds_fft<double, N2, +1, NT, N1, true>(tile, nxa, S1, [&](int t) { return xa[t]; }, tw2, tid);
+    lds_fft<double, N1, +1, NT, N2, true>(tile, N2, 1, [](int t) { return t * S1; }, tw1, tid, hk);
+    const double wb = wts[b];
+#pragma unroll
+    for (int q = 0; q < PTS; ++q) {
+      const int idx = tid + q * NT;
+      if (idx < N1 * N2) {
+        const int y = idx / N1, x = idx - y * N1;
+        const cx<double> v = tile[y * S1 + x];
+        acc[q] += wb * (v.x * v.x + v.y * v.y);
+      }
+    }
+  }
+  double* out = part + ((size_t)g0 * m.nr3 + z) * (size_t)(N1 * N2);
+#pragma unroll
+  for (int q = 0; q < PTS; ++q) {
+    const int idx = tid + q * NT;
+    if (idx < N1 * N2) out[idx] = acc[q];
+  }
+}
+
+// rho[i] = sum_g part[g][i], groups summed in order (deterministic)
+__global__ void k_sum_groups(const double* __restrict__ part, int groups, size_t n, double* __restrict__ rho) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int g = 0; g < groups; ++g) s += part[(size_t)g * n + i];
+  rho[i] = s;
+}
+
+// ---------------------------------------------------------------------------
 // Pass 2 with the 1D transforms in registers: two lanes per transform
 // ---------------------------------------------------------------------------
 // exp(SIGN 2 pi i e / N) for an e that is a compile-time constant after unrolling:
@@ -916,6 +1000,44 @@ hipError_t launch_zfwd_epilogue(const SphereMaps& m, const void* tw3, const cx<T
 #undef PWH_CASE
     default: return hipErrorInvalidValue;
   }
+}
+
+template <int N>
+static hipError_t launch_plane_rho_n(const SphereMaps& m, const void* tw1, const void* tw2, const cx<double>* W,
+                                     int nb, int groups, const double* wts, double* part, hipStream_t s) {
+  constexpr size_t smem = plane_smem<double, N, N>();
+  if constexpr (smem > 160 * 1024) {
+    return hipErrorInvalidValue;
+  } else {
+    static bool attr = false;
+    if (!attr) {
+      hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_rho<N, N>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+      if (e != hipSuccess) return e;
+      attr = true;
+    }
+    hipLaunchKernelGGL((k_plane_rho<N, N>), dim3(m.nr3, groups), dim3(kRhoThreads), smem, s, m,
+                       static_cast<const cx<double>*>(tw1), static_cast<const cx<double>*>(tw2), W, nb,
+                       groups, wts, part);
+    return hipGetLastError();
+  }
+}
+
+hipError_t launch_plane_rho(const SphereMaps& m, const void* tw1, const void* tw2, const cx<double>* W, int nb,
+                            int groups, const double* wts, double* part, double* rho, hipStream_t s) {
+  if (!plane_supported(m.nr1, m.nr2) || groups < 1) return hipErrorInvalidValue;
+  hipError_t e = hipErrorInvalidValue;
+  switch (m.nr1) {
+#define PWH_CASE(N) \
+  case N: e = launch_plane_rho_n<N>(m, tw1, tw2, W, nb, groups, wts, part, s); break;
+    PWH_FFT_SIZES(PWH_CASE)
+#undef PWH_CASE
+    default: return hipErrorInvalidValue;
+  }
+  if (e != hipSuccess) return e;
+  const size_t n = (size_t)m.nnr;
+  hipLaunchKernelGGL(k_sum_groups, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part, groups, n, rho);
+  return hipGetLastError();
 }
 
 hipError_t launch_fft_lines(cx<double>* data, int n, int inner, int nlines, const cx<double>* tw,

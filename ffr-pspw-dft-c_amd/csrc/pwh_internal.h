@@ -84,6 +84,10 @@ size_t heev_small_scratch_doubles();
 constexpr int kHeevSmallMax = 128;
 hipError_t launch_heev_small(cx<double>* A, int lda, int n, double* w, int* info, double* scratch,
                              hipStream_t s);
+// Density of a block in sphere order (FP64): W (after launch_zinv) -> rho[nr3][nr2][nr1] =
+// sum_b wts[b] |psi_b(r)|^2, the bands split over `groups` partial grids part[groups][nnr].
+hipError_t launch_plane_rho(const SphereMaps& m, const void* tw1, const void* tw2, const cx<double>* W, int nb,
+                            int groups, const double* wts, double* part, double* rho, hipStream_t s);
 // Dense FP64 transform of length n along an axis of stride `inner` for every line of an
 // [outer][n][inner] array, in place; tw[m] = exp(+2 pi i m / n); sign +1 inverse, -1 forward.
 hipError_t launch_fft_lines(cx<double>* data, int n, int inner, int nlines, const cx<double>* tw,

@@ -41,6 +41,7 @@ EXPORTS = [
     "pwh_resident_apply", "pwh_resident_chebyfilt", "pwh_synchronize", "pwh_timing_enable",
     "pwh_timing_reset", "pwh_timing_get", "pwh_batch_create", "pwh_batch_destroy",
     "pwh_batch_set", "pwh_batch_get", "pwh_batch_chebyfilt", "pwh_fft3d", "pwh_eig_zheevd",
+    "pwh_eval_rho_psi",
 ]
 
 
@@ -123,6 +124,7 @@ def lib():
         "pwh_batch_chebyfilt": [vp, C.c_int, _dp, _dp, C.c_int, C.c_int],
         "pwh_fft3d": [vp, C.c_int, C.c_int, C.c_int, C.c_int],
         "pwh_eig_zheevd": [vp, C.c_int, _dp, C.c_int],
+        "pwh_eval_rho_psi": [vp, vp, C.c_int, C.c_int, _dp, C.c_double, _dp, C.POINTER(C.c_double), _dp],
     }
     for name, args in proto.items():
         f = getattr(L, name)
@@ -351,6 +353,19 @@ class System:
         _check(lib().pwh_chebyfilt(self.h, _zptr(X), X.shape[0], int(degree), float(lb), float(ub),
                                    _prec(precision)))
         return X
+
+    def eval_rho_psi(self, eigvec, w, omega, rhoe=None, ld=None):
+        """EvalRhoPsi.cpp:83-157 on eigvec (nstates, >= ngw) rows = states; w per state.
+        Returns (rhoe + density [nnr], EKIN increment, charge per state)."""
+        ev = np.ascontiguousarray(eigvec, dtype=np.complex128)
+        nst, ldv = ev.shape
+        rho = np.zeros(self.nnr) if rhoe is None else np.array(rhoe, dtype=np.float64)
+        charge = np.zeros(nst)
+        ek = C.c_double(0.0)
+        _check(lib().pwh_eval_rho_psi(self.h, _zptr(ev), ldv if ld is None else ld, nst,
+                                      np.ascontiguousarray(w, dtype=np.float64), float(omega), rho,
+                                      C.byref(ek), charge))
+        return rho, ek.value, charge
 
     def lanczos(self, v0, k):
         """lanczos.cpp:8-101.  Returns (T[k,k] as the reference fills it, f, v0 normalised)."""

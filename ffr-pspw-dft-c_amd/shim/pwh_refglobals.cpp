@@ -13,3 +13,8 @@ double TPIBA2 = 0.0;
 double *WNL = 0;
 double *PKG_A = 0;
 zcomplex *EIGR = 0;
+double OMEGA = 0.0;
+double *WKPT = 0;
+double *FOCC = 0;
+double EKIN = 0.0;
+zcomplex *C0 = 0;

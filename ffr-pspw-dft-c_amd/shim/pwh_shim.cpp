@@ -9,6 +9,7 @@
 //   lanczos         lanczos.cpp:8-101          -> pwh_lanczos
 //   eig_zheevd      eig_zheevd.cpp:5-69        -> pwh_eig_zheevd (both overloads)
 //   fft_fftw3       fft_fftw3.cpp:13-46        -> pwh_fft3d
+//   EvalRhoPsi      EvalRhoPsi.cpp:5-157       -> pwh_eval_rho_psi (both overloads)
 //
 // Link it in place of those .cpp files.  It reads the reference's globals
 // (global_variables.cpp) and keeps one device system per k-point, rebuilt when the
@@ -22,6 +23,7 @@
 // reference does after zheevd_.
 //
 // Compiled as gnu++98 so `double _Complex` is the reference's `double complex`.
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -36,6 +38,9 @@ extern int *NGW, *NA, *L_MAX, *L_LOC;
 extern unsigned int *N123;
 extern double *XKG, TPIBA2, *WNL, *PKG_A;
 extern zcomplex *EIGR;
+// read and written by EvalRhoPsi (EvalRhoPsi.cpp:20-31,154)
+extern double OMEGA, *WKPT, *FOCC, EKIN;
+extern zcomplex *C0;
 
 namespace {
 
@@ -161,4 +166,44 @@ void eig_zheevd(zcomplex *A, int ldA, int N) {
 void fft_fftw3(zcomplex *data, int NR1_, int NR2_, int NR3_, bool inverse) {
   if (pwh_fft3d(reinterpret_cast<double *>(data), NR1_, NR2_, NR3_, inverse ? 1 : 0) != PWH_OK)
     die("fft_fftw3");
+}
+
+// EvalRhoPsi.cpp:83-157.  C0(:, ist, ik) receives eigvec's NGWX entries per state exactly
+// as the reference copies them (:108-115).  The charge of each state (coefficients 1 ..
+// NGW_ik-1, :117-126) is checked on the host first, so a state off by more than 6e-3
+// stops the program with the reference's message (:129-133) before any GPU work; the
+// density, the charges and the kinetic sums are then formed on the GPU, and EKIN gets the
+// kinetic increment (:154).
+void EvalRhoPsi(int ik, double *rhoe, zcomplex *eigvec, int ldEigvec, int NGW_ik) {
+  for (int ist = 0; ist < NX; ++ist)
+    for (int ig = 0; ig < NGWX; ++ig)
+      C0[ig + (size_t)NGWX * (ist + (size_t)NX * (ik - 1))] = eigvec[ig + (size_t)ldEigvec * ist];
+  for (int ist = 0; ist < NX; ++ist) {
+    const double *c = reinterpret_cast<const double *>(eigvec + (size_t)ldEigvec * ist);
+    double ss = 0.0;
+    for (int ig = 0; ig + 1 < NGW_ik; ++ig) ss += c[2 * ig] * c[2 * ig] + c[2 * ig + 1] * c[2 * ig + 1];
+    if (fabs(ss - 1.0) > 6.e-3) {
+      printf("Wrong charge density in G-space: %18.10f\n", ss);
+      printf("for k-point %d and states %d\n", ik, ist + 1);
+      fflush(stdout);
+      abort();
+    }
+  }
+  if (NGW_ik != NGW[ik - 1]) die("EvalRhoPsi: NGW_ik differs from NGW(ik)");
+  pwh_system *s = system_for(ik);
+  double *w = static_cast<double *>(malloc(sizeof(double) * NX));
+  double *charge = static_cast<double *>(malloc(sizeof(double) * NX));
+  for (int ist = 0; ist < NX; ++ist) w[ist] = WKPT[ik - 1] * FOCC[ist + (size_t)NX * (ik - 1)];
+  double ekin = 0.0;
+  if (pwh_eval_rho_psi(s, reinterpret_cast<const double *>(eigvec), ldEigvec, NX, w, OMEGA, rhoe, &ekin,
+                       charge) != PWH_OK)
+    die("EvalRhoPsi");
+  EKIN = EKIN + ekin;
+  free(w);
+  free(charge);
+}
+
+// EvalRhoPsi.cpp:5-79: the same with ldEigvec = NGWX
+void EvalRhoPsi(int ik, double *rhoe, zcomplex *eigvec, int NGW_ik) {
+  EvalRhoPsi(ik, rhoe, eigvec, NGWX, NGW_ik);
 }

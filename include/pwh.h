@@ -109,6 +109,17 @@ int pwh_rayleigh_ritz(pwh_system *sys, double *X, int nstates, double *eig);
 /* Residual report, ChebySCFLoop.cpp:415-433: res[j] = || H x_j - X (X^H H X)_sym e_j ||. */
 int pwh_residual_norms(pwh_system *sys, const double *X, int nstates, double *res);
 
+/* ---- density from the wave functions ----------------------------------------- */
+/* EvalRhoPsi.cpp:83-157 on the GPU (FP64): for nstates states of eigvec [ld x nstates]
+ * (reference order, ld >= ngw), rhoe[nnr] += sum_ist w[ist] / omega |psi_ist(r)|^2 with
+ * psi(r) the unscaled inverse FFT of coefficients 1 .. ngw-1 (the reference's loop stops
+ * one short, :117); w[ist] = WKPT(ik) * FOCC(ist, ik).  charge[ist] = sum |c|^2 over the
+ * same coefficients (the reference aborts when |charge - 1| > 6e-3, :129-133; the caller
+ * checks); *ekin = sum_ist w[ist] / 2 * sum XKG |c|^2 * TPIBA2, the reference's EKIN
+ * increment (:154). */
+int pwh_eval_rho_psi(pwh_system *sys, const double *eigvec, int ld, int nstates, const double *w,
+                     double omega, double *rhoe, double *ekin, double *charge);
+
 /* ---- adjacent dense routines (no system needed) ------------------------------ */
 /* fft_fftw3.cpp:13-46: in-place 3D DFT of data [nr3][nr2][nr1] complex (x fastest).
  * inverse != 0: exp(+i G.r), unscaled (FFTW_BACKWARD); inverse == 0: exp(-i G.r) then
@@ -116,8 +127,9 @@ int pwh_residual_norms(pwh_system *sys, const double *X, int nstates, double *re
  * compiled plan (2,3,5-smooth, 12..128), else PWH_ERR_UNSUPPORTED. */
 int pwh_fft3d(double *data, int nr1, int nr2, int nr3, int inverse);
 /* eig_zheevd.cpp:5-34: Hermitian eigensolve of A [ldA x n] column-major, upper triangle
- * referenced ('V','U'); A <- eigenvectors, lambda [n] ascending.  FP64 on the GPU
- * (rocSOLVER); info != 0 -> PWH_ERR_LAPACK. */
+ * referenced ('V','U'); A <- eigenvectors, lambda [n] ascending.  FP64 on the GPU: the
+ * library's own solver for n <= 128 (rocSOLVER zheevd when it flags a non-finite value),
+ * rocSOLVER zheevd above; info != 0 -> PWH_ERR_LAPACK. */
 int pwh_eig_zheevd(double *A, int ldA, double *lambda, int n);
 
 /* ---- ChFSI outer loop (eigen part of ChebySCFLoop.cpp:271-311) -------------- */
@@ -138,7 +150,9 @@ int pwh_chfsi(pwh_system *sys, double *X, int nstates, double *v0, pwh_chfsi_opt
               double *eig, double *res);
 
 /* ---- device-resident entry points (inputs already in HBM; used for timing) --- */
-/* Copy a block into / out of the system's resident FP64 wave-function buffer. */
+/* Copy a block into / out of the system's resident FP64 wave-function buffer.  The
+ * host-pointer entry points above stage their input in that same buffer, so a call to
+ * any of them replaces the resident block. */
 int pwh_resident_set(pwh_system *sys, const double *X, int nstates);
 int pwh_resident_get(pwh_system *sys, double *X, int nstates);
 /* One H|psi> over the resident block (nb bands) in the given precision. */

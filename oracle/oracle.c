@@ -679,3 +679,51 @@ int orc_apply_ham_block_par(const orc_sys *S, const zc *vin, zc *vout, int ngw_i
                               rho, nnr, 1);
   return rc;
 }
+
+/*
+ * EvalRhoPsi.cpp:83-157 (the ldEigvec form; the NGWX form at :5-79 is the same with
+ * ld = NGWX).  For each of nstates states: scatter coefficients ig = 1 .. NGW_ik-1 (the
+ * reference's loop stops one short of NGW_ik, :117) onto the grid through
+ * IGK -> (N1,N2,N3), which equals N123 - 1 (GkVectorsGen.cpp:30); charge ss = sum |c|^2
+ * and sk = sum XKG |c|^2 over the same range; inverse FFT (unscaled, e^{+i});
+ * rhoe += |c_fft|^2 * w / omega, with w = WKPT(ik) * FOCC(ist, ik) passed per state.
+ * ekin_out = sum_states sk * w / 2 * TPIBA2 (what the reference adds to EKIN, :154).
+ * charge[ist] = ss; returns the 0-based index of the first state with |ss - 1| > 6e-3
+ * (where the reference prints and aborts, :129-133), or -1.  That state's density is
+ * not added, as in the reference.
+ * Pinned by: a single plane wave (|psi(r)|^2 constant), two plane waves (closed-form
+ * cosine), Parseval (sum_r |psi|^2 = NNR sum_G |c|^2), the skipped last coefficient.
+ */
+int orc_eval_rho_psi(const orc_sys *S, int ik, const zc *eigvec, int ld, int ngw_ik, int nstates,
+                     const double *w, double omega, double *rhoe, double *ekin_out, double *charge)
+{
+  const unsigned int *n123 = S->n123 + (size_t)S->ngwx*(ik-1);
+  const double *xkg = S->xkg + (size_t)S->ngwx*(ik-1);
+  zc *c_fft = (zc*)malloc(sizeof(zc)*(size_t)S->nnr);
+  double xkin = 0.0;
+  int bad = -1;
+  for (int ist = 0; ist < nstates; ist++) {
+    const zc *c = eigvec + (size_t)ld*ist;
+    const double wdotfbo = w[ist] / omega, wdot2fb = w[ist] * 0.5;
+    double ss = 0.0, sk = 0.0;
+    memset(c_fft, 0, sizeof(zc)*(size_t)S->nnr);
+    for (int ig = 0; ig < ngw_ik - 1; ig++) {
+      c_fft[n123[ig]-1] = c[ig];
+      const double a = cabs(c[ig]);
+      const double scg = a*a;
+      ss += scg;
+      sk += xkg[ig]*scg;
+    }
+    charge[ist] = ss;
+    if (fabs(ss - 1.0) > 6.e-3 && bad < 0) { bad = ist; break; }
+    xkin += sk*wdot2fb;
+    orc_fft3d(c_fft, S->nr1, S->nr2, S->nr3, 1);
+    for (int i = 0; i < S->nnr; i++) {
+      const double a = cabs(c_fft[i]);
+      rhoe[i] += a*a*wdotfbo;
+    }
+  }
+  *ekin_out = xkin*S->tpiba2;
+  free(c_fft);
+  return bad;
+}

@@ -61,6 +61,9 @@ def lib():
         L.orc_phase_factor.argtypes = [_dp, _ip, C.c_int, _ip, C.c_int, _dp, C.c_int, _ip, C.c_int,
                                        _ip, C.c_int, _ip, _zp]
         L.orc_fft3d.argtypes = [_zp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_eval_rho_psi.restype = C.c_int
+        L.orc_eval_rho_psi.argtypes = [C.c_void_p, C.c_int, _zp, C.c_int, C.c_int, C.c_int, _dp,
+                                       C.c_double, _dp, C.POINTER(C.c_double), _dp]
         L.orc_apply_ham_block.restype = C.c_int
         L.orc_apply_ham_block.argtypes = [C.c_void_p, _zp, _zp, C.c_int, C.c_int, _dp, C.c_int, C.c_int]
         L.orc_apply_ham_block_par.restype = C.c_int
@@ -211,6 +214,21 @@ def apply_ham_block(g: Globals, vin, rho, ngw, ik=1, nthreads=1):
     if rc != 0:
         raise RuntimeError("ApplyHam_block: reference abort condition (nnr != NNR or lm_end > 4)")
     return vout
+
+
+def eval_rho_psi(g: Globals, eigvec, ngw, w, omega, rhoe=None, ik=1):
+    """EvalRhoPsi.cpp:83-157.  eigvec: (nstates, ld) rows = states (column-major, ldEigvec = ld);
+    w[ist] = WKPT(ik) * FOCC(ist, ik).  Returns (rhoe + density, EKIN increment, charge per
+    state, first state whose charge is off by more than 6e-3 or -1)."""
+    ev = np.ascontiguousarray(eigvec, dtype=np.complex128)
+    nst, ld = ev.shape
+    rho = np.zeros(g.nnr) if rhoe is None else np.array(rhoe, dtype=np.float64)
+    charge = np.zeros(nst)
+    ek = C.c_double(0.0)
+    bad = lib().orc_eval_rho_psi(g.cstruct(), ik, ev.reshape(-1), ld, ngw, nst,
+                                 np.ascontiguousarray(w, dtype=np.float64), float(omega), rho,
+                                 C.byref(ek), charge)
+    return rho, ek.value, charge, bad
 
 
 def chebyfilt(g: Globals, X, degree, lb, ub, rho, ngw, ik=1):

@@ -14,11 +14,13 @@ from conftest import PKG, ROOT
 HEADER = os.path.join(ROOT, "include", "pwh.h")
 
 # Itanium-mangled names of the reference's hot-path routines (common_pspw.h:176-178,187,222)
-# and the adjacent ones the shim also replaces (lanczos :185-186, eig_zheevd :217,
-# fft_fftw3 :348), as g++ -std=gnu++98 mangles the reference's declarations
+# and the adjacent ones the shim also replaces (lanczos :185-186, eig_zheevd :217 and its
+# eig_zheevd.cpp:37 overload, fft_fftw3 :348, EvalRhoPsi :179-180), as g++ -std=gnu++98
+# mangles the reference's declarations
 REF_SYMBOLS = ["_Z14ApplyHam_blockPCdS0_iiPdii", "_Z8ApplyHamPCdS0_iiPdi",
                "_Z9chebyfiltiPCdiddPdi", "_Z8ortho_qrPCdii", "_Z7lanczosiPCdiPdiS0_S1_i",
-               "_Z10eig_zheevdPCdiPdi", "_Z10eig_zheevdPCdii", "_Z9fft_fftw3PCdiiib"]
+               "_Z10eig_zheevdPCdiPdi", "_Z10eig_zheevdPCdii", "_Z9fft_fftw3PCdiiib",
+               "_Z10EvalRhoPsiiPdPCdi", "_Z10EvalRhoPsiiPdPCdii"]
 
 
 def declared_functions():
@@ -129,3 +131,29 @@ def test_shim_aborts_on_nnr_mismatch_like_the_reference():
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
     assert "nnr != NNR : 999 1000" in r.stdout
+
+
+def test_shim_eval_rho_psi_aborts_on_wrong_charge_like_the_reference():
+    """EvalRhoPsi.cpp:117-133: a state whose charge over coefficients 1 .. NGW-1 is off by
+    more than 6e-3 stops the program with the reference's two lines; the shim checks the
+    charges on the host before any device work (run in a child process)."""
+    import subprocess
+    import sys
+    code = (
+        "import ctypes as C, os, numpy as np\n"
+        f"G = C.CDLL(os.path.join({PKG!r}, 'libpwh_refglobals.so'), mode=C.RTLD_GLOBAL)\n"
+        f"S = C.CDLL(os.path.join({PKG!r}, 'libpwh_shim.so'))\n"
+        "for n, v in (('NX', 2), ('NGWX', 4), ('NKPT', 1)): C.c_int.in_dll(G, n).value = v\n"
+        "ngw = np.array([4], np.int32); c0 = np.zeros(8, np.complex128)\n"
+        "C.c_void_p.in_dll(G, 'NGW').value = ngw.ctypes.data\n"
+        "C.c_void_p.in_dll(G, 'C0').value = c0.ctypes.data\n"
+        # state 1: charge 1 over the first three coefficients; state 2: 0.99 (+ a last
+        # coefficient that the check leaves out)
+        "ev = np.array([0.6, 0.8, 0, 5, 0.99 ** 0.5, 0, 0, 5], np.complex128)\n"
+        "f = getattr(S, '_Z10EvalRhoPsiiPdPCdi')\n"
+        "f.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int]\n"
+        "f(1, None, ev.ctypes.data, 4)\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "Wrong charge density in G-space:       0.9900000000" in r.stdout
+    assert "for k-point 1 and states 2" in r.stdout

@@ -10,7 +10,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import ROOT, relerr
+from conftest import ROOT, oracle_globals, relerr
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
@@ -342,3 +342,72 @@ def test_ortho_qr_and_eig_against_compiled_reference(oracle):
     H = rng.standard_normal((7, 7)) + 1j * rng.standard_normal((7, 7))
     H = H + H.conj().T
     assert np.abs(oracle.eig_zheevd(H)[0] - R.eig_zheevd(H)[0]).max() < 1e-12
+
+
+# ---- EvalRhoPsi (EvalRhoPsi.cpp:83-157) ------------------------------------------------
+
+def _signed_index(n123, nr):
+    """1-based N123 -> signed grid indices (n1, n2, n3) and the 0-based grid position."""
+    idx = int(n123) - 1
+    i1, i2, i3 = idx % nr[0], (idx // nr[0]) % nr[1], idx // (nr[0] * nr[1])
+    s = [i if i < n // 2 else i - n for i, n in zip((i1, i2, i3), nr)]
+    return s, idx
+
+
+def test_rho_single_plane_wave_is_constant(oracle, si8):
+    g = oracle_globals(si8)
+    n = si8.ngw
+    ev = np.zeros((2, n), np.complex128)
+    ev[0, 5] = np.exp(0.3j)
+    ev[1, 17] = 1.0
+    w, omega = np.array([2.0, 0.5]), 1080.0
+    rho, ek, charge, bad = oracle.eval_rho_psi(g, ev, n, w, omega)
+    assert bad == -1 and np.allclose(charge, 1.0)
+    assert np.abs(rho - (2.0 + 0.5) / omega).max() < 1e-15
+    xkg = si8.gv["xkg"]
+    assert abs(ek - (2.0 * 0.5 * xkg[5] + 0.5 * 0.5 * xkg[17]) * g.tpiba2) < 1e-13
+
+
+def test_rho_two_plane_waves_closed_form(oracle, si8):
+    g = oracle_globals(si8)
+    n, nr = si8.ngw, si8.nr
+    a, b, c1, c2 = 3, 40, 0.6 + 0.2j, -0.3 + 0.7j
+    norm = np.sqrt(abs(c1) ** 2 + abs(c2) ** 2)
+    c1, c2 = c1 / norm, c2 / norm
+    ev = np.zeros((1, n), np.complex128)
+    ev[0, a], ev[0, b] = c1, c2
+    rho, _, charge, bad = oracle.eval_rho_psi(g, ev, n, [1.0], 1.0)
+    (ga, _), (gb, _) = _signed_index(si8.gv["n123"][a], nr), _signed_index(si8.gv["n123"][b], nr)
+    i3, i2, i1 = np.meshgrid(np.arange(nr[2]), np.arange(nr[1]), np.arange(nr[0]), indexing="ij")
+    ph = 2 * np.pi * ((ga[0] - gb[0]) * i1 / nr[0] + (ga[1] - gb[1]) * i2 / nr[1] + (ga[2] - gb[2]) * i3 / nr[2])
+    ref = abs(c1) ** 2 + abs(c2) ** 2 + 2 * np.real(c1 * np.conj(c2) * np.exp(1j * ph))
+    assert bad == -1 and abs(charge[0] - 1.0) < 1e-14
+    assert np.abs(rho - ref.reshape(-1)).max() < 1e-13
+
+
+def test_rho_parseval_and_accumulation(oracle, si8):
+    g = oracle_globals(si8)
+    n = si8.ngw
+    X = si8.psi(6, seed=41)
+    X[:, n - 1] = 0.0          # the reference never reads the last coefficient
+    X /= np.linalg.norm(X, axis=1)[:, None]
+    w = np.array([2.0, 2.0, 2.0, 1.0, 0.5, 0.0])
+    base = np.full(int(np.prod(si8.nr)), 0.25)
+    rho, _, charge, bad = oracle.eval_rho_psi(g, X, n, w, 300.0, rhoe=base)
+    assert bad == -1
+    assert abs((rho - base).sum() - np.prod(si8.nr) * w.sum() / 300.0) < 1e-9
+    assert np.all(rho - base >= -1e-15)
+
+
+def test_rho_skips_last_coefficient_and_checks_charge(oracle, si8):
+    g = oracle_globals(si8)
+    n = si8.ngw
+    ev = np.zeros((3, n), np.complex128)
+    ev[0, 2] = np.sqrt(1.005)          # inside the 6e-3 window
+    ev[1, 9] = np.sqrt(1.007)          # outside: the reference aborts here
+    ev[2, n - 1] = 1.0                 # only the skipped coefficient
+    rho, _, charge, bad = oracle.eval_rho_psi(g, ev, n, [1.0, 1.0, 1.0], 1.0)
+    assert bad == 1 and abs(charge[1] - 1.007) < 1e-12
+    assert np.abs(rho - 1.005).max() < 1e-13     # state 0's density only
+    _, _, charge2, bad2 = oracle.eval_rho_psi(g, ev[2:], n, [1.0], 1.0)
+    assert bad2 == 0 and charge2[0] == 0.0
