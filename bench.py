@@ -166,6 +166,77 @@ def measured_traffic(kernel, path):
     return None
 
 
+def rho_flops(s, nxa, nb):
+    """FP64 FFT arithmetic of the density plane pass (k_plane_rho) over nb states: per
+    plane, nxa inverse transforms along i2 (the sphere's i1 columns), nr2 inverse transforms
+    along i1 with the out-of-sphere inputs known zero, and |v|^2 w accumulation (4 flops)."""
+    n1, n2, n3 = s.nr
+    per_plane = nxa * _transform_flops(n2, None, False) + n2 * _transform_flops(n1, None, False) + 4 * n1 * n2
+    return per_plane * n3 * nb
+
+
+def rho_section(s, sysm, psi, info, steps, cpu):
+    """EvalRhoPsi.cpp:83-157 over the whole block: host-inclusive rate of pwh_eval_rho_psi
+    and the device passes timed with HIP events; the oracle (C restatement) beside it."""
+    import pwh
+    nb = psi.shape[0]
+    rng = np.random.default_rng(11)
+    w = 2.0 * rng.uniform(0.2, 1.0, nb)
+    omega = s.a ** 3
+    for _ in range(2):
+        sysm.eval_rho_psi(psi, w, omega)
+    pwh.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sysm.eval_rho_psi(psi, w, omega)
+    dt = (time.perf_counter() - t0) / steps
+    pwh.timing_reset()
+    pwh.timing_enable(True)
+    for _ in range(steps):
+        sysm.eval_rho_psi(psi, w, omega)
+    pwh.synchronize()
+    dev = {}
+    for k in ("rho_sums", "rho_zinv", "rho_plane", "rho"):
+        ms, n = pwh.timing_get(k)
+        dev[k] = ms / max(n, 1)
+    pwh.timing_enable(False)
+    pwh.timing_reset()
+    n1, n2, n3 = s.nr
+    nnr = n1 * n2 * n3
+    groups = max(1, min(nb, 512 // n3))
+    # W read once per state, the per-group partial densities written and read back, rho written
+    plane_bytes = nb * n3 * info["ncol"] * 16 + 2 * groups * nnr * 8 + nnr * 8
+    fl = rho_flops(s, info["nxa"], nb)
+    tpl = dev["rho_plane"] * 1e-3
+    out = {"states": nb, "ms_host_inclusive": round(1e3 * dt, 4),
+           "states_per_s_host_inclusive": round(nb / dt, 1),
+           "device_ms": {k: round(v, 5) for k, v in dev.items()},
+           "states_per_s_device": round(nb / (dev["rho"] * 1e-3), 1),
+           "plane_pass": {"bound": "valu", "flops": int(fl), "achieved": round(fl / tpl / 1e12, 2),
+                          "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                          "frac": round(fl / tpl / 1e12 / FP64_PEAK_TFLOPS, 4),
+                          "algorithmic_bytes": int(plane_bytes),
+                          "GB/s": round(plane_bytes / tpl / 1e9, 1)}}
+    if cpu:
+        from oracle import oracle as O
+        g = s.globals()
+        og = O.Globals(nr=s.nr, ngwx=g.ngwx, n123=s.gv["n123"], xkg=s.gv["xkg"], tpiba2=g.tpiba2,
+                       na=g.na, nax=g.nax, l_max=g.l_max, l_loc=g.l_loc, nlmax=g.nlmax, wnl=g.wnl,
+                       eigr=g.eigr, pkg_a=g.pkg_a)
+        t0 = time.perf_counter()
+        O.eval_rho_psi(og, psi[:1], s.ngw, w[:1], omega)
+        t1 = time.perf_counter() - t0
+        ns = int(max(1, min(nb, 8.0 / max(t1, 1e-3))))
+        t0 = time.perf_counter()
+        O.eval_rho_psi(og, psi[:ns], s.ngw, w[:ns], omega)
+        dtc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(ns / dtc, 2), "unit": "states/s", "cores": 1, "kind": "port",
+                               "sample": f"{ns} states through the oracle's EvalRhoPsi (C restatement, "
+                                         f"own mixed-radix FFT, per-state loop as EvalRhoPsi.cpp:97-150), "
+                                         f"1 thread, {dtc:.1f} s"}
+    return out
+
+
 def c3_batch(steps, degree=10):
     """24 uneven cells filtered together (one captured graph) vs one at a time."""
     import pwh
@@ -360,6 +431,11 @@ def main():
         chfsi["converged_30_iterations"] = conv
         log(f"bench: chfsi 30 iterations: {json.dumps(conv)}")
 
+    rho = None
+    if not args.only:
+        rho = rho_section(s, sysm, psi, info, steps=max(5, min(args.steps, 20)), cpu=not args.no_cpu)
+        log("bench: eval_rho_psi " + json.dumps(rho))
+
     extra = {}
     if not args.no_extra and not args.only:
         extra["c3_batched_uneven"] = c3_batch(steps=5)
@@ -396,6 +472,7 @@ def main():
         "roofline": roof,
         "kernels": per_kernel,
         "chfsi": chfsi or None,
+        "eval_rho_psi": rho,
         "c3_batched_uneven": extra.get("c3_batched_uneven"),
         "c4_si216": extra.get("c4_si216"),
         "cpu_baseline": cpu,

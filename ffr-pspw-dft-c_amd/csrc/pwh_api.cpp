@@ -50,9 +50,11 @@ int fail(int code, const std::string& msg) {
     if (r_ != PWH_OK) return r_;  \
   } while (0)
 
-enum TimerId { TM_ZINV, TM_PLANE, TM_ZFWD, TM_PROJ, TM_EXPAND, TM_HPSI, TM_COUNT };
+enum TimerId { TM_ZINV, TM_PLANE, TM_ZFWD, TM_PROJ, TM_EXPAND, TM_HPSI, TM_RHO_SUMS, TM_RHO_ZINV, TM_RHO_PLANE,
+               TM_RHO, TM_COUNT };
 constexpr size_t kRocblasWorkspace = size_t(128) << 20;   // bytes
-const char* kTimerNames[TM_COUNT] = {"zinv", "plane", "zfwd", "proj", "expand", "hpsi"};
+const char* kTimerNames[TM_COUNT] = {"zinv", "plane", "zfwd", "proj", "expand", "hpsi",
+                                     "rho_sums", "rho_zinv", "rho_plane", "rho"};
 
 struct Ctx {
   bool ready = false;
@@ -1045,34 +1047,53 @@ int pwh_eval_rho_psi(pwh_system* s, const double* eigvec, int ld, int nstates, c
   HIPCHK(hipMemcpy2DAsync(s->stage.p, sizeof(cx<double>) * ngw, eigvec, sizeof(cx<double>) * ld,
                           sizeof(cx<double>) * ngw, nb, hipMemcpyHostToDevice, g.stream));
   CHK(s->X.ensure((size_t)nb * ldp));
-  HIPCHK(permute_in_f64(s->stage.p, ngw, s->d_perm.p, ngw, ldp, s->X.p, nb, g.stream));
-  int last = 0;
-  for (int k = 0; k < ngw; ++k)
-    if (s->perm[k] == ngw - 1) last = k;
-  HIPCHK(hipMemset2DAsync(s->X.p + last, sizeof(cx<double>) * ldp, 0, sizeof(cx<double>), nb, g.stream));
-  CHK(s->rho_q.ensure(nb));
-  CHK(s->rho_k.ensure(nb));
-  HIPCHK(band_sums(s->X.p, ldp, ngw, s->kin64.p, nb, s->rho_q.p, s->rho_k.p, g.stream));
-  // inverse transform and w |psi|^2 summed over bands (FP64)
+  CHK(s->rho_q.ensure((size_t)nb * kBandSlices));
+  CHK(s->rho_k.ensure((size_t)nb * kBandSlices));
   CHK(s->W64.ensure((size_t)nb * s->nr[2] * s->maps.ncol_pad));
-  HIPCHK(launch_zinv<double>(s->maps, s->tw64[2].p, s->X.p, ldp, s->W64.p, nb, g.stream));
   std::vector<double> wo(nb);
   for (int b = 0; b < nb; ++b) wo[b] = w[b] / omega;
   CHK(upload(s->rho_w, wo.data(), wo.size()));
   const int groups = std::max(1, std::min(nb, 512 / s->nr[2]));
   CHK(s->rho_part.ensure((size_t)groups * nnr));
   CHK(s->rho.ensure(nnr));
-  HIPCHK(launch_plane_rho(s->maps, s->tw64[0].p, s->tw64[1].p, s->W64.p, nb, groups, s->rho_w.p,
-                          s->rho_part.p, s->rho.p, g.stream));
-  std::vector<double> rho(nnr), ks(nb);
+  int last = 0;
+  for (int k = 0; k < ngw; ++k)
+    if (s->perm[k] == ngw - 1) last = k;
+  {
+    Timed tall(TM_RHO, g.stream);
+    {
+      Timed t(TM_RHO_SUMS, g.stream);
+      HIPCHK(permute_in_f64(s->stage.p, ngw, s->d_perm.p, ngw, ldp, s->X.p, nb, g.stream));
+      HIPCHK(hipMemset2DAsync(s->X.p + last, sizeof(cx<double>) * ldp, 0, sizeof(cx<double>), nb, g.stream));
+      HIPCHK(band_sums(s->X.p, ldp, ngw, s->kin64.p, nb, s->rho_q.p, s->rho_k.p, g.stream));
+    }
+    {
+      // inverse transform along i3, then the planes with w |psi|^2 summed over bands (FP64)
+      Timed t(TM_RHO_ZINV, g.stream);
+      HIPCHK(launch_zinv<double>(s->maps, s->tw64[2].p, s->X.p, ldp, s->W64.p, nb, g.stream));
+    }
+    Timed t(TM_RHO_PLANE, g.stream);
+    HIPCHK(launch_plane_rho(s->maps, s->tw64[0].p, s->tw64[1].p, s->W64.p, nb, groups, s->rho_w.p,
+                            s->rho_part.p, s->rho.p, g.stream));
+  }
+  const size_t nsl = (size_t)nb * kBandSlices;
+  std::vector<double> rho(nnr), qs(nsl), ks(nsl);
   HIPCHK(hipMemcpyAsync(rho.data(), s->rho.p, sizeof(double) * nnr, hipMemcpyDeviceToHost, g.stream));
-  HIPCHK(hipMemcpyAsync(charge, s->rho_q.p, sizeof(double) * nb, hipMemcpyDeviceToHost, g.stream));
-  HIPCHK(hipMemcpyAsync(ks.data(), s->rho_k.p, sizeof(double) * nb, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipMemcpyAsync(qs.data(), s->rho_q.p, sizeof(double) * nsl, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipMemcpyAsync(ks.data(), s->rho_k.p, sizeof(double) * nsl, hipMemcpyDeviceToHost, g.stream));
   HIPCHK(hipStreamSynchronize(g.stream));
   for (size_t i = 0; i < nnr; ++i) rhoe[i] += rho[i];
   // sum_b w_b/2 sum_G XKG |c|^2 TPIBA2 = sum_b w_b sum_G kin |c|^2 (kin = XKG TPIBA2 / 2)
   double e = 0.0;
-  for (int b = 0; b < nb; ++b) e += w[b] * ks[b];
+  for (int b = 0; b < nb; ++b) {
+    double q = 0.0, k = 0.0;
+    for (int p = 0; p < kBandSlices; ++p) {
+      q += qs[(size_t)b * kBandSlices + p];
+      k += ks[(size_t)b * kBandSlices + p];
+    }
+    charge[b] = q;
+    e += w[b] * k;
+  }
   *ekin = e;
   return PWH_OK;
 }

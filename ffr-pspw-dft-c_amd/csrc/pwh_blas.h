@@ -11,6 +11,8 @@ hipError_t axpby(const cx<double>* x, double a, const cx<double>* z, double b, c
                  hipStream_t s);
 hipError_t symmetrize(cx<double>* A, int n, int ld, hipStream_t s);
 hipError_t zero_lower(cx<double>* A, int n, int ld, hipStream_t s);
+// charge / ksum: [nb][kBandSlices] partial sums per band (sum the slices in order)
+constexpr int kBandSlices = 16;
 hipError_t band_sums(const cx<double>* X, int ld, int n, const double* kin, int nb, double* charge,
                      double* ksum, hipStream_t s);
 hipError_t resid_norms(const cx<double>* H, const cx<double>* X, const double* lam, int ld, int n,

@@ -156,16 +156,17 @@ __global__ void k_resid(const cx<double>* __restrict__ H, const cx<double>* __re
   }
   if (t == 0) res[b] = sqrt(sh[0]);
 }
-// per band b: charge[b] = sum_k |X[b][k]|^2, ksum[b] = sum_k kin[k] |X[b][k]|^2 (one block per
-// band, fixed-order tree reduction)
+// per band b and slice p: charge[b][p] = sum_k |X[b][k]|^2, ksum[b][p] = sum_k kin[k] |X[b][k]|^2
+// over the k of slice p (kBandSlices slices of 256-element chunks per band, fixed-order tree
+// reduction); the caller sums the slices in order.
 __global__ __launch_bounds__(256) void k_band_sums(const cx<double>* __restrict__ X, int ld, int n,
                                                    const double* __restrict__ kin, double* __restrict__ charge,
                                                    double* __restrict__ ksum) {
   __shared__ double s1[256], s2[256];
-  const int b = blockIdx.x, t = threadIdx.x;
+  const int b = blockIdx.y, sl = blockIdx.x, t = threadIdx.x;
   const cx<double>* x = X + (size_t)b * ld;
   double a = 0.0, c = 0.0;
-  for (int k = t; k < n; k += 256) {
+  for (int k = sl * 256 + t; k < n; k += 256 * kBandSlices) {
     const double m = x[k].x * x[k].x + x[k].y * x[k].y;
     a += m;
     c += kin[k] * m;
@@ -177,12 +178,15 @@ __global__ __launch_bounds__(256) void k_band_sums(const cx<double>* __restrict_
     if (t < h) { s1[t] += s1[t + h]; s2[t] += s2[t + h]; }
     __syncthreads();
   }
-  if (t == 0) { charge[b] = s1[0]; ksum[b] = s2[0]; }
+  if (t == 0) {
+    charge[(size_t)b * kBandSlices + sl] = s1[0];
+    ksum[(size_t)b * kBandSlices + sl] = s2[0];
+  }
 }
 
 hipError_t band_sums(const cx<double>* X, int ld, int n, const double* kin, int nb, double* charge,
                      double* ksum, hipStream_t s) {
-  hipLaunchKernelGGL(k_band_sums, dim3(nb), dim3(256), 0, s, X, ld, n, kin, charge, ksum);
+  hipLaunchKernelGGL(k_band_sums, dim3(kBandSlices, nb), dim3(256), 0, s, X, ld, n, kin, charge, ksum);
   return hipGetLastError();
 }
 

@@ -184,7 +184,10 @@ int pwh_batch_chebyfilt(pwh_batch *batch, int degree, const double *lb, const do
  * nonlocal GEMMs with the FFT passes), so each event pair times one kernel alone. */
 int pwh_timing_enable(int on);
 int pwh_timing_reset(void);
-/* name: "zinv", "plane", "zfwd", "proj", "expand", "hpsi". total_ms / count of launches. */
+/* name: "zinv", "plane", "zfwd", "proj", "expand", "hpsi" (H|psi> passes), or "rho_sums",
+ * "rho_zinv", "rho_plane", "rho" (the device part of pwh_eval_rho_psi: permutation and
+ * per-state sums, i3 transform, plane pass with the band sum, all three).  total_ms /
+ * count of launches. */
 int pwh_timing_get(const char *name, double *total_ms, int *count);
 
 #ifdef __cplusplus

@@ -301,6 +301,189 @@ def residual_norms(X, HX):
     return np.sqrt(np.real(np.sum(R.conj() * R, axis=0)))
 
 
+# --------------------------------------------------------------------------
+# Block eigensolvers of the SCF driver: Davidson and LOBPCG (FP64, rows = bands)
+# --------------------------------------------------------------------------
+
+def random_col(nelem, seed):
+    """util_random_cpu.cpp:5-25: the start-vector generator.  The seed is advanced by the
+    linear congruential step a <- ((214013 a + 2531011) >> 16) & 32767 and each part is
+    (a % 100 + 1) / a with the advanced a, real part first."""
+    out = np.empty(nelem, np.complex128)
+    a = int(seed)
+    for i in range(nelem):
+        a = ((a * 214013 + 2531011) >> 16) & 32767
+        re = (a % 100 + 1) / a
+        a = ((a * 214013 + 2531011) >> 16) & 32767
+        im = (a % 100 + 1) / a
+        out[i] = complex(re, im)
+    return out
+
+
+def start_vectors(nbasis, nstates):
+    """davidson_cpu.cpp:47-51 / lobpcg_cpu.cpp:80-84: column ist from random_col with seed
+    100 + ist - 1.  Rows = bands."""
+    return np.stack([random_col(nbasis, 100 + i) for i in range(nstates)])
+
+
+def gen_prec(xkg, tpiba2, alat, tau):
+    """GenPrec.cpp:5-27: the Teter-like kinetic preconditioner (tau <= 0) or tau / T(G)."""
+    xkg = np.asarray(xkg, np.float64)
+    if tau <= 0.0:
+        x = xkg * (2.0 * np.pi / alat) ** 2
+        y = 27.0 + x * (18.0 + x * (12.0 + 8.0 * x))
+        return y / (y + 16.0 * x ** 4)
+    return tau / (0.5 * (xkg + 1.0) * tpiba2)
+
+
+ZHEGV_SMALL = 12.220446049250313e-16   # eig_zhegv.cpp:8
+
+
+def eig_zhegv_eval(A, B):
+    """eig_zhegv.cpp:82-149 (and eig_zhegv, :5-79, which drops the eigenvalues): canonical
+    orthogonalisation.  B = V diag(b) V^H (zheevd, upper triangle); columns with b > SMALL
+    scaled by 1/sqrt(b), the others kept; A' = V^H A V; A' = Z diag(e) Z^H (zheevd, upper
+    triangle); eigenvectors V Z.  Returns (e, V Z, number of b > SMALL, the small b)."""
+    b, V = np.linalg.eigh(B, UPLO="U")
+    keep = b > ZHEGV_SMALL
+    V = V.copy()
+    V[:, keep] /= np.sqrt(b[keep])
+    E = A @ V
+    A2 = V.conj().T @ E
+    e, Z = np.linalg.eigh(A2, UPLO="U")
+    return e, V @ Z, int(keep.sum()), b[~keep].copy()
+
+
+def _cn(X, Y):
+    """X^H Y for blocks stored as rows (column-major nbasis x n matrices)."""
+    return X.conj() @ Y.T
+
+
+def _nn(X, M):
+    """X M (X rows = columns) -> rows."""
+    return M.T @ X
+
+
+@dataclass
+class DiagResult:
+    evals: np.ndarray
+    X: np.ndarray
+    res: np.ndarray
+    iterations: int
+    nconv: int
+    reason: int          # 0: iterations exhausted, 1: nconv >= nstates - nempty, 2: rnorm
+    history: list        # (iteration, nconv, rnorm) at the top of each iteration
+
+
+def davidson(H, X0, prec, nempty=0, max_iter=None, ortho=None):
+    """davidson_cpu.cpp:9-247 with H(X) the block H|psi> (rows = bands) and X0 the start
+    vectors (before orthonormalisation).  Tolerance 1e-6 on each residual norm, stop when
+    nconv >= nstates - nempty or the mean residual < 1e-7."""
+    n = X0.shape[0]
+    ortho = ortho_qr if ortho is None else ortho
+    X = ortho(np.asarray(X0, np.complex128))
+    HX = H(X)
+    evals = np.real(np.sum(X.conj() * HX, axis=1))
+    R = evals[:, None] * X - HX
+    res = np.sqrt(np.real(np.sum(R.conj() * R, axis=1)))
+    if max_iter is None:
+        max_iter = max(30, n // 2)
+    mz, tol = 2.220446049250313e-16, 1.0e-6
+    rnorm, nconv, reason, hist, it = 1.0, 0, 0, [], 0
+    for it in range(1, max_iter + 1):
+        hist.append((it, nconv, rnorm))
+        sc = np.where(res > mz, 1.0 / np.where(res > mz, res, 1.0), 1.0)
+        R = (R * sc[:, None]) * prec[None, :]
+        HR = H(R)
+        h = np.zeros((2 * n, 2 * n), np.complex128)
+        o = np.zeros((2 * n, 2 * n), np.complex128)
+        h[:n, :n] = _cn(X, HX) if it == 1 else np.diag(evals)
+        h[:n, n:] = _cn(X, HR)
+        h[n:, :n] = _cn(HR, X)
+        h[n:, n:] = _cn(R, HR)
+        o[:n, :n] = np.eye(n)
+        o[:n, n:] = _cn(X, R)
+        o[n:, :n] = _cn(R, X)
+        o[n:, n:] = _cn(R, R)
+        er, E, _, _ = eig_zhegv_eval(h, o)
+        evals = er[:n].copy()
+        X = _nn(X, E[:n, :n]) + _nn(R, E[n:, :n])
+        HX = _nn(HX, E[:n, :n]) + _nn(HR, E[n:, :n])
+        R = evals[:, None] * X - HX
+        res = np.sqrt(np.real(np.sum(R.conj() * R, axis=1)))
+        nconv = int(np.sum(res < tol))
+        if nconv >= n - nempty:
+            reason = 1
+            break
+        rnorm = float(np.mean(res))
+        if rnorm < tol / 10.0:
+            reason = 2
+            break
+    return DiagResult(evals, X, res, it, nconv, reason, hist)
+
+
+def lobpcg(H, X0, prec, nempty=0, max_iter=None, ortho=None):
+    """lobpcg_cpu.cpp:32-389 with H(X) the block H|psi> (rows = bands).  Q = [X | W | P];
+    the first iteration uses [X | W] only (:93-181); later ones the full block (:186-367).
+    Returns lambda from the last X^H HX, the final X, and the residual norms of that X."""
+    n = X0.shape[0]
+    ortho = ortho_qr if ortho is None else ortho
+    tol, tfudge = 1.0e-6, 1.0e10
+    if max_iter is None:
+        max_iter = max(30, n // 2)
+    X = ortho(np.asarray(X0, np.complex128))
+    HX = H(X)
+    P = HP = None
+    lam = res = None
+    nconv, reason, hist, it = 0, 0, [], 0
+    rnorm = 0.0
+
+    def chol_upper(Cm):
+        Cm = symmetrize(Cm)
+        L = np.linalg.cholesky(Cm)        # Cm = L L^H, U = L^H (zpotrf 'U')
+        return L.conj().T
+
+    def right_solve(Y, U):
+        # Y U^{-1} for Y rows = columns: (U^{-T} applied to the row stack)
+        return np.linalg.solve(U.T, Y)
+
+    for it in range(1, max(1, max_iter) + 1):
+        if it > 1:
+            hist.append((it, nconv, rnorm))
+        t1 = _cn(X, HX)
+        W = HX - _nn(X, t1)
+        lam = np.linalg.eigh(t1, UPLO="U")[0]
+        res = np.sqrt(np.real(np.sum(W.conj() * W, axis=1)))
+        nconv = int(np.sum(res < tol))
+        rnorm = float(np.mean(res))
+        if nconv >= n - nempty:
+            reason = 1
+            break
+        if rnorm < tol:
+            reason = 2
+            break
+        W = W * prec[None, :]
+        HW = H(W)
+        U = chol_upper(_cn(W, W))
+        W, HW = right_solve(W, U), right_solve(HW, U)
+        if it == 1:
+            Q, HQ = np.concatenate([X, W]), np.concatenate([HX, HW])
+        else:
+            Q, HQ = np.concatenate([X, W, P]), np.concatenate([HX, HW, HP])
+        T = symmetrize(_cn(Q, HQ))
+        G = symmetrize(_cn(Q, Q))
+        _, Uv, _, _ = eig_zhegv_eval(T, G)
+        X, HX = _nn(Q, Uv[:, :n]), _nn(HQ, Uv[:, :n])
+        if it == 1:
+            P, HP = W.copy(), HW.copy()
+        else:
+            P = _nn(W, Uv[n:2 * n, :n]) + _nn(P, Uv[2 * n:, :n])
+            HP = _nn(HW, Uv[n:2 * n, :n]) + _nn(HP, Uv[2 * n:, :n])
+            U = chol_upper(_cn(P, P))
+            P, HP = right_solve(P, U), right_solve(HP, U)
+    return DiagResult(lam, X, res, it, nconv, reason, hist)
+
+
 class Reference:
     """ctypes wrapper of oracle/_ref/libref.so (the reference's own compiled setup code)."""
 
@@ -314,6 +497,9 @@ class Reference:
         L.ref_sort1.argtypes = [C.c_int, _dp, _ip]
         L.ref_ortho_qr.argtypes = [_zp, C.c_int, C.c_int]
         L.ref_eig_zheevd.argtypes = [_zp, C.c_int, _dp, C.c_int]
+        L.ref_eig_zhegv.argtypes = [_zp, C.c_int, _zp, C.c_int, _zp, C.c_int, C.c_int]
+        L.ref_eig_zhegv_eval.argtypes = [_zp, C.c_int, _zp, C.c_int, _dp, _zp, C.c_int, C.c_int]
+        L.ref_random_col.argtypes = [_zp, C.c_int, C.c_long]
         self.L = L
 
     @staticmethod
@@ -360,3 +546,26 @@ class Reference:
         w = np.zeros(n)
         self.L.ref_eig_zheevd(A.reshape(-1), n, w, n)
         return w, A.T.copy()
+
+    def eig_zhegv_eval(self, A, B):
+        """Returns (eigenvalues, eigenvectors) of the reference's eig_zhegv_eval."""
+        n = A.shape[0]
+        Ac = np.array(np.asarray(A, np.complex128).T, order="C", copy=True)
+        Bc = np.array(np.asarray(B, np.complex128).T, order="C", copy=True)
+        E = np.zeros((n, n), np.complex128)
+        w = np.zeros(n)
+        self.L.ref_eig_zhegv_eval(Ac.reshape(-1), n, Bc.reshape(-1), n, w, E.reshape(-1), n, n)
+        return w, E.T.copy()
+
+    def eig_zhegv(self, A, B):
+        n = A.shape[0]
+        Ac = np.array(np.asarray(A, np.complex128).T, order="C", copy=True)
+        Bc = np.array(np.asarray(B, np.complex128).T, order="C", copy=True)
+        E = np.zeros((n, n), np.complex128)
+        self.L.ref_eig_zhegv(Ac.reshape(-1), n, Bc.reshape(-1), n, E.reshape(-1), n, n)
+        return E.T.copy()
+
+    def random_col(self, nelem, seed):
+        out = np.zeros(nelem, np.complex128)
+        self.L.ref_random_col(out, nelem, seed)
+        return out

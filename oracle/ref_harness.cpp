@@ -1,6 +1,8 @@
 // ref_harness.cpp -- drives the reference's own compiled setup routines
-// (GVectorsGen0, GVectorsGen, sort1, GkVectorsGen, PhaseFactor) and its LAPACK
-// wrappers (ortho_qr, eig_zheevd) so the oracle can be checked against them.
+// (GVectorsGen0, GVectorsGen, sort1, GkVectorsGen, PhaseFactor), its LAPACK
+// wrappers (ortho_qr, eig_zheevd, eig_zhegv, eig_zhegv_eval) and the start-vector
+// generator of the Davidson / LOBPCG solvers (random_col) so the oracle can be checked
+// against them.
 //
 // TEST INFRASTRUCTURE ONLY (built into oracle/_ref/, never shipped).
 //
@@ -9,6 +11,8 @@
 // scalar lines of Setup.cpp:39-66 before calling the reference routines in the
 // order Setup.cpp:128,139 and PrepareSCF.cpp:12 call them.
 #include "common_pspw.h"
+
+void random_col(double complex *A, int nelem, long a);   // util_random_cpu.cpp:15
 
 extern "C" {
 
@@ -108,6 +112,16 @@ void ref_sort1(int n, double *arr, int *index) { sort1(n, arr, index); }
 void ref_ortho_qr(double complex *X, int nbasis, int nstates) { ortho_qr(X, nbasis, nstates); }
 
 void ref_eig_zheevd(double complex *A, int ldA, double *lambda, int N) { eig_zheevd(A, ldA, lambda, N); }
+
+void ref_eig_zhegv(double complex *A, int ldA, double complex *B, int ldB, double complex *evec, int ldE,
+                   int N) { eig_zhegv(A, ldA, B, ldB, evec, ldE, N); }
+
+void ref_eig_zhegv_eval(double complex *A, int ldA, double complex *B, int ldB, double *eval,
+                        double complex *evec, int ldE, int N) {
+  eig_zhegv_eval(A, ldA, B, ldB, eval, evec, ldE, N);
+}
+
+void ref_random_col(double complex *A, int nelem, long a) { random_col(A, nelem, a); }
 
 void ref_free() { free_all(); }
 
