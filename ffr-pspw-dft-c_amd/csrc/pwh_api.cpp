@@ -192,6 +192,10 @@ struct pwh_system {
   Buf<cx<double>> X, HX, A64, B64, C64, W64, Hnl64, proj64, Q64, part64, stage, S, Vl;
   Buf<double> evals, rwork, dres;
   Buf<double> rho_part, rho, rho_w, rho_q, rho_k;   // density pass (pwh_eval_rho_psi)
+  // block solvers (pwh_davidson, pwh_lobpcg): Q = [X | R] or [X | W | P] and H Q, one block
+  // of scratch, the small projected matrices and per-band vectors
+  Buf<cx<double>> dq, dhq, dtmp, dT, dG, dE, dT1, dt1;
+  Buf<double> dlam, dnrm, dpart, dprec, dbo;
   Buf<rocblas_int> info;
   // FP32 work
   Buf<cx<float>> X32, Y32, Y1_32, W32, Hnl32, proj32, Q32, part32;
@@ -920,6 +924,8 @@ int pwh_system_info(const pwh_system* s, int* ngw, int* ncol, int* nxa, size_t* 
     add(s->W64.cap, 16); add(s->W32.cap, 8); add(s->Hnl64.cap, 16); add(s->Hnl32.cap, 8);
     add(s->X32.cap, 8); add(s->Y32.cap, 8); add(s->Y1_32.cap, 8); add(s->part64.cap, 16);
     add(s->part32.cap, 8); add(s->V64.cap, 8); add(s->V32.cap, 4); add(s->Vl.cap, 16);
+    add(s->dq.cap, 16); add(s->dhq.cap, 16); add(s->dtmp.cap, 16);
+    add(s->rho_part.cap, 8);
     *bytes = b;
   }
   return PWH_OK;
@@ -1205,6 +1211,313 @@ done:
   dv0.release();
   df.release();
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Block eigensolvers of the SCF driver on the device: davidson_cpu.cpp:9-247 and
+// lobpcg_cpu.cpp:32-389.  Blocks stay in HBM in sphere order for the whole solve; the
+// host sees the per-band residual norms once per iteration (the convergence tests).
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr double kDiagTol = 1.0e-6;                     // davidson_cpu.cpp:85, lobpcg_cpu.cpp:35
+constexpr double kZhegvSmall = 12.220446049250313e-16;  // eig_zhegv.cpp:8
+
+struct DiagRun {
+  pwh_system* s;
+  pwh_diag_opts* o;
+  int ld, ngw;
+  size_t pe;   // partial-sum elements of the largest projection
+  void report(int ev, int a, int b, double v) const {
+    if (o->report) o->report(o->report_ctx, ev, a, b, v);
+  }
+  // C (column-major, ldc) = A^H B for blocks of ma and nb bands
+  int cn(const cx<double>* A, int ma, const cx<double>* B, int nb, cx<double>* C, int ldc) const {
+    HIPCHK(gemm_cn_f64(A, ld, ma, B, ld, nb, ngw, C, ldc, s->part64.p, pe, g.stream));
+    return PWH_OK;
+  }
+  // out (nb bands) = A (ja bands) M[:, 0:nb] with M column-major (ldm)
+  int nn(const cx<double>* A, int ja, const cx<double>* M, int ldm, int nb, cx<double>* out) const {
+    HIPCHK(gemm_nn_f64(A, ld, ja, M, ldm, nb, ngw, out, ld, g.stream));
+    return PWH_OK;
+  }
+  int copy(cx<double>* dst, const cx<double>* src, int nb) const {
+    HIPCHK(hipMemcpyAsync(dst, src, sizeof(cx<double>) * nb * ld, hipMemcpyDeviceToDevice, g.stream));
+    return PWH_OK;
+  }
+  int info_value(int* v) const {
+    rocblas_int h = 0;
+    HIPCHK(hipMemcpyAsync(&h, s->info.p, sizeof(h), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    *v = h;
+    return PWH_OK;
+  }
+};
+
+// C = op(A) B on column-major N x N device matrices (rocBLAS, host scalars)
+int small_zgemm(rocblas_operation ta, int N, const cx<double>* A, const cx<double>* B, cx<double>* C) {
+  const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+  RBCHK(rocblas_zgemm(g.rb, ta, rocblas_operation_none, N, N, N, &one,
+                      reinterpret_cast<const rocblas_double_complex*>(A), N,
+                      reinterpret_cast<const rocblas_double_complex*>(B), N, &zero,
+                      reinterpret_cast<rocblas_double_complex*>(C), N));
+  return PWH_OK;
+}
+
+// eig_zhegv.cpp:5-149 on column-major N x N device matrices: B = V diag(b) V^H (upper
+// triangle), columns with b > SMALL scaled by 1/sqrt(b), A' = V^H A V (upper triangle),
+// A' = Z diag(e) Z^H, E = V Z.  A and B are overwritten.  Overlap eigenvalues <= SMALL are
+// reported (the reference prints them); a failed eigensolve is reported and leaves E as
+// it was, as the reference returns early.
+int zhegv_dev(const DiagRun& r, cx<double>* A, cx<double>* B, int N, cx<double>* E, double* e) {
+  pwh_system* s = r.s;
+  CHK(s->dbo.ensure(N));
+  CHK(s->rwork.ensure(N));
+  CHK(s->dT1.ensure((size_t)N * N));
+  HIPCHK(fill_lower(B, N, N, g.stream));
+  CHK(hermitian_eig_impl(B, N, s->dbo.p, s->rwork.p, s->info.p));
+  int info = 0;
+  CHK(r.info_value(&info));
+  if (info != 0) {
+    r.report(PWH_DIAG_EIG_FAILED, info, N, 0.0);
+    return PWH_OK;
+  }
+  std::vector<double> b(N);
+  HIPCHK(hipMemcpyAsync(b.data(), s->dbo.p, sizeof(double) * N, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  int nn = 0;
+  for (int i = 0; i < N; ++i) {
+    if (b[i] > kZhegvSmall) ++nn;
+    else r.report(PWH_DIAG_SMALL_OVERLAP, i + 1, N, b[i]);
+  }
+  if (nn < N) r.report(PWH_DIAG_RANK, nn, N, 0.0);
+  HIPCHK(zhegv_scale(B, N, N, s->dbo.p, g.stream));
+  CHK(small_zgemm(rocblas_operation_none, N, A, B, s->dT1.p));
+  CHK(small_zgemm(rocblas_operation_conjugate_transpose, N, B, s->dT1.p, A));
+  HIPCHK(fill_lower(A, N, N, g.stream));
+  CHK(hermitian_eig_impl(A, N, e, s->rwork.p, s->info.p));
+  CHK(r.info_value(&info));
+  if (info != 0) {
+    r.report(PWH_DIAG_EIG_FAILED, info, N, 0.0);
+    return PWH_OK;
+  }
+  return small_zgemm(rocblas_operation_none, N, B, A, E);
+}
+
+// A <- A C^{-1}, HA <- HA C^{-1} with C^H C = sym(A^H A) (zpotrf 'U' + ztrsm 'R','U','N','N',
+// lobpcg_cpu.cpp:138-166, 330-366); the inverse factor multiplies on the MFMA GEMM
+int chol_right(const DiagRun& r, cx<double>* A, cx<double>* HA, int n) {
+  pwh_system* s = r.s;
+  cx<double>* C = s->S.p;
+  CHK(r.cn(A, n, A, n, C, n));
+  HIPCHK(symmetrize(C, n, n, g.stream));
+  RBCHK(rocsolver_zpotrf(g.rb, rocblas_fill_upper, n, reinterpret_cast<rocblas_double_complex*>(C), n,
+                         s->info.p));
+  int info = 0;
+  CHK(r.info_value(&info));
+  if (info != 0)
+    return fail(PWH_ERR_LAPACK, "Cholesky factorisation of the block overlap failed (zpotrf info " +
+                                    std::to_string(info) + ")");
+  RBCHK(rocsolver_ztrtri(g.rb, rocblas_fill_upper, rocblas_diagonal_non_unit, n,
+                         reinterpret_cast<rocblas_double_complex*>(C), n, s->info.p));
+  HIPCHK(zero_lower(C, n, n, g.stream));
+  CHK(r.nn(A, n, C, n, n, s->dtmp.p));
+  CHK(r.copy(A, s->dtmp.p, n));
+  CHK(r.nn(HA, n, C, n, n, s->dtmp.p));
+  return r.copy(HA, s->dtmp.p, n);
+}
+
+// Shared set-up: buffers for nq blocks of n bands, X (host, reference order) into the
+// first block, prec into sphere order.
+int diag_prepare(pwh_system* s, const double* X, int n, int nq, const double* prec, DiagRun* r) {
+  const int ld = s->ngw_pad;
+  const size_t m = (size_t)nq * n;
+  CHK(s->dq.ensure(m * ld));
+  CHK(s->dhq.ensure(m * ld));
+  CHK(s->dtmp.ensure((size_t)n * ld));
+  CHK(s->dT.ensure(m * m));
+  CHK(s->dG.ensure(m * m));
+  CHK(s->dE.ensure(m * m));
+  CHK(s->dt1.ensure((size_t)n * n));
+  CHK(s->S.ensure((size_t)n * n));
+  CHK(s->dlam.ensure(m + n));   // n estimates + the m eigenvalues of the projected problem
+  CHK(s->dnrm.ensure(n));
+  CHK(s->dpart.ensure((size_t)n * kBandSlices));
+  CHK(s->info.ensure(2));
+  r->pe = std::max(proj_part_elems((int)m, (int)m, s->ngw), proj_part_elems(n, n, s->ngw));
+  CHK(s->part64.ensure(r->pe));
+  CHK(s->A64.ensure((size_t)n * ld));
+  std::vector<double> p(ld, 0.0);
+  for (int k = 0; k < s->ngw; ++k) p[k] = prec[s->perm[k]];
+  CHK(upload(s->dprec, p.data(), p.size()));
+  CHK(upload(s->stage, reinterpret_cast<const cx<double>*>(X), (size_t)s->ngw * n));
+  HIPCHK(permute_in_f64(s->stage.p, s->ngw, s->d_perm.p, s->ngw, ld, s->dq.p, n, g.stream));
+  r->s = s;
+  r->ld = ld;
+  r->ngw = s->ngw;
+  return PWH_OK;
+}
+
+int download_vec(double* h, const double* d, int n) {
+  HIPCHK(hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return PWH_OK;
+}
+
+int check_diag_args(pwh_system* s, double* X, int n, const double* prec, pwh_diag_opts* o, double* ev,
+                    double* res) {
+  CHK(check_ld(s, n));
+  if (!X || !prec || !o || !ev || !res) return fail(PWH_ERR_ARG, "null argument");
+  if (o->max_iter < 1) return fail(PWH_ERR_ARG, "max_iter < 1");
+  return PWH_OK;
+}
+
+}  // namespace
+
+int pwh_davidson(pwh_system* s, double* X, int nstates, const double* prec, pwh_diag_opts* o, double* evals,
+                 double* res) {
+  CHK(check_diag_args(s, X, nstates, prec, o, evals, res));
+  const double t0 = now_s();
+  const int n = nstates, n2 = 2 * nstates;
+  DiagRun r{};
+  r.o = o;
+  CHK(diag_prepare(s, X, n, 2, prec, &r));
+  const int ld = r.ld;
+  cx<double>*Q = s->dq.p, *Xd = Q, *R = Q + (size_t)n * ld;
+  cx<double>*HQ = s->dhq.p, *HX = HQ, *HR = HQ + (size_t)n * ld;
+  cx<double>*h = s->dT.p, *ov = s->dG.p, *E = s->dE.p;
+  double *ev = s->dlam.p, *dr = s->dnrm.p, *part = s->dpart.p;
+  o->band_applies = 0;
+  o->iterations = 0;
+  o->reason = 0;
+  // orthonormal start block, H X, Rayleigh quotients and residuals (davidson_cpu.cpp:47-71)
+  CHK(cholqr2(Xd, ld, s->ngw, n, s->A64.p, s->S.p, s->part64.p, r.pe, s->info.p));
+  CHK(hpsi<double>(s, Xd, HX, n, 1.0, 0.0, 0.0, nullptr, PREC_FP64));
+  o->band_applies += n;
+  HIPCHK(dotc_re(Xd, HX, ld, s->ngw, n, part, ev, g.stream));
+  HIPCHK(resid_block(Xd, HX, ev, ld, s->ngw, n, 0, R, part, dr, g.stream));
+  CHK(download_vec(res, dr, n));
+  int nconv = 0;
+  double rnorm = 1.0;
+  for (int istep = 1; istep <= o->max_iter; ++istep) {
+    o->iterations = istep;
+    r.report(PWH_DIAG_ITER, istep, nconv, rnorm);
+    // residuals normalised and preconditioned, H R (:91-123)
+    HIPCHK(scale_prec(R, ld, s->ngw, n, dr, s->dprec.p, g.stream));
+    CHK(hpsi<double>(s, R, HR, n, 1.0, 0.0, 0.0, nullptr, PREC_FP64));
+    o->band_applies += n;
+    // reduced Hamiltonian and overlap, 2n x 2n (:125-184)
+    if (istep == 1) CHK(r.cn(Xd, n, HX, n, h, n2));
+    else HIPCHK(set_diag_block(h, n2, n, ev, g.stream));
+    CHK(r.cn(Q, n2, HR, n, h + (size_t)n * n2, n2));     // <x|H|r>, <r|H|r>
+    CHK(r.cn(HR, n, Xd, n, h + n, n2));                  // (H r)^H x
+    HIPCHK(set_diag_block(ov, n2, n, nullptr, g.stream));
+    CHK(r.cn(Q, n2, R, n, ov + (size_t)n * n2, n2));     // <x|r>, <r|r>
+    CHK(r.cn(R, n, Xd, n, ov + n, n2));                  // <r|x>
+    CHK(zhegv_dev(r, h, ov, n2, E, ev));
+    // X <- [X R] E(:, 1:n), HX <- [HX HR] E(:, 1:n) (:189-213)
+    CHK(r.nn(Q, n2, E, n2, n, s->dtmp.p));
+    CHK(r.copy(Xd, s->dtmp.p, n));
+    CHK(r.nn(HQ, n2, E, n2, n, s->dtmp.p));
+    CHK(r.copy(HX, s->dtmp.p, n));
+    HIPCHK(resid_block(Xd, HX, ev, ld, s->ngw, n, 0, R, part, dr, g.stream));
+    CHK(download_vec(res, dr, n));
+    // convergence tests (:215-236)
+    nconv = 0;
+    for (int i = 0; i < n; ++i) nconv += res[i] < kDiagTol;
+    if (nconv >= n - o->nempty) { o->reason = 1; break; }
+    rnorm = 0.0;
+    for (int i = 0; i < n; ++i) rnorm += res[i];
+    rnorm /= n;
+    if (rnorm < kDiagTol / 10.0) { o->reason = 2; break; }
+  }
+  o->nconv = nconv;
+  CHK(download_vec(evals, ev, n));
+  CHK(store_block(s, Xd, X, n));
+  o->t_total_s = now_s() - t0;
+  return PWH_OK;
+}
+
+int pwh_lobpcg(pwh_system* s, double* X, int nstates, const double* prec, pwh_diag_opts* o, double* lambda,
+               double* res) {
+  CHK(check_diag_args(s, X, nstates, prec, o, lambda, res));
+  const double t0 = now_s();
+  const int n = nstates, n2 = 2 * nstates, n3 = 3 * nstates;
+  DiagRun r{};
+  r.o = o;
+  CHK(diag_prepare(s, X, n, 3, prec, &r));
+  const int ld = r.ld;
+  cx<double>*Q = s->dq.p, *Xd = Q, *W = Q + (size_t)n * ld, *P = Q + (size_t)n2 * ld;
+  cx<double>*HQ = s->dhq.p, *HX = HQ, *HW = HQ + (size_t)n * ld, *HP = HQ + (size_t)n2 * ld;
+  cx<double>*T = s->dT.p, *G = s->dG.p, *U = s->dE.p, *t1 = s->dt1.p;
+  double *lam = s->dlam.p, *dr = s->dnrm.p, *part = s->dpart.p;
+  o->band_applies = 0;
+  o->iterations = 0;
+  o->reason = 0;
+  // orthonormal start block and H X (lobpcg_cpu.cpp:80-91)
+  CHK(cholqr2(Xd, ld, s->ngw, n, s->A64.p, s->S.p, s->part64.p, r.pe, s->info.p));
+  CHK(hpsi<double>(s, Xd, HX, n, 1.0, 0.0, 0.0, nullptr, PREC_FP64));
+  o->band_applies += n;
+  int nconv = 0;
+  double rnorm = 0.0;
+  const int iters = std::max(1, o->max_iter);
+  for (int it = 1; it <= iters; ++it) {
+    o->iterations = it;
+    if (it > 1) r.report(PWH_DIAG_ITER, it, nconv, rnorm);
+    // X^H H X, W = HX - X (X^H H X), its eigenvalues, residual norms (:93-124, 189-222)
+    CHK(r.cn(Xd, n, HX, n, t1, n));
+    CHK(r.nn(Xd, n, t1, n, n, s->dtmp.p));
+    HIPCHK(resid_block(s->dtmp.p, HX, nullptr, ld, s->ngw, n, 1, W, part, dr, g.stream));
+    HIPCHK(fill_lower(t1, n, n, g.stream));
+    CHK(s->rwork.ensure(n));
+    CHK(hermitian_eig_impl(t1, n, lam, s->rwork.p, s->info.p));
+    int info = 0;
+    CHK(r.info_value(&info));
+    if (info != 0) r.report(PWH_DIAG_EIG_FAILED, info, n, 0.0);
+    CHK(download_vec(res, dr, n));
+    CHK(download_vec(lambda, lam, n));
+    nconv = 0;
+    rnorm = 0.0;
+    for (int i = 0; i < n; ++i) {
+      nconv += res[i] < kDiagTol;
+      rnorm += res[i];
+    }
+    rnorm /= n;
+    if (nconv >= n - o->nempty) { o->reason = 1; break; }
+    if (rnorm < kDiagTol) { o->reason = 2; break; }
+    // preconditioned W, H W, W orthonormalised by its Cholesky factor (:126-166)
+    HIPCHK(scale_prec(W, ld, s->ngw, n, nullptr, s->dprec.p, g.stream));
+    CHK(hpsi<double>(s, W, HW, n, 1.0, 0.0, 0.0, nullptr, PREC_FP64));
+    o->band_applies += n;
+    CHK(chol_right(r, W, HW, n));
+    // Rayleigh-Ritz on [X W] (first iteration) or [X W P] (:168-186, 268-287)
+    const int m = it == 1 ? n2 : n3;
+    CHK(r.cn(Q, m, HQ, m, T, m));
+    HIPCHK(symmetrize(T, m, m, g.stream));
+    CHK(r.cn(Q, m, Q, m, G, m));
+    HIPCHK(symmetrize(G, m, m, g.stream));
+    CHK(s->dbo.ensure(m));
+    CHK(zhegv_dev(r, T, G, m, U, s->dlam.p + n));
+    // X <- Q U(:, 1:n), HX <- HQ U(:, 1:n); P from W and the previous P (:188-201, 289-328)
+    CHK(r.nn(Q, m, U, m, n, s->dtmp.p));
+    CHK(r.copy(Xd, s->dtmp.p, n));
+    CHK(r.nn(HQ, m, U, m, n, s->dtmp.p));
+    CHK(r.copy(HX, s->dtmp.p, n));
+    if (it == 1) {
+      CHK(r.copy(P, W, n));
+      CHK(r.copy(HP, HW, n));
+    } else {
+      CHK(r.nn(W, n2, U + n, m, n, s->dtmp.p));
+      CHK(r.copy(P, s->dtmp.p, n));
+      CHK(r.nn(HW, n2, U + n, m, n, s->dtmp.p));
+      CHK(r.copy(HP, s->dtmp.p, n));
+      CHK(chol_right(r, P, HP, n));
+    }
+  }
+  o->nconv = nconv;
+  CHK(store_block(s, Xd, X, n));
+  o->t_total_s = now_s() - t0;
+  return PWH_OK;
 }
 
 int pwh_resident_set(pwh_system* s, const double* X, int nstates) {

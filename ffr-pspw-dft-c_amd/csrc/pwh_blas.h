@@ -15,6 +15,18 @@ hipError_t zero_lower(cx<double>* A, int n, int ld, hipStream_t s);
 constexpr int kBandSlices = 16;
 hipError_t band_sums(const cx<double>* X, int ld, int n, const double* kin, int nb, double* charge,
                      double* ksum, hipStream_t s);
+// block solvers: R = lam X - Y (mode 0) or Y - X (mode 1), norms[b] = ||R_b||; part holds
+// nb * kBandSlices doubles of partial sums
+hipError_t resid_block(const cx<double>* X, const cx<double>* Y, const double* lam, int ld, int n, int nb,
+                       int mode, cx<double>* R, double* part, double* norms, hipStream_t s);
+// out[b] = Re (X_b^H Y_b)
+hipError_t dotc_re(const cx<double>* X, const cx<double>* Y, int ld, int n, int nb, double* part, double* out,
+                   hipStream_t s);
+hipError_t scale_prec(cx<double>* R, int ld, int n, int nb, const double* res, const double* prec,
+                      hipStream_t s);
+hipError_t set_diag_block(cx<double>* M, int ld, int n, const double* vals, hipStream_t s);
+hipError_t fill_lower(cx<double>* A, int n, int ld, hipStream_t s);
+hipError_t zhegv_scale(cx<double>* V, int n, int ld, const double* b, hipStream_t s);
 hipError_t resid_norms(const cx<double>* H, const cx<double>* X, const double* lam, int ld, int n,
                        int nb, double* res, hipStream_t s);
 size_t proj_part_elems(int M, int N, int K);

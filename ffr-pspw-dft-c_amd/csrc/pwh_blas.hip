@@ -190,6 +190,147 @@ hipError_t band_sums(const cx<double>* X, int ld, int n, const double* kin, int 
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Block-solver helpers (davidson_cpu.cpp, lobpcg_cpu.cpp).  Per-band reductions run over
+// kBandSlices slices of 256-element chunks and are finished in slice order
+// (deterministic).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void slice_reduce(double v, double* sh, double* out) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) sh[t] += sh[t + h];
+    __syncthreads();
+  }
+  if (t == 0) *out = sh[0];
+}
+
+// mode 0: R = lam[b] X - Y (davidson_cpu.cpp:64-70); mode 1: R = Y - X (W = HX - X T,
+// lobpcg_cpu.cpp:98-104, with X holding X T).  part[b][slice] = sum |R|^2 over the slice.
+__global__ __launch_bounds__(256) void k_resid_block(const cx<double>* __restrict__ X,
+                                                     const cx<double>* __restrict__ Y,
+                                                     const double* __restrict__ lam, int ld, int n, int mode,
+                                                     cx<double>* __restrict__ R, double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int b = blockIdx.y, sl = blockIdx.x;
+  const size_t o = (size_t)b * ld;
+  const double l = mode == 0 ? lam[b] : 0.0;
+  double acc = 0.0;
+  for (int k = sl * 256 + threadIdx.x; k < n; k += 256 * kBandSlices) {
+    const cx<double> x = X[o + k], y = Y[o + k];
+    const cx<double> r = mode == 0 ? mk<double>(l * x.x - y.x, l * x.y - y.y) : mk<double>(y.x - x.x, y.y - x.y);
+    R[o + k] = r;
+    acc += r.x * r.x + r.y * r.y;
+  }
+  slice_reduce(acc, sh, part + (size_t)b * kBandSlices + sl);
+}
+
+// part[b][slice] = Re sum conj(X) Y over the slice (zdotc, davidson_cpu.cpp:58-61)
+__global__ __launch_bounds__(256) void k_dotc_re(const cx<double>* __restrict__ X, const cx<double>* __restrict__ Y,
+                                                 int ld, int n, double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int b = blockIdx.y, sl = blockIdx.x;
+  const size_t o = (size_t)b * ld;
+  double acc = 0.0;
+  for (int k = sl * 256 + threadIdx.x; k < n; k += 256 * kBandSlices) {
+    const cx<double> x = X[o + k], y = Y[o + k];
+    acc += x.x * y.x + x.y * y.y;
+  }
+  slice_reduce(acc, sh, part + (size_t)b * kBandSlices + sl);
+}
+
+// out[b] = sum of the slices (sqrt when root)
+__global__ void k_finish_slices(const double* __restrict__ part, int nb, int root, double* __restrict__ out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  double s = 0.0;
+  for (int p = 0; p < kBandSlices; ++p) s += part[(size_t)b * kBandSlices + p];
+  out[b] = root ? sqrt(s) : s;
+}
+
+hipError_t resid_block(const cx<double>* X, const cx<double>* Y, const double* lam, int ld, int n, int nb,
+                       int mode, cx<double>* R, double* part, double* norms, hipStream_t s) {
+  hipLaunchKernelGGL(k_resid_block, dim3(kBandSlices, nb), dim3(256), 0, s, X, Y, lam, ld, n, mode, R, part);
+  hipLaunchKernelGGL(k_finish_slices, dim3((nb + 63) / 64), dim3(64), 0, s, part, nb, 1, norms);
+  return hipGetLastError();
+}
+
+hipError_t dotc_re(const cx<double>* X, const cx<double>* Y, int ld, int n, int nb, double* part, double* out,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_dotc_re, dim3(kBandSlices, nb), dim3(256), 0, s, X, Y, ld, n, part);
+  hipLaunchKernelGGL(k_finish_slices, dim3((nb + 63) / 64), dim3(64), 0, s, part, nb, 0, out);
+  return hipGetLastError();
+}
+
+// R[b][k] = (R[b][k] * c_b) * prec[k] with c_b = 1/res[b] when res[b] > machine zero, else 1
+// (davidson_cpu.cpp:92-107); res null: c_b = 1 (lobpcg_cpu.cpp:126-131)
+__global__ void k_scale_prec(cx<double>* __restrict__ R, int ld, int n, const double* __restrict__ res,
+                             const double* __restrict__ prec) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (k >= n) return;
+  cx<double> v = R[(size_t)b * ld + k];
+  if (res) {
+    const double r = res[b];
+    const double c = r > 2.220446049250313e-16 ? 1.0 / r : 1.0;
+    v = mk<double>(v.x * c, v.y * c);
+  }
+  const double p = prec[k];
+  R[(size_t)b * ld + k] = mk<double>(v.x * p, v.y * p);
+}
+hipError_t scale_prec(cx<double>* R, int ld, int n, int nb, const double* res, const double* prec,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_prec, dim3((n + 255) / 256, nb), dim3(256), 0, s, R, ld, n, res, prec);
+  return hipGetLastError();
+}
+
+// Column-major M (ld): the leading n x n block becomes diag(vals) (vals null: identity)
+__global__ void k_set_diag_block(cx<double>* M, int ld, int n, const double* __restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= n) return;
+  M[i + (size_t)j * ld] = i == j ? mk<double>(vals ? vals[i] : 1.0, 0.0) : mk<double>(0.0, 0.0);
+}
+hipError_t set_diag_block(cx<double>* M, int ld, int n, const double* vals, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_diag_block, dim3((n + 63) / 64, n), dim3(64), 0, s, M, ld, n, vals);
+  return hipGetLastError();
+}
+
+// Column-major n x n: the strict lower triangle from the upper one, the diagonal real
+// (what zheevd with uplo = 'U' reads of a matrix)
+__global__ void k_fill_lower(cx<double>* A, int n, int ld) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i > j || j >= n) return;
+  if (i == j) {
+    A[i + (size_t)i * ld].y = 0.0;
+  } else {
+    A[j + (size_t)i * ld] = conjg(A[i + (size_t)j * ld]);
+  }
+}
+hipError_t fill_lower(cx<double>* A, int n, int ld, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_lower, dim3((n + 63) / 64, n), dim3(64), 0, s, A, n, ld);
+  return hipGetLastError();
+}
+
+// eig_zhegv.cpp:40-46: column i of V scaled by 1/sqrt(b[i]) when b[i] > SMALL
+__global__ void k_zhegv_scale(cx<double>* V, int n, int ld, const double* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= n) return;
+  const double e = b[j];
+  if (e > 12.220446049250313e-16) {
+    const double c = 1.0 / sqrt(e);
+    cx<double>& v = V[i + (size_t)j * ld];
+    v = mk<double>(v.x * c, v.y * c);
+  }
+}
+hipError_t zhegv_scale(cx<double>* V, int n, int ld, const double* b, hipStream_t s) {
+  hipLaunchKernelGGL(k_zhegv_scale, dim3((n + 63) / 64, n), dim3(64), 0, s, V, n, ld, b);
+  return hipGetLastError();
+}
+
 hipError_t resid_norms(const cx<double>* H, const cx<double>* X, const double* lam, int ld, int n,
                        int nb, double* res, hipStream_t s) {
   hipLaunchKernelGGL(k_resid, dim3(nb), dim3(256), 0, s, H, X, lam, ld, n, res);

@@ -41,7 +41,7 @@ EXPORTS = [
     "pwh_resident_apply", "pwh_resident_chebyfilt", "pwh_synchronize", "pwh_timing_enable",
     "pwh_timing_reset", "pwh_timing_get", "pwh_batch_create", "pwh_batch_destroy",
     "pwh_batch_set", "pwh_batch_get", "pwh_batch_chebyfilt", "pwh_fft3d", "pwh_eig_zheevd",
-    "pwh_eval_rho_psi",
+    "pwh_eval_rho_psi", "pwh_davidson", "pwh_lobpcg",
 ]
 
 
@@ -62,6 +62,16 @@ class SystemDesc(C.Structure):
                 ("tpiba2", C.c_double), ("nsp", C.c_int), ("nax", C.c_int), ("nlmax", C.c_int),
                 ("na", C.c_void_p), ("l_max", C.c_void_p), ("l_loc", C.c_void_p),
                 ("wnl", C.c_void_p), ("eigr", C.c_void_p), ("pkg_a", C.c_void_p)]
+
+
+DIAG_REPORT = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double)
+PWH_DIAG_ITER, PWH_DIAG_SMALL_OVERLAP, PWH_DIAG_RANK, PWH_DIAG_EIG_FAILED = 1, 2, 3, 4
+
+
+class DiagOpts(C.Structure):
+    _fields_ = [("max_iter", C.c_int), ("nempty", C.c_int), ("report", DIAG_REPORT),
+                ("report_ctx", C.c_void_p), ("iterations", C.c_int), ("nconv", C.c_int),
+                ("reason", C.c_int), ("t_total_s", C.c_double), ("band_applies", C.c_longlong)]
 
 
 class ChfsiOpts(C.Structure):
@@ -125,6 +135,8 @@ def lib():
         "pwh_fft3d": [vp, C.c_int, C.c_int, C.c_int, C.c_int],
         "pwh_eig_zheevd": [vp, C.c_int, _dp, C.c_int],
         "pwh_eval_rho_psi": [vp, vp, C.c_int, C.c_int, _dp, C.c_double, _dp, C.POINTER(C.c_double), _dp],
+        "pwh_davidson": [vp, vp, C.c_int, _dp, C.POINTER(DiagOpts), _dp, _dp],
+        "pwh_lobpcg": [vp, vp, C.c_int, _dp, C.POINTER(DiagOpts), _dp, _dp],
     }
     for name, args in proto.items():
         f = getattr(L, name)
@@ -366,6 +378,28 @@ class System:
                                       np.ascontiguousarray(w, dtype=np.float64), float(omega), rho,
                                       C.byref(ek), charge))
         return rho, ek.value, charge
+
+    def _diag(self, fn, X0, prec, max_iter, nempty):
+        X = np.array(_block(X0), order="C")
+        n = X.shape[0]
+        ev = np.zeros(n)
+        res = np.zeros(n)
+        events = []
+        cb = DIAG_REPORT(lambda ctx, e, a, b, v: events.append((e, a, b, v)))
+        o = DiagOpts(max_iter=int(max_iter if max_iter else max(30, n // 2)), nempty=int(nempty), report=cb)
+        _check(fn(self.h, _zptr(X), n, np.ascontiguousarray(prec, dtype=np.float64), C.byref(o), ev, res))
+        stats = {f: getattr(o, f) for f, _ in DiagOpts._fields_ if f not in ("report", "report_ctx")}
+        stats["events"] = events
+        return ev, X, res, stats
+
+    def davidson(self, X0, prec, max_iter=None, nempty=0):
+        """davidson_cpu.cpp:9-247 from start vectors X0 (nstates, ngw).  Returns (Ritz values,
+        final block, residual norms, stats incl. the reported events)."""
+        return self._diag(lib().pwh_davidson, X0, prec, max_iter, nempty)
+
+    def lobpcg(self, X0, prec, max_iter=None, nempty=0):
+        """lobpcg_cpu.cpp:32-389 from start vectors X0 (nstates, ngw)."""
+        return self._diag(lib().pwh_lobpcg, X0, prec, max_iter, nempty)
 
     def lanczos(self, v0, k):
         """lanczos.cpp:8-101.  Returns (T[k,k] as the reference fills it, f, v0 normalised)."""

@@ -18,3 +18,7 @@ double *WKPT = 0;
 double *FOCC = 0;
 double EKIN = 0.0;
 zcomplex *C0 = 0;
+bool USER_MAX_DIAG_ITER = false;
+int MAX_DIAG_ITER = 0;
+bool DIAG_VERBOSE = false;
+int NEMPTY = 0;

@@ -10,6 +10,8 @@
 //   eig_zheevd      eig_zheevd.cpp:5-69        -> pwh_eig_zheevd (both overloads)
 //   fft_fftw3       fft_fftw3.cpp:13-46        -> pwh_fft3d
 //   EvalRhoPsi      EvalRhoPsi.cpp:5-157       -> pwh_eval_rho_psi (both overloads)
+//   davidson_cpu    davidson_cpu.cpp:9-247     -> pwh_davidson
+//   lobpcg_cpu      lobpcg_cpu.cpp:32-389      -> pwh_lobpcg
 //
 // Link it in place of those .cpp files.  It reads the reference's globals
 // (global_variables.cpp) and keeps one device system per k-point, rebuilt when the
@@ -41,6 +43,9 @@ extern zcomplex *EIGR;
 // read and written by EvalRhoPsi (EvalRhoPsi.cpp:20-31,154)
 extern double OMEGA, *WKPT, *FOCC, EKIN;
 extern zcomplex *C0;
+// read by davidson_cpu / lobpcg_cpu (common_pspw.h:47-49,67)
+extern bool USER_MAX_DIAG_ITER, DIAG_VERBOSE;
+extern int MAX_DIAG_ITER, NEMPTY;
 
 namespace {
 
@@ -105,6 +110,61 @@ void check_nnr(int nnr) {
     fflush(stdout);
     abort();
   }
+}
+
+// util_random_cpu.cpp:5-25: column generator of the start vectors; each part is
+// (a % 100 + 1) / a with a the seed after one step of a <- ((214013 a + 2531011) >> 16) & 32767
+void random_col(zcomplex *A, int nelem, long a) {
+  double *d = reinterpret_cast<double *>(A);
+  for (int i = 0; i < 2 * nelem; ++i) {
+    a = ((a * 214013L + 2531011L) >> 16) & 32767;
+    const long r = a % 100 + 1;
+    d[i] = (double)r / (double)a;
+  }
+}
+
+// davidson_cpu.cpp:47-51, lobpcg_cpu.cpp:80-84: column ist from seed 100 + ist - 1
+void start_block(zcomplex *X, int nbasis, int nstates) {
+  for (int ist = 0; ist < nstates; ++ist) random_col(X + (size_t)nbasis * ist, nbasis, 100 + ist);
+}
+
+int diag_max_iter(int nstates) {
+  if (USER_MAX_DIAG_ITER) return MAX_DIAG_ITER;
+  return nstates / 2 > 30 ? nstates / 2 : 30;
+}
+
+// The lines the reference prints inside the solvers (verbose progress, eig_zhegv warnings)
+struct DiagPrint {
+  int lobpcg;
+};
+void diag_report(void *ctx, int event, int a, int b, double value) {
+  const DiagPrint *p = static_cast<const DiagPrint *>(ctx);
+  switch (event) {
+    case PWH_DIAG_ITER:
+      if (DIAG_VERBOSE) {
+        if (p->lobpcg) printf("iter = %5d %5d %18.10f\n", a, b, value);
+        else printf("davidson_cpu: %5d %5d %18.10f\n", a, b, value);
+      }
+      break;
+    case PWH_DIAG_SMALL_OVERLAP:
+      printf("Small eigenvalue detected = %f\n", value);
+      break;
+    case PWH_DIAG_RANK:
+      printf("Warning: Number of linearly independent vectors = %d\n", a);
+      printf("         while size of the problem: %d\n", b);
+      break;
+    case PWH_DIAG_EIG_FAILED:
+      printf("ERROR calling zheevd in eig_zhegv : info = %d\n", a);
+      break;
+  }
+}
+
+pwh_system *diag_system(int ik, int nbasis, double *rhoe, int nnr) {
+  check_nnr(nnr);
+  if (nbasis != NGW[ik - 1]) die("block solver: nbasis differs from NGW(ik)");
+  pwh_system *s = system_for(ik);
+  if (pwh_set_vloc(s, rhoe, nnr) != PWH_OK) die("block solver");
+  return s;
 }
 
 }  // namespace
@@ -206,4 +266,77 @@ void EvalRhoPsi(int ik, double *rhoe, zcomplex *eigvec, int ldEigvec, int NGW_ik
 // EvalRhoPsi.cpp:5-79: the same with ldEigvec = NGWX
 void EvalRhoPsi(int ik, double *rhoe, zcomplex *eigvec, int NGW_ik) {
   EvalRhoPsi(ik, rhoe, eigvec, NGWX, NGW_ik);
+}
+
+// davidson_cpu.cpp:9-247.  The start vectors come from random_col as in the reference;
+// everything else runs on the GPU (pwh_davidson).  The printed lines follow the
+// reference's.
+void davidson_cpu(int ik, double *evals, zcomplex *X, int nbasis, int nstates, double *rhoe, int nnr,
+                  double *prec) {
+  const int nstates2 = 2 * nstates;
+  double mem = 4.0 * nbasis * nstates * sizeof(zcomplex);
+  mem = mem + 3.0 * nstates2 * nstates2 * sizeof(zcomplex);
+  mem = mem + 4.0 * nstates * sizeof(double);
+  printf("Allocated dynamic memory in davidson_cpu: %15.10f MB\n", mem / 1024. / 1024.);
+  pwh_system *s = diag_system(ik, nbasis, rhoe, nnr);
+  start_block(X, nbasis, nstates);
+  double *res = static_cast<double *>(malloc(sizeof(double) * nstates));
+  DiagPrint pr = {0};
+  pwh_diag_opts o;
+  memset(&o, 0, sizeof(o));
+  o.max_iter = diag_max_iter(nstates);
+  o.nempty = NEMPTY;
+  o.report = diag_report;
+  o.report_ctx = &pr;
+  if (pwh_davidson(s, reinterpret_cast<double *>(X), nstates, prec, &o, evals, res) != PWH_OK) die("davidson_cpu");
+  if (o.reason == 1) printf("In iter = %5d: nconv = %5d\n", o.iterations, o.nconv);
+  if (o.reason == 2)
+    printf("davidson_cpu: Convergence achieved in iter = %5d: nconv = %5d\n", o.iterations, o.nconv);
+  double rnorm = 0.0;
+  printf("\n");
+  printf("Final Davidson result nconv=%d from %d states:\n", o.nconv, nstates);
+  for (int ist = 0; ist < nstates; ++ist) {
+    printf("%5d %18.10f %18.10f\n", ist + 1, evals[ist], res[ist]);
+    rnorm = rnorm + res[ist];
+  }
+  rnorm = rnorm / (double)nstates;
+  printf("End of Davidson iteration: rnorm = %18.10f\n", rnorm);
+  free(res);
+}
+
+// lobpcg_cpu.cpp:32-389 on the GPU (pwh_lobpcg), start vectors from random_col.
+void lobpcg_cpu(int ik, double *lambda, zcomplex *X, int nbasis, int nstates, double *rhoe, int nnr,
+                double *prec) {
+  const int nstates3 = 3 * nstates;
+  double mem = (7.0 * nbasis * nstates3 + nstates * nstates + 3.0 * nstates3 * nstates3) * 16.0;
+  mem = mem + nstates * 8.0;
+  printf("Allocated dynamic memory in lobpcg_cpu: %15.10f MB\n", mem / 1024. / 1024.);
+  pwh_system *s = diag_system(ik, nbasis, rhoe, nnr);
+  start_block(X, nbasis, nstates);
+  double *res = static_cast<double *>(malloc(sizeof(double) * nstates));
+  DiagPrint pr = {1};
+  pwh_diag_opts o;
+  memset(&o, 0, sizeof(o));
+  o.max_iter = diag_max_iter(nstates);
+  o.nempty = NEMPTY;
+  o.report = diag_report;
+  o.report_ctx = &pr;
+  const int rc = pwh_lobpcg(s, reinterpret_cast<double *>(X), nstates, prec, &o, lambda, res);
+  if (rc == PWH_ERR_LAPACK) {
+    printf("ERROR calculating Cholesky decomposition\n");
+    fflush(stdout);
+    abort();
+  }
+  if (rc != PWH_OK) die("lobpcg_cpu");
+  if (o.reason == 1 && o.iterations == 1)
+    printf("Convergence achieved for %d eigenvalues in LOBPCG at first iteration.\n", o.nconv);
+  else if (o.reason == 2 && o.iterations == 1)
+    printf("Convergence achieved for %d eigenvalues in LOBPCG (rnorm).\n", o.nconv);
+  else if (o.reason == 1)
+    printf("Convergence achieved for %d eigenvalues in LOBPCG at iter = %d\n", o.nconv, o.iterations);
+  else if (o.reason == 2)
+    printf("Convergence achieved for %d eigenvalues in LOBPCG (rnorm) at iter = %d\n", o.nconv, o.iterations);
+  printf("Final LOBPCG result\n");
+  for (int i = 0; i < nstates; ++i) printf("%d %18.10f %18.10f\n", i + 1, lambda[i], res[i]);
+  free(res);
 }

@@ -149,6 +149,38 @@ typedef struct {
 int pwh_chfsi(pwh_system *sys, double *X, int nstates, double *v0, pwh_chfsi_opts *opts,
               double *eig, double *res);
 
+/* ---- block eigensolvers of the SCF driver ------------------------------------ */
+/* Events passed to pwh_diag_opts.report (the lines the reference prints, in order). */
+#define PWH_DIAG_ITER 1           /* top of an outer iteration: a = iteration, b = nconv, value = mean residual */
+#define PWH_DIAG_SMALL_OVERLAP 2  /* eig_zhegv.cpp:47-49: overlap eigenvalue a (1-based) of b is <= 12.22e-16 (value) */
+#define PWH_DIAG_RANK 3           /* eig_zhegv.cpp:51-54: a linearly independent vectors of b */
+#define PWH_DIAG_EIG_FAILED 4     /* zheevd failed with info a on an order-b matrix (the reference prints, goes on) */
+typedef void (*pwh_diag_report_fn)(void *ctx, int event, int a, int b, double value);
+typedef struct {
+  int max_iter;                 /* in: outer iterations (the reference: max(30, nstates/2), or MAX_DIAG_ITER) */
+  int nempty;                   /* in: NEMPTY; stop when nconv >= nstates - nempty */
+  pwh_diag_report_fn report;    /* in: optional progress / warning callback */
+  void *report_ctx;
+  int iterations;               /* out: outer iterations run */
+  int nconv;                    /* out: residual norms below 1e-6 at exit */
+  int reason;                   /* out: 0 iterations exhausted, 1 nconv test, 2 mean-residual test */
+  double t_total_s;             /* out: wall time */
+  long long band_applies;       /* out: H|psi> band applications */
+} pwh_diag_opts;
+/* davidson_cpu.cpp:9-247 on the GPU (FP64): X [ngw x nstates] holds the start vectors
+ * (the reference's random_col columns; orthonormalised here as by ortho_qr) and receives
+ * the final block; prec [ngw] the preconditioner (GenPrec.cpp); evals [nstates] the
+ * Ritz values, res [nstates] the residual norms of the final block.  Reduced problems of
+ * order 2 nstates are solved as eig_zhegv_eval does. */
+int pwh_davidson(pwh_system *sys, double *X, int nstates, const double *prec, pwh_diag_opts *opts,
+                 double *evals, double *res);
+/* lobpcg_cpu.cpp:32-389 on the GPU (FP64), same arguments: lambda [nstates] the
+ * eigenvalues of X^H H X and res [nstates] the residual norms from the last convergence
+ * test.  A Cholesky failure of a block overlap returns PWH_ERR_LAPACK (the reference
+ * prints "ERROR calculating Cholesky decomposition" and aborts). */
+int pwh_lobpcg(pwh_system *sys, double *X, int nstates, const double *prec, pwh_diag_opts *opts,
+               double *lambda, double *res);
+
 /* ---- device-resident entry points (inputs already in HBM; used for timing) --- */
 /* Copy a block into / out of the system's resident FP64 wave-function buffer.  The
  * host-pointer entry points above stage their input in that same buffer, so a call to

@@ -373,6 +373,7 @@ class DiagResult:
     nconv: int
     reason: int          # 0: iterations exhausted, 1: nconv >= nstates - nempty, 2: rnorm
     history: list        # (iteration, nconv, rnorm) at the top of each iteration
+    evals_history: list  # eigenvalue estimates after each iteration
 
 
 def davidson(H, X0, prec, nempty=0, max_iter=None, ortho=None):
@@ -389,7 +390,7 @@ def davidson(H, X0, prec, nempty=0, max_iter=None, ortho=None):
     if max_iter is None:
         max_iter = max(30, n // 2)
     mz, tol = 2.220446049250313e-16, 1.0e-6
-    rnorm, nconv, reason, hist, it = 1.0, 0, 0, [], 0
+    rnorm, nconv, reason, hist, it, eh = 1.0, 0, 0, [], 0, []
     for it in range(1, max_iter + 1):
         hist.append((it, nconv, rnorm))
         sc = np.where(res > mz, 1.0 / np.where(res > mz, res, 1.0), 1.0)
@@ -407,6 +408,7 @@ def davidson(H, X0, prec, nempty=0, max_iter=None, ortho=None):
         o[n:, n:] = _cn(R, R)
         er, E, _, _ = eig_zhegv_eval(h, o)
         evals = er[:n].copy()
+        eh.append(evals)
         X = _nn(X, E[:n, :n]) + _nn(R, E[n:, :n])
         HX = _nn(HX, E[:n, :n]) + _nn(HR, E[n:, :n])
         R = evals[:, None] * X - HX
@@ -419,7 +421,7 @@ def davidson(H, X0, prec, nempty=0, max_iter=None, ortho=None):
         if rnorm < tol / 10.0:
             reason = 2
             break
-    return DiagResult(evals, X, res, it, nconv, reason, hist)
+    return DiagResult(evals, X, res, it, nconv, reason, hist, eh)
 
 
 def lobpcg(H, X0, prec, nempty=0, max_iter=None, ortho=None):
@@ -435,7 +437,7 @@ def lobpcg(H, X0, prec, nempty=0, max_iter=None, ortho=None):
     HX = H(X)
     P = HP = None
     lam = res = None
-    nconv, reason, hist, it = 0, 0, [], 0
+    nconv, reason, hist, it, eh = 0, 0, [], 0, []
     rnorm = 0.0
 
     def chol_upper(Cm):
@@ -453,6 +455,7 @@ def lobpcg(H, X0, prec, nempty=0, max_iter=None, ortho=None):
         t1 = _cn(X, HX)
         W = HX - _nn(X, t1)
         lam = np.linalg.eigh(t1, UPLO="U")[0]
+        eh.append(lam)
         res = np.sqrt(np.real(np.sum(W.conj() * W, axis=1)))
         nconv = int(np.sum(res < tol))
         rnorm = float(np.mean(res))
@@ -481,7 +484,7 @@ def lobpcg(H, X0, prec, nempty=0, max_iter=None, ortho=None):
             HP = _nn(HW, Uv[n:2 * n, :n]) + _nn(HP, Uv[2 * n:, :n])
             U = chol_upper(_cn(P, P))
             P, HP = right_solve(P, U), right_solve(HP, U)
-    return DiagResult(lam, X, res, it, nconv, reason, hist)
+    return DiagResult(lam, X, res, it, nconv, reason, hist, eh)
 
 
 class Reference:

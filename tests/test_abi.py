@@ -20,7 +20,8 @@ HEADER = os.path.join(ROOT, "include", "pwh.h")
 REF_SYMBOLS = ["_Z14ApplyHam_blockPCdS0_iiPdii", "_Z8ApplyHamPCdS0_iiPdi",
                "_Z9chebyfiltiPCdiddPdi", "_Z8ortho_qrPCdii", "_Z7lanczosiPCdiPdiS0_S1_i",
                "_Z10eig_zheevdPCdiPdi", "_Z10eig_zheevdPCdii", "_Z9fft_fftw3PCdiiib",
-               "_Z10EvalRhoPsiiPdPCdi", "_Z10EvalRhoPsiiPdPCdii"]
+               "_Z10EvalRhoPsiiPdPCdi", "_Z10EvalRhoPsiiPdPCdii", "_Z12davidson_cpuiPdPCdiiS_iS_",
+               "_Z10lobpcg_cpuiPdPCdiiS_iS_"]
 
 
 def declared_functions():
