@@ -237,6 +237,39 @@ def rho_section(s, sysm, psi, info, steps, cpu):
     return out
 
 
+def block_solvers(s, sysm, iters, cpu):
+    """LOBPCG and Davidson (lobpcg_cpu.cpp, davidson_cpu.cpp) on the whole block from the
+    reference's start vectors and GenPrec preconditioner (tau = -10, SCFLoop.cpp:160-169),
+    a fixed number of outer iterations; the oracle's LOBPCG (one iteration) beside it."""
+    from pwh import synth
+    nb = s.nbands
+    X0 = synth.start_vectors(s.ngw, nb)
+    prec = synth.gen_prec(s.gv["xkg"][:s.ngw], s.tpiba2, s.a, -10.0)
+    out = {"states": nb, "iterations": iters}
+    sysm.lobpcg(X0, prec, max_iter=1)          # first call: solver modules and buffers
+    sysm.davidson(X0, prec, max_iter=1)
+    for name in ("lobpcg", "davidson"):
+        ev, _, res, st = getattr(sysm, name)(X0, prec, max_iter=iters)
+        out[name] = {"s_per_iter": round(st["t_total_s"] / st["iterations"], 5),
+                     "iterations": st["iterations"], "band_applies": st["band_applies"],
+                     "max_residual": float(res.max()), "lowest": float(ev[0])}
+    if cpu:
+        from oracle import oracle as O
+        g = s.globals()
+        og = O.Globals(nr=s.nr, ngwx=g.ngwx, n123=s.gv["n123"], xkg=s.gv["xkg"], tpiba2=g.tpiba2,
+                       na=g.na, nax=g.nax, l_max=g.l_max, l_loc=g.l_loc, nlmax=g.nlmax, wnl=g.wnl,
+                       eigr=g.eigr, pkg_a=g.pkg_a)
+        H = lambda X: O.apply_ham_block(og, np.ascontiguousarray(X), s.vloc, s.ngw)
+        t0 = time.perf_counter()
+        O.lobpcg(H, X0, prec, max_iter=1)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(1.0 / dt, 4), "unit": "LOBPCG iterations/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"one LOBPCG iteration (start block, {2 * nb} band H|psi>, Rayleigh-Ritz on "
+                                         f"[X W]) of the oracle's restatement, 1 thread for H|psi>, {dt:.1f} s"}
+    return out
+
+
 def c3_batch(steps, degree=10):
     """24 uneven cells filtered together (one captured graph) vs one at a time."""
     import pwh
@@ -436,6 +469,11 @@ def main():
         rho = rho_section(s, sysm, psi, info, steps=max(5, min(args.steps, 20)), cpu=not args.no_cpu)
         log("bench: eval_rho_psi " + json.dumps(rho))
 
+    solvers = None
+    if not args.only and not args.no_chfsi:
+        solvers = block_solvers(s, sysm, iters=10, cpu=not args.no_cpu)
+        log("bench: block solvers " + json.dumps(solvers))
+
     extra = {}
     if not args.no_extra and not args.only:
         extra["c3_batched_uneven"] = c3_batch(steps=5)
@@ -473,6 +511,7 @@ def main():
         "kernels": per_kernel,
         "chfsi": chfsi or None,
         "eval_rho_psi": rho,
+        "block_solvers": solvers,
         "c3_batched_uneven": extra.get("c3_batched_uneven"),
         "c4_si216": extra.get("c4_si216"),
         "cpu_baseline": cpu,

@@ -169,3 +169,29 @@ def uneven_batch(count=24, seed=2026):
         nb = int(min(96, max(8, 2 * nat + 4)))
         out.append(make_system(a, ecut, 1, (n, n, n), nb, name=f"cell{i}", frac=frac))
     return out
+
+
+def start_vectors(nbasis, nstates):
+    """The start block of the reference's Davidson and LOBPCG (davidson_cpu.cpp:47-51,
+    lobpcg_cpu.cpp:80-84): column ist from random_col with seed 100 + ist - 1
+    (util_random_cpu.cpp:5-25: each part is (a % 100 + 1) / a after one step of
+    a <- ((214013 a + 2531011) >> 16) & 32767).  Rows = states; the columns run side by
+    side."""
+    a = np.arange(100, 100 + nstates, dtype=np.int64)
+    out = np.empty((nstates, 2 * nbasis))
+    for i in range(2 * nbasis):
+        a = ((a * 214013 + 2531011) >> 16) & 32767
+        out[:, i] = (a % 100 + 1) / a
+    return np.ascontiguousarray(out.view(np.complex128))
+
+
+def gen_prec(xkg, tpiba2, alat, tau):
+    """GenPrec.cpp:5-27, the preconditioner Diagon_*_cpu hands to the block solvers:
+    y / (y + 16 x^4), y = 27 + x (18 + x (12 + 8 x)), x = |G|^2 for tau <= 0, else
+    tau / (0.5 (XKG + 1) TPIBA2)."""
+    xkg = np.asarray(xkg, np.float64)
+    if tau <= 0.0:
+        x = xkg * (2.0 * np.pi / alat) ** 2
+        y = 27.0 + x * (18.0 + x * (12.0 + 8.0 * x))
+        return y / (y + 16.0 * x ** 4)
+    return tau / (0.5 * (xkg + 1.0) * tpiba2)

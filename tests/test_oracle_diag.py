@@ -30,6 +30,16 @@ def test_random_col_matches_reference_bit_for_bit(oracle):
     assert np.array_equal(X[3], R.random_col(50, 103))
 
 
+def test_synth_start_vectors_and_prec_equal_the_oracle(oracle):
+    """The package's vectorised start block and preconditioner (used by bench.py) are the
+    oracle's random_col / GenPrec restatements."""
+    from pwh import synth
+    assert np.array_equal(synth.start_vectors(700, 5), oracle.start_vectors(700, 5))
+    x = np.linspace(0.0, 40.0, 101)
+    for tau in (-10.0, 0.3):
+        assert np.array_equal(synth.gen_prec(x, 0.4, 10.26, tau), oracle.gen_prec(x, 0.4, 10.26, tau))
+
+
 def test_random_col_known_answer(oracle):
     """First element for seed 100: a1 = (100 * 214013 + 2531011) >> 16 & 32767 = 365,
     a2 = (365 * 214013 + 2531011) >> 16 & 32767 = 1230 -> (66/365, 31/1230)."""
