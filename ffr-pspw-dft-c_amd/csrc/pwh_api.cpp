@@ -14,12 +14,14 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "../../include/pwh.h"
 #include "pwh_blas.h"
 #include "pwh_internal.h"
+#include "pwh_scf.h"
 
 using namespace pwh;
 
@@ -1218,6 +1220,8 @@ done:
 // lobpcg_cpu.cpp:32-389.  Blocks stay in HBM in sphere order for the whole solve; the
 // host sees the per-band residual norms once per iteration (the convergence tests).
 // ---------------------------------------------------------------------------
+}  // extern "C"
+
 namespace {
 
 constexpr double kDiagTol = 1.0e-6;                     // davidson_cpu.cpp:85, lobpcg_cpu.cpp:35
@@ -1374,6 +1378,8 @@ int check_diag_args(pwh_system* s, double* X, int n, const double* prec, pwh_dia
 
 }  // namespace
 
+extern "C" {
+
 int pwh_davidson(pwh_system* s, double* X, int nstates, const double* prec, pwh_diag_opts* o, double* evals,
                  double* res) {
   CHK(check_diag_args(s, X, nstates, prec, o, evals, res));
@@ -1517,6 +1523,592 @@ int pwh_lobpcg(pwh_system* s, double* X, int nstates, const double* prec, pwh_di
   o->nconv = nconv;
   CHK(store_block(s, Xd, X, n));
   o->t_total_s = now_s() - t0;
+  return PWH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// SCF steps around the eigensolver: EvalEnergy.cpp, xc_PZ.cpp, the density mixers,
+// NormalizeRho.cpp and EvalENL.cpp, FP64 on the device.  The entry points take host
+// arrays as the reference's do; reductions are per-block partials added on the host in
+// block order (deterministic).
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct pwh_grid {
+  int nr[3] = {0, 0, 0};
+  size_t nnr = 0;
+  int ng = 0, ngw = 0;
+  double tpiba2 = 0, omega = 0;
+  Buf<int> gidx, kidx;
+  Buf<double> g2, xkg;
+  Buf<cx<double>> vp, rp, tw[3], c;
+};
+
+namespace {
+
+// device vectors of the grid-sized steps (one set for the library; the library runs one
+// caller thread)
+struct ScfWork {
+  Buf<double> v[24];
+  Buf<double> part, vxc, eps;
+  Buf<unsigned char> neg;
+  Buf<double> small;
+  Buf<rocblas_int> ipiv, info;
+};
+ScfWork& scfw() {
+  static ScfWork* w = new ScfWork();
+  return *w;
+}
+
+double sum_partials(const double* d, int n, double* host) {
+  hipMemcpyAsync(host, d, sizeof(double) * n, hipMemcpyDeviceToHost, g.stream);
+  hipStreamSynchronize(g.stream);
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += host[i];
+  return s;
+}
+
+// sum over the grid of x*y (device vectors), block partials added in order
+int dot_dev(const double* x, const double* y, size_t n, double* out) {
+  ScfWork& w = scfw();
+  CHK(w.part.ensure(4 * kScfBlocks));
+  HIPCHK(dot_partials(x, y, n, w.part.p, g.stream));
+  std::vector<double> h(kScfBlocks);
+  *out = sum_partials(w.part.p, kScfBlocks, h.data());
+  return PWH_OK;
+}
+
+// out = sum_k w_k v_k; with p: returns sum (p - q)^2 in *sq
+int comb_dev(std::initializer_list<std::pair<double, const double*>> terms, double* out, size_t n,
+             double* out2 = nullptr, const double* p = nullptr, const double* q = nullptr, double* sq = nullptr) {
+  Comb cb{};
+  cb.nt = 0;
+  for (auto& t : terms) {
+    if (cb.nt == kMaxTerms) return fail(PWH_ERR_ARG, "too many terms");
+    cb.w[cb.nt] = t.first;
+    cb.v[cb.nt] = t.second;
+    ++cb.nt;
+  }
+  cb.out = out;
+  cb.out2 = out2;
+  cb.p = p;
+  cb.q = q;
+  ScfWork& w = scfw();
+  CHK(w.part.ensure(4 * kScfBlocks));
+  HIPCHK(comb(cb, n, p ? w.part.p : nullptr, g.stream));
+  if (p && sq) {
+    std::vector<double> h(kScfBlocks);
+    *sq = sum_partials(w.part.p, kScfBlocks, h.data());
+  }
+  return PWH_OK;
+}
+
+int scf_up(Buf<double>& b, const double* h, size_t n) { return upload(b, h, n); }
+int scf_down(double* h, const double* d, size_t n) {
+  HIPCHK(hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost, g.stream));
+  return PWH_OK;
+}
+int scf_sync() {
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return PWH_OK;
+}
+
+// 3D FFT of the grid's complex work array (fft_fftw3.cpp conventions)
+int grid_fft(pwh_grid* gr, cx<double>* c, bool inverse) {
+  const int sign = inverse ? +1 : -1;
+  const double scale = inverse ? 1.0 : 1.0 / (double)gr->nnr;
+  const int n1 = gr->nr[0], n2 = gr->nr[1], n3 = gr->nr[2];
+  HIPCHK(launch_fft_lines(c, n3, n1 * n2, n1 * n2, gr->tw[2].p, sign, 1.0, g.stream));
+  HIPCHK(launch_fft_lines(c, n2, n1, n1 * n3, gr->tw[1].p, sign, 1.0, g.stream));
+  HIPCHK(launch_fft_lines(c, n1, 1, n2 * n3, gr->tw[0].p, sign, scale, g.stream));
+  return PWH_OK;
+}
+
+// KerkerRhoMix.cpp:66-79 on the device vector x (in place)
+int kerker_dev(pwh_grid* gr, double* x, double beta, double q0) {
+  CHK(gr->c.ensure(gr->nnr));
+  HIPCHK(real_to_cx(x, nullptr, gr->nnr, gr->c.p, g.stream));
+  CHK(grid_fft(gr, gr->c.p, false));
+  HIPCHK(kerker_filter(gr->c.p, gr->kidx.p, gr->xkg.p, gr->ngw, beta, q0, g.stream));
+  CHK(grid_fft(gr, gr->c.p, true));
+  HIPCHK(cx_real(gr->c.p, gr->nnr, x, g.stream));
+  return PWH_OK;
+}
+
+// NormalizeRho.cpp:7-33 on a device vector; *diff = electron-count difference
+int normalize_dev(double* x, size_t n, double omega, double nel, double* diff) {
+  ScfWork& w = scfw();
+  CHK(w.part.ensure(4 * kScfBlocks));
+  HIPCHK(clip_sum(x, n, w.part.p, g.stream));
+  std::vector<double> h(kScfBlocks);
+  const double sum = sum_partials(w.part.p, kScfBlocks, h.data());
+  const double truesum = (double)n * nel / omega;
+  if (sum < 1.0e-15) return fail(PWH_ERR_ARG, "NormalizeRho: sum(rho) < 1e-15");
+  if (diff) *diff = (truesum - sum) * omega / (double)n;
+  HIPCHK(scale_vec(x, n, truesum, sum, g.stream));   // rho * truesum / sum (NormalizeRho.cpp:29-31)
+  return PWH_OK;
+}
+
+// Solve A x = b (m x m, column-major) on the device with LU; b is replaced by x
+int lu_solve_dev(std::vector<double>& A, std::vector<double>& b, int m) {
+  ScfWork& w = scfw();
+  CHK(w.small.ensure((size_t)m * m + m));
+  CHK(w.ipiv.ensure(m));
+  CHK(w.info.ensure(1));
+  HIPCHK(hipMemcpyAsync(w.small.p, A.data(), sizeof(double) * m * m, hipMemcpyHostToDevice, g.stream));
+  HIPCHK(hipMemcpyAsync(w.small.p + (size_t)m * m, b.data(), sizeof(double) * m, hipMemcpyHostToDevice,
+                        g.stream));
+  RBCHK(rocsolver_dgetrf(g.rb, m, m, w.small.p, m, w.ipiv.p, w.info.p));
+  rocblas_int info = 0;
+  HIPCHK(hipMemcpyAsync(&info, w.info.p, sizeof(info), hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  if (info != 0) return fail(PWH_ERR_LAPACK, "dgetrf info " + std::to_string(info));
+  RBCHK(rocsolver_dgetrs(g.rb, rocblas_operation_none, m, 1, w.small.p, m, w.ipiv.p, w.small.p + (size_t)m * m,
+                         m));
+  HIPCHK(hipMemcpyAsync(b.data(), w.small.p + (size_t)m * m, sizeof(double) * m, hipMemcpyDeviceToHost,
+                        g.stream));
+  return scf_sync();
+}
+
+// Inverse of the leading m x m block of A (column-major, ld) on the device (dgetrf + dgetri)
+int lu_inverse_dev(std::vector<double>& A, int m, int ld) {
+  ScfWork& w = scfw();
+  CHK(w.small.ensure((size_t)m * m));
+  CHK(w.ipiv.ensure(m));
+  CHK(w.info.ensure(1));
+  std::vector<double> Ap((size_t)m * m);
+  for (int j = 0; j < m; ++j)
+    for (int i = 0; i < m; ++i) Ap[i + (size_t)j * m] = A[i + (size_t)j * ld];
+  HIPCHK(hipMemcpyAsync(w.small.p, Ap.data(), sizeof(double) * m * m, hipMemcpyHostToDevice, g.stream));
+  RBCHK(rocsolver_dgetrf(g.rb, m, m, w.small.p, m, w.ipiv.p, w.info.p));
+  rocblas_int info = 0;
+  HIPCHK(hipMemcpyAsync(&info, w.info.p, sizeof(info), hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  if (info != 0) return fail(PWH_ERR_LAPACK, "dgetrf info " + std::to_string(info));
+  RBCHK(rocsolver_dgetri(g.rb, m, w.small.p, m, w.ipiv.p, w.info.p));
+  HIPCHK(hipMemcpyAsync(&info, w.info.p, sizeof(info), hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipMemcpyAsync(Ap.data(), w.small.p, sizeof(double) * m * m, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  if (info != 0) return fail(PWH_ERR_LAPACK, "dgetri info " + std::to_string(info));
+  for (int j = 0; j < m; ++j)
+    for (int i = 0; i < m; ++i) A[i + (size_t)j * ld] = Ap[i + (size_t)j * m];
+  return PWH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pwh_grid_create(const pwh_grid_desc* d, pwh_grid** out) {
+  if (!d || !out) return fail(PWH_ERR_ARG, "null argument");
+  *out = nullptr;
+  const int nr[3] = {d->nr1, d->nr2, d->nr3};
+  for (int k = 0; k < 3; ++k)
+    if (!fft_size_supported(nr[k]))
+      return fail(PWH_ERR_UNSUPPORTED, "no compiled FFT plan for length " + std::to_string(nr[k]));
+  if (d->ng < 1 || !d->n1 || !d->n2 || !d->n3 || !d->g2 || d->nsp < 1 || !d->sfac || !d->vps || !d->rhops ||
+      !(d->omega > 0) || !(d->tpiba2 > 0) || d->ngw < 0 || (d->ngw > 0 && (!d->n123 || !d->xkg)))
+    return fail(PWH_ERR_ARG, "bad grid description");
+  CHK(ensure_init());
+  std::unique_ptr<pwh_grid> gr(new pwh_grid());
+  for (int k = 0; k < 3; ++k) gr->nr[k] = nr[k];
+  gr->nnr = (size_t)nr[0] * nr[1] * nr[2];
+  gr->ng = d->ng;
+  gr->ngw = d->ngw;
+  gr->tpiba2 = d->tpiba2;
+  gr->omega = d->omega;
+  std::vector<int> gi(d->ng);
+  for (int ig = 0; ig < d->ng; ++ig) {
+    const int a = d->n1[ig] - 1, b = d->n2[ig] - 1, c = d->n3[ig] - 1;
+    if (a < 0 || a >= nr[0] || b < 0 || b >= nr[1] || c < 0 || c >= nr[2])
+      return fail(PWH_ERR_ARG, "G-vector grid index out of range");
+    gi[ig] = a + nr[0] * (b + nr[1] * c);
+  }
+  if (gi[0] != 0) return fail(PWH_ERR_ARG, "the first G-vector must be G = 0 (EvalEnergy.cpp:55-57)");
+  {
+    // each G-vector owns its grid point (a grid too small for the cutoff would alias them)
+    std::vector<int> sorted(gi);
+    std::sort(sorted.begin(), sorted.end());
+    if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+      return fail(PWH_ERR_ARG, "two G-vectors share a grid point: the grid is too small for the cutoff");
+  }
+  // species summed in order (EvalEnergy.cpp:51-53, 62-65)
+  std::vector<cx<double>> vp(d->ng, mk<double>(0, 0)), rp(d->ng, mk<double>(0, 0));
+  for (int ig = 0; ig < d->ng; ++ig)
+    for (int is = 0; is < d->nsp; ++is) {
+      const size_t k = (size_t)is + (size_t)d->nsp * ig;
+      const double sx = d->sfac[2 * k], sy = d->sfac[2 * k + 1];
+      vp[ig] = mk<double>(vp[ig].x + sx * d->vps[k], vp[ig].y + sy * d->vps[k]);
+      rp[ig] = mk<double>(rp[ig].x + sx * d->rhops[k], rp[ig].y + sy * d->rhops[k]);
+    }
+  std::vector<int> ki(d->ngw);
+  for (int ig = 0; ig < d->ngw; ++ig) {
+    if (d->n123[ig] < 1 || d->n123[ig] > gr->nnr) return fail(PWH_ERR_ARG, "N123 out of range");
+    ki[ig] = (int)d->n123[ig] - 1;
+  }
+  CHK(upload(gr->gidx, gi.data(), gi.size()));
+  CHK(upload(gr->g2, d->g2, d->ng));
+  CHK(upload(gr->vp, vp.data(), vp.size()));
+  CHK(upload(gr->rp, rp.data(), rp.size()));
+  if (d->ngw > 0) {
+    CHK(upload(gr->kidx, ki.data(), ki.size()));
+    CHK(upload(gr->xkg, d->xkg, d->ngw));
+  }
+  for (int k = 0; k < 3; ++k) {
+    const auto t = twiddles(nr[k]);
+    CHK(upload(gr->tw[k], t.data(), t.size()));
+  }
+  CHK(gr->c.ensure(gr->nnr));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  *out = gr.release();
+  return PWH_OK;
+}
+
+int pwh_grid_destroy(pwh_grid* gr) {
+  if (g.stream) hipStreamSynchronize(g.stream);
+  delete gr;
+  return PWH_OK;
+}
+
+int pwh_xc_pz(const double* rho, double* vxc, double* exc, int nnr, int* n_negative) {
+  if (!rho || !vxc || !exc || nnr < 1) return fail(PWH_ERR_ARG, "bad xc_pz arguments");
+  CHK(ensure_init());
+  ScfWork& w = scfw();
+  const size_t n = (size_t)nnr;
+  CHK(scf_up(w.v[0], rho, n));
+  CHK(w.vxc.ensure(n));
+  CHK(w.eps.ensure(n));
+  CHK(w.neg.ensure(n));
+  CHK(w.part.ensure(4 * kScfBlocks));
+  HIPCHK(xc_pz(w.v[0].p, n, w.vxc.p, w.eps.p, w.neg.p, nullptr, nullptr, w.part.p, g.stream));
+  std::vector<double> h(kScfBlocks);
+  const double cnt = sum_partials(w.part.p, kScfBlocks, h.data());
+  if (cnt > 0) HIPCHK(xc_pz_fix(w.v[0].p, n, w.neg.p, w.vxc.p, w.eps.p, nullptr, nullptr, g.stream));
+  CHK(dot_dev(w.v[0].p, w.eps.p, n, exc));
+  CHK(scf_down(vxc, w.vxc.p, n));
+  CHK(scf_sync());
+  if (n_negative) *n_negative = (int)cnt;
+  return PWH_OK;
+}
+
+int pwh_eval_energy(pwh_grid* gr, double* rhoe, pwh_energy* e) {
+  if (!gr || !rhoe || !e) return fail(PWH_ERR_ARG, "null argument");
+  ScfWork& w = scfw();
+  const size_t n = gr->nnr;
+  CHK(scf_up(w.v[0], rhoe, n));
+  CHK(w.v[1].ensure(n));
+  CHK(w.vxc.ensure(n));
+  CHK(w.eps.ensure(n));
+  CHK(w.neg.ensure(n));
+  CHK(w.part.ensure(4 * kScfBlocks));
+  // forward transform of the density, G-space sums and the electrostatic potential,
+  // inverse transform (EvalEnergy.cpp:38-117)
+  HIPCHK(real_to_cx(w.v[0].p, nullptr, n, gr->c.p, g.stream));
+  CHK(grid_fft(gr, gr->c.p, false));
+  HIPCHK(energy_g(gr->c.p, gr->gidx.p, gr->g2.p, gr->vp.p, gr->rp.p, gr->ng, gr->tpiba2, w.part.p, g.stream));
+  std::vector<double> hp(4 * kScfBlocks);
+  HIPCHK(hipMemcpyAsync(hp.data(), w.part.p, sizeof(double) * 4 * kScfBlocks, hipMemcpyDeviceToHost, g.stream));
+  CHK(grid_fft(gr, gr->c.p, true));
+  CHK(scf_sync());
+  double sums[4] = {0, 0, 0, 0};
+  for (int q = 0; q < 4; ++q)
+    for (int b = 0; b < kScfBlocks; ++b) sums[q] += hp[(size_t)q * kScfBlocks + b];
+  // xc_PZ and V_eff = Re(V_es) + V_xc (:124-135)
+  HIPCHK(xc_pz(w.v[0].p, n, w.vxc.p, w.eps.p, w.neg.p, gr->c.p, w.v[1].p, w.part.p, g.stream));
+  std::vector<double> h(kScfBlocks);
+  const double cnt = sum_partials(w.part.p, kScfBlocks, h.data());
+  if (cnt > 0) HIPCHK(xc_pz_fix(w.v[0].p, n, w.neg.p, w.vxc.p, w.eps.p, gr->c.p, w.v[1].p, g.stream));
+  double exc = 0;
+  CHK(dot_dev(w.v[0].p, w.eps.p, n, &exc));
+  CHK(scf_down(rhoe, w.v[1].p, n));
+  CHK(scf_sync());
+  const double om = gr->omega;
+  e->exc = exc * om / (double)n;
+  e->eht = sums[0] * om + e->esr - e->eself;
+  e->epseu = sums[1] * om;
+  e->eh_electron = sums[2] * om;
+  e->eh_pseudo = sums[3] * om;
+  e->etot = e->ekin + e->eht + e->epseu + e->enl + e->exc;
+  e->n_negative = (int)cnt;
+  return PWH_OK;
+}
+
+int pwh_normalize_rho(double* rho, int nnr, double omega, double nel, double* diff) {
+  if (!rho || nnr < 1 || !(omega > 0)) return fail(PWH_ERR_ARG, "bad normalize_rho arguments");
+  CHK(ensure_init());
+  ScfWork& w = scfw();
+  CHK(scf_up(w.v[0], rho, nnr));
+  CHK(normalize_dev(w.v[0].p, nnr, omega, nel, diff));
+  CHK(scf_down(rho, w.v[0].p, nnr));
+  return scf_sync();
+}
+
+int pwh_mix_simple(double beta, double* rhoe, double* rho_old, int nnr, double* d) {
+  if (!rhoe || !rho_old || !d || nnr < 1) return fail(PWH_ERR_ARG, "bad mix arguments");
+  CHK(ensure_init());
+  ScfWork& w = scfw();
+  const size_t n = nnr;
+  CHK(scf_up(w.v[0], rhoe, n));
+  CHK(scf_up(w.v[1], rho_old, n));
+  double sq = 0;
+  CHK(comb_dev({{beta, w.v[0].p}, {1.0 - beta, w.v[1].p}}, w.v[0].p, n, w.v[1].p, w.v[0].p, w.v[1].p, &sq));
+  *d = std::sqrt(sq / (double)n);
+  CHK(scf_down(rhoe, w.v[0].p, n));
+  CHK(scf_down(rho_old, w.v[1].p, n));
+  return scf_sync();
+}
+
+int pwh_mix_kerker(pwh_grid* gr, double beta, double q0, double* rhoe, double* rho_old, double* d) {
+  if (!gr || !rhoe || !rho_old || !d) return fail(PWH_ERR_ARG, "bad mix arguments");
+  ScfWork& w = scfw();
+  const size_t n = gr->nnr;
+  CHK(scf_up(w.v[0], rhoe, n));
+  CHK(scf_up(w.v[1], rho_old, n));
+  CHK(w.v[2].ensure(n));
+  CHK(comb_dev({{1.0, w.v[0].p}, {-1.0, w.v[1].p}}, w.v[2].p, n));
+  CHK(kerker_dev(gr, w.v[2].p, beta, q0));
+  double sq = 0;
+  CHK(comb_dev({{1.0, w.v[1].p}, {1.0, w.v[2].p}}, w.v[0].p, n, w.v[1].p, w.v[0].p, w.v[1].p, &sq));
+  *d = std::sqrt(sq / (double)n);
+  CHK(scf_down(rhoe, w.v[0].p, n));
+  CHK(scf_down(rho_old, w.v[1].p, n));
+  return scf_sync();
+}
+
+int pwh_kerker_prec(pwh_grid* gr, double* diff_rhoe, int idx1, double beta, double q0) {
+  if (!gr || !diff_rhoe || idx1 < 1) return fail(PWH_ERR_ARG, "bad kerker_prec arguments");
+  ScfWork& w = scfw();
+  const size_t n = gr->nnr;
+  double* col = diff_rhoe + (size_t)(idx1 - 1) * n;
+  CHK(scf_up(w.v[0], col, n));
+  CHK(kerker_dev(gr, w.v[0].p, beta, q0));
+  CHK(scf_down(col, w.v[0].p, n));
+  return scf_sync();
+}
+
+int pwh_mix_anderson(int iter, double beta, double* rhoe, double* rho_old, double* f, double* gv, int nnr,
+                     double* d) {
+  if (!rhoe || !rho_old || !f || !gv || !d || nnr < 1 || iter < 1) return fail(PWH_ERR_ARG, "bad mix arguments");
+  CHK(ensure_init());
+  ScfWork& w = scfw();
+  const size_t n = nnr;
+  // device: 0 rhoe, 1..3 rho_old columns, 4..6 f columns, 7..8 g columns
+  CHK(scf_up(w.v[0], rhoe, n));
+  for (int k = 0; k < 3; ++k) CHK(scf_up(w.v[1 + k], rho_old + k * n, n));
+  for (int k = 0; k < 3; ++k) CHK(scf_up(w.v[4 + k], f + k * n, n));
+  for (int k = 0; k < 2; ++k) CHK(scf_up(w.v[7 + k], gv + k * n, n));
+  double* R = w.v[0].p;
+  auto O = [&](int j) { return w.v[1 + j].p; };
+  auto F = [&](int j) { return w.v[4 + j].p; };
+  auto G = [&](int j) { return w.v[7 + j].p; };
+  if (iter == 1) {
+    // AndersonRhoMix.cpp:18-23
+    CHK(comb_dev({{1.0, R}}, O(0), n));
+    CHK(comb_dev({}, F(0), n));
+    *d = 1.0;
+  } else {
+    const int j0 = (iter - 1) % 3, j1 = (iter - 2) % 3, j2 = (iter - 3) % 3;
+    double sq = 0;
+    CHK(comb_dev({{1.0, R}, {-1.0, O(j1)}}, F(j0), n, nullptr, R, O(j1), &sq));
+    *d = sq;
+    if (iter == 2) {
+      CHK(comb_dev({{beta, R}, {1.0 - beta, O(j1)}}, R, n, O(j0)));
+    } else {
+      CHK(comb_dev({{1.0, F(j1)}, {-1.0, F(j0)}}, G(0), n));
+      double d11 = 0, r1 = 0;
+      CHK(dot_dev(G(0), G(0), n, &d11));
+      CHK(dot_dev(F(j0), G(0), n, &r1));
+      if (iter == 3) {
+        const double t1 = -r1 / d11;
+        CHK(comb_dev({{1.0 - t1, O(j1)}, {t1, O(j2)}, {beta, F(j0)}, {beta * t1, G(0)}}, R, n, O(j0)));
+      } else {
+        CHK(comb_dev({{1.0, F(j2)}, {-1.0, F(j0)}}, G(1), n));
+        double d22 = 0, d12 = 0, r2 = 0;
+        CHK(dot_dev(G(1), G(1), n, &d22));
+        CHK(dot_dev(G(0), G(1), n, &d12));
+        CHK(dot_dev(F(j0), G(1), n, &r2));
+        double t1 = d11 * d22, t2 = t1 - d12 * d12;
+        if (std::fabs(t2) > std::fabs(t1) * 1.0e-8) {
+          t1 = (-r1 * d22 + r2 * d12) / t2;
+          t2 = (r1 * d12 - r2 * d11) / t2;
+        } else {
+          t1 = 0.0;
+          t2 = 0.0;
+        }
+        // the reference adds t2 g(:,1), not t2 g(:,2) (AndersonRhoMix.cpp:76-78)
+        CHK(comb_dev({{1.0 - t1 - t2, O(j1)}, {t1, O(j2)}, {t2, O(j0)}, {beta, F(j0)}, {beta * t1, G(0)},
+                      {beta * t2, G(0)}},
+                     R, n, O(j0)));
+      }
+    }
+  }
+  CHK(scf_down(rhoe, R, n));
+  for (int k = 0; k < 3; ++k) CHK(scf_down(rho_old + k * n, O(k), n));
+  for (int k = 0; k < 3; ++k) CHK(scf_down(f + k * n, F(k), n));
+  for (int k = 0; k < 2; ++k) CHK(scf_down(gv + k * n, G(k), n));
+  return scf_sync();
+}
+
+int pwh_mix_pulay(pwh_grid* gr, int iter, double* rhoe, double* rho_old, double* f, int mixdim, double q0,
+                  double nel, double* d, double* nel_diff) {
+  if (!gr || !rhoe || !rho_old || !f || !d || iter < 1) return fail(PWH_ERR_ARG, "bad mix arguments");
+  if (mixdim < 2 || mixdim > 7) return fail(PWH_ERR_ARG, "PulayRhoMix: mixdim must be in 2..7");
+  ScfWork& w = scfw();
+  const size_t n = gr->nnr;
+  const double beta = 0.25;
+  // device: 0 rhoe, 1..mixdim rho_old columns, 8..7+mixdim f columns
+  CHK(scf_up(w.v[0], rhoe, n));
+  for (int k = 0; k < mixdim; ++k) CHK(scf_up(w.v[1 + k], rho_old + k * n, n));
+  for (int k = 0; k < mixdim; ++k) CHK(scf_up(w.v[8 + k], f + k * n, n));
+  double* R = w.v[0].p;
+  auto O = [&](int j) { return w.v[1 + j].p; };
+  auto F = [&](int j) { return w.v[8 + j].p; };
+  double sq = 0;
+  if (iter < mixdim) {
+    // PulayRhoMix.cpp:22-35
+    CHK(comb_dev({{beta, R}, {1.0 - beta, O(iter - 1)}}, R, n));
+    CHK(comb_dev({{1.0, R}, {-1.0, O(iter - 1)}}, F(iter), n));
+    CHK(kerker_dev(gr, F(iter), 1.0, q0));
+    CHK(comb_dev({{1.0, R}}, O(iter), n));
+    CHK(dot_dev(F(iter), F(iter), n, &sq));
+    *d = std::sqrt(sq / (double)n);
+  } else {
+    const int jc = iter % mixdim, jn = (iter + 1) % mixdim;
+    const int m = std::min(iter + 1, mixdim) + 1;
+    CHK(comb_dev({{1.0, R}, {-1.0, O(jc)}}, F(jc), n));
+    CHK(kerker_dev(gr, F(jc), 1.0, q0));
+    CHK(dot_dev(F(jc), F(jc), n, &sq));
+    *d = std::sqrt(sq / (double)n);
+    // the bordered Gram system (:54-71), symmetric from its upper triangle
+    std::vector<double> A((size_t)m * m, 0.0), b(m, 0.0);
+    for (int i = 0; i < m - 1; ++i) {
+      for (int j = i; j < m - 1; ++j) {
+        double v = 0;
+        CHK(dot_dev(F(i), F(j), n, &v));
+        A[i + (size_t)j * m] = v;
+        A[j + (size_t)i * m] = v;
+      }
+      A[i + (size_t)(m - 1) * m] = 1.0;
+      A[(m - 1) + (size_t)i * m] = 1.0;
+    }
+    b[m - 1] = 1.0;
+    CHK(lu_solve_dev(A, b, m));
+    // rhoe = sum_j alpha_j (rho_old_j + f_j), three histories per pass
+    CHK(comb_dev({}, R, n));
+    for (int j0 = 0; j0 < m - 1; j0 += 2) {
+      if (j0 + 1 < m - 1)
+        CHK(comb_dev({{1.0, R}, {b[j0], O(j0)}, {b[j0], F(j0)}, {b[j0 + 1], O(j0 + 1)}, {b[j0 + 1], F(j0 + 1)}},
+                     R, n));
+      else
+        CHK(comb_dev({{1.0, R}, {b[j0], O(j0)}, {b[j0], F(j0)}}, R, n));
+    }
+    CHK(normalize_dev(R, n, gr->omega, nel, nel_diff));
+    CHK(comb_dev({{1.0, R}}, O(jn), n));
+  }
+  CHK(scf_down(rhoe, R, n));
+  for (int k = 0; k < mixdim; ++k) CHK(scf_down(rho_old + k * n, O(k), n));
+  for (int k = 0; k < mixdim; ++k) CHK(scf_down(f + k * n, F(k), n));
+  return scf_sync();
+}
+
+int pwh_mix_broyden(int iter, double alpha, double w0, double* rho, double* rho_old, double* f, double* df,
+                    double* u, double* a, int mixdim, int nnr, double* d) {
+  if (!rho || !rho_old || !f || !df || !u || !a || !d || nnr < 1 || iter < 1)
+    return fail(PWH_ERR_ARG, "bad mix arguments");
+  if (mixdim < 2 || mixdim > 8) return fail(PWH_ERR_ARG, "BroydenRhoMix: mixdim must be in 2..8");
+  CHK(ensure_init());
+  ScfWork& w = scfw();
+  const size_t n = nnr;
+  // device: 0 rho, 1..2 rho_old, 3..4 f, 5..4+mixdim df, 11..10+mixdim u
+  CHK(scf_up(w.v[0], rho, n));
+  for (int k = 0; k < 2; ++k) CHK(scf_up(w.v[1 + k], rho_old + k * n, n));
+  for (int k = 0; k < 2; ++k) CHK(scf_up(w.v[3 + k], f + k * n, n));
+  for (int k = 0; k < mixdim; ++k) CHK(scf_up(w.v[5 + k], df + k * n, n));
+  for (int k = 0; k < mixdim; ++k) CHK(scf_up(w.v[11 + k], u + k * n, n));
+  double* R = w.v[0].p;
+  auto O = [&](int j) { return w.v[1 + j].p; };
+  auto F = [&](int j) { return w.v[3 + j].p; };
+  auto DF = [&](int j) { return w.v[5 + j].p; };
+  auto U = [&](int j) { return w.v[11 + j].p; };
+  // BroydenRhoMix.cpp:33-103
+  const int m = std::min(iter + 1, mixdim);
+  const int jc = iter % m, kp = (iter - 1) % 2, kc = iter % 2;
+  double sq = 0;
+  CHK(comb_dev({{1.0, R}, {-1.0, O(kp)}}, F(kc), n, nullptr, R, O(kp), &sq));
+  *d = std::sqrt(sq / (double)n);
+  CHK(comb_dev({{1.0, F(kc)}, {-1.0, F(kp)}}, DF(jc), n));
+  double t1 = 0;
+  CHK(dot_dev(DF(jc), DF(jc), n, &t1));
+  t1 = std::sqrt(t1);
+  if (t1 > 1.e-8) t1 = 1.0 / t1;
+  CHK(comb_dev({{t1, DF(jc)}}, DF(jc), n));
+  CHK(comb_dev({{alpha, DF(jc)}, {t1, O(kp)}, {-t1, O(kc)}}, U(jc), n));
+  std::vector<double> c(m);
+  for (int k = 0; k < m; ++k) CHK(dot_dev(DF(k), F(kc), n, &c[k]));
+  for (int k = 0; k < m; ++k) {
+    double v = 0;
+    CHK(dot_dev(DF(jc), DF(k), n, &v));
+    a[k + (size_t)jc * mixdim] = v;
+    a[jc + (size_t)k * mixdim] = v;
+  }
+  std::vector<double> beta(a, a + (size_t)mixdim * mixdim);
+  for (int k = 0; k < m; ++k) beta[k + (size_t)k * mixdim] += w0 * w0;
+  CHK(lu_inverse_dev(beta, m, mixdim));
+  std::vector<double> gamma(m, 0.0);
+  for (int l = 0; l < m; ++l)
+    for (int k = 0; k < m; ++k) gamma[l] = gamma[l] + c[k] * beta[k + (size_t)l * mixdim];
+  CHK(comb_dev({{1.0, O(kp)}, {alpha, F(kc)}}, R, n));
+  for (int l = 0; l < m; ++l) CHK(comb_dev({{1.0, R}, {-gamma[l], U(l)}}, R, n));
+  CHK(comb_dev({{1.0, R}}, O(kc), n));
+  CHK(scf_down(rho, R, n));
+  for (int k = 0; k < 2; ++k) CHK(scf_down(rho_old + k * n, O(k), n));
+  for (int k = 0; k < 2; ++k) CHK(scf_down(f + k * n, F(k), n));
+  for (int k = 0; k < mixdim; ++k) CHK(scf_down(df + k * n, DF(k), n));
+  for (int k = 0; k < mixdim; ++k) CHK(scf_down(u + k * n, U(k), n));
+  return scf_sync();
+}
+
+int pwh_eval_enl(pwh_system* s, const double* c0, int ld, int nstates, const double* focc, double wkpt,
+                 double* fnl, double* enl) {
+  CHK(check_ld(s, nstates));
+  if (!c0 || !focc || !enl || ld < s->ngw) return fail(PWH_ERR_ARG, "bad eval_enl arguments");
+  const int nb = nstates, ldp = s->ngw_pad;
+  CHK(s->stage.ensure((size_t)s->ngw * nb));
+  HIPCHK(hipMemcpy2DAsync(s->stage.p, sizeof(cx<double>) * s->ngw, c0, sizeof(cx<double>) * ld,
+                          sizeof(cx<double>) * s->ngw, nb, hipMemcpyHostToDevice, g.stream));
+  CHK(s->X.ensure((size_t)nb * ldp));
+  HIPCHK(permute_in_f64(s->stage.p, s->ngw, s->d_perm.p, s->ngw, ldp, s->X.p, nb, g.stream));
+  CHK(upload(s->rho_w, focc, nb));
+  const int nproj = s->poff.empty() ? 0 : s->poff.back() + s->na.back() * s->lm.back();
+  std::vector<double> rows(std::max(nproj, 1), 0.0);
+  CHK(s->Q64.ensure((size_t)std::max(nproj, 1) * nb));
+  CHK(s->rho_k.ensure(std::max(nproj, 1)));
+  // FNL = beta^H C0 per species (EvalENL.cpp:28-34), then sum_ist FOCC |FNL|^2 per row
+  for (int sp = 0; sp < s->nsp; ++sp) {
+    if (s->na[sp] == 0 || s->lm[sp] == 0) continue;
+    const int M = s->na[sp] * s->lm[sp];
+    const size_t pe = proj_part_elems(M, nb, s->ngw);
+    CHK(s->part64.ensure(pe));
+    BetaGen bg{s->Bt64.p + (size_t)s->poff[sp] * ldp, M, s->lm[sp], ldp};
+    HIPCHK(gemm_proj<double>(bg, s->X.p, ldp, nb, s->ngw, s->Q64.p + (size_t)s->poff[sp] * nb, s->part64.p, pe,
+                             PREC_FP64, g.stream));
+  }
+  if (nproj > 0) {
+    HIPCHK(enl_rows(s->Q64.p, nb, nproj, nb, s->rho_w.p, s->rho_k.p, g.stream));
+    HIPCHK(hipMemcpyAsync(rows.data(), s->rho_k.p, sizeof(double) * nproj, hipMemcpyDeviceToHost, g.stream));
+    if (fnl)
+      HIPCHK(hipMemcpyAsync(fnl, s->Q64.p, sizeof(cx<double>) * nproj * nb, hipMemcpyDeviceToHost, g.stream));
+  }
+  HIPCHK(hipStreamSynchronize(g.stream));
+  // ENL = sum over (species, atom, lm) in the reference's loop order of WKPT enl_m WNL
+  double e = 0.0;
+  for (int sp = 0; sp < s->nsp; ++sp) {
+    if (s->na[sp] == 0 || s->lm[sp] == 0) continue;
+    for (int a = 0; a < s->na[sp]; ++a)
+      for (int l = 0; l < s->lm[sp]; ++l)
+        e = e + wkpt * rows[s->poff[sp] + a * s->lm[sp] + l] * s->wl[4 * sp + l];
+  }
+  *enl = e;
   return PWH_OK;
 }
 

@@ -41,7 +41,9 @@ EXPORTS = [
     "pwh_resident_apply", "pwh_resident_chebyfilt", "pwh_synchronize", "pwh_timing_enable",
     "pwh_timing_reset", "pwh_timing_get", "pwh_batch_create", "pwh_batch_destroy",
     "pwh_batch_set", "pwh_batch_get", "pwh_batch_chebyfilt", "pwh_fft3d", "pwh_eig_zheevd",
-    "pwh_eval_rho_psi", "pwh_davidson", "pwh_lobpcg",
+    "pwh_eval_rho_psi", "pwh_davidson", "pwh_lobpcg", "pwh_grid_create", "pwh_grid_destroy",
+    "pwh_eval_energy", "pwh_xc_pz", "pwh_normalize_rho", "pwh_mix_simple", "pwh_mix_kerker",
+    "pwh_kerker_prec", "pwh_mix_anderson", "pwh_mix_pulay", "pwh_mix_broyden", "pwh_eval_enl",
 ]
 
 
@@ -72,6 +74,20 @@ class DiagOpts(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("nempty", C.c_int), ("report", DIAG_REPORT),
                 ("report_ctx", C.c_void_p), ("iterations", C.c_int), ("nconv", C.c_int),
                 ("reason", C.c_int), ("t_total_s", C.c_double), ("band_applies", C.c_longlong)]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("nr1", C.c_int), ("nr2", C.c_int), ("nr3", C.c_int), ("ng", C.c_int),
+                ("n1", C.c_void_p), ("n2", C.c_void_p), ("n3", C.c_void_p), ("g2", C.c_void_p),
+                ("nsp", C.c_int), ("sfac", C.c_void_p), ("vps", C.c_void_p), ("rhops", C.c_void_p),
+                ("tpiba2", C.c_double), ("omega", C.c_double), ("ngw", C.c_int), ("n123", C.c_void_p),
+                ("xkg", C.c_void_p)]
+
+
+class Energy(C.Structure):
+    _fields_ = [("ekin", C.c_double), ("enl", C.c_double), ("esr", C.c_double), ("eself", C.c_double),
+                ("etot", C.c_double), ("eht", C.c_double), ("epseu", C.c_double), ("exc", C.c_double),
+                ("eh_electron", C.c_double), ("eh_pseudo", C.c_double), ("n_negative", C.c_int)]
 
 
 class ChfsiOpts(C.Structure):
@@ -136,6 +152,20 @@ def lib():
         "pwh_eig_zheevd": [vp, C.c_int, _dp, C.c_int],
         "pwh_eval_rho_psi": [vp, vp, C.c_int, C.c_int, _dp, C.c_double, _dp, C.POINTER(C.c_double), _dp],
         "pwh_davidson": [vp, vp, C.c_int, _dp, C.POINTER(DiagOpts), _dp, _dp],
+        "pwh_grid_create": [C.POINTER(GridDesc), C.POINTER(vp)],
+        "pwh_grid_destroy": [vp],
+        "pwh_eval_energy": [vp, _dp, C.POINTER(Energy)],
+        "pwh_xc_pz": [_dp, _dp, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int)],
+        "pwh_normalize_rho": [_dp, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_double)],
+        "pwh_mix_simple": [C.c_double, _dp, _dp, C.c_int, C.POINTER(C.c_double)],
+        "pwh_mix_kerker": [vp, C.c_double, C.c_double, _dp, _dp, C.POINTER(C.c_double)],
+        "pwh_kerker_prec": [vp, _dp, C.c_int, C.c_double, C.c_double],
+        "pwh_mix_anderson": [C.c_int, C.c_double, _dp, _dp, _dp, _dp, C.c_int, C.POINTER(C.c_double)],
+        "pwh_mix_pulay": [vp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_double),
+                          C.POINTER(C.c_double)],
+        "pwh_mix_broyden": [C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int,
+                            C.POINTER(C.c_double)],
+        "pwh_eval_enl": [vp, vp, C.c_int, C.c_int, _dp, C.c_double, vp, C.POINTER(C.c_double)],
         "pwh_lobpcg": [vp, vp, C.c_int, _dp, C.POINTER(DiagOpts), _dp, _dp],
     }
     for name, args in proto.items():
@@ -328,6 +358,9 @@ class System:
         self.h = h
         self.ngw = int(g.ngw)
         self.nnr = int(g.nr[0] * g.nr[1] * g.nr[2])
+        # projectors: NA(is) x lm_end(is) with lm_end = L_MAX^2 - 2 L_LOC + 1 (EvalENL.cpp:25)
+        lm_end = np.maximum(lmx.astype(np.int64) ** 2 - 2 * llc.astype(np.int64) + 1, 0)
+        self.nproj = int(np.sum(na.astype(np.int64) * lm_end))
         self._keep = None
 
     def close(self):
@@ -400,6 +433,17 @@ class System:
     def lobpcg(self, X0, prec, max_iter=None, nempty=0):
         """lobpcg_cpu.cpp:32-389 from start vectors X0 (nstates, ngw)."""
         return self._diag(lib().pwh_lobpcg, X0, prec, max_iter, nempty)
+
+    def eval_enl(self, c0, focc, wkpt, ld=None):
+        """EvalENL.cpp:5-49: returns (ENL, fnl [nproj][nstates] in species, atom, lm row order)."""
+        c0 = np.ascontiguousarray(c0, dtype=np.complex128)
+        nst, ldc = c0.shape
+        enl = C.c_double()
+        fnl = np.zeros((max(self.nproj, 1), nst), np.complex128)
+        _check(lib().pwh_eval_enl(self.h, _zptr(c0), ldc if ld is None else ld, nst,
+                                  np.ascontiguousarray(focc, dtype=np.float64), float(wkpt),
+                                  _zptr(fnl), C.byref(enl)))
+        return enl.value, fnl[:self.nproj]
 
     def lanczos(self, v0, k):
         """lanczos.cpp:8-101.  Returns (T[k,k] as the reference fills it, f, v0 normalised)."""
@@ -495,3 +539,97 @@ class Batch:
         lb = np.ascontiguousarray(lb, np.float64)
         ub = np.ascontiguousarray(ub, np.float64)
         _check(lib().pwh_batch_chebyfilt(self.h, int(degree), lb, ub, _prec(precision), int(mode)))
+
+
+# --------------------------------------------------------------------------
+# SCF steps around the eigensolver (EvalEnergy, xc_PZ, mixers, NormalizeRho)
+# --------------------------------------------------------------------------
+
+class Grid:
+    """The density G-vectors, local form factors and Kerker sphere (pwh_grid_create)."""
+
+    def __init__(self, nr, n1, n2, n3, g2, sfac, vps, rhops, tpiba2, omega, n123=None, xkg=None):
+        init()
+        self._keep = dict(n1=np.ascontiguousarray(n1, np.int32), n2=np.ascontiguousarray(n2, np.int32),
+                          n3=np.ascontiguousarray(n3, np.int32), g2=np.ascontiguousarray(g2, np.float64),
+                          sfac=np.ascontiguousarray(np.asarray(sfac, np.complex128).T),    # species fastest
+                          vps=np.ascontiguousarray(np.asarray(vps, np.float64).T),
+                          rhops=np.ascontiguousarray(np.asarray(rhops, np.float64).T),
+                          n123=np.ascontiguousarray(n123 if n123 is not None else [], np.uint32),
+                          xkg=np.ascontiguousarray(xkg if xkg is not None else [], np.float64))
+        k = self._keep
+        nsp = np.asarray(vps).shape[0] if np.ndim(vps) == 2 else 1
+        d = GridDesc(nr1=int(nr[0]), nr2=int(nr[1]), nr3=int(nr[2]), ng=len(k["n1"]),
+                     n1=k["n1"].ctypes.data, n2=k["n2"].ctypes.data, n3=k["n3"].ctypes.data,
+                     g2=k["g2"].ctypes.data, nsp=nsp, sfac=k["sfac"].ctypes.data, vps=k["vps"].ctypes.data,
+                     rhops=k["rhops"].ctypes.data, tpiba2=float(tpiba2), omega=float(omega),
+                     ngw=len(k["n123"]), n123=k["n123"].ctypes.data if len(k["n123"]) else None,
+                     xkg=k["xkg"].ctypes.data if len(k["xkg"]) else None)
+        h = C.c_void_p()
+        _check(lib().pwh_grid_create(C.byref(d), C.byref(h)))
+        self.h = h
+        self.nnr = int(nr[0]) * int(nr[1]) * int(nr[2])
+
+    def close(self):
+        if self.h:
+            lib().pwh_grid_destroy(self.h)
+            self.h = None
+
+    def eval_energy(self, rhoe, ekin=0.0, enl=0.0, esr=0.0, eself=0.0):
+        """EvalEnergy.cpp:5-163: returns (V_eff, dict of the energy terms)."""
+        r = np.array(rhoe, np.float64)
+        e = Energy(ekin=ekin, enl=enl, esr=esr, eself=eself)
+        _check(lib().pwh_eval_energy(self.h, r, C.byref(e)))
+        return r, {f: getattr(e, f) for f, _ in Energy._fields_}
+
+    def mix_kerker(self, beta, q0, rhoe, rho_old):
+        d = C.c_double()
+        _check(lib().pwh_mix_kerker(self.h, beta, q0, rhoe, rho_old, C.byref(d)))
+        return d.value
+
+    def kerker_prec(self, diff, idx1, beta, q0):
+        _check(lib().pwh_kerker_prec(self.h, diff.reshape(-1), idx1, beta, q0))
+
+    def mix_pulay(self, it, rhoe, rho_old, f, mixdim, q0, nel):
+        d = C.c_double()
+        nd = C.c_double()
+        _check(lib().pwh_mix_pulay(self.h, it, rhoe, rho_old.reshape(-1), f.reshape(-1), mixdim, q0, nel,
+                                   C.byref(d), C.byref(nd)))
+        return d.value
+
+
+def xc_pz(rho):
+    """xc_PZ.cpp:7-63: returns (vxc, Exc, number of points below -0.1)."""
+    r = np.ascontiguousarray(rho, np.float64)
+    v = np.zeros_like(r)
+    exc = C.c_double()
+    nneg = C.c_int()
+    _check(lib().pwh_xc_pz(r, v, C.byref(exc), r.size, C.byref(nneg)))
+    return v, exc.value, nneg.value
+
+
+def normalize_rho(rho, omega, nel):
+    """NormalizeRho.cpp:7-33 in place; returns the electron-count difference."""
+    dd = C.c_double()
+    _check(lib().pwh_normalize_rho(rho, rho.size, omega, nel, C.byref(dd)))
+    return dd.value
+
+
+def mix_simple(beta, rhoe, rho_old):
+    d = C.c_double()
+    _check(lib().pwh_mix_simple(beta, rhoe, rho_old, rhoe.size, C.byref(d)))
+    return d.value
+
+
+def mix_anderson(it, beta, rhoe, rho_old, f, g):
+    d = C.c_double()
+    _check(lib().pwh_mix_anderson(it, beta, rhoe, rho_old.reshape(-1), f.reshape(-1), g.reshape(-1), rhoe.size,
+                                  C.byref(d)))
+    return d.value
+
+
+def mix_broyden(it, alpha, w0, rho, rho_old, f, df, u, a, mixdim):
+    d = C.c_double()
+    _check(lib().pwh_mix_broyden(it, alpha, w0, rho, rho_old.reshape(-1), f.reshape(-1), df.reshape(-1),
+                                 u.reshape(-1), a.reshape(-1), mixdim, rho.size, C.byref(d)))
+    return d.value

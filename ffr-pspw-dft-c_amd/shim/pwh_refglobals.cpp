@@ -22,3 +22,9 @@ bool USER_MAX_DIAG_ITER = false;
 int MAX_DIAG_ITER = 0;
 bool DIAG_VERBOSE = false;
 int NEMPTY = 0;
+int NG = 0;
+int *N1 = 0, *N2 = 0, *N3 = 0;
+double *G = 0, *VPS = 0, *RHOPS = 0;
+zcomplex *SFAC = 0, *FNL = 0;
+double ESR = 0.0, ESELF = 0.0, ETOT = 0.0, EHT = 0.0, EPSEU = 0.0, ENL = 0.0, EXC = 0.0;
+double NEL = 0.0, KERKER_Q0 = 0.0;

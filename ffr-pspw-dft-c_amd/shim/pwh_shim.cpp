@@ -12,6 +12,12 @@
 //   EvalRhoPsi      EvalRhoPsi.cpp:5-157       -> pwh_eval_rho_psi (both overloads)
 //   davidson_cpu    davidson_cpu.cpp:9-247     -> pwh_davidson
 //   lobpcg_cpu      lobpcg_cpu.cpp:32-389      -> pwh_lobpcg
+//   EvalEnergy      EvalEnergy.cpp:5-163       -> pwh_eval_energy
+//   xc_PZ           xc_PZ.cpp:7-63             -> pwh_xc_pz
+//   EvalENL         EvalENL.cpp:5-49           -> pwh_eval_enl per k-point
+//   NormalizeRho    NormalizeRho.cpp:7-33      -> pwh_normalize_rho
+//   SimpleRhoMix, KerkerRhoMix, KerkerPrec, AndersonRhoMix, PulayRhoMix, BroydenRhoMix
+//                   *RhoMix.cpp                -> pwh_mix_*
 //
 // Link it in place of those .cpp files.  It reads the reference's globals
 // (global_variables.cpp) and keeps one device system per k-point, rebuilt when the
@@ -46,6 +52,10 @@ extern zcomplex *C0;
 // read by davidson_cpu / lobpcg_cpu (common_pspw.h:47-49,67)
 extern bool USER_MAX_DIAG_ITER, DIAG_VERBOSE;
 extern int MAX_DIAG_ITER, NEMPTY;
+// read and written by EvalEnergy, EvalENL and the mixers (common_pspw.h:70-136)
+extern int NG, *N1, *N2, *N3;
+extern double *G, *VPS, *RHOPS, ESR, ESELF, ETOT, EHT, EPSEU, ENL, EXC, NEL, KERKER_Q0;
+extern zcomplex *SFAC, *FNL;
 
 namespace {
 
@@ -58,7 +68,9 @@ struct ShimState {
   pwh_system *sys;
   Key key;
 };
-ShimState g_shim = {0, Key()};
+// one device system per k-point (slot (ik - 1) mod kSlots)
+const int kSlots = 64;
+ShimState g_slots[kSlots];
 
 void die(const char *where) {
   printf("%s: %s\n", where, pwh_last_error());
@@ -80,6 +92,7 @@ pwh_system *system_for(int ik) {
   k.ik = ik; k.nr1 = NR1; k.nr2 = NR2; k.nr3 = NR3; k.ngw = NGW[ik - 1]; k.ngwx = NGWX;
   k.nsp = NSP; k.nax = NAX; k.nlmax = NLMAX;
   k.n123 = N123; k.xkg = XKG; k.eigr = EIGR; k.pkg_a = PKG_A; k.wnl = WNL; k.na = NA;
+  ShimState &g_shim = g_slots[(ik - 1) % kSlots];
   if (g_shim.sys && memcmp(&k, &g_shim.key, sizeof(k)) == 0) return g_shim.sys;
   if (g_shim.sys) pwh_system_destroy(g_shim.sys);
   g_shim.sys = 0;
@@ -157,6 +170,67 @@ void diag_report(void *ctx, int event, int a, int b, double value) {
       printf("ERROR calling zheevd in eig_zhegv : info = %d\n", a);
       break;
   }
+}
+
+// The density G-vectors and form factors of EvalEnergy and k-point 1's sphere of the
+// Kerker filter, rebuilt when the global arrays or sizes they come from change
+struct GridKey {
+  int nr1, nr2, nr3, ng, nsp, ngw;
+  double tpiba2, omega;
+  const void *n1, *n2, *n3, *g, *sfac, *vps, *rhops, *n123, *xkg;
+};
+struct GridState {
+  pwh_grid *grid;
+  GridKey key;
+};
+GridState g_grid = {0, GridKey()};
+
+pwh_grid *grid_for() {
+  GridKey k;
+  memset(&k, 0, sizeof(k));
+  k.nr1 = NR1; k.nr2 = NR2; k.nr3 = NR3; k.ng = NG; k.nsp = NSP; k.ngw = NGW ? NGW[0] : 0;
+  k.tpiba2 = TPIBA2; k.omega = OMEGA;
+  k.n1 = N1; k.n2 = N2; k.n3 = N3; k.g = G; k.sfac = SFAC; k.vps = VPS; k.rhops = RHOPS;
+  k.n123 = N123; k.xkg = XKG;
+  if (g_grid.grid && memcmp(&k, &g_grid.key, sizeof(k)) == 0) return g_grid.grid;
+  if (g_grid.grid) pwh_grid_destroy(g_grid.grid);
+  g_grid.grid = 0;
+  pwh_grid_desc d;
+  memset(&d, 0, sizeof(d));
+  d.nr1 = NR1; d.nr2 = NR2; d.nr3 = NR3;
+  d.ng = NG; d.n1 = N1; d.n2 = N2; d.n3 = N3; d.g2 = G;
+  d.nsp = NSP; d.sfac = reinterpret_cast<const double *>(SFAC); d.vps = VPS; d.rhops = RHOPS;
+  d.tpiba2 = TPIBA2; d.omega = OMEGA;
+  d.ngw = NGW ? NGW[0] : 0; d.n123 = N123; d.xkg = XKG;
+  if (pwh_grid_create(&d, &g_grid.grid) != PWH_OK) die("pwh_grid_create");
+  g_grid.key = k;
+  return g_grid.grid;
+}
+
+// Kerker-only grids (the mixers) need no form factors: a grid without them is made
+// from a single G = 0 entry when EvalEnergy's arrays are not set yet
+pwh_grid *mixer_grid() {
+  if (N1 && N2 && N3 && G && SFAC && VPS && RHOPS && NG > 0) return grid_for();
+  static pwh_grid *bare = 0;
+  static GridKey bkey;
+  GridKey k;
+  memset(&k, 0, sizeof(k));
+  k.nr1 = NR1; k.nr2 = NR2; k.nr3 = NR3; k.ngw = NGW ? NGW[0] : 0; k.n123 = N123; k.xkg = XKG;
+  k.tpiba2 = TPIBA2; k.omega = OMEGA;
+  if (bare && memcmp(&k, &bkey, sizeof(k)) == 0) return bare;
+  if (bare) pwh_grid_destroy(bare);
+  static int one[1] = {1};
+  static double zero[2] = {0.0, 0.0};
+  pwh_grid_desc d;
+  memset(&d, 0, sizeof(d));
+  d.nr1 = NR1; d.nr2 = NR2; d.nr3 = NR3;
+  d.ng = 1; d.n1 = one; d.n2 = one; d.n3 = one; d.g2 = zero;
+  d.nsp = 1; d.sfac = zero; d.vps = zero; d.rhops = zero;
+  d.tpiba2 = TPIBA2 > 0 ? TPIBA2 : 1.0; d.omega = OMEGA > 0 ? OMEGA : 1.0;
+  d.ngw = NGW ? NGW[0] : 0; d.n123 = N123; d.xkg = XKG;
+  if (pwh_grid_create(&d, &bare) != PWH_OK) die("pwh_grid_create");
+  bkey = k;
+  return bare;
 }
 
 pwh_system *diag_system(int ik, int nbasis, double *rhoe, int nnr) {
@@ -339,4 +413,153 @@ void lobpcg_cpu(int ik, double *lambda, zcomplex *X, int nbasis, int nstates, do
   printf("Final LOBPCG result\n");
   for (int i = 0; i < nstates; ++i) printf("%d %18.10f %18.10f\n", i + 1, lambda[i], res[i]);
   free(res);
+}
+
+// EvalEnergy.cpp:5-163: rhoe (the density) becomes V_eff; EXC, EHT, EPSEU, ETOT are set
+// from EKIN, ENL, ESR, ESELF; the reference's lines are printed.
+void EvalEnergy(double *rhoe) {
+  printf("Evaluating Kohn-Sham energy terms:\n");
+  pwh_grid *gr = grid_for();
+  // xc_PZ warns for each point below -0.1 in grid order (xc_PZ.cpp:26-29)
+  for (int i = 0; i < NNR; ++i)
+    if (rhoe[i] < -0.1) printf("***WARNING: Negative electron density: %18.10f\n", rhoe[i]);
+  pwh_energy e;
+  memset(&e, 0, sizeof(e));
+  e.ekin = EKIN; e.enl = ENL; e.esr = ESR; e.eself = ESELF;
+  if (pwh_eval_energy(gr, rhoe, &e) != PWH_OK) die("EvalEnergy");
+  EXC = e.exc;
+  EHT = e.eht;
+  EPSEU = e.epseu;
+  ETOT = e.etot;
+  printf("\n");
+  printf("ETOT   = %18.10f\n", ETOT);
+  printf("EKIN   = %18.10f\n", EKIN);
+  printf("EHT    = %18.10f\n", EHT);
+  printf("EPSEU  = %18.10f\n", EPSEU);
+  printf("ENL    = %18.10f\n", ENL);
+  printf("EXC    = %18.10f\n", EXC);
+}
+
+// xc_PZ.cpp:7-63
+void xc_PZ(double *rho, double *vxc, double &Exc, int nnr) {
+  for (int i = 0; i < nnr; ++i)
+    if (rho[i] < -0.1) printf("***WARNING: Negative electron density: %18.10f\n", rho[i]);
+  if (pwh_xc_pz(rho, vxc, &Exc, nnr, 0) != PWH_OK) die("xc_PZ");
+}
+
+// NormalizeRho.cpp:7-33
+void NormalizeRho(double *rho, int nnr, double omega, double nel) {
+  double diff = 0.0;
+  if (pwh_normalize_rho(rho, nnr, omega, nel, &diff) != PWH_OK) {
+    double sum = 0.0;
+    for (int i = 0; i < nnr; ++i) sum = sum + (rho[i] < 0.0 ? 0.0 : rho[i]);
+    printf("\nERROR in NormalizeRho: sum(rho)=%f < %f\n", sum, 1.0e-15);
+    fflush(stdout);
+    abort();
+  }
+  if (fabs(diff) > 1.0e-5) {
+    printf("\nWARNING in NormalizeRho: difference between correct and actual\n");
+    printf("number of electrons: %f\n", diff);
+  }
+}
+
+// SimpleRhoMix.cpp:5-20
+void SimpleRhoMix(double beta, double *rhoe, double *rho_old, int nnr, double &d) {
+  printf("Simple charge density mixing with mixing parameter %15.10f\n", beta);
+  if (pwh_mix_simple(beta, rhoe, rho_old, nnr, &d) != PWH_OK) die("SimpleRhoMix");
+  printf("In SimpleRhoMix: d = %f\n", d);
+}
+
+// KerkerRhoMix.cpp:5-55
+void KerkerRhoMix(double beta, double q0, double *rhoe, double *rho_old, int nnr, double &d) {
+  printf("Kerker rho mixing with beta = %f\n", beta);
+  check_nnr(nnr);
+  if (pwh_mix_kerker(mixer_grid(), beta, q0, rhoe, rho_old, &d) != PWH_OK) die("KerkerRhoMix");
+  printf("In KerkerRhoMix: d = %f\n", d);
+}
+
+// KerkerRhoMix.cpp:57-82
+void KerkerPrec(double *diff_rhoe, int idx1, int nnr, double beta, double q0) {
+  check_nnr(nnr);
+  if (pwh_kerker_prec(mixer_grid(), diff_rhoe, idx1, beta, q0) != PWH_OK) die("KerkerPrec");
+}
+
+// AndersonRhoMix.cpp:9-85
+void AndersonRhoMix(int iter, double beta, double *rhoe, double *rho_old, double *f, double *g, int NNR_,
+                    double &d) {
+  printf("Anderson mixing with beta = %f\n", beta);
+  if (pwh_mix_anderson(iter, beta, rhoe, rho_old, f, g, NNR_, &d) != PWH_OK) die("AndersonRhoMix");
+  if (iter >= 3) printf("In AndersonRhoMix: d = %18.10f\n", d);
+}
+
+// PulayRhoMix.cpp:5-91
+void PulayRhoMix(int iter, double *rhoe, double *rho_old, double *f, int mixdim, int nnr, double &d) {
+  printf("Pulay mixing with MIXDIM = %d\n", mixdim);
+  if (mixdim < 2) {
+    printf("ERROR in PulayRhoMix: mixdim < 2 : %d\n", mixdim);
+    exit(1);
+  }
+  check_nnr(nnr);
+  double diff = 0.0;
+  const int rc = pwh_mix_pulay(mixer_grid(), iter, rhoe, rho_old, f, mixdim, KERKER_Q0, NEL, &d, &diff);
+  if (rc == PWH_ERR_LAPACK) {
+    printf("ERROR calling dsysv in PulayRhoMix: info = %d\n", 1);
+    exit(1);
+  }
+  if (rc != PWH_OK) die("PulayRhoMix");
+  if (iter >= mixdim && fabs(diff) > 1.0e-5) {
+    printf("\nWARNING in NormalizeRho: difference between correct and actual\n");
+    printf("number of electrons: %f\n", diff);
+  }
+  printf("In PulayRhoMix: d = %f\n", d);
+}
+
+// BroydenRhoMix.cpp:10-107
+void BroydenRhoMix(int iter, double alpha, double w0, double *rho, double *rho_old, double *f, double *df,
+                   double *u, double *a, int mixdim, int nnr, double &d) {
+  printf("Broyden mixing with MIXDIM=%5d\n", mixdim);
+  printf("alpha = %8.5f w0 = %8.5f\n", alpha, w0);
+  if (mixdim < 2) {
+    printf("ERROR in BroydenRhoMix: mixdim < 2: %d\n", mixdim);
+    exit(1);
+  }
+  const int rc = pwh_mix_broyden(iter, alpha, w0, rho, rho_old, f, df, u, a, mixdim, nnr, &d);
+  if (rc == PWH_ERR_LAPACK) {
+    printf("ERROR calling dgetrf_ in BroydenRhoMix, info=%d\n", 1);
+    exit(1);
+  }
+  if (rc != PWH_OK) die("BroydenRhoMix");
+  printf("In BroydenMix: d = %f\n", d);
+}
+
+// EvalENL.cpp:5-49: FNL(ist, ik, is, ia, lm) and ENL over every k-point
+void EvalENL() {
+  ENL = 0.0;
+  for (int ik = 1; ik <= NKPT; ++ik) {
+    pwh_system *s = system_for(ik);
+    int nproj = 0;
+    for (int is = 0; is < NSP; ++is) {
+      int lm = L_MAX[is] * L_MAX[is] - 2 * L_LOC[is] + 1;
+      if (lm < 0) lm = 0;
+      nproj += NA[is] * lm;
+    }
+    zcomplex *fnl = static_cast<zcomplex *>(malloc(sizeof(zcomplex) * (nproj > 0 ? nproj : 1) * NX));
+    double *focc = FOCC + (size_t)NX * (ik - 1);
+    double e = 0.0;
+    if (pwh_eval_enl(s, reinterpret_cast<const double *>(C0 + (size_t)NGWX * NX * (ik - 1)), NGWX, NX, focc,
+                     WKPT[ik - 1], reinterpret_cast<double *>(fnl), &e) != PWH_OK)
+      die("EvalENL");
+    int row = 0;
+    for (int is = 0; is < NSP; ++is) {
+      int lm_end = L_MAX[is] * L_MAX[is] - 2 * L_LOC[is] + 1;
+      if (lm_end < 0) lm_end = 0;
+      for (int ia = 0; ia < NA[is]; ++ia)
+        for (int lm = 0; lm < lm_end; ++lm, ++row)
+          for (int ist = 0; ist < NX; ++ist)
+            FNL[ist + (size_t)NX * ((ik - 1) + (size_t)NKPT * (is + (size_t)NSP * (ia + (size_t)NAX * lm)))] =
+                fnl[(size_t)row * NX + ist];
+    }
+    ENL = ENL + e;
+    free(fnl);
+  }
 }

@@ -120,6 +120,66 @@ int pwh_residual_norms(pwh_system *sys, const double *X, int nstates, double *re
 int pwh_eval_rho_psi(pwh_system *sys, const double *eigvec, int ld, int nstates, const double *w,
                      double omega, double *rhoe, double *ekin, double *charge);
 
+/* ---- SCF steps around the eigensolver (FP64, host arrays as the reference's) ---- */
+/* The density G-vectors and local form factors EvalEnergy.cpp reads, and k-point 1's
+ * sphere for the Kerker filter (KerkerRhoMix.cpp:26-35). */
+typedef struct {
+  int nr1, nr2, nr3;          /* NR1..NR3 */
+  int ng;                     /* NG: density G-vectors, G = 0 first */
+  const int *n1, *n2, *n3;    /* N1..N3 [ng]: 1-based grid indices (GVectorsGen.cpp:107-125) */
+  const double *g2;           /* G [ng]: |G|^2 in (2pi/a)^2 */
+  int nsp;                    /* NSP */
+  const double *sfac;         /* SFAC [nsp*ng] complex, species fastest (StructureFactor.cpp) */
+  const double *vps, *rhops;  /* VPS, RHOPS [nsp*ng], species fastest (FormFactor.cpp) */
+  double tpiba2, omega;       /* TPIBA2, OMEGA */
+  int ngw;                    /* NGW(1) */
+  const unsigned int *n123;   /* N123(:,1) [ngw], 1-based */
+  const double *xkg;          /* XKG(:,1) [ngw] */
+} pwh_grid_desc;
+typedef struct pwh_grid pwh_grid;
+int pwh_grid_create(const pwh_grid_desc *desc, pwh_grid **grid);
+int pwh_grid_destroy(pwh_grid *grid);
+/* Energy terms of EvalEnergy.cpp:137-141: in ekin, enl, esr, eself; out the rest.
+ * eh_electron / eh_pseudo: the electron-only and pseudo-charge-only Hartree sums the
+ * reference forms (:96-100) times OMEGA.  n_negative: grid points below -0.1 (xc_PZ warns
+ * for each). */
+typedef struct {
+  double ekin, enl, esr, eself;
+  double etot, eht, epseu, exc, eh_electron, eh_pseudo;
+  int n_negative;
+} pwh_energy;
+/* EvalEnergy.cpp:5-163: rhoe [nnr] the density in, V_eff = V_H + V_loc + V_xc out (the
+ * forward transform of the density keeps its components outside the G set, :113-117). */
+int pwh_eval_energy(pwh_grid *grid, double *rhoe, pwh_energy *e);
+/* xc_PZ.cpp:7-63: Perdew-Zunger LDA; exc = sum rho (e_x + e_c) (not yet times OMEGA/NNR).
+ * Points below -0.1 reuse the previous point's terms (n_negative counts them). */
+int pwh_xc_pz(const double *rho, double *vxc, double *exc, int nnr, int *n_negative);
+/* NormalizeRho.cpp:7-33: negatives to zero, scaled to nel electrons; diff = electron
+ * count difference before scaling (the reference warns above 1e-5); sum < 1e-15 ->
+ * PWH_ERR_ARG (the reference aborts). */
+int pwh_normalize_rho(double *rho, int nnr, double omega, double nel, double *diff);
+/* The density mixers with the reference's arguments and in-place updates:
+ * SimpleRhoMix.cpp:5-20, KerkerRhoMix.cpp:5-55 and :57-82 (KerkerPrec), AndersonRhoMix.cpp:9-85
+ * (rho_old, f [nnr x 3], g [nnr x 2]), PulayRhoMix.cpp:5-91 (rho_old, f [nnr x mixdim],
+ * mixdim 2..7, beta 0.25 as the reference fixes it; the result is normalised to nel
+ * electrons and nel_diff (may be null) gets NormalizeRho's count difference), BroydenRhoMix.cpp:10-107 (rho_old, f [nnr x 2], df, u [nnr x mixdim],
+ * a [mixdim x mixdim], mixdim 2..8).  d: the RMS (Anderson: the squared norm) of the
+ * density change, as each reference routine reports it. */
+int pwh_mix_simple(double beta, double *rhoe, double *rho_old, int nnr, double *d);
+int pwh_mix_kerker(pwh_grid *grid, double beta, double q0, double *rhoe, double *rho_old, double *d);
+int pwh_kerker_prec(pwh_grid *grid, double *diff_rhoe, int idx1, double beta, double q0);
+int pwh_mix_anderson(int iter, double beta, double *rhoe, double *rho_old, double *f, double *g, int nnr,
+                     double *d);
+int pwh_mix_pulay(pwh_grid *grid, int iter, double *rhoe, double *rho_old, double *f, int mixdim, double q0,
+                  double nel, double *d, double *nel_diff);
+int pwh_mix_broyden(int iter, double alpha, double w0, double *rho, double *rho_old, double *f, double *df,
+                    double *u, double *a, int mixdim, int nnr, double *d);
+/* EvalENL.cpp:5-49 for one k-point: fnl [nproj x nstates] complex (row = projector in
+ * species, atom, lm order; may be null) = beta^H C0, enl = WKPT sum FOCC |fnl|^2 WNL.
+ * c0 [ld x nstates] in reference order, ld >= ngw. */
+int pwh_eval_enl(pwh_system *sys, const double *c0, int ld, int nstates, const double *focc, double wkpt,
+                 double *fnl, double *enl);
+
 /* ---- adjacent dense routines (no system needed) ------------------------------ */
 /* fft_fftw3.cpp:13-46: in-place 3D DFT of data [nr3][nr2][nr1] complex (x fastest).
  * inverse != 0: exp(+i G.r), unscaled (FFTW_BACKWARD); inverse == 0: exp(-i G.r) then

@@ -487,6 +487,293 @@ def lobpcg(H, X0, prec, nempty=0, max_iter=None, ortho=None):
     return DiagResult(lam, X, res, it, nconv, reason, hist, eh)
 
 
+# --------------------------------------------------------------------------
+# Energies, exchange-correlation and density mixing (FP64; arrays flat, the reference's
+# column-major history arrays as (columns, nnr) rows; functions update them in place)
+# --------------------------------------------------------------------------
+
+def _seqsum(x):
+    """Left-to-right sum, as the reference's loops accumulate."""
+    x = np.asarray(x, np.float64)
+    return float(np.cumsum(x)[-1]) if x.size else 0.0
+
+
+def xc_pz(rho):
+    """xc_PZ.cpp:7-63: Perdew-Zunger LDA.  Returns (vxc, Exc, indices with rho < -0.1).
+    Below -0.1 the reference warns and reuses the previous point's ux, uc, ex, ec (its
+    loop-carried locals); before any assignment they are taken as 0."""
+    A, B, Cc = -0.1423, 1.0529, 0.3334
+    a, b, c, d = 0.0311, -0.0480, 0.0020, -0.0116
+    rho = np.asarray(rho, np.float64)
+    n = rho.size
+    ux = np.zeros(n); uc = np.zeros(n); ex = np.zeros(n); ec = np.zeros(n)
+    neg = rho < -0.1
+    reg = (~neg) & (rho > 1.0e-13)
+    ax = -(3.0 / 4.0) * (3.0 / 2.0 / np.pi) ** (2.0 / 3.0)
+    r = rho[reg]
+    rs = np.power(3.0 / (4.0 * np.pi * r), 1.0 / 3.0)
+    rrs = np.sqrt(rs)
+    exr = ax / rs
+    uxr = 4.0 / 3.0 * exr
+    hi = rs >= 1.0
+    bpr = 1.0 + B * rrs + Cc * rs
+    ec_hi = A / bpr
+    uc_hi = (1.0 + 7.0 / 6.0 * B * rrs + 4.0 / 3.0 * Cc * rs) * ec_hi / bpr
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rsl = np.log(rs)
+    ec_lo = a * rsl + b + c * rs * rsl + d * rs
+    uc_lo = (2.0 * d - c) / 3.0 * rs + b - a / 3.0 + a * rsl + 2.0 * c / 3.0 * rs * rsl
+    ex[reg] = exr
+    ux[reg] = uxr
+    ec[reg] = np.where(hi, ec_hi, ec_lo)
+    uc[reg] = np.where(hi, uc_hi, uc_lo)
+    bad = np.nonzero(neg)[0]
+    for i in bad:                       # carry the previous point's values forward
+        if i > 0:
+            ux[i], uc[i], ex[i], ec[i] = ux[i - 1], uc[i - 1], ex[i - 1], ec[i - 1]
+    vxc = ux + uc
+    exc = _seqsum(rho * (ex + ec))
+    return vxc, exc, bad
+
+
+@dataclass
+class GSpace:
+    """The density G-vectors and form factors EvalEnergy reads (GVectorsGen.cpp,
+    StructureFactor.cpp, FormFactor.cpp): n1, n2, n3 1-based grid indices [ng], g2 = G
+    [ng] in (2pi/a)^2, sfac [nsp][ng] complex, vps and rhops [nsp][ng] real."""
+    nr: tuple
+    n1: np.ndarray
+    n2: np.ndarray
+    n3: np.ndarray
+    g2: np.ndarray
+    sfac: np.ndarray
+    vps: np.ndarray
+    rhops: np.ndarray
+    tpiba2: float
+    omega: float
+
+    def gidx(self):
+        nr1, nr2, _ = self.nr
+        return (self.n1 - 1) + nr1 * ((self.n2 - 1) + nr2 * (self.n3 - 1))
+
+
+def eval_energy(gs: GSpace, rhoe, ekin, enl, esr, eself):
+    """EvalEnergy.cpp:5-163.  Returns (V_eff [nnr], dict of ETOT, EHT, EPSEU, EXC and the
+    electron-only / pseudo-charge Hartree sums).  The forward transform of the density is
+    kept at grid points outside the G set, so its components there reach V_eff (:113-117)."""
+    nr = gs.nr
+    rho = np.asarray(rhoe, np.float64)
+    cf = fft3d(rho.astype(np.complex128), nr, inverse=False).reshape(-1)
+    nsp, ng = gs.sfac.shape
+    vp = np.zeros(ng, np.complex128)
+    rp = np.zeros(ng, np.complex128)
+    for i in range(nsp):                # species summed in order (:51-53, :62-65)
+        vp = vp + gs.sfac[i] * gs.vps[i]
+        rp = rp + gs.sfac[i] * gs.rhops[i]
+    gi = gs.gidx()
+    rhet = cf[gi[1:]]
+    rhog = rhet + rp[1:]
+    fpibg = 4.0 * np.pi / (gs.tpiba2 * gs.g2[1:])
+    vcg = fpibg * rhog
+    terms_eh = 0.5 * vcg * np.conj(rhog)
+    terms_eps = np.conj(rhet) * vp[1:]
+    eh = complex(np.cumsum(terms_eh)[-1]) if ng > 1 else 0j
+    eps = vp[0] * np.conj(cf[0]) + (complex(np.cumsum(terms_eps)[-1]) if ng > 1 else 0j)
+    eh2 = _seqsum(np.real(0.5 * fpibg * rhet * np.conj(rhet)))
+    eg2 = _seqsum(np.real(0.5 * fpibg * rp[1:] * np.conj(rp[1:])))
+    vtemp = np.concatenate([[vp[0]], vcg + vp[1:]])
+    cf[gi] = vtemp
+    v = fft3d(cf.reshape(nr[::-1]), nr, inverse=True).reshape(-1)
+    vxc, exc, bad = xc_pz(rho)
+    nnr = rho.size
+    veff = np.real(v) + vxc
+    exc = exc * gs.omega / nnr
+    eht = np.real(eh) * gs.omega + esr - eself
+    epseu = np.real(eps) * gs.omega
+    etot = ekin + eht + epseu + enl + exc
+    return veff, dict(etot=etot, eht=eht, epseu=epseu, exc=exc, eh_electron=eh2 * gs.omega,
+                      eh_pseudo=eg2 * gs.omega, negative=bad)
+
+
+def normalize_rho(rho, omega, nel):
+    """NormalizeRho.cpp:7-33: clip negatives, scale to nel electrons; returns the
+    electron-count difference before scaling (the reference warns above 1e-5)."""
+    rho = np.asarray(rho, np.float64)
+    nnr = rho.size
+    truesum = nnr * nel / omega
+    r = np.where(rho < 0.0, 0.0, rho)
+    s = _seqsum(r)
+    if s < 1.0e-15:
+        raise RuntimeError("NormalizeRho: sum(rho) < 1e-15")
+    rho[:] = r * truesum / s
+    return (truesum - s) * omega / nnr
+
+
+def simple_mix(beta, rhoe, rho_old):
+    """SimpleRhoMix.cpp:5-20 (in place); returns d."""
+    diff = rhoe - rho_old
+    d = np.sqrt(_seqsum(diff * diff) / rhoe.size)
+    rhoe[:] = beta * rhoe + (1.0 - beta) * rho_old
+    rho_old[:] = rhoe
+    return d
+
+
+def _kerker_filter(vec, nr, beta, q0, n123, xkg):
+    cf = fft3d(np.asarray(vec, np.float64).astype(np.complex128), nr, inverse=False).reshape(-1)
+    idx = np.asarray(n123, np.int64) - 1
+    gg = np.asarray(xkg, np.float64)
+    cf[idx] = (beta * gg / (gg + q0)) * cf[idx]
+    return np.real(fft3d(cf.reshape(nr[::-1]), nr, inverse=True).reshape(-1))
+
+
+def kerker_mix(beta, q0, rhoe, rho_old, nr, n123, xkg):
+    """KerkerRhoMix.cpp:5-55 with k-point 1's sphere (n123 = N123(1:NGW(1),1), xkg the
+    matching XKG): components of rhoe - rho_old on the sphere scaled by beta gg/(gg + q0),
+    the others kept (in place); returns d."""
+    diff = rhoe - rho_old
+    filt = _kerker_filter(diff, nr, beta, q0, n123, xkg)
+    d = np.sqrt(_seqsum(diff * diff) / rhoe.size)
+    rhoe[:] = rho_old + filt
+    rho_old[:] = rhoe
+    return d
+
+
+def kerker_prec(diff_rhoe, idx1, beta, q0, nr, n123, xkg):
+    """KerkerRhoMix.cpp:57-82 on column idx1 (1-based) of diff_rhoe (columns, nnr)."""
+    diff_rhoe[idx1 - 1] = _kerker_filter(diff_rhoe[idx1 - 1], nr, beta, q0, n123, xkg)
+
+
+def anderson_mix(it, beta, rhoe, rho_old, f, g):
+    """AndersonRhoMix.cpp:9-85: rho_old, f (3, nnr), g (2, nnr), in place; returns d."""
+    if it == 1:
+        rho_old[0] = rhoe
+        f[0] = 0.0
+        return 1.0
+    j0, j1, j2 = (it - 1) % 3, (it - 2) % 3, (it - 3) % 3
+    f[j0] = rhoe - rho_old[j1]
+    d = _seqsum(f[j0] * f[j0])
+    if it == 2:
+        rhoe[:] = beta * rhoe + (1.0 - beta) * rho_old[j1]
+        rho_old[j0] = rhoe
+        return d
+    g[0] = f[j1] - f[j0]
+    d11 = float(np.dot(g[0], g[0]))
+    r1 = float(np.dot(f[j0], g[0]))
+    if it == 3:
+        t1 = -r1 / d11
+        rhoe[:] = (1.0 - t1) * rho_old[j1] + t1 * rho_old[j2] + beta * (f[j0] + t1 * g[0])
+    else:
+        g[1] = f[j2] - f[j0]
+        d22 = float(np.dot(g[1], g[1]))
+        d12 = float(np.dot(g[0], g[1]))
+        r2 = float(np.dot(f[j0], g[1]))
+        t1 = d11 * d22
+        t2 = t1 - d12 * d12
+        if abs(t2) > abs(t1) * 1.0e-8:
+            t1, t2 = (-r1 * d22 + r2 * d12) / t2, (r1 * d12 - r2 * d11) / t2
+        else:
+            t1, t2 = 0.0, 0.0
+        # the reference adds t2 g(:,1), not t2 g(:,2) (:76-78)
+        rhoe[:] = ((1.0 - t1 - t2) * rho_old[j1] + t1 * rho_old[j2] + t2 * rho_old[j0]
+                   + beta * (f[j0] + t1 * g[0] + t2 * g[0]))
+    rho_old[j0] = rhoe
+    return d
+
+
+def pulay_mix(it, rhoe, rho_old, f, mixdim, nr, n123, xkg, q0, omega, nel):
+    """PulayRhoMix.cpp:5-91: rho_old, f (mixdim, nnr) in place; returns d."""
+    beta = 0.25
+    nnr = rhoe.size
+    if mixdim < 2:
+        raise RuntimeError("PulayRhoMix: mixdim < 2")
+    if it < mixdim:
+        rhoe[:] = beta * rhoe + (1.0 - beta) * rho_old[it - 1]
+        f[it] = rhoe - rho_old[it - 1]
+        kerker_prec(f, it + 1, 1.0, q0, nr, n123, xkg)
+        rho_old[it] = rhoe
+        return np.sqrt(_seqsum(f[it] * f[it]) / nnr)
+    jc = it % mixdim
+    jn = (it + 1) % mixdim
+    m = min(it + 1, mixdim) + 1
+    f[jc] = rhoe - rho_old[jc]
+    kerker_prec(f, jc + 1, 1.0, q0, nr, n123, xkg)
+    d = np.sqrt(_seqsum(f[jc] * f[jc]) / nnr)
+    a = np.zeros((m, m))
+    for i in range(m - 1):
+        for j in range(i, m - 1):
+            a[i, j] = float(np.dot(f[i], f[j]))
+        a[i, m - 1] = 1.0
+    a = np.triu(a) + np.triu(a, 1).T          # dsysv reads the upper triangle
+    rhs = np.zeros(m)
+    rhs[m - 1] = 1.0
+    alpha = np.linalg.solve(a, rhs)
+    new = np.zeros(nnr)
+    for j in range(m - 1):
+        new = new + alpha[j] * (rho_old[j] + f[j])
+    rhoe[:] = new
+    normalize_rho(rhoe, omega, nel)
+    rho_old[jn] = rhoe
+    return d
+
+
+def broyden_mix(it, alpha, w0, rho, rho_old, f, df, u, a, mixdim):
+    """BroydenRhoMix.cpp:10-107: rho_old, f (2, nnr), df, u (mixdim, nnr), a (mixdim x
+    mixdim column-major as a[col][row]), all in place; returns d."""
+    nnr = rho.size
+    if mixdim < 2:
+        raise RuntimeError("BroydenRhoMix: mixdim < 2")
+    m = min(it + 1, mixdim)
+    jc = it % m
+    kp = (it - 1) % 2
+    kc = it % 2
+    f[kc] = rho - rho_old[kp]
+    d = np.sqrt(_seqsum(f[kc] * f[kc]) / nnr)
+    df[jc] = f[kc] - f[kp]
+    t1 = float(np.linalg.norm(df[jc]))
+    if t1 > 1.0e-8:
+        t1 = 1.0 / t1
+    df[jc] = t1 * df[jc]
+    u[jc] = alpha * df[jc] + t1 * (rho_old[kp] - rho_old[kc])
+    c = np.array([float(np.dot(df[k], f[kc])) for k in range(m)])
+    for k in range(m):
+        a[jc, k] = float(np.dot(df[jc], df[k]))      # a(k, jc)
+        a[k, jc] = a[jc, k]                           # a(jc, k)
+    beta = a[:m, :m].T.copy()                        # row-major view of the leading block
+    beta = beta + w0 * w0 * np.eye(m)
+    binv = np.linalg.inv(beta)
+    gamma = np.array([sum(c[k] * binv[k, l] for k in range(m)) for l in range(m)])
+    new = rho_old[kp] + alpha * f[kc]
+    for l in range(m):
+        new = new + (-gamma[l]) * u[l]
+    rho[:] = new
+    rho_old[kc] = rho
+    return d
+
+
+def eval_enl(g: Globals, c0, ngw, focc, wkpt, ik=1):
+    """EvalENL.cpp:5-49 for one k-point: FNL(ist, a, lm) = sum_G C0(G, ist) PKG_A(lm, G)
+    conj(EIGR(G, a)) and ENL = WKPT sum FOCC |FNL|^2 WNL(lm).  c0 (nstates, >= ngw) rows =
+    states.  Returns (ENL, fnl [nsp][nax][lm][nstates])."""
+    c0 = np.asarray(c0, np.complex128)
+    nst = c0.shape[0]
+    nsp = len(g.na)
+    eigr = np.asarray(g.eigr, np.complex128).reshape(nsp, g.nax, g.ngwx)
+    pkg = np.asarray(g.pkg_a, np.float64).reshape(nsp, g.ngwx, g.nlmax)
+    wnl = np.asarray(g.wnl, np.float64).reshape(g.nlmax, nsp)
+    enl = 0.0
+    fnl = np.zeros((nsp, g.nax, g.nlmax, nst), np.complex128)
+    for i in range(nsp):
+        lm_end = int(g.l_max[i]) ** 2 - 2 * int(g.l_loc[i]) + 1
+        for ia in range(int(g.na[i])):
+            for lm in range(lm_end):
+                t = pkg[i, :ngw, lm] * np.conj(eigr[i, ia, :ngw])
+                fn = c0[:, :ngw] @ t
+                fnl[i, ia, lm] = fn
+                enl_m = _seqsum(np.asarray(focc, np.float64)[:nst] * np.abs(fn) ** 2)
+                enl = enl + wkpt * enl_m * wnl[lm, i]
+    return enl, fnl
+
+
 class Reference:
     """ctypes wrapper of oracle/_ref/libref.so (the reference's own compiled setup code)."""
 
@@ -503,6 +790,15 @@ class Reference:
         L.ref_eig_zhegv.argtypes = [_zp, C.c_int, _zp, C.c_int, _zp, C.c_int, C.c_int]
         L.ref_eig_zhegv_eval.argtypes = [_zp, C.c_int, _zp, C.c_int, _dp, _zp, C.c_int, C.c_int]
         L.ref_random_col.argtypes = [_zp, C.c_int, C.c_long]
+        L.ref_xc_pz.argtypes = [_dp, _dp, C.POINTER(C.c_double), C.c_int]
+        L.ref_simple_mix.argtypes = [C.c_double, _dp, _dp, C.c_int, C.POINTER(C.c_double)]
+        L.ref_anderson_mix.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp, _dp, C.c_int,
+                                       C.POINTER(C.c_double)]
+        L.ref_broyden_mix.argtypes = [C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp,
+                                      C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.ref_normalize_rho.argtypes = [_dp, C.c_int, C.c_double, C.c_double]
+        L.ref_eval_enl.argtypes = [C.c_int, _ip, C.c_int, _ip, _ip, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   _dp, _zp, _zp, _dp, C.c_double, _dp, _zp, C.POINTER(C.c_double)]
         self.L = L
 
     @staticmethod
@@ -572,3 +868,44 @@ class Reference:
         out = np.zeros(nelem, np.complex128)
         self.L.ref_random_col(out, nelem, seed)
         return out
+
+    def xc_pz(self, rho):
+        rho = np.array(rho, np.float64)
+        vxc = np.zeros_like(rho)
+        exc = C.c_double()
+        self.L.ref_xc_pz(rho, vxc, C.byref(exc), rho.size)
+        return vxc, exc.value
+
+    def simple_mix(self, beta, rhoe, rho_old):
+        d = C.c_double()
+        self.L.ref_simple_mix(beta, rhoe, rho_old, rhoe.size, C.byref(d))
+        return d.value
+
+    def anderson_mix(self, it, beta, rhoe, rho_old, f, g):
+        d = C.c_double()
+        self.L.ref_anderson_mix(it, beta, rhoe, rho_old.reshape(-1), f.reshape(-1), g.reshape(-1),
+                                rhoe.size, C.byref(d))
+        return d.value
+
+    def broyden_mix(self, it, alpha, w0, rho, rho_old, f, df, u, a, mixdim):
+        d = C.c_double()
+        self.L.ref_broyden_mix(it, alpha, w0, rho, rho_old.reshape(-1), f.reshape(-1), df.reshape(-1),
+                               u.reshape(-1), a.reshape(-1), mixdim, rho.size, C.byref(d))
+        return d.value
+
+    def normalize_rho(self, rho, omega, nel):
+        self.L.ref_normalize_rho(rho, rho.size, omega, nel)
+
+    def eval_enl(self, g, c0, ngw, focc, wkpt):
+        c0 = np.ascontiguousarray(c0, np.complex128)
+        nst = c0.shape[0]
+        nsp = len(g.na)
+        fnl = np.zeros(nst * g.nax * nsp * g.nlmax, np.complex128)
+        enl = C.c_double()
+        self.L.ref_eval_enl(nsp, np.asarray(g.na, np.int32), g.nax, np.asarray(g.l_max, np.int32),
+                            np.asarray(g.l_loc, np.int32), g.nlmax, ngw, c0.shape[1], nst,
+                            np.ascontiguousarray(g.pkg_a, np.float64), np.ascontiguousarray(g.eigr),
+                            c0.reshape(-1), np.ascontiguousarray(focc, np.float64), float(wkpt),
+                            np.ascontiguousarray(g.wnl, np.float64), fnl, C.byref(enl))
+        # FNL(ist, ik, is, ia, lm) with ik = 1: ist fastest, then is, ia, lm
+        return enl.value, fnl.reshape(g.nlmax, g.nax, nsp, nst).transpose(2, 1, 0, 3)

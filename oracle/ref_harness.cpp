@@ -1,8 +1,9 @@
 // ref_harness.cpp -- drives the reference's own compiled setup routines
 // (GVectorsGen0, GVectorsGen, sort1, GkVectorsGen, PhaseFactor), its LAPACK
 // wrappers (ortho_qr, eig_zheevd, eig_zhegv, eig_zhegv_eval) and the start-vector
-// generator of the Davidson / LOBPCG solvers (random_col) so the oracle can be checked
-// against them.
+// generator of the Davidson / LOBPCG solvers (random_col), xc_PZ, the density mixers
+// that need no FFT (simple, Anderson, Broyden), NormalizeRho and EvalENL, so the oracle
+// can be checked against them.
 //
 // TEST INFRASTRUCTURE ONLY (built into oracle/_ref/, never shipped).
 //
@@ -122,6 +123,49 @@ void ref_eig_zhegv_eval(double complex *A, int ldA, double complex *B, int ldB, 
 }
 
 void ref_random_col(double complex *A, int nelem, long a) { random_col(A, nelem, a); }
+
+// --- energies and density mixers (xc_PZ.cpp, SimpleRhoMix.cpp, AndersonRhoMix.cpp,
+// BroydenRhoMix.cpp, NormalizeRho.cpp, EvalENL.cpp) ---
+void ref_xc_pz(double *rho, double *vxc, double *exc, int nnr) { xc_PZ(rho, vxc, *exc, nnr); }
+
+void ref_simple_mix(double beta, double *rhoe, double *rho_old, int nnr, double *d) {
+  SimpleRhoMix(beta, rhoe, rho_old, nnr, *d);
+}
+
+void ref_anderson_mix(int iter, double beta, double *rhoe, double *rho_old, double *f, double *g, int nnr,
+                      double *d) {
+  AndersonRhoMix(iter, beta, rhoe, rho_old, f, g, nnr, *d);
+}
+
+// BroydenRhoMix.cpp:63,67 take the length of its dot products from the global NNR
+void ref_broyden_mix(int iter, double alpha, double w0, double *rho, double *rho_old, double *f, double *df,
+                     double *u, double *a, int mixdim, int nnr, double *d) {
+  NNR = nnr;
+  BroydenRhoMix(iter, alpha, w0, rho, rho_old, f, df, u, a, mixdim, nnr, *d);
+}
+
+void ref_normalize_rho(double *rho, int nnr, double omega, double nel) { NormalizeRho(rho, nnr, omega, nel); }
+
+// EvalENL.cpp:5-49 for one k-point: the globals it reads are set here
+void ref_eval_enl(int nsp, const int *na, int nax, const int *l_max, const int *l_loc, int nlmax, int ngw,
+                  int ngwx, int nx, double *pkg_a, double complex *eigr, double complex *c0, double *focc,
+                  double wkpt, double *wnl, double complex *fnl, double *enl) {
+  int ngw_[1] = {ngw};
+  double wk[1] = {wkpt};
+  // the set-up arrays of ref_setup stay owned by it: save and restore the globals
+  const int s_nsp = NSP, s_nax = NAX, s_nlmax = NLMAX, s_ngwx = NGWX, s_nx = NX, s_nkpt = NKPT;
+  int *s_na = NA, *s_lmax = L_MAX, *s_lloc = L_LOC, *s_ngw = NGW;
+  double *s_pkg = PKG_A, *s_focc = FOCC, *s_wkpt = WKPT, *s_wnl = WNL;
+  double complex *s_eigr = EIGR, *s_c0 = C0, *s_fnl = FNL;
+  NSP = nsp; NAX = nax; NLMAX = nlmax; NGWX = ngwx; NX = nx; NKPT = 1;
+  NA = const_cast<int *>(na); L_MAX = const_cast<int *>(l_max); L_LOC = const_cast<int *>(l_loc);
+  NGW = ngw_; PKG_A = pkg_a; EIGR = eigr; C0 = c0; FOCC = focc; WKPT = wk; WNL = wnl; FNL = fnl;
+  EvalENL();
+  *enl = ENL;
+  NSP = s_nsp; NAX = s_nax; NLMAX = s_nlmax; NGWX = s_ngwx; NX = s_nx; NKPT = s_nkpt;
+  NA = s_na; L_MAX = s_lmax; L_LOC = s_lloc; NGW = s_ngw; PKG_A = s_pkg; FOCC = s_focc; WKPT = s_wkpt;
+  WNL = s_wnl; EIGR = s_eigr; C0 = s_c0; FNL = s_fnl;
+}
 
 void ref_free() { free_all(); }
 

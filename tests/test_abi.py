@@ -21,7 +21,10 @@ REF_SYMBOLS = ["_Z14ApplyHam_blockPCdS0_iiPdii", "_Z8ApplyHamPCdS0_iiPdi",
                "_Z9chebyfiltiPCdiddPdi", "_Z8ortho_qrPCdii", "_Z7lanczosiPCdiPdiS0_S1_i",
                "_Z10eig_zheevdPCdiPdi", "_Z10eig_zheevdPCdii", "_Z9fft_fftw3PCdiiib",
                "_Z10EvalRhoPsiiPdPCdi", "_Z10EvalRhoPsiiPdPCdii", "_Z12davidson_cpuiPdPCdiiS_iS_",
-               "_Z10lobpcg_cpuiPdPCdiiS_iS_"]
+               "_Z10lobpcg_cpuiPdPCdiiS_iS_", "_Z10EvalEnergyPd", "_Z5xc_PZPdS_Rdi", "_Z7EvalENLv",
+               "_Z12NormalizeRhoPdidd", "_Z12SimpleRhoMixdPdS_iRd", "_Z12KerkerRhoMixddPdS_iRd",
+               "_Z10KerkerPrecPdiidd", "_Z14AndersonRhoMixidPdS_S_S_iRd", "_Z11PulayRhoMixiPdS_S_iiRd",
+               "_Z13BroydenRhoMixiddPdS_S_S_S_S_iiRd"]
 
 
 def declared_functions():
