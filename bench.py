@@ -270,6 +270,67 @@ def block_solvers(s, sysm, iters, cpu):
     return out
 
 
+def scf_steps(s, sysm, psi, steps, cpu):
+    """EvalEnergy, xc_PZ and the density mixers (EvalEnergy.cpp, xc_PZ.cpp, *RhoMix.cpp) on
+    the cell's grid with the density of the block, FP64.  Host-inclusive times per call
+    (the entry points take host arrays, as the reference's do) and the device time of
+    EvalEnergy from HIP events; the oracle's EvalEnergy beside it."""
+    import pwh
+    from pwh import synth
+    nb = psi.shape[0]
+    gsd = synth.density_gspace(s)
+    ngw = s.ngw
+    grid = pwh.Grid(s.nr, gsd["n1"], gsd["n2"], gsd["n3"], gsd["g2"], gsd["sfac"], gsd["vps"], gsd["rhops"],
+                    gsd["tpiba2"], gsd["omega"], n123=s.gv["n123"][:ngw], xkg=s.gv["xkg"][:ngw])
+    rho, _, _ = sysm.eval_rho_psi(psi, np.full(nb, 2.0), gsd["omega"])
+    n = rho.size
+
+    def per_call(fn, k=steps):
+        fn()
+        pwh.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        return 1e3 * (time.perf_counter() - t0) / k
+
+    out = {"grid": list(s.nr), "density_gvectors": int(len(gsd["n1"]))}
+    out["eval_energy_ms"] = round(per_call(lambda: grid.eval_energy(rho)), 4)
+    pwh.timing_reset()
+    pwh.timing_enable(True)
+    for _ in range(steps):
+        grid.eval_energy(rho)
+    pwh.synchronize()
+    ms, cnt = pwh.timing_get("energy")
+    pwh.timing_enable(False)
+    pwh.timing_reset()
+    out["eval_energy_device_ms"] = round(ms / steps, 4)
+    out["xc_pz_ms"] = round(per_call(lambda: pwh.xc_pz(rho)), 4)
+    old = rho * 0.97
+    out["mix_simple_ms"] = round(per_call(lambda: pwh.mix_simple(0.3, rho.copy(), old.copy())), 4)
+    out["mix_kerker_ms"] = round(per_call(lambda: grid.mix_kerker(0.3, 1.0, rho.copy(), old.copy())), 4)
+    st = [np.zeros((3, n)), np.zeros((3, n)), np.zeros((2, n))]
+    for it in (1, 2, 3):
+        pwh.mix_anderson(it, 0.3, rho * (1 + 0.01 * it), *st)
+    out["mix_anderson_ms"] = round(per_call(lambda: pwh.mix_anderson(5, 0.3, rho.copy(), *[x.copy() for x in st])), 4)
+    md = 4
+    sb = [np.zeros((2, n)), np.zeros((2, n)), np.zeros((md, n)), np.zeros((md, n)), np.zeros((md, md))]
+    sb[0][:] = rho
+    for it in (1, 2, 3):
+        pwh.mix_broyden(it, 0.25, 0.01, rho * (1 + 0.01 * it), *sb, md)
+    out["mix_broyden_ms"] = round(per_call(lambda: pwh.mix_broyden(4, 0.25, 0.01, rho.copy(), *[x.copy() for x in sb], md)), 4)
+    if cpu:
+        from oracle import oracle as O
+        gs = O.GSpace(nr=tuple(s.nr), **gsd)
+        t0 = time.perf_counter()
+        O.eval_energy(gs, rho, 0.0, 0.0, 0.0, 0.0)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(1e3 * dt, 2), "unit": "ms per EvalEnergy", "cores": 1, "kind": "port",
+                               "sample": f"one EvalEnergy of the oracle (C FFT, numpy G-space sums, xc_PZ) on the "
+                                         f"{s.nr[0]}^3 grid"}
+    grid.close()
+    return out
+
+
 def c3_batch(steps, degree=10):
     """24 uneven cells filtered together (one captured graph) vs one at a time."""
     import pwh
@@ -469,6 +530,11 @@ def main():
         rho = rho_section(s, sysm, psi, info, steps=max(5, min(args.steps, 20)), cpu=not args.no_cpu)
         log("bench: eval_rho_psi " + json.dumps(rho))
 
+    scf = None
+    if not args.only:
+        scf = scf_steps(s, sysm, psi, steps=max(5, min(args.steps, 20)), cpu=not args.no_cpu)
+        log("bench: scf steps " + json.dumps(scf))
+
     solvers = None
     if not args.only and not args.no_chfsi:
         solvers = block_solvers(s, sysm, iters=10, cpu=not args.no_cpu)
@@ -512,6 +578,7 @@ def main():
         "chfsi": chfsi or None,
         "eval_rho_psi": rho,
         "block_solvers": solvers,
+        "scf_steps": scf,
         "c3_batched_uneven": extra.get("c3_batched_uneven"),
         "c4_si216": extra.get("c4_si216"),
         "cpu_baseline": cpu,

@@ -53,10 +53,10 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 enum TimerId { TM_ZINV, TM_PLANE, TM_ZFWD, TM_PROJ, TM_EXPAND, TM_HPSI, TM_RHO_SUMS, TM_RHO_ZINV, TM_RHO_PLANE,
-               TM_RHO, TM_COUNT };
+               TM_RHO, TM_ENERGY, TM_COUNT };
 constexpr size_t kRocblasWorkspace = size_t(128) << 20;   // bytes
 const char* kTimerNames[TM_COUNT] = {"zinv", "plane", "zfwd", "proj", "expand", "hpsi",
-                                     "rho_sums", "rho_zinv", "rho_plane", "rho"};
+                                     "rho_sums", "rho_zinv", "rho_plane", "rho", "energy"};
 
 struct Ctx {
   bool ready = false;
@@ -1803,18 +1803,25 @@ int pwh_eval_energy(pwh_grid* gr, double* rhoe, pwh_energy* e) {
   CHK(w.part.ensure(4 * kScfBlocks));
   // forward transform of the density, G-space sums and the electrostatic potential,
   // inverse transform (EvalEnergy.cpp:38-117)
-  HIPCHK(real_to_cx(w.v[0].p, nullptr, n, gr->c.p, g.stream));
-  CHK(grid_fft(gr, gr->c.p, false));
-  HIPCHK(energy_g(gr->c.p, gr->gidx.p, gr->g2.p, gr->vp.p, gr->rp.p, gr->ng, gr->tpiba2, w.part.p, g.stream));
   std::vector<double> hp(4 * kScfBlocks);
-  HIPCHK(hipMemcpyAsync(hp.data(), w.part.p, sizeof(double) * 4 * kScfBlocks, hipMemcpyDeviceToHost, g.stream));
-  CHK(grid_fft(gr, gr->c.p, true));
+  {
+    Timed t(TM_ENERGY, g.stream);
+    HIPCHK(real_to_cx(w.v[0].p, nullptr, n, gr->c.p, g.stream));
+    CHK(grid_fft(gr, gr->c.p, false));
+    HIPCHK(energy_g(gr->c.p, gr->gidx.p, gr->g2.p, gr->vp.p, gr->rp.p, gr->ng, gr->tpiba2, w.part.p, g.stream));
+    HIPCHK(hipMemcpyAsync(hp.data(), w.part.p, sizeof(double) * 4 * kScfBlocks, hipMemcpyDeviceToHost,
+                          g.stream));
+    CHK(grid_fft(gr, gr->c.p, true));
+  }
   CHK(scf_sync());
   double sums[4] = {0, 0, 0, 0};
   for (int q = 0; q < 4; ++q)
     for (int b = 0; b < kScfBlocks; ++b) sums[q] += hp[(size_t)q * kScfBlocks + b];
   // xc_PZ and V_eff = Re(V_es) + V_xc (:124-135)
-  HIPCHK(xc_pz(w.v[0].p, n, w.vxc.p, w.eps.p, w.neg.p, gr->c.p, w.v[1].p, w.part.p, g.stream));
+  {
+    Timed t(TM_ENERGY, g.stream);
+    HIPCHK(xc_pz(w.v[0].p, n, w.vxc.p, w.eps.p, w.neg.p, gr->c.p, w.v[1].p, w.part.p, g.stream));
+  }
   std::vector<double> h(kScfBlocks);
   const double cnt = sum_partials(w.part.p, kScfBlocks, h.data());
   if (cnt > 0) HIPCHK(xc_pz_fix(w.v[0].p, n, w.neg.p, w.vxc.p, w.eps.p, gr->c.p, w.v[1].p, g.stream));

@@ -195,3 +195,32 @@ def gen_prec(xkg, tpiba2, alat, tau):
         y = 27.0 + x * (18.0 + x * (12.0 + 8.0 * x))
         return y / (y + 16.0 * x ** 4)
     return tau / (0.5 * (xkg + 1.0) * tpiba2)
+
+
+def density_gspace(s):
+    """The density G-vectors of a synthetic cell (GVectorsGen.cpp: |G|^2 <= 4 ecut for the
+    wave-function cutoff ecut), their structure factor for the cell's atoms and Gaussian
+    local form factors (synthetic, like the cells' projectors).  Returns the arrays
+    pwh.Grid takes: n1, n2, n3 (1-based grid indices), g2 (|G|^2 in (2pi/a)^2), sfac
+    [1][ng], vps [1][ng], rhops [1][ng], tpiba2, omega."""
+    from . import gvectors
+    gv = gvectors(s.A, s.ecut, s.nr)
+    ng, ngx = gv["ng"], gv["ngx"]
+    in1, in2, in3 = (gv["in123"][k * ngx:k * ngx + ng].astype(np.int64) for k in range(3))
+    nr = s.nr
+    n1 = np.where(in1 < 0, in1 + 1 + nr[0], in1 + 1)
+    n2 = np.where(in2 < 0, in2 + 1 + nr[1], in2 + 1)
+    n3 = np.where(in3 < 0, in3 + 1 + nr[2], in3 + 1)
+    B = np.linalg.inv(s.A).T * s.a                  # reciprocal vectors in 2pi/a
+    Gv = np.stack([in1, in2, in3], axis=1) @ B
+    g2 = np.sum(Gv * Gv, axis=1)
+    tau = s.tau.reshape(-1, 3) / s.a
+    sfac = np.sum(np.exp(-2j * np.pi * (Gv @ tau.T)), axis=1)[None, :]
+    omega = abs(np.linalg.det(s.A))
+    tpiba2 = (2 * np.pi / s.a) ** 2
+    gg = g2 * tpiba2
+    safe = np.where(gg > 0, gg, 1.0)
+    vps = np.where(gg > 0, -4 * np.pi * 4.0 / (omega * safe) * np.exp(-0.25 * gg), 0.3)
+    rhops = -4.0 / omega * np.exp(-0.25 * gg)
+    return dict(n1=n1, n2=n2, n3=n3, g2=g2, sfac=sfac, vps=vps[None, :], rhops=rhops[None, :],
+                tpiba2=tpiba2, omega=omega)

@@ -278,8 +278,9 @@ int pwh_timing_enable(int on);
 int pwh_timing_reset(void);
 /* name: "zinv", "plane", "zfwd", "proj", "expand", "hpsi" (H|psi> passes), or "rho_sums",
  * "rho_zinv", "rho_plane", "rho" (the device part of pwh_eval_rho_psi: permutation and
- * per-state sums, i3 transform, plane pass with the band sum, all three).  total_ms /
- * count of launches. */
+ * per-state sums, i3 transform, plane pass with the band sum, all three), or "energy" (the
+ * two device sections of pwh_eval_energy: transforms and G-space sums, then xc_PZ).
+ * total_ms / count of launches. */
 int pwh_timing_get(const char *name, double *total_ms, int *count);
 
 #ifdef __cplusplus

@@ -53,28 +53,9 @@ def gpu_si8(si8):
 
 
 def density_gspace(s):
-    """The density G-vectors of a synthetic cell (GVectorsGen.cpp: |G|^2 <= 4 ecut for the
-    wave-function cutoff ecut) with its structure factor and Gaussian local form factors
-    (synthetic, as the cells are): returns an oracle GSpace."""
+    """The synthetic cell's density G-vectors and form factors (pwh.synth.density_gspace)
+    as an oracle GSpace."""
     from oracle import oracle as O
-    from pwh import gvectors
-    gv = gvectors(s.A, s.ecut, s.nr)
-    ng, ngx = gv["ng"], gv["ngx"]
-    in1, in2, in3 = (gv["in123"][k * ngx:k * ngx + ng].astype(np.int64) for k in range(3))
-    nr = s.nr
-    n1 = np.where(in1 < 0, in1 + 1 + nr[0], in1 + 1)
-    n2 = np.where(in2 < 0, in2 + 1 + nr[1], in2 + 1)
-    n3 = np.where(in3 < 0, in3 + 1 + nr[2], in3 + 1)
-    B = np.linalg.inv(s.A).T * s.a                  # reciprocal vectors in 2pi/a
-    Gv = np.stack([in1, in2, in3], axis=1) @ B
-    g2 = np.sum(Gv * Gv, axis=1)
-    tau = s.tau.reshape(-1, 3) / s.a
-    sfac = np.sum(np.exp(-2j * np.pi * (Gv @ tau.T)), axis=1)[None, :]
-    omega = abs(np.linalg.det(s.A))
-    tpiba2 = (2 * np.pi / s.a) ** 2
-    gg = g2 * tpiba2
-    with np.errstate(divide="ignore"):
-        vps = np.where(gg > 0, -4 * np.pi * 4.0 / (omega * np.where(gg > 0, gg, 1.0)) * np.exp(-0.25 * gg), 0.3)
-    rhops = (-4.0 / omega * np.exp(-0.25 * gg))
-    return O.GSpace(nr=tuple(nr), n1=n1, n2=n2, n3=n3, g2=g2, sfac=sfac, vps=vps[None, :],
-                    rhops=rhops[None, :], tpiba2=tpiba2, omega=omega)
+    from pwh import synth
+    d = synth.density_gspace(s)
+    return O.GSpace(nr=tuple(s.nr), **d)
