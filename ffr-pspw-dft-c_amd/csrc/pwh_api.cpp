@@ -22,6 +22,7 @@
 #include "pwh_blas.h"
 #include "pwh_internal.h"
 #include "pwh_scf.h"
+#include "pwh_setup_dev.h"
 
 using namespace pwh;
 
@@ -2116,6 +2117,227 @@ int pwh_eval_enl(pwh_system* s, const double* c0, int ld, int nstates, const dou
         e = e + wkpt * rows[s->poff[sp] + a * s->lm[sp] + l] * s->wl[4 * sp + l];
   }
   *enl = e;
+  return PWH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Set-up on the GPU: PhaseFactor, StructureFactor, NLFormFactor, FormFactor
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace pwh {
+void reduced_coords(const double A[9], const double* tau, int n, double* out);
+}
+
+namespace {
+
+struct EiTables {
+  Buf<cx<double>> ei;
+  Buf<double> taup;
+  Buf<int> na;
+  int ldt = 0;
+  int idx0[3] = {0, 0, 0};
+};
+
+// EI tables of every (species, atom) slot; atoms past NA(is) keep zeros
+int ei_build(EiTables& t, const double A[9], const int nr[3], int nsp, const int* na, int nax, const double* tau) {
+  const int slots = nsp * nax;
+  std::vector<double> tp((size_t)3 * slots, 0.0);
+  for (int is = 0; is < nsp; ++is) {
+    if (na[is] < 0 || na[is] > nax) return fail(PWH_ERR_ARG, "NA(is) outside 0..NAX");
+    pwh::reduced_coords(A, tau + (size_t)3 * nax * is, na[is], tp.data() + (size_t)3 * nax * is);
+  }
+  t.ldt = std::max(nr[0], std::max(nr[1], nr[2])) + 2;
+  for (int d = 0; d < 3; ++d) t.idx0[d] = (nr[d] + 1) / 2;
+  CHK(t.ei.ensure((size_t)slots * 3 * t.ldt));
+  HIPCHK(hipMemsetAsync(t.ei.p, 0, sizeof(cx<double>) * slots * 3 * t.ldt, g.stream));
+  CHK(upload(t.taup, tp.data(), tp.size()));
+  CHK(upload(t.na, na, nsp));
+  // the tables of every slot are written; empty slots are overwritten by zeros below
+  HIPCHK(ei_tables(t.taup.p, slots, nr, t.ldt, t.ei.p, g.stream));
+  for (int is = 0; is < nsp; ++is)
+    if (na[is] < nax)
+      HIPCHK(hipMemsetAsync(t.ei.p + (size_t)(is * nax + na[is]) * 3 * t.ldt, 0,
+                            sizeof(cx<double>) * (nax - na[is]) * 3 * t.ldt, g.stream));
+  return PWH_OK;
+}
+
+int pseudo_check(const pwh_pseudo_desc* ps) {
+  if (!ps || ps->nsp < 1 || !ps->mmax || !ps->clog || !ps->r || !ps->vion || !ps->l_loc || ps->maxr < 1 ||
+      ps->nang < 1)
+    return fail(PWH_ERR_ARG, "bad pseudopotential description");
+  for (int is = 0; is < ps->nsp; ++is)
+    if (ps->mmax[is] < 3 || ps->mmax[is] > ps->maxr || ps->mmax[is] % 2 == 0)
+      return fail(PWH_ERR_ARG, "MMAX must be odd and <= MAXR (simpson.cpp:8-11)");
+  return PWH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pwh_phase_factor_gpu(const double A[9], const int nr[3], int nsp, const int* na, int nax, const double* tau,
+                         int nkpt, const int* ngw, int ngwx, const int* igk, int ngx, const int* in123,
+                         double* eigr) {
+  if (!A || !nr || nsp < 1 || !na || nax < 1 || !tau || nkpt < 1 || !ngw || !igk || !in123 || !eigr || ngwx < 1)
+    return fail(PWH_ERR_ARG, "bad phase_factor arguments");
+  CHK(ensure_init());
+  EiTables t;
+  CHK(ei_build(t, A, nr, nsp, na, nax, tau));
+  Buf<int> digk, din;
+  Buf<cx<double>> de;
+  CHK(upload(din, in123, (size_t)3 * ngx));
+  CHK(digk.ensure((size_t)ngwx));
+  const size_t per_k = (size_t)ngwx * nax * nsp;
+  CHK(de.ensure(per_k));
+  for (int ik = 0; ik < nkpt; ++ik) {
+    if (ngw[ik] < 0 || ngw[ik] > ngwx) return fail(PWH_ERR_ARG, "NGW(ik) outside 0..NGWX");
+    for (int ig = 0; ig < ngw[ik]; ++ig)
+      if (igk[ig + (size_t)ngwx * ik] < 1 || igk[ig + (size_t)ngwx * ik] > ngx)
+        return fail(PWH_ERR_ARG, "IGK outside 1..NGX");
+    HIPCHK(hipMemcpyAsync(digk.p, igk + (size_t)ngwx * ik, sizeof(int) * ngwx, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemsetAsync(de.p, 0, sizeof(cx<double>) * per_k, g.stream));
+    if (ngw[ik] > 0)
+      HIPCHK(eigr_dev(t.ei.p, t.ldt, t.na.p, nsp, nax, digk.p, din.p, ngx, ngw[ik], ngwx, t.idx0, de.p, g.stream));
+    HIPCHK(hipMemcpyAsync(eigr + 2 * per_k * ik, de.p, sizeof(cx<double>) * per_k, hipMemcpyDeviceToHost,
+                          g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+  }
+  return PWH_OK;
+}
+
+int pwh_structure_factor(const double A[9], const int nr[3], int nsp, const int* na, int nax, const double* tau,
+                         int ng, int ngx, const int* in123, double* sfac) {
+  if (!A || !nr || nsp < 1 || !na || nax < 1 || !tau || ng < 1 || ngx < ng || !in123 || !sfac)
+    return fail(PWH_ERR_ARG, "bad structure_factor arguments");
+  CHK(ensure_init());
+  EiTables t;
+  CHK(ei_build(t, A, nr, nsp, na, nax, tau));
+  Buf<int> din;
+  Buf<cx<double>> ds;
+  CHK(upload(din, in123, (size_t)3 * ngx));
+  CHK(ds.ensure((size_t)nsp * ng));
+  HIPCHK(sfac_dev(t.ei.p, t.ldt, t.na.p, nax, nsp, din.p, ngx, ng, t.idx0, ds.p, g.stream));
+  HIPCHK(hipMemcpyAsync(sfac, ds.p, sizeof(cx<double>) * nsp * ng, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return PWH_OK;
+}
+
+int pwh_nl_form_factor(const pwh_pseudo_desc* ps, double omega, double tpiba, int nkpt, const int* ngw, int ngwx,
+                       const double* xkg, const double* ggk, const double* xk, int nlmax, double* wnl, double* pkg,
+                       double* pkg_a) {
+  CHK(pseudo_check(ps));
+  if (!ps->psi || !ps->l_max || nkpt < 1 || !ngw || ngwx < 1 || !xkg || !ggk || !xk || nlmax < 1 || !wnl ||
+      !pkg_a || !(omega > 0))
+    return fail(PWH_ERR_ARG, "bad nl_form_factor arguments");
+  CHK(ensure_init());
+  const int nsp = ps->nsp, maxr = ps->maxr;
+  for (int is = 0; is < nsp; ++is) {
+    int ml = 0;
+    for (int l = 1; l <= ps->l_max[is]; ++l) {
+      if (l == ps->l_loc[is]) continue;
+      if (l >= 4) return fail(PWH_ERR_UNSUPPORTED, "not implemented i_l = " + std::to_string(l));
+      if (l > ps->nang || ps->l_loc[is] < 1 || ps->l_loc[is] > ps->nang)
+        return fail(PWH_ERR_ARG, "channel outside the pseudopotential data");
+      ml += 2 * l - 1;
+    }
+    if (ml > nlmax) return fail(PWH_ERR_ARG, "more projector components than NLMAX");
+  }
+  const double fpibo = 4.0 * M_PI / omega;
+  Buf<double> dr, dfun, dfun1, dvl, dvloc, dpsi, dxkg, dggk, dw;
+  Buf<double> dpkg;
+  CHK(dr.ensure(maxr));
+  CHK(dfun.ensure(maxr));
+  CHK(dfun1.ensure(maxr));
+  CHK(dw.ensure(1));
+  CHK(dxkg.ensure(ngwx));
+  CHK(dggk.ensure((size_t)3 * ngwx));
+  const size_t per_k = (size_t)nlmax * ngwx * nsp;
+  CHK(dpkg.ensure(per_k));
+  std::vector<double> hp(per_k);
+  for (int ik = 0; ik < nkpt; ++ik) {
+    if (ngw[ik] < 0 || ngw[ik] > ngwx) return fail(PWH_ERR_ARG, "NGW(ik) outside 0..NGWX");
+    HIPCHK(hipMemcpyAsync(dxkg.p, xkg + (size_t)ngwx * ik, sizeof(double) * ngwx, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemcpyAsync(dggk.p, ggk + (size_t)3 * ngwx * ik, sizeof(double) * 3 * ngwx, hipMemcpyHostToDevice,
+                          g.stream));
+    // start from the caller's PKG_A so components no channel writes keep their values
+    HIPCHK(hipMemcpyAsync(dpkg.p, pkg_a + per_k * ik, sizeof(double) * per_k, hipMemcpyHostToDevice, g.stream));
+    for (int is = 0; is < nsp; ++is) {
+      const int m = ps->mmax[is];
+      const double h = ps->clog[is];
+      CHK(upload(dr, ps->r + (size_t)maxr * is, m));
+      CHK(upload(dvloc, ps->vion + (size_t)maxr * (is + (size_t)nsp * (ps->l_loc[is] - 1)), m));
+      int ml = 0;
+      for (int l = 1; l <= ps->l_max[is]; ++l) {
+        if (l == ps->l_loc[is]) continue;
+        CHK(upload(dvl, ps->vion + (size_t)maxr * (is + (size_t)nsp * (l - 1)), m));
+        CHK(upload(dpsi, ps->psi + (size_t)maxr * (is + (size_t)nsp * (l - 1)), m));
+        HIPCHK(nl_prep(dr.p, dvl.p, dvloc.p, dpsi.p, m, dfun.p, dfun1.p, g.stream));
+        if (ik == 0) {
+          HIPCHK(nl_wnl(dfun1.p, m, h, fpibo, l, dw.p, g.stream));
+          double w = 0;
+          HIPCHK(hipMemcpyAsync(&w, dw.p, sizeof(double), hipMemcpyDeviceToHost, g.stream));
+          HIPCHK(hipStreamSynchronize(g.stream));
+          for (int j = 0; j < 2 * l - 1; ++j) wnl[is + (size_t)nsp * (ml + j)] = w;
+        }
+        if (ngw[ik] > 0)
+          HIPCHK(nl_pkg(dr.p, dfun.p, m, h, l, ml, dxkg.p, dggk.p, xk + 3 * ik, ngw[ik], tpiba, nlmax,
+                        dpkg.p + (size_t)nlmax * ngwx * is, g.stream));
+        ml += 2 * l - 1;
+      }
+    }
+    HIPCHK(hipMemcpyAsync(hp.data(), dpkg.p, sizeof(double) * per_k, hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    for (size_t i = 0; i < per_k; ++i) pkg_a[per_k * ik + i] = hp[i];
+    if (pkg) {
+      // PKG(ig, is, ik, m) = PKG_A(m, ig, is, ik) for the written components (:119-121)
+      for (int is = 0; is < nsp; ++is) {
+        int ml = 0;
+        for (int l = 1; l <= ps->l_max[is]; ++l) {
+          if (l == ps->l_loc[is]) continue;
+          for (int j = 0; j < 2 * l - 1; ++j)
+            for (int ig = 0; ig < ngw[ik]; ++ig)
+              pkg[ig + (size_t)ngwx * (is + (size_t)nsp * (ik + (size_t)nkpt * (ml + j)))] =
+                  hp[(ml + j) + (size_t)nlmax * (ig + (size_t)ngwx * is)];
+          ml += 2 * l - 1;
+        }
+      }
+    }
+  }
+  return PWH_OK;
+}
+
+int pwh_form_factor(const pwh_pseudo_desc* ps, const int* na, double omega, double tpiba2, int ng, const double* g2,
+                    double* vps, double* rhops, double* eself) {
+  CHK(pseudo_check(ps));
+  if (!na || ng < 1 || !g2 || !vps || !rhops || !eself || !ps->zv || !ps->rgauss || !(omega > 0) || !(tpiba2 > 0))
+    return fail(PWH_ERR_ARG, "bad form_factor arguments");
+  for (int is = 0; is < ps->nsp; ++is)
+    if (ps->l_loc[is] < 1 || ps->l_loc[is] > ps->nang) return fail(PWH_ERR_ARG, "ERROR in formf: ll < 1");
+  CHK(ensure_init());
+  const int nsp = ps->nsp, maxr = ps->maxr;
+  // ESELF (FormFactor.cpp:19-24)
+  double es = 0.0;
+  for (int is = 0; is < nsp; ++is) es = es + ps->zv[is] * ps->zv[is] / ps->rgauss[is] * na[is];
+  *eself = es * (1.0 / std::sqrt(2.0 * M_PI));
+  const double tpiba = std::sqrt(tpiba2), fpibom = 4.0 * M_PI / omega;
+  Buf<double> dr, dvl, dvloc, dg, dv, drh;
+  CHK(upload(dg, g2, ng));
+  CHK(upload(dv, vps, (size_t)nsp * ng));
+  CHK(upload(drh, rhops, (size_t)nsp * ng));
+  for (int is = 0; is < nsp; ++is) {
+    const int m = ps->mmax[is];
+    CHK(upload(dr, ps->r + (size_t)maxr * is, m));
+    CHK(upload(dvl, ps->vion + (size_t)maxr * (is + (size_t)nsp * (ps->l_loc[is] - 1)), m));
+    CHK(dvloc.ensure(m));
+    HIPCHK(ff_prep(dr.p, dvl.p, m, ps->zv[is], ps->rgauss[is], dvloc.p, g.stream));
+    const double facc = 0.25 * tpiba2 * ps->rgauss[is] * ps->rgauss[is];
+    HIPCHK(ff_vps(dr.p, dvloc.p, m, ps->clog[is], dg.p, ng, tpiba, fpibom, ps->zv[is], omega, facc, nsp, is, dv.p,
+                  drh.p, g.stream));
+  }
+  HIPCHK(hipMemcpyAsync(vps, dv.p, sizeof(double) * nsp * ng, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipMemcpyAsync(rhops, drh.p, sizeof(double) * nsp * ng, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
   return PWH_OK;
 }
 

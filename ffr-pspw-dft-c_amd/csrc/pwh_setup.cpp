@@ -128,6 +128,24 @@ void inverse3(const double* hm, double* hi) {
 
 }  // namespace
 
+namespace pwh {
+// Reduced coordinates tau' = A^-1 tau of n atoms (PhaseFactor.cpp:31-40), the inverse with
+// the cofactor order of matinv3x3.cpp; used by the GPU phase and structure factors.
+void reduced_coords(const double A[9], const double* tau, int n, double* out) {
+  double xmat[9], xmati[9];
+  for (int i = 0; i < 9; ++i) xmat[i] = A[i];
+  inverse3(xmat, xmati);
+  for (int a = 0; a < n; ++a)
+    for (int d = 0; d < 3; ++d) {
+      const double* t = tau + 3 * a;
+      double s1 = xmati[d] * t[0];
+      s1 = s1 + xmati[d + 3] * t[1];
+      s1 = s1 + xmati[d + 6] * t[2];
+      out[3 * a + d] = s1;
+    }
+}
+}  // namespace pwh
+
 extern "C" {
 
 int pwh_fft_grid(const double A[9], double ecut, int nr[3]) {

@@ -44,6 +44,7 @@ EXPORTS = [
     "pwh_eval_rho_psi", "pwh_davidson", "pwh_lobpcg", "pwh_grid_create", "pwh_grid_destroy",
     "pwh_eval_energy", "pwh_xc_pz", "pwh_normalize_rho", "pwh_mix_simple", "pwh_mix_kerker",
     "pwh_kerker_prec", "pwh_mix_anderson", "pwh_mix_pulay", "pwh_mix_broyden", "pwh_eval_enl",
+    "pwh_phase_factor_gpu", "pwh_structure_factor", "pwh_nl_form_factor", "pwh_form_factor",
 ]
 
 
@@ -82,6 +83,12 @@ class GridDesc(C.Structure):
                 ("nsp", C.c_int), ("sfac", C.c_void_p), ("vps", C.c_void_p), ("rhops", C.c_void_p),
                 ("tpiba2", C.c_double), ("omega", C.c_double), ("ngw", C.c_int), ("n123", C.c_void_p),
                 ("xkg", C.c_void_p)]
+
+
+class PseudoDesc(C.Structure):
+    _fields_ = [("nsp", C.c_int), ("maxr", C.c_int), ("nang", C.c_int), ("mmax", C.c_void_p),
+                ("clog", C.c_void_p), ("r", C.c_void_p), ("psi", C.c_void_p), ("vion", C.c_void_p),
+                ("zv", C.c_void_p), ("rgauss", C.c_void_p), ("l_max", C.c_void_p), ("l_loc", C.c_void_p)]
 
 
 class Energy(C.Structure):
@@ -166,6 +173,12 @@ def lib():
         "pwh_mix_broyden": [C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_int,
                             C.POINTER(C.c_double)],
         "pwh_eval_enl": [vp, vp, C.c_int, C.c_int, _dp, C.c_double, vp, C.POINTER(C.c_double)],
+        "pwh_phase_factor_gpu": [_dp, _ip, C.c_int, _ip, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, C.c_int, _ip, vp],
+        "pwh_structure_factor": [_dp, _ip, C.c_int, _ip, C.c_int, _dp, C.c_int, C.c_int, _ip, vp],
+        "pwh_nl_form_factor": [C.POINTER(PseudoDesc), C.c_double, C.c_double, C.c_int, _ip, C.c_int, _dp, _dp, _dp,
+                               C.c_int, _dp, vp, _dp],
+        "pwh_form_factor": [C.POINTER(PseudoDesc), _ip, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp,
+                            C.POINTER(C.c_double)],
         "pwh_lobpcg": [vp, vp, C.c_int, _dp, C.POINTER(DiagOpts), _dp, _dp],
     }
     for name, args in proto.items():
@@ -262,6 +275,76 @@ def phase_factor(A, nr, na, nax, tau, gv):
                                   gv["nkpt"], gv["ngw"], gv["ngwx"], gv["igk"], gv["ngx"],
                                   gv["in123"], _zptr(eigr)))
     return eigr
+
+
+def phase_factor_gpu(A, nr, na, nax, tau, gv):
+    """PhaseFactor.cpp:5-80 on the GPU -> EIGR [ngwx*nax*nsp*nkpt] complex128."""
+    na = np.asarray(na, np.int32)
+    eigr = np.zeros(gv["ngwx"] * nax * len(na) * gv["nkpt"], np.complex128)
+    _check(lib().pwh_phase_factor_gpu(_lat(A), np.asarray(nr, np.int32), len(na), na, nax,
+                                      np.ascontiguousarray(np.asarray(tau, np.float64).reshape(-1)),
+                                      gv["nkpt"], gv["ngw"], gv["ngwx"], gv["igk"], gv["ngx"], gv["in123"],
+                                      _zptr(eigr)))
+    return eigr
+
+
+def structure_factor(A, nr, na, nax, tau, gv):
+    """StructureFactor.cpp:5-29 on the GPU for the density G-vectors of gv -> [nsp][ng]."""
+    na = np.asarray(na, np.int32)
+    sf = np.zeros(len(na) * gv["ng"], np.complex128)
+    _check(lib().pwh_structure_factor(_lat(A), np.asarray(nr, np.int32), len(na), na, nax,
+                                      np.ascontiguousarray(np.asarray(tau, np.float64).reshape(-1)), gv["ng"],
+                                      gv["ngx"], gv["in123"], _zptr(sf)))
+    return sf.reshape(gv["ng"], len(na)).T.copy()
+
+
+def _pseudo(ps, maxr=570, maxam=3):
+    """ctypes description + kept arrays of an oracle-style Pseudo (mmax, clog, r [nsp][m],
+    psi, vion [nsp][nang][m], zv, rgauss, l_max, l_loc)."""
+    nsp = len(ps.mmax)
+    r = np.zeros(maxr * nsp)
+    psi = np.zeros(maxr * nsp * maxam)
+    vion = np.zeros(maxr * nsp * maxam)
+    for i in range(nsp):
+        m = int(ps.mmax[i])
+        r[i * maxr:i * maxr + m] = ps.r[i][:m]
+        for l in range(min(maxam, np.asarray(ps.psi).shape[1])):
+            o = (l * nsp + i) * maxr
+            psi[o:o + m] = ps.psi[i][l][:m]
+            vion[o:o + m] = ps.vion[i][l][:m]
+    keep = dict(mmax=np.asarray(ps.mmax, np.int32), clog=np.asarray(ps.clog, np.float64), r=r, psi=psi,
+                vion=vion, zv=np.asarray(ps.zv, np.float64), rgauss=np.asarray(ps.rgauss, np.float64),
+                l_max=np.asarray(ps.l_max, np.int32), l_loc=np.asarray(ps.l_loc, np.int32))
+    d = PseudoDesc(nsp=nsp, maxr=maxr, nang=maxam, **{k: v.ctypes.data for k, v in keep.items()})
+    return d, keep
+
+
+def nl_form_factor(ps, omega, tpiba, ngw, xkg, ggk, xk, nlmax=4):
+    """NLFormFactor.cpp:5-133 on the GPU for one k-point: (WNL [nsp][nlmax], PKG_A [nsp][ngw][nlmax])."""
+    d, keep = _pseudo(ps)
+    nsp = len(ps.mmax)
+    wnl = np.zeros(nsp * nlmax)
+    pkg_a = np.zeros(nlmax * ngw * nsp)
+    _check(lib().pwh_nl_form_factor(C.byref(d), float(omega), float(tpiba), 1, np.array([ngw], np.int32), ngw,
+                                    np.ascontiguousarray(xkg[:ngw], np.float64),
+                                    np.ascontiguousarray(np.asarray(ggk).reshape(-1)[:3 * ngw], np.float64),
+                                    np.ascontiguousarray(xk, np.float64), nlmax, wnl, None, pkg_a))
+    del keep
+    return wnl.reshape(nlmax, nsp).T.copy(), pkg_a.reshape(nsp, ngw, nlmax)
+
+
+def form_factor(ps, na, omega, tpiba2, g2):
+    """FormFactor.cpp:5-72 on the GPU: (VPS [nsp][ng], RHOPS [nsp][ng], ESELF)."""
+    d, keep = _pseudo(ps)
+    nsp = len(ps.mmax)
+    ng = len(g2)
+    vps = np.zeros(nsp * ng)
+    rhops = np.zeros(nsp * ng)
+    es = C.c_double()
+    _check(lib().pwh_form_factor(C.byref(d), np.asarray(na, np.int32), float(omega), float(tpiba2), ng,
+                                 np.ascontiguousarray(g2, np.float64), vps, rhops, C.byref(es)))
+    del keep
+    return vps.reshape(ng, nsp).T.copy(), rhops.reshape(ng, nsp).T.copy(), es.value
 
 
 def ortho_qr(X):

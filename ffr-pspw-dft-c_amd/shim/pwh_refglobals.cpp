@@ -28,3 +28,5 @@ double *G = 0, *VPS = 0, *RHOPS = 0;
 zcomplex *SFAC = 0, *FNL = 0;
 double ESR = 0.0, ESELF = 0.0, ETOT = 0.0, EHT = 0.0, EPSEU = 0.0, ENL = 0.0, EXC = 0.0;
 double NEL = 0.0, KERKER_Q0 = 0.0;
+int *MMAX = 0;
+double *CLOG = 0, *R = 0, *PSI = 0, *VION = 0, *ZV = 0, *RGAUSS = 0, TPIBA = 0.0, *GGK = 0, *XK = 0, *PKG = 0;

@@ -18,6 +18,8 @@
 //   NormalizeRho    NormalizeRho.cpp:7-33      -> pwh_normalize_rho
 //   SimpleRhoMix, KerkerRhoMix, KerkerPrec, AndersonRhoMix, PulayRhoMix, BroydenRhoMix
 //                   *RhoMix.cpp                -> pwh_mix_*
+//   NLFormFactor    NLFormFactor.cpp:5-133     -> pwh_nl_form_factor
+//   FormFactor      FormFactor.cpp:5-72        -> pwh_form_factor
 //
 // Link it in place of those .cpp files.  It reads the reference's globals
 // (global_variables.cpp) and keeps one device system per k-point, rebuilt when the
@@ -54,6 +56,9 @@ extern bool USER_MAX_DIAG_ITER, DIAG_VERBOSE;
 extern int MAX_DIAG_ITER, NEMPTY;
 // read and written by EvalEnergy, EvalENL and the mixers (common_pspw.h:70-136)
 extern int NG, *N1, *N2, *N3;
+// read and written by NLFormFactor / FormFactor (common_pspw.h:59-113)
+extern int *MMAX;
+extern double *CLOG, *R, *PSI, *VION, *ZV, *RGAUSS, TPIBA, *GGK, *XK, *PKG;
 extern double *G, *VPS, *RHOPS, ESR, ESELF, ETOT, EHT, EPSEU, ENL, EXC, NEL, KERKER_Q0;
 extern zcomplex *SFAC, *FNL;
 
@@ -562,4 +567,44 @@ void EvalENL() {
     ENL = ENL + e;
     free(fnl);
   }
+}
+
+namespace {
+const int kMaxR = 570;   // MAXR (common_pspw.h:37)
+const int kMaxAm = 3;    // MAXAM (common_pspw.h:35)
+
+pwh_pseudo_desc pseudo_desc() {
+  pwh_pseudo_desc d;
+  memset(&d, 0, sizeof(d));
+  d.nsp = NSP; d.maxr = kMaxR; d.nang = kMaxAm;
+  d.mmax = MMAX; d.clog = CLOG; d.r = R; d.psi = PSI; d.vion = VION;
+  d.zv = ZV; d.rgauss = RGAUSS; d.l_max = L_MAX; d.l_loc = L_LOC;
+  return d;
+}
+}  // namespace
+
+// NLFormFactor.cpp:5-133: WNL, PKG and PKG_A from the radial pseudopotentials
+void NLFormFactor() {
+  for (int is = 0; is < NSP; ++is)
+    for (int l = 1; l <= L_MAX[is]; ++l)
+      if (l != L_LOC[is] && l >= 4) {
+        printf("ERROR: not implemented i_l = %d\n", l);
+        fflush(stdout);
+        abort();
+      }
+  pwh_pseudo_desc d = pseudo_desc();
+  if (pwh_nl_form_factor(&d, OMEGA, TPIBA, NKPT, NGW, NGWX, XKG, GGK, XK, NLMAX, WNL, PKG, PKG_A) != PWH_OK)
+    die("NLFormFactor");
+}
+
+// FormFactor.cpp:5-72: ESELF, RHOPS, VPS
+void FormFactor() {
+  for (int is = 0; is < NSP; ++is)
+    if (L_LOC[is] < 1) {
+      printf("ERROR in formf: ll < 1: %d\n", L_LOC[is]);
+      fflush(stdout);
+      abort();
+    }
+  pwh_pseudo_desc d = pseudo_desc();
+  if (pwh_form_factor(&d, NA, OMEGA, TPIBA2, NG, G, VPS, RHOPS, &ESELF) != PWH_OK) die("FormFactor");
 }

@@ -63,6 +63,39 @@ int pwh_phase_factor(const double A[9], const int nr[3], int nsp, const int *na,
                      const double *tau, int nkpt, const int *ngw, int ngwx, const int *igk,
                      int ngx, const int *in123, double *eigr /* complex */);
 
+/* ---- set-up on the GPU: phases, structure factor, form factors (FP64) ------------ */
+/* PhaseFactor.cpp:5-80 computed on the GPU: same arguments and output as
+ * pwh_phase_factor (agrees to rounding: the device sin/cos differ from libm by an ulp). */
+int pwh_phase_factor_gpu(const double A[9], const int nr[3], int nsp, const int *na, int nax,
+                         const double *tau, int nkpt, const int *ngw, int ngwx, const int *igk,
+                         int ngx, const int *in123, double *eigr /* complex */);
+/* StructureFactor.cpp:5-29 on the GPU: sfac [nsp*ng] complex (species fastest) for the
+ * first ng density G-vectors of in123 [3*ngx] (IN1, IN2, IN3). */
+int pwh_structure_factor(const double A[9], const int nr[3], int nsp, const int *na, int nax,
+                         const double *tau, int ng, int ngx, const int *in123, double *sfac /* complex */);
+/* The radial pseudopotential data ReadPseudopotentials.cpp:26-63 fills. */
+typedef struct {
+  int nsp;                    /* NSP */
+  int maxr, nang;             /* leading dimension of r (MAXR) and channel count of psi/vion (MAXAM) */
+  const int *mmax;            /* MMAX [nsp], odd */
+  const double *clog;         /* CLOG [nsp]: log of the mesh ratio */
+  const double *r;            /* R [maxr*nsp] */
+  const double *psi, *vion;   /* PSI, VION [maxr*nsp*nang] */
+  const double *zv, *rgauss;  /* ZV, RGAUSS [nsp] */
+  const int *l_max, *l_loc;   /* L_MAX, L_LOC [nsp] */
+} pwh_pseudo_desc;
+/* NLFormFactor.cpp:5-133 on the GPU: wnl [nsp*nlmax] (WNL(is, m), species fastest),
+ * pkg_a [nlmax*ngwx*nsp*nkpt] (PKG_A(m, ig, is, ik)), pkg [ngwx*nsp*nkpt*nlmax] (PKG(ig,
+ * is, ik, m); may be null).  xkg [ngwx*nkpt], ggk [3*ngwx*nkpt], xk [3*nkpt].  A channel
+ * above l = 3 -> PWH_ERR_UNSUPPORTED (the reference aborts). */
+int pwh_nl_form_factor(const pwh_pseudo_desc *ps, double omega, double tpiba, int nkpt, const int *ngw,
+                       int ngwx, const double *xkg, const double *ggk, const double *xk, int nlmax,
+                       double *wnl, double *pkg, double *pkg_a);
+/* FormFactor.cpp:5-72 on the GPU: vps, rhops [nsp*ng] (species fastest; RHOPS at G = 0 is
+ * left as it was), eself.  g [ng] = |G|^2 in (2pi/a)^2.  L_LOC < 1 -> PWH_ERR_ARG. */
+int pwh_form_factor(const pwh_pseudo_desc *ps, const int *na, double omega, double tpiba2, int ng,
+                    const double *g, double *vps, double *rhops, double *eself);
+
 /* ---- the device-resident system (one k-point) ------------------------------ */
 typedef struct {
   int nr1, nr2, nr3;          /* NR1..NR3 */

@@ -774,6 +774,156 @@ def eval_enl(g: Globals, c0, ngw, focc, wkpt, ik=1):
     return enl, fnl
 
 
+# --------------------------------------------------------------------------
+# Form factors and structure factor (set-up of the projectors and local potential)
+# --------------------------------------------------------------------------
+
+def simpson(f, h):
+    """simpson.cpp:5-17 with dsum.cpp (left-to-right sums): h/3 (4 sum f_odd + 2 sum
+    f_even-interior + f_0 + f_(N-1)) for odd N."""
+    f = np.asarray(f, np.float64)
+    n = f.size
+    if n % 2 != 1:
+        raise RuntimeError("simpson: N must be odd")
+    m = (n - 1) // 2
+    s = 4.0 * _seqsum(f[1:2 * m:2]) + 2.0 * _seqsum(f[2:2 * m - 1:2]) + f[0] + f[n - 1]
+    return s * h / 3.0
+
+
+@dataclass
+class Pseudo:
+    """Radial pseudopotential data per species as ReadPseudopotentials.cpp:26-63 holds it:
+    mmax [nsp], clog = log(mesh ratio) [nsp], r [nsp][mmax], psi and vion [nsp][nang][mmax],
+    zv, rgauss [nsp], l_max, l_loc [nsp]."""
+    mmax: np.ndarray
+    clog: np.ndarray
+    r: np.ndarray
+    psi: np.ndarray
+    vion: np.ndarray
+    zv: np.ndarray
+    rgauss: np.ndarray
+    l_max: np.ndarray
+    l_loc: np.ndarray
+
+
+def _bessel_fac(l, x):
+    if l == 1:
+        return np.sin(x) / x
+    if l == 2:
+        return (np.sin(x) / x - np.cos(x)) / x
+    return ((3.0 / (x * x) - 1.0) * np.sin(x) - 3.0 / x * np.cos(x)) / x
+
+
+def nl_form_factor(ps: Pseudo, omega, tpiba, ngw, xkg, ggk, xk, nlmax=4):
+    """NLFormFactor.cpp:5-133 for one k-point: WNL [nsp][nlmax] (WNL(is, m)) and PKG_A
+    [nsp][ngw][nlmax] (PKG_A(m, ig, is)); channels l = 1..L_MAX except L_LOC, m = the
+    2l - 1 components in the reference's order (s; p_x, p_y, p_z; d_z2, d_zx, d_zy, d_xy,
+    d_x2-y2).  xkg [ngw], ggk [ngw][3], xk [3]."""
+    nsp = len(ps.mmax)
+    fpibo = 4.0 * np.pi / omega
+    wnl = np.zeros((nsp, nlmax))
+    pkg = np.zeros((nsp, ngw, nlmax))
+    xkg = np.asarray(xkg, np.float64)[:ngw]
+    ggk = np.asarray(ggk, np.float64).reshape(-1, 3)[:ngw]
+    for i in range(nsp):
+        m = int(ps.mmax[i])
+        h = float(ps.clog[i])
+        r = ps.r[i][:m]
+        vref = ps.vion[i][int(ps.l_loc[i]) - 1][:m]
+        ml = 0
+        for l in range(1, int(ps.l_max[i]) + 1):
+            if l == int(ps.l_loc[i]):
+                continue
+            if l >= 4:
+                raise RuntimeError(f"not implemented i_l = {l}")
+            dv = ps.vion[i][l - 1][:m] - vref
+            fun = dv * ps.psi[i][l - 1][:m] * r * r
+            q = simpson(dv * ps.psi[i][l - 1][:m] ** 2 * r, h)
+            wnl[i, ml:ml + 2 * l - 1] = (2.0 * l - 1.0) * fpibo / q
+            for ig in range(ngw):
+                t = np.sqrt(xkg[ig])
+                arg = t * tpiba
+                small = t < 1.0e-4
+                if l == 1:
+                    q1 = simpson(fun if small else _bessel_fac(1, arg * r) * fun, h)
+                    pkg[i, ig, ml] = q1
+                    continue
+                if small:
+                    pkg[i, ig, ml:ml + 2 * l - 1] = 0.0
+                    continue
+                q1 = simpson(_bessel_fac(l, arg * r) * fun, h)
+                c = (ggk[ig] + np.asarray(xk, np.float64)) / t
+                if l == 2:
+                    pkg[i, ig, ml:ml + 3] = q1 * c
+                else:
+                    cx_, cy_, cz_ = c
+                    s3 = np.sqrt(3.0)
+                    pkg[i, ig, ml:ml + 5] = [q1 * 0.5 * (3.0 * cz_ * cz_ - 1.0), q1 * s3 * cz_ * cx_,
+                                             q1 * s3 * cz_ * cy_, q1 * s3 * cx_ * cy_,
+                                             q1 * s3 / 2.0 * (cx_ * cx_ - cy_ * cy_)]
+            ml += 2 * l - 1
+    return wnl, pkg
+
+
+def form_factor(ps: Pseudo, na, omega, tpiba2, g2):
+    """FormFactor.cpp:5-72: VPS [nsp][ng] (the local pseudopotential plus the field of
+    the Gaussian pseudo-charges), RHOPS [nsp][ng] (G = 0 left 0), ESELF.  g2 = G [ng]
+    sorted by shells (the transform is computed once per distinct |G|^2)."""
+    from math import erf
+    nsp = len(ps.mmax)
+    tpiba = np.sqrt(tpiba2)
+    fpibom = 4.0 * np.pi / omega
+    g2 = np.asarray(g2, np.float64)
+    ng = g2.size
+    eself = 0.0
+    for i in range(nsp):
+        eself = eself + ps.zv[i] * ps.zv[i] / ps.rgauss[i] * na[i]
+    eself = eself * (1.0 / np.sqrt(2.0 * np.pi))
+    vps = np.zeros((nsp, ng))
+    rhops = np.zeros((nsp, ng))
+    for i in range(nsp):
+        facc = 0.25 * tpiba2 * ps.rgauss[i] ** 2
+        rhops[i, 1:] = -ps.zv[i] / omega * np.exp(-facc * g2[1:])
+        m = int(ps.mmax[i])
+        r = ps.r[i][:m]
+        ll = int(ps.l_loc[i])
+        if ll < 1:
+            raise RuntimeError("ll < 1")
+        vv = -ps.zv[i] / r * np.array([erf(x) for x in r / ps.rgauss[i]])
+        vloc = ps.vion[i][ll - 1][:m] - vv
+        h = float(ps.clog[i])
+        vps[i, 0] = simpson(vloc * r ** 3, h) * fpibom
+        g_old, q1 = -1.0, 0.0
+        for ig in range(1, ng):
+            if g2[ig] != g_old:
+                g_old = g2[ig]
+                arg = r * np.sqrt(g2[ig]) * tpiba
+                q1 = simpson(vloc * np.sin(arg) / arg * r ** 3, h)
+            vps[i, ig] = q1 * fpibom
+    return vps, rhops, eself
+
+
+def structure_factor(A, nr, na, tau, in123, ng, ngx):
+    """StructureFactor.cpp:5-29 with PhaseFactor.cpp:32-61's tables: SFAC [nsp][ng] =
+    sum over atoms of EI1 EI2 EI3 for the density G-vectors (IN1..IN3 = in123)."""
+    A = np.asarray(A, np.float64).reshape(3, 3)
+    xmat = np.stack([A[0], A[1], A[2]], axis=1)          # columns A1, A2, A3
+    xmati = np.linalg.inv(xmat)
+    nsp = len(na)
+    tau = np.asarray(tau, np.float64).reshape(nsp, -1, 3)
+    in1, in2, in3 = (np.asarray(in123[k * ngx:k * ngx + ng], np.int64) for k in range(3))
+    sfac = np.zeros((nsp, ng), np.complex128)
+    for i in range(nsp):
+        for a in range(int(na[i])):
+            tp = xmati @ tau[i, a]
+            ar = -2.0 * np.pi * tp
+            e1 = np.cos(in1 * ar[0]) + 1j * np.sin(in1 * ar[0])
+            e2 = np.cos(in2 * ar[1]) + 1j * np.sin(in2 * ar[1])
+            e3 = np.cos(in3 * ar[2]) + 1j * np.sin(in3 * ar[2])
+            sfac[i] = sfac[i] + e1 * e2 * e3
+    return sfac
+
+
 class Reference:
     """ctypes wrapper of oracle/_ref/libref.so (the reference's own compiled setup code)."""
 
@@ -790,6 +940,11 @@ class Reference:
         L.ref_eig_zhegv.argtypes = [_zp, C.c_int, _zp, C.c_int, _zp, C.c_int, C.c_int]
         L.ref_eig_zhegv_eval.argtypes = [_zp, C.c_int, _zp, C.c_int, _dp, _zp, C.c_int, C.c_int]
         L.ref_random_col.argtypes = [_zp, C.c_int, C.c_long]
+        L.ref_nl_form_factor.argtypes = [C.c_int, _ip, _dp, _dp, _dp, _dp, _ip, _ip, C.c_double, C.c_double,
+                                         C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp]
+        L.ref_form_factor.argtypes = [C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _ip, _ip, C.c_double, C.c_double,
+                                      C.c_int, _dp, _dp, _dp, C.POINTER(C.c_double)]
+        L.ref_structure_factor.argtypes = [_zp]
         L.ref_xc_pz.argtypes = [_dp, _dp, C.POINTER(C.c_double), C.c_int]
         L.ref_simple_mix.argtypes = [C.c_double, _dp, _dp, C.c_int, C.POINTER(C.c_double)]
         L.ref_anderson_mix.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp, _dp, C.c_int,
@@ -909,3 +1064,52 @@ class Reference:
                             np.ascontiguousarray(g.wnl, np.float64), fnl, C.byref(enl))
         # FNL(ist, ik, is, ia, lm) with ik = 1: ist fastest, then is, ia, lm
         return enl.value, fnl.reshape(g.nlmax, g.nax, nsp, nst).transpose(2, 1, 0, 3)
+
+    @staticmethod
+    def _radial(ps, maxr=570, maxam=3):
+        nsp = len(ps.mmax)
+        r = np.zeros(maxr * nsp)
+        psi = np.zeros(maxr * nsp * maxam)
+        vion = np.zeros(maxr * nsp * maxam)
+        for i in range(nsp):
+            m = int(ps.mmax[i])
+            r[i * maxr:i * maxr + m] = ps.r[i][:m]
+            for l in range(min(maxam, ps.psi.shape[1])):
+                o = (l * nsp + i) * maxr
+                psi[o:o + m] = ps.psi[i][l][:m]
+                vion[o:o + m] = ps.vion[i][l][:m]
+        return r, psi, vion
+
+    def nl_form_factor(self, ps, omega, tpiba, ngw, xkg, ggk, xk, nlmax=4):
+        """Returns (WNL [nsp][nlmax], PKG_A [nsp][ngw][nlmax]) of the reference's NLFormFactor."""
+        nsp = len(ps.mmax)
+        r, psi, vion = self._radial(ps)
+        ngwx = ngw
+        wnl = np.zeros(nsp * nlmax)
+        pkg = np.zeros(ngwx * nsp * nlmax)
+        pkg_a = np.zeros(nlmax * ngwx * nsp)
+        self.L.ref_nl_form_factor(nsp, np.asarray(ps.mmax, np.int32), np.asarray(ps.clog, np.float64), r, psi,
+                                  vion, np.asarray(ps.l_max, np.int32), np.asarray(ps.l_loc, np.int32), omega,
+                                  tpiba, ngw, ngwx, np.ascontiguousarray(xkg[:ngw], np.float64),
+                                  np.ascontiguousarray(np.asarray(ggk).reshape(-1)[:3 * ngw], np.float64),
+                                  np.ascontiguousarray(xk, np.float64), nlmax, wnl, pkg, pkg_a)
+        return wnl.reshape(nlmax, nsp).T.copy(), pkg_a.reshape(nsp, ngwx, nlmax)
+
+    def form_factor(self, ps, na, omega, tpiba2, g2):
+        nsp = len(ps.mmax)
+        r, psi, vion = self._radial(ps)
+        ng = len(g2)
+        vps = np.zeros(nsp * ng)
+        rhops = np.zeros(nsp * ng)
+        es = C.c_double()
+        self.L.ref_form_factor(nsp, np.asarray(ps.mmax, np.int32), np.asarray(ps.clog, np.float64), r, vion,
+                               np.asarray(ps.zv, np.float64), np.asarray(ps.rgauss, np.float64),
+                               np.asarray(na, np.int32), np.asarray(ps.l_loc, np.int32), omega, tpiba2, ng,
+                               np.ascontiguousarray(g2, np.float64), vps, rhops, C.byref(es))
+        return vps.reshape(ng, nsp).T.copy(), rhops.reshape(ng, nsp).T.copy(), es.value
+
+    def structure_factor(self, nsp, ng):
+        """SFAC [nsp][ng] of the reference's StructureFactor on the set-up of setup()."""
+        sf = np.zeros(nsp * ng, np.complex128)
+        self.L.ref_structure_factor(sf)
+        return sf.reshape(ng, nsp).T.copy()

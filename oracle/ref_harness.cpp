@@ -167,6 +167,53 @@ void ref_eval_enl(int nsp, const int *na, int nax, const int *l_max, const int *
   WNL = s_wnl; EIGR = s_eigr; C0 = s_c0; FNL = s_fnl;
 }
 
+// --- form factors and structure factor (NLFormFactor.cpp, FormFactor.cpp,
+// StructureFactor.cpp, with simpson.cpp and dsum.cpp) ---
+// The radial arrays are [MAXR x nsp (x MAXAM)] as ReadPseudopotentials.cpp fills them.
+void ref_nl_form_factor(int nsp, int *mmax, double *clog, double *r, double *psi, double *vion, int *l_max,
+                        int *l_loc, double omega, double tpiba, int ngw, int ngwx, double *xkg, double *ggk,
+                        double *xk, int nlmax, double *wnl, double *pkg, double *pkg_a) {
+  int ngw_[1] = {ngw};
+  const int s_nsp = NSP, s_nkpt = NKPT, s_ngwx = NGWX, s_nlmax = NLMAX;
+  const double s_omega = OMEGA, s_tpiba = TPIBA;
+  int *s_mmax = MMAX, *s_lmax = L_MAX, *s_lloc = L_LOC, *s_ngw = NGW;
+  double *s_clog = CLOG, *s_r = R, *s_psi = PSI, *s_vion = VION, *s_xkg = XKG, *s_ggk = GGK, *s_xk = XK;
+  double *s_wnl = WNL, *s_pkg = PKG, *s_pkga = PKG_A;
+  NSP = nsp; NKPT = 1; NGWX = ngwx; NLMAX = nlmax; OMEGA = omega; TPIBA = tpiba;
+  MMAX = mmax; L_MAX = l_max; L_LOC = l_loc; NGW = ngw_; CLOG = clog; R = r; PSI = psi; VION = vion;
+  XKG = xkg; GGK = ggk; XK = xk; WNL = wnl; PKG = pkg; PKG_A = pkg_a;
+  NLFormFactor();
+  NSP = s_nsp; NKPT = s_nkpt; NGWX = s_ngwx; NLMAX = s_nlmax; OMEGA = s_omega; TPIBA = s_tpiba;
+  MMAX = s_mmax; L_MAX = s_lmax; L_LOC = s_lloc; NGW = s_ngw; CLOG = s_clog; R = s_r; PSI = s_psi;
+  VION = s_vion; XKG = s_xkg; GGK = s_ggk; XK = s_xk; WNL = s_wnl; PKG = s_pkg; PKG_A = s_pkga;
+}
+
+void ref_form_factor(int nsp, int *mmax, double *clog, double *r, double *vion, double *zv, double *rgauss,
+                     int *na, int *l_loc, double omega, double tpiba2, int ng, double *g, double *vps,
+                     double *rhops, double *eself) {
+  const int s_nsp = NSP, s_ng = NG;
+  const double s_omega = OMEGA, s_tpiba2 = TPIBA2, s_tpiba = TPIBA, s_eself = ESELF;
+  int *s_mmax = MMAX, *s_na = NA, *s_lloc = L_LOC;
+  double *s_clog = CLOG, *s_r = R, *s_vion = VION, *s_zv = ZV, *s_rg = RGAUSS, *s_g = G, *s_vps = VPS,
+         *s_rhops = RHOPS;
+  NSP = nsp; NG = ng; OMEGA = omega; TPIBA2 = tpiba2; TPIBA = sqrt(tpiba2);
+  MMAX = mmax; NA = na; L_LOC = l_loc; CLOG = clog; R = r; VION = vion; ZV = zv; RGAUSS = rgauss; G = g;
+  VPS = vps; RHOPS = rhops;
+  FormFactor();
+  *eself = ESELF;
+  NSP = s_nsp; NG = s_ng; OMEGA = s_omega; TPIBA2 = s_tpiba2; TPIBA = s_tpiba; ESELF = s_eself;
+  MMAX = s_mmax; NA = s_na; L_LOC = s_lloc; CLOG = s_clog; R = s_r; VION = s_vion; ZV = s_zv; RGAUSS = s_rg;
+  G = s_g; VPS = s_vps; RHOPS = s_rhops;
+}
+
+// StructureFactor.cpp:5-29 on the tables ref_setup's PhaseFactor filled; sfac [nsp*NG]
+void ref_structure_factor(double complex *sfac) {
+  double complex *s_sfac = SFAC;
+  SFAC = sfac;
+  StructureFactor();
+  SFAC = s_sfac;
+}
+
 void ref_free() { free_all(); }
 
 }  // extern "C"

@@ -24,7 +24,7 @@ REF_SYMBOLS = ["_Z14ApplyHam_blockPCdS0_iiPdii", "_Z8ApplyHamPCdS0_iiPdi",
                "_Z10lobpcg_cpuiPdPCdiiS_iS_", "_Z10EvalEnergyPd", "_Z5xc_PZPdS_Rdi", "_Z7EvalENLv",
                "_Z12NormalizeRhoPdidd", "_Z12SimpleRhoMixdPdS_iRd", "_Z12KerkerRhoMixddPdS_iRd",
                "_Z10KerkerPrecPdiidd", "_Z14AndersonRhoMixidPdS_S_S_iRd", "_Z11PulayRhoMixiPdS_S_iiRd",
-               "_Z13BroydenRhoMixiddPdS_S_S_S_S_iiRd"]
+               "_Z13BroydenRhoMixiddPdS_S_S_S_S_iiRd", "_Z12NLFormFactorv", "_Z10FormFactorv"]
 
 
 def declared_functions():
