@@ -331,6 +331,47 @@ def scf_steps(s, sysm, psi, steps, cpu):
     return out
 
 
+def setup_steps(s, steps, cpu):
+    """The per-G set-up (NLFormFactor.cpp, FormFactor.cpp, PhaseFactor.cpp) for the cell with
+    the silicon pseudopotential of the reference's example input (tests/golden/si_fhi.npz):
+    GPU times per call (host arrays in and out) and the oracle beside them."""
+    import pwh
+    from pwh import synth
+    from oracle import oracle as O
+    d = np.load(os.path.join(ROOT, "tests", "golden", "si_fhi.npz"))
+    ps = O.Pseudo(mmax=np.array([int(d["mmax"][0])], np.int32), clog=np.array([np.log(float(d["ratio"][0]))]),
+                  r=d["r"][:1].copy(), psi=d["psi"][None, :3], vion=d["vion"][None, :3], zv=np.array([4.0]),
+                  rgauss=np.array([1.0]), l_max=np.array([3], np.int32), l_loc=np.array([3], np.int32))
+    ngw = s.ngw
+    xkg, ggk = s.gv["xkg"][:ngw], s.gv["ggk"][:3 * ngw]
+    omega, tpiba = s.a ** 3, 2 * np.pi / s.a
+    g2 = synth.density_gspace(s)["g2"]
+    nat = s.tau.reshape(-1, 3).shape[0]
+
+    def per_call(fn):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        return 1e3 * (time.perf_counter() - t0) / steps
+
+    out = {"plane_waves": int(ngw), "density_gvectors": int(len(g2)), "atoms": int(nat)}
+    out["nl_form_factor_ms"] = round(per_call(lambda: pwh.nl_form_factor(ps, omega, tpiba, ngw, xkg, ggk,
+                                                                          np.zeros(3))), 3)
+    out["form_factor_ms"] = round(per_call(lambda: pwh.form_factor(ps, [nat], omega, tpiba ** 2, g2)), 3)
+    tau = s.tau.reshape(1, nat, 3)
+    out["phase_factor_gpu_ms"] = round(per_call(lambda: pwh.phase_factor_gpu(s.A, s.nr, [nat], nat, tau, s.gv)), 3)
+    out["phase_factor_host_ms"] = round(per_call(lambda: pwh.phase_factor(s.A, s.nr, [nat], nat, tau, s.gv)), 3)
+    if cpu:
+        t0 = time.perf_counter()
+        O.nl_form_factor(ps, omega, tpiba, ngw, xkg, ggk, np.zeros(3))
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(1e3 * dt, 1), "unit": "ms per NLFormFactor", "cores": 1, "kind": "port",
+                               "sample": f"the oracle's NLFormFactor (numpy per G, {ngw} plane waves, s and p "
+                                         f"channels of the silicon pseudopotential)"}
+    return out
+
+
 def c3_batch(steps, degree=10):
     """24 uneven cells filtered together (one captured graph) vs one at a time."""
     import pwh
@@ -535,6 +576,11 @@ def main():
         scf = scf_steps(s, sysm, psi, steps=max(5, min(args.steps, 20)), cpu=not args.no_cpu)
         log("bench: scf steps " + json.dumps(scf))
 
+    setup = None
+    if not args.only:
+        setup = setup_steps(s, steps=5, cpu=not args.no_cpu)
+        log("bench: setup " + json.dumps(setup))
+
     solvers = None
     if not args.only and not args.no_chfsi:
         solvers = block_solvers(s, sysm, iters=10, cpu=not args.no_cpu)
@@ -579,6 +625,7 @@ def main():
         "eval_rho_psi": rho,
         "block_solvers": solvers,
         "scf_steps": scf,
+        "setup_steps": setup,
         "c3_batched_uneven": extra.get("c3_batched_uneven"),
         "c4_si216": extra.get("c4_si216"),
         "cpu_baseline": cpu,
