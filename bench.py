@@ -166,13 +166,17 @@ def measured_traffic(kernel, path):
     return None
 
 
-def rho_flops(s, nxa, nb):
-    """FP64 FFT arithmetic of the density plane pass (k_plane_rho) over nb states: per
-    plane, nxa inverse transforms along i2 (the sphere's i1 columns), nr2 inverse transforms
-    along i1 with the out-of-sphere inputs known zero, and |v|^2 w accumulation (4 flops)."""
+def rho_flops(s, nxa, nb, band=True):
+    """FP64 FFT arithmetic of the density plane pass over nb states, per plane: with band
+    pruning (k_plane_rho_band) n/2 inverse transforms along i2 and n along i1, each with
+    the first stage's out-of-band inputs zero; without (k_plane_rho) nxa full transforms
+    along i2 and n along i1; then |v|^2 w accumulation (4 flops per point)."""
     n1, n2, n3 = s.nr
-    per_plane = nxa * _transform_flops(n2, None, False) + n2 * _transform_flops(n1, None, False) + 4 * n1 * n2
-    return per_plane * n3 * nb
+    if band:
+        per_plane = (n1 // 2) * _transform_flops(n2, "in", False) + n2 * _transform_flops(n1, "in", False)
+    else:
+        per_plane = nxa * _transform_flops(n2, None, False) + n2 * _transform_flops(n1, None, False)
+    return (per_plane + 4 * n1 * n2) * n3 * nb
 
 
 def rho_section(s, sysm, psi, info, steps, cpu):

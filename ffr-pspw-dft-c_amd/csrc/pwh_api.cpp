@@ -269,8 +269,9 @@ int hpsi_prepare(pwh_system* s, int nb) {
     if (s->na[sp] == 0 || s->lm[sp] == 0) continue;
     const int M = s->na[sp] * s->lm[sp];
     CHK(Sel<T>::Hnl(s).ensure((size_t)nb * ld));
-    CHK(Sel<T>::proj(s).ensure((size_t)M * nb));
-    CHK(Sel<T>::Q(s).ensure((size_t)M * nb));
+    const size_t nproj = (size_t)s->poff.back() + (size_t)s->na.back() * s->lm.back();
+    CHK(Sel<T>::proj(s).ensure(nproj * nb));
+    CHK(Sel<T>::Q(s).ensure(nproj * nb));
     CHK(Sel<T>::part(s).ensure(proj_part_elems(M, nb, s->ngw)));
   }
   return PWH_OK;
@@ -286,7 +287,8 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
   CHK(Sel<T>::W(s).ensure((size_t)nb * s->nr[2] * s->maps.ncol_pad));
   bool any_nl = false;
   for (int sp = 0; sp < s->nsp; ++sp) any_nl = any_nl || (s->na[sp] > 0 && s->lm[sp] > 0);
-  // the nonlocal GEMMs run on a second stream, overlapping the FFT passes
+  // the nonlocal GEMMs run on a second stream, overlapping the FFT passes (running the
+  // projections first and overlapping only the expansions measured no better)
   hipStream_t sn = st;
   const bool overlap = any_nl && overlap_enabled(s, nb, st);
   if (overlap) {
@@ -295,25 +297,34 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
     HIPCHK(hipStreamWaitEvent(s->nl_stream, s->ev_fork, 0));
     sn = s->nl_stream;
   }
-  bool have_nl = false;
   for (int sp = 0; sp < s->nsp; ++sp) {
     if (s->na[sp] == 0 || s->lm[sp] == 0) continue;
     const int M = s->na[sp] * s->lm[sp];
     CHK(Sel<T>::Hnl(s).ensure((size_t)nb * ld));
-    CHK(Sel<T>::proj(s).ensure((size_t)M * nb));
-    CHK(Sel<T>::Q(s).ensure((size_t)M * nb));
+    const size_t nproj = (size_t)s->poff.back() + (size_t)s->na.back() * s->lm.back();
+    CHK(Sel<T>::proj(s).ensure(nproj * nb));
+    CHK(Sel<T>::Q(s).ensure(nproj * nb));
     const size_t pe = proj_part_elems(M, nb, s->ngw);
     CHK(Sel<T>::part(s).ensure(pe));
     BetaGen bg{Sel<T>::Bt(s) + (size_t)s->poff[sp] * ld, M, s->lm[sp], ld};
+    hipStream_t sp_st = sn;
     {
-      Timed t(TM_PROJ, sn);
-      HIPCHK(gemm_proj<T>(bg, X, ld, nb, s->ngw, Sel<T>::proj(s).p, Sel<T>::part(s).p, pe, mode, sn));
+      Timed t(TM_PROJ, sp_st);
+      HIPCHK(gemm_proj<T>(bg, X, ld, nb, s->ngw, Sel<T>::proj(s).p + (size_t)s->poff[sp] * nb,
+                          Sel<T>::part(s).p, pe, mode, sp_st));
     }
-    HIPCHK(scale_proj<T>(Sel<T>::proj(s).p, Sel<T>::wl(s) + 4 * sp, s->na[sp], s->lm[sp], nb,
-                         Sel<T>::Q(s).p, sn));
+    HIPCHK(scale_proj<T>(Sel<T>::proj(s).p + (size_t)s->poff[sp] * nb, Sel<T>::wl(s) + 4 * sp, s->na[sp],
+                         s->lm[sp], nb, Sel<T>::Q(s).p + (size_t)s->poff[sp] * nb, sp_st));
+  }
+  bool have_nl = false;
+  for (int sp = 0; sp < s->nsp; ++sp) {
+    if (s->na[sp] == 0 || s->lm[sp] == 0) continue;
+    const int M = s->na[sp] * s->lm[sp];
+    BetaGen bg{Sel<T>::Bt(s) + (size_t)s->poff[sp] * ld, M, s->lm[sp], ld};
     {
       Timed t(TM_EXPAND, sn);
-      HIPCHK(gemm_expand<T>(bg, Sel<T>::Q(s).p, nb, s->ngw, Sel<T>::Hnl(s).p, ld, have_nl, mode, sn));
+      HIPCHK(gemm_expand<T>(bg, Sel<T>::Q(s).p + (size_t)s->poff[sp] * nb, nb, s->ngw, Sel<T>::Hnl(s).p, ld,
+                            have_nl, mode, sn));
     }
     have_nl = true;
   }

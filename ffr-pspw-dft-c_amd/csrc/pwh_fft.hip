@@ -602,6 +602,64 @@ __global__ __launch_bounds__(kRhoThreads) void k_plane_rho(SphereMaps m, const c
   }
 }
 
+// The same with band pruning (the sphere's i1, i2 in [-n/4, n/4)): inverse along i2 on
+// the n/2 band columns and along i1 on every row, both with the first stage's out-of-band
+// inputs known zero; w_b |psi_b|^2 is added in registers at the last row stage.
+template <int N, int NT>
+__global__ __launch_bounds__(NT) void k_plane_rho_band(SphereMaps m, const cx<double>* __restrict__ twg,
+                                                       const cx<double>* __restrict__ W, int nb, int groups,
+                                                       const double* __restrict__ wts,
+                                                       double* __restrict__ part) {
+  constexpr int S1 = N + 1, Q = N / 4, NBC = N / 2;
+  static_assert(N % 4 == 0, "band pruning needs n % 4 == 0");
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<double>* tile = reinterpret_cast<cx<double>*>(smem);
+  cx<double>* tw = tile + N * S1;
+  const int tid = threadIdx.x;
+  const int z = blockIdx.x, g0 = blockIdx.y;
+  // outputs of the last inverse row stage: lane slot q, element r -> row t, column j + r NSL
+  constexpr int RL = radix_at(N, nstages(N) - 1, false), NSL = N / RL;
+  constexpr int TOTL = N * NSL, MBL = (TOTL + NT - 1) / NT;
+  double acc[MBL][RL];
+#pragma unroll
+  for (int q = 0; q < MBL; ++q)
+#pragma unroll
+    for (int r = 0; r < RL; ++r) acc[q][r] = 0.0;
+  auto band_x = [](int t) { return t < Q ? t : t + NBC; };
+  for (int i = tid; i < N; i += NT) tw[i] = twg[i];
+  auto col_base = [&](int t) { return band_x(t); };
+  auto row_base = [](int t) { return t * S1; };
+  const NoPost none;
+  for (int b = g0; b < nb; b += groups) {
+    __syncthreads();
+    for (int i = tid; i < NBC * NBC; i += NT) {
+      const int yy = i / NBC, xx = i - yy * NBC;
+      tile[band_x(yy) * S1 + band_x(xx)] = mk<double>(0, 0);
+    }
+    __syncthreads();
+    const cx<double>* wrow = W + ((size_t)b * m.nr3 + z) * m.ncol_pad;
+    for (int c = tid; c < m.ncol; c += NT) {
+      const int xy = m.colxy[c];
+      tile[(xy / N) * S1 + xy % N] = wrow[c];
+    }
+    __syncthreads();
+    const double wb = wts[b];
+    auto accum = [&](int q, int r, cx<double>& v) { acc[q][r] += wb * (v.x * v.x + v.y * v.y); };
+    band_fft<double, N, +1, false, NT, NBC, S1, true, false>(tile, col_base, tw, twg, tid, none);
+    band_fft<double, N, +1, false, NT, N, 1, true, false>(tile, row_base, tw, twg, tid, accum);
+  }
+  double* out = part + ((size_t)g0 * m.nr3 + z) * (size_t)(N * N);
+#pragma unroll
+  for (int q = 0; q < MBL; ++q) {
+    const int idx = tid + q * NT;
+    if (TOTL % NT == 0 || idx < TOTL) {
+      const int j = idx / N, t = idx - j * N;
+#pragma unroll
+      for (int r = 0; r < RL; ++r) out[t * N + j + r * NSL] = acc[q][r];
+    }
+  }
+}
+
 // rho[i] = sum_g part[g][i], groups summed in order (deterministic)
 __global__ void k_sum_groups(const double* __restrict__ part, int groups, size_t n, double* __restrict__ rho) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1005,6 +1063,26 @@ hipError_t launch_zfwd_epilogue(const SphereMaps& m, const void* tw3, const cx<T
 template <int N>
 static hipError_t launch_plane_rho_n(const SphereMaps& m, const void* tw1, const void* tw2, const cx<double>* W,
                                      int nb, int groups, const double* wts, double* part, hipStream_t s) {
+  static const bool band_on = [] {
+    const char* e = getenv("PWH_PLANE_BAND");
+    return !(e && e[0] == '0');
+  }();
+  if constexpr (N % 4 == 0 && band_plane_smem<double, N>() <= 160 * 1024) {
+    if (m.band_xy && band_on) {
+      constexpr int NT = band_plane_threads<N>();
+      constexpr size_t bsm = band_plane_smem<double, N>();
+      static bool battr = false;
+      if (!battr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_rho_band<N, NT>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bsm);
+        if (e != hipSuccess) return e;
+        battr = true;
+      }
+      hipLaunchKernelGGL((k_plane_rho_band<N, NT>), dim3(m.nr3, groups), dim3(NT), bsm, s, m,
+                         static_cast<const cx<double>*>(tw1), W, nb, groups, wts, part);
+      return hipGetLastError();
+    }
+  }
   constexpr size_t smem = plane_smem<double, N, N>();
   if constexpr (smem > 160 * 1024) {
     return hipErrorInvalidValue;
