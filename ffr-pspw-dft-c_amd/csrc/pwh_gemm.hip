@@ -234,27 +234,33 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
 }
 
 // C = scale(m) * sum_s part[s][m][n], stored C[m*ldc + n] (row-major) or C[m + n*ldc]
-// (column-major, the LAPACK layout handed to rocSOLVER); scale(m) = wl[m % lm] when wl != null
+// (column-major, the LAPACK layout handed to rocSOLVER); scale(m) = wl[m % lm] when wl != null.
+// One output per thread: eight independent partial sums (split s goes to sum s % 8) keep
+// eight loads in flight, then a fixed tree adds them (deterministic).
 template <typename T, typename TO>
 __global__ __launch_bounds__(256) void k_reduce_parts(const cx<T>* __restrict__ part, int S, int M,
                                                       int N, cx<TO>* __restrict__ C, int ldc,
                                                       const T* __restrict__ wl, int lm, int colmajor) {
-  // 64 outputs per block; the 4 waves sum every 4th split, then combine in LDS
-  __shared__ TO sr[4][64], si[4][64];
-  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
-  const int idx = blockIdx.x * 64 + lane;
-  TO ar = 0, ai = 0;
-  if (idx < M * N) {
-    for (int s = grp; s < S; s += 4) {
-      const cx<T> v = part[(size_t)s * M * N + idx];
-      ar += (TO)v.x; ai += (TO)v.y;
-    }
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  const size_t MN = (size_t)M * N;
+  if (idx >= M * N) return;
+  TO ar[8], ai[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) { ar[j] = 0; ai[j] = 0; }
+  int s = 0;
+  for (; s + 8 <= S; s += 8) {
+    cx<T> v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = part[(size_t)(s + j) * MN + idx];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) { ar[j] += (TO)v[j].x; ai[j] += (TO)v[j].y; }
   }
-  sr[grp][lane] = ar; si[grp][lane] = ai;
-  __syncthreads();
-  if (grp != 0 || idx >= M * N) return;
-  TO tr = (sr[0][lane] + sr[1][lane]) + (sr[2][lane] + sr[3][lane]);
-  TO ti = (si[0][lane] + si[1][lane]) + (si[2][lane] + si[3][lane]);
+  for (int j = 0; s < S; ++s, ++j) {
+    const cx<T> v = part[(size_t)s * MN + idx];
+    ar[j] += (TO)v.x; ai[j] += (TO)v.y;
+  }
+  TO tr = ((ar[0] + ar[1]) + (ar[2] + ar[3])) + ((ar[4] + ar[5]) + (ar[6] + ar[7]));
+  TO ti = ((ai[0] + ai[1]) + (ai[2] + ai[3])) + ((ai[4] + ai[5]) + (ai[6] + ai[7]));
   const int m = idx / N, n = idx - m * N;
   if (wl) { const TO w = (TO)wl[m % lm]; tr *= w; ti *= w; }
   if (colmajor) C[m + (size_t)n * ldc] = mk<TO>(tr, ti);
@@ -514,7 +520,7 @@ hipError_t gemm_proj(const BetaGen& bg, const cx<T>* X, int ldx, int nb, int K, 
   hipError_t e = run_proj<T, false>(ld, X, ldx, M, nb, K, part, part_elems, mode == PREC_BF16X3, s, &S);
   if (e != hipSuccess) return e;
   const int tot = M * nb;
-  hipLaunchKernelGGL((k_reduce_parts<T, T>), dim3((tot + 63) / 64), dim3(256), 0, s, part, S, M,
+  hipLaunchKernelGGL((k_reduce_parts<T, T>), dim3((tot + 255) / 256), dim3(256), 0, s, part, S, M,
                      nb, C, nb, static_cast<const T*>(nullptr), bg.lm, 0);
   return hipGetLastError();
 }
@@ -544,7 +550,7 @@ hipError_t gemm_cn_f64(const cx<double>* A, int lda, int M, const cx<double>* B,
   hipError_t e = run_proj<double, false>(ld, B, ldb, M, N, K, part, part_elems, false, s, &S);
   if (e != hipSuccess) return e;
   const int tot = M * N;
-  hipLaunchKernelGGL((k_reduce_parts<double, double>), dim3((tot + 63) / 64), dim3(256), 0, s, part,
+  hipLaunchKernelGGL((k_reduce_parts<double, double>), dim3((tot + 255) / 256), dim3(256), 0, s, part,
                      S, M, N, C, ldc, static_cast<const double*>(nullptr), 1, 1);
   return hipGetLastError();
 }
