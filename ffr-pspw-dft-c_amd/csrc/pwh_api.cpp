@@ -853,6 +853,13 @@ int pwh_system_create(const pwh_system_desc* d, pwh_system** out) {
     for (int c = 0; band && c < ncol; ++c)
       band = in_band(colxy[c] % nr1, nr1) && in_band(colxy[c] / nr1, nr2);
     s->maps.band_xy = band ? 1 : 0;
+    for (int t = 0; t < 4; ++t) {
+      const int ct = 8 << t;
+      int mx = 0;
+      for (int c0 = 0; c0 < ncol_pad; c0 += ct)
+        mx = std::max(mx, colstart[std::min(c0 + ct, ncol_pad)] - colstart[c0]);
+      s->maps.tile_max[t] = mx;
+    }
   }
   s->maps.colstart = s->d_colstart.p; s->maps.kcol = s->d_kcol.p; s->maps.kz = s->d_kz.p;
   s->maps.colxy = s->d_colxy.p; s->maps.xa = s->d_xa.p;

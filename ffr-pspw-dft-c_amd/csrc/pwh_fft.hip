@@ -211,32 +211,64 @@ constexpr bool smooth235(int n) {
 // ---------------------------------------------------------------------------
 // Pass 1: inverse transform along i3 of the sphere columns
 // ---------------------------------------------------------------------------
-constexpr int kColTile = 16;
+constexpr int kColTile = 16;      // column tile of the dense line transforms (k_fft_lines)
 constexpr int kColThreads = 128;
 constexpr int kPlaneThreads = 512;
 
-template <typename T, int NZ>
-__global__ __launch_bounds__(kColThreads) void k_zinv(SphereMaps m, const cx<T>* __restrict__ tw3g,
-                                                      const cx<T>* __restrict__ X, int ldx,
-                                                      cx<T>* __restrict__ W) {
-  constexpr int CT = kColTile, NT = kColThreads;
+// Column passes: a block transforms CT consecutive sphere columns of one band with NT
+// lanes.  Every global load a block needs is issued before its first barrier: the tile's
+// column range, the twiddles and up to PE sphere entries per lane (k_zinv), or the W tile
+// and the epilogue operands (k_zfwd), so a block waits for one memory latency rather
+// than a chain of them.  Entries beyond PE * NT in a tile (PE is picked from the largest
+// tile) go through a plain loop.  The last tile may be short: cn = min(CT, ncol_pad - c0).
+template <typename T, int NZ, int CT, int NT, int PE>
+__global__ __launch_bounds__(NT) void k_zinv(SphereMaps m, const cx<T>* __restrict__ tw3g,
+                                             const cx<T>* __restrict__ X, int ldx,
+                                             cx<T>* __restrict__ W) {
   __shared__ cx<T> tile[NZ * CT];
   __shared__ cx<T> tw[NZ];
+  constexpr int TWP = (NZ + NT - 1) / NT;
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * CT;
   const int b = blockIdx.y;
-  for (int i = tid; i < NZ; i += NT) tw[i] = tw3g[i];
-  for (int i = tid; i < NZ * CT; i += NT) tile[i] = mk<T>(0, 0);
-  __syncthreads();
-  const int kb = m.colstart[c0], ke = m.colstart[c0 + CT];
+  const int cn = min(CT, m.ncol_pad - c0);
+  const int kb = m.colstart[c0], ke = m.colstart[c0 + cn];
   const cx<T>* xb = X + (size_t)b * ldx;
-  for (int k = kb + tid; k < ke; k += NT) tile[m.kz[k] * CT + (m.kcol[k] - c0)] = xb[k];
+  // Loads use clamped, always valid indices and no branch, so they issue back to back;
+  // out-of-range lanes are discarded afterwards.  Register copies are separate real
+  // arrays (arrays of complex structs end up in scratch).
+  T twr[TWP], twi[TWP];
+#pragma unroll
+  for (int j = 0; j < TWP; ++j) {
+    const cx<T> v = tw3g[min(tid + j * NT, NZ - 1)];
+    twr[j] = v.x; twi[j] = v.y;
+  }
+  const int klast = max(ke - 1, 0);
+  T xr[PE], xi[PE];
+  int kzv[PE], kcv[PE];
+#pragma unroll
+  for (int j = 0; j < PE; ++j) {
+    const int k = min(kb + tid + j * NT, klast);
+    const cx<T> v = xb[k];
+    xr[j] = v.x; xi[j] = v.y;
+    kzv[j] = m.kz[k];
+    kcv[j] = m.kcol[k];
+  }
+  for (int i = tid; i < NZ * CT; i += NT) tile[i] = mk<T>(0, 0);
+#pragma unroll
+  for (int j = 0; j < TWP; ++j)
+    if (tid + j * NT < NZ) tw[tid + j * NT] = mk<T>(twr[j], twi[j]);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < PE; ++j)
+    if (kb + tid + j * NT < ke) tile[kzv[j] * CT + (kcv[j] - c0)] = mk<T>(xr[j], xi[j]);
+  for (int k = kb + PE * NT + tid; k < ke; k += NT) tile[m.kz[k] * CT + (m.kcol[k] - c0)] = xb[k];
   __syncthreads();
   lds_fft<T, NZ, +1, NT, CT, true>(tile, CT, CT, [](int t) { return t; }, tw, tid);
   cx<T>* wb = W + (size_t)b * NZ * m.ncol_pad + c0;
   for (int i = tid; i < NZ * CT; i += NT) {
     const int z = i / CT, c = i - z * CT;
-    wb[(size_t)z * m.ncol_pad + c] = tile[i];
+    if (c < cn) wb[(size_t)z * m.ncol_pad + c] = tile[i];
   }
 }
 
@@ -842,32 +874,87 @@ constexpr bool use_pair_plane() { return (N / 2) * 2 * (int)sizeof(T) <= 128 * 4
 // ---------------------------------------------------------------------------
 // Pass 3: forward transform along i3, gather the sphere, fused epilogue
 // ---------------------------------------------------------------------------
-template <typename T, int NZ>
-__global__ __launch_bounds__(kColThreads) void k_zfwd(SphereMaps m, const cx<T>* __restrict__ tw3g,
-                                                      const cx<T>* __restrict__ W,
-                                                      const T* __restrict__ kin,
-                                                      const cx<T>* __restrict__ X,
-                                                      const cx<T>* __restrict__ Hnl,
-                                                      const cx<T>* __restrict__ Z,
-                                                      cx<T>* __restrict__ out, int ld, T inv_nnr,
-                                                      T alpha, T beta, T gamma) {
-  constexpr int CT = kColTile, NT = kColThreads;
+template <typename T, int NZ, int CT, int NT, int PE>
+__global__ __launch_bounds__(NT) void k_zfwd(SphereMaps m, const cx<T>* __restrict__ tw3g,
+                                             const cx<T>* __restrict__ W,
+                                             const T* __restrict__ kin,
+                                             const cx<T>* __restrict__ X,
+                                             const cx<T>* __restrict__ Hnl,
+                                             const cx<T>* __restrict__ Z,
+                                             cx<T>* __restrict__ out, int ld, T inv_nnr,
+                                             T alpha, T beta, T gamma) {
   __shared__ cx<T> tile[NZ * CT];
   __shared__ cx<T> tw[NZ];
+  constexpr int TWP = (NZ + NT - 1) / NT;
+  constexpr int WP = (NZ * CT + NT - 1) / NT;
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * CT;
   const int b = blockIdx.y;
-  for (int i = tid; i < NZ; i += NT) tw[i] = tw3g[i];
-  const cx<T>* wb = W + (size_t)b * NZ * m.ncol_pad + c0;
-  for (int i = tid; i < NZ * CT; i += NT) {
-    const int z = i / CT, c = i - z * CT;
-    tile[i] = wb[(size_t)z * m.ncol_pad + c];
+  const int cn = min(CT, m.ncol_pad - c0);
+  const int kb = m.colstart[c0], ke = m.colstart[c0 + cn];
+  const size_t ob = (size_t)b * ld;
+  // Loads use clamped, always valid indices and no branch, so they issue back to back;
+  // out-of-range lanes are discarded afterwards.  Register copies are separate real
+  // arrays (arrays of complex structs end up in scratch).
+  T twr[TWP], twi[TWP];
+#pragma unroll
+  for (int j = 0; j < TWP; ++j) {
+    const cx<T> v = tw3g[min(tid + j * NT, NZ - 1)];
+    twr[j] = v.x; twi[j] = v.y;
   }
+  const cx<T>* wb = W + (size_t)b * NZ * m.ncol_pad + c0;
+  T wr[WP], wi[WP];
+#pragma unroll
+  for (int j = 0; j < WP; ++j) {
+    const int i = min(tid + j * NT, NZ * CT - 1);
+    const int z = i / CT, c = min(i - z * CT, cn - 1);
+    const cx<T> v = wb[(size_t)z * m.ncol_pad + c];
+    wr[j] = v.x; wi[j] = v.y;
+  }
+  // epilogue operands, loaded before the transform and combined after it:
+  // out = alpha/nnr * FFT + (alpha * kin + beta) * X + alpha * Hnl + gamma * Z; a null
+  // Hnl or Z reads X again with a zero coefficient
+  const cx<T>* hp = Hnl ? Hnl : X;
+  const cx<T>* zp = Z ? Z : X;
+  const T hc = Hnl ? alpha : T(0), zc = Z ? gamma : T(0);
+  const int klast = max(ke - 1, 0);
+  T xr[PE], xi[PE], hr[PE], hi[PE], zr[PE], zi[PE], kv[PE];
+  int kzv[PE], kcv[PE];
+#pragma unroll
+  for (int j = 0; j < PE; ++j) {
+    const int k = min(kb + tid + j * NT, klast);
+    const cx<T> x = X[ob + k], h = hp[ob + k], zz = zp[ob + k];
+    xr[j] = x.x; xi[j] = x.y;
+    hr[j] = h.x; hi[j] = h.y;
+    zr[j] = zz.x; zi[j] = zz.y;
+    kv[j] = kin[k];
+    kzv[j] = m.kz[k];
+    kcv[j] = m.kcol[k];
+  }
+#pragma unroll
+  for (int j = 0; j < WP; ++j) {
+    const int i = tid + j * NT;
+    if (i < NZ * CT) tile[i] = ((i % CT) < cn) ? mk<T>(wr[j], wi[j]) : mk<T>(0, 0);
+  }
+#pragma unroll
+  for (int j = 0; j < TWP; ++j)
+    if (tid + j * NT < NZ) tw[tid + j * NT] = mk<T>(twr[j], twi[j]);
   __syncthreads();
   lds_fft<T, NZ, -1, NT, CT, true>(tile, CT, CT, [](int t) { return t; }, tw, tid);
-  const int kb = m.colstart[c0], ke = m.colstart[c0 + CT];
-  const size_t ob = (size_t)b * ld;
-  for (int k = kb + tid; k < ke; k += NT) {
+  const T an = alpha * inv_nnr;
+#pragma unroll
+  for (int j = 0; j < PE; ++j) {
+    const int k = kb + tid + j * NT;
+    if (k < ke) {
+      const T cxk = alpha * kv[j] + beta;
+      const cx<T> t = tile[kzv[j] * CT + (kcv[j] - c0)];
+      cx<T> o = an * t + cxk * mk<T>(xr[j], xi[j]);
+      o = o + hc * mk<T>(hr[j], hi[j]);
+      o = o + zc * mk<T>(zr[j], zi[j]);
+      out[ob + k] = o;
+    }
+  }
+  for (int k = kb + PE * NT + tid; k < ke; k += NT) {
     const cx<T> x = X[ob + k];
     cx<T> h = inv_nnr * tile[m.kz[k] * CT + (m.kcol[k] - c0)];
     h = h + kin[k] * x;
@@ -1010,17 +1097,49 @@ static hipError_t launch_plane_n(const SphereMaps& m, const void* tw1, const voi
   }
 }
 
+static int col_tile_index(int ct) { return ct == 8 ? 0 : ct == 16 ? 1 : ct == 32 ? 2 : 3; }
+
+// prefetch depth: entries per lane that cover the largest tile, capped at 4
+static int col_pe(const SphereMaps& m, int ct, int nt) {
+  const int per = (m.tile_max[col_tile_index(ct)] + nt - 1) / nt;
+  return per <= 1 ? 1 : per <= 2 ? 2 : 4;
+}
+
+template <typename T, int N, int CT, int NT, int PE>
+static hipError_t zinv_pe(const SphereMaps& m, const void* tw3, const cx<T>* X, int ldx, cx<T>* W, int nb,
+                          hipStream_t s) {
+  dim3 grid((m.ncol_pad + CT - 1) / CT, nb);
+  hipLaunchKernelGGL((k_zinv<T, N, CT, NT, PE>), grid, dim3(NT), 0, s, m, static_cast<const cx<T>*>(tw3), X,
+                     ldx, W);
+  return hipGetLastError();
+}
+
+template <typename T, int N, int CT, int NT>
+static hipError_t zinv_shape(const SphereMaps& m, const void* tw3, const cx<T>* X, int ldx, cx<T>* W, int nb,
+                             hipStream_t s) {
+  switch (col_pe(m, CT, NT)) {
+    case 1: return zinv_pe<T, N, CT, NT, 1>(m, tw3, X, ldx, W, nb, s);
+    case 2: return zinv_pe<T, N, CT, NT, 2>(m, tw3, X, ldx, W, nb, s);
+    default: return zinv_pe<T, N, CT, NT, 4>(m, tw3, X, ldx, W, nb, s);
+  }
+}
+
+// Launch shapes measured at the 64^3 Si64 block (FP32 and FP64): the gather-and-write
+// pass runs best as one-wave blocks of 8 columns, the transform-and-epilogue pass as
+// 256-lane blocks of 32 columns (16 x 128 when that tile would exceed 32 KB of LDS).
+template <typename T, int N>
+static hipError_t zinv_n(const SphereMaps& m, const void* tw3, const cx<T>* X, int ldx, cx<T>* W, int nb,
+                         hipStream_t s) {
+  return zinv_shape<T, N, 8, 64>(m, tw3, X, ldx, W, nb, s);
+}
+
 template <typename T>
 hipError_t launch_zinv(const SphereMaps& m, const void* tw3, const cx<T>* X, int ldx, cx<T>* W,
                        int nb, hipStream_t s) {
-  if (!fft_size_supported(m.nr3) || m.ncol_pad % kColTile) return hipErrorInvalidValue;
-  dim3 cgrid(m.ncol_pad / kColTile, nb);
+  if (!fft_size_supported(m.nr3)) return hipErrorInvalidValue;
   switch (m.nr3) {
-#define PWH_CASE(N)                                                                               \
-  case N:                                                                                         \
-    hipLaunchKernelGGL((k_zinv<T, N>), cgrid, dim3(kColThreads), 0, s, m,                         \
-                       static_cast<const cx<T>*>(tw3), X, ldx, W);                                \
-    return hipGetLastError();
+#define PWH_CASE(N) \
+  case N: return zinv_n<T, N>(m, tw3, X, ldx, W, nb, s);
     PWH_FFT_SIZES(PWH_CASE)
 #undef PWH_CASE
     default: return hipErrorInvalidValue;
@@ -1040,20 +1159,55 @@ hipError_t launch_plane(const SphereMaps& m, const void* tw1, const void* tw2, c
   }
 }
 
+struct ZfwdArgs {
+  const void* tw3;
+  const void* W;
+  const void* kin;
+  const void* X;
+  const void* Hnl;
+  const void* Z;
+  void* out;
+  int ld, nb;
+  double inv_nnr, alpha, beta, gamma;
+};
+
+template <typename T, int N, int CT, int NT, int PE>
+static hipError_t zfwd_pe(const SphereMaps& m, const ZfwdArgs& a, hipStream_t s) {
+  dim3 grid((m.ncol_pad + CT - 1) / CT, a.nb);
+  hipLaunchKernelGGL((k_zfwd<T, N, CT, NT, PE>), grid, dim3(NT), 0, s, m, static_cast<const cx<T>*>(a.tw3),
+                     static_cast<const cx<T>*>(a.W), static_cast<const T*>(a.kin),
+                     static_cast<const cx<T>*>(a.X), static_cast<const cx<T>*>(a.Hnl),
+                     static_cast<const cx<T>*>(a.Z), static_cast<cx<T>*>(a.out), a.ld, (T)a.inv_nnr, (T)a.alpha,
+                     (T)a.beta, (T)a.gamma);
+  return hipGetLastError();
+}
+
+template <typename T, int N, int CT, int NT>
+static hipError_t zfwd_shape(const SphereMaps& m, const ZfwdArgs& a, hipStream_t s) {
+  switch (col_pe(m, CT, NT)) {
+    case 1: return zfwd_pe<T, N, CT, NT, 1>(m, a, s);
+    case 2: return zfwd_pe<T, N, CT, NT, 2>(m, a, s);
+    default: return zfwd_pe<T, N, CT, NT, 4>(m, a, s);
+  }
+}
+
+template <typename T, int N>
+static hipError_t zfwd_n(const SphereMaps& m, const ZfwdArgs& a, hipStream_t s) {
+  if constexpr ((size_t)N * 32 * sizeof(cx<T>) <= 32 * 1024) return zfwd_shape<T, N, 32, 256>(m, a, s);
+  else return zfwd_shape<T, N, 16, 128>(m, a, s);
+}
+
 template <typename T>
 hipError_t launch_zfwd_epilogue(const SphereMaps& m, const void* tw3, const cx<T>* W, const T* kin,
                                 const cx<T>* X, const cx<T>* Hnl, const cx<T>* Z, cx<T>* out,
                                 int ld, int nb, T inv_nnr, T alpha, T beta, T gamma,
                                 hipStream_t s) {
-  if (!fft_size_supported(m.nr3) || m.ncol_pad % kColTile) return hipErrorInvalidValue;
-  dim3 cgrid(m.ncol_pad / kColTile, nb);
+  if (!fft_size_supported(m.nr3)) return hipErrorInvalidValue;
+  const ZfwdArgs a{tw3, W, kin, X, Hnl, Z, out, ld, nb, (double)inv_nnr, (double)alpha, (double)beta,
+                   (double)gamma};
   switch (m.nr3) {
-#define PWH_CASE(N)                                                                               \
-  case N:                                                                                         \
-    hipLaunchKernelGGL((k_zfwd<T, N>), cgrid, dim3(kColThreads), 0, s, m,                         \
-                       static_cast<const cx<T>*>(tw3), W, kin, X, Hnl, Z, out, ld, inv_nnr,       \
-                       alpha, beta, gamma);                                                       \
-    return hipGetLastError();
+#define PWH_CASE(N) \
+  case N: return zfwd_n<T, N>(m, a, s);
     PWH_FFT_SIZES(PWH_CASE)
 #undef PWH_CASE
     default: return hipErrorInvalidValue;

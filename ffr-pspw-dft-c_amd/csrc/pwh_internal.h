@@ -57,6 +57,8 @@ struct SphereMaps {
   // [0, n/4) U [n - n/4, n), i.e. signed index in [-n/4, n/4): true for grids sized
   // for the density cutoff (4 ecut), where the sphere radius is below n/4
   int band_xy;
+  // largest number of sphere entries in one tile of 8 << t consecutive columns (t = 0..3)
+  int tile_max[4];
   int* colstart;        // [ncol_pad+1] first internal slot of each column
   int* kcol;            // [ngw] column of each internal slot
   int* kz;              // [ngw] i3 of each internal slot
