@@ -79,6 +79,8 @@ __device__ __forceinline__ s4v neg4(s4v v) {
 }
 
 constexpr int BT = 64;    // block tile (both output dims)
+// FP32 complex products take three real MFMAs (3M) instead of four
+template <typename T> constexpr bool THREE_M = std::is_same<T, float>::value;
 constexpr int LPAD = 4;
 constexpr int GT = 256;   // threads per block
 
@@ -115,11 +117,16 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
   const int ks = blockIdx.y * kchunk, ke = min(K, ks + kchunk);
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int li = lane & 15, lq = lane >> 4;
-  acc_t cr[2][2], ci[2][2];
+  // FP32 uses three real products per complex one (3M): cr, c2, ci accumulate
+  // S1 = sum ar br, S2 = sum ai bi and S3 = sum of the (sum/difference) product; FP64 and
+  // BF16x3 accumulate Re and Im directly in cr, ci
+  acc_t cr[2][2], ci[2][2], c2[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) { cr[a][b] = acc_t{0, 0, 0, 0}; ci[a][b] = acc_t{0, 0, 0, 0}; }
+    for (int b = 0; b < 2; ++b) {
+      cr[a][b] = acc_t{0, 0, 0, 0}; ci[a][b] = acc_t{0, 0, 0, 0}; c2[a][b] = acc_t{0, 0, 0, 0};
+    }
 
   // register prefetch of the next K tile overlaps its global loads with the MFMAs
   // element e of a stage: 16 consecutive k of one row per 16 lanes (row r = (e/16) % 64,
@@ -168,6 +175,21 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
           ar[t] = sAr[kr][wm + t * 16 + li]; ai[t] = sAi[kr][wm + t * 16 + li];
           br[t] = sBr[kr][wn + t * 16 + li]; bi[t] = sBi[kr][wn + t * 16 + li];
         }
+        if constexpr (THREE_M<T>) {
+          // conj(a) b: S1 = ar br, S2 = ai bi, S3 = (ar - ai)(br + bi); Re = S1 + S2,
+          // Im = S3 - S1 + S2.  Consecutive MFMAs never feed each other.
+          T as[2], bs[2];
+#pragma unroll
+          for (int t = 0; t < 2; ++t) { as[t] = ar[t] - ai[t]; bs[t] = br[t] + bi[t]; }
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+              cr[a][b] = mfma4(ar[a], br[b], cr[a][b]);
+              c2[a][b] = mfma4(ai[a], bi[b], c2[a][b]);
+              ci[a][b] = mfma4(as[a], bs[b], ci[a][b]);
+            }
+        } else {
         // first halves of all 8 accumulators, then the second halves: consecutive
         // MFMAs never feed each other
 #pragma unroll
@@ -184,6 +206,7 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
             cr[a][b] = mfma4(ai[a], bi[b], cr[a][b]);
             ci[a][b] = mfma4(-ai[a], br[b], ci[a][b]);
           }
+        }
       }
     } else {
       Split4 ar[2], ai[2], br[2], bi[2];
@@ -220,6 +243,20 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
     }
     }
     __syncthreads();
+  }
+  if constexpr (THREE_M<T>) {
+    if constexpr (!BF16X3) {
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const T s1 = cr[a][b][r], s2 = c2[a][b][r], s3 = ci[a][b][r];
+            cr[a][b][r] = s1 + s2;
+            ci[a][b][r] = (s3 - s1) + s2;
+          }
+    }
   }
   cx<T>* P = part + (size_t)blockIdx.y * M * N;
 #pragma unroll
@@ -286,11 +323,16 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
   const int k0 = blockIdx.x * BT, n0 = blockIdx.y * BT;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int li = lane & 15, lq = lane >> 4;
-  acc_t cr[2][2], ci[2][2];
+  // FP32 uses three real products per complex one (3M): cr, c2, ci accumulate
+  // S1 = sum ar br, S2 = sum ai bi and S3 = sum of the (sum/difference) product; FP64 and
+  // BF16x3 accumulate Re and Im directly in cr, ci
+  acc_t cr[2][2], ci[2][2], c2[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) { cr[a][b] = acc_t{0, 0, 0, 0}; ci[a][b] = acc_t{0, 0, 0, 0}; }
+    for (int b = 0; b < 2; ++b) {
+      cr[a][b] = acc_t{0, 0, 0, 0}; ci[a][b] = acc_t{0, 0, 0, 0}; c2[a][b] = acc_t{0, 0, 0, 0};
+    }
 
   constexpr int PER = BT * KS / GT;
   cx<T> pa[PER], pq[PER];
@@ -332,6 +374,21 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
           ar[t] = sAr[jr][wm + t * 16 + li]; ai[t] = sAi[jr][wm + t * 16 + li];
           br[t] = sQr[jr][wn + t * 16 + li]; bi[t] = sQi[jr][wn + t * 16 + li];
         }
+        if constexpr (THREE_M<T>) {
+          // a q: S1 = ar qr, S2 = ai qi, S3 = (ar + ai)(qr + qi); Re = S1 - S2,
+          // Im = S3 - S1 - S2
+          T as[2], bs[2];
+#pragma unroll
+          for (int t = 0; t < 2; ++t) { as[t] = ar[t] + ai[t]; bs[t] = br[t] + bi[t]; }
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+              cr[a][b] = mfma4(ar[a], br[b], cr[a][b]);
+              c2[a][b] = mfma4(ai[a], bi[b], c2[a][b]);
+              ci[a][b] = mfma4(as[a], bs[b], ci[a][b]);
+            }
+        } else {
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -346,6 +403,7 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
             cr[a][b] = mfma4(-ai[a], bi[b], cr[a][b]);
             ci[a][b] = mfma4(ai[a], br[b], ci[a][b]);
           }
+        }
       }
     } else {
       Split4 ar[2], ai[2], br[2], bi[2];
@@ -381,6 +439,20 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
     }
     }
     __syncthreads();
+  }
+  if constexpr (THREE_M<T>) {
+    if constexpr (!BF16X3) {
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const T s1 = cr[a][b][r], s2 = c2[a][b][r], s3 = ci[a][b][r];
+            cr[a][b][r] = s1 - s2;
+            ci[a][b][r] = (s3 - s1) - s2;
+          }
+    }
   }
 #pragma unroll
   for (int a = 0; a < 2; ++a)
