@@ -467,9 +467,28 @@ int filter_on_stream(pwh_system* s, int nb, int degree, double lb, double ub, in
   return PWH_OK;
 }
 
-// CholeskyQR2 (FP64) of the nb columns of X [nb][ld] (nrows valid rows), using tmp [nb][ld].
+// CholeskyQR2 (FP64) of the nb columns of X [nb][ld] (nrows valid rows), using tmp [nb][ld];
+// dinfo: 2 ints of device scratch.  For nb <= 128 the factor inverse comes from the
+// one-workgroup kernel and both passes are checked with one synchronisation at the end.
 int cholqr2(cx<double>* X, int ld, int nrows, int nb, cx<double>* tmp, cx<double>* S,
             cx<double>* part, size_t part_elems, rocblas_int* dinfo) {
+  if (nb <= kCholSmallMax) {
+    for (int pass = 0; pass < 2; ++pass) {
+      cx<double>* src = (pass == 0) ? X : tmp;
+      cx<double>* dst = (pass == 0) ? tmp : X;
+      HIPCHK(gemm_cn_f64(src, ld, nb, src, ld, nb, nrows, S, nb, part, part_elems, g.stream));
+      HIPCHK(launch_chol_inv_small(S, nb, nb, reinterpret_cast<int*>(dinfo) + pass, g.stream));
+      HIPCHK(gemm_nn_f64(src, ld, nb, S, nb, nb, nrows, dst, ld, g.stream));
+    }
+    rocblas_int hinfo[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(hinfo, dinfo, sizeof(hinfo), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    for (int pass = 0; pass < 2; ++pass)
+      if (hinfo[pass] != 0)
+        return fail(PWH_ERR_LAPACK, "CholeskyQR: overlap matrix not positive definite (zpotrf info " +
+                                        std::to_string(hinfo[pass]) + ")");
+    return PWH_OK;
+  }
   for (int pass = 0; pass < 2; ++pass) {
     cx<double>* src = (pass == 0) ? X : tmp;
     cx<double>* dst = (pass == 0) ? tmp : X;
@@ -496,7 +515,7 @@ int sys_cholqr2(pwh_system* s, cx<double>* X, int nb) {
   CHK(s->S.ensure((size_t)nb * nb));
   const size_t pe = proj_part_elems(nb, nb, s->ngw);
   CHK(s->part64.ensure(pe));
-  CHK(s->info.ensure(1));
+  CHK(s->info.ensure(2));
   return cholqr2(X, s->ngw_pad, s->ngw, nb, s->A64.p, s->S.p, s->part64.p, pe, s->info.p);
 }
 
@@ -996,7 +1015,7 @@ int pwh_ortho_qr(double* X, int nbasis, int nstates) {
   const size_t pe = proj_part_elems(nstates, nstates, nbasis);
   int rc;
   if ((rc = upload(dX, reinterpret_cast<const cx<double>*>(X), n)) || (rc = dT.ensure(n)) ||
-      (rc = dS.ensure((size_t)nstates * nstates)) || (rc = dP.ensure(pe)) || (rc = di.ensure(1))) {
+      (rc = dS.ensure((size_t)nstates * nstates)) || (rc = dP.ensure(pe)) || (rc = di.ensure(2))) {
     dX.release(); dT.release(); dS.release(); dP.release(); di.release();
     return rc;
   }
@@ -1334,16 +1353,22 @@ int chol_right(const DiagRun& r, cx<double>* A, cx<double>* HA, int n) {
   cx<double>* C = s->S.p;
   CHK(r.cn(A, n, A, n, C, n));
   HIPCHK(symmetrize(C, n, n, g.stream));
-  RBCHK(rocsolver_zpotrf(g.rb, rocblas_fill_upper, n, reinterpret_cast<rocblas_double_complex*>(C), n,
-                         s->info.p));
+  if (n <= kCholSmallMax) {
+    HIPCHK(launch_chol_inv_small(C, n, n, reinterpret_cast<int*>(s->info.p), g.stream));
+  } else {
+    RBCHK(rocsolver_zpotrf(g.rb, rocblas_fill_upper, n, reinterpret_cast<rocblas_double_complex*>(C), n,
+                           s->info.p));
+  }
   int info = 0;
   CHK(r.info_value(&info));
   if (info != 0)
     return fail(PWH_ERR_LAPACK, "Cholesky factorisation of the block overlap failed (zpotrf info " +
                                     std::to_string(info) + ")");
-  RBCHK(rocsolver_ztrtri(g.rb, rocblas_fill_upper, rocblas_diagonal_non_unit, n,
-                         reinterpret_cast<rocblas_double_complex*>(C), n, s->info.p));
-  HIPCHK(zero_lower(C, n, n, g.stream));
+  if (n > kCholSmallMax) {
+    RBCHK(rocsolver_ztrtri(g.rb, rocblas_fill_upper, rocblas_diagonal_non_unit, n,
+                           reinterpret_cast<rocblas_double_complex*>(C), n, s->info.p));
+    HIPCHK(zero_lower(C, n, n, g.stream));
+  }
   CHK(r.nn(A, n, C, n, n, s->dtmp.p));
   CHK(r.copy(A, s->dtmp.p, n));
   CHK(r.nn(HA, n, C, n, n, s->dtmp.p));

@@ -84,6 +84,11 @@ bool plane_supported(int n1, int n2);
 // lda, upper triangle read) -> eigenvectors, w ascending; info[0] != 0 on a non-finite value.
 size_t heev_small_scratch_doubles();
 constexpr int kHeevSmallMax = 128;
+// One-workgroup inverse Cholesky factor (pwh_chol.hip): S (column-major, lda, upper
+// triangle read as Hermitian positive definite) -> R^{-1} with S = R^H R, zeros below the
+// diagonal; info[0] = 0, or k + 1 when the k-th pivot is not positive (S then undefined).
+constexpr int kCholSmallMax = 128;
+hipError_t launch_chol_inv_small(cx<double>* S, int lda, int n, int* info, hipStream_t s);
 hipError_t launch_heev_small(cx<double>* A, int lda, int n, double* w, int* info, double* scratch,
                              hipStream_t s);
 // Density of a block in sphere order (FP64): W (after launch_zinv) -> rho[nr3][nr2][nr1] =

@@ -62,6 +62,26 @@ def test_ortho_qr_column_space(si8):
     assert np.all(np.real(np.diag(Rq)) > 0)
 
 
+@pytest.mark.parametrize("nst", [128, 150])
+def test_ortho_qr_small_kernel_and_rocsolver_paths(nst):
+    """n <= 128 takes the one-workgroup inverse Cholesky factor, n > 128 rocSOLVER's
+    zpotrf + ztrtri: both give Q = X R^{-1} with R upper, positive real diagonal."""
+    import pwh
+    rng = np.random.default_rng(nst)
+    X = rng.standard_normal((nst, 700)) + 1j * rng.standard_normal((nst, 700))
+    X[:, :nst] += 4 * np.eye(nst)
+    Q = pwh.ortho_qr(X)
+    assert np.abs(Q.conj() @ Q.T - np.eye(nst)).max() < 1e-13
+    Rq = Q.conj() @ X.T
+    assert np.abs(np.tril(Rq, -1)).max() < 1e-10 * np.abs(Rq).max()
+    d = np.diag(Rq)
+    assert np.all(d.real > 0) and np.abs(d.imag).max() < 1e-10 * np.abs(d).max()
+    # same R as numpy's QR up to the column phases fixed by the positive diagonal
+    _, Rn = np.linalg.qr(X.T)
+    ph = np.sign(np.diag(Rn).real)
+    assert np.abs(Rq - ph[:, None] * Rn).max() < 1e-10 * np.abs(Rn).max()
+
+
 def test_ortho_qr_rank_deficient_reports_error():
     import pwh
     X = np.ones((3, 50), np.complex128)
