@@ -112,9 +112,18 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
   Row* sBr = sAi + KS;
   Row* sBi = sBr + KS;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int tiles_n = (N + BT - 1) / BT;
-  const int m0 = (blockIdx.x / tiles_n) * BT, n0 = (blockIdx.x % tiles_n) * BT;
-  const int ks = blockIdx.y * kchunk, ke = min(K, ks + kchunk);
+  const int tiles_n = (N + BT - 1) / BT, tiles = ((M + BT - 1) / BT) * tiles_n;
+  // XCD-aware order: workgroup L of the 1D grid runs on XCD L % 8 (round-robin dispatch).
+  // The tiles of one K-split read the same chunk of both operands, so they get the same
+  // L % 8 and neighbouring L: the re-reads hit that XCD's L2.  Splits past the last exit.
+  // With a split count that is not a multiple of 8 the plain order (tile fastest) is used.
+  const int L = blockIdx.x, q = L >> 3;
+  const bool xcd = ((K + kchunk - 1) / kchunk) % 8 == 0;
+  const int tile = xcd ? q % tiles : L % tiles;
+  const int split = xcd ? (L & 7) + 8 * (q / tiles) : L / tiles;
+  if (split * kchunk >= K) return;
+  const int m0 = (tile / tiles_n) * BT, n0 = (tile % tiles_n) * BT;
+  const int ks = split * kchunk, ke = min(K, ks + kchunk);
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int li = lane & 15, lq = lane >> 4;
   // FP32 uses three real products per complex one (3M): cr, c2, ci accumulate
@@ -258,7 +267,7 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
           }
     }
   }
-  cx<T>* P = part + (size_t)blockIdx.y * M * N;
+  cx<T>* P = part + (size_t)split * M * N;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -320,7 +329,12 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
   Row* sQr = sAi + KS;
   Row* sQi = sQr + KS;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int k0 = blockIdx.x * BT, n0 = blockIdx.y * BT;
+  // XCD-aware order (as in k_gemm_proj): the band tiles of one plane-wave tile read the
+  // same rows of A, so they share L % 8 and run close together; tiles past K exit.
+  const int tiles_n = (N + BT - 1) / BT;
+  const int L = blockIdx.x, q = L >> 3;
+  const int k0 = ((L & 7) + 8 * (q / tiles_n)) * BT, n0 = (q % tiles_n) * BT;
+  if (k0 >= K) return;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int li = lane & 15, lq = lane >> 4;
   // FP32 uses three real products per complex one (3M): cr, c2, ci accumulate
@@ -483,9 +497,11 @@ static int gemm_target_blocks() {
 }
 
 static int choose_split(int tiles, int K, size_t part_elems, size_t MN) {
-  // aim for ~2 blocks per CU on 256 CUs, at least 128 plane waves per block
+  // aim for ~2 blocks per CU on 256 CUs, at least 128 plane waves per block; a multiple
+  // of 8 where K allows, so the XCD-aware order keeps all 8 XCDs busy
   int s = (gemm_target_blocks() + tiles - 1) / tiles;
   int smax = (K + 127) / 128;
+  s = (s + 7) / 8 * 8;
   if (s > smax) s = smax;
   if (s < 1) s = 1;
   while (s > 1 && (size_t)s * MN > part_elems) --s;
@@ -518,6 +534,12 @@ static hipError_t allow_smem(F* kern, size_t bytes) {
   if (bytes <= 65536) return hipSuccess;
   return hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)bytes);
+}
+
+// 1D grid of k_gemm_expand: plane-wave tiles rounded up to 8, times the band tiles
+static int expand_blocks(int K, int N) {
+  const int tk = (K + BT - 1) / BT, tn = (N + BT - 1) / BT;
+  return ((tk + 7) / 8) * 8 * tn;
 }
 
 template <typename T, bool BETA, bool BF, int KS>
@@ -569,7 +591,7 @@ static hipError_t run_proj(RowLoader<T, BETA> ld, const cx<T>* B, int ldb, int M
   kchunk = ((kchunk + ks - 1) / ks) * ks;
   S = (K + kchunk - 1) / kchunk;
   if ((size_t)S * M * N > part_elems) return hipErrorInvalidValue;
-  dim3 grid(tiles, S);
+  dim3 grid(tiles * ((S + 7) / 8) * 8);   // 1D, decoded XCD-aware in the kernel
   hipError_t e;
   if constexpr (std::is_same<T, float>::value) {
     e = bf16 ? launch_proj<T, BETA, true>(ks, grid, s, ld, B, ldb, M, N, K, kchunk, part)
@@ -604,7 +626,7 @@ hipError_t gemm_expand(const BetaGen& bg, const cx<T>* Q, int nb, int K, cx<T>* 
   ld.A = static_cast<const cx<T>*>(bg.B);
   ld.lda = bg.ld;
   const int J = bg.rows;
-  dim3 grid((K + BT - 1) / BT, (nb + BT - 1) / BT);
+  dim3 grid(expand_blocks(K, nb));
   const int ks = gemm_ks<T>();
   if constexpr (std::is_same<T, float>::value) {
     if (mode == PREC_BF16X3)
@@ -631,7 +653,7 @@ hipError_t gemm_nn_f64(const cx<double>* A, int lda, int J, const cx<double>* Q,
                        int K, cx<double>* out, int ldo, hipStream_t s) {
   RowLoader<double, false> ld{};
   ld.A = A; ld.lda = lda;
-  dim3 grid((K + BT - 1) / BT, (N + BT - 1) / BT);
+  dim3 grid(expand_blocks(K, N));
   return launch_expand<double, false, false, true>(gemm_ks<double>(), grid, s, ld, Q, ldq, J, N, K, out, ldo, 0);
 }
 
