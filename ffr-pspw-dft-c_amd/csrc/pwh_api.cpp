@@ -351,7 +351,8 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
 // chebyfilt.cpp:6-74 on device buffers b0 (= X in), b1, b2 (work); *result -> filtered block
 template <typename T>
 int chebyfilt_dev(pwh_system* s, cx<T>* b0, cx<T>* b1, cx<T>* b2, int nb, int degree, double lb,
-                  double ub, int mode, cx<T>** result, hipStream_t st = nullptr) {
+                  double ub, int mode, cx<T>** result, hipStream_t st = nullptr, cx<T>** previous = nullptr,
+                  double* sigma_out = nullptr) {
   const double e = (ub - lb) / 2.0;
   const double c = (ub + lb) / 2.0;
   double sigma = e / (lb - ub);
@@ -370,7 +371,33 @@ int chebyfilt_dev(pwh_system* s, cx<T>* b0, cx<T>* b1, cx<T>* b2, int nb, int de
     sigma = sigma2;
   }
   *result = Y;
+  if (previous) *previous = X;
+  if (sigma_out) *sigma_out = sigma;
   return PWH_OK;
+}
+
+// The mixed-precision filters run their last recurrence step in FP64: rounding injected
+// by the FP32 steps is damped by the steps after it, except in the last one, which would
+// leave an FP32-sized error in the filtered block (it showed as a ~3e-7 Ha wander of the
+// converged SCF total energy).  prev/cur: FP32 iterates m-2 and m-1; out: FP64 result.
+bool filter_last_fp64() {
+  static const bool on = [] {
+    const char* e = getenv("PWH_FILTER_LAST64");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+int cheby_last_step_fp64(pwh_system* s, const cx<float>* prev, const cx<float>* cur, cx<double>* out, int nb,
+                         double lb, double ub, double sigma, hipStream_t st) {
+  const size_t n = (size_t)nb * s->ngw_pad;
+  const double e = (ub - lb) / 2.0, c = (ub + lb) / 2.0;
+  const double sigma1 = e / (lb - ub);
+  const double sigma2 = 1.0 / (2.0 / sigma1 - sigma);
+  HIPCHK((convert_cx<float, double>(prev, s->A64.p, n, st)));
+  HIPCHK((convert_cx<float, double>(cur, s->B64.p, n, st)));
+  return hpsi<double>(s, s->B64.p, out, nb, 2.0 * sigma2 / e, -c * 2.0 * sigma2 / e, -sigma * sigma2, s->A64.p,
+                      PREC_FP64, st);
 }
 
 int check_ld(pwh_system* s, int nb) {
@@ -427,6 +454,15 @@ int filter_prec(pwh_system* s, cx<double>* X, int nb, int degree, double lb, dou
   CHK(s->Y1_32.ensure(n));
   HIPCHK((convert_cx<double, float>(X, s->X32.p, n, g.stream)));
   cx<float>* res = nullptr;
+  if (degree >= 2 && filter_last_fp64()) {
+    CHK(s->A64.ensure(n));
+    CHK(s->B64.ensure(n));
+    cx<float>* prev = nullptr;
+    double sigma = 0;
+    CHK(chebyfilt_dev<float>(s, s->X32.p, s->Y32.p, s->Y1_32.p, nb, degree - 1, lb, ub, precision, &res,
+                             g.stream, &prev, &sigma));
+    return cheby_last_step_fp64(s, prev, res, X, nb, lb, ub, sigma, g.stream);
+  }
   CHK(chebyfilt_dev<float>(s, s->X32.p, s->Y32.p, s->Y1_32.p, nb, degree, lb, ub, precision, &res));
   HIPCHK((convert_cx<float, double>(res, X, n, g.stream)));
   return PWH_OK;
@@ -444,6 +480,11 @@ int filter_prepare(pwh_system* s, int nb, int precision) {
   CHK(s->X32.ensure(n));
   CHK(s->Y32.ensure(n));
   CHK(s->Y1_32.ensure(n));
+  if (filter_last_fp64()) {
+    CHK(s->A64.ensure(n));
+    CHK(s->B64.ensure(n));
+    CHK(hpsi_prepare<double>(s, nb));
+  }
   return hpsi_prepare<float>(s, nb);
 }
 
@@ -462,6 +503,13 @@ int filter_on_stream(pwh_system* s, int nb, int degree, double lb, double ub, in
   }
   HIPCHK((convert_cx<double, float>(s->X.p, s->X32.p, n, st)));
   cx<float>* res = nullptr;
+  if (degree >= 2 && filter_last_fp64()) {
+    cx<float>* prev = nullptr;
+    double sigma = 0;
+    CHK(chebyfilt_dev<float>(s, s->X32.p, s->Y32.p, s->Y1_32.p, nb, degree - 1, lb, ub, precision, &res, st,
+                             &prev, &sigma));
+    return cheby_last_step_fp64(s, prev, res, s->X.p, nb, lb, ub, sigma, st);
+  }
   CHK(chebyfilt_dev<float>(s, s->X32.p, s->Y32.p, s->Y1_32.p, nb, degree, lb, ub, precision, &res, st));
   HIPCHK((convert_cx<float, double>(res, s->X.p, n, st)));
   return PWH_OK;

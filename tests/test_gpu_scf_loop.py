@@ -1,0 +1,135 @@
+"""The whole Kohn-Sham cycle on the GPU (pwh.scf.cheby_scf, ChebySCFLoop.cpp:171-372 over
+the library's entry points) on the Si8 cell: ChFSI step with V_eff, EvalRhoPsi, EvalENL,
+mixing, EvalEnergy.
+
+* FP64 trajectory against the oracle composed the same way on the CPU (oracle chebyfilt,
+  ortho_qr, apply_ham_block + Rayleigh-Ritz, eval_rho_psi, eval_enl, simple_mix,
+  eval_energy): ETOT of every iteration to 1e-8 relative.
+* Mixed precision against FP64 at self-consistency: with an FP32 filter throughout, the
+  converged energy wanders with the filter's rounding, its median distance to the FP64
+  energy below 1e-6 Ha (the acceptance figure) and the occupied eigenvalues within 5e-8 Ha;
+  with the last iterations in FP64 (fp64_refine), the FP64 fixed point to 1e-8 Ha.
+
+The synthetic cell's levels come in degenerate shells (6, 6, 3, 3, 6 states at the bottom);
+12 occupied states close a shell, so the loop converges with the reference's integer
+occupations (16 would half-fill a shell, which needs smearing the reference lacks), and 15
+states close the next one, so the subspace edge does not split a multiplet."""
+import numpy as np
+import pytest
+
+from conftest import density_gspace, oracle_globals
+
+pytestmark = pytest.mark.gpu
+
+NOCC, NST, DEGREE, CHFSI_ITERS, BETA = 12, 15, 10, 2, 0.3
+
+
+@pytest.fixture(scope="module")
+def scf_setup(si8):
+    import pwh
+    from pwh import synth
+    pwh.init(0)
+    d = synth.density_gspace(si8)
+    grid = pwh.Grid(si8.nr, d["n1"], d["n2"], d["n3"], d["g2"], d["sfac"], d["vps"], d["rhops"], d["tpiba2"],
+                    d["omega"], n123=si8.gv["n123"][:si8.ngw].astype(np.int64), xkg=si8.gv["xkg"][:si8.ngw])
+    sysm = pwh.System(si8.globals())
+    rng = np.random.default_rng(5)
+    X0 = pwh.ortho_qr(rng.standard_normal((NST, si8.ngw)) + 1j * rng.standard_normal((NST, si8.ngw)))
+    rho0 = np.full(int(np.prod(si8.nr)), 2.0 * NOCC / d["omega"])
+    yield sysm, grid, X0, rho0, d["omega"]
+    sysm.close()
+    grid.close()
+
+
+def _v0(ngw):
+    rng = np.random.default_rng(17)
+    return rng.random(ngw) + 1j * rng.random(ngw)
+
+
+def _run(scf_setup, precision, max_iter=200, conv=1e-10, **kw):
+    from pwh.scf import cheby_scf
+    sysm, grid, X0, rho0, omega = scf_setup
+    return cheby_scf(sysm, grid, X0, NOCC, np.full(NOCC, 2.0), omega, rho0, degree=DEGREE, precision=precision,
+                     mixer="simple", beta=BETA, max_iter=max_iter, conv=conv, chfsi_iters=CHFSI_ITERS,
+                     v0=_v0(X0.shape[1]), **kw)
+
+
+def _oracle_scf(oracle, si8, X0, rho0, omega, iters):
+    """The same cycle from the oracle's pieces (bounds: 20-step Lanczos once, ub = largest
+    Ritz value + ||f||, lb = w0 + frac (ub - w0); after each Rayleigh-Ritz step lb = its
+    largest value)."""
+    g = oracle_globals(si8)
+    gs = density_gspace(si8)
+    ngw = si8.ngw
+    rho = rho0.copy()
+    rho_old = rho.copy()
+    veff, _ = oracle.eval_energy(gs, rho.copy(), 0.0, 0.0, 0.0, 0.0)
+    T, f, _ = oracle.lanczos(g, _v0(ngw), 20, veff, ngw)
+    w = np.linalg.eigvalsh(T, UPLO="U")    # dsyev uplo U, as ChebySCFLoop.cpp:180
+    ub = w[-1] + np.sqrt(np.vdot(f, f).real)
+    frac = min(0.5, max(0.02, (NST / ngw) ** (2.0 / 3.0)))
+    lb = w[0] + frac * (ub - w[0])
+    X = X0.copy()
+    out = []
+    for _ in range(iters):
+        for _ in range(CHFSI_ITERS):
+            X = oracle.chebyfilt(g, X, DEGREE, lb, ub, veff, ngw)
+            X = oracle.ortho_qr(X)
+            HX = oracle.apply_ham_block(g, X, veff, ngw)
+            eig, X = oracle.rayleigh_ritz(X, HX)
+            lb = eig[-1]
+        focc = np.full(NOCC, 2.0)
+        rho, ekin, _, bad = oracle.eval_rho_psi(g, X[:NOCC], ngw, focc, omega)
+        assert bad < 0
+        c0 = np.zeros((NOCC, g.ngwx + 3), np.complex128)
+        c0[:, :ngw] = X[:NOCC]
+        enl, _ = oracle.eval_enl(g, c0, ngw, focc, 1.0)
+        oracle.simple_mix(BETA, rho, rho_old)
+        veff, e = oracle.eval_energy(gs, rho.copy(), ekin, enl, 0.0, 0.0)
+        out.append(e["etot"])
+    return out
+
+
+def test_scf_fp64_trajectory_matches_oracle(scf_setup, si8, oracle):
+    sysm, grid, X0, rho0, omega = scf_setup
+    r = _run(scf_setup, "fp64", max_iter=3, conv=0.0)
+    ref = _oracle_scf(oracle, si8, X0, rho0, omega, 3)
+    got = [h[1] for h in r.history]
+    assert len(got) == 3
+    assert np.abs(np.array(got) - np.array(ref)).max() < 1e-8 * abs(ref[-1]), (got, ref)
+
+
+def test_scf_converges_and_mixed_total_energy_within_1e6(scf_setup):
+    import pwh
+    r64 = _run(scf_setup, "fp64")
+    assert r64.converged and r64.iterations < 200
+    # the energy terms add up (EvalEnergy.cpp:142-144; EHT carries ESR - ESELF)
+    e = r64.energies
+    parts = e["ekin"] + e["enl"] + e["eht"] + e["epseu"] + e["exc"]
+    assert abs(e["etot"] - parts) < 1e-9 * abs(e["etot"])
+    # mixed filters throughout: past convergence the energy wanders with the filter's
+    # rounding; its median distance to the FP64 fixed point is the figure
+    for prec, tol in (("fp32", 1e-6), ("bf16x3", 1e-4)):
+        r = _run(scf_setup, prec, max_iter=150, conv=-1.0)
+        dev = np.abs(np.array([h[1] for h in r.history[100:]]) - r64.etot)
+        assert np.median(dev) < tol, (prec, np.median(dev), dev.max())
+        if prec == "fp32":
+            assert np.abs(r.eig[:NOCC] - r64.eig[:NOCC]).max() < 5e-8
+    # FP32 until |dE| <= 1e-6, then FP64: the FP64 fixed point
+    r = _run(scf_setup, "fp32", fp64_refine=1e-6)
+    assert r.converged
+    n32 = sum(1 for h in r.history if h[6] == pwh.FP32)
+    assert 10 < n32 < r.iterations
+    assert abs(r.etot - r64.etot) < 1e-8, (r.etot, r64.etot)
+    assert np.abs(r.eig[:NOCC] - r64.eig[:NOCC]).max() < 1e-9
+
+
+def test_scf_driver_rejects_bad_inputs(scf_setup):
+    from pwh.scf import cheby_scf
+    sysm, grid, X0, rho0, omega = scf_setup
+    with pytest.raises(ValueError):
+        cheby_scf(sysm, grid, X0, NOCC, np.full(NOCC, 2.0), omega, rho0[:-1])
+    with pytest.raises(ValueError):
+        cheby_scf(sysm, grid, X0, NOCC, np.full(NOCC - 1, 2.0), omega, rho0)
+    with pytest.raises(ValueError):
+        cheby_scf(sysm, grid, X0, NOCC, np.full(NOCC, 2.0), omega, rho0, mixer="pulay")
