@@ -335,6 +335,54 @@ def scf_steps(s, sysm, psi, steps, cpu):
     return out
 
 
+def scf_loop(s):
+    """The Kohn-Sham loop on the GPU (pwh.scf.cheby_scf, shaped like ChebySCFLoop.cpp) on the
+    cell's block shape: 128 states, the lowest 90 occupied (a closed shell of the synthetic
+    cell's spectrum, so integer occupations converge), simple mixing 0.3, degree-10 ChFSI,
+    two ChFSI iterations per SCF step.  FP64 throughout; FP32 throughout (the converged
+    energy's median distance to FP64 over the last 40 of 100 steps); FP32 with the last
+    steps in FP64 (switch at |dE| <= 1e-6).  Times are host-inclusive: the entry points
+    take host blocks, so every step moves the 27 MB block to and from the device."""
+    import pwh
+    from pwh import synth
+    from pwh.scf import cheby_scf
+    nst, nocc = 128, 90
+    gsd = synth.density_gspace(s)
+    ngw = s.ngw
+    grid = pwh.Grid(s.nr, gsd["n1"], gsd["n2"], gsd["n3"], gsd["g2"], gsd["sfac"], gsd["vps"], gsd["rhops"],
+                    gsd["tpiba2"], gsd["omega"], n123=s.gv["n123"][:ngw], xkg=s.gv["xkg"][:ngw])
+    sysm = pwh.System(s.globals())
+    rng = np.random.default_rng(5)
+    X0 = pwh.ortho_qr(rng.standard_normal((nst, ngw)) + 1j * rng.standard_normal((nst, ngw)))
+    omega = gsd["omega"]
+    rho0 = np.full(int(np.prod(s.nr)), 2.0 * nocc / omega)
+    kw = dict(mixer="simple", beta=0.3, chfsi_iters=2, degree=10)
+    focc = np.full(nocc, 2.0)
+    out = {"workload": f"{s.name}: {nst} states, {nocc} occupied (closed shell), simple mixing 0.3, "
+                       "degree-10 ChFSI x2 per step, start from a uniform density and random states"}
+    r64 = cheby_scf(sysm, grid, X0, nocc, focc, omega, rho0, precision="fp64", max_iter=150, conv=1e-10, **kw)
+    out["fp64"] = {"iterations": r64.iterations, "converged": r64.converged, "etot_Ha": r64.etot,
+                   "s_per_step": round(r64.t_total_s / r64.iterations, 5),
+                   "chfsi_s_per_step": round(r64.t_chfsi_s / r64.iterations, 5)}
+    rr = cheby_scf(sysm, grid, X0, nocc, focc, omega, rho0, precision="fp32", max_iter=150, conv=1e-10,
+                   fp64_refine=1e-6, **kw)
+    n32 = sum(1 for h in rr.history if h[6] == pwh.FP32)
+    out["fp32_then_fp64"] = {"iterations": rr.iterations, "fp32_steps": n32, "converged": rr.converged,
+                             "etot_Ha": rr.etot, "abs_etot_diff_vs_fp64_Ha": abs(rr.etot - r64.etot),
+                             "max_abs_occ_eig_diff_Ha": float(np.abs(rr.eig[:nocc] - r64.eig[:nocc]).max()),
+                             "s_per_step": round(rr.t_total_s / rr.iterations, 5)}
+    r32 = cheby_scf(sysm, grid, X0, nocc, focc, omega, rho0, precision="fp32", max_iter=100, conv=-1.0, **kw)
+    dev = np.abs(np.array([h[1] for h in r32.history[60:]]) - r64.etot)
+    out["fp32"] = {"steps": r32.iterations, "median_abs_etot_diff_vs_fp64_Ha": float(np.median(dev)),
+                   "max_abs_etot_diff_vs_fp64_Ha": float(dev.max()),
+                   "max_abs_occ_eig_diff_Ha": float(np.abs(r32.eig[:nocc] - r64.eig[:nocc]).max()),
+                   "s_per_step": round(r32.t_total_s / r32.iterations, 5),
+                   "chfsi_s_per_step": round(r32.t_chfsi_s / r32.iterations, 5)}
+    sysm.close()
+    grid.close()
+    return out
+
+
 def setup_steps(s, steps, cpu):
     """The per-G set-up (NLFormFactor.cpp, FormFactor.cpp, PhaseFactor.cpp) for the cell with
     the silicon pseudopotential of the reference's example input (tests/golden/si_fhi.npz):
@@ -580,6 +628,11 @@ def main():
         scf = scf_steps(s, sysm, psi, steps=max(5, min(args.steps, 20)), cpu=not args.no_cpu)
         log("bench: scf steps " + json.dumps(scf))
 
+    loop = None
+    if not args.only and not args.no_chfsi:
+        loop = scf_loop(s)
+        log("bench: scf loop " + json.dumps(loop))
+
     setup = None
     if not args.only:
         setup = setup_steps(s, steps=5, cpu=not args.no_cpu)
@@ -629,6 +682,7 @@ def main():
         "eval_rho_psi": rho,
         "block_solvers": solvers,
         "scf_steps": scf,
+        "scf_loop": loop,
         "setup_steps": setup,
         "c3_batched_uneven": extra.get("c3_batched_uneven"),
         "c4_si216": extra.get("c4_si216"),
