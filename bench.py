@@ -341,8 +341,10 @@ def scf_loop(s):
     cell's spectrum, so integer occupations converge), simple mixing 0.3, degree-10 ChFSI,
     two ChFSI iterations per SCF step.  FP64 throughout; FP32 throughout (the converged
     energy's median distance to FP64 over the last 40 of 100 steps); FP32 with the last
-    steps in FP64 (switch at |dE| <= 1e-6).  Times are host-inclusive: the entry points
-    take host blocks, so every step moves the 27 MB block to and from the device."""
+    steps in FP64 (switch at |dE| <= 1e-6).  The block stays resident in HBM for the whole
+    loop (pwh_resident_chfsi / _eval_rho_psi / _eval_enl); per step the density and the
+    potential (2 MB each) cross to the host for the mixer and EvalEnergy calls, whose entry
+    points take host arrays as the reference's do.  Times are wall times per step."""
     import pwh
     from pwh import synth
     from pwh.scf import cheby_scf

@@ -1128,19 +1128,13 @@ int pwh_eig_zheevd(double* A, int ldA, double* lambda, int n) {
   return PWH_OK;
 }
 
-int pwh_eval_rho_psi(pwh_system* s, const double* eigvec, int ld, int nstates, const double* w, double omega,
-                     double* rhoe, double* ekin, double* charge) {
-  CHK(check_ld(s, nstates));
-  if (!eigvec || !w || !rhoe || !ekin || !charge || ld < s->ngw || !(omega > 0.0))
-    return fail(PWH_ERR_ARG, "bad eval_rho_psi arguments");
-  const int nb = nstates, ngw = s->ngw, ldp = s->ngw_pad;
+// Density of nb states (EvalRhoPsi.cpp:83-157) held in sphere order in Xw, which the
+// call modifies (the last plane wave is zeroed).  stage: the states in reference order to
+// permute into Xw first, or null when Xw already holds them.
+static int eval_rho_core(pwh_system* s, const cx<double>* stage, cx<double>* Xw, int nb, const double* w,
+                         double omega, double* rhoe, double* ekin, double* charge) {
+  const int ngw = s->ngw, ldp = s->ngw_pad;
   const size_t nnr = (size_t)s->nnr;
-  // coefficients in sphere order; the reference's loop stops at ig < NGW_ik, so the last
-  // plane wave contributes to neither the density nor the sums (EvalRhoPsi.cpp:117)
-  CHK(s->stage.ensure((size_t)ngw * nb));
-  HIPCHK(hipMemcpy2DAsync(s->stage.p, sizeof(cx<double>) * ngw, eigvec, sizeof(cx<double>) * ld,
-                          sizeof(cx<double>) * ngw, nb, hipMemcpyHostToDevice, g.stream));
-  CHK(s->X.ensure((size_t)nb * ldp));
   CHK(s->rho_q.ensure((size_t)nb * kBandSlices));
   CHK(s->rho_k.ensure((size_t)nb * kBandSlices));
   CHK(s->W64.ensure((size_t)nb * s->nr[2] * s->maps.ncol_pad));
@@ -1157,14 +1151,14 @@ int pwh_eval_rho_psi(pwh_system* s, const double* eigvec, int ld, int nstates, c
     Timed tall(TM_RHO, g.stream);
     {
       Timed t(TM_RHO_SUMS, g.stream);
-      HIPCHK(permute_in_f64(s->stage.p, ngw, s->d_perm.p, ngw, ldp, s->X.p, nb, g.stream));
-      HIPCHK(hipMemset2DAsync(s->X.p + last, sizeof(cx<double>) * ldp, 0, sizeof(cx<double>), nb, g.stream));
-      HIPCHK(band_sums(s->X.p, ldp, ngw, s->kin64.p, nb, s->rho_q.p, s->rho_k.p, g.stream));
+      if (stage) HIPCHK(permute_in_f64(stage, ngw, s->d_perm.p, ngw, ldp, Xw, nb, g.stream));
+      HIPCHK(hipMemset2DAsync(Xw + last, sizeof(cx<double>) * ldp, 0, sizeof(cx<double>), nb, g.stream));
+      HIPCHK(band_sums(Xw, ldp, ngw, s->kin64.p, nb, s->rho_q.p, s->rho_k.p, g.stream));
     }
     {
       // inverse transform along i3, then the planes with w |psi|^2 summed over bands (FP64)
       Timed t(TM_RHO_ZINV, g.stream);
-      HIPCHK(launch_zinv<double>(s->maps, s->tw64[2].p, s->X.p, ldp, s->W64.p, nb, g.stream));
+      HIPCHK(launch_zinv<double>(s->maps, s->tw64[2].p, Xw, ldp, s->W64.p, nb, g.stream));
     }
     Timed t(TM_RHO_PLANE, g.stream);
     HIPCHK(launch_plane_rho(s->maps, s->tw64[0].p, s->tw64[1].p, s->W64.p, nb, groups, s->rho_w.p,
@@ -1190,6 +1184,33 @@ int pwh_eval_rho_psi(pwh_system* s, const double* eigvec, int ld, int nstates, c
   }
   *ekin = e;
   return PWH_OK;
+}
+
+int pwh_eval_rho_psi(pwh_system* s, const double* eigvec, int ld, int nstates, const double* w, double omega,
+                     double* rhoe, double* ekin, double* charge) {
+  CHK(check_ld(s, nstates));
+  if (!eigvec || !w || !rhoe || !ekin || !charge || ld < s->ngw || !(omega > 0.0))
+    return fail(PWH_ERR_ARG, "bad eval_rho_psi arguments");
+  const int nb = nstates, ngw = s->ngw;
+  // coefficients in sphere order; the reference's loop stops at ig < NGW_ik, so the last
+  // plane wave contributes to neither the density nor the sums (EvalRhoPsi.cpp:117)
+  CHK(s->stage.ensure((size_t)ngw * nb));
+  HIPCHK(hipMemcpy2DAsync(s->stage.p, sizeof(cx<double>) * ngw, eigvec, sizeof(cx<double>) * ld,
+                          sizeof(cx<double>) * ngw, nb, hipMemcpyHostToDevice, g.stream));
+  CHK(s->X.ensure((size_t)nb * s->ngw_pad));
+  return eval_rho_core(s, s->stage.p, s->X.p, nb, w, omega, rhoe, ekin, charge);
+}
+
+int pwh_resident_eval_rho_psi(pwh_system* s, int nstates, const double* w, double omega, double* rhoe,
+                              double* ekin, double* charge) {
+  CHK(check_ld(s, nstates));
+  if (!w || !rhoe || !ekin || !charge || !(omega > 0.0)) return fail(PWH_ERR_ARG, "bad eval_rho_psi arguments");
+  if (nstates > s->resident_nb) return fail(PWH_ERR_ARG, "more states than resident");
+  // a copy of the first nstates rows: the resident block keeps its last plane wave
+  const size_t n = (size_t)nstates * s->ngw_pad;
+  CHK(s->dtmp.ensure(n));
+  HIPCHK(hipMemcpyAsync(s->dtmp.p, s->X.p, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, g.stream));
+  return eval_rho_core(s, nullptr, s->dtmp.p, nstates, w, omega, rhoe, ekin, charge);
 }
 
 int pwh_lanczos(pwh_system* s, double* v0, int k, double* T, int ldT, double* f) {
@@ -1219,10 +1240,12 @@ int pwh_residual_norms(pwh_system* s, const double* X, int nstates, double* res)
   return residuals_dev(s, s->X.p, nstates, res);
 }
 
-int pwh_chfsi(pwh_system* s, double* X, int nstates, double* v0, pwh_chfsi_opts* o, double* eig,
-              double* res) {
+// X: host block to load and store, or null for the resident block (first nstates rows)
+static int chfsi_impl(pwh_system* s, double* X, int nstates, double* v0, pwh_chfsi_opts* o, double* eig,
+                      double* res) {
   CHK(check_ld(s, nstates));
-  if (!X || !o || !eig || !res) return fail(PWH_ERR_ARG, "null argument");
+  if (!o || !eig || !res) return fail(PWH_ERR_ARG, "null argument");
+  if (!X && nstates > s->resident_nb) return fail(PWH_ERR_ARG, "more states than resident");
   if (o->degree < 1 || o->niter < 0) return fail(PWH_ERR_ARG, "bad degree / niter");
   if (o->lanczos_k > 0 && !v0) return fail(PWH_ERR_ARG, "Lanczos needs v0");
   const int nb = nstates, NX = nstates - 2;
@@ -1263,7 +1286,7 @@ int pwh_chfsi(pwh_system* s, double* X, int nstates, double* v0, pwh_chfsi_opts*
       lb = w[0] + frac * (ub - w[0]);
     }
   }
-  if ((rc = load_block(s, X, nb))) goto done;
+  if (X && (rc = load_block(s, X, nb))) goto done;
   for (int it = 0; it < o->niter; ++it) {
     double ta = now_s();
     if ((rc = filter_prec(s, s->X.p, nb, o->degree, lb, ub, o->precision))) goto done;
@@ -1291,7 +1314,14 @@ int pwh_chfsi(pwh_system* s, double* X, int nstates, double* v0, pwh_chfsi_opts*
   }
   // residual report of the final block
   if ((rc = residuals_dev(s, s->X.p, nb, res))) goto done;
-  if ((rc = store_block(s, s->X.p, X, nb))) goto done;
+  if (X) {
+    if ((rc = store_block(s, s->X.p, X, nb))) goto done;
+  } else {
+    // keep the resident FP32 copy (pwh_resident_apply) equal to the new block
+    const size_t n = (size_t)s->resident_nb * s->ngw_pad;
+    if ((rc = s->X32.ensure(n))) goto done;
+    HIPCHK((convert_cx<double, float>(s->X.p, s->X32.p, n, g.stream)));
+  }
   o->lb = lb;
   o->ub = ub;
   o->t_total_s = now_s() - t0;
@@ -1299,6 +1329,16 @@ done:
   dv0.release();
   df.release();
   return rc;
+}
+
+int pwh_chfsi(pwh_system* s, double* X, int nstates, double* v0, pwh_chfsi_opts* o, double* eig,
+              double* res) {
+  if (!X) return fail(PWH_ERR_ARG, "null argument");
+  return chfsi_impl(s, X, nstates, v0, o, eig, res);
+}
+
+int pwh_resident_chfsi(pwh_system* s, int nstates, double* v0, pwh_chfsi_opts* o, double* eig, double* res) {
+  return chfsi_impl(s, nullptr, nstates, v0, o, eig, res);
 }
 
 // ---------------------------------------------------------------------------
@@ -2167,16 +2207,10 @@ int pwh_mix_broyden(int iter, double alpha, double w0, double* rho, double* rho_
   return scf_sync();
 }
 
-int pwh_eval_enl(pwh_system* s, const double* c0, int ld, int nstates, const double* focc, double wkpt,
-                 double* fnl, double* enl) {
-  CHK(check_ld(s, nstates));
-  if (!c0 || !focc || !enl || ld < s->ngw) return fail(PWH_ERR_ARG, "bad eval_enl arguments");
-  const int nb = nstates, ldp = s->ngw_pad;
-  CHK(s->stage.ensure((size_t)s->ngw * nb));
-  HIPCHK(hipMemcpy2DAsync(s->stage.p, sizeof(cx<double>) * s->ngw, c0, sizeof(cx<double>) * ld,
-                          sizeof(cx<double>) * s->ngw, nb, hipMemcpyHostToDevice, g.stream));
-  CHK(s->X.ensure((size_t)nb * ldp));
-  HIPCHK(permute_in_f64(s->stage.p, s->ngw, s->d_perm.p, s->ngw, ldp, s->X.p, nb, g.stream));
+// ENL (EvalENL.cpp:5-49) of nb states held in sphere order in Xs (read only)
+static int eval_enl_core(pwh_system* s, const cx<double>* Xs, int nb, const double* focc, double wkpt,
+                         double* fnl, double* enl) {
+  const int ldp = s->ngw_pad;
   CHK(upload(s->rho_w, focc, nb));
   const int nproj = s->poff.empty() ? 0 : s->poff.back() + s->na.back() * s->lm.back();
   std::vector<double> rows(std::max(nproj, 1), 0.0);
@@ -2189,7 +2223,7 @@ int pwh_eval_enl(pwh_system* s, const double* c0, int ld, int nstates, const dou
     const size_t pe = proj_part_elems(M, nb, s->ngw);
     CHK(s->part64.ensure(pe));
     BetaGen bg{s->Bt64.p + (size_t)s->poff[sp] * ldp, M, s->lm[sp], ldp};
-    HIPCHK(gemm_proj<double>(bg, s->X.p, ldp, nb, s->ngw, s->Q64.p + (size_t)s->poff[sp] * nb, s->part64.p, pe,
+    HIPCHK(gemm_proj<double>(bg, Xs, ldp, nb, s->ngw, s->Q64.p + (size_t)s->poff[sp] * nb, s->part64.p, pe,
                              PREC_FP64, g.stream));
   }
   if (nproj > 0) {
@@ -2209,6 +2243,26 @@ int pwh_eval_enl(pwh_system* s, const double* c0, int ld, int nstates, const dou
   }
   *enl = e;
   return PWH_OK;
+}
+
+int pwh_eval_enl(pwh_system* s, const double* c0, int ld, int nstates, const double* focc, double wkpt,
+                 double* fnl, double* enl) {
+  CHK(check_ld(s, nstates));
+  if (!c0 || !focc || !enl || ld < s->ngw) return fail(PWH_ERR_ARG, "bad eval_enl arguments");
+  const int nb = nstates, ldp = s->ngw_pad;
+  CHK(s->stage.ensure((size_t)s->ngw * nb));
+  HIPCHK(hipMemcpy2DAsync(s->stage.p, sizeof(cx<double>) * s->ngw, c0, sizeof(cx<double>) * ld,
+                          sizeof(cx<double>) * s->ngw, nb, hipMemcpyHostToDevice, g.stream));
+  CHK(s->X.ensure((size_t)nb * ldp));
+  HIPCHK(permute_in_f64(s->stage.p, s->ngw, s->d_perm.p, s->ngw, ldp, s->X.p, nb, g.stream));
+  return eval_enl_core(s, s->X.p, nb, focc, wkpt, fnl, enl);
+}
+
+int pwh_resident_eval_enl(pwh_system* s, int nstates, const double* focc, double wkpt, double* fnl, double* enl) {
+  CHK(check_ld(s, nstates));
+  if (!focc || !enl) return fail(PWH_ERR_ARG, "bad eval_enl arguments");
+  if (nstates > s->resident_nb) return fail(PWH_ERR_ARG, "more states than resident");
+  return eval_enl_core(s, s->X.p, nstates, focc, wkpt, fnl, enl);
 }
 
 // ---------------------------------------------------------------------------

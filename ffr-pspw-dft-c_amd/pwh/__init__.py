@@ -45,6 +45,7 @@ EXPORTS = [
     "pwh_eval_energy", "pwh_xc_pz", "pwh_normalize_rho", "pwh_mix_simple", "pwh_mix_kerker",
     "pwh_kerker_prec", "pwh_mix_anderson", "pwh_mix_pulay", "pwh_mix_broyden", "pwh_eval_enl",
     "pwh_phase_factor_gpu", "pwh_structure_factor", "pwh_nl_form_factor", "pwh_form_factor",
+    "pwh_resident_chfsi", "pwh_resident_eval_rho_psi", "pwh_resident_eval_enl",
 ]
 
 
@@ -142,6 +143,9 @@ def lib():
         "pwh_rayleigh_ritz": [vp, vp, C.c_int, _dp],
         "pwh_residual_norms": [vp, vp, C.c_int, _dp],
         "pwh_chfsi": [vp, vp, C.c_int, vp, C.POINTER(ChfsiOpts), _dp, _dp],
+        "pwh_resident_chfsi": [vp, C.c_int, vp, C.POINTER(ChfsiOpts), _dp, _dp],
+        "pwh_resident_eval_rho_psi": [vp, C.c_int, _dp, C.c_double, _dp, C.POINTER(C.c_double), _dp],
+        "pwh_resident_eval_enl": [vp, C.c_int, _dp, C.c_double, vp, C.POINTER(C.c_double)],
         "pwh_resident_set": [vp, vp, C.c_int],
         "pwh_resident_get": [vp, vp, C.c_int],
         "pwh_resident_apply": [vp, C.c_int, C.c_int],
@@ -581,6 +585,36 @@ class System:
     def resident_chebyfilt(self, nstates, degree, lb, ub, precision=FP64):
         _check(lib().pwh_resident_chebyfilt(self.h, int(nstates), int(degree), float(lb),
                                             float(ub), _prec(precision)))
+
+    def resident_chfsi(self, nstates, degree=10, niter=1, precision=FP64, lanczos_k=20, bounds_mode=0,
+                       v0=None, lb=0.0, ub=0.0):
+        """chfsi on the first nstates resident states; the Ritz vectors stay resident."""
+        if v0 is None:
+            v0 = np.ones(self.ngw, np.complex128)
+        v0 = np.array(np.asarray(v0, np.complex128).reshape(-1), order="C")
+        o = ChfsiOpts(int(degree), int(niter), _prec(precision), int(lanczos_k), int(bounds_mode),
+                      float(lb), float(ub), 0, 0, 0, 0, 0, 0)
+        eig = np.zeros(nstates)
+        res = np.zeros(nstates)
+        _check(lib().pwh_resident_chfsi(self.h, int(nstates), _zptr(v0), C.byref(o), eig, res))
+        return eig, res, {f: getattr(o, f) for f, _ in ChfsiOpts._fields_}
+
+    def resident_eval_rho_psi(self, nstates, w, omega, rhoe=None):
+        """eval_rho_psi on the first nstates resident states."""
+        rho = np.zeros(self.nnr) if rhoe is None else np.array(rhoe, dtype=np.float64)
+        charge = np.zeros(nstates)
+        ek = C.c_double(0.0)
+        _check(lib().pwh_resident_eval_rho_psi(self.h, int(nstates), np.ascontiguousarray(w, dtype=np.float64),
+                                               float(omega), rho, C.byref(ek), charge))
+        return rho, ek.value, charge
+
+    def resident_eval_enl(self, nstates, focc, wkpt):
+        """eval_enl on the first nstates resident states: returns (ENL, fnl [nproj][nstates])."""
+        enl = C.c_double()
+        fnl = np.zeros((max(self.nproj, 1), nstates), np.complex128)
+        _check(lib().pwh_resident_eval_enl(self.h, int(nstates), np.ascontiguousarray(focc, dtype=np.float64),
+                                           float(wkpt), _zptr(fnl), C.byref(enl)))
+        return enl.value, fnl[:self.nproj]
 
 
 class Batch:

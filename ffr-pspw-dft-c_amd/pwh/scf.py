@@ -46,7 +46,7 @@ class ScfResult:
 
 def cheby_scf(sysm, grid, X0, nocc, focc, omega, rho0, degree=10, precision=FP64, mixer="anderson",
               beta=0.3, max_iter=60, conv=1e-10, lanczos_k=20, v0=None, esr=0.0, eself=0.0,
-              mixdim=4, kerker_q0=1.0, chfsi_iters=1, seed=1, fp64_refine=None):
+              mixdim=4, kerker_q0=1.0, chfsi_iters=1, seed=1, fp64_refine=None, resident=True):
     """Runs the SCF loop from start vectors X0 (nstates, ngw) and density rho0 [nnr].
     nocc: occupied states (the first nocc Ritz vectors, ascending); focc [nocc]: their
     occupations times the k-point weight.  mixer: simple, kerker, anderson or broyden
@@ -58,7 +58,9 @@ def cheby_scf(sysm, grid, X0, nocc, focc, omega, rho0, degree=10, precision=FP64
     fp64_refine: with a mixed-precision filter, the |dE| at which the loop switches the
     filter to FP64 for the remaining iterations.  An FP32-filtered subspace carries
     FP32-sized noise, so the converged energy wanders by ~1e-7 relative to its size between
-    iterations; the last iterations in FP64 remove it (None: no switch).  Stops when
+    iterations; the last iterations in FP64 remove it (None: no switch).  resident: keep the
+    block in HBM for the whole loop (pwh_resident_chfsi / _eval_rho_psi / _eval_enl);
+    otherwise every step passes the host block in and out.  Stops when
     |ETOT - ETOT_previous| <= conv (SCF_CONV_CRIT) or after max_iter iterations."""
     import time
     t0 = time.perf_counter()
@@ -89,15 +91,26 @@ def cheby_scf(sysm, grid, X0, nocc, focc, omega, rho0, degree=10, precision=FP64
     hist, t_chfsi, converged, it = [], 0.0, False, 0
     eig = None
     prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    nst = X.shape[0]
+    if resident:
+        sysm.resident_set(X)
     for it in range(1, max_iter + 1):
         sysm.set_vloc(veff)
         ta = time.perf_counter()
-        eig, X, _, st = sysm.chfsi(X, degree=degree, niter=chfsi_iters, precision=prec,
-                                   lanczos_k=lanczos_k if it == 1 else 0, bounds_mode=0, v0=v0, lb=lb, ub=ub)
+        ck = dict(degree=degree, niter=chfsi_iters, precision=prec, lanczos_k=lanczos_k if it == 1 else 0,
+                  bounds_mode=0, v0=v0, lb=lb, ub=ub)
+        if resident:
+            eig, _, st = sysm.resident_chfsi(nst, **ck)
+        else:
+            eig, X, _, st = sysm.chfsi(X, **ck)
         t_chfsi += time.perf_counter() - ta
         lb, ub = st["lb"], st["ub"]
-        rnew, ekin, _ = sysm.eval_rho_psi(X[:nocc], focc, omega)
-        enl, _ = sysm.eval_enl(X[:nocc], focc, 1.0)
+        if resident:
+            rnew, ekin, _ = sysm.resident_eval_rho_psi(nocc, focc, omega)
+            enl, _ = sysm.resident_eval_enl(nocc, focc, 1.0)
+        else:
+            rnew, ekin, _ = sysm.eval_rho_psi(X[:nocc], focc, omega)
+            enl, _ = sysm.eval_enl(X[:nocc], focc, 1.0)
         rho = rnew
         if mixer == "anderson":
             d = mix_anderson(it, beta, rho, *state)
@@ -116,5 +129,7 @@ def cheby_scf(sysm, grid, X0, nocc, focc, omega, rho0, degree=10, precision=FP64
             break
         if fp64_refine is not None and prec != FP64 and detot <= fp64_refine:
             prec = FP64
+    if resident:
+        X = sysm.resident_get(nst)
     return ScfResult(etot=e["etot"], energies=e, eig=eig, X=X, rho=rho, iterations=it, converged=converged,
                      history=hist, t_chfsi_s=t_chfsi, t_total_s=time.perf_counter() - t0)

@@ -285,6 +285,17 @@ int pwh_resident_apply(pwh_system *sys, int nstates, int precision);
 /* Chebyshev filter of the resident block (FP32 modes convert in and out once). */
 int pwh_resident_chebyfilt(pwh_system *sys, int nstates, int degree, double lb, double ub,
                            int precision);
+/* pwh_chfsi (ChebySCFLoop.cpp:271-311) on the first nstates resident states: the Ritz
+ * vectors stay resident (and its FP32 copy is refreshed); eig/res come back. */
+int pwh_resident_chfsi(pwh_system *sys, int nstates, double *v0, pwh_chfsi_opts *opts, double *eig,
+                       double *res);
+/* pwh_eval_rho_psi (EvalRhoPsi.cpp:83-157) on the first nstates resident states (the
+ * resident block is left as it is). */
+int pwh_resident_eval_rho_psi(pwh_system *sys, int nstates, const double *w, double omega,
+                              double *rhoe, double *ekin, double *charge);
+/* pwh_eval_enl (EvalENL.cpp:5-49) on the first nstates resident states. */
+int pwh_resident_eval_enl(pwh_system *sys, int nstates, const double *focc, double wkpt, double *fnl,
+                          double *enl);
 int pwh_synchronize(void);
 
 /* ---- many uneven systems through one captured filter ------------------------ */

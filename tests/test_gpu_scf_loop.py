@@ -124,6 +124,37 @@ def test_scf_converges_and_mixed_total_energy_within_1e6(scf_setup):
     assert np.abs(r.eig[:NOCC] - r64.eig[:NOCC]).max() < 1e-9
 
 
+def test_scf_resident_and_host_paths_agree(scf_setup):
+    """The resident entry points (block kept in HBM) and the host-block ones give the same
+    steps: ETOT per step to 1e-10 relative, final eigenvalues to 1e-10 Ha."""
+    a = _run(scf_setup, "fp64", max_iter=6, conv=0.0, resident=True)
+    b = _run(scf_setup, "fp64", max_iter=6, conv=0.0, resident=False)
+    ea, eb = np.array([h[1] for h in a.history]), np.array([h[1] for h in b.history])
+    assert np.abs(ea - eb).max() < 1e-10 * abs(eb[-1])
+    assert np.abs(a.eig - b.eig).max() < 1e-10
+
+
+def test_resident_entry_points_match_host_ones(scf_setup):
+    import pwh
+    sysm, grid, X0, rho0, omega = scf_setup
+    focc = np.full(NOCC, 2.0)
+    sysm.resident_set(X0)
+    before = sysm.resident_get(NST)
+    r1, k1, c1 = sysm.resident_eval_rho_psi(NOCC, focc, omega)
+    e1, f1 = sysm.resident_eval_enl(NOCC, focc, 0.5)
+    # the resident block is unchanged by both (the density works on a copy)
+    assert np.array_equal(sysm.resident_get(NST), before)
+    r2, k2, c2 = sysm.eval_rho_psi(X0[:NOCC], focc, omega)
+    e2, f2 = sysm.eval_enl(X0[:NOCC], focc, 0.5)
+    assert np.array_equal(r1, r2) and k1 == k2 and np.array_equal(c1, c2)
+    assert e1 == e2 and np.array_equal(f1, f2)
+    sysm.resident_set(X0)
+    Xr = sysm.resident_get(NST)
+    assert np.abs(Xr - X0).max() == 0.0
+    with pytest.raises(pwh.PwhError):
+        sysm.resident_eval_rho_psi(NST + 1, np.full(NST + 1, 2.0), omega)
+
+
 def test_scf_driver_rejects_bad_inputs(scf_setup):
     from pwh.scf import cheby_scf
     sysm, grid, X0, rho0, omega = scf_setup
