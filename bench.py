@@ -478,6 +478,26 @@ def c4_rates(steps):
     for p in ("fp64", "fp32", "bf16x3"):
         dt = time_steps(lambda: x.resident_apply(s.nbands, p), steps, 1)
         out[p] = {"ms_per_block": round(1e3 * dt / steps, 3), "band_applies_per_s": round(s.nbands * steps / dt, 1)}
+    # ChFSI at degree 12 (FP64 orthonormalisation and Rayleigh-Ritz; rocSOLVER's Cholesky
+    # and zheevd at n = 432): five outer iterations per filter precision from the same
+    # orthonormal random start, 20-step FP64 Lanczos bounds.  The eigenvalue difference is
+    # given for all states and for the lower half (the upper states near lb are still far
+    # from converged after five iterations)
+    X0 = pwh.ortho_qr(psi)
+    v0 = rng.random(s.ngw) + 1j * rng.random(s.ngw)
+    x.chfsi(X0, degree=2, niter=1, precision="fp32", lanczos_k=4, v0=v0)   # first-call set-up
+    eigs = {}
+    for p in ("fp64", "fp32"):
+        eig, _, res, st = x.chfsi(X0, degree=12, niter=5, precision=p, lanczos_k=20, v0=v0)
+        eigs[p] = eig
+        out[f"chfsi_{p}"] = {"s_per_iter": round((st["t_total_s"] - st["t_lanczos_s"]) / 5, 4),
+                             "filter_s_per_iter": round(st["t_filter_s"] / 5, 4),
+                             "ortho_s_per_iter": round(st["t_ortho_s"] / 5, 4),
+                             "rr_s_per_iter": round(st["t_rr_s"] / 5, 4)}
+    nx = s.nbands - 2
+    d = np.abs(eigs["fp32"][:nx] - eigs["fp64"][:nx])
+    out["chfsi_fp32_vs_fp64_after_5_iterations"] = {"max_abs_eig_diff_Ha": float(d.max()),
+                                                    "lower_half_max_abs_eig_diff_Ha": float(d[:nx // 2].max())}
     x.close()
     return out
 
