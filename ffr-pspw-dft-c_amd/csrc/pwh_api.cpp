@@ -380,6 +380,7 @@ int chebyfilt_dev(pwh_system* s, cx<T>* b0, cx<T>* b1, cx<T>* b2, int nb, int de
 // by the FP32 steps is damped by the steps after it, except in the last one, which would
 // leave an FP32-sized error in the filtered block (it showed as a ~3e-7 Ha wander of the
 // converged SCF total energy).  prev/cur: FP32 iterates m-2 and m-1; out: FP64 result.
+// PWH_FILTER_LAST64=0 keeps every step in the filter's precision (for measurement).
 bool filter_last_fp64() {
   static const bool on = [] {
     const char* e = getenv("PWH_FILTER_LAST64");
