@@ -192,7 +192,7 @@ struct pwh_system {
   Buf<cx<double>> tw64[3];
   Buf<cx<float>> tw32[3];
   // FP64 work
-  Buf<cx<double>> X, HX, A64, B64, C64, W64, Hnl64, proj64, Q64, part64, stage, S, Vl;
+  Buf<cx<double>> X, HX, A64, B64, C64, W64, Hnl64, Q64, part64, stage, S, Vl;
   Buf<double> evals, rwork, dres;
   Buf<double> rho_part, rho, rho_w, rho_q, rho_k;   // density pass (pwh_eval_rho_psi)
   // block solvers (pwh_davidson, pwh_lobpcg): Q = [X | R] or [X | W | P] and H Q, one block
@@ -201,7 +201,7 @@ struct pwh_system {
   Buf<double> dlam, dnrm, dpart, dprec, dbo;
   Buf<rocblas_int> info;
   // FP32 work
-  Buf<cx<float>> X32, Y32, Y1_32, W32, Hnl32, proj32, Q32, part32;
+  Buf<cx<float>> X32, Y32, Y1_32, W32, Hnl32, Q32, part32;
   int resident_nb = 0;
   // second stream for the nonlocal GEMMs, which overlap the FFT passes (fork/join events)
   hipStream_t nl_stream = nullptr;
@@ -219,7 +219,6 @@ template <> struct Sel<double> {
   static void* tw(pwh_system* s, int d) { return s->tw64[d].p; }
   static Buf<cx<double>>& W(pwh_system* s) { return s->W64; }
   static Buf<cx<double>>& Hnl(pwh_system* s) { return s->Hnl64; }
-  static Buf<cx<double>>& proj(pwh_system* s) { return s->proj64; }
   static Buf<cx<double>>& Q(pwh_system* s) { return s->Q64; }
   static Buf<cx<double>>& part(pwh_system* s) { return s->part64; }
 };
@@ -231,7 +230,6 @@ template <> struct Sel<float> {
   static void* tw(pwh_system* s, int d) { return s->tw32[d].p; }
   static Buf<cx<float>>& W(pwh_system* s) { return s->W32; }
   static Buf<cx<float>>& Hnl(pwh_system* s) { return s->Hnl32; }
-  static Buf<cx<float>>& proj(pwh_system* s) { return s->proj32; }
   static Buf<cx<float>>& Q(pwh_system* s) { return s->Q32; }
   static Buf<cx<float>>& part(pwh_system* s) { return s->part32; }
 };
@@ -270,7 +268,6 @@ int hpsi_prepare(pwh_system* s, int nb) {
     const int M = s->na[sp] * s->lm[sp];
     CHK(Sel<T>::Hnl(s).ensure((size_t)nb * ld));
     const size_t nproj = (size_t)s->poff.back() + (size_t)s->na.back() * s->lm.back();
-    CHK(Sel<T>::proj(s).ensure(nproj * nb));
     CHK(Sel<T>::Q(s).ensure(nproj * nb));
     CHK(Sel<T>::part(s).ensure(proj_part_elems(M, nb, s->ngw)));
   }
@@ -302,19 +299,17 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
     const int M = s->na[sp] * s->lm[sp];
     CHK(Sel<T>::Hnl(s).ensure((size_t)nb * ld));
     const size_t nproj = (size_t)s->poff.back() + (size_t)s->na.back() * s->lm.back();
-    CHK(Sel<T>::proj(s).ensure(nproj * nb));
     CHK(Sel<T>::Q(s).ensure(nproj * nb));
     const size_t pe = proj_part_elems(M, nb, s->ngw);
     CHK(Sel<T>::part(s).ensure(pe));
     BetaGen bg{Sel<T>::Bt(s) + (size_t)s->poff[sp] * ld, M, s->lm[sp], ld};
     hipStream_t sp_st = sn;
     {
+      // <beta|psi> scaled by WNL straight into Q by the split-K reduction
       Timed t(TM_PROJ, sp_st);
-      HIPCHK(gemm_proj<T>(bg, X, ld, nb, s->ngw, Sel<T>::proj(s).p + (size_t)s->poff[sp] * nb,
-                          Sel<T>::part(s).p, pe, mode, sp_st));
+      HIPCHK(gemm_proj<T>(bg, X, ld, nb, s->ngw, Sel<T>::Q(s).p + (size_t)s->poff[sp] * nb, Sel<T>::part(s).p,
+                          pe, mode, sp_st, Sel<T>::wl(s) + 4 * sp));
     }
-    HIPCHK(scale_proj<T>(Sel<T>::proj(s).p + (size_t)s->poff[sp] * nb, Sel<T>::wl(s) + 4 * sp, s->na[sp],
-                         s->lm[sp], nb, Sel<T>::Q(s).p + (size_t)s->poff[sp] * nb, sp_st));
   }
   bool have_nl = false;
   for (int sp = 0; sp < s->nsp; ++sp) {
@@ -982,13 +977,13 @@ int pwh_system_destroy(pwh_system* s) {
   if (!s) return PWH_OK;
   if (g.stream) hipStreamSynchronize(g.stream);
   for (auto* b : {&s->d_perm, &s->d_colstart, &s->d_kcol, &s->d_kz, &s->d_colxy, &s->d_xa}) b->release();
-  for (auto* b : {&s->Bt64, &s->X, &s->HX, &s->A64, &s->B64, &s->C64, &s->W64, &s->Hnl64, &s->proj64,
+  for (auto* b : {&s->Bt64, &s->X, &s->HX, &s->A64, &s->B64, &s->C64, &s->W64, &s->Hnl64,
                   &s->Q64, &s->part64, &s->stage, &s->S, &s->Vl})
     b->release();
   for (auto* b : {&s->kin64, &s->wl64, &s->V64, &s->evals, &s->rwork, &s->dres, &s->rho_part, &s->rho,
                   &s->rho_w, &s->rho_q, &s->rho_k})
     b->release();
-  for (auto* b : {&s->Bt32, &s->X32, &s->Y32, &s->Y1_32, &s->W32, &s->Hnl32, &s->proj32, &s->Q32, &s->part32})
+  for (auto* b : {&s->Bt32, &s->X32, &s->Y32, &s->Y1_32, &s->W32, &s->Hnl32, &s->Q32, &s->part32})
     b->release();
   for (auto* b : {&s->kin32, &s->wl32, &s->V32}) b->release();
   for (int d = 0; d < 3; ++d) { s->tw64[d].release(); s->tw32[d].release(); }

@@ -605,7 +605,7 @@ static hipError_t run_proj(RowLoader<T, BETA> ld, const cx<T>* B, int ldb, int M
 
 template <typename T>
 hipError_t gemm_proj(const BetaGen& bg, const cx<T>* X, int ldx, int nb, int K, cx<T>* C,
-                     cx<T>* part, size_t part_elems, int mode, hipStream_t s) {
+                     cx<T>* part, size_t part_elems, int mode, hipStream_t s, const T* wl) {
   RowLoader<T, false> ld{};
   ld.A = static_cast<const cx<T>*>(bg.B);
   ld.lda = bg.ld;
@@ -615,7 +615,7 @@ hipError_t gemm_proj(const BetaGen& bg, const cx<T>* X, int ldx, int nb, int K, 
   if (e != hipSuccess) return e;
   const int tot = M * nb;
   hipLaunchKernelGGL((k_reduce_parts<T, T>), dim3((tot + 255) / 256), dim3(256), 0, s, part, S, M,
-                     nb, C, nb, static_cast<const T*>(nullptr), bg.lm, 0);
+                     nb, C, nb, wl, bg.lm, 0);
   return hipGetLastError();
 }
 
@@ -655,24 +655,6 @@ hipError_t gemm_nn_f64(const cx<double>* A, int lda, int J, const cx<double>* Q,
   ld.A = A; ld.lda = lda;
   dim3 grid(expand_blocks(K, N));
   return launch_expand<double, false, false, true>(gemm_ks<double>(), grid, s, ld, Q, ldq, J, N, K, out, ldo, 0);
-}
-
-// scale_proj: Q[(a,l)][b] = wl[l] * C[(a,l)][b]
-template <typename T>
-__global__ void k_scale_proj(const cx<T>* __restrict__ C, const T* __restrict__ wl, int lm, int rows,
-                             int nb, cx<T>* __restrict__ Q) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= rows * nb) return;
-  const int m = idx / nb;
-  Q[idx] = wl[m % lm] * C[idx];
-}
-
-template <typename T>
-hipError_t scale_proj(const cx<T>* Cin, const T* wl, int na, int lm, int nb, cx<T>* Q, hipStream_t s) {
-  const int tot = na * lm * nb;
-  hipLaunchKernelGGL((k_scale_proj<T>), dim3((tot + 255) / 256), dim3(256), 0, s, Cin, wl, lm, na * lm,
-                     nb, Q);
-  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -728,16 +710,12 @@ hipError_t mfma_selftest(int kind, double* max_err) {
 }
 
 template hipError_t gemm_proj<double>(const BetaGen&, const cx<double>*, int, int, int, cx<double>*,
-                                      cx<double>*, size_t, int, hipStream_t);
+                                      cx<double>*, size_t, int, hipStream_t, const double*);
 template hipError_t gemm_proj<float>(const BetaGen&, const cx<float>*, int, int, int, cx<float>*,
-                                     cx<float>*, size_t, int, hipStream_t);
+                                     cx<float>*, size_t, int, hipStream_t, const float*);
 template hipError_t gemm_expand<double>(const BetaGen&, const cx<double>*, int, int, cx<double>*, int,
                                         bool, int, hipStream_t);
 template hipError_t gemm_expand<float>(const BetaGen&, const cx<float>*, int, int, cx<float>*, int,
                                        bool, int, hipStream_t);
-template hipError_t scale_proj<double>(const cx<double>*, const double*, int, int, int, cx<double>*,
-                                       hipStream_t);
-template hipError_t scale_proj<float>(const cx<float>*, const float*, int, int, int, cx<float>*,
-                                      hipStream_t);
 
 }  // namespace pwh

@@ -107,9 +107,10 @@ struct BetaGen {
   const void* B;   // cx<T>
   int rows, lm, ld;
 };
+// C[(a,l)][b] = wl[l] * <beta_(a,l)|X_b> (wl null: unscaled), row-major [rows][nb]
 template <typename T>
 hipError_t gemm_proj(const BetaGen& bg, const cx<T>* X, int ldx, int nb, int K, cx<T>* C,
-                     cx<T>* part, size_t part_elems, int mode, hipStream_t s);
+                     cx<T>* part, size_t part_elems, int mode, hipStream_t s, const T* wl = nullptr);
 template <typename T>
 hipError_t gemm_expand(const BetaGen& bg, const cx<T>* Q, int nb, int K, cx<T>* out, int ldo,
                        bool accumulate, int mode, hipStream_t s);
@@ -129,7 +130,5 @@ hipError_t permute_in_f64(const cx<double>* ref, int ldr, const int* perm, int n
                           cx<double>* out, int nb, hipStream_t s);
 hipError_t permute_out_f64(const cx<double>* in, int ngw_pad, const int* perm, int ngw,
                            cx<double>* ref, int ldr, int nb, hipStream_t s);
-template <typename T>
-hipError_t scale_proj(const cx<T>* Cin, const T* wl, int na, int lm, int nb, cx<T>* Q, hipStream_t s);
 
 }  // namespace pwh
