@@ -347,7 +347,7 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
 template <typename T>
 int chebyfilt_dev(pwh_system* s, cx<T>* b0, cx<T>* b1, cx<T>* b2, int nb, int degree, double lb,
                   double ub, int mode, cx<T>** result, hipStream_t st = nullptr, cx<T>** previous = nullptr,
-                  double* sigma_out = nullptr) {
+                  double* sigma_out = nullptr, bool first_in_b1 = false) {
   const double e = (ub - lb) / 2.0;
   const double c = (ub + lb) / 2.0;
   double sigma = e / (lb - ub);
@@ -355,7 +355,8 @@ int chebyfilt_dev(pwh_system* s, cx<T>* b0, cx<T>* b1, cx<T>* b2, int nb, int de
   cx<T>* X = b0;
   cx<T>* Y = b1;
   cx<T>* Y1 = b2;
-  CHK(hpsi<T>(s, X, Y, nb, sigma1 / e, -c * sigma1 / e, 0.0, nullptr, mode, st));
+  // first iterate (H X - c X) sigma1 / e, unless the caller formed it in b1 already
+  if (!first_in_b1) CHK(hpsi<T>(s, X, Y, nb, sigma1 / e, -c * sigma1 / e, 0.0, nullptr, mode, st));
   for (int id = 2; id <= degree; ++id) {
     const double sigma2 = 1.0 / (2.0 / sigma1 - sigma);
     CHK(hpsi<T>(s, Y, Y1, nb, 2.0 * sigma2 / e, -c * 2.0 * sigma2 / e, -sigma * sigma2, X, mode, st));
@@ -432,16 +433,21 @@ int apply_prec(pwh_system* s, const cx<double>* X, cx<double>* out, int nb, int 
   return PWH_OK;
 }
 
-// Chebyshev filter of the FP64 sphere-order block X in place.
+// Chebyshev filter of the FP64 sphere-order block X in place.  HX: H X of this block in
+// FP64 if the caller has it (the rotated H X of the Rayleigh-Ritz step just before, under
+// the same potential); the first iterate is then formed from it without an H|psi>.
 int filter_prec(pwh_system* s, cx<double>* X, int nb, int degree, double lb, double ub,
-                int precision) {
+                int precision, const cx<double>* HX = nullptr) {
   const size_t n = (size_t)nb * s->ngw_pad;
   if (degree < 1) return PWH_OK;
+  const double e = (ub - lb) / 2.0, c = (ub + lb) / 2.0, sigma1 = e / (lb - ub);
+  CHK(s->A64.ensure(n));
+  CHK(s->B64.ensure(n));
   if (precision == PWH_FP64) {
-    CHK(s->A64.ensure(n));
-    CHK(s->B64.ensure(n));
+    if (HX) HIPCHK(axpby(HX, sigma1 / e, X, -c * sigma1 / e, s->A64.p, (int)n, g.stream));
     cx<double>* res = nullptr;
-    CHK(chebyfilt_dev<double>(s, X, s->A64.p, s->B64.p, nb, degree, lb, ub, PREC_FP64, &res));
+    CHK(chebyfilt_dev<double>(s, X, s->A64.p, s->B64.p, nb, degree, lb, ub, PREC_FP64, &res, g.stream,
+                              nullptr, nullptr, HX != nullptr));
     if (res != X) HIPCHK(hipMemcpyAsync(X, res, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, g.stream));
     return PWH_OK;
   }
@@ -449,17 +455,20 @@ int filter_prec(pwh_system* s, cx<double>* X, int nb, int degree, double lb, dou
   CHK(s->Y32.ensure(n));
   CHK(s->Y1_32.ensure(n));
   HIPCHK((convert_cx<double, float>(X, s->X32.p, n, g.stream)));
+  if (HX) {
+    HIPCHK(axpby(HX, sigma1 / e, X, -c * sigma1 / e, s->B64.p, (int)n, g.stream));
+    HIPCHK((convert_cx<double, float>(s->B64.p, s->Y32.p, n, g.stream)));
+  }
   cx<float>* res = nullptr;
   if (degree >= 2 && filter_last_fp64()) {
-    CHK(s->A64.ensure(n));
-    CHK(s->B64.ensure(n));
     cx<float>* prev = nullptr;
     double sigma = 0;
     CHK(chebyfilt_dev<float>(s, s->X32.p, s->Y32.p, s->Y1_32.p, nb, degree - 1, lb, ub, precision, &res,
-                             g.stream, &prev, &sigma));
+                             g.stream, &prev, &sigma, HX != nullptr));
     return cheby_last_step_fp64(s, prev, res, X, nb, lb, ub, sigma, g.stream);
   }
-  CHK(chebyfilt_dev<float>(s, s->X32.p, s->Y32.p, s->Y1_32.p, nb, degree, lb, ub, precision, &res));
+  CHK(chebyfilt_dev<float>(s, s->X32.p, s->Y32.p, s->Y1_32.p, nb, degree, lb, ub, precision, &res, g.stream,
+                           nullptr, nullptr, HX != nullptr));
   HIPCHK((convert_cx<float, double>(res, X, n, g.stream)));
   return PWH_OK;
 }
@@ -639,7 +648,8 @@ int rayleigh_ritz_dev(pwh_system* s, cx<double>* X, int nb) {
 }
 
 // Residual report (ChebySCFLoop.cpp:415-433) for X: HX, G = sym(X^H HX), R = HX - X G.
-int residuals_dev(pwh_system* s, const cx<double>* X, int nb, double* res_host) {
+// hx_fresh: s->HX already holds H X for this X (after rayleigh_ritz_dev, same potential)
+int residuals_dev(pwh_system* s, const cx<double>* X, int nb, double* res_host, bool hx_fresh = false) {
   const int ld = s->ngw_pad;
   const size_t n = (size_t)nb * ld;
   CHK(s->HX.ensure(n));
@@ -648,7 +658,7 @@ int residuals_dev(pwh_system* s, const cx<double>* X, int nb, double* res_host) 
   CHK(s->dres.ensure(nb));
   const size_t pe = proj_part_elems(nb, nb, s->ngw);
   CHK(s->part64.ensure(pe));
-  CHK(hpsi<double>(s, X, s->HX.p, nb, 1.0, 0.0, 0.0, nullptr, PREC_FP64));
+  if (!hx_fresh) CHK(hpsi<double>(s, X, s->HX.p, nb, 1.0, 0.0, 0.0, nullptr, PREC_FP64));
   HIPCHK(gemm_cn_f64(X, ld, nb, s->HX.p, ld, nb, s->ngw, s->S.p, nb, s->part64.p, pe, g.stream));
   HIPCHK(symmetrize(s->S.p, nb, nb, g.stream));
   HIPCHK(gemm_nn_f64(X, ld, nb, s->S.p, nb, nb, s->ngw, s->B64.p, ld, g.stream));
@@ -1285,7 +1295,9 @@ static int chfsi_impl(pwh_system* s, double* X, int nstates, double* v0, pwh_chf
   if (X && (rc = load_block(s, X, nb))) goto done;
   for (int it = 0; it < o->niter; ++it) {
     double ta = now_s();
-    if ((rc = filter_prec(s, s->X.p, nb, o->degree, lb, ub, o->precision))) goto done;
+    // from the second iteration on, H X of the block is the rotated one of the last RR
+    if ((rc = filter_prec(s, s->X.p, nb, o->degree, lb, ub, o->precision, it > 0 ? s->HX.p : nullptr)))
+      goto done;
     HIPCHK(hipStreamSynchronize(g.stream));
     o->t_filter_s += now_s() - ta;
     o->band_applies += (long long)o->degree * nb;
@@ -1309,7 +1321,7 @@ static int chfsi_impl(pwh_system* s, double* X, int nstates, double* v0, pwh_chf
     }
   }
   // residual report of the final block
-  if ((rc = residuals_dev(s, s->X.p, nb, res))) goto done;
+  if ((rc = residuals_dev(s, s->X.p, nb, res, o->niter > 0))) goto done;
   if (X) {
     if ((rc = store_block(s, s->X.p, X, nb))) goto done;
   } else {
