@@ -335,7 +335,7 @@ def scf_steps(s, sysm, psi, steps, cpu):
     return out
 
 
-def scf_loop(s):
+def scf_loop(s, tuned=None):
     """The Kohn-Sham loop on the GPU (pwh.scf.cheby_scf, shaped like ChebySCFLoop.cpp) on the
     cell's block shape: 128 states, the lowest 90 occupied (a closed shell of the synthetic
     cell's spectrum, so integer occupations converge), simple mixing 0.3, degree-10 ChFSI,
@@ -354,6 +354,8 @@ def scf_loop(s):
     grid = pwh.Grid(s.nr, gsd["n1"], gsd["n2"], gsd["n3"], gsd["g2"], gsd["sfac"], gsd["vps"], gsd["rhops"],
                     gsd["tpiba2"], gsd["omega"], n123=s.gv["n123"][:ngw], xkg=s.gv["xkg"][:ngw])
     sysm = pwh.System(s.globals())
+    for p, t in (tuned or {}).items():      # the split-K targets tuned on the same block shape
+        sysm.autotune(0, p, target=t)
     rng = np.random.default_rng(5)
     X0 = pwh.ortho_qr(rng.standard_normal((nst, ngw)) + 1j * rng.standard_normal((nst, ngw)))
     omega = gsd["omega"]
@@ -473,11 +475,13 @@ def c4_rates(steps):
     psi = rng.standard_normal((s.nbands, s.ngw)) + 1j * rng.standard_normal((s.nbands, s.ngw))
     psi /= np.linalg.norm(psi, axis=1, keepdims=True)
     x.resident_set(psi)
+    tuned = {p: x.autotune(s.nbands, p)[0] for p in ("fp64", "fp32", "bf16x3")}
     out = {"workload": f"si216: a={s.a} bohr, {s.natoms} atoms, grid {s.nr[0]}^3, {s.ngw} PW, "
                        f"{s.nbands} bands, {4 * s.natoms} projectors"}
     for p in ("fp64", "fp32", "bf16x3"):
         dt = time_steps(lambda: x.resident_apply(s.nbands, p), steps, 1)
-        out[p] = {"ms_per_block": round(1e3 * dt / steps, 3), "band_applies_per_s": round(s.nbands * steps / dt, 1)}
+        out[p] = {"ms_per_block": round(1e3 * dt / steps, 3), "band_applies_per_s": round(s.nbands * steps / dt, 1),
+                  "gemm_target_blocks": tuned[p]}
     # ChFSI at degree 12 (FP64 orthonormalisation and Rayleigh-Ritz; rocSOLVER's Cholesky
     # and zheevd at n = 432): five outer iterations per filter precision from the same
     # orthonormal random start, 20-step FP64 Lanczos bounds.  The eigenvalue difference is
@@ -513,6 +517,11 @@ def main():
     ap.add_argument("--no-chfsi", action="store_true", help="skip the ChFSI timing")
     ap.add_argument("--only", default="", help="fp64|fp32|bf16x3: time one path only (profiling)")
     ap.add_argument("--no-extra", action="store_true", help="skip the C3 batch and C4 sections")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default GEMM split-K target (no pwh_autotune search before timing)")
+    ap.add_argument("--targets", default="",
+                    help="p=N[,p=N]: set these split-K targets without the search (e.g. a profiled "
+                         "run repeating a tuned one, fp64=2048)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -544,6 +553,15 @@ def main():
     info = sysm.info()
 
     paths = ["fp64", "fp32", "bf16x3"] if not args.only else [args.only]
+    # set-up, outside the timed region: the projection GEMMs' split-K target per precision
+    fixed = {k: int(v) for k, v in (kv.split("=") for kv in args.targets.split(",") if kv)}
+    tuned = {}
+    for p in paths:
+        if p in fixed:
+            tuned[p] = sysm.autotune(nb, p, target=fixed[p])[0]
+        else:
+            tuned[p] = 512 if args.no_tune else sysm.autotune(nb, p)[0]
+    log("bench: gemm split-K block targets " + json.dumps(tuned))
     rates, ms_step = {}, {}
     for p in paths:
         dt = time_steps(lambda: sysm.resident_apply(nb, p), args.steps, args.warmup)
@@ -655,7 +673,7 @@ def main():
 
     loop = None
     if not args.only and not args.no_chfsi:
-        loop = scf_loop(s)
+        loop = scf_loop(s, tuned)
         log("bench: scf loop " + json.dumps(loop))
 
     setup = None
@@ -700,6 +718,7 @@ def main():
                    "path": main_path, "global_batch": nb, "parallelism": "single GPU"},
         "band_applies_per_s": {k: round(v, 1) for k, v in rates.items()},
         "ms_per_block": {k: round(v, 4) for k, v in ms_step.items()},
+        "gemm_target_blocks": tuned,
         "mixed_vs_fp64": round(rates.get("fp32", 0) / rates["fp64"], 3) if "fp64" in rates and "fp32" in rates else None,
         "roofline": roof,
         "kernels": per_kernel,

@@ -62,6 +62,9 @@ const char* kTimerNames[TM_COUNT] = {"zinv", "plane", "zfwd", "proj", "expand", 
 struct Ctx {
   bool ready = false;
   int device = 0;
+  // bumped whenever a work buffer moves or a launch shape is retuned: a captured batch
+  // graph holds buffer addresses and grid sizes, and is captured again after a change
+  unsigned long long layout_gen = 0;
   hipStream_t stream = nullptr;
   rocblas_handle rb = nullptr;
   bool timing = false;
@@ -144,6 +147,7 @@ struct Buf {
     }
     p = nullptr;
     cap = 0;
+    ++g.layout_gen;
     hipError_t e = hipMalloc(&p, sizeof(T) * std::max<size_t>(n, 1));
     if (e != hipSuccess) return fail(PWH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
     hipMemsetAsync(p, 0, sizeof(T) * std::max<size_t>(n, 1), g.stream);
@@ -203,6 +207,9 @@ struct pwh_system {
   // FP32 work
   Buf<cx<float>> X32, Y32, Y1_32, W32, Hnl32, Q32, part32;
   int resident_nb = 0;
+  // split-K block target of the projections per precision (PREC_FP64/FP32/BF16X3; 0: the
+  // library default), set by pwh_autotune
+  int tune[3] = {0, 0, 0};
   // second stream for the nonlocal GEMMs, which overlap the FFT passes (fork/join events)
   hipStream_t nl_stream = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -269,7 +276,11 @@ int hpsi_prepare(pwh_system* s, int nb) {
     CHK(Sel<T>::Hnl(s).ensure((size_t)nb * ld));
     const size_t nproj = (size_t)s->poff.back() + (size_t)s->na.back() * s->lm.back();
     CHK(Sel<T>::Q(s).ensure(nproj * nb));
-    CHK(Sel<T>::part(s).ensure(proj_part_elems(M, nb, s->ngw)));
+    size_t pe = 0;
+    for (int m = 0; m < 3; ++m)
+      if ((m == PREC_FP64) == std::is_same<T, double>::value)
+        pe = std::max(pe, proj_part_elems(M, nb, s->ngw, s->tune[m]));
+    CHK(Sel<T>::part(s).ensure(pe));
   }
   return PWH_OK;
 }
@@ -300,9 +311,11 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
     CHK(Sel<T>::Hnl(s).ensure((size_t)nb * ld));
     const size_t nproj = (size_t)s->poff.back() + (size_t)s->na.back() * s->lm.back();
     CHK(Sel<T>::Q(s).ensure(nproj * nb));
-    const size_t pe = proj_part_elems(M, nb, s->ngw);
+    const int target = s->tune[std::is_same<T, double>::value ? PREC_FP64
+                                                                : (mode == PREC_BF16X3 ? PREC_BF16X3 : PREC_FP32)];
+    const size_t pe = proj_part_elems(M, nb, s->ngw, target);
     CHK(Sel<T>::part(s).ensure(pe));
-    BetaGen bg{Sel<T>::Bt(s) + (size_t)s->poff[sp] * ld, M, s->lm[sp], ld};
+    BetaGen bg{Sel<T>::Bt(s) + (size_t)s->poff[sp] * ld, M, s->lm[sp], ld, target};
     hipStream_t sp_st = sn;
     {
       // <beta|psi> scaled by WNL straight into Q by the split-K reduction
@@ -2524,6 +2537,69 @@ int pwh_resident_apply(pwh_system* s, int nstates, int precision) {
   return hpsi<float>(s, s->X32.p, s->Y32.p, nstates, 1.0, 0.0, 0.0, nullptr, precision);
 }
 
+int pwh_autotune(pwh_system* s, int nstates, int precision, int* target_blocks, double* best_ms) {
+  if (!s) return fail(PWH_ERR_ARG, "null system");
+  if (precision < PWH_FP64 || precision > PWH_BF16X3) return fail(PWH_ERR_ARG, "bad precision");
+  if (target_blocks && *target_blocks != 0) {
+    // a target known from an earlier run: set it, time nothing
+    if (*target_blocks < 64 || *target_blocks > 65536) return fail(PWH_ERR_ARG, "target outside [64, 65536]");
+    if (s->tune[precision] != *target_blocks) ++g.layout_gen;
+    s->tune[precision] = *target_blocks;
+    if (best_ms) *best_ms = 0.0;
+    return PWH_OK;
+  }
+  CHK(check_ld(s, nstates));
+  if (nstates > s->resident_nb) return fail(PWH_ERR_ARG, "more states than resident");
+  if (!s->vset) return fail(PWH_ERR_ARG, "local potential not set (pwh_set_vloc)");
+  // split-K block targets of the projection GEMMs: their best value moves with the band
+  // count, the projector count and the precision (one to eight blocks per CU)
+  static const int cand[] = {256, 512, 1024, 2048};
+  const int nc = (int)(sizeof(cand) / sizeof(cand[0]));
+  const bool timing = g.timing;
+  g.timing = false;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = PWH_OK;
+  double ms[4] = {1e300, 1e300, 1e300, 1e300};
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    rc = fail(PWH_ERR_HIP, "hipEventCreate");
+  // two passes over the candidates, the smaller time of each kept, so a slow box moment
+  // does not pick the target
+  for (int pass = 0; pass < 2 && rc == PWH_OK; ++pass) {
+    for (int c = 0; c < nc && rc == PWH_OK; ++c) {
+      s->tune[precision] = cand[c];
+      const int reps = 5;
+      for (int r = 0; r < 2 && rc == PWH_OK; ++r) rc = pwh_resident_apply(s, nstates, precision);
+      if (rc != PWH_OK) break;
+      if (hipEventRecord(e0, g.stream) != hipSuccess) { rc = fail(PWH_ERR_HIP, "hipEventRecord"); break; }
+      for (int r = 0; r < reps && rc == PWH_OK; ++r) rc = pwh_resident_apply(s, nstates, precision);
+      if (rc != PWH_OK) break;
+      float el = 0.f;
+      if (hipEventRecord(e1, g.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&el, e0, e1) != hipSuccess) {
+        rc = fail(PWH_ERR_HIP, "event timing");
+        break;
+      }
+      ms[c] = std::min(ms[c], (double)el / reps);
+    }
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  g.timing = timing;
+  if (rc != PWH_OK) {
+    s->tune[precision] = 0;
+    return rc;
+  }
+  // the library default (512) unless another target is faster by more than 1%
+  int best = 1;
+  for (int c = 0; c < nc; ++c)
+    if (ms[c] < 0.99 * ms[best]) best = c;
+  s->tune[precision] = cand[best];
+  ++g.layout_gen;
+  if (target_blocks) *target_blocks = cand[best];
+  if (best_ms) *best_ms = ms[best];
+  return PWH_OK;
+}
+
 int pwh_resident_chebyfilt(pwh_system* s, int nstates, int degree, double lb, double ub,
                            int precision) {
   CHK(check_ld(s, nstates));
@@ -2543,6 +2619,7 @@ struct pwh_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int g_degree = -1, g_prec = -1;
+  unsigned long long g_gen = 0;   // layout generation the graph was captured at
   std::vector<double> g_lb, g_ub;
 };
 
@@ -2620,7 +2697,7 @@ int pwh_batch_chebyfilt(pwh_batch* b, int degree, const double* lb, const double
       CHK(filter_on_stream(b->sys[i], b->nb[i], degree, lb[i], ub[i], precision, g.stream));
     return PWH_OK;
   }
-  const bool same = b->exec && b->g_degree == degree && b->g_prec == precision &&
+  const bool same = b->exec && b->g_degree == degree && b->g_prec == precision && b->g_gen == g.layout_gen &&
                     std::equal(lb, lb + n, b->g_lb.begin()) && std::equal(ub, ub + n, b->g_ub.begin());
   if (!same) {
     batch_drop_graph(b);
@@ -2644,6 +2721,7 @@ int pwh_batch_chebyfilt(pwh_batch* b, int degree, const double* lb, const double
     HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     b->g_degree = degree;
     b->g_prec = precision;
+    b->g_gen = g.layout_gen;
     b->g_lb.assign(lb, lb + n);
     b->g_ub.assign(ub, ub + n);
   }

@@ -29,5 +29,6 @@ hipError_t fill_lower(cx<double>* A, int n, int ld, hipStream_t s);
 hipError_t zhegv_scale(cx<double>* V, int n, int ld, const double* b, hipStream_t s);
 hipError_t resid_norms(const cx<double>* H, const cx<double>* X, const double* lam, int ld, int n,
                        int nb, double* res, hipStream_t s);
-size_t proj_part_elems(int M, int N, int K);
+// split-K partial buffer (elements) of a projection; target: block target (0: default)
+size_t proj_part_elems(int M, int N, int K, int target = 0);
 }  // namespace pwh

@@ -496,10 +496,10 @@ static int gemm_target_blocks() {
   return t;
 }
 
-static int choose_split(int tiles, int K, size_t part_elems, size_t MN) {
-  // aim for ~2 blocks per CU on 256 CUs, at least 128 plane waves per block; a multiple
-  // of 8 where K allows, so the XCD-aware order keeps all 8 XCDs busy
-  int s = (gemm_target_blocks() + tiles - 1) / tiles;
+static int choose_split(int tiles, int K, size_t part_elems, size_t MN, int target = 0) {
+  // aim for ~2 blocks per CU on 256 CUs (or the tuned target), at least 128 plane waves
+  // per block; a multiple of 8 where K allows, so the XCD-aware order keeps all 8 XCDs busy
+  int s = ((target > 0 ? target : gemm_target_blocks()) + tiles - 1) / tiles;
   int smax = (K + 127) / 128;
   s = (s + 7) / 8 * 8;
   if (s > smax) s = smax;
@@ -508,9 +508,9 @@ static int choose_split(int tiles, int K, size_t part_elems, size_t MN) {
   return s;
 }
 
-size_t proj_part_elems(int M, int N, int K) {
+size_t proj_part_elems(int M, int N, int K, int target) {
   const int tiles = ((M + BT - 1) / BT) * ((N + BT - 1) / BT);
-  const int s = choose_split(tiles, K, (size_t)-1, (size_t)M * N);
+  const int s = choose_split(tiles, K, (size_t)-1, (size_t)M * N, target);
   return (size_t)s * M * N;
 }
 
@@ -583,10 +583,11 @@ static hipError_t launch_expand(int ks, dim3 grid, hipStream_t s, RowLoader<T, B
 
 template <typename T, bool BETA>
 static hipError_t run_proj(RowLoader<T, BETA> ld, const cx<T>* B, int ldb, int M, int N, int K,
-                           cx<T>* part, size_t part_elems, bool bf16, hipStream_t s, int* S_out) {
+                           cx<T>* part, size_t part_elems, bool bf16, hipStream_t s, int* S_out,
+                           int target = 0) {
   const int ks = gemm_ks<T>();
   const int tiles = ((M + BT - 1) / BT) * ((N + BT - 1) / BT);
-  int S = choose_split(tiles, K, part_elems, (size_t)M * N);
+  int S = choose_split(tiles, K, part_elems, (size_t)M * N, target);
   int kchunk = (K + S - 1) / S;
   kchunk = ((kchunk + ks - 1) / ks) * ks;
   S = (K + kchunk - 1) / kchunk;
@@ -611,7 +612,8 @@ hipError_t gemm_proj(const BetaGen& bg, const cx<T>* X, int ldx, int nb, int K, 
   ld.lda = bg.ld;
   const int M = bg.rows;
   int S = 0;
-  hipError_t e = run_proj<T, false>(ld, X, ldx, M, nb, K, part, part_elems, mode == PREC_BF16X3, s, &S);
+  hipError_t e = run_proj<T, false>(ld, X, ldx, M, nb, K, part, part_elems, mode == PREC_BF16X3, s, &S,
+                                    bg.target);
   if (e != hipSuccess) return e;
   const int tot = M * nb;
   hipLaunchKernelGGL((k_reduce_parts<T, T>), dim3((tot + 255) / 256), dim3(256), 0, s, part, S, M,

@@ -106,6 +106,7 @@ struct BetaGen {
   // row-major [rows = na * lm][ld] (row (a, l) at a * lm + l)
   const void* B;   // cx<T>
   int rows, lm, ld;
+  int target = 0;  // split-K block target of the projection (0: the library default)
 };
 // C[(a,l)][b] = wl[l] * <beta_(a,l)|X_b> (wl null: unscaled), row-major [rows][nb]
 template <typename T>

@@ -38,7 +38,7 @@ EXPORTS = [
     "pwh_system_create", "pwh_system_destroy", "pwh_system_info", "pwh_set_vloc",
     "pwh_apply_ham_block", "pwh_chebyfilt", "pwh_ortho_qr", "pwh_lanczos", "pwh_rayleigh_ritz",
     "pwh_residual_norms", "pwh_chfsi", "pwh_resident_set", "pwh_resident_get",
-    "pwh_resident_apply", "pwh_resident_chebyfilt", "pwh_synchronize", "pwh_timing_enable",
+    "pwh_resident_apply", "pwh_autotune", "pwh_resident_chebyfilt", "pwh_synchronize", "pwh_timing_enable",
     "pwh_timing_reset", "pwh_timing_get", "pwh_batch_create", "pwh_batch_destroy",
     "pwh_batch_set", "pwh_batch_get", "pwh_batch_chebyfilt", "pwh_fft3d", "pwh_eig_zheevd",
     "pwh_eval_rho_psi", "pwh_davidson", "pwh_lobpcg", "pwh_grid_create", "pwh_grid_destroy",
@@ -149,6 +149,7 @@ def lib():
         "pwh_resident_set": [vp, vp, C.c_int],
         "pwh_resident_get": [vp, vp, C.c_int],
         "pwh_resident_apply": [vp, C.c_int, C.c_int],
+        "pwh_autotune": [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)],
         "pwh_resident_chebyfilt": [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int],
         "pwh_synchronize": [],
         "pwh_timing_enable": [C.c_int],
@@ -581,6 +582,15 @@ class System:
 
     def resident_apply(self, nstates, precision=FP64):
         _check(lib().pwh_resident_apply(self.h, int(nstates), _prec(precision)))
+
+    def autotune(self, nstates, precision=FP64, target=0):
+        """Picks the projection GEMMs' split-K block target for H|psi> of the first nstates
+        resident states in this precision (pwh_autotune); returns (target, ms per H|psi>).
+        target > 0 sets that target without timing."""
+        t = C.c_int(int(target))
+        ms = C.c_double(0.0)
+        _check(lib().pwh_autotune(self.h, int(nstates), _prec(precision), C.byref(t), C.byref(ms)))
+        return t.value, ms.value
 
     def resident_chebyfilt(self, nstates, degree, lb, ub, precision=FP64):
         _check(lib().pwh_resident_chebyfilt(self.h, int(nstates), int(degree), float(lb),

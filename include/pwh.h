@@ -282,6 +282,16 @@ int pwh_resident_set(pwh_system *sys, const double *X, int nstates);
 int pwh_resident_get(pwh_system *sys, double *X, int nstates);
 /* One H|psi> over the resident block (nb bands) in the given precision. */
 int pwh_resident_apply(pwh_system *sys, int nstates, int precision);
+/* Split-K block target of the projection GEMMs for this system and precision.  With
+ * *target_blocks == 0 (or a null pointer) it times H|psi> of the resident block (nstates
+ * bands) for targets 256, 512, 1024 and 2048 and keeps the fastest (512, the default,
+ * unless another is faster by more than 1%), returning it in *target_blocks and its time
+ * per H|psi> in *best_ms (both optional).  With *target_blocks in [64, 65536] it sets that
+ * target without timing (nstates unused), e.g. one found by an earlier run.  Every later
+ * H|psi> of the system in that precision uses the target.  No counterpart in the
+ * reference: it tunes a launch shape of this library (the reference's GEMMs are cuBLAS
+ * calls, ApplyHam.cpp). */
+int pwh_autotune(pwh_system *sys, int nstates, int precision, int *target_blocks, double *best_ms);
 /* Chebyshev filter of the resident block (FP32 modes convert in and out once). */
 int pwh_resident_chebyfilt(pwh_system *sys, int nstates, int degree, double lb, double ub,
                            int precision);

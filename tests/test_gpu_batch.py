@@ -48,6 +48,22 @@ def test_batch_filter_matches_single_system(cells, oracle, prec):
             assert np.array_equal(B.get(i), single[i]), (mode, i)
         # eager work on the same systems between replays
         systems[0].apply_ham_block(X0[0][:2])
+    # a retuned projection target changes buffers and grids: the graph is captured again,
+    # so its replay equals the sequential schedule at the new target
+    for s in systems:
+        s.autotune(0, prec, target=2048)
+    seq = []
+    for mode in (0, 1, 1):
+        for i, x in enumerate(X0):
+            B.set(i, x)
+        B.chebyfilt(5, lb, ub, prec, mode)
+        pwh.synchronize()
+        out = [B.get(i) for i in range(len(cells))]
+        if mode == 0:
+            seq = out
+        for i in range(len(cells)):
+            assert np.array_equal(out[i], seq[i]), ("retuned", mode, i)
+            assert relerr(out[i], single[i]) < (1e-13 if prec == "fp64" else 1e-5)
     if prec == "fp64":
         c = cells[2]
         ref = oracle.chebyfilt(oracle_globals(c), X0[2], 5, lb[2], ub[2], c.vloc, c.ngw)

@@ -120,3 +120,33 @@ def test_si64_fp64_subset_matches_oracle(oracle):
         o2 = g.apply_ham_block(psi, prec)
         assert band_relerr(o2[idx], ref) < tol
     g.close()
+
+
+def test_autotune_targets_keep_results(oracle):
+    """pwh_autotune: every split-K block target gives the default target's H|psi> up to the
+    order of the split-K sums (FP64 1e-13, FP32 2e-6 relative per band), the timed search
+    picks one of its candidates, and bad arguments are refused."""
+    import pwh
+    from pwh import synth
+    s = synth.config("si64")
+    pwh.init(0)
+    g = pwh.System(s.globals())
+    g.set_vloc(s.vloc)
+    psi = s.psi(128, seed=9)
+    base = {p: g.apply_ham_block(psi, p) for p in ("fp64", "fp32")}
+    for t in (64, 256, 1024, 2048, 4096):
+        for p, tol in (("fp64", 1e-13), ("fp32", 2e-6)):
+            g.autotune(0, p, target=t)
+            assert band_relerr(g.apply_ham_block(psi, p), base[p]) < tol, (t, p)
+    g.resident_set(psi)
+    for p in ("fp64", "fp32", "bf16x3"):
+        t, ms = g.autotune(128, p)
+        assert t in (256, 512, 1024, 2048) and ms > 0.0
+    idx = [0, 77, 127]
+    ref = oracle.apply_ham_block(oracle_globals(s), psi[idx], s.vloc, s.ngw, nthreads=8)
+    assert band_relerr(g.apply_ham_block(psi)[idx], ref) < 1e-12
+    for bad in (dict(nstates=129), dict(nstates=128, precision=3), dict(nstates=0, target=32)):
+        with pytest.raises(pwh.PwhError) as e:
+            g.autotune(**bad)
+        assert e.value.code == pwh.PWH_ERR_ARG
+    g.close()
