@@ -46,7 +46,8 @@ class ScfResult:
 
 def cheby_scf(sysm, grid, X0, nocc, focc, omega, rho0, degree=10, precision=FP64, mixer="anderson",
               beta=0.3, max_iter=60, conv=1e-10, lanczos_k=20, v0=None, esr=0.0, eself=0.0,
-              mixdim=4, kerker_q0=1.0, chfsi_iters=1, seed=1, fp64_refine=None, resident=True):
+              mixdim=4, kerker_q0=1.0, chfsi_iters=1, seed=1, fp64_refine=None, resident=True,
+              autotune=False):
     """Runs the SCF loop from start vectors X0 (nstates, ngw) and density rho0 [nnr].
     nocc: occupied states (the first nocc Ritz vectors, ascending); focc [nocc]: their
     occupations times the k-point weight.  mixer: simple, kerker, anderson or broyden
@@ -60,7 +61,10 @@ def cheby_scf(sysm, grid, X0, nocc, focc, omega, rho0, degree=10, precision=FP64
     FP32-sized noise, so the converged energy wanders by ~1e-7 relative to its size between
     iterations; the last iterations in FP64 remove it (None: no switch).  resident: keep the
     block in HBM for the whole loop (pwh_resident_chfsi / _eval_rho_psi / _eval_enl);
-    otherwise every step passes the host block in and out.  Stops when
+    otherwise every step passes the host block in and out.  autotune: before the first
+    ChFSI step, pick the projection GEMMs' split-K targets for this block (pwh_autotune) for
+    FP64 (Rayleigh-Ritz, the filter's last step) and the filter precision; needs
+    resident=True.  Stops when
     |ETOT - ETOT_previous| <= conv (SCF_CONV_CRIT) or after max_iter iterations."""
     import time
     t0 = time.perf_counter()
@@ -92,10 +96,15 @@ def cheby_scf(sysm, grid, X0, nocc, focc, omega, rho0, degree=10, precision=FP64
     eig = None
     prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
     nst = X.shape[0]
+    if autotune and not resident:
+        raise ValueError("autotune times the resident block: use resident=True")
     if resident:
         sysm.resident_set(X)
     for it in range(1, max_iter + 1):
         sysm.set_vloc(veff)
+        if autotune and it == 1:
+            for p in sorted({FP64, prec}):
+                sysm.autotune(nst, p)
         ta = time.perf_counter()
         ck = dict(degree=degree, niter=chfsi_iters, precision=prec, lanczos_k=lanczos_k if it == 1 else 0,
                   bounds_mode=0, v0=v0, lb=lb, ub=ub)

@@ -134,6 +134,17 @@ def test_scf_resident_and_host_paths_agree(scf_setup):
     assert np.abs(a.eig - b.eig).max() < 1e-10
 
 
+def test_scf_autotuned_targets_keep_the_trajectory(scf_setup):
+    """With the split-K targets tuned first the steps only change by the order of the
+    split-K sums: ETOT per step to 1e-10 relative."""
+    a = _run(scf_setup, "fp64", max_iter=4, conv=0.0)
+    b = _run(scf_setup, "fp64", max_iter=4, conv=0.0, autotune=True)
+    ea, eb = np.array([h[1] for h in a.history]), np.array([h[1] for h in b.history])
+    assert np.abs(ea - eb).max() < 1e-10 * abs(ea[-1])
+    with pytest.raises(ValueError):
+        _run(scf_setup, "fp64", max_iter=1, autotune=True, resident=False)
+
+
 def test_resident_entry_points_match_host_ones(scf_setup):
     import pwh
     sysm, grid, X0, rho0, omega = scf_setup
