@@ -143,8 +143,8 @@ __global__ __launch_bounds__(kCholThreads) void k_chol_inv(cx<double>* __restric
 #pragma unroll
           for (int p = 0; p < 4; ++p)
             if (p <= r) {
-              orr[c] += v[p].x * ar[p][c] + v[p].y * ai[p][c];
-              oii[c] += v[p].x * ai[p][c] - v[p].y * ar[p][c];
+              orr[c] = fma(v[p].x, ar[p][c], fma(v[p].y, ai[p][c], orr[c]));
+              oii[c] = fma(v[p].x, ai[p][c], fma(-v[p].y, ar[p][c], oii[c]));
             }
         }
 #pragma unroll
@@ -169,9 +169,9 @@ __global__ __launch_bounds__(kCholThreads) void k_chol_inv(cx<double>* __restric
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
-          for (int c = 0; c < 4; ++c) {   // a -= conj(x_r) y_c
-            ar[r][c] -= x[r].x * y[c].x + x[r].y * y[c].y;
-            ai[r][c] -= x[r].x * y[c].y - x[r].y * y[c].x;
+          for (int c = 0; c < 4; ++c) {   // a -= conj(x_r) y_c, as four FMAs
+            ar[r][c] = fma(-x[r].y, y[c].y, fma(-x[r].x, y[c].x, ar[r][c]));
+            ai[r][c] = fma(x[r].y, y[c].x, fma(-x[r].x, y[c].y, ai[r][c]));
           }
       }
     }
@@ -202,8 +202,8 @@ __global__ __launch_bounds__(kCholThreads) void k_chol_inv(cx<double>* __restric
 #pragma unroll
           for (int q = 0; q < 4; ++q)
             if (q >= r) {
-              orr[c] += v[q].x * ar[q][c] - v[q].y * ai[q][c];
-              oii[c] += v[q].x * ai[q][c] + v[q].y * ar[q][c];
+              orr[c] = fma(v[q].x, ar[q][c], fma(-v[q].y, ai[q][c], orr[c]));
+              oii[c] = fma(v[q].x, ai[q][c], fma(v[q].y, ar[q][c], oii[c]));
             }
         }
 #pragma unroll
@@ -226,9 +226,9 @@ __global__ __launch_bounds__(kCholThreads) void k_chol_inv(cx<double>* __restric
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
-          for (int c = 0; c < 4; ++c) {
-            ar[r][c] -= x[r].x * y[c].x - x[r].y * y[c].y;
-            ai[r][c] -= x[r].x * y[c].y + x[r].y * y[c].x;
+          for (int c = 0; c < 4; ++c) {   // b -= x_r y_c, as four FMAs
+            ar[r][c] = fma(x[r].y, y[c].y, fma(-x[r].x, y[c].x, ar[r][c]));
+            ai[r][c] = fma(-x[r].y, y[c].x, fma(-x[r].x, y[c].y, ai[r][c]));
           }
       }
     }

@@ -212,7 +212,10 @@ __global__ __launch_bounds__(kEigT) void k_heev_tridiag(const cx<double>* __rest
         if (kEigG * ia + kEigG - 1 > k) {          // block has rows of the trailing matrix
 #pragma unroll
           for (int jb = 0; jb < kEigCB; ++jb)
-            if (32 * jb + 31 > k) sum = sum + cmul(a[ia][jb], vj[jb]);
+            if (32 * jb + 31 > k) {   // sum += a v_j as four FMAs
+              sum.x = fma(-a[ia][jb].y, vj[jb].y, fma(a[ia][jb].x, vj[jb].x, sum.x));
+              sum.y = fma(a[ia][jb].y, vj[jb].x, fma(a[ia][jb].x, vj[jb].y, sum.y));
+            }
         }
         rs[ia] = sum;
       }
@@ -252,9 +255,14 @@ __global__ __launch_bounds__(kEigT) void k_heev_tridiag(const cx<double>* __rest
 #pragma unroll
         for (int jb = 0; jb < kEigCB; ++jb) {
           if (32 * jb + 31 <= k) continue;           // finished columns
-          // v_i conj(w_j) + w_i conj(v_j); zero outside the trailing block
-          const cx<double> u = cmulc(wj[jb], vi) + cmulc(vj[jb], wi);
-          a[ia][jb] = a[ia][jb] - u;
+          // a -= v_i conj(w_j) + w_i conj(v_j) as eight FMAs; zero outside the trailing block
+          const cx<double> wjj = wj[jb], vjj = vj[jb];
+          double xr = a[ia][jb].x, xi = a[ia][jb].y;
+          xr = fma(-wjj.x, vi.x, xr); xr = fma(-wjj.y, vi.y, xr);
+          xr = fma(-vjj.x, wi.x, xr); xr = fma(-vjj.y, wi.y, xr);
+          xi = fma(-wjj.x, vi.y, xi); xi = fma(wjj.y, vi.x, xi);
+          xi = fma(-vjj.x, wi.y, xi); xi = fma(vjj.y, wi.x, xi);
+          a[ia][jb] = mk<double>(xr, xi);
         }
       }
     }
