@@ -55,12 +55,6 @@ __device__ __forceinline__ cx<double> cmulc(cx<double> a, cx<double> b) {   // c
 __device__ __forceinline__ cx<double> shfl_xor_cx(cx<double> v, int m) {
   return mk<double>(__shfl_xor(v.x, m), __shfl_xor(v.y, m));
 }
-// sum over the 32 lanes of a half-wave (every lane gets the sum)
-__device__ __forceinline__ cx<double> halfwave_sum(cx<double> v) {
-#pragma unroll
-  for (int m = 16; m >= 1; m >>= 1) v = v + shfl_xor_cx(v, m);
-  return v;
-}
 // Row sums over the 32 lanes of a half-wave for 8 rows at once (reduce-scatter): on return
 // the lane holds the complete sum of row 4*b4 + 2*b3 + b2 (b_k = bit k of its index tc in
 // the half-wave) -- 4 + 2 + 1 + 2 shuffles instead of 8 x 5.
@@ -101,11 +95,6 @@ __device__ __forceinline__ double halfwave_sum_dpp(double v) {
   v += dpp_f64<0x128>(v);   // row_ror:8
   return v + __shfl_xor(v, 16);
 }
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
-  return v;
-}
 
 // Workgroup barrier ordering LDS only: global stores still in flight are not waited for
 // (the tridiagonal reduction's Householder vectors are read by a later kernel).
@@ -125,6 +114,7 @@ __global__ __launch_bounds__(kEigT) void k_heev_tridiag(const cx<double>* __rest
   __shared__ cx<double> pv[kEigN];      // p = tau A v
   __shared__ cx<double> gpart[kEigG];   // per-row-group partial sums of p^H v
   __shared__ cx<double> tau_s[kEigN];
+  __shared__ double nrm2p[kEigG];       // per-row-group partial sums of |x_i|^2, i >= k + 2
   const int tid = threadIdx.x;
   const int tr = tid >> 5, tc = tid & 31, lane = tid & 63;
   if (tid == 0) info[0] = 0;
@@ -149,34 +139,40 @@ __global__ __launch_bounds__(kEigT) void k_heev_tridiag(const cx<double>* __rest
     const int cur = k & 1;
     // x = A[k+1:, k] from the lanes holding column k; d_k from the diagonal's owner
     if (tc == (k & 31)) {
-      // column block k >> 5 selected with compile-time indices (no dynamic register indexing)
+      // column block k >> 5 selected with compile-time indices (no dynamic register indexing);
+      // the owners also sum |x_i|^2 below the subdiagonal for the Householder norm
+      double part = 0.0;
 #pragma unroll
       for (int jb = 0; jb < kEigCB; ++jb) {
         if (jb != (k >> 5)) continue;
 #pragma unroll
         for (int ia = 0; ia < kEigRA; ++ia) {
           const int i = tr + kEigG * ia;
-          if (i > k && i < n) vx[cur][i] = a[ia][jb];
-          if (i == k) sc.d[k] = a[ia][jb].x;
+          const cx<double> x = a[ia][jb];
+          if (i > k && i < n) vx[cur][i] = x;
+          if (i > k + 1 && i < n) part = fma(x.x, x.x, fma(x.y, x.y, part));
+          if (i == k) sc.d[k] = x.x;
         }
       }
+      nrm2p[tr] = part;
     }
     lds_barrier();
     // Householder vector (zlarfg): H^H x = beta e_1, beta real
     if (tid < 64) {
       const cx<double> alpha = vx[cur][k + 1];
-      double s = 0.0;
-      for (int i = k + 2 + lane; i < n; i += 64) s += vx[cur][i].x * vx[cur][i].x + vx[cur][i].y * vx[cur][i].y;
-      const double xn2 = wave_sum(s);
+      double xn2 = 0.0;
+#pragma unroll
+      for (int t = 0; t < kEigG; ++t) xn2 += nrm2p[t];
       cx<double> tau = mk<double>(0.0, 0.0), scale = mk<double>(0.0, 0.0);
       double beta = alpha.x;
       if (xn2 != 0.0 || alpha.y != 0.0) {
         const double r = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
         beta = alpha.x >= 0.0 ? -r : r;
-        tau = mk<double>((beta - alpha.x) / beta, -alpha.y / beta);
+        const double rb = 1.0 / beta;
+        tau = mk<double>((beta - alpha.x) * rb, -alpha.y * rb);
         const cx<double> dnm = mk<double>(alpha.x - beta, alpha.y);      // 1 / (alpha - beta)
-        const double q = dnm.x * dnm.x + dnm.y * dnm.y;
-        scale = mk<double>(dnm.x / q, -dnm.y / q);
+        const double rq = 1.0 / (dnm.x * dnm.x + dnm.y * dnm.y);
+        scale = mk<double>(dnm.x * rq, -dnm.y * rq);
       }
       for (int i = k + 2 + lane; i < n; i += 64) {
         const cx<double> v = cmul(vx[cur][i], scale);
@@ -228,7 +224,7 @@ __global__ __launch_bounds__(kEigT) void k_heev_tridiag(const cx<double>* __rest
         pv[i] = pi;
         g = cmulc(pi, vx[cur][i]);
       }
-      g = halfwave_sum(g);
+      g = mk<double>(halfwave_sum_dpp(g.x), halfwave_sum_dpp(g.y));
       if (tc == 0) gpart[tr] = g;
     }
     lds_barrier();
