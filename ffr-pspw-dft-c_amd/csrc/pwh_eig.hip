@@ -55,29 +55,6 @@ __device__ __forceinline__ cx<double> cmulc(cx<double> a, cx<double> b) {   // c
 __device__ __forceinline__ cx<double> shfl_xor_cx(cx<double> v, int m) {
   return mk<double>(__shfl_xor(v.x, m), __shfl_xor(v.y, m));
 }
-// Row sums over the 32 lanes of a half-wave for 8 rows at once (reduce-scatter): on return
-// the lane holds the complete sum of row 4*b4 + 2*b3 + b2 (b_k = bit k of its index tc in
-// the half-wave) -- 4 + 2 + 1 + 2 shuffles instead of 8 x 5.
-__device__ __forceinline__ cx<double> reduce_scatter8(const cx<double> (&v)[8], int tc, int& row) {
-  const bool h4 = (tc >> 4) & 1, h3 = (tc >> 3) & 1, h2 = (tc >> 2) & 1;
-  cx<double> a4[4], a2[2];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const cx<double> send = h4 ? v[r] : v[r + 4], keep = h4 ? v[r + 4] : v[r];
-    a4[r] = keep + shfl_xor_cx(send, 16);
-  }
-#pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    const cx<double> send = h3 ? a4[r] : a4[r + 2], keep = h3 ? a4[r + 2] : a4[r];
-    a2[r] = keep + shfl_xor_cx(send, 8);
-  }
-  const cx<double> send = h2 ? a2[0] : a2[1], keep = h2 ? a2[1] : a2[0];
-  cx<double> t = keep + shfl_xor_cx(send, 4);
-  t = t + shfl_xor_cx(t, 2);
-  t = t + shfl_xor_cx(t, 1);
-  row = 4 * (int)h4 + 2 * (int)h3 + (int)h2;
-  return t;
-}
 // Cross-lane move of a double through DPP (CTRL: quad_perm / row_ror patterns, every
 // source lane valid).
 template <int CTRL>
@@ -94,6 +71,50 @@ __device__ __forceinline__ double halfwave_sum_dpp(double v) {
   v += dpp_f64<0x124>(v);   // row_ror:4
   v += dpp_f64<0x128>(v);   // row_ror:8
   return v + __shfl_xor(v, 16);
+}
+
+// value of lane ^ M (M = 1, 2, 4, 8; tc = lane index) within a row of 16 lanes, by DPP:
+// quad permutations for 1 and 2, a rotation by 8, and for 4 a rotation by 4 (lanes with
+// bit 2 set, whose partner is 4 below) or by 12 (the others)
+template <int M>
+__device__ __forceinline__ cx<double> xor_row_cx(cx<double> v, int tc) {
+  if constexpr (M == 1) {
+    return mk<double>(dpp_f64<0xB1>(v.x), dpp_f64<0xB1>(v.y));
+  } else if constexpr (M == 2) {
+    return mk<double>(dpp_f64<0x4E>(v.x), dpp_f64<0x4E>(v.y));
+  } else if constexpr (M == 8) {
+    return mk<double>(dpp_f64<0x128>(v.x), dpp_f64<0x128>(v.y));
+  } else {
+    static_assert(M == 4, "xor distance 1, 2, 4 or 8");
+    const bool up = (tc >> 2) & 1;
+    const cx<double> a = mk<double>(dpp_f64<0x124>(v.x), dpp_f64<0x124>(v.y));   // row_ror:4
+    const cx<double> b = mk<double>(dpp_f64<0x12C>(v.x), dpp_f64<0x12C>(v.y));   // row_ror:12
+    return up ? a : b;
+  }
+}
+// Row sums over the 32 lanes of a half-wave for 8 rows at once (reduce-scatter): on return
+// the lane holds the complete sum of row 4*b4 + 2*b3 + b2 (b_k = bit k of its index tc in
+// the half-wave) -- 4 + 2 + 1 + 2 exchanges instead of 8 x 5; all but the first (across
+// the two rows of 16) are DPP moves.
+__device__ __forceinline__ cx<double> reduce_scatter8(const cx<double> (&v)[8], int tc, int& row) {
+  const bool h4 = (tc >> 4) & 1, h3 = (tc >> 3) & 1, h2 = (tc >> 2) & 1;
+  cx<double> a4[4], a2[2];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const cx<double> send = h4 ? v[r] : v[r + 4], keep = h4 ? v[r + 4] : v[r];
+    a4[r] = keep + shfl_xor_cx(send, 16);
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const cx<double> send = h3 ? a4[r] : a4[r + 2], keep = h3 ? a4[r + 2] : a4[r];
+    a2[r] = keep + xor_row_cx<8>(send, tc);
+  }
+  const cx<double> send = h2 ? a2[0] : a2[1], keep = h2 ? a2[1] : a2[0];
+  cx<double> t = keep + xor_row_cx<4>(send, tc);
+  t = t + xor_row_cx<2>(t, tc);
+  t = t + xor_row_cx<1>(t, tc);
+  row = 4 * (int)h4 + 2 * (int)h3 + (int)h2;
+  return t;
 }
 
 // Workgroup barrier ordering LDS only: global stores still in flight are not waited for
