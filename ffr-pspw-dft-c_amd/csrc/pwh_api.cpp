@@ -213,6 +213,10 @@ struct pwh_system {
   // second stream for the nonlocal GEMMs, which overlap the FFT passes (fork/join events)
   hipStream_t nl_stream = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // third stream: ChFSI runs the eigenvector half of a Rayleigh-Ritz solve on it while the
+  // next filter runs on the library stream (fork/join events)
+  hipStream_t eig_stream = nullptr;
+  hipEvent_t ev_eig_fork = nullptr, ev_eig_done = nullptr;
 };
 
 namespace {
@@ -594,6 +598,22 @@ int sys_cholqr2(pwh_system* s, cx<double>* X, int nb) {
 // non-finite value, the saved input is solved again by rocSOLVER (zheevd).
 // work: n doubles; info: 2 ints of device scratch.  Synchronises the library stream when
 // the small solver ran (to read its flag).
+// scratch of the small solver and the saved input for its fallback, kept for the life of
+// the process
+Buf<double>& heev_ws() {
+  static Buf<double>* b = new Buf<double>();
+  return *b;
+}
+Buf<cx<double>>& heev_save() {
+  static Buf<cx<double>>* b = new Buf<cx<double>>();
+  return *b;
+}
+
+bool small_eigensolver(int n) {
+  const char* e = getenv("PWH_EIGSOLVER");
+  return n <= kHeevSmallMax && (!e || !strcmp(e, "own"));
+}
+
 int hermitian_eig_impl(cx<double>* A, int n, double* w, double* work, rocblas_int* info) {
   const int mode = [] {
     const char* e = getenv("PWH_EIGSOLVER");   // read per call: a caller may switch solvers
@@ -605,8 +625,8 @@ int hermitian_eig_impl(cx<double>* A, int n, double* w, double* work, rocblas_in
   }();
   auto* a = reinterpret_cast<rocblas_double_complex*>(A);
   if (mode == 0 && n <= kHeevSmallMax) {
-    static Buf<double>* ws = new Buf<double>();      // kept for the life of the process
-    static Buf<cx<double>>* save = new Buf<cx<double>>();
+    Buf<double>* ws = &heev_ws();
+    Buf<cx<double>>* save = &heev_save();
     CHK(ws->ensure(heev_small_scratch_doubles()));
     CHK(save->ensure((size_t)n * n));
     HIPCHK(hipMemcpyAsync(save->p, A, sizeof(cx<double>) * n * n, hipMemcpyDeviceToDevice, g.stream));
@@ -657,6 +677,89 @@ int rayleigh_ritz_dev(pwh_system* s, cx<double>* X, int nb) {
   HIPCHK(hipMemcpyAsync(X, s->B64.p, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, g.stream));
   HIPCHK(gemm_nn_f64(s->HX.p, ld, nb, s->S.p, nb, nb, s->ngw, s->B64.p, ld, g.stream));
   HIPCHK(hipMemcpyAsync(s->HX.p, s->B64.p, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, g.stream));
+  return PWH_OK;
+}
+
+// The Rayleigh-Ritz step of rayleigh_ritz_dev in two halves, for the ChFSI loop: the
+// eigenvector half runs on the system's third stream while the next filter runs on the
+// library stream.  The filter is linear, p(H)(X Q) = (p(H) X) Q, so the next filter takes
+// the unrotated X with its H X, and Q is applied to the filtered block afterwards.
+int eig_overlap_prepare(pwh_system* s) {
+  if (s->eig_stream) return PWH_OK;
+  HIPCHK(hipStreamCreateWithFlags(&s->eig_stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&s->ev_eig_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&s->ev_eig_done, hipEventDisableTiming));
+  return PWH_OK;
+}
+
+bool chfsi_overlap_on(int nb) {
+  static const bool on = [] {
+    const char* e = getenv("PWH_CHFSI_OVERLAP");
+    return !(e && e[0] == '0');
+  }();
+  return on && small_eigensolver(nb);
+}
+
+// First half (ChebySCFLoop.cpp:292-300): HX = H X, S = sym(X^H HX), the eigenvalues of S
+// (ascending) into s->evals and eig_host; then the eigenvector half is launched on the third
+// stream.  rr_finish joins it.
+int rr_begin(pwh_system* s, cx<double>* X, int nb, double* eig_host) {
+  const int ld = s->ngw_pad;
+  const size_t n = (size_t)nb * ld;
+  CHK(s->HX.ensure(n));
+  CHK(s->B64.ensure(n));
+  CHK(s->S.ensure((size_t)nb * nb));
+  CHK(s->evals.ensure(nb));
+  CHK(s->rwork.ensure(nb));
+  CHK(s->info.ensure(2));
+  CHK(heev_ws().ensure(heev_small_scratch_doubles()));
+  CHK(heev_save().ensure((size_t)nb * nb));
+  CHK(eig_overlap_prepare(s));
+  const size_t pe = proj_part_elems(nb, nb, s->ngw);
+  CHK(s->part64.ensure(pe));
+  CHK(hpsi<double>(s, X, s->HX.p, nb, 1.0, 0.0, 0.0, nullptr, PREC_FP64));
+  HIPCHK(gemm_cn_f64(X, ld, nb, s->HX.p, ld, nb, s->ngw, s->S.p, nb, s->part64.p, pe, g.stream));
+  HIPCHK(symmetrize(s->S.p, nb, nb, g.stream));
+  HIPCHK(hipMemcpyAsync(heev_save().p, s->S.p, sizeof(cx<double>) * nb * nb, hipMemcpyDeviceToDevice, g.stream));
+  int* info = reinterpret_cast<int*>(s->info.p);
+  HIPCHK(launch_heev_small_values(s->S.p, nb, nb, s->evals.p, info, heev_ws().p, g.stream));
+  HIPCHK(hipEventRecord(s->ev_eig_fork, g.stream));
+  HIPCHK(hipStreamWaitEvent(s->eig_stream, s->ev_eig_fork, 0));
+  HIPCHK(launch_heev_small_vectors(s->S.p, nb, nb, s->evals.p, info, heev_ws().p, s->eig_stream));
+  HIPCHK(hipEventRecord(s->ev_eig_done, s->eig_stream));
+  HIPCHK(hipMemcpyAsync(eig_host, s->evals.p, sizeof(double) * nb, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  return PWH_OK;
+}
+
+// Second half: waits for the eigenvectors (the small solver's flag sends the saved matrix to
+// rocSOLVER's zheevd, as hermitian_eig_impl does, and refreshes eig_host), then X <- X Q and,
+// with rotate_hx, HX <- HX Q.
+int rr_finish(pwh_system* s, cx<double>* X, int nb, bool rotate_hx, double* eig_host) {
+  const int ld = s->ngw_pad;
+  const size_t n = (size_t)nb * ld;
+  HIPCHK(hipStreamWaitEvent(g.stream, s->ev_eig_done, 0));
+  rocblas_int flag = 0;
+  HIPCHK(hipMemcpyAsync(&flag, s->info.p, sizeof(flag), hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  if (flag != 0) {
+    HIPCHK(hipMemcpyAsync(s->S.p, heev_save().p, sizeof(cx<double>) * nb * nb, hipMemcpyDeviceToDevice,
+                          g.stream));
+    RBCHK(rocsolver_zheevd(g.rb, rocblas_evect_original, rocblas_fill_upper, nb,
+                           reinterpret_cast<rocblas_double_complex*>(s->S.p), nb, s->evals.p, s->rwork.p,
+                           s->info.p));
+    rocblas_int hinfo = 0;
+    HIPCHK(hipMemcpyAsync(&hinfo, s->info.p, sizeof(hinfo), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipMemcpyAsync(eig_host, s->evals.p, sizeof(double) * nb, hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (hinfo != 0) return fail(PWH_ERR_LAPACK, "zheevd info " + std::to_string(hinfo));
+  }
+  HIPCHK(gemm_nn_f64(X, ld, nb, s->S.p, nb, nb, s->ngw, s->B64.p, ld, g.stream));
+  HIPCHK(hipMemcpyAsync(X, s->B64.p, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, g.stream));
+  if (rotate_hx) {
+    HIPCHK(gemm_nn_f64(s->HX.p, ld, nb, s->S.p, nb, nb, s->ngw, s->B64.p, ld, g.stream));
+    HIPCHK(hipMemcpyAsync(s->HX.p, s->B64.p, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, g.stream));
+  }
   return PWH_OK;
 }
 
@@ -1014,6 +1117,9 @@ int pwh_system_destroy(pwh_system* s) {
   if (s->nl_stream) { hipStreamSynchronize(s->nl_stream); hipStreamDestroy(s->nl_stream); }
   if (s->ev_fork) hipEventDestroy(s->ev_fork);
   if (s->ev_join) hipEventDestroy(s->ev_join);
+  if (s->eig_stream) { hipStreamSynchronize(s->eig_stream); hipStreamDestroy(s->eig_stream); }
+  if (s->ev_eig_fork) hipEventDestroy(s->ev_eig_fork);
+  if (s->ev_eig_done) hipEventDestroy(s->ev_eig_done);
   delete s;
   return PWH_OK;
 }
@@ -1289,6 +1395,10 @@ static int chfsi_impl(pwh_system* s, double* X, int nstates, double* v0, pwh_chf
     return PWH_OK;
   };
   int rc = PWH_OK;
+  // the reference bound rule (bounds_mode 1) runs a Lanczos after every RR, whose dot
+  // products use s->S, where the eigenvector half writes: no overlap then
+  const bool overlap = chfsi_overlap_on(nb) && o->bounds_mode != 1;
+  bool pending = false;
   if (o->lanczos_k > 0) {
     const int k = (o->bounds_mode == 1) ? kref : o->lanczos_k;
     double fn2 = 0;
@@ -1308,9 +1418,15 @@ static int chfsi_impl(pwh_system* s, double* X, int nstates, double* v0, pwh_chf
   if (X && (rc = load_block(s, X, nb))) goto done;
   for (int it = 0; it < o->niter; ++it) {
     double ta = now_s();
-    // from the second iteration on, H X of the block is the rotated one of the last RR
+    // from the second iteration on, H X of the block is that of the last RR: rotated, or,
+    // while that RR's eigenvectors are still being computed on the third stream, the
+    // unrotated pair, whose filtered block is rotated once they are ready
     if ((rc = filter_prec(s, s->X.p, nb, o->degree, lb, ub, o->precision, it > 0 ? s->HX.p : nullptr)))
       goto done;
+    if (pending) {
+      pending = false;
+      if ((rc = rr_finish(s, s->X.p, nb, false, eig))) goto done;
+    }
     HIPCHK(hipStreamSynchronize(g.stream));
     o->t_filter_s += now_s() - ta;
     o->band_applies += (long long)o->degree * nb;
@@ -1319,9 +1435,15 @@ static int chfsi_impl(pwh_system* s, double* X, int nstates, double* v0, pwh_chf
     HIPCHK(hipStreamSynchronize(g.stream));
     o->t_ortho_s += now_s() - ta;
     ta = now_s();
-    if ((rc = rayleigh_ritz_dev(s, s->X.p, nb))) goto done;
-    HIPCHK(hipMemcpyAsync(eig, s->evals.p, sizeof(double) * nb, hipMemcpyDeviceToHost, g.stream));
-    HIPCHK(hipStreamSynchronize(g.stream));
+    if (overlap) {
+      if ((rc = rr_begin(s, s->X.p, nb, eig))) goto done;
+      if (it + 1 < o->niter) pending = true;
+      else if ((rc = rr_finish(s, s->X.p, nb, true, eig))) goto done;
+    } else {
+      if ((rc = rayleigh_ritz_dev(s, s->X.p, nb))) goto done;
+      HIPCHK(hipMemcpyAsync(eig, s->evals.p, sizeof(double) * nb, hipMemcpyDeviceToHost, g.stream));
+      HIPCHK(hipStreamSynchronize(g.stream));
+    }
     o->t_rr_s += now_s() - ta;
     o->band_applies += nb;
     // bounds update (ChebySCFLoop.cpp:355-368)
@@ -1347,6 +1469,8 @@ static int chfsi_impl(pwh_system* s, double* X, int nstates, double* v0, pwh_chf
   o->ub = ub;
   o->t_total_s = now_s() - t0;
 done:
+  // an error left an eigenvector half running: let it finish before the buffers are reused
+  if (pending && s->eig_stream) hipStreamSynchronize(s->eig_stream);
   dv0.release();
   df.release();
   return rc;

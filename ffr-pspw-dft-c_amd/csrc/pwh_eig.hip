@@ -345,7 +345,7 @@ __device__ void load_tridiag(int n, const EigScratch& sc, double* dv, double* ev
 }
 
 // Phase 2: eigenvalue i = blockIdx.x * 16 + (tid / 32) by a half-wave; lam (scaled units)
-__global__ __launch_bounds__(kEigT) void k_heev_bisect(int n, EigScratch sc) {
+__global__ __launch_bounds__(kEigT) void k_heev_bisect(int n, double* __restrict__ w, EigScratch sc) {
   __shared__ double dv[kEigN], ev[kEigN], e2v[kEigN], stat[4];
   __shared__ double pivmin_s;
   const int tid = threadIdx.x, lane = tid & 63;
@@ -392,7 +392,11 @@ __global__ __launch_bounds__(kEigT) void k_heev_bisect(int n, EigScratch sc) {
     lo = nlo;
     hi = nhi;
   }
-  if (s == 0) sc.lam[i] = 0.5 * (lo + hi);
+  if (s == 0) {
+    const double l = 0.5 * (lo + hi);
+    sc.lam[i] = l;
+    w[i] = l / stat[0];   // the same value k_heev_orth stores, available before the vectors
+  }
 }
 
 // Phase 3: eigenvector of T for eigenvalue i = blockIdx.x * 16 + lane (lanes 0..15), by
@@ -677,9 +681,7 @@ __global__ __launch_bounds__(kEigT) void k_heev_backtr(cx<double>* __restrict__ 
 
 size_t heev_small_scratch_doubles() { return (size_t)kEigN * kEigN * (2 + 1 + 1) + 6 * kEigN + 8; }
 
-hipError_t launch_heev_small(cx<double>* A, int lda, int n, double* w, int* info, double* scratch,
-                             hipStream_t s) {
-  if (n < 1 || n > kEigN || lda < n) return hipErrorInvalidValue;
+static EigScratch eig_scratch(double* scratch) {
   EigScratch sc;
   sc.V = scratch;                               // 2 kEigN^2
   sc.Z = sc.V + 2 * kEigN * kEigN;              // kEigN^2
@@ -689,6 +691,23 @@ hipError_t launch_heev_small(cx<double>* A, int lda, int n, double* w, int* info
   sc.e = sc.d + kEigN;
   sc.lam = sc.e + kEigN;
   sc.st = sc.lam + kEigN;                       // 4 (+ padding)
+  return sc;
+}
+
+hipError_t launch_heev_small_values(cx<double>* A, int lda, int n, double* w, int* info, double* scratch,
+                                    hipStream_t s) {
+  if (n < 1 || n > kEigN || lda < n) return hipErrorInvalidValue;
+  const EigScratch sc = eig_scratch(scratch);
+  const int per_bis = kEigT / kBisS;
+  hipLaunchKernelGGL(k_heev_tridiag, dim3(1), dim3(kEigT), 0, s, A, lda, n, info, sc);
+  hipLaunchKernelGGL(k_heev_bisect, dim3((n + per_bis - 1) / per_bis), dim3(kEigT), 0, s, n, w, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_heev_small_vectors(cx<double>* A, int lda, int n, double* w, int* info, double* scratch,
+                                     hipStream_t s) {
+  if (n < 1 || n > kEigN || lda < n) return hipErrorInvalidValue;
+  const EigScratch sc = eig_scratch(scratch);
   constexpr size_t ism = sizeof(double) * ((size_t)2 * kEigN + 6 * (size_t)kEigN * kInvE + 8);
   constexpr size_t osm = sizeof(double) * (size_t)kEigN * (kEigN + 1);
   constexpr size_t bsm = sizeof(cx<double>) * ((size_t)kEigN * (kEigN - 1) / 2 + kEigN);
@@ -704,13 +723,17 @@ hipError_t launch_heev_small(cx<double>* A, int lda, int n, double* w, int* info
       return e;
     attr = true;
   }
-  const int per_bis = kEigT / kBisS;
-  hipLaunchKernelGGL(k_heev_tridiag, dim3(1), dim3(kEigT), 0, s, A, lda, n, info, sc);
-  hipLaunchKernelGGL(k_heev_bisect, dim3((n + per_bis - 1) / per_bis), dim3(kEigT), 0, s, n, sc);
   hipLaunchKernelGGL(k_heev_invit, dim3((n + kInvE - 1) / kInvE), dim3(64), ism, s, n, info, sc);
   hipLaunchKernelGGL(k_heev_orth, dim3(1), dim3(kEigT), osm, s, n, w, info, sc);
   hipLaunchKernelGGL(k_heev_backtr, dim3((n + kBtCols - 1) / kBtCols), dim3(kEigT), bsm, s, A, lda, n, sc);
   return hipGetLastError();
+}
+
+hipError_t launch_heev_small(cx<double>* A, int lda, int n, double* w, int* info, double* scratch,
+                             hipStream_t s) {
+  hipError_t e = launch_heev_small_values(A, lda, n, w, info, scratch, s);
+  if (e != hipSuccess) return e;
+  return launch_heev_small_vectors(A, lda, n, w, info, scratch, s);
 }
 
 }  // namespace pwh

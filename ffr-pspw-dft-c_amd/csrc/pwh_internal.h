@@ -91,6 +91,14 @@ constexpr int kCholSmallMax = 128;
 hipError_t launch_chol_inv_small(cx<double>* S, int lda, int n, int* info, hipStream_t s);
 hipError_t launch_heev_small(cx<double>* A, int lda, int n, double* w, int* info, double* scratch,
                              hipStream_t s);
+// The same solver in two halves on one scratch: the values half (tridiagonal reduction and
+// multisection) leaves w complete; the vectors half (inverse iteration, multiplet
+// orthonormalisation, back-transformation) then writes the eigenvectors into A and may set
+// info.  Together they are launch_heev_small.
+hipError_t launch_heev_small_values(cx<double>* A, int lda, int n, double* w, int* info, double* scratch,
+                                    hipStream_t s);
+hipError_t launch_heev_small_vectors(cx<double>* A, int lda, int n, double* w, int* info, double* scratch,
+                                     hipStream_t s);
 // Density of a block in sphere order (FP64): W (after launch_zinv) -> rho[nr3][nr2][nr1] =
 // sum_b wts[b] |psi_b(r)|^2, the bands split over `groups` partial grids part[groups][nnr].
 hipError_t launch_plane_rho(const SphereMaps& m, const void* tw1, const void* tw2, const cx<double>* W, int nb,
