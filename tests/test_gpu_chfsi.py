@@ -123,6 +123,19 @@ def test_chfsi_fp64_converges_to_dense_spectrum(si8, gpu_si8, si8_dense_eigs):
     assert res[:nx].max() < 1e-4
 
 
+def test_chfsi_above_128_states_converges_to_dense_spectrum(si8, gpu_si8, si8_dense_eigs):
+    """150 states: the Rayleigh-Ritz eigensolve is rocSOLVER's zheevd, in order (the
+    eigenvector overlap with the next filter is for the small solver only)."""
+    X0 = si8.psi(150, seed=21)
+    v0 = np.ones(si8.ngw, np.complex128)
+    eig, X, res, st = gpu_si8.chfsi(X0, degree=10, niter=30, precision="fp64", lanczos_k=20, v0=v0)
+    nx = 120
+    assert np.abs(eig[:nx] - si8_dense_eigs[:nx]).max() < 1e-8
+    assert np.all(np.diff(eig) >= 0)
+    G = X.conj() @ X.T
+    assert np.abs(G - np.eye(150)).max() < 1e-12
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
 def test_chfsi_mixed_matches_fp64(si8, gpu_si8, prec):
     X0 = si8.psi(16, seed=18)
