@@ -202,6 +202,8 @@ struct pwh_system {
   // block solvers (pwh_davidson, pwh_lobpcg): Q = [X | R] or [X | W | P] and H Q, one block
   // of scratch, the small projected matrices and per-band vectors
   Buf<cx<double>> dq, dhq, dtmp, dT, dG, dE, dT1, dt1;
+  Buf<cx<double>> lw;    // two n x n work matrices of the series second pass of CholeskyQR2
+  Buf<double> dmax;      // its max |G - I|
   Buf<double> dlam, dnrm, dpart, dprec, dbo;
   Buf<rocblas_int> info;
   // FP32 work
@@ -537,11 +539,75 @@ int filter_on_stream(pwh_system* s, int nb, int degree, double lb, double ub, in
   return PWH_OK;
 }
 
+// PWH_CHOLQR_SERIES=0 keeps the Cholesky second pass in ChFSI
+bool cholqr_series_on() {
+  static const bool on = [] {
+    const char* e = getenv("PWH_CHOLQR_SERIES");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+// C = A B for column-major n x n device matrices (rocBLAS on the library stream)
+int zgemm_nn_small(int n, const cx<double>* A, const cx<double>* B, cx<double>* C) {
+  const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+  RBCHK(rocblas_zgemm(g.rb, rocblas_operation_none, rocblas_operation_none, n, n, n, &one,
+                      reinterpret_cast<const rocblas_double_complex*>(A), n,
+                      reinterpret_cast<const rocblas_double_complex*>(B), n, &zero,
+                      reinterpret_cast<rocblas_double_complex*>(C), n));
+  return PWH_OK;
+}
+
 // CholeskyQR2 (FP64) of the nb columns of X [nb][ld] (nrows valid rows), using tmp [nb][ld];
 // dinfo: 2 ints of device scratch.  For nb <= 128 the factor inverse comes from the
 // one-workgroup kernel and both passes are checked with one synchronisation at the end.
+// span_only (a caller that needs an orthonormal basis of span(X), not the R of a QR: the
+// ChFSI loop, whose Rayleigh-Ritz step rotates the basis anyway), with lw (2 nb^2) and
+// dmax (1) given: after the first pass G = X^H X = I + E is nearly the identity, and when
+// nb max|E| <= 1e-3 the second pass is X (I + E)^{-1/2} by the series
+// I - E/2 + 3E^2/8 - 5E^3/16 + 35E^4/128 (truncation below |E|^5 / 4 <= 2.5e-16), three
+// rocBLAS n x n products in place of the one-workgroup inverse Cholesky factor.
 int cholqr2(cx<double>* X, int ld, int nrows, int nb, cx<double>* tmp, cx<double>* S,
-            cx<double>* part, size_t part_elems, rocblas_int* dinfo) {
+            cx<double>* part, size_t part_elems, rocblas_int* dinfo, cx<double>* lw = nullptr,
+            double* dmax = nullptr) {
+  if (nb <= kCholSmallMax && lw && dmax) {
+    HIPCHK(gemm_cn_f64(X, ld, nb, X, ld, nb, nrows, S, nb, part, part_elems, g.stream));
+    HIPCHK(launch_chol_inv_small(S, nb, nb, reinterpret_cast<int*>(dinfo), g.stream));
+    HIPCHK(gemm_nn_f64(X, ld, nb, S, nb, nb, nrows, tmp, ld, g.stream));
+    HIPCHK(gemm_cn_f64(tmp, ld, nb, tmp, ld, nb, nrows, S, nb, part, part_elems, g.stream));
+    HIPCHK(eye_residual(S, nb, dmax, g.stream));
+    rocblas_int hinfo = 0;
+    double emax = 0.0;
+    HIPCHK(hipMemcpyAsync(&hinfo, dinfo, sizeof(hinfo), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipMemcpyAsync(&emax, dmax, sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (hinfo != 0)
+      return fail(PWH_ERR_LAPACK, "CholeskyQR: overlap matrix not positive definite (zpotrf info " +
+                                      std::to_string(hinfo) + ")");
+    if ((double)nb * emax <= 1e-3) {
+      cx<double>* A = lw;
+      cx<double>* B = lw + (size_t)nb * nb;
+      HIPCHK(axpy_eye(A, S, 35.0 / 128.0, -5.0 / 16.0, nb, g.stream));
+      CHK(zgemm_nn_small(nb, S, A, B));
+      HIPCHK(axpy_eye(B, B, 1.0, 3.0 / 8.0, nb, g.stream));
+      CHK(zgemm_nn_small(nb, S, B, A));
+      HIPCHK(axpy_eye(A, A, 1.0, -0.5, nb, g.stream));
+      CHK(zgemm_nn_small(nb, S, A, B));
+      HIPCHK(axpy_eye(B, B, 1.0, 1.0, nb, g.stream));
+      HIPCHK(gemm_nn_f64(tmp, ld, nb, B, nb, nb, nrows, X, ld, g.stream));
+      return PWH_OK;
+    }
+    // not close enough to the identity: the Cholesky second pass on G = E + I
+    HIPCHK(axpy_eye(S, S, 1.0, 1.0, nb, g.stream));
+    HIPCHK(launch_chol_inv_small(S, nb, nb, reinterpret_cast<int*>(dinfo) + 1, g.stream));
+    HIPCHK(gemm_nn_f64(tmp, ld, nb, S, nb, nb, nrows, X, ld, g.stream));
+    HIPCHK(hipMemcpyAsync(&hinfo, dinfo + 1, sizeof(hinfo), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (hinfo != 0)
+      return fail(PWH_ERR_LAPACK, "CholeskyQR: overlap matrix not positive definite (zpotrf info " +
+                                      std::to_string(hinfo) + ")");
+    return PWH_OK;
+  }
   if (nb <= kCholSmallMax) {
     for (int pass = 0; pass < 2; ++pass) {
       cx<double>* src = (pass == 0) ? X : tmp;
@@ -579,14 +645,20 @@ int cholqr2(cx<double>* X, int ld, int nrows, int nb, cx<double>* tmp, cx<double
   return PWH_OK;
 }
 
-int sys_cholqr2(pwh_system* s, cx<double>* X, int nb) {
+int sys_cholqr2(pwh_system* s, cx<double>* X, int nb, bool span_only = false) {
   const size_t n = (size_t)nb * s->ngw_pad;
   CHK(s->A64.ensure(n));
   CHK(s->S.ensure((size_t)nb * nb));
   const size_t pe = proj_part_elems(nb, nb, s->ngw);
   CHK(s->part64.ensure(pe));
   CHK(s->info.ensure(2));
-  return cholqr2(X, s->ngw_pad, s->ngw, nb, s->A64.p, s->S.p, s->part64.p, pe, s->info.p);
+  const bool series = span_only && cholqr_series_on();
+  if (series) {
+    CHK(s->lw.ensure((size_t)2 * nb * nb));
+    CHK(s->dmax.ensure(1));
+  }
+  return cholqr2(X, s->ngw_pad, s->ngw, nb, s->A64.p, s->S.p, s->part64.p, pe, s->info.p,
+                 series ? s->lw.p : nullptr, series ? s->dmax.p : nullptr);
 }
 
 // FP64 Hermitian eigensolve on the GPU, eig_zheevd.cpp:5-34 ('V', 'U'): A (n x n,
@@ -1431,7 +1503,7 @@ static int chfsi_impl(pwh_system* s, double* X, int nstates, double* v0, pwh_chf
     o->t_filter_s += now_s() - ta;
     o->band_applies += (long long)o->degree * nb;
     ta = now_s();
-    if ((rc = sys_cholqr2(s, s->X.p, nb))) goto done;
+    if ((rc = sys_cholqr2(s, s->X.p, nb, true))) goto done;
     HIPCHK(hipStreamSynchronize(g.stream));
     o->t_ortho_s += now_s() - ta;
     ta = now_s();

@@ -11,6 +11,10 @@ hipError_t axpby(const cx<double>* x, double a, const cx<double>* z, double b, c
                  hipStream_t s);
 hipError_t symmetrize(cx<double>* A, int n, int ld, hipStream_t s);
 hipError_t zero_lower(cx<double>* A, int n, int ld, hipStream_t s);
+// Column-major n x n, ld = n, one workgroup: G <- G - I, maxabs[0] = max |Re|, |Im| of it
+hipError_t eye_residual(cx<double>* G, int n, double* maxabs, hipStream_t s);
+// out = a in + b I (column-major n x n, ld = n; out may be in)
+hipError_t axpy_eye(cx<double>* out, const cx<double>* in, double a, double b, int n, hipStream_t s);
 // charge / ksum: [nb][kBandSlices] partial sums per band (sum the slices in order)
 constexpr int kBandSlices = 16;
 hipError_t band_sums(const cx<double>* X, int ld, int n, const double* kin, int nb, double* charge,

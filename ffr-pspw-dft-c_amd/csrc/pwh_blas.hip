@@ -136,6 +136,49 @@ hipError_t zero_lower(cx<double>* A, int n, int ld, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Column-major n x n, ld = n, one workgroup: G <- G - I in place, maxabs[0] = the largest
+// |Re| or |Im| of the result (infinite when an entry is not a number)
+__global__ __launch_bounds__(256) void k_eye_residual(cx<double>* G, int n, double* maxabs) {
+  __shared__ double red[256];
+  const int tid = threadIdx.x;
+  double m = 0.0;
+  for (int e = tid; e < n * n; e += 256) {
+    cx<double> v = G[e];
+    if (e % (n + 1) == 0) {
+      v.x -= 1.0;
+      G[e] = v;
+    }
+    const double a = fmax(fabs(v.x), fabs(v.y));
+    m = (a == a) ? fmax(m, a) : INFINITY;
+  }
+  red[tid] = m;
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if (tid < h) red[tid] = fmax(red[tid], red[tid + h]);
+    __syncthreads();
+  }
+  if (tid == 0) maxabs[0] = red[0];
+}
+hipError_t eye_residual(cx<double>* G, int n, double* maxabs, hipStream_t s) {
+  hipLaunchKernelGGL(k_eye_residual, dim3(1), dim3(256), 0, s, G, n, maxabs);
+  return hipGetLastError();
+}
+
+// out = a in + b I (column-major n x n, ld = n; out may be in)
+__global__ void k_axpy_eye(cx<double>* out, const cx<double>* in, double a, double b, int n) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= n * n) return;
+  cx<double> v = in[e];
+  v.x *= a;
+  v.y *= a;
+  if (e % (n + 1) == 0) v.x += b;
+  out[e] = v;
+}
+hipError_t axpy_eye(cx<double>* out, const cx<double>* in, double a, double b, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy_eye, dim3((n * n + 255) / 256), dim3(256), 0, s, out, in, a, b, n);
+  return hipGetLastError();
+}
+
 // res[b] = || H[b] - lam[b] X[b] ||  (lam null -> || H[b] - X[b] ||)
 __global__ void k_resid(const cx<double>* __restrict__ H, const cx<double>* __restrict__ X,
                         const double* __restrict__ lam, int ld, int n, double* __restrict__ res) {

@@ -121,6 +121,9 @@ def test_chfsi_fp64_converges_to_dense_spectrum(si8, gpu_si8, si8_dense_eigs):
     nx = 14
     assert np.abs(eig[:nx] - si8_dense_eigs[:nx]).max() < 1e-8
     assert res[:nx].max() < 1e-4
+    # the Ritz vectors are orthonormal (the in-loop orthonormalisation's second pass is the
+    # (I + E)^{-1/2} series when the first pass leaves |E| small)
+    assert np.abs(X.conj() @ X.T - np.eye(16)).max() < 1e-12
 
 
 def test_chfsi_above_128_states_converges_to_dense_spectrum(si8, gpu_si8, si8_dense_eigs):
