@@ -539,13 +539,22 @@ int filter_on_stream(pwh_system* s, int nb, int degree, double lb, double ub, in
   return PWH_OK;
 }
 
-// PWH_CHOLQR_SERIES=0 keeps the Cholesky second pass in ChFSI
+// PWH_CHOLQR_SERIES=0 keeps the Cholesky second pass in ChFSI; PWH_CHOLQR_SERIES_TOL sets
+// the bound on n max|E| below which the series replaces it (default 1e-3; 0 never)
 bool cholqr_series_on() {
   static const bool on = [] {
     const char* e = getenv("PWH_CHOLQR_SERIES");
     return !(e && e[0] == '0');
   }();
   return on;
+}
+double cholqr_series_tol() {
+  static const double t = [] {
+    const char* e = getenv("PWH_CHOLQR_SERIES_TOL");
+    const double v = e ? atof(e) : 1e-3;
+    return (v >= 0.0 && v <= 1e-3) ? v : 1e-3;
+  }();
+  return t;
 }
 
 // C = A B for column-major n x n device matrices (rocBLAS on the library stream)
@@ -584,7 +593,7 @@ int cholqr2(cx<double>* X, int ld, int nrows, int nb, cx<double>* tmp, cx<double
     if (hinfo != 0)
       return fail(PWH_ERR_LAPACK, "CholeskyQR: overlap matrix not positive definite (zpotrf info " +
                                       std::to_string(hinfo) + ")");
-    if ((double)nb * emax <= 1e-3) {
+    if ((double)nb * emax <= cholqr_series_tol() && cholqr_series_tol() > 0.0) {
       cx<double>* A = lw;
       cx<double>* B = lw + (size_t)nb * nb;
       HIPCHK(axpy_eye(A, S, 35.0 / 128.0, -5.0 / 16.0, nb, g.stream));

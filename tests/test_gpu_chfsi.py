@@ -175,3 +175,42 @@ def test_reference_bound_rules_match_oracle(si8, gpu_si8, oracle):
     ub_ref = oracle.induced_norm_mat(Tk, 10) + np.linalg.norm(f10)
     assert abs(st1["lb"] - eig[nb - 1]) < 1e-12
     assert abs(st1["ub"] - ub_ref) < 1e-6 * ub_ref
+
+
+_PATHS_SCRIPT = r"""
+import json, sys
+sys.path.insert(0, {pkg!r})
+import numpy as np
+import pwh
+from pwh import synth
+s = synth.config("si8")
+pwh.init(0)
+g = pwh.System(s.globals())
+g.set_vloc(s.vloc)
+eig, X, res, st = g.chfsi(s.psi(16, seed=17), degree=10, niter=30, precision="fp64", lanczos_k=20,
+                          v0=np.ones(s.ngw, np.complex128))
+print(json.dumps({{"eig": eig.tolist(), "orth": float(np.abs(X.conj() @ X.T - np.eye(16)).max())}}))
+"""
+
+
+@pytest.mark.parametrize("env", [{"PWH_CHOLQR_SERIES_TOL": "0"}, {"PWH_CHOLQR_SERIES": "0"},
+                                 {"PWH_CHFSI_OVERLAP": "0"}])
+def test_chfsi_loop_variants_agree(si8, gpu_si8, env):
+    """The ChFSI loop's alternatives give the default run's eigenvalues to 1e-10 Ha: the
+    Cholesky second pass reached from the series test (tolerance 0), the plain CholeskyQR2,
+    and the Rayleigh-Ritz solve finished before the next filter.  Each runs in a child
+    process, since the switches are read once per process."""
+    import json
+    import os
+    import subprocess
+    import sys
+    import pwh
+    eig, X, res, st = gpu_si8.chfsi(si8.psi(16, seed=17), degree=10, niter=30, precision="fp64",
+                                    lanczos_k=20, v0=np.ones(si8.ngw, np.complex128))
+    code = _PATHS_SCRIPT.format(pkg=pwh.PKG_DIR)
+    out = subprocess.run([sys.executable, "-c", code], env={**os.environ, **env}, capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stderr[-2000:]
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    assert np.abs(np.array(r["eig"]) - eig).max() < 1e-10
+    assert r["orth"] < 1e-12
