@@ -139,6 +139,15 @@ __global__ __launch_bounds__(kEigT) void k_heev_tridiag(const cx<double>* __rest
   const int tid = threadIdx.x;
   const int tr = tid >> 5, tc = tid & 31, lane = tid & 63;
   if (tid == 0) info[0] = 0;
+  // v and p are zero outside the trailing block (indices <= k, and >= n), so the products
+  // below need no masks: both v buffers and p start at zero, and each step clears the two
+  // v entries and the p entry that fall out of the block
+  for (int i = tid; i < kEigN; i += kEigT) {
+    vx[0][i] = mk<double>(0.0, 0.0);
+    vx[1][i] = mk<double>(0.0, 0.0);
+    pv[i] = mk<double>(0.0, 0.0);
+  }
+  lds_barrier();
 
   // ---- phase 1: load (upper triangle, Hermitian completion), tridiagonalise ----------
   cx<double> a[kEigRA][kEigCB];
@@ -176,6 +185,11 @@ __global__ __launch_bounds__(kEigT) void k_heev_tridiag(const cx<double>* __rest
         }
       }
       nrm2p[tr] = part;
+    }
+    if (tid == 0) {
+      vx[cur][k] = mk<double>(0.0, 0.0);                 // written two steps ago
+      if (k > 0) vx[cur][k - 1] = mk<double>(0.0, 0.0);
+      pv[k] = mk<double>(0.0, 0.0);                      // written last step
     }
     lds_barrier();
     // Householder vector (zlarfg): H^H x = beta e_1, beta real
@@ -217,10 +231,7 @@ __global__ __launch_bounds__(kEigT) void k_heev_tridiag(const cx<double>* __rest
     {
       cx<double> vj[kEigCB];
 #pragma unroll
-      for (int jb = 0; jb < kEigCB; ++jb) {
-        const int j = tc + 32 * jb;
-        vj[jb] = (j > k && j < n) ? vx[cur][j] : mk<double>(0.0, 0.0);
-      }
+      for (int jb = 0; jb < kEigCB; ++jb) vj[jb] = vx[cur][tc + 32 * jb];   // zero outside the block
       static_assert(kEigRA == 8, "reduce_scatter8 handles 8 row blocks");
       cx<double> rs[kEigRA];
 #pragma unroll
@@ -256,19 +267,17 @@ __global__ __launch_bounds__(kEigT) void k_heev_tridiag(const cx<double>* __rest
       const cx<double> gam = cmul(mk<double>(-0.5 * tau.x, -0.5 * tau.y), gs);
       cx<double> vj[kEigCB], wj[kEigCB];
 #pragma unroll
-      for (int jb = 0; jb < kEigCB; ++jb) {
+      for (int jb = 0; jb < kEigCB; ++jb) {   // v and p are zero outside the block, so is w
         const int j = tc + 32 * jb;
-        const bool in = j > k && j < n;
-        vj[jb] = in ? vx[cur][j] : mk<double>(0.0, 0.0);
-        wj[jb] = in ? pv[j] + cmul(gam, vj[jb]) : mk<double>(0.0, 0.0);
+        vj[jb] = vx[cur][j];
+        wj[jb] = pv[j] + cmul(gam, vj[jb]);
       }
 #pragma unroll
       for (int ia = 0; ia < kEigRA; ++ia) {
         if (kEigG * ia + kEigG - 1 <= k) continue;  // finished rows (uniform branch)
         const int i = tr + kEigG * ia;
-        const bool in = i > k && i < n;
-        const cx<double> vi = in ? vx[cur][i] : mk<double>(0.0, 0.0);
-        const cx<double> wi = in ? pv[i] + cmul(gam, vi) : mk<double>(0.0, 0.0);
+        const cx<double> vi = vx[cur][i];
+        const cx<double> wi = pv[i] + cmul(gam, vi);
 #pragma unroll
         for (int jb = 0; jb < kEigCB; ++jb) {
           if (32 * jb + 31 <= k) continue;           // finished columns
