@@ -187,11 +187,15 @@ __global__ __launch_bounds__(kEigT) void k_heev_tridiag(const cx<double>* __rest
       nrm2p[tr] = part;
     }
     if (tid == 0) {
-      vx[cur][k] = mk<double>(0.0, 0.0);                 // written two steps ago
+      // this buffer was last read in step k - 2, before step k - 1's first barrier
+      vx[cur][k] = mk<double>(0.0, 0.0);
       if (k > 0) vx[cur][k - 1] = mk<double>(0.0, 0.0);
-      pv[k] = mk<double>(0.0, 0.0);                      // written last step
     }
     lds_barrier();
+    // p_k (written last step) is cleared only now: until the barrier above, other waves may
+    // still read it in last step's rank-2 update.  Nothing reads p between this barrier and
+    // the next, which orders the clear before this step's reads.
+    if (tid == 0) pv[k] = mk<double>(0.0, 0.0);
     // Householder vector (zlarfg): H^H x = beta e_1, beta real
     if (tid < 64) {
       const cx<double> alpha = vx[cur][k + 1];

@@ -702,176 +702,6 @@ __global__ void k_sum_groups(const double* __restrict__ part, int groups, size_t
 }
 
 // ---------------------------------------------------------------------------
-// Pass 2 with the 1D transforms in registers: two lanes per transform
-// ---------------------------------------------------------------------------
-// exp(SIGN 2 pi i e / N) for an e that is a compile-time constant after unrolling:
-// the cos/sin calls fold to immediates.
-template <typename T, int N, int SIGN>
-__device__ __forceinline__ cx<T> twc(int e) {
-  const double a = 6.28318530717958647692528676655900577 * (double)e / (double)N;
-  return mk<T>((T)__builtin_cos(a), (T)(SIGN * __builtin_sin(a)));
-}
-
-// In-register mixed-radix DFT (decimation in time), natural order in and out.
-template <typename T, int N, int SIGN>
-__device__ __forceinline__ void reg_fft(cx<T> (&v)[N]) {
-  if constexpr (N > 1) {
-    constexpr int R = pick_radix(N);
-    constexpr int M = N / R;
-    if constexpr (M == 1) {
-      dft<T, R, SIGN>(v);
-    } else {
-      cx<T> u[R][M];
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-#pragma unroll
-        for (int m = 0; m < M; ++m) u[r][m] = v[r + R * m];
-#pragma unroll
-      for (int r = 0; r < R; ++r) reg_fft<T, M, SIGN>(u[r]);
-#pragma unroll
-      for (int k = 0; k < M; ++k) {
-        cx<T> t[R];
-        t[0] = u[0][k];
-#pragma unroll
-        for (int r = 1; r < R; ++r) t[r] = u[r][k] * twc<T, N, SIGN>((r * k) % N);
-        dft<T, R, SIGN>(t);
-#pragma unroll
-        for (int q = 0; q < R; ++q) v[k + M * q] = t[q];
-      }
-    }
-  }
-}
-
-// value of the partner lane (lane ^ 1) through a quad permutation [1,0,3,2]
-__device__ __forceinline__ float swap_pair(float x) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));
-}
-__device__ __forceinline__ double swap_pair(double x) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), 0xB1, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-template <typename T>
-__device__ __forceinline__ cx<T> swap_pair(cx<T> v) { return mk<T>(swap_pair(v.x), swap_pair(v.y)); }
-
-// Length-N DFT over a lane pair.  DIT: lane p holds x[p + 2j] (j < N/2) on input and
-// X[k + (N/2) p] on output.  DIF: lane p holds x[k + (N/2) p] on input and X[2m + p] on
-// output.  The pair exchange is one DPP move per word.
-template <typename T, int N, int SIGN>
-__device__ __forceinline__ void pair_fft_dit(cx<T> (&v)[N / 2], int p) {
-  constexpr int M = N / 2;
-  reg_fft<T, M, SIGN>(v);
-#pragma unroll
-  for (int k = 0; k < M; ++k) {
-    const cx<T> w = twc<T, N, SIGN>(k);
-    const cx<T> y = p ? v[k] * w : v[k];
-    const cx<T> o = swap_pair(y);
-    v[k] = p ? (o - y) : (y + o);
-  }
-}
-template <typename T, int N, int SIGN>
-__device__ __forceinline__ void pair_fft_dif(cx<T> (&v)[N / 2], int p) {
-  constexpr int M = N / 2;
-#pragma unroll
-  for (int k = 0; k < M; ++k) {
-    const cx<T> o = swap_pair(v[k]);
-    v[k] = p ? (o - v[k]) * twc<T, N, SIGN>(k) : (v[k] + o);
-  }
-  reg_fft<T, M, SIGN>(v);
-}
-
-template <int N>
-constexpr int pair_threads() { return ((2 * N + 63) / 64) * 64; }
-
-// One (i3, band) plane per workgroup, 2 lanes per 1D transform.  LDS holds only the
-// nxa active i1 columns, column-major with an even stride S = N + 2 (each pair reads
-// and writes its own column or row, bank-conflict free).
-template <typename T, int N>
-__global__ __launch_bounds__(pair_threads<N>()) void k_plane_pair(SphereMaps m, const T* __restrict__ V,
-                                                                cx<T>* __restrict__ W) {
-  constexpr int M = N / 2;
-  constexpr int S = N + 2;
-  constexpr int NT = pair_threads<N>();
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int nxa = m.nxa;
-  cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
-  int* cmap = reinterpret_cast<int*>(tile + (size_t)nxa * S);
-  const int tid = threadIdx.x, p = tid & 1, tr = tid >> 1;
-  const int z = blockIdx.x, b = blockIdx.y;
-  for (int i = tid; i < N; i += NT) cmap[i] = -1;
-  for (int i = tid; i < nxa * S; i += NT) tile[i] = mk<T>(0, 0);
-  __syncthreads();
-  for (int i = tid; i < nxa; i += NT) cmap[m.xa[i]] = i;
-  __syncthreads();
-  cx<T>* wrow = W + ((size_t)b * m.nr3 + z) * m.ncol_pad;
-  for (int c = tid; c < m.ncol; c += NT) {
-    const int xy = m.colxy[c];
-    tile[cmap[xy % N] * S + xy / N] = wrow[c];
-  }
-  __syncthreads();
-  cx<T> v[M];
-  // inverse along i2 on the active columns
-  {
-    const bool act = tr < nxa;
-    cx<T>* col = tile + (act ? tr : 0) * S;
-#pragma unroll
-    for (int j = 0; j < M; ++j) v[j] = act ? col[p + 2 * j] : mk<T>(0, 0);
-    pair_fft_dit<T, N, +1>(v, p);
-    if (act) {
-#pragma unroll
-      for (int k = 0; k < M; ++k) col[k + M * p] = v[k];
-    }
-  }
-  __syncthreads();
-  // rows: inverse along i1, times V, forward along i1 (inactive columns are zeros in and
-  // are not stored out)
-  {
-    const bool act = tr < N;
-    const int y = act ? tr : 0;
-#pragma unroll
-    for (int j = 0; j < M; ++j) {
-      const int c = cmap[p + 2 * j];
-      v[j] = (act && c >= 0) ? tile[c * S + y] : mk<T>(0, 0);
-    }
-    pair_fft_dit<T, N, +1>(v, p);
-    const T* vr = V + ((size_t)z * N + y) * N + M * p;
-#pragma unroll
-    for (int k = 0; k < M; ++k) v[k] = vr[k] * v[k];
-    pair_fft_dif<T, N, -1>(v, p);
-    if (act) {
-#pragma unroll
-      for (int j = 0; j < M; ++j) {
-        const int c = cmap[2 * j + p];
-        if (c >= 0) tile[c * S + y] = v[j];
-      }
-    }
-  }
-  __syncthreads();
-  // forward along i2 on the active columns
-  {
-    const bool act = tr < nxa;
-    cx<T>* col = tile + (act ? tr : 0) * S;
-#pragma unroll
-    for (int j = 0; j < M; ++j) v[j] = act ? col[p + 2 * j] : mk<T>(0, 0);
-    pair_fft_dit<T, N, -1>(v, p);
-    if (act) {
-#pragma unroll
-      for (int k = 0; k < M; ++k) col[k + M * p] = v[k];
-    }
-  }
-  __syncthreads();
-  for (int c = tid; c < m.ncol; c += NT) {
-    const int xy = m.colxy[c];
-    wrow[c] = tile[cmap[xy % N] * S + xy / N];
-  }
-}
-
-// The register plane pass is used while a lane's half transform fits in 128 VGPRs.
-template <typename T, int N>
-constexpr bool use_pair_plane() { return (N / 2) * 2 * (int)sizeof(T) <= 128 * 4 && N % 2 == 0; }
-
-// ---------------------------------------------------------------------------
 // Pass 3: forward transform along i3, gather the sphere, fused epilogue
 // ---------------------------------------------------------------------------
 template <typename T, int NZ, int CT, int NT, int PE>
@@ -1038,35 +868,12 @@ static hipError_t launch_plane_lds(const SphereMaps& m, const void* tw1, const v
   return hipGetLastError();
 }
 
-// Plane pass dispatch.  Default: the LDS Stockham kernel with kPlaneThreads (512) lanes
-// (measured faster than 256 at Si64).  PWH_PLANE_NT=256 selects 256 lanes;
-// PWH_PLANE_PAIR=1 the register lane-pair kernel (measured slower: 2 waves/SIMD).
+// Plane pass dispatch: the band-pruned kernel when the sphere's columns lie in the band,
+// else the LDS Stockham kernel with kPlaneThreads (512) lanes (measured faster than 256
+// at Si64).
 template <typename T, int N>
 static hipError_t launch_plane_n(const SphereMaps& m, const void* tw1, const void* tw2, const T* V,
                                  cx<T>* W, int nb, hipStream_t s) {
-  static const int variant = [] {
-    const char* p = getenv("PWH_PLANE_PAIR");
-    const char* t = getenv("PWH_PLANE_NT");
-    if (p && p[0] == '1') return 2;
-    if (t && atoi(t) == 256) return 1;
-    return 0;
-  }();
-  if constexpr (use_pair_plane<T, N>()) {
-    if (variant == 2) {
-      const size_t psm = sizeof(cx<T>) * (size_t)m.nxa * (N + 2) + sizeof(int) * N;
-      static bool pattr = false;
-      if (!pattr) {
-        const size_t mx = sizeof(cx<T>) * (size_t)N * (N + 2) + sizeof(int) * N;
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_pair<T, N>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)mx);
-        if (e != hipSuccess) return e;
-        pattr = true;
-      }
-      dim3 grid(m.nr3, nb);
-      hipLaunchKernelGGL((k_plane_pair<T, N>), grid, dim3(pair_threads<N>()), psm, s, m, V, W);
-      return hipGetLastError();
-    }
-  }
   static const bool band_on = [] {
     const char* e = getenv("PWH_PLANE_BAND");
     return !(e && e[0] == '0');
@@ -1092,7 +899,6 @@ static hipError_t launch_plane_n(const SphereMaps& m, const void* tw1, const voi
   if constexpr (smem > 160 * 1024) {
     return hipErrorInvalidValue;
   } else {
-    if (variant == 1) return launch_plane_lds<T, N, 256>(m, tw1, tw2, V, W, nb, s);
     return launch_plane_lds<T, N, kPlaneThreads>(m, tw1, tw2, V, W, nb, s);
   }
 }

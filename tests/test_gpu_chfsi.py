@@ -139,8 +139,10 @@ def test_chfsi_above_128_states_converges_to_dense_spectrum(si8, gpu_si8, si8_de
     assert np.abs(G - np.eye(150)).max() < 1e-12
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
-def test_chfsi_mixed_matches_fp64(si8, gpu_si8, prec):
+def test_chfsi_mixed_matches_fp64(si8, gpu_si8):
+    """FP32 filter: the 1e-8 Ha gate.  (BF16x3 meets it on this 16-state cell but not at
+    the 128-state Si64 shape; its bound is asserted there, in test_gpu_configs.py.)"""
+    prec = "fp32"
     X0 = si8.psi(16, seed=18)
     v0 = np.ones(si8.ngw, np.complex128)
     e64, _, _, _ = gpu_si8.chfsi(X0, degree=10, niter=30, precision="fp64", lanczos_k=20, v0=v0)
