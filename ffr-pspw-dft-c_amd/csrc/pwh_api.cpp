@@ -1334,6 +1334,25 @@ int pwh_eig_zheevd(double* A, int ldA, double* lambda, int n) {
   return PWH_OK;
 }
 
+// The small solver on device memory with its scratch in the library's device-resident
+// array: no allocation, no synchronisation, no rocSOLVER fallback (info reports a
+// non-finite value instead).  Nothing here may allocate device memory, so neither the
+// library's init nor its scratch buffers are touched.
+int pwh_eig_zheevd_noalloc(double* A_device, int ldA, double* lambda_device, int n, int* info_device,
+                           void* stream) {
+  if (!A_device || !lambda_device || !info_device || n < 1 || ldA < n)
+    return fail(PWH_ERR_ARG, "bad eig_zheevd_noalloc arguments");
+  if (n > kHeevSmallMax)
+    return fail(PWH_ERR_ARG, "eig_zheevd_noalloc: n = " + std::to_string(n) + " exceeds " +
+                                 std::to_string(kHeevSmallMax) + ", the largest matrix its device scratch holds");
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : g.stream;
+  if (!st) return fail(PWH_ERR_ARG, "eig_zheevd_noalloc: no stream given and pwh_init not called");
+  double* sc = nullptr;
+  HIPCHK(heev_small_static_scratch(&sc));
+  HIPCHK(launch_heev_small(reinterpret_cast<cx<double>*>(A_device), ldA, n, lambda_device, info_device, sc, st));
+  return PWH_OK;
+}
+
 // Density of nb states (EvalRhoPsi.cpp:83-157) held in sphere order in Xw, which the
 // call modifies (the last plane wave is zeroed).  stage: the states in reference order to
 // permute into Xw first, or null when Xw already holds them.

@@ -692,7 +692,33 @@ __global__ __launch_bounds__(kEigT) void k_heev_backtr(cx<double>* __restrict__ 
   }
 }
 
-size_t heev_small_scratch_doubles() { return (size_t)kEigN * kEigN * (2 + 1 + 1) + 6 * kEigN + 8; }
+// V (2 n^2), Z (n^2), lu (n^2), tau (2 n), d, e, lam (n each), st (4, padded to 8)
+constexpr size_t kHeevScratchDoubles = (size_t)kEigN * kEigN * (2 + 1 + 1) + 6 * kEigN + 8;
+size_t heev_small_scratch_doubles() { return kHeevScratchDoubles; }
+
+// The solver's scratch for callers that cannot allocate (pwh_eig_zheevd_noalloc): a fixed
+// 530 KB array in device memory, one copy per GPU, held until the process ends.  Host
+// code reaches it only through the address hipGetSymbolAddress returns on each device;
+// every call on a GPU shares that GPU's copy.
+static __device__ __align__(256) double scratch[kHeevScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+
+hipError_t heev_small_static_scratch(double** ptr) {
+  static double* addr[kMaxDevices] = {};
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  if (!addr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[dev] = static_cast<double*>(p);
+  }
+  *ptr = addr[dev];
+  return hipSuccess;
+}
 
 static EigScratch eig_scratch(double* scratch) {
   EigScratch sc;

@@ -91,6 +91,9 @@ constexpr int kCholSmallMax = 128;
 hipError_t launch_chol_inv_small(cx<double>* S, int lda, int n, int* info, hipStream_t s);
 hipError_t launch_heev_small(cx<double>* A, int lda, int n, double* w, int* info, double* scratch,
                              hipStream_t s);
+// Address, on the current device, of the solver scratch held in a device-resident array
+// (heev_small_scratch_doubles() doubles): no allocation, shared by every call on the device.
+hipError_t heev_small_static_scratch(double** ptr);
 // The same solver in two halves on one scratch: the values half (tridiagonal reduction and
 // multisection) leaves w complete; the vectors half (inverse iteration, multiplet
 // orthonormalisation, back-transformation) then writes the eigenvectors into A and may set

@@ -46,6 +46,7 @@ EXPORTS = [
     "pwh_kerker_prec", "pwh_mix_anderson", "pwh_mix_pulay", "pwh_mix_broyden", "pwh_eval_enl",
     "pwh_phase_factor_gpu", "pwh_structure_factor", "pwh_nl_form_factor", "pwh_form_factor",
     "pwh_resident_chfsi", "pwh_resident_eval_rho_psi", "pwh_resident_eval_enl",
+    "pwh_eig_zheevd_noalloc",
 ]
 
 
@@ -162,6 +163,7 @@ def lib():
         "pwh_batch_chebyfilt": [vp, C.c_int, _dp, _dp, C.c_int, C.c_int],
         "pwh_fft3d": [vp, C.c_int, C.c_int, C.c_int, C.c_int],
         "pwh_eig_zheevd": [vp, C.c_int, _dp, C.c_int],
+        "pwh_eig_zheevd_noalloc": [vp, C.c_int, vp, C.c_int, vp, vp],
         "pwh_eval_rho_psi": [vp, vp, C.c_int, C.c_int, _dp, C.c_double, _dp, C.POINTER(C.c_double), _dp],
         "pwh_davidson": [vp, vp, C.c_int, _dp, C.POINTER(DiagOpts), _dp, _dp],
         "pwh_grid_create": [C.POINTER(GridDesc), C.POINTER(vp)],
@@ -381,6 +383,13 @@ def eig_zheevd(A, ldA=None):
     lam = np.zeros(n)
     _check(lib().pwh_eig_zheevd(_zptr(buf), ld, lam, n))
     return lam, np.ascontiguousarray(buf[:, :n].T)
+
+
+def eig_zheevd_noalloc(A_device, ldA, lambda_device, n, info_device, stream=None):
+    """pwh_eig_zheevd_noalloc on device addresses (ints); enqueued on `stream` (None: the
+    library stream) without waiting."""
+    _check(lib().pwh_eig_zheevd_noalloc(C.c_void_p(A_device), int(ldA), C.c_void_p(lambda_device), int(n),
+                                        C.c_void_p(info_device), C.c_void_p(stream) if stream else None))
 
 
 def synchronize():

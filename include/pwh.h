@@ -224,6 +224,17 @@ int pwh_fft3d(double *data, int nr1, int nr2, int nr3, int inverse);
  * library's own solver for n <= 128 (rocSOLVER zheevd when it flags a non-finite value),
  * rocSOLVER zheevd above; info != 0 -> PWH_ERR_LAPACK. */
 int pwh_eig_zheevd(double *A, int ldA, double *lambda, int n);
+/* No counterpart in the reference: pwh_eig_zheevd's solver (n <= 128) for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  A_device [ldA x n] (upper triangle read, eigenvectors out),
+ * lambda_device [n] and info_device [1] are device pointers; the work is enqueued on
+ * `stream` (a hipStream_t; null: the library stream, which needs pwh_init) and the call
+ * returns without waiting.  Its scratch is a fixed 530 KB device array per GPU, shared by
+ * every call on that GPU: calls that overlap on one GPU race.  n > 128 -> PWH_ERR_ARG.
+ * *info_device = 0, or nonzero when a non-finite value appeared (pwh_eig_zheevd would then
+ * rerun rocSOLVER; this entry allocates nothing and reports it instead). */
+int pwh_eig_zheevd_noalloc(double *A_device, int ldA, double *lambda_device, int n, int *info_device,
+                           void *stream);
 
 /* ---- ChFSI outer loop (eigen part of ChebySCFLoop.cpp:271-311) -------------- */
 typedef struct {
