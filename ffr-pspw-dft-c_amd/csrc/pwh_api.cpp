@@ -182,7 +182,8 @@ struct pwh_system {
   double tpiba2 = 0;
   SphereMaps maps{};
   std::vector<int> perm;
-  Buf<int> d_perm, d_colstart, d_kcol, d_kz, d_colxy, d_xa;
+  Buf<int> d_perm, d_colstart, d_kcol, d_kz, d_colxy, d_xa, d_colxa, d_xcols, d_xcstart;
+  int fft_request = FFT_AUTO;   // the FftPath asked for (pwh_fft_path)
   int nsp = 0, nat = 0;
   std::vector<int> na, lm, aoff, poff;   // poff: first projector row of each species
   std::vector<double> wl;   // [nsp*4]
@@ -208,6 +209,9 @@ struct pwh_system {
   Buf<rocblas_int> info;
   // FP32 work
   Buf<cx<float>> X32, Y32, Y1_32, W32, Hnl32, Q32, part32;
+  // intermediate of the split plane pass (grids whose planes do not fit one workgroup)
+  Buf<cx<double>> Ysp64;
+  Buf<cx<float>> Ysp32;
   int resident_nb = 0;
   // split-K block target of the projections per precision (PREC_FP64/FP32/BF16X3; 0: the
   // library default), set by pwh_autotune
@@ -234,6 +238,7 @@ template <> struct Sel<double> {
   static Buf<cx<double>>& Hnl(pwh_system* s) { return s->Hnl64; }
   static Buf<cx<double>>& Q(pwh_system* s) { return s->Q64; }
   static Buf<cx<double>>& part(pwh_system* s) { return s->part64; }
+  static Buf<cx<double>>& Ysp(pwh_system* s) { return s->Ysp64; }
 };
 template <> struct Sel<float> {
   static cx<float>* Bt(pwh_system* s) { return s->Bt32.p; }
@@ -245,7 +250,18 @@ template <> struct Sel<float> {
   static Buf<cx<float>>& Hnl(pwh_system* s) { return s->Hnl32; }
   static Buf<cx<float>>& Q(pwh_system* s) { return s->Q32; }
   static Buf<cx<float>>& part(pwh_system* s) { return s->part32; }
+  static Buf<cx<float>>& Ysp(pwh_system* s) { return s->Ysp32; }
 };
+
+// Band chunk of the split plane pass for nb bands in this precision (0: the plane pass of
+// this precision does not split), with its intermediate ensured.
+template <typename T>
+int split_prepare(pwh_system* s, int nb, int* chunk) {
+  *chunk = 0;
+  if (!plane_uses_split(s->maps, std::is_same<T, double>::value)) return PWH_OK;
+  *chunk = plane_split_chunk(s->maps, nb, sizeof(cx<T>), kSplitBudget);
+  return Sel<T>::Ysp(s).ensure(plane_split_elems(s->maps, *chunk));
+}
 
 // out = alpha * H X + beta * X + gamma * Z  on a block of nb bands in sphere order.
 int overlap_prepare(pwh_system* s) {
@@ -276,6 +292,8 @@ int hpsi_prepare(pwh_system* s, int nb) {
   const int ld = s->ngw_pad;
   CHK(overlap_prepare(s));
   CHK(Sel<T>::W(s).ensure((size_t)nb * s->nr[2] * s->maps.ncol_pad));
+  int chunk;
+  CHK(split_prepare<T>(s, nb, &chunk));
   for (int sp = 0; sp < s->nsp; ++sp) {
     if (s->na[sp] == 0 || s->lm[sp] == 0) continue;
     const int M = s->na[sp] * s->lm[sp];
@@ -299,6 +317,8 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
   Timed th(TM_HPSI, st);
   const int ld = s->ngw_pad;
   CHK(Sel<T>::W(s).ensure((size_t)nb * s->nr[2] * s->maps.ncol_pad));
+  int chunk;
+  CHK(split_prepare<T>(s, nb, &chunk));
   bool any_nl = false;
   for (int sp = 0; sp < s->nsp; ++sp) any_nl = any_nl || (s->na[sp] > 0 && s->lm[sp] > 0);
   // the nonlocal GEMMs run on a second stream, overlapping the FFT passes (running the
@@ -350,7 +370,7 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
   {
     Timed t(TM_PLANE, st);
     HIPCHK(launch_plane<T>(s->maps, Sel<T>::tw(s, 0), Sel<T>::tw(s, 1), Sel<T>::V(s), Sel<T>::W(s).p,
-                           nb, st));
+                           nb, Sel<T>::Ysp(s).p, chunk, st));
   }
   if (overlap) HIPCHK(hipStreamWaitEvent(st, s->ev_join, 0));
   {
@@ -1041,7 +1061,8 @@ int pwh_selftest_mfma(int kind, double* max_err) {
 }
 
 int pwh_grid_supported(int nr1, int nr2, int nr3) {
-  return plane_supported(nr1, nr2) && fft_size_supported(nr3) ? 1 : 0;
+  FftPlan p;
+  return fft_plan(nr1, &p) && fft_plan(nr2, &p) && fft_plan(nr3, &p) ? 1 : 0;
 }
 
 int pwh_system_create(const pwh_system_desc* d, pwh_system** out) {
@@ -1051,8 +1072,9 @@ int pwh_system_create(const pwh_system_desc* d, pwh_system** out) {
       !d->xkg || d->nsp < 0 || d->nax < 0 || d->nlmax < 1)
     return fail(PWH_ERR_ARG, "invalid system description");
   if (!pwh_grid_supported(d->nr1, d->nr2, d->nr3))
-    return fail(PWH_ERR_UNSUPPORTED, "no compiled FFT plan for grid " + std::to_string(d->nr1) + "x" +
-                                         std::to_string(d->nr2) + "x" + std::to_string(d->nr3));
+    return fail(PWH_ERR_UNSUPPORTED, "no FFT plan for grid " + std::to_string(d->nr1) + "x" +
+                                         std::to_string(d->nr2) + "x" + std::to_string(d->nr3) +
+                                         " (each length must be 2.." + std::to_string(kFftMaxLen) + ")");
   CHK(ensure_init());
   auto s = new pwh_system();
   auto bail = [&](int code) { pwh_system_destroy(s); return code; };
@@ -1108,11 +1130,26 @@ int pwh_system_create(const pwh_system_desc* d, pwh_system** out) {
   const int ncol = (int)colxy.size();
   const int ncol_pad = (ncol + 15) / 16 * 16;
   while ((int)colstart.size() < ncol_pad + 1) colstart.push_back(ngw);
+  // the split plane pass's maps: each column's active-i1 index, the columns ordered by
+  // (i1, i2) and where each active i1 starts in that order
+  std::vector<int> xidx(nr1, -1), colxa(ncol), xcols(ncol), xcstart(xa.size() + 1, 0);
+  for (size_t a = 0; a < xa.size(); ++a) xidx[xa[a]] = (int)a;
+  for (int c = 0; c < ncol; ++c) {
+    colxa[c] = xidx[colxy[c] % nr1];
+    ++xcstart[colxa[c] + 1];
+  }
+  for (size_t a = 0; a < xa.size(); ++a) xcstart[a + 1] += xcstart[a];
+  {
+    std::vector<int> fill(xcstart.begin(), xcstart.end() - 1);
+    for (int c = 0; c < ncol; ++c) xcols[fill[colxa[c]]++] = c;   // colxy ascends in i2 within an i1
+  }
   int rc;
   if ((rc = upload(s->d_perm, s->perm.data(), ngw)) ||
       (rc = upload(s->d_colstart, colstart.data(), colstart.size())) ||
       (rc = upload(s->d_kcol, kcol.data(), ngw)) || (rc = upload(s->d_kz, kz.data(), ngw)) ||
-      (rc = upload(s->d_colxy, colxy.data(), ncol)) || (rc = upload(s->d_xa, xa.data(), xa.size())))
+      (rc = upload(s->d_colxy, colxy.data(), ncol)) || (rc = upload(s->d_xa, xa.data(), xa.size())) ||
+      (rc = upload(s->d_colxa, colxa.data(), ncol)) || (rc = upload(s->d_xcols, xcols.data(), ncol)) ||
+      (rc = upload(s->d_xcstart, xcstart.data(), xcstart.size())))
     return bail(rc);
   s->maps.nr1 = nr1; s->maps.nr2 = nr2; s->maps.nr3 = d->nr3; s->maps.nnr = s->nnr;
   s->maps.ngw = ngw; s->maps.ngw_pad = s->ngw_pad;
@@ -1133,6 +1170,9 @@ int pwh_system_create(const pwh_system_desc* d, pwh_system** out) {
   }
   s->maps.colstart = s->d_colstart.p; s->maps.kcol = s->d_kcol.p; s->maps.kz = s->d_kz.p;
   s->maps.colxy = s->d_colxy.p; s->maps.xa = s->d_xa.p;
+  s->maps.colxa = s->d_colxa.p; s->maps.xcols = s->d_xcols.p; s->maps.xcstart = s->d_xcstart.p;
+  for (int dd = 0; dd < 3; ++dd) fft_plan(s->nr[dd], &s->maps.plan[dd]);
+  s->maps.path = compiled_plans_cover(nr1, nr2, d->nr3) ? FFT_COMPILED : FFT_RUNTIME;
 
   // constants in sphere order; the projectors beta = EIGR x PKG_A (ApplyHam_block.cpp:88-95)
   // are formed once in FP64 and rounded once for the FP32 copy
@@ -1225,6 +1265,22 @@ int pwh_system_info(const pwh_system* s, int* ngw, int* ncol, int* nxa, size_t* 
   return PWH_OK;
 }
 
+int pwh_fft_path(pwh_system* s, int path, int* used) {
+  if (!s || path < FFT_AUTO || path > FFT_SPLIT) return fail(PWH_ERR_ARG, "bad fft_path arguments");
+  const bool compiled = compiled_plans_cover(s->nr[0], s->nr[1], s->nr[2]);
+  if (path == FFT_COMPILED && !compiled)
+    return fail(PWH_ERR_UNSUPPORTED, "no compiled FFT plans for this grid (2,3,5-smooth 12..128, square planes)");
+  const int eff = path == FFT_AUTO ? (compiled ? FFT_COMPILED : FFT_RUNTIME) : path;
+  if (eff != s->maps.path) {
+    if (g.stream) HIPCHK(hipStreamSynchronize(g.stream));
+    s->maps.path = eff;
+    ++g.layout_gen;   // captured batch graphs hold the old launch arguments
+  }
+  s->fft_request = path;
+  if (used) *used = eff;
+  return PWH_OK;
+}
+
 int pwh_set_vloc(pwh_system* s, const double* rho, int nnr) {
   if (!s || !rho) return fail(PWH_ERR_ARG, "null argument");
   if (nnr != s->nnr)
@@ -1287,9 +1343,8 @@ int pwh_ortho_qr(double* X, int nbasis, int nstates) {
 int pwh_fft3d(double* data, int nr1, int nr2, int nr3, int inverse) {
   if (!data || nr1 < 1 || nr2 < 1 || nr3 < 1) return fail(PWH_ERR_ARG, "bad fft3d arguments");
   const int nr[3] = {nr1, nr2, nr3};
-  for (int d = 0; d < 3; ++d)
-    if (!fft_size_supported(nr[d]))
-      return fail(PWH_ERR_UNSUPPORTED, "no compiled FFT plan for length " + std::to_string(nr[d]));
+  if (!pwh_grid_supported(nr1, nr2, nr3))
+    return fail(PWH_ERR_UNSUPPORTED, "no FFT plan for a length outside 2.." + std::to_string(kFftMaxLen));
   CHK(ensure_init());
   Scratch& w = scratch();
   const size_t nnr = (size_t)nr1 * nr2 * nr3;
@@ -1369,6 +1424,8 @@ static int eval_rho_core(pwh_system* s, const cx<double>* stage, cx<double>* Xw,
   const int groups = std::max(1, std::min(nb, 512 / s->nr[2]));
   CHK(s->rho_part.ensure((size_t)groups * nnr));
   CHK(s->rho.ensure(nnr));
+  int chunk;
+  CHK(split_prepare<double>(s, nb, &chunk));
   int last = 0;
   for (int k = 0; k < ngw; ++k)
     if (s->perm[k] == ngw - 1) last = k;
@@ -1387,7 +1444,7 @@ static int eval_rho_core(pwh_system* s, const cx<double>* stage, cx<double>* Xw,
     }
     Timed t(TM_RHO_PLANE, g.stream);
     HIPCHK(launch_plane_rho(s->maps, s->tw64[0].p, s->tw64[1].p, s->W64.p, nb, groups, s->rho_w.p,
-                            s->rho_part.p, s->rho.p, g.stream));
+                            s->rho_part.p, s->rho.p, s->Ysp64.p, chunk, g.stream));
   }
   const size_t nsl = (size_t)nb * kBandSlices;
   std::vector<double> rho(nnr), qs(nsl), ks(nsl);
@@ -2080,9 +2137,8 @@ int pwh_grid_create(const pwh_grid_desc* d, pwh_grid** out) {
   if (!d || !out) return fail(PWH_ERR_ARG, "null argument");
   *out = nullptr;
   const int nr[3] = {d->nr1, d->nr2, d->nr3};
-  for (int k = 0; k < 3; ++k)
-    if (!fft_size_supported(nr[k]))
-      return fail(PWH_ERR_UNSUPPORTED, "no compiled FFT plan for length " + std::to_string(nr[k]));
+  if (!pwh_grid_supported(nr[0], nr[1], nr[2]))
+    return fail(PWH_ERR_UNSUPPORTED, "no FFT plan for a length outside 2.." + std::to_string(kFftMaxLen));
   if (d->ng < 1 || !d->n1 || !d->n2 || !d->n3 || !d->g2 || d->nsp < 1 || !d->sfac || !d->vps || !d->rhops ||
       !(d->omega > 0) || !(d->tpiba2 > 0) || d->ngw < 0 || (d->ngw > 0 && (!d->n123 || !d->xkg)))
     return fail(PWH_ERR_ARG, "bad grid description");

@@ -20,7 +20,7 @@
 // its butterflies into registers before a barrier so a single LDS buffer suffices.
 #include <cstdlib>
 
-#include "pwh_internal.h"
+#include "pwh_dft.h"
 
 namespace pwh {
 
@@ -32,71 +32,6 @@ constexpr int pick_radix(int rem) {
        : (rem % 5 == 0)              ? 5
                                      : rem;
 }
-
-// ---------------------------------------------------------------------------
-// Small DFTs, w = exp(SIGN * 2 pi i / R)
-// ---------------------------------------------------------------------------
-template <typename T, int SIGN>
-__device__ __forceinline__ void dft4(cx<T>& a0, cx<T>& a1, cx<T>& a2, cx<T>& a3) {
-  cx<T> t0 = a0 + a2, t1 = a0 - a2, t2 = a1 + a3, t3 = mul_si<SIGN>(a1 - a3);
-  a0 = t0 + t2; a2 = t0 - t2; a1 = t1 + t3; a3 = t1 - t3;
-}
-
-template <typename T, int R, int SIGN>
-__device__ __forceinline__ void dft(cx<T>* v) {
-  if constexpr (R == 2) {
-    cx<T> a = v[0], b = v[1];
-    v[0] = a + b; v[1] = a - b;
-  } else if constexpr (R == 4) {
-    dft4<T, SIGN>(v[0], v[1], v[2], v[3]);
-  } else if constexpr (R == 8) {
-    cx<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
-    cx<T> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
-    dft4<T, SIGN>(e0, e1, e2, e3);
-    dft4<T, SIGN>(o0, o1, o2, o3);
-    const T c = T(0.70710678118654752440084436210485);
-    // o1 *= w, o2 *= w^2, o3 *= w^3 with w = (1 + SIGN i)/sqrt2
-    cx<T> w1 = mk<T>(c * (o1.x - SIGN * o1.y), c * (o1.y + SIGN * o1.x));
-    cx<T> w2 = mul_si<SIGN>(o2);
-    cx<T> w3 = mk<T>(c * (-o3.x - SIGN * o3.y), c * (-o3.y + SIGN * o3.x));
-    v[0] = e0 + o0; v[4] = e0 - o0;
-    v[1] = e1 + w1; v[5] = e1 - w1;
-    v[2] = e2 + w2; v[6] = e2 - w2;
-    v[3] = e3 + w3; v[7] = e3 - w3;
-  } else if constexpr (R == 3) {
-    const T h = T(0.86602540378443864676372317075294);
-    cx<T> s = v[1] + v[2], d = v[1] - v[2];
-    cx<T> t = v[0] - T(0.5) * s;
-    cx<T> u = mul_si<SIGN>(h * d);
-    v[0] = v[0] + s; v[1] = t + u; v[2] = t - u;
-  } else if constexpr (R == 5) {
-    const T c1 = T(0.30901699437494742410229341718282);    // cos(2pi/5)
-    const T c2 = T(-0.80901699437494742410229341718282);   // cos(4pi/5)
-    const T s1 = T(0.95105651629515357211643933337938);    // sin(2pi/5)
-    const T s2 = T(0.58778525229247312916870595463907);    // sin(4pi/5)
-    cx<T> a0 = v[0];
-    cx<T> b1 = v[1] + v[4], b2 = v[2] + v[3], d1 = v[1] - v[4], d2 = v[2] - v[3];
-    cx<T> r1 = a0 + c1 * b1 + c2 * b2;
-    cx<T> r2 = a0 + c2 * b1 + c1 * b2;
-    cx<T> i1 = mul_si<SIGN>(s1 * d1 + s2 * d2);
-    cx<T> i2 = mul_si<SIGN>(s2 * d1 - s1 * d2);
-    v[0] = a0 + b1 + b2;
-    v[1] = r1 + i1; v[4] = r1 - i1;
-    v[2] = r2 + i2; v[3] = r2 - i2;
-  } else {
-    static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 8, "unsupported radix");
-  }
-}
-
-// Stage hooks: which input elements of the first stage are live (the others are known
-// zeros and are not read), which outputs of the last stage are stored, and a pointwise
-// operation applied to the last stage's outputs in registers.
-struct NoHooks {
-  __device__ __forceinline__ bool live_in(int, int) const { return true; }
-  __device__ __forceinline__ bool live_out(int, int) const { return true; }
-  template <typename V>
-  __device__ __forceinline__ void post(int, int, V&) const {}
-};
 
 // One Stockham stage over `count` transforms of length N held in LDS.  Element e of
 // transform t lives at buf[base(t) + e*es].  TFAST: consecutive threads take
@@ -177,29 +112,6 @@ __device__ __forceinline__ void lds_fft(cx<T>* buf, int count, int es, BaseF bas
     lds_fft<T, N, SIGN, NT, MAXCOUNT, TFAST, NS * R>(buf, count, es, base, tw, tid, hk);
   }
 }
-
-// x-direction hooks of the plane pass: inactive i1 columns are zero on input of the
-// inverse transform; its outputs are multiplied by V; the forward transform stores only
-// the active columns.
-template <typename T>
-struct XInvHooks {
-  const unsigned char* act;  // LDS [N1]
-  const T* vrow;             // V of this plane, [N2][N1]
-  int n1;
-  __device__ __forceinline__ bool live_in(int, int e) const { return act[e] != 0; }
-  __device__ __forceinline__ bool live_out(int, int) const { return true; }
-  __device__ __forceinline__ void post(int t, int e, cx<T>& v) const {
-    const T s = vrow[t * n1 + e];
-    v = s * v;
-  }
-};
-struct XFwdHooks {
-  const unsigned char* act;
-  __device__ __forceinline__ bool live_in(int, int) const { return true; }
-  __device__ __forceinline__ bool live_out(int, int e) const { return act[e] != 0; }
-  template <typename V>
-  __device__ __forceinline__ void post(int, int, V&) const {}
-};
 
 constexpr bool smooth235(int n) {
   while (n % 2 == 0) n /= 2;
@@ -563,14 +475,6 @@ __global__ __launch_bounds__(NT) void k_plane_band(SphereMaps m, const cx<T>* __
 // Density pass (EvalRhoPsi.cpp:138-150): per (i3, band group) plane, inverse FFT along i2
 // on the active columns and along i1, then sum w_b |psi_b(r)|^2 over the group's bands
 // ---------------------------------------------------------------------------
-struct XLiveHooks {
-  const unsigned char* act;
-  __device__ __forceinline__ bool live_in(int, int e) const { return act[e] != 0; }
-  __device__ __forceinline__ bool live_out(int, int) const { return true; }
-  template <typename V>
-  __device__ __forceinline__ void post(int, int, V&) const {}
-};
-
 constexpr int kRhoThreads = 256;
 
 // part[g][i3][i2][i1] = sum over bands b = g, g + groups, ... of wts[b] |psi_b|^2 (FP64)
@@ -851,6 +755,23 @@ bool fft_size_supported(int n) {
 
 bool plane_supported(int n1, int n2) { return n1 == n2 && fft_size_supported(n1); }
 
+// the compiled plane kernel of this precision fits the LDS of one CU
+static bool plane_compiled_fits(const SphereMaps& m, bool fp64) {
+  if (!plane_supported(m.nr1, m.nr2)) return false;
+  const int n = m.nr1;
+  const size_t cxb = fp64 ? 16 : 8;
+  const size_t lds = m.band_xy && n % 4 == 0 ? cxb * ((size_t)n * (n + 1) + n)
+                                               : cxb * ((size_t)n * (n + 1) + 2 * n) + 5 * (size_t)n;
+  return lds <= 160 * 1024;
+}
+
+bool plane_uses_split(const SphereMaps& m, bool fp64) {
+  if (m.path == FFT_COMPILED && plane_compiled_fits(m, fp64)) return false;
+  return m.path == FFT_SPLIT || !plane_rt_fits(m.nr1, m.nr2, fp64);
+}
+
+bool compiled_plans_cover(int n1, int n2, int n3) { return plane_supported(n1, n2) && fft_size_supported(n3); }
+
 template <typename T, int N, int NT>
 static hipError_t launch_plane_lds(const SphereMaps& m, const void* tw1, const void* tw2, const T* V,
                                    cx<T>* W, int nb, hipStream_t s) {
@@ -942,7 +863,7 @@ static hipError_t zinv_n(const SphereMaps& m, const void* tw3, const cx<T>* X, i
 template <typename T>
 hipError_t launch_zinv(const SphereMaps& m, const void* tw3, const cx<T>* X, int ldx, cx<T>* W,
                        int nb, hipStream_t s) {
-  if (!fft_size_supported(m.nr3)) return hipErrorInvalidValue;
+  if (m.path != FFT_COMPILED || !fft_size_supported(m.nr3)) return launch_zinv_rt<T>(m, tw3, X, ldx, W, nb, s);
   switch (m.nr3) {
 #define PWH_CASE(N) \
   case N: return zinv_n<T, N>(m, tw3, X, ldx, W, nb, s);
@@ -954,8 +875,9 @@ hipError_t launch_zinv(const SphereMaps& m, const void* tw3, const cx<T>* X, int
 
 template <typename T>
 hipError_t launch_plane(const SphereMaps& m, const void* tw1, const void* tw2, const T* V, cx<T>* W,
-                        int nb, hipStream_t s) {
-  if (!plane_supported(m.nr1, m.nr2)) return hipErrorInvalidValue;
+                        int nb, cx<T>* Y, int chunk, hipStream_t s) {
+  if (m.path != FFT_COMPILED || !plane_compiled_fits(m, sizeof(T) == sizeof(double)))
+    return launch_plane_rt<T>(m, tw1, tw2, V, W, nb, Y, chunk, s);
   switch (m.nr1) {
 #define PWH_CASE(N) \
   case N: return launch_plane_n<T, N>(m, tw1, tw2, V, W, nb, s);
@@ -1008,7 +930,8 @@ hipError_t launch_zfwd_epilogue(const SphereMaps& m, const void* tw3, const cx<T
                                 const cx<T>* X, const cx<T>* Hnl, const cx<T>* Z, cx<T>* out,
                                 int ld, int nb, T inv_nnr, T alpha, T beta, T gamma,
                                 hipStream_t s) {
-  if (!fft_size_supported(m.nr3)) return hipErrorInvalidValue;
+  if (m.path != FFT_COMPILED || !fft_size_supported(m.nr3))
+    return launch_zfwd_rt<T>(m, tw3, W, kin, X, Hnl, Z, out, ld, nb, inv_nnr, alpha, beta, gamma, s);
   const ZfwdArgs a{tw3, W, kin, X, Hnl, Z, out, ld, nb, (double)inv_nnr, (double)alpha, (double)beta,
                    (double)gamma};
   switch (m.nr3) {
@@ -1062,8 +985,11 @@ static hipError_t launch_plane_rho_n(const SphereMaps& m, const void* tw1, const
 }
 
 hipError_t launch_plane_rho(const SphereMaps& m, const void* tw1, const void* tw2, const cx<double>* W, int nb,
-                            int groups, const double* wts, double* part, double* rho, hipStream_t s) {
-  if (!plane_supported(m.nr1, m.nr2) || groups < 1) return hipErrorInvalidValue;
+                            int groups, const double* wts, double* part, double* rho, cx<double>* Y, int chunk,
+                            hipStream_t s) {
+  if (groups < 1) return hipErrorInvalidValue;
+  if (m.path != FFT_COMPILED || !plane_compiled_fits(m, true))
+    return launch_plane_rho_rt(m, tw1, tw2, W, nb, groups, wts, part, rho, Y, chunk, s);
   hipError_t e = hipErrorInvalidValue;
   switch (m.nr1) {
 #define PWH_CASE(N) \
@@ -1080,7 +1006,12 @@ hipError_t launch_plane_rho(const SphereMaps& m, const void* tw1, const void* tw
 
 hipError_t launch_fft_lines(cx<double>* data, int n, int inner, int nlines, const cx<double>* tw,
                             int sign, double scale, hipStream_t s) {
-  if (!fft_size_supported(n) || nlines < 1 || inner < 1) return hipErrorInvalidValue;
+  if (nlines < 1 || inner < 1) return hipErrorInvalidValue;
+  if (!fft_size_supported(n)) {
+    FftPlan p;
+    if (!fft_plan(n, &p)) return hipErrorInvalidValue;
+    return launch_fft_lines_rt(data, p, inner, nlines, tw, sign, scale, s);
+  }
   dim3 grid((nlines + kColTile - 1) / kColTile);
   switch (n) {
 #define PWH_CASE(N)                                                                                \
@@ -1103,9 +1034,9 @@ template hipError_t launch_zinv<double>(const SphereMaps&, const void*, const cx
 template hipError_t launch_zinv<float>(const SphereMaps&, const void*, const cx<float>*, int,
                                        cx<float>*, int, hipStream_t);
 template hipError_t launch_plane<double>(const SphereMaps&, const void*, const void*, const double*,
-                                         cx<double>*, int, hipStream_t);
+                                         cx<double>*, int, cx<double>*, int, hipStream_t);
 template hipError_t launch_plane<float>(const SphereMaps&, const void*, const void*, const float*,
-                                        cx<float>*, int, hipStream_t);
+                                        cx<float>*, int, cx<float>*, int, hipStream_t);
 template hipError_t launch_zfwd_epilogue<double>(const SphereMaps&, const void*, const cx<double>*,
                                                  const double*, const cx<double>*,
                                                  const cx<double>*, const cx<double>*, cx<double>*,

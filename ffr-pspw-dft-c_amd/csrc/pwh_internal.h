@@ -47,6 +47,28 @@ __host__ __device__ __forceinline__ cx<T> mul_si(cx<T> a) {
 
 enum Precision { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2 };
 
+// A runtime-length FFT plan: the Stockham stages' radices in order (2, 3, 4, 5, 7, 8 have
+// register butterflies; any other prime p runs as a direct length-p DFT per output).
+constexpr int kFftMaxStages = 16;
+constexpr int kFftMaxLen = 4096;
+struct FftPlan {
+  int n;
+  int nst;
+  int radix[kFftMaxStages];
+};
+// Factorises n (1 <= n <= kFftMaxLen): radix 8 while 8 divides the rest (4, 4 for 16), then
+// 4, 2, 3, 5, 7 and the remaining primes in increasing order.  False when n is out of range.
+bool fft_plan(int n, FftPlan* p);
+
+// Which implementation the FFT passes of a system use.
+enum FftPath {
+  FFT_AUTO = 0,      // compiled plans when they cover the grid, else runtime plans in LDS,
+                     // else runtime plans with the plane split into three passes
+  FFT_COMPILED = 1,  // compiled per-length kernels (2,3,5-smooth lengths 12..128, square planes)
+  FFT_RUNTIME = 2,   // runtime plans, one (i3, band) plane per workgroup in LDS
+  FFT_SPLIT = 3,     // runtime plans, plane pass as y / x / y line passes through HBM
+};
+
 // Device-side description of the sphere <-> grid maps used by the FFT passes.
 struct SphereMaps {
   int nr1, nr2, nr3, nnr;
@@ -64,15 +86,23 @@ struct SphereMaps {
   int* kz;              // [ngw] i3 of each internal slot
   int* colxy;           // [ncol] i1 + nr1*i2 of each column
   int* xa;              // [nxa] active i1 values
+  // runtime plans and the maps of the split plane pass
+  int path;             // FftPath actually used (never FFT_AUTO once the system exists)
+  FftPlan plan[3];      // lengths nr1, nr2, nr3
+  int* colxa;           // [ncol] index in xa of each column's i1
+  int* xcols;           // [ncol] the columns ordered by (i1, i2)
+  int* xcstart;         // [nxa + 1] first position in xcols of each active i1
 };
 
 // Launchers (pwh_fft.hip)
 template <typename T>
 hipError_t launch_zinv(const SphereMaps& m, const void* tw3, const cx<T>* X, int ldx, cx<T>* W,
                        int nb, hipStream_t s);
+// Y: the split pass's intermediate (plane_split_elems(m, chunk) elements, chunk =
+// plane_split_chunk(...)); unused by the other paths (may be null there).
 template <typename T>
 hipError_t launch_plane(const SphereMaps& m, const void* tw1, const void* tw2, const T* V, cx<T>* W,
-                        int nb, hipStream_t s);
+                        int nb, cx<T>* Y, int chunk, hipStream_t s);
 template <typename T>
 hipError_t launch_zfwd_epilogue(const SphereMaps& m, const void* tw3, const cx<T>* W, const T* kin,
                                 const cx<T>* X, const cx<T>* Hnl, const cx<T>* Z, cx<T>* out,
@@ -80,6 +110,18 @@ hipError_t launch_zfwd_epilogue(const SphereMaps& m, const void* tw3, const cx<T
                                 hipStream_t s);
 bool fft_size_supported(int n);
 bool plane_supported(int n1, int n2);
+// Runtime-plan passes (pwh_fft_rt.hip).  plane_rt_fits: the (i1, i2) plane and its
+// transforms fit one workgroup (LDS and registers) in this precision.
+bool plane_rt_fits(int n1, int n2, bool fp64);
+// Elements of the split plane pass's intermediate Y[nbc][nr3][nxa][nr2] for nbc bands,
+// and the band chunk that keeps it within `budget_bytes`.
+size_t plane_split_elems(const SphereMaps& m, int nbc);
+// the plane pass (and the density pass, for fp64) of this precision takes the split path
+bool plane_uses_split(const SphereMaps& m, bool fp64);
+// the compiled per-length kernels cover every pass of this grid
+bool compiled_plans_cover(int n1, int n2, int n3);
+constexpr size_t kSplitBudget = size_t(1) << 30;   // bytes of the split pass's intermediate
+int plane_split_chunk(const SphereMaps& m, int nb, size_t bytes_per_complex, size_t budget_bytes);
 // One-workgroup FP64 Hermitian eigensolver for n <= 128 (pwh_eig.hip): A (column-major,
 // lda, upper triangle read) -> eigenvectors, w ascending; info[0] != 0 on a non-finite value.
 size_t heev_small_scratch_doubles();
@@ -105,11 +147,30 @@ hipError_t launch_heev_small_vectors(cx<double>* A, int lda, int n, double* w, i
 // Density of a block in sphere order (FP64): W (after launch_zinv) -> rho[nr3][nr2][nr1] =
 // sum_b wts[b] |psi_b(r)|^2, the bands split over `groups` partial grids part[groups][nnr].
 hipError_t launch_plane_rho(const SphereMaps& m, const void* tw1, const void* tw2, const cx<double>* W, int nb,
-                            int groups, const double* wts, double* part, double* rho, hipStream_t s);
+                            int groups, const double* wts, double* part, double* rho, cx<double>* Y, int chunk,
+                            hipStream_t s);
 // Dense FP64 transform of length n along an axis of stride `inner` for every line of an
 // [outer][n][inner] array, in place; tw[m] = exp(+2 pi i m / n); sign +1 inverse, -1 forward.
 hipError_t launch_fft_lines(cx<double>* data, int n, int inner, int nlines, const cx<double>* tw,
                             int sign, double scale, hipStream_t s);
+
+// Runtime-plan launchers (pwh_fft_rt.hip), used by the ones above when the compiled plans
+// do not cover the grid (or the system's path asks for them).
+template <typename T>
+hipError_t launch_zinv_rt(const SphereMaps& m, const void* tw3, const cx<T>* X, int ldx, cx<T>* W, int nb,
+                          hipStream_t s);
+template <typename T>
+hipError_t launch_zfwd_rt(const SphereMaps& m, const void* tw3, const cx<T>* W, const T* kin, const cx<T>* X,
+                          const cx<T>* Hnl, const cx<T>* Z, cx<T>* out, int ld, int nb, T inv_nnr, T alpha,
+                          T beta, T gamma, hipStream_t s);
+template <typename T>
+hipError_t launch_plane_rt(const SphereMaps& m, const void* tw1, const void* tw2, const T* V, cx<T>* W, int nb,
+                           cx<T>* Y, int chunk, hipStream_t s);
+hipError_t launch_plane_rho_rt(const SphereMaps& m, const void* tw1, const void* tw2, const cx<double>* W, int nb,
+                               int groups, const double* wts, double* part, double* rho, cx<double>* Y, int chunk,
+                               hipStream_t s);
+hipError_t launch_fft_lines_rt(cx<double>* data, const FftPlan& p, int inner, int nlines, const cx<double>* tw,
+                               int sign, double scale, hipStream_t s);
 
 // GEMM launchers (pwh_gemm.hip)
 struct BetaGen {

@@ -46,7 +46,7 @@ EXPORTS = [
     "pwh_kerker_prec", "pwh_mix_anderson", "pwh_mix_pulay", "pwh_mix_broyden", "pwh_eval_enl",
     "pwh_phase_factor_gpu", "pwh_structure_factor", "pwh_nl_form_factor", "pwh_form_factor",
     "pwh_resident_chfsi", "pwh_resident_eval_rho_psi", "pwh_resident_eval_enl",
-    "pwh_eig_zheevd_noalloc",
+    "pwh_eig_zheevd_noalloc", "pwh_fft_path",
 ]
 
 
@@ -137,6 +137,7 @@ def lib():
         "pwh_system_info": [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                             C.POINTER(C.c_size_t)],
         "pwh_set_vloc": [vp, _dp, C.c_int],
+        "pwh_fft_path": [vp, C.c_int, C.POINTER(C.c_int)],
         "pwh_apply_ham_block": [vp, vp, vp, C.c_int, C.c_int, C.c_int],
         "pwh_chebyfilt": [vp, vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int],
         "pwh_ortho_qr": [vp, C.c_int, C.c_int],
@@ -476,6 +477,15 @@ class System:
         _check(lib().pwh_system_info(self.h, C.byref(ngw), C.byref(ncol), C.byref(nxa),
                                      C.byref(nbytes)))
         return dict(ngw=ngw.value, ncol=ncol.value, nxa=nxa.value, device_bytes=nbytes.value)
+
+    FFT_PATHS = {"auto": 0, "compiled": 1, "runtime": 2, "split": 3}
+
+    def fft_path(self, path="auto"):
+        """Selects the FFT implementation (pwh_fft_path); returns the mode in effect."""
+        used = C.c_int(0)
+        _check(lib().pwh_fft_path(self.h, self.FFT_PATHS[path] if isinstance(path, str) else int(path),
+                                  C.byref(used)))
+        return {v: k for k, v in self.FFT_PATHS.items()}[used.value]
 
     def set_vloc(self, rho):
         rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)

@@ -50,14 +50,15 @@ def diamond_fractional(ncell):
 
 
 def gaussian_vloc(a, nr, tau):
-    """V on the grid, x fastest (flat [nr1*nr2*nr3]); cubic cell of edge a, images +-1."""
-    axes = [np.arange(n) / n * a for n in nr]
+    """V on the grid, x fastest (flat [nr1*nr2*nr3]); orthorhombic cell of edges a (a
+    number: cubic), images +-1 along each axis."""
+    edges = np.broadcast_to(np.asarray(a, np.float64), (3,))
+    axes = [np.arange(n) / n * edges[d] for d, n in enumerate(nr)]
     V = np.zeros((nr[2], nr[1], nr[0]))
-    shifts = np.array([-1.0, 0.0, 1.0]) * a
     for R in tau:
         g = []
         for d in range(3):
-            x = axes[d][:, None] - R[d] + shifts[None, :]
+            x = axes[d][:, None] - R[d] + np.array([-1.0, 0.0, 1.0])[None, :] * edges[d]
             g.append(np.exp(-x * x / (2.0 * V_WIDTH ** 2)).sum(axis=1))
         V += g[2][:, None, None] * g[1][None, :, None] * g[0][None, None, :]
     return (V_DEPTH * V).reshape(-1)
@@ -104,13 +105,20 @@ class SynthSystem:
         return np.ascontiguousarray(Q.T)
 
 
-def make_system(a, ecut, ncell, nr, nbands, name="custom", natoms=None, frac=None):
-    A = np.diag([a, a, a]).astype(np.float64)
+def make_system(a, ecut, ncell, nr, nbands, name="custom", natoms=None, frac=None, edges=None):
+    """A cubic cell of edge a, or the orthorhombic cell of the given edges (then ALAT =
+    edges[0], Setup.cpp:39, and a is ignored)."""
+    if edges is not None:
+        edges = np.asarray(edges, np.float64)
+        a = float(edges[0])
+    else:
+        edges = np.array([a, a, a], np.float64)
+    A = np.diag(edges).astype(np.float64)
     if frac is None:
         frac = diamond_fractional(ncell)
         if natoms is not None:
             frac = frac[:natoms]
-    tau = np.asarray(frac, np.float64) * a
+    tau = np.asarray(frac, np.float64) * edges[None, :]
     gv = gvectors(A, ecut, nr)
     nat = tau.shape[0]
     eigr = phase_factor(A, nr, [nat], nat, tau.reshape(1, nat, 3), gv)
@@ -127,7 +135,7 @@ def make_system(a, ecut, ncell, nr, nbands, name="custom", natoms=None, frac=Non
         pkg[:ngw, 1 + d][nz] = q[nz] * ggk[nz, d] / t[nz]
     pkg_a = np.ascontiguousarray(pkg.reshape(-1))     # l fastest: [l + 4*ig]
     wnl = np.array(WNL_SP, np.float64)                 # WNL(1,l) at 0 + 1*l
-    vloc = gaussian_vloc(a, nr, tau)
+    vloc = gaussian_vloc(edges, nr, tau)
     return SynthSystem(name=name, a=a, ecut=ecut, nr=tuple(nr), nbands=nbands, A=A, tau=tau, gv=gv,
                        eigr=eigr, pkg_a=pkg_a, wnl=wnl, vloc=vloc, tpiba2=tpiba * tpiba)
 
@@ -135,6 +143,28 @@ def make_system(a, ecut, ncell, nr, nbands, name="custom", natoms=None, frac=Non
 def config(name):
     a, ecut, ncell, nr, nb = CONFIGS[name]
     return make_system(a, ecut, ncell, nr, nb, name=name)
+
+
+# Cells on the grids the reference's own runs use, which the compiled FFT lengths do not
+# cover: name -> (edges [bohr], ecut [Ry], grid, atoms, bands).  The grids are those
+# Setup.cpp:20-34 picks for the edges and cutoff (28^3: works/Si8/log_15Ry_lobpcg:54;
+# 22 x 12 x 16: works/NH3/log_davidson_cpu_10Ry:53; 40 x 40 x 82), or given as a user grid
+# (70 x 70 x 140: works/TiO2_anatase001/INPUT:13-14).  Atoms sit at seeded positions.
+GRID_CELLS = {
+    "si8_28": ((10.266, 10.266, 10.266), 15.0, (28, 28, 28), 8, 16),
+    "ortho_22x12x16": ((10.0, 5.0, 7.2), 10.0, (22, 12, 16), 3, 8),
+    "slab_40x40x82": ((19.0, 19.0, 40.0), 10.0, (40, 40, 82), 6, 16),
+    "anatase_70x70x140": ((19.1464, 19.1464, 40.0), 30.0, (70, 70, 140), 12, 8),
+}
+
+
+def grid_cell(name, seed=11):
+    edges, ecut, nr, nat, nb = GRID_CELLS[name]
+    if name == "si8_28":
+        frac = DIAMOND8
+    else:
+        frac = np.random.default_rng(seed).random((nat, 3))
+    return make_system(None, ecut, 1, nr, nb, name=name, frac=frac, edges=edges)
 
 
 def smooth_grid(n):
@@ -150,10 +180,11 @@ def smooth_grid(n):
     return n
 
 
-def uneven_batch(count=24, seed=2026):
+def uneven_batch(count=24, seed=2026, setup_grid=False):
     """Independent cubic cells with a ~ U[8,18] bohr and ecut ~ U[15,20] Ry.
 
-    The grid is the Setup.cpp:20-34 size rounded up to an even 2,3,5-smooth length.
+    The grid is the Setup.cpp:20-34 size rounded up to an even 2,3,5-smooth length
+    (setup_grid=True: the Setup.cpp size itself, any even length, as the reference runs).
     The cell holds round(8 (a/10.26)^3) atoms (Si density) at seeded random fractional
     positions, with 4 projectors per atom and 2*natoms + 4 bands (clipped to 8..96).
     """
@@ -163,7 +194,9 @@ def uneven_batch(count=24, seed=2026):
     for i in range(count):
         a = float(rng.uniform(8.0, 18.0))
         ecut = float(rng.uniform(15.0, 20.0))
-        n = smooth_grid(fft_grid(np.diag([a, a, a]), ecut)[0])
+        n = fft_grid(np.diag([a, a, a]), ecut)[0]
+        if not setup_grid:
+            n = smooth_grid(n)
         nat = max(1, int(round(8.0 * (a / 10.26) ** 3)))
         frac = rng.random((nat, 3))
         nb = int(min(96, max(8, 2 * nat + 4)))

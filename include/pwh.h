@@ -43,7 +43,8 @@ const char *pwh_last_error(void);
 int pwh_device_count(int *count);
 /* Self-test of the matrix-core lane layouts: kind 0 f64, 1 f32, 2 bf16. */
 int pwh_selftest_mfma(int kind, double *max_err);
-/* Whether the FFT passes have a compiled plan for this grid. */
+/* Whether the FFT passes can transform this grid: every length in 2..4096 (any factors; the
+ * reference plans any size through FFTW, fft_fftw3.cpp:29,35). */
 int pwh_grid_supported(int nr1, int nr2, int nr3);
 
 /* ---- setup: producers of the reference's hot-path globals (host, FP64) ------ */
@@ -118,6 +119,14 @@ typedef struct {
 int pwh_system_create(const pwh_system_desc *desc, pwh_system **sys);
 int pwh_system_destroy(pwh_system *sys);
 int pwh_system_info(const pwh_system *sys, int *ngw, int *ncol, int *nxa, size_t *device_bytes);
+/* No reference counterpart (FFTW picks its own plans): which implementation the system's FFT
+ * passes use.  0 auto (the compiled per-length kernels when they cover the grid: 2,3,5-smooth
+ * lengths 12..128 with NR1 = NR2; otherwise runtime plans), 1 compiled kernels (else
+ * PWH_ERR_UNSUPPORTED), 2 runtime plans with one (i3, band) plane per workgroup, 3 runtime
+ * plans with the plane pass split into i2 / i1 / i2 passes through HBM.  A plane that does
+ * not fit one workgroup's LDS in a precision takes the split passes in any mode.  *used
+ * (may be null) receives the mode in effect.  Results agree across modes to rounding. */
+int pwh_fft_path(pwh_system *sys, int path, int *used);
 /* The rho argument of ApplyHam_block (V_eff on the grid, EvalEnergy.cpp:128-131).
  * nnr != NR1*NR2*NR3 -> PWH_ERR_ARG (ApplyHam_block.cpp:18-21). */
 int pwh_set_vloc(pwh_system *sys, const double *rho, int nnr);
@@ -217,7 +226,8 @@ int pwh_eval_enl(pwh_system *sys, const double *c0, int ld, int nstates, const d
 /* fft_fftw3.cpp:13-46: in-place 3D DFT of data [nr3][nr2][nr1] complex (x fastest).
  * inverse != 0: exp(+i G.r), unscaled (FFTW_BACKWARD); inverse == 0: exp(-i G.r) then
  * 1/(nr1 nr2 nr3) (FFTW_FORWARD + zdscal).  FP64 on the GPU; each length must have a
- * compiled plan (2,3,5-smooth, 12..128), else PWH_ERR_UNSUPPORTED. */
+ * length in 2..4096 (compiled kernels for 2,3,5-smooth 12..128, runtime plans otherwise),
+ * else PWH_ERR_UNSUPPORTED. */
 int pwh_fft3d(double *data, int nr1, int nr2, int nr3, int inverse);
 /* eig_zheevd.cpp:5-34: Hermitian eigensolve of A [ldA x n] column-major, upper triangle
  * referenced ('V','U'); A <- eigenvectors, lambda [n] ascending.  FP64 on the GPU: the

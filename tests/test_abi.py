@@ -106,8 +106,11 @@ def test_grid_support_table():
     import pwh
     for n in (24, 30, 32, 36, 40, 48, 50, 54, 60, 64, 72, 80, 90, 96):
         assert pwh.grid_supported((n, n, n))
-    assert not pwh.grid_supported((22, 22, 22))        # prime factor 11
-    assert not pwh.grid_supported((32, 30, 32))        # non-square (i1,i2) planes
+    # any length 2..4096 (runtime plans): prime factors, non-square planes, long axes
+    for nr in ((22, 22, 22), (32, 30, 32), (22, 12, 16), (70, 70, 140), (41, 43, 47), (2, 2, 4096)):
+        assert pwh.grid_supported(nr), nr
+    for nr in ((1, 8, 8), (8, 8, 4097), (0, 8, 8)):
+        assert not pwh.grid_supported(nr), nr
 
 
 def test_no_device_is_a_clean_error():

@@ -62,10 +62,23 @@ def test_fft3d_plane_wave_closed_form():
     assert np.abs(y - spike).max() < 1e-13
 
 
+@pytest.mark.parametrize("nr", [(22, 12, 16), (14, 26, 38), (82, 6, 4), (41, 43, 2), (140, 70, 10),
+                                (121, 13, 7), (500, 3, 2)])
+def test_fft3d_any_length_matches_numpy(nr):
+    """Lengths outside the compiled set (runtime plans: radix 7, and primes 11, 13, 19, 41, 43
+    as direct stages)."""
+    import pwh
+    pwh.init(0)
+    x = _grid((nr[2], nr[1], nr[0]), seed=sum(nr) + 1)
+    n = x.size
+    assert relerr(pwh.fft3d(x, inverse=True), np.fft.ifftn(x) * n) < 1e-13
+    assert relerr(pwh.fft3d(x, inverse=False), np.fft.fftn(x) / n) < 1e-13
+
+
 def test_fft3d_unsupported_length():
     import pwh
     with pytest.raises(pwh.PwhError) as e:
-        pwh.fft3d(np.zeros((22, 22, 22), np.complex128), inverse=True)
+        pwh.fft3d(np.zeros((8, 8, 1), np.complex128), inverse=True)
     assert e.value.code == pwh.PWH_ERR_UNSUPPORTED
 
 

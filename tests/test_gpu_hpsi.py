@@ -94,7 +94,7 @@ def test_error_paths(si8, gpu_si8):
         pwh.System(g)
     assert e.value.code == pwh.PWH_ERR_ARG
     g = si8.globals()
-    g.nr = (22, 22, 22)                          # 22 = 2*11: no compiled plan
+    g.nr = (32, 32, 5000)                        # a length above 4096: no FFT plan
     with pytest.raises(pwh.PwhError) as e:
         pwh.System(g)
     assert e.value.code == pwh.PWH_ERR_UNSUPPORTED

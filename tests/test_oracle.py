@@ -24,7 +24,9 @@ def have_ref():
 # FFT conventions (fft_fftw3.cpp:13-46)
 # --------------------------------------------------------------------------
 
-@pytest.mark.parametrize("nr", [(3, 3, 2), (6, 8, 10), (12, 10, 14), (32, 32, 32), (5, 7, 9)])
+@pytest.mark.parametrize("nr", [(3, 3, 2), (6, 8, 10), (12, 10, 14), (32, 32, 32), (5, 7, 9),
+                                (22, 12, 16), (14, 26, 38), (82, 6, 4), (41, 43, 2), (28, 28, 28),
+                                (70, 4, 140)])
 def test_fft3d_matches_numpy(oracle, nr):
     rng = np.random.default_rng(0)
     n = int(np.prod(nr))
@@ -85,6 +87,21 @@ def test_fft_grid_matches_logs(oracle):
     assert oracle.fft_grid(np.diag([10.266] * 3), 15.0) == (28, 28, 28)
     assert oracle.fft_grid(np.diag([10.266] * 3), 8.0) == (20, 20, 20)
     assert oracle.fft_grid(np.diag([20.532] * 3), 8.0) == (38, 38, 38)
+
+
+def test_reference_example_grids():
+    """Every FFT grid the reference's example inputs lead to (tests/golden/ref_example_grids.json:
+    Setup.cpp:20-34's rule, a user fft_grid, or a grid its logs print) is one the library
+    transforms, and the library's Setup restatement reproduces the rule's grids."""
+    import json
+    import os
+    import pwh
+    d = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ref_example_grids.json")))
+    assert len(d) >= 30
+    for name, c in d.items():
+        assert pwh.grid_supported(c["grid"]), (name, c["grid"])
+        if "setup_grid" in c:
+            assert tuple(pwh.fft_grid(np.array(c["A"]).reshape(3, 3), c["ecut"])) == tuple(c["setup_grid"]), name
 
 
 def test_gvector_invariants(oracle):
