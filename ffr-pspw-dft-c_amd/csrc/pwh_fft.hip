@@ -472,6 +472,139 @@ __global__ __launch_bounds__(NT) void k_plane_band(SphereMaps m, const cx<T>* __
 }
 
 // ---------------------------------------------------------------------------
+// Pass 2 for 64 x 64 planes with the sphere in the band: wave-local four-step transforms
+// ---------------------------------------------------------------------------
+// Each 64-point line is an 8 x 8 four-step FFT owned by 8 lanes of one wave.  With
+// x[j + 8r] in lane j (register r):  y_j[s] = w64^(js) DFT8_r(x[j + 8r])[s], then
+// X[s + 8t] = DFT8_j(y_j[s])[t] in lane s.  The transpose between the two DFT8 steps goes
+// through LDS within the wave (wave-level ordering, no workgroup barrier), and lane s
+// ends up holding X[s + 8t] -- exactly the inputs of the next line transform in the same
+// direction, so inverse i1, x V and forward i1 chain in registers.  A plane needs five
+// workgroup barriers instead of the eighteen of the Stockham stages.
+//
+// Layout: LDS serves 16 consecutive lanes of 8-byte words (8 of 16-byte words) per cycle,
+// so consecutive lanes must fall on distinct banks.  Row stride 72 complex (8 mod 16) and
+// a skewed exchange slot 8s + ((j + s) & 7) give that for every access below: the column
+// passes use waves 0-3 (8 band columns per wave, lane l: column l % 8, j = l / 8), the row
+// pass all 8 waves (8 rows per wave, lane l: j = l % 8, row l / 8).
+constexpr int kW64Stride = 72;
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Exchange slot of value s from lane j.  Columns (element stride 72, 64 elements): skewed
+// within blocks of 8, 8s + ((j + s) & 7).  Rows (element stride 1, 72 elements of room):
+// 9s + j, whose offsets are compile-time constants from a per-lane base (9 is odd and a
+// row is 8 mod 16, so 16 consecutive lanes -- two rows of 8 -- still hit distinct banks).
+template <bool ROW>
+__device__ __forceinline__ int w64_slot(int s, int j) { return ROW ? 9 * s + j : 8 * s + ((j + s) & 7); }
+
+// The four-step line transform on 8 lanes: lane j passes x[j + 8r] in v (only r in
+// {0, 1, 6, 7} when PIN: the band inputs), gets X[j + 8t] back in v (only t in
+// {0, 1, 6, 7} when POUT).  Element e of the line is at line[e * es].  tw[r] =
+// exp(+2 pi i j r / 64) (r = 1..7) for this lane; SIGN < 0 uses its conjugate.
+template <typename T, int SIGN, bool PIN, bool POUT, bool ROW>
+__device__ __forceinline__ void w64_line(cx<T> (&v)[8], cx<T>* line, int es, int j, const cx<T> (&tw)[8]) {
+  if constexpr (PIN) dft8_in0167<T, SIGN>(v);
+  else dft<T, 8, SIGN>(v);
+#pragma unroll
+  for (int r = 1; r < 8; ++r) v[r] = v[r] * (SIGN > 0 ? tw[r] : conjg(tw[r]));
+#pragma unroll
+  for (int r = 0; r < 8; ++r) line[w64_slot<ROW>(r, j) * es] = v[r];
+  wave_lds_sync();
+#pragma unroll
+  for (int i = 0; i < 8; ++i) v[i] = line[w64_slot<ROW>(j, i) * es];
+  if constexpr (POUT) dft8_out0167<T, SIGN>(v);
+  else dft<T, 8, SIGN>(v);
+}
+
+// FP32: 77 VGPRs (6 waves per SIMD); capping it at 64 for the 8 waves the LDS would allow
+// spilled and measured slower (83.9 against 77.1 us at 64^3 x 128 bands)
+template <typename T>
+__global__ __launch_bounds__(512) void k_plane_w64(SphereMaps m, const cx<T>* __restrict__ twg,
+                                                   const T* __restrict__ V, cx<T>* __restrict__ W) {
+  constexpr int N = 64, S = kW64Stride, Q = N / 4, NBC = N / 2;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+  const int z = blockIdx.x, b = blockIdx.y;
+  const int jc = l >> 3, cl = l & 7;   // column passes: column cl of the wave, j = jc
+  const int jr = l & 7, yl = l >> 3;   // row pass: row yl of the wave, j = jr
+  // V at the row pass's points (row 8 wave + yl, x = jr + 8t), loaded first
+  const int y = 8 * wave + yl;
+  T vr[8];
+  const T* Vrow = V + ((size_t)z * N + y) * N;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) vr[t] = Vrow[jr + 8 * t];
+  // this lane's twiddles exp(+2 pi i j r / 64), r = 1..7, for the j of a pass
+  cx<T> twi[8];
+  auto load_tw = [&](int j) {
+    twi[0] = mk<T>(1, 0);
+#pragma unroll
+    for (int r = 1; r < 8; ++r) twi[r] = twg[j * r];
+  };
+  auto band_x = [](int t) { return t < Q ? t : t + NBC; };
+  for (int i = tid; i < NBC * NBC; i += 512) {
+    const int yy = i / NBC, xx = i - yy * NBC;
+    tile[band_x(yy) * S + band_x(xx)] = mk<T>(0, 0);
+  }
+  __syncthreads();
+  cx<T>* wrow = W + ((size_t)b * m.nr3 + z) * m.ncol_pad;
+  for (int c = tid; c < m.ncol; c += 512) {
+    const int xy = m.colxy[c];
+    tile[(xy >> 6) * S + (xy & 63)] = wrow[c];
+  }
+  __syncthreads();
+  cx<T> v[8];
+  // inverse along i2 on the 32 band columns (inputs in the band rows)
+  if (wave < 4) {
+    load_tw(jc);
+    cx<T>* col = tile + band_x(8 * wave + cl);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = (r < 2 || r >= 6) ? col[(jc + 8 * r) * S] : mk<T>(0, 0);
+    w64_line<T, +1, true, false, false>(v, col, S, jc, twi);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) col[(jc + 8 * t) * S] = v[t];
+  }
+  __syncthreads();
+  // rows: inverse along i1 (inputs in the band columns), times V, forward along i1 keeping
+  // the band columns
+  {
+    load_tw(jr);
+    cx<T>* row = tile + y * S;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = (r < 2 || r >= 6) ? row[jr + 8 * r] : mk<T>(0, 0);
+    w64_line<T, +1, true, false, true>(v, row, 1, jr, twi);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = vr[t] * v[t];
+    w64_line<T, -1, false, true, true>(v, row, 1, jr, twi);
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+      if (t < 2 || t >= 6) row[jr + 8 * t] = v[t];
+  }
+  __syncthreads();
+  // forward along i2 on the band columns, keeping the band rows
+  if (wave < 4) {
+    load_tw(jc);
+    cx<T>* col = tile + band_x(8 * wave + cl);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = col[(jc + 8 * r) * S];
+    w64_line<T, -1, false, true, false>(v, col, S, jc, twi);
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+      if (t < 2 || t >= 6) col[(jc + 8 * t) * S] = v[t];
+  }
+  __syncthreads();
+  for (int c = tid; c < m.ncol; c += 512) {
+    const int xy = m.colxy[c];
+    wrow[c] = tile[(xy >> 6) * S + (xy & 63)];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Density pass (EvalRhoPsi.cpp:138-150): per (i3, band group) plane, inverse FFT along i2
 // on the active columns and along i1, then sum w_b |psi_b(r)|^2 over the group's bands
 // ---------------------------------------------------------------------------
@@ -795,6 +928,22 @@ static hipError_t launch_plane_lds(const SphereMaps& m, const void* tw1, const v
 template <typename T, int N>
 static hipError_t launch_plane_n(const SphereMaps& m, const void* tw1, const void* tw2, const T* V,
                                  cx<T>* W, int nb, hipStream_t s) {
+  static const bool w64_on = [] {
+    const char* e = getenv("PWH_PLANE_W64");
+    return !(e && e[0] == '0');
+  }();
+  if constexpr (N == 64) {
+    if (m.band_xy && w64_on) {
+      constexpr size_t sm = sizeof(cx<T>) * (size_t)N * kW64Stride;
+      static const hipError_t attr = sm > 65536 ? hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_w64<T>),
+                                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm)
+                                                : hipSuccess;
+      if (attr != hipSuccess) return attr;
+      hipLaunchKernelGGL((k_plane_w64<T>), dim3(m.nr3, nb), dim3(512), sm, s, m, static_cast<const cx<T>*>(tw1),
+                         V, W);
+      return hipGetLastError();
+    }
+  }
   static const bool band_on = [] {
     const char* e = getenv("PWH_PLANE_BAND");
     return !(e && e[0] == '0');
