@@ -593,9 +593,10 @@ int zgemm_nn_small(int n, const cx<double>* A, const cx<double>* B, cx<double>* 
 // span_only (a caller that needs an orthonormal basis of span(X), not the R of a QR: the
 // ChFSI loop, whose Rayleigh-Ritz step rotates the basis anyway), with lw (2 nb^2) and
 // dmax (1) given: after the first pass G = X^H X = I + E is nearly the identity, and when
-// nb max|E| <= 1e-3 the second pass is X (I + E)^{-1/2} by the series
-// I - E/2 + 3E^2/8 - 5E^3/16 + 35E^4/128 (truncation below |E|^5 / 4 <= 2.5e-16), three
-// rocBLAS n x n products in place of the one-workgroup inverse Cholesky factor.
+// nb max|e_ij| <= 1e-3 (so ||E||_2 <= ||E||_F <= 1e-3) the second pass is X (I + E)^{-1/2}
+// by the series I - E/2 + 3E^2/8 - 5E^3/16 + 35E^4/128 (truncation 63/256 ||E||^5 / (1 -
+// ||E||) <= 2.5e-16), three rocBLAS n x n products in place of the one-workgroup inverse
+// Cholesky factor.
 int cholqr2(cx<double>* X, int ld, int nrows, int nb, cx<double>* tmp, cx<double>* S,
             cx<double>* part, size_t part_elems, rocblas_int* dinfo, cx<double>* lw = nullptr,
             double* dmax = nullptr) {
@@ -1330,6 +1331,18 @@ int pwh_ortho_qr(double* X, int nbasis, int nstates) {
     return rc;
   }
   rc = cholqr2(dX.p, nbasis, nbasis, nstates, dT.p, dS.p, dP.p, pe, di.p);
+  if (rc == PWH_ERR_LAPACK) {
+    // X^H X is not numerically positive definite (cond(X) above ~1e8, or rank deficient):
+    // Householder QR and the explicit Q, as the reference does (ortho_qr.cpp:19-26), on the
+    // untouched input.  dS (nstates^2 >= nstates) holds tau.
+    rc = upload(dX, reinterpret_cast<const cx<double>*>(X), n);
+    auto* a = reinterpret_cast<rocblas_double_complex*>(dX.p);
+    auto* tau = reinterpret_cast<rocblas_double_complex*>(dS.p);
+    if (rc == PWH_OK && rocsolver_zgeqrf(g.rb, nbasis, nstates, a, nbasis, tau) != rocblas_status_success)
+      rc = fail(PWH_ERR_LAPACK, "rocsolver_zgeqrf failed");
+    if (rc == PWH_OK && rocsolver_zungqr(g.rb, nbasis, nstates, nstates, a, nbasis, tau) != rocblas_status_success)
+      rc = fail(PWH_ERR_LAPACK, "rocsolver_zungqr failed");
+  }
   if (rc == PWH_OK) {
     hipError_t e = hipMemcpyAsync(X, dX.p, sizeof(cx<double>) * n, hipMemcpyDeviceToHost, g.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(g.stream);

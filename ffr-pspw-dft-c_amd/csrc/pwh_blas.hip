@@ -148,7 +148,8 @@ __global__ __launch_bounds__(256) void k_eye_residual(cx<double>* G, int n, doub
       v.x -= 1.0;
       G[e] = v;
     }
-    const double a = fmax(fabs(v.x), fabs(v.y));
+    // |e_ij| (the modulus, not the larger component), so n max|e_ij| bounds ||E||_2
+    const double a = hypot(v.x, v.y);
     m = (a == a) ? fmax(m, a) : INFINITY;
   }
   red[tid] = m;

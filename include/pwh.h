@@ -140,7 +140,9 @@ int pwh_chebyfilt(pwh_system *sys, double *X, int nstates, int degree, double lb
                   int precision);
 /* ortho_qr.cpp:3-34: orthonormalise the columns of X [nbasis x nstates] in place (FP64
  * CholeskyQR2 on the GPU; columns span the same space as Householder QR, column phases
- * follow the Cholesky convention R_jj > 0). */
+ * follow the Cholesky convention R_jj > 0).  When X^H X is not numerically positive
+ * definite (cond(X) above ~1e8, or X rank deficient) it runs Householder QR and the
+ * explicit Q (rocSOLVER zgeqrf + zungqr), as the reference does. */
 int pwh_ortho_qr(double *X, int nbasis, int nstates);
 /* lanczos.cpp:8-101: k-step Lanczos (FP64) from v0 [ngw] (normalised in place);
  * T [ldT x k] column-major as the reference fills it, f [ngw] the final residual. */

@@ -82,12 +82,32 @@ def test_ortho_qr_small_kernel_and_rocsolver_paths(nst):
     assert np.abs(Rq - ph[:, None] * Rn).max() < 1e-10 * np.abs(Rn).max()
 
 
-def test_ortho_qr_rank_deficient_reports_error():
+def test_ortho_qr_rank_deficient_falls_back_to_householder():
+    """X^H X singular: the Cholesky factor fails and ortho_qr runs Householder QR and the
+    explicit Q instead, as the reference does (ortho_qr.cpp:19-26): Q is orthonormal and
+    spans the columns of X."""
     import pwh
     X = np.ones((3, 50), np.complex128)
-    with pytest.raises(pwh.PwhError) as e:
-        pwh.ortho_qr(X)
-    assert e.value.code == pwh.PWH_ERR_LAPACK
+    Q = pwh.ortho_qr(X)
+    assert np.abs(Q.conj() @ Q.T - np.eye(3)).max() < 1e-13
+    P = Q.T @ Q.conj()
+    assert np.abs(P @ X.T - X.T).max() < 1e-12
+
+
+def test_ortho_qr_ill_conditioned_full_rank():
+    """cond(X) = 1e12 (CholeskyQR2 cannot factor X^H X, cond 1e24): the Householder
+    fallback gives an orthonormal Q with numpy's column space (to the 1e-16 * cond
+    sensitivity of that space's weakest direction)."""
+    import pwh
+    rng = np.random.default_rng(23)
+    n, m = 24, 400
+    U, _ = np.linalg.qr(rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    X = ((U * np.logspace(0, -12, n)) @ V.conj().T).T
+    Q = pwh.ortho_qr(X)
+    assert np.abs(Q.conj() @ Q.T - np.eye(n)).max() < 1e-13
+    Qn, _ = np.linalg.qr(X.T)
+    assert np.abs(Q.T @ Q.conj() - Qn @ Qn.conj().T).max() < 1e-3
 
 
 def test_rayleigh_ritz_and_residuals(si8, gpu_si8, oracle):
