@@ -206,6 +206,7 @@ struct pwh_system {
   Buf<cx<double>> lw;    // two n x n work matrices of the series second pass of CholeskyQR2
   Buf<double> dmax;      // its max |G - I|
   Buf<double> dlam, dnrm, dpart, dprec, dbo;
+  Buf<double> dTl;       // the Lanczos T on the device
   Buf<rocblas_int> info;
   // FP32 work
   Buf<cx<float>> X32, Y32, Y1_32, W32, Hnl32, Q32, part32;
@@ -888,69 +889,73 @@ int residuals_dev(pwh_system* s, const cx<double>* X, int nb, double* res_host, 
 
 // lanczos.cpp:8-101 on the sphere-order vector v0 (device, normalised in place).
 // T (host, ldT x k column-major), fnorm2 = ||f||^2, f (device) the final residual.
-int lanczos_dev(pwh_system* s, cx<double>* v0, int k, double* T, int ldT, cx<double>* f,
-                double* fnorm2) {
+// lanczos.cpp:8-101 on device vectors, enqueued on the library stream with every scalar of
+// the recurrence (norms, the Gram-Schmidt coefficients h and s, T in s->dTl, ld = k) on the
+// device: no host round trip between the k steps.
+constexpr int kLanczosDotSlices = 16;
+int lanczos_enqueue(pwh_system* s, cx<double>* v0, int k, cx<double>* f) {
   const int ld = s->ngw_pad, n = s->ngw;
-  CHK(s->Vl.ensure((size_t)k * ld));
-  CHK(s->C64.ensure((size_t)ld));          // Hv
-  CHK(s->S.ensure((size_t)2 * k + 2));     // h, s, coefficients
   cx<double>* V = s->Vl.p;
   cx<double>* Hv = s->C64.p;
   cx<double>* dh = s->S.p;
-  cx<double>* dcoef = s->S.p + k + 1;
-  std::vector<cx<double>> h(k), sv(k), coef(k);
-  for (int i = 0; i < k * k; ++i) T[i] = 0.0;   // the reference clears k*k entries (lanczos.cpp:26)
-  auto dot = [&](const cx<double>* x, const cx<double>* y, cx<double>* out) -> int {
-    HIPCHK(multi_dotc(x, ld, 1, y, n, dh, g.stream));
-    HIPCHK(hipMemcpyAsync(out, dh, sizeof(cx<double>), hipMemcpyDeviceToHost, g.stream));
-    HIPCHK(hipStreamSynchronize(g.stream));
-    return PWH_OK;
+  cx<double>* ds = s->S.p + k;
+  cx<double>* dn = s->S.p + 2 * k;
+  cx<double>* dpart = s->S.p + 2 * k + 2;      // k * kLanczosDotSlices partial sums
+  double* dT = s->dTl.p;
+  auto multi_dotc = [&](const cx<double>* A, int lda, int j, const cx<double>* x, int nn, cx<double>* out,
+                        hipStream_t st) {
+    return multi_dotc_split(A, lda, j, x, nn, dpart, kLanczosDotSlices, out, st);
   };
-  cx<double> tz;
-  CHK(dot(v0, v0, &tz));
-  double beta = 1.0 / std::sqrt(tz.x);
-  HIPCHK(axpby(v0, beta, nullptr, 0.0, v0, n, g.stream));
-  HIPCHK(hipMemcpyAsync(V, v0, sizeof(cx<double>) * ld, hipMemcpyDeviceToDevice, g.stream));
+  HIPCHK(fill_zero(dT, (size_t)k * k, g.stream));
+  // v0 <- v0 / ||v0|| (the reference scales the caller's vector in place), V(:,1) = v0
+  HIPCHK(multi_dotc(v0, ld, 1, v0, n, dn, g.stream));
+  HIPCHK(scale_rnorm(v0, dn, v0, n, nullptr, g.stream));
+  HIPCHK(axpby(v0, 1.0, nullptr, 0.0, V, n, g.stream));
   CHK(hpsi<double>(s, V, Hv, 1, 1.0, 0.0, 0.0, nullptr, PREC_FP64));
-  CHK(dot(V, Hv, &tz));
-  h[0] = tz;
-  T[0] = tz.x;
-  HIPCHK(axpby(Hv, 1.0, nullptr, 0.0, f, n, g.stream));
-  coef[0] = mk<double>(-h[0].x, -h[0].y);
-  HIPCHK(hipMemcpyAsync(dcoef, coef.data(), sizeof(cx<double>), hipMemcpyHostToDevice, g.stream));
-  HIPCHK(multi_axpy(V, ld, 1, dcoef, f, n, g.stream));
-  CHK(dot(V, f, &sv[0]));
-  h[0] = h[0] + sv[0];
-  coef[0] = mk<double>(-sv[0].x, -sv[0].y);
-  HIPCHK(hipMemcpyAsync(dcoef, coef.data(), sizeof(cx<double>), hipMemcpyHostToDevice, g.stream));
-  HIPCHK(multi_axpy(V, ld, 1, dcoef, f, n, g.stream));
+  // h_1 = v^H H v, T(1,1) = Re h_1; f = Hv - h_1 v; s_1 = v^H f; f -= s_1 v
+  HIPCHK(multi_dotc(V, ld, 1, Hv, n, dh, g.stream));
+  HIPCHK(tcol(dh, nullptr, 1, dT, g.stream));
+  HIPCHK(multi_axpy_sub(V, ld, 1, dh, Hv, f, n, g.stream));
+  HIPCHK(multi_dotc(V, ld, 1, f, n, ds, g.stream));
+  HIPCHK(multi_axpy_sub(V, ld, 1, ds, f, f, n, g.stream));
   for (int j = 2; j <= k; ++j) {
-    CHK(dot(f, f, &tz));
-    T[(j - 1) + (size_t)ldT * (j - 2)] = std::sqrt(tz.x);
-    beta = 1.0 / std::sqrt(tz.x);
+    // T(j, j-1) = ||f||, V(:,j) = f / ||f||, Hv = H V(:,j)
     cx<double>* vj = V + (size_t)(j - 1) * ld;
-    HIPCHK(axpby(f, beta, nullptr, 0.0, vj, n, g.stream));
+    HIPCHK(multi_dotc(f, ld, 1, f, n, dn, g.stream));
+    HIPCHK(scale_rnorm(f, dn, vj, n, dT + (j - 1) + (size_t)k * (j - 2), g.stream));
     CHK(hpsi<double>(s, vj, Hv, 1, 1.0, 0.0, 0.0, nullptr, PREC_FP64));
-    HIPCHK(axpby(Hv, 1.0, nullptr, 0.0, f, n, g.stream));
+    // h = V^H Hv, f = Hv - V h; one re-orthogonalisation: s = V^H f, f -= V s;
+    // T(1:j, j) = Re(h + s)
     HIPCHK(multi_dotc(V, ld, j, Hv, n, dh, g.stream));
-    HIPCHK(hipMemcpyAsync(h.data(), dh, sizeof(cx<double>) * j, hipMemcpyDeviceToHost, g.stream));
-    HIPCHK(hipStreamSynchronize(g.stream));
-    for (int i = 0; i < j; ++i) coef[i] = mk<double>(-h[i].x, -h[i].y);
-    HIPCHK(hipMemcpyAsync(dcoef, coef.data(), sizeof(cx<double>) * j, hipMemcpyHostToDevice, g.stream));
-    HIPCHK(multi_axpy(V, ld, j, dcoef, f, n, g.stream));
-    HIPCHK(multi_dotc(V, ld, j, f, n, dh, g.stream));
-    HIPCHK(hipMemcpyAsync(sv.data(), dh, sizeof(cx<double>) * j, hipMemcpyDeviceToHost, g.stream));
-    HIPCHK(hipStreamSynchronize(g.stream));
-    for (int i = 0; i < j; ++i) {
-      h[i] = h[i] + sv[i];
-      coef[i] = mk<double>(-sv[i].x, -sv[i].y);
-    }
-    HIPCHK(hipMemcpyAsync(dcoef, coef.data(), sizeof(cx<double>) * j, hipMemcpyHostToDevice, g.stream));
-    HIPCHK(multi_axpy(V, ld, j, dcoef, f, n, g.stream));
-    for (int i = 0; i < j; ++i) T[i + (size_t)ldT * (j - 1)] = h[i].x;
+    HIPCHK(multi_axpy_sub(V, ld, j, dh, Hv, f, n, g.stream));
+    HIPCHK(multi_dotc(V, ld, j, f, n, ds, g.stream));
+    HIPCHK(multi_axpy_sub(V, ld, j, ds, f, f, n, g.stream));
+    HIPCHK(tcol(dh, ds, j, dT + (size_t)k * (j - 1), g.stream));
   }
-  CHK(dot(f, f, &tz));
-  *fnorm2 = tz.x;
+  HIPCHK(multi_dotc(f, ld, 1, f, n, dn, g.stream));
+  return PWH_OK;
+}
+
+// T [ldT x k] (host) is filled as the reference fills it: T(i, j) = Re(h_i + s_i) for
+// j >= 2, T(1, 1) = Re h_1 without the re-orthogonalisation correction, T(j, j-1) = ||f||;
+// *fnorm2 = ||f||^2 of the final residual.
+int lanczos_dev(pwh_system* s, cx<double>* v0, int k, double* T, int ldT, cx<double>* f,
+                double* fnorm2) {
+  const int ld = s->ngw_pad;
+  CHK(s->Vl.ensure((size_t)k * ld));
+  CHK(s->C64.ensure((size_t)ld));              // Hv
+  CHK(s->S.ensure((size_t)(2 + kLanczosDotSlices) * k + 2));   // h, s, ||.||^2, dot partials
+  CHK(s->dTl.ensure((size_t)k * k));
+  CHK(lanczos_enqueue(s, v0, k, f));
+  std::vector<double> Th((size_t)k * k);
+  cx<double> nf;
+  HIPCHK(hipMemcpyAsync(Th.data(), s->dTl.p, sizeof(double) * k * k, hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipMemcpyAsync(&nf, s->S.p + 2 * k, sizeof(nf), hipMemcpyDeviceToHost, g.stream));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  for (int i = 0; i < k * k; ++i) T[i] = 0.0;   // the reference clears k*k entries (lanczos.cpp:26)
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < k; ++i) T[i + (size_t)ldT * j] = Th[i + (size_t)k * j];
+  *fnorm2 = nf.x;
   return PWH_OK;
 }
 

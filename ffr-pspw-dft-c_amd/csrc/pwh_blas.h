@@ -9,6 +9,19 @@ hipError_t multi_axpy(const cx<double>* V, int ld, int j, const cx<double>* coef
                       int n, hipStream_t s);
 hipError_t axpby(const cx<double>* x, double a, const cx<double>* z, double b, cx<double>* y, int n,
                  hipStream_t s);
+// Lanczos steps with their scalars kept on the device: f = src - V c (c read on the device);
+// y = x / sqrt(Re nrm[0]) and *tpos = sqrt(Re nrm[0]); out[i] = Re(h[i] + s[i]) (s may be null)
+hipError_t multi_axpy_sub(const cx<double>* V, int ld, int j, const cx<double>* c, const cx<double>* src,
+                          cx<double>* f, int n, hipStream_t s);
+hipError_t scale_rnorm(const cx<double>* x, const cx<double>* nrm, cx<double>* y, int n, double* tpos,
+                       hipStream_t s);
+hipError_t tcol(const cx<double>* h, const cx<double>* sv, int j, double* out, hipStream_t s);
+// multi_dotc over P slices of the reduction (part: j * P complex of scratch), for few
+// vectors j on a long n: (P, j) blocks instead of j, partials added in a fixed order
+hipError_t multi_dotc_split(const cx<double>* V, int ld, int j, const cx<double>* f, int n, cx<double>* part,
+                            int P, cx<double>* out, hipStream_t s);
+// p[0:n] = 0 by a kernel (a memset node does not replay reliably inside a captured graph)
+hipError_t fill_zero(double* p, size_t n, hipStream_t s);
 hipError_t symmetrize(cx<double>* A, int n, int ld, hipStream_t s);
 hipError_t zero_lower(cx<double>* A, int n, int ld, hipStream_t s);
 // Column-major n x n, ld = n, one workgroup: G <- G - I, maxabs[0] = max |Re|, |Im| of it

@@ -83,6 +83,96 @@ __global__ void k_multi_axpy(const cx<double>* __restrict__ V, int ld, int j,
   f[k] = acc;
 }
 
+// f[k] = src[k] - sum_{i<j} c[i] V[i][k], the coefficients read on the device (src may be f)
+__global__ void k_multi_axpy_sub(const cx<double>* __restrict__ V, int ld, int j, const cx<double>* __restrict__ c,
+                                 const cx<double>* src, cx<double>* f, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  cx<double> acc = src[k];
+  for (int i = 0; i < j; ++i) acc = acc - c[i] * V[(size_t)i * ld + k];
+  f[k] = acc;
+}
+
+// y = x / sqrt(Re nrm[0]) (the reciprocal formed once, as zdscal's beta); thread 0 also
+// writes sqrt(Re nrm[0]) to *tpos when tpos is given
+__global__ void k_scale_rnorm(const cx<double>* __restrict__ x, const cx<double>* __restrict__ nrm,
+                              cx<double>* __restrict__ y, int n, double* __restrict__ tpos) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const double r = sqrt(nrm[0].x);
+  if (tpos && k == 0) *tpos = r;
+  if (k >= n) return;
+  y[k] = (1.0 / r) * x[k];
+}
+
+// tcol[i] = Re(h[i] + s[i]) for i < j (s null: Re h[i])
+__global__ void k_tcol(const cx<double>* __restrict__ h, const cx<double>* __restrict__ s, int j,
+                       double* __restrict__ tcol) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= j) return;
+  tcol[i] = s ? h[i].x + s[i].x : h[i].x;
+}
+
+// out[i] = sum_k conj(V[i][k]) f[k] over P slices of k: grid (P, j) writes part[i][p],
+// then one thread per i adds its P partials in order (deterministic)
+__global__ __launch_bounds__(256) void k_dotc_part(const cx<double>* __restrict__ V, int ld,
+                                                   const cx<double>* __restrict__ f, int n, int P,
+                                                   cx<double>* __restrict__ part) {
+  __shared__ double sr[256], si[256];
+  const int p = blockIdx.x, i = blockIdx.y, t = threadIdx.x;
+  const int chunk = (n + P - 1) / P, k0 = p * chunk, k1 = min(n, k0 + chunk);
+  const cx<double>* v = V + (size_t)i * ld;
+  double ar = 0, ai = 0;
+  for (int k = k0 + t; k < k1; k += 256) {
+    const cx<double> a = v[k], b = f[k];
+    ar += a.x * b.x + a.y * b.y;
+    ai += a.x * b.y - a.y * b.x;
+  }
+  sr[t] = ar; si[t] = ai;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) { sr[t] += sr[t + w]; si[t] += si[t + w]; }
+    __syncthreads();
+  }
+  if (t == 0) part[(size_t)i * P + p] = mk<double>(sr[0], si[0]);
+}
+__global__ void k_dotc_fin(const cx<double>* __restrict__ part, int P, int j, cx<double>* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= j) return;
+  cx<double> s = mk<double>(0, 0);
+  for (int p = 0; p < P; ++p) s = s + part[(size_t)i * P + p];
+  out[i] = s;
+}
+hipError_t multi_dotc_split(const cx<double>* V, int ld, int j, const cx<double>* f, int n, cx<double>* part,
+                            int P, cx<double>* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_dotc_part, dim3(P, j), dim3(256), 0, s, V, ld, f, n, P, part);
+  hipLaunchKernelGGL(k_dotc_fin, dim3((j + 63) / 64), dim3(64), 0, s, part, P, j, out);
+  return hipGetLastError();
+}
+
+__global__ void k_fill_zero(double* __restrict__ p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+hipError_t fill_zero(double* p, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t multi_axpy_sub(const cx<double>* V, int ld, int j, const cx<double>* c, const cx<double>* src,
+                          cx<double>* f, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_multi_axpy_sub, dim3((n + 255) / 256), dim3(256), 0, s, V, ld, j, c, src, f, n);
+  return hipGetLastError();
+}
+hipError_t scale_rnorm(const cx<double>* x, const cx<double>* nrm, cx<double>* y, int n, double* tpos,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_rnorm, dim3((n + 255) / 256), dim3(256), 0, s, x, nrm, y, n, tpos);
+  return hipGetLastError();
+}
+hipError_t tcol(const cx<double>* h, const cx<double>* sv, int j, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_tcol, dim3((j + 255) / 256), dim3(256), 0, s, h, sv, j, out);
+  return hipGetLastError();
+}
+
 // y[k] = a * x[k] + b * z[k]   (z may be null)
 __global__ void k_axpby(const cx<double>* __restrict__ x, double a, const cx<double>* __restrict__ z,
                         double b, cx<double>* __restrict__ y, int n) {

@@ -485,6 +485,65 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
 }
 
 // ---------------------------------------------------------------------------
+// One band (nb = 1, the single-vector H|psi> of the Lanczos bounds and ApplyHam): the
+// products are matrix-vector, bound by reading beta once, so they skip the 64-wide MFMA
+// tiles.  Both reduce in a fixed order (deterministic).
+// ---------------------------------------------------------------------------
+// C[m] = wl[m % lm] sum_k conj(A[m][k]) x[k], one block per row
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_proj(const cx<T>* __restrict__ A, int lda, const cx<T>* __restrict__ x,
+                                                   int K, cx<T>* __restrict__ C, const T* __restrict__ wl, int lm) {
+  __shared__ T sr[256], si[256];
+  const int m = blockIdx.x, t = threadIdx.x;
+  const cx<T>* a = A + (size_t)m * lda;
+  T ar0 = 0, ai0 = 0, ar1 = 0, ai1 = 0;
+  int k = t;
+  for (; k + 256 < K; k += 512) {
+    const cx<T> u0 = a[k], v0 = x[k], u1 = a[k + 256], v1 = x[k + 256];
+    ar0 += u0.x * v0.x + u0.y * v0.y; ai0 += u0.x * v0.y - u0.y * v0.x;
+    ar1 += u1.x * v1.x + u1.y * v1.y; ai1 += u1.x * v1.y - u1.y * v1.x;
+  }
+  if (k < K) {
+    const cx<T> u0 = a[k], v0 = x[k];
+    ar0 += u0.x * v0.x + u0.y * v0.y; ai0 += u0.x * v0.y - u0.y * v0.x;
+  }
+  sr[t] = ar0 + ar1;
+  si[t] = ai0 + ai1;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) { sr[t] += sr[t + w]; si[t] += si[t + w]; }
+    __syncthreads();
+  }
+  if (t == 0) {
+    const T sc = wl ? wl[m % lm] : T(1);
+    C[m] = mk<T>(sc * sr[0], sc * si[0]);
+  }
+}
+
+// out[k] (+)= sum_m A[m][k] q[m]: a block takes 64 plane waves, its four waves the rows
+// m = c, c + 4, ... (c = wave), added in a fixed order
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_expand(const cx<T>* __restrict__ A, int lda, int M,
+                                                     const cx<T>* __restrict__ q, int K, cx<T>* __restrict__ out,
+                                                     int accumulate) {
+  __shared__ cx<T> part[4][64];
+  const int kk = threadIdx.x & 63, c = threadIdx.x >> 6;
+  const int k = blockIdx.x * 64 + kk;
+  cx<T> acc = mk<T>(0, 0);
+  if (k < K) {
+#pragma unroll 8
+    for (int m = c; m < M; m += 4) acc = acc + A[(size_t)m * lda + k] * q[m];
+  }
+  part[c][kk] = acc;
+  __syncthreads();
+  if (c == 0 && k < K) {
+    cx<T> v = (part[0][kk] + part[1][kk]) + (part[2][kk] + part[3][kk]);
+    if (accumulate) v = v + out[k];
+    out[k] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 static int gemm_target_blocks() {
@@ -611,6 +670,10 @@ hipError_t gemm_proj(const BetaGen& bg, const cx<T>* X, int ldx, int nb, int K, 
   ld.A = static_cast<const cx<T>*>(bg.B);
   ld.lda = bg.ld;
   const int M = bg.rows;
+  if (nb == 1) {
+    hipLaunchKernelGGL((k_gemv_proj<T>), dim3(M), dim3(256), 0, s, ld.A, ld.lda, X, K, C, wl, bg.lm);
+    return hipGetLastError();
+  }
   int S = 0;
   hipError_t e = run_proj<T, false>(ld, X, ldx, M, nb, K, part, part_elems, mode == PREC_BF16X3, s, &S,
                                     bg.target);
@@ -628,6 +691,11 @@ hipError_t gemm_expand(const BetaGen& bg, const cx<T>* Q, int nb, int K, cx<T>* 
   ld.A = static_cast<const cx<T>*>(bg.B);
   ld.lda = bg.ld;
   const int J = bg.rows;
+  if (nb == 1) {
+    hipLaunchKernelGGL((k_gemv_expand<T>), dim3((K + 63) / 64), dim3(256), 0, s, ld.A, ld.lda, J, Q, K, out,
+                       accumulate ? 1 : 0);
+    return hipGetLastError();
+  }
   dim3 grid(expand_blocks(K, nb));
   const int ks = gemm_ks<T>();
   if constexpr (std::is_same<T, float>::value) {
