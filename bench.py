@@ -475,10 +475,10 @@ def c4_rates(steps):
     psi = rng.standard_normal((s.nbands, s.ngw)) + 1j * rng.standard_normal((s.nbands, s.ngw))
     psi /= np.linalg.norm(psi, axis=1, keepdims=True)
     x.resident_set(psi)
-    tuned = {p: x.autotune(s.nbands, p)[0] for p in ("fp64", "fp32", "bf16x3")}
+    tuned = {p: x.autotune(s.nbands, p)[0] for p in ("fp64", "fp32", "bf16x3", "bf16x6")}
     out = {"workload": f"si216: a={s.a} bohr, {s.natoms} atoms, grid {s.nr[0]}^3, {s.ngw} PW, "
                        f"{s.nbands} bands, {4 * s.natoms} projectors"}
-    for p in ("fp64", "fp32", "bf16x3"):
+    for p in ("fp64", "fp32", "bf16x3", "bf16x6"):
         dt = time_steps(lambda: x.resident_apply(s.nbands, p), steps, 1)
         out[p] = {"ms_per_block": round(1e3 * dt / steps, 3), "band_applies_per_s": round(s.nbands * steps / dt, 1),
                   "gemm_target_blocks": tuned[p]}
@@ -515,7 +515,7 @@ def main():
     ap.add_argument("--config", default="si64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-chfsi", action="store_true", help="skip the ChFSI timing")
-    ap.add_argument("--only", default="", help="fp64|fp32|bf16x3: time one path only (profiling)")
+    ap.add_argument("--only", default="", help="fp64|fp32|bf16x3|bf16x6: time one path only (profiling)")
     ap.add_argument("--no-extra", action="store_true", help="skip the C3 batch and C4 sections")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default GEMM split-K target (no pwh_autotune search before timing)")
@@ -552,7 +552,7 @@ def main():
     sysm.resident_set(psi)
     info = sysm.info()
 
-    paths = ["fp64", "fp32", "bf16x3"] if not args.only else [args.only]
+    paths = ["fp64", "fp32", "bf16x3", "bf16x6"] if not args.only else [args.only]
     # set-up, outside the timed region: the projection GEMMs' split-K target per precision
     fixed = {k: int(v) for k, v in (kv.split("=") for kv in args.targets.split(",") if kv)}
     tuned = {}
@@ -613,7 +613,7 @@ def main():
         v0 = np.ones(s.ngw, np.complex128)
         ref_eig = None
         # one untimed iteration per path loads the solver kernels (first-call cost)
-        for p in ("fp64", "fp32", "bf16x3"):
+        for p in ("fp64", "fp32", "bf16x3", "bf16x6"):
             sysm.chfsi(X0, degree=2, niter=1, precision=p, lanczos_k=4, v0=v0)
         # the 128x128 FP64 Rayleigh-Ritz eigensolve: the library's own five-launch solver
         # (default), rocSOLVER D&C, QR iteration, Jacobi, Jacobi-D&C
@@ -628,7 +628,7 @@ def main():
         chfsi["rr_ms_per_iter_by_eigensolver"] = rr_ms
         chfsi["eigensolver"] = best
         log(f"bench: RR ms/iter by eigensolver {rr_ms}, using {best}")
-        for p in ("fp64", "fp32", "bf16x3"):
+        for p in ("fp64", "fp32", "bf16x3", "bf16x6"):
             sysm.resident_set(X0)       # the block is in HBM before the timed loop starts
             eig, res, st = sysm.resident_chfsi(nb, degree=10, niter=10, precision=p, lanczos_k=20, v0=v0)
             nx = nb - 2
@@ -647,7 +647,7 @@ def main():
         # the same start converged further (30 outer iterations): eigenvalue error of the
         # mixed paths against the FP64 run once both have converged
         conv, ref30 = {}, None
-        for p in ("fp64", "fp32", "bf16x3"):
+        for p in ("fp64", "fp32", "bf16x3", "bf16x6"):
             sysm.resident_set(X0)
             eig, res, st = sysm.resident_chfsi(nb, degree=10, niter=30, precision=p, lanczos_k=20, v0=v0)
             nx = nb - 2

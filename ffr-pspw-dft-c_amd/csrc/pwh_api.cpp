@@ -214,9 +214,9 @@ struct pwh_system {
   Buf<cx<double>> Ysp64;
   Buf<cx<float>> Ysp32;
   int resident_nb = 0;
-  // split-K block target of the projections per precision (PREC_FP64/FP32/BF16X3; 0: the
+  // split-K block target of the projections per precision (PREC_FP64/FP32/BF16X3/BF16X6; 0: the
   // library default), set by pwh_autotune
-  int tune[3] = {0, 0, 0};
+  int tune[4] = {0, 0, 0, 0};
   // second stream for the nonlocal GEMMs, which overlap the FFT passes (fork/join events)
   hipStream_t nl_stream = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -302,7 +302,7 @@ int hpsi_prepare(pwh_system* s, int nb) {
     const size_t nproj = (size_t)s->poff.back() + (size_t)s->na.back() * s->lm.back();
     CHK(Sel<T>::Q(s).ensure(nproj * nb));
     size_t pe = 0;
-    for (int m = 0; m < 3; ++m)
+    for (int m = 0; m < 4; ++m)
       if ((m == PREC_FP64) == std::is_same<T, double>::value)
         pe = std::max(pe, proj_part_elems(M, nb, s->ngw, s->tune[m]));
     CHK(Sel<T>::part(s).ensure(pe));
@@ -339,7 +339,7 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
     const size_t nproj = (size_t)s->poff.back() + (size_t)s->na.back() * s->lm.back();
     CHK(Sel<T>::Q(s).ensure(nproj * nb));
     const int target = s->tune[std::is_same<T, double>::value ? PREC_FP64
-                                                                : (mode == PREC_BF16X3 ? PREC_BF16X3 : PREC_FP32)];
+                               : (mode == PREC_BF16X3 || mode == PREC_BF16X6) ? mode : PREC_FP32];
     const size_t pe = proj_part_elems(M, nb, s->ngw, target);
     CHK(Sel<T>::part(s).ensure(pe));
     BetaGen bg{Sel<T>::Bt(s) + (size_t)s->poff[sp] * ld, M, s->lm[sp], ld, target};
@@ -1304,7 +1304,7 @@ int pwh_apply_ham_block(pwh_system* s, const double* vin, double* vout, int ngw_
   CHK(check_ld(s, nstates));
   if (!vin || !vout) return fail(PWH_ERR_ARG, "null vector");
   if (ngw_ik != s->ngw) return fail(PWH_ERR_ARG, "NGW_ik does not match the system");
-  if (precision < PWH_FP64 || precision > PWH_BF16X3) return fail(PWH_ERR_ARG, "bad precision");
+  if (precision < PWH_FP64 || precision > PWH_BF16X6) return fail(PWH_ERR_ARG, "bad precision");
   CHK(load_block(s, vin, nstates));
   CHK(s->HX.ensure((size_t)nstates * s->ngw_pad));
   CHK(apply_prec(s, s->X.p, s->HX.p, nstates, precision));
@@ -1315,7 +1315,7 @@ int pwh_chebyfilt(pwh_system* s, double* X, int nstates, int degree, double lb, 
                   int precision) {
   CHK(check_ld(s, nstates));
   if (!X) return fail(PWH_ERR_ARG, "null X");
-  if (precision < PWH_FP64 || precision > PWH_BF16X3) return fail(PWH_ERR_ARG, "bad precision");
+  if (precision < PWH_FP64 || precision > PWH_BF16X6) return fail(PWH_ERR_ARG, "bad precision");
   if (!(ub > lb)) return fail(PWH_ERR_ARG, "chebyfilt needs ub > lb");
   CHK(load_block(s, X, nstates));
   CHK(filter_prec(s, s->X.p, nstates, degree, lb, ub, precision));
@@ -1547,6 +1547,7 @@ static int chfsi_impl(pwh_system* s, double* X, int nstates, double* v0, pwh_chf
   if (!o || !eig || !res) return fail(PWH_ERR_ARG, "null argument");
   if (!X && nstates > s->resident_nb) return fail(PWH_ERR_ARG, "more states than resident");
   if (o->degree < 1 || o->niter < 0) return fail(PWH_ERR_ARG, "bad degree / niter");
+  if (o->precision < PWH_FP64 || o->precision > PWH_BF16X6) return fail(PWH_ERR_ARG, "bad precision");
   if (o->lanczos_k > 0 && !v0) return fail(PWH_ERR_ARG, "Lanczos needs v0");
   const int nb = nstates, NX = nstates - 2;
   const double t0 = now_s();
@@ -2825,6 +2826,7 @@ int pwh_resident_get(pwh_system* s, double* X, int nstates) {
 int pwh_resident_apply(pwh_system* s, int nstates, int precision) {
   CHK(check_ld(s, nstates));
   if (nstates > s->resident_nb) return fail(PWH_ERR_ARG, "more states than resident");
+  if (precision < PWH_FP64 || precision > PWH_BF16X6) return fail(PWH_ERR_ARG, "bad precision");
   const size_t n = (size_t)nstates * s->ngw_pad;
   if (precision == PWH_FP64) {
     CHK(s->HX.ensure(n));
@@ -2837,7 +2839,7 @@ int pwh_resident_apply(pwh_system* s, int nstates, int precision) {
 
 int pwh_autotune(pwh_system* s, int nstates, int precision, int* target_blocks, double* best_ms) {
   if (!s) return fail(PWH_ERR_ARG, "null system");
-  if (precision < PWH_FP64 || precision > PWH_BF16X3) return fail(PWH_ERR_ARG, "bad precision");
+  if (precision < PWH_FP64 || precision > PWH_BF16X6) return fail(PWH_ERR_ARG, "bad precision");
   if (target_blocks && *target_blocks != 0) {
     // a target known from an earlier run: set it, time nothing
     if (*target_blocks < 64 || *target_blocks > 65536) return fail(PWH_ERR_ARG, "target outside [64, 65536]");
@@ -2983,7 +2985,7 @@ int pwh_batch_get(pwh_batch* b, int i, double* X) {
 int pwh_batch_chebyfilt(pwh_batch* b, int degree, const double* lb, const double* ub, int precision,
                         int mode) {
   if (!b || !lb || !ub || degree < 1) return fail(PWH_ERR_ARG, "bad batch filter arguments");
-  if (precision < PWH_FP64 || precision > PWH_BF16X3) return fail(PWH_ERR_ARG, "bad precision");
+  if (precision < PWH_FP64 || precision > PWH_BF16X6) return fail(PWH_ERR_ARG, "bad precision");
   const int n = (int)b->sys.size();
   for (int i = 0; i < n; ++i) {
     if (!(ub[i] > lb[i])) return fail(PWH_ERR_ARG, "chebyfilt needs ub > lb");

@@ -14,12 +14,12 @@
 //            (ApplyHam_block.cpp:118-163)
 //          - rotation:  X <- X Q (ChebySCFLoop.cpp:304-306, CholeskyQR's X R^-1)
 //
-// Complex products use four real MFMAs (Re/Im of each operand).  Precisions:
-//   FP64   v_mfma_f64_16x16x4_f64
-//   FP32   v_mfma_f32_16x16x4_f32
-//   BF16X3 v_mfma_f32_16x16x16_bf16 on hi/lo bf16 splits of FP32 operands
-//          (a*b ~ ah*bh + ah*bl + al*bh), FP32 accumulation.
-// Block tile 64x64 complex, 4 waves of 32x32, LDS staging of Re/Im planes.
+// Precisions:
+//   FP64   v_mfma_f64_16x16x4_f64, four real products per complex one
+//   FP32   v_mfma_f32_16x16x4_f32, three real products per complex one (3M)
+//   BF16X3 / BF16X6  v_mfma_f32_16x16x32_bf16 on two- / three-term bf16 splits of the FP32
+//          operands (3 / 6 split products per real product), FP32 accumulation
+// Block tile 64x64 complex, 4 waves of 32x32, LDS staging.
 #include <cmath>
 #include <cstdlib>
 #include <type_traits>
@@ -32,7 +32,6 @@ namespace pwh {
 
 typedef double d4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
-typedef short s4v __attribute__((ext_vector_type(4)));
 
 template <typename T> struct AccT;
 template <> struct AccT<double> { using type = d4v; };
@@ -43,9 +42,6 @@ __device__ __forceinline__ d4v mfma4(double a, double b, d4v c) {
 }
 __device__ __forceinline__ f4v mfma4(float a, float b, f4v c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
-__device__ __forceinline__ f4v mfma16bf(s4v a, s4v b, f4v c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
 // Row of the 16x16 result held in accumulator register r of a lane in row-group lq
@@ -61,21 +57,6 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
   unsigned u = __float_as_uint(f);
   u += 0x7fffu + ((u >> 16) & 1u);
   return (unsigned short)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float(((unsigned)h) << 16); }
-
-struct Split4 { s4v hi, lo; };
-__device__ __forceinline__ Split4 split4(float a, float b, float c, float d) {
-  Split4 s;
-  unsigned short h0 = f2bf(a), h1 = f2bf(b), h2 = f2bf(c), h3 = f2bf(d);
-  s.hi = s4v{(short)h0, (short)h1, (short)h2, (short)h3};
-  s.lo = s4v{(short)f2bf(a - bf2f(h0)), (short)f2bf(b - bf2f(h1)), (short)f2bf(c - bf2f(h2)),
-             (short)f2bf(d - bf2f(h3))};
-  return s;
-}
-__device__ __forceinline__ s4v neg4(s4v v) {
-  return s4v{(short)(v[0] ^ (short)0x8000), (short)(v[1] ^ (short)0x8000),
-             (short)(v[2] ^ (short)0x8000), (short)(v[3] ^ (short)0x8000)};
 }
 
 constexpr int BT = 64;    // block tile (both output dims)
@@ -99,7 +80,7 @@ struct RowLoader {
 template <typename T, int KS>
 constexpr size_t gemm_smem() { return 4 * sizeof(T) * (size_t)KS * (BT + LPAD); }
 
-template <typename T, bool BETA, bool BF16X3, int KS>
+template <typename T, bool BETA, int KS>
 __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const cx<T>* __restrict__ B,
                                                   int ldb, int M, int N, int K, int kchunk,
                                                   cx<T>* __restrict__ part) {
@@ -172,7 +153,6 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
     if (k0 + KS < ke) fetch(k0 + KS);
 #pragma unroll
     for (int kc = 0; kc < KS; kc += 16) {
-    if constexpr (!BF16X3) {
       // substep sub reduces over k = kc + 4*lq + sub: lane quarters then read rows 4 apart,
       // 4*(BT+LPAD) = 16 banks, so the four quarters of a wave never share a bank
 #pragma unroll
@@ -217,44 +197,11 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
           }
         }
       }
-    } else {
-      Split4 ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int c = wm + t * 16 + li, d = wn + t * 16 + li, q = kc + 4 * lq;
-        ar[t] = split4(sAr[q][c], sAr[q + 1][c], sAr[q + 2][c], sAr[q + 3][c]);
-        ai[t] = split4(sAi[q][c], sAi[q + 1][c], sAi[q + 2][c], sAi[q + 3][c]);
-        br[t] = split4(sBr[q][d], sBr[q + 1][d], sBr[q + 2][d], sBr[q + 3][d]);
-        bi[t] = split4(sBi[q][d], sBi[q + 1][d], sBi[q + 2][d], sBi[q + 3][d]);
-      }
-      // conj(a) b = (ar br + ai bi) + i (ar bi - ai br), each real product as
-      // hi*hi + hi*lo + lo*hi; the six terms are issued over all 8 accumulators in turn
-      Split4 nai[2];
-#pragma unroll
-      for (int a = 0; a < 2; ++a) { nai[a].hi = neg4(ai[a].hi); nai[a].lo = neg4(ai[a].lo); }
-#pragma unroll
-      for (int term = 0; term < 6; ++term) {
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-          for (int b = 0; b < 2; ++b) {
-            const Split4& xr = term < 3 ? ar[a] : ai[a];
-            const Split4& yr = term < 3 ? br[b] : bi[b];
-            const Split4& xi = term < 3 ? ar[a] : nai[a];
-            const Split4& yi = term < 3 ? bi[b] : br[b];
-            const int t = term % 3;
-            const s4v xa = t == 2 ? xr.lo : xr.hi, ya = t == 1 ? yr.lo : yr.hi;
-            const s4v xb = t == 2 ? xi.lo : xi.hi, yb = t == 1 ? yi.lo : yi.hi;
-            cr[a][b] = mfma16bf(xa, ya, cr[a][b]);
-            ci[a][b] = mfma16bf(xb, yb, ci[a][b]);
-          }
-      }
-    }
     }
     __syncthreads();
   }
   if constexpr (THREE_M<T>) {
-    if constexpr (!BF16X3) {
+    {
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -316,7 +263,7 @@ __global__ __launch_bounds__(256) void k_reduce_parts(const cx<T>* __restrict__ 
 // ---------------------------------------------------------------------------
 // expand: out[n][k] (+)= sum_j A[j][k] Q[j][n]
 // ---------------------------------------------------------------------------
-template <typename T, bool BETA, bool BF16X3, bool QCM, int KS>
+template <typename T, bool BETA, bool QCM, int KS>
 __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const cx<T>* __restrict__ Q,
                                                     int ldq, int J, int N, int K,
                                                     cx<T>* __restrict__ out, int ldo, int accumulate) {
@@ -377,7 +324,6 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
     if (j0 + KS < J) fetch(j0 + KS);
 #pragma unroll
     for (int kc = 0; kc < KS; kc += 16) {
-    if constexpr (!BF16X3) {
       // reduction index j = kc + 4*lq + sub (conflict-free rows, as in k_gemm_proj)
 #pragma unroll
       for (int sub = 0; sub < 4; ++sub) {
@@ -419,43 +365,11 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
           }
         }
       }
-    } else {
-      Split4 ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int c = wm + t * 16 + li, d = wn + t * 16 + li, q = kc + 4 * lq;
-        ar[t] = split4(sAr[q][c], sAr[q + 1][c], sAr[q + 2][c], sAr[q + 3][c]);
-        ai[t] = split4(sAi[q][c], sAi[q + 1][c], sAi[q + 2][c], sAi[q + 3][c]);
-        br[t] = split4(sQr[q][d], sQr[q + 1][d], sQr[q + 2][d], sQr[q + 3][d]);
-        bi[t] = split4(sQi[q][d], sQi[q + 1][d], sQi[q + 2][d], sQi[q + 3][d]);
-      }
-      // a b = (ar br - ai bi) + i (ar bi + ai br), terms issued over all accumulators in turn
-      Split4 nai[2];
-#pragma unroll
-      for (int a = 0; a < 2; ++a) { nai[a].hi = neg4(ai[a].hi); nai[a].lo = neg4(ai[a].lo); }
-#pragma unroll
-      for (int term = 0; term < 6; ++term) {
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-          for (int b = 0; b < 2; ++b) {
-            const Split4& xr = term < 3 ? ar[a] : nai[a];
-            const Split4& yr = term < 3 ? br[b] : bi[b];
-            const Split4& xi = term < 3 ? ar[a] : ai[a];
-            const Split4& yi = term < 3 ? bi[b] : br[b];
-            const int t = term % 3;
-            const s4v xa = t == 2 ? xr.lo : xr.hi, ya = t == 1 ? yr.lo : yr.hi;
-            const s4v xb = t == 2 ? xi.lo : xi.hi, yb = t == 1 ? yi.lo : yi.hi;
-            cr[a][b] = mfma16bf(xa, ya, cr[a][b]);
-            ci[a][b] = mfma16bf(xb, yb, ci[a][b]);
-          }
-      }
-    }
     }
     __syncthreads();
   }
   if constexpr (THREE_M<T>) {
-    if constexpr (!BF16X3) {
+    {
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -478,6 +392,304 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
         if (k < K && n < N) {
           cx<T> v = mk<T>(cr[a][b][r], ci[a][b][r]);
           cx<T>* o = out + (size_t)n * ldo + k;
+          if (accumulate) v = v + *o;
+          *o = v;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// BF16 split GEMMs on the gfx950 16x16x32 BF16 MFMA
+// ---------------------------------------------------------------------------
+// Each FP32 operand x is split into NS bf16 terms, x ~ x_0 + x_1 (+ x_2), each term the
+// rounding of what the earlier ones leave, and a real product keeps the split products of
+// order <= 2^-8 (NS = 2: x0 y0 + x0 y1 + x1 y0, ~16 significant bits) or <= 2^-16 (NS = 3:
+// the six products with i + j <= 2, ~24 bits, FP32 level), accumulated in FP32 (the
+// construction of Henry, Tang and Heinecke, "Leveraging the bfloat16 artificial
+// intelligence datatype for higher-precision computations", ARITH 2019).  Operands are
+// split once, when a block stages its tile into LDS, as planes [component][term][row][k]
+// with the reduction index contiguous, so a lane reads its 8-element MFMA fragment as one
+// 16-byte word.  Complex products take four real ones (Re and Im of each operand; the
+// negated imaginary fragment is a sign flip of the bf16 words).  Block tile 64 x 64, four
+// waves of 32 x 32, 32 reduction steps per stage.
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+typedef unsigned int u2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ f4v mfma32bf(bf8v a, bf8v b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+constexpr int BKS = 32;          // reduction depth of one stage (one MFMA)
+constexpr int BKP = BKS + 8;     // LDS row length in bf16 (80 bytes, 16-byte aligned)
+
+template <int NS>
+constexpr size_t bf_smem() { return sizeof(unsigned short) * 2 * 2 * NS * (size_t)BT * BKP; }
+
+// plane (operand op, component c, term i): BT rows of BKP bf16
+template <int NS>
+__device__ __forceinline__ unsigned short* bf_plane(unsigned short* base, int op, int c, int i) {
+  return base + (size_t)((op * 2 + c) * NS + i) * BT * BKP;
+}
+
+// Splits 8 consecutive values (of one component) and stores the NS packed 16-byte words at
+// row `row`, reduction offset k8 of the operand's planes.
+// A pair is rounded to nearest-even by one v_cvt_pk_bf16_f32; the remainder r - rn(r) is
+// exact in FP32.
+template <int NS>
+__device__ __forceinline__ void bf_store8(unsigned short* base, int op, int c, int row, int k8, const float (&x)[8]) {
+  unsigned int w[NS][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    float r0 = x[2 * q], r1 = x[2 * q + 1];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      const unsigned int u = __builtin_bit_cast(unsigned int, __builtin_convertvector(f2v{r0, r1}, bf2v));
+      w[i][q] = u;
+      if (i + 1 < NS) {
+        r0 -= __uint_as_float(u << 16);
+        r1 -= __uint_as_float(u & 0xffff0000u);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NS; ++i)
+    *reinterpret_cast<u4v*>(bf_plane<NS>(base, op, c, i) + row * BKP + k8) = u4v{w[i][0], w[i][1], w[i][2], w[i][3]};
+}
+
+// The same for 4 consecutive values: NS 8-byte words at row `row`, reduction offset k4.
+template <int NS>
+__device__ __forceinline__ void bf_store4(unsigned short* base, int op, int c, int row, int k4, const float (&x)[4]) {
+  unsigned int w[NS][2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    float r0 = x[2 * q], r1 = x[2 * q + 1];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      const unsigned int u = __builtin_bit_cast(unsigned int, __builtin_convertvector(f2v{r0, r1}, bf2v));
+      w[i][q] = u;
+      if (i + 1 < NS) {
+        r0 -= __uint_as_float(u << 16);
+        r1 -= __uint_as_float(u & 0xffff0000u);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NS; ++i)
+    *reinterpret_cast<u2v*>(bf_plane<NS>(base, op, c, i) + row * BKP + k4) = u2v{w[i][0], w[i][1]};
+}
+
+__device__ __forceinline__ bf8v bf_neg(bf8v v) {
+  u4v u = __builtin_bit_cast(u4v, v);
+  u ^= 0x80008000u;
+  return __builtin_bit_cast(bf8v, u);
+}
+
+// One 32-deep stage of a wave's 32 x 32 tile: acc_re/acc_im (+)= sum of the split products of
+// (xr + i sx xi)(yr + i yi) with sx = -1 (conj(x) y, CONJ) or +1 (x y); rows of x are the
+// output rows (operand 0), of y the output columns (operand 1)
+template <int NS, bool CONJ>
+__device__ __forceinline__ void bf_stage(unsigned short* base, int wm, int wn, int li, int lq, f4v (&cr)[2][2],
+                                         f4v (&ci)[2][2]) {
+  bf8v xr[2][NS], xi[2][NS], yr[2][NS], yi[2][NS];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      const int rx = (wm + 16 * t + li) * BKP + 8 * lq, ry = (wn + 16 * t + li) * BKP + 8 * lq;
+      xr[t][i] = *reinterpret_cast<const bf8v*>(bf_plane<NS>(base, 0, 0, i) + rx);
+      xi[t][i] = *reinterpret_cast<const bf8v*>(bf_plane<NS>(base, 0, 1, i) + rx);
+      yr[t][i] = *reinterpret_cast<const bf8v*>(bf_plane<NS>(base, 1, 0, i) + ry);
+      yi[t][i] = *reinterpret_cast<const bf8v*>(bf_plane<NS>(base, 1, 1, i) + ry);
+    }
+  // CONJ:  Re = xr yr + xi yi, Im = xr yi - xi yr;  else Re = xr yr - xi yi, Im = xr yi + xi yr
+#pragma unroll
+  for (int p = 0; p < NS; ++p)
+#pragma unroll
+    for (int q = 0; p + q < NS; ++q) {
+      bf8v nx[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) nx[t] = bf_neg(xi[t][p]);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) cr[a][b] = mfma32bf(xr[a][p], yr[b][q], cr[a][b]);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) ci[a][b] = mfma32bf(xr[a][p], yi[b][q], ci[a][b]);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) cr[a][b] = mfma32bf(CONJ ? xi[a][p] : nx[a], yi[b][q], cr[a][b]);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) ci[a][b] = mfma32bf(CONJ ? nx[a] : xi[a][p], yr[b][q], ci[a][b]);
+    }
+}
+
+// proj: part[s][m][n] = sum_{k in chunk s} conj(A[m][k]) B[n][k]  (same tiling, split-K
+// decoding and XCD-aware order as k_gemm_proj)
+template <int NS>
+__global__ __launch_bounds__(GT) void k_gemm_proj_bf(const cx<float>* __restrict__ A, int lda,
+                                                     const cx<float>* __restrict__ B, int ldb, int M, int N,
+                                                     int K, int kchunk, cx<float>* __restrict__ part) {
+  extern __shared__ __align__(16) unsigned char gsm[];
+  unsigned short* base = reinterpret_cast<unsigned short*>(gsm);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tiles_n = (N + BT - 1) / BT, tiles = ((M + BT - 1) / BT) * tiles_n;
+  const int L = blockIdx.x, q = L >> 3;
+  const bool xcd = ((K + kchunk - 1) / kchunk) % 8 == 0;
+  const int tile = xcd ? q % tiles : L % tiles;
+  const int split = xcd ? (L & 7) + 8 * (q / tiles) : L / tiles;
+  if (split * kchunk >= K) return;
+  const int m0 = (tile / tiles_n) * BT, n0 = (tile % tiles_n) * BT;
+  const int ks = split * kchunk, ke = min(K, ks + kchunk);
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int li = lane & 15, lq = lane >> 4;
+  f4v cr[2][2], ci[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) { cr[a][b] = f4v{0, 0, 0, 0}; ci[a][b] = f4v{0, 0, 0, 0}; }
+  // staging: thread -> rows (t + 256 p) / 8 (p = 0, 1) of both operands, 4 consecutive k at
+  // 4 (t % 8), read as two 16-byte loads (global loads need 4-byte alignment only, so any
+  // lda works): a wave's load touches 16 rows of 32 bytes x 8 lanes.  Rows past the matrix
+  // read row 0 and k past the chunk the chunk's last column, both then replaced by zero, so
+  // every load is in bounds and the loop has no branches.
+  const int sk = 4 * (tid & 7);
+  int srow[2];
+  bool arow[2], brow[2];
+  const cx<float>* ap[2];
+  const cx<float>* bp[2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    srow[p] = (tid + GT * p) >> 3;
+    arow[p] = m0 + srow[p] < M;
+    brow[p] = n0 + srow[p] < N;
+    ap[p] = A + (size_t)(arow[p] ? m0 + srow[p] : 0) * lda;
+    bp[p] = B + (size_t)(brow[p] ? n0 + srow[p] : 0) * ldb;
+  }
+  f4v pa[2][2], pb[2][2];        // [p][h]: complex k0 + sk + 2h, + 1 as (re, im, re, im)
+  auto fetch = [&](int k0) {
+    if (k0 + BKS <= ke) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          f4v va, vb;
+          __builtin_memcpy(&va, ap[p] + k0 + sk + 2 * h, 16);
+          __builtin_memcpy(&vb, bp[p] + k0 + sk + 2 * h, 16);
+          pa[p][h] = arow[p] ? va : f4v{0, 0, 0, 0};
+          pb[p][h] = brow[p] ? vb : f4v{0, 0, 0, 0};
+        }
+    } else {
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int k = k0 + sk + e, kc = min(k, ke - 1);
+          const cx<float> va = ap[p][kc], vb = bp[p][kc];
+          const bool ka = arow[p] && k < ke, kb = brow[p] && k < ke;
+          pa[p][e >> 1][2 * (e & 1)] = ka ? va.x : 0.f;
+          pa[p][e >> 1][2 * (e & 1) + 1] = ka ? va.y : 0.f;
+          pb[p][e >> 1][2 * (e & 1)] = kb ? vb.x : 0.f;
+          pb[p][e >> 1][2 * (e & 1) + 1] = kb ? vb.y : 0.f;
+        }
+    }
+  };
+  fetch(ks);
+  for (int k0 = ks; k0 < ke; k0 += BKS) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const float ar[4] = {pa[p][0][0], pa[p][0][2], pa[p][1][0], pa[p][1][2]};
+      const float ai[4] = {pa[p][0][1], pa[p][0][3], pa[p][1][1], pa[p][1][3]};
+      const float br[4] = {pb[p][0][0], pb[p][0][2], pb[p][1][0], pb[p][1][2]};
+      const float bi[4] = {pb[p][0][1], pb[p][0][3], pb[p][1][1], pb[p][1][3]};
+      bf_store4<NS>(base, 0, 0, srow[p], sk, ar);
+      bf_store4<NS>(base, 0, 1, srow[p], sk, ai);
+      bf_store4<NS>(base, 1, 0, srow[p], sk, br);
+      bf_store4<NS>(base, 1, 1, srow[p], sk, bi);
+    }
+    __syncthreads();
+    if (k0 + BKS < ke) fetch(k0 + BKS);
+    bf_stage<NS, true>(base, wm, wn, li, lq, cr, ci);
+    __syncthreads();
+  }
+  cx<float>* P = part + (size_t)split * M * N;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm + a * 16 + 4 * lq + r, n = n0 + wn + b * 16 + li;
+        if (m < M && n < N) P[(size_t)m * N + n] = mk<float>(cr[a][b][r], ci[a][b][r]);
+      }
+}
+
+// expand: out[n][k] (+)= sum_j A[j][k] Q[j][n]  (same tiling and order as k_gemm_expand);
+// the staging transposes both tiles so that j is contiguous in LDS
+template <int NS>
+__global__ __launch_bounds__(GT) void k_gemm_expand_bf(const cx<float>* __restrict__ A, int lda,
+                                                       const cx<float>* __restrict__ Q, int ldq, int J, int N,
+                                                       int K, cx<float>* __restrict__ out, int ldo, int accumulate) {
+  extern __shared__ __align__(16) unsigned char gsm[];
+  unsigned short* base = reinterpret_cast<unsigned short*>(gsm);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tiles_n = (N + BT - 1) / BT;
+  const int L = blockIdx.x, q = L >> 3;
+  const int k0 = ((L & 7) + 8 * (q / tiles_n)) * BT, n0 = (q % tiles_n) * BT;
+  if (k0 >= K) return;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int li = lane & 15, lq = lane >> 4;
+  f4v cr[2][2], ci[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) { cr[a][b] = f4v{0, 0, 0, 0}; ci[a][b] = f4v{0, 0, 0, 0}; }
+  // staging: thread -> column t % 64 (a plane wave of A, a band of Q), 8 consecutive j at
+  // 8 (t / 64); consecutive threads read consecutive addresses of a row
+  const int sc = tid & 63, sj = 8 * (tid >> 6);
+  const bool kin = k0 + sc < K, nin = n0 + sc < N;
+  cx<float> pa[8], pq[8];
+  auto fetch = [&](int j0) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int j = j0 + sj + e;
+      pa[e] = (kin && j < J) ? A[(size_t)j * lda + k0 + sc] : mk<float>(0, 0);
+      pq[e] = (nin && j < J) ? Q[(size_t)j * ldq + n0 + sc] : mk<float>(0, 0);
+    }
+  };
+  fetch(0);
+  for (int j0 = 0; j0 < J; j0 += BKS) {
+    {
+      float ar[8], ai[8], qr[8], qi[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) { ar[e] = pa[e].x; ai[e] = pa[e].y; qr[e] = pq[e].x; qi[e] = pq[e].y; }
+      bf_store8<NS>(base, 0, 0, sc, sj, ar);
+      bf_store8<NS>(base, 0, 1, sc, sj, ai);
+      bf_store8<NS>(base, 1, 0, sc, sj, qr);
+      bf_store8<NS>(base, 1, 1, sc, sj, qi);
+    }
+    __syncthreads();
+    if (j0 + BKS < J) fetch(j0 + BKS);
+    bf_stage<NS, false>(base, wm, wn, li, lq, cr, ci);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = k0 + wm + a * 16 + 4 * lq + r, n = n0 + wn + b * 16 + li;
+        if (k < K && n < N) {
+          cx<float> v = mk<float>(cr[a][b][r], ci[a][b][r]);
+          cx<float>* o = out + (size_t)n * ldo + k;
           if (accumulate) v = v + *o;
           *o = v;
         }
@@ -601,50 +813,75 @@ static int expand_blocks(int K, int N) {
   return ((tk + 7) / 8) * 8 * tn;
 }
 
-template <typename T, bool BETA, bool BF, int KS>
+template <typename T, bool BETA, int KS>
 static hipError_t launch_proj_ks(dim3 grid, hipStream_t s, RowLoader<T, BETA> ld, const cx<T>* B, int ldb,
                                  int M, int N, int K, int kchunk, cx<T>* part) {
   constexpr size_t sm = gemm_smem<T, KS>();
-  static hipError_t attr = allow_smem(&k_gemm_proj<T, BETA, BF, KS>, sm);
+  static hipError_t attr = allow_smem(&k_gemm_proj<T, BETA, KS>, sm);
   if (attr != hipSuccess) return attr;
-  hipLaunchKernelGGL((k_gemm_proj<T, BETA, BF, KS>), grid, dim3(GT), sm, s, ld, B, ldb, M, N, K, kchunk, part);
+  hipLaunchKernelGGL((k_gemm_proj<T, BETA, KS>), grid, dim3(GT), sm, s, ld, B, ldb, M, N, K, kchunk, part);
   return hipGetLastError();
 }
 
-template <typename T, bool BETA, bool BF>
+template <typename T, bool BETA>
 static hipError_t launch_proj(int ks, dim3 grid, hipStream_t s, RowLoader<T, BETA> ld, const cx<T>* B,
                               int ldb, int M, int N, int K, int kchunk, cx<T>* part) {
-  if (ks == 32) return launch_proj_ks<T, BETA, BF, 32>(grid, s, ld, B, ldb, M, N, K, kchunk, part);
+  if (ks == 32) return launch_proj_ks<T, BETA, 32>(grid, s, ld, B, ldb, M, N, K, kchunk, part);
   if constexpr (sizeof(T) == 4)
-    if (ks == 64) return launch_proj_ks<T, BETA, BF, 64>(grid, s, ld, B, ldb, M, N, K, kchunk, part);
-  return launch_proj_ks<T, BETA, BF, 16>(grid, s, ld, B, ldb, M, N, K, kchunk, part);
+    if (ks == 64) return launch_proj_ks<T, BETA, 64>(grid, s, ld, B, ldb, M, N, K, kchunk, part);
+  return launch_proj_ks<T, BETA, 16>(grid, s, ld, B, ldb, M, N, K, kchunk, part);
 }
 
-template <typename T, bool BETA, bool BF, bool QCM, int KS>
+template <typename T, bool BETA, bool QCM, int KS>
 static hipError_t launch_expand_ks(dim3 grid, hipStream_t s, RowLoader<T, BETA> ld, const cx<T>* Q, int ldq,
                                    int J, int N, int K, cx<T>* out, int ldo, int acc) {
   constexpr size_t sm = gemm_smem<T, KS>();
-  static hipError_t attr = allow_smem(&k_gemm_expand<T, BETA, BF, QCM, KS>, sm);
+  static hipError_t attr = allow_smem(&k_gemm_expand<T, BETA, QCM, KS>, sm);
   if (attr != hipSuccess) return attr;
-  hipLaunchKernelGGL((k_gemm_expand<T, BETA, BF, QCM, KS>), grid, dim3(GT), sm, s, ld, Q, ldq, J, N, K, out,
+  hipLaunchKernelGGL((k_gemm_expand<T, BETA, QCM, KS>), grid, dim3(GT), sm, s, ld, Q, ldq, J, N, K, out,
                      ldo, acc);
   return hipGetLastError();
 }
 
-template <typename T, bool BETA, bool BF, bool QCM>
+template <typename T, bool BETA, bool QCM>
 static hipError_t launch_expand(int ks, dim3 grid, hipStream_t s, RowLoader<T, BETA> ld, const cx<T>* Q,
                                 int ldq, int J, int N, int K, cx<T>* out, int ldo, int acc) {
-  if (ks == 32) return launch_expand_ks<T, BETA, BF, QCM, 32>(grid, s, ld, Q, ldq, J, N, K, out, ldo, acc);
+  if (ks == 32) return launch_expand_ks<T, BETA, QCM, 32>(grid, s, ld, Q, ldq, J, N, K, out, ldo, acc);
   if constexpr (sizeof(T) == 4)
-    if (ks == 64) return launch_expand_ks<T, BETA, BF, QCM, 64>(grid, s, ld, Q, ldq, J, N, K, out, ldo, acc);
-  return launch_expand_ks<T, BETA, BF, QCM, 16>(grid, s, ld, Q, ldq, J, N, K, out, ldo, acc);
+    if (ks == 64) return launch_expand_ks<T, BETA, QCM, 64>(grid, s, ld, Q, ldq, J, N, K, out, ldo, acc);
+  return launch_expand_ks<T, BETA, QCM, 16>(grid, s, ld, Q, ldq, J, N, K, out, ldo, acc);
 }
 
+// terms of the bf16 split of a precision mode (0: not a split mode)
+static int bf_terms(int mode) { return mode == PREC_BF16X3 ? 2 : mode == PREC_BF16X6 ? 3 : 0; }
+
+template <int NS>
+static hipError_t launch_proj_bf(dim3 grid, hipStream_t s, const cx<float>* A, int lda, const cx<float>* B, int ldb,
+                                 int M, int N, int K, int kchunk, cx<float>* part) {
+  constexpr size_t sm = bf_smem<NS>();
+  static hipError_t attr = allow_smem(&k_gemm_proj_bf<NS>, sm);
+  if (attr != hipSuccess) return attr;
+  hipLaunchKernelGGL((k_gemm_proj_bf<NS>), grid, dim3(GT), sm, s, A, lda, B, ldb, M, N, K, kchunk, part);
+  return hipGetLastError();
+}
+
+template <int NS>
+static hipError_t launch_expand_bf(dim3 grid, hipStream_t s, const cx<float>* A, int lda, const cx<float>* Q, int ldq,
+                                   int J, int N, int K, cx<float>* out, int ldo, int acc) {
+  constexpr size_t sm = bf_smem<NS>();
+  static hipError_t attr = allow_smem(&k_gemm_expand_bf<NS>, sm);
+  if (attr != hipSuccess) return attr;
+  hipLaunchKernelGGL((k_gemm_expand_bf<NS>), grid, dim3(GT), sm, s, A, lda, Q, ldq, J, N, K, out, ldo, acc);
+  return hipGetLastError();
+}
+
+// split-K projection: S chunks of kchunk plane waves (a multiple of the stage depth)
 template <typename T, bool BETA>
 static hipError_t run_proj(RowLoader<T, BETA> ld, const cx<T>* B, int ldb, int M, int N, int K,
-                           cx<T>* part, size_t part_elems, bool bf16, hipStream_t s, int* S_out,
+                           cx<T>* part, size_t part_elems, int mode, hipStream_t s, int* S_out,
                            int target = 0) {
-  const int ks = gemm_ks<T>();
+  const int nsplit = std::is_same<T, float>::value ? bf_terms(mode) : 0;
+  const int ks = nsplit ? BKS : gemm_ks<T>();
   const int tiles = ((M + BT - 1) / BT) * ((N + BT - 1) / BT);
   int S = choose_split(tiles, K, part_elems, (size_t)M * N, target);
   int kchunk = (K + S - 1) / S;
@@ -654,10 +891,11 @@ static hipError_t run_proj(RowLoader<T, BETA> ld, const cx<T>* B, int ldb, int M
   dim3 grid(tiles * ((S + 7) / 8) * 8);   // 1D, decoded XCD-aware in the kernel
   hipError_t e;
   if constexpr (std::is_same<T, float>::value) {
-    e = bf16 ? launch_proj<T, BETA, true>(ks, grid, s, ld, B, ldb, M, N, K, kchunk, part)
-             : launch_proj<T, BETA, false>(ks, grid, s, ld, B, ldb, M, N, K, kchunk, part);
+    if (nsplit == 3) e = launch_proj_bf<3>(grid, s, ld.A, ld.lda, B, ldb, M, N, K, kchunk, part);
+    else if (nsplit == 2) e = launch_proj_bf<2>(grid, s, ld.A, ld.lda, B, ldb, M, N, K, kchunk, part);
+    else e = launch_proj<T, BETA>(ks, grid, s, ld, B, ldb, M, N, K, kchunk, part);
   } else {
-    e = launch_proj<T, BETA, false>(ks, grid, s, ld, B, ldb, M, N, K, kchunk, part);
+    e = launch_proj<T, BETA>(ks, grid, s, ld, B, ldb, M, N, K, kchunk, part);
   }
   *S_out = S;
   return e;
@@ -675,8 +913,7 @@ hipError_t gemm_proj(const BetaGen& bg, const cx<T>* X, int ldx, int nb, int K, 
     return hipGetLastError();
   }
   int S = 0;
-  hipError_t e = run_proj<T, false>(ld, X, ldx, M, nb, K, part, part_elems, mode == PREC_BF16X3, s, &S,
-                                    bg.target);
+  hipError_t e = run_proj<T, false>(ld, X, ldx, M, nb, K, part, part_elems, mode, s, &S, bg.target);
   if (e != hipSuccess) return e;
   const int tot = M * nb;
   hipLaunchKernelGGL((k_reduce_parts<T, T>), dim3((tot + 255) / 256), dim3(256), 0, s, part, S, M,
@@ -697,12 +934,13 @@ hipError_t gemm_expand(const BetaGen& bg, const cx<T>* Q, int nb, int K, cx<T>* 
     return hipGetLastError();
   }
   dim3 grid(expand_blocks(K, nb));
-  const int ks = gemm_ks<T>();
+  const int acc = accumulate ? 1 : 0;
   if constexpr (std::is_same<T, float>::value) {
-    if (mode == PREC_BF16X3)
-      return launch_expand<T, false, true, false>(ks, grid, s, ld, Q, nb, J, nb, K, out, ldo, accumulate ? 1 : 0);
+    const int nsplit = bf_terms(mode);
+    if (nsplit == 3) return launch_expand_bf<3>(grid, s, ld.A, ld.lda, Q, nb, J, nb, K, out, ldo, acc);
+    if (nsplit == 2) return launch_expand_bf<2>(grid, s, ld.A, ld.lda, Q, nb, J, nb, K, out, ldo, acc);
   }
-  return launch_expand<T, false, false, false>(ks, grid, s, ld, Q, nb, J, nb, K, out, ldo, accumulate ? 1 : 0);
+  return launch_expand<T, false, false>(gemm_ks<T>(), grid, s, ld, Q, nb, J, nb, K, out, ldo, acc);
 }
 
 hipError_t gemm_cn_f64(const cx<double>* A, int lda, int M, const cx<double>* B, int ldb, int N,
@@ -711,7 +949,7 @@ hipError_t gemm_cn_f64(const cx<double>* A, int lda, int M, const cx<double>* B,
   RowLoader<double, false> ld{};
   ld.A = A; ld.lda = lda;
   int S = 0;
-  hipError_t e = run_proj<double, false>(ld, B, ldb, M, N, K, part, part_elems, false, s, &S);
+  hipError_t e = run_proj<double, false>(ld, B, ldb, M, N, K, part, part_elems, PREC_FP64, s, &S);
   if (e != hipSuccess) return e;
   const int tot = M * N;
   hipLaunchKernelGGL((k_reduce_parts<double, double>), dim3((tot + 255) / 256), dim3(256), 0, s, part,
@@ -724,12 +962,12 @@ hipError_t gemm_nn_f64(const cx<double>* A, int lda, int J, const cx<double>* Q,
   RowLoader<double, false> ld{};
   ld.A = A; ld.lda = lda;
   dim3 grid(expand_blocks(K, N));
-  return launch_expand<double, false, false, true>(gemm_ks<double>(), grid, s, ld, Q, ldq, J, N, K, out, ldo, 0);
+  return launch_expand<double, false, true>(gemm_ks<double>(), grid, s, ld, Q, ldq, J, N, K, out, ldo, 0);
 }
 
 // ---------------------------------------------------------------------------
 // Self-test of the MFMA operand / result lane layouts assumed above: one wave
-// computes D = A(16x4) B(4x16) (kind 0: f64, 1: f32) or A(16x16) B(16x16)
+// computes D = A(16x4) B(4x16) (kind 0: f64, 1: f32) or A(16x32) B(32x16)
 // (kind 2: bf16) and the host compares with a plain product.
 // ---------------------------------------------------------------------------
 __global__ void k_mfma_probe(int kind, const double* A, const double* B, double* D) {
@@ -743,19 +981,20 @@ __global__ void k_mfma_probe(int kind, const double* A, const double* B, double*
     c = mfma4((float)A[li * 4 + lq], (float)B[lq * 16 + li], c);
     for (int r = 0; r < 4; ++r) D[acc_row<float>(lq, r) * 16 + li] = c[r];
   } else {
+    // 16x16x32 bf16: lane (li, lq) holds A[li][8 lq + v] and B[8 lq + v][li], v < 8
     f4v c = {0, 0, 0, 0};
-    s4v a, b;
-    for (int v = 0; v < 4; ++v) {
-      a[v] = (short)f2bf((float)A[li * 16 + 4 * lq + v]);
-      b[v] = (short)f2bf((float)B[(4 * lq + v) * 16 + li]);
+    bf8v a, b;
+    for (int v = 0; v < 8; ++v) {
+      a[v] = __builtin_bit_cast(__bf16, f2bf((float)A[li * 32 + 8 * lq + v]));
+      b[v] = __builtin_bit_cast(__bf16, f2bf((float)B[(8 * lq + v) * 16 + li]));
     }
-    c = mfma16bf(a, b, c);
+    c = mfma32bf(a, b, c);
     for (int r = 0; r < 4; ++r) D[acc_row<float>(lq, r) * 16 + li] = c[r];
   }
 }
 
 hipError_t mfma_selftest(int kind, double* max_err) {
-  const int KD = (kind == 2) ? 16 : 4;
+  const int KD = (kind == 2) ? 32 : 4;
   std::vector<double> hA(16 * KD), hB(KD * 16), hD(256), ref(256, 0.0);
   for (int i = 0; i < 16 * KD; ++i) hA[i] = (double)((i * 7) % 13) - 6.0;   // small integers: exact
   for (int i = 0; i < KD * 16; ++i) hB[i] = (double)((i * 5) % 11) - 5.0;

@@ -45,7 +45,7 @@ __host__ __device__ __forceinline__ cx<T> mul_si(cx<T> a) {
   return SIGN > 0 ? mk<T>(-a.y, a.x) : mk<T>(a.y, -a.x);
 }
 
-enum Precision { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2 };
+enum Precision { PREC_FP64 = 0, PREC_FP32 = 1, PREC_BF16X3 = 2, PREC_BF16X6 = 3 };
 
 // A runtime-length FFT plan: the Stockham stages' radices in order (2, 3, 4, 5, 7, 8 have
 // register butterflies; any other prime p runs as a direct length-p DFT per output).

@@ -22,8 +22,8 @@ LIB_PATH = os.path.join(PKG_DIR, "libpwh.so")
 SHIM_PATH = os.path.join(PKG_DIR, "libpwh_shim.so")
 REFGLOBALS_PATH = os.path.join(PKG_DIR, "libpwh_refglobals.so")
 
-FP64, FP32, BF16X3 = 0, 1, 2
-PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
+FP64, FP32, BF16X3, BF16X6 = 0, 1, 2, 3
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "bf16x6": BF16X6}
 
 PWH_OK = 0
 PWH_ERR_ARG = -1

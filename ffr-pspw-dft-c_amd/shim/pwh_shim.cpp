@@ -26,7 +26,7 @@
 // global arrays it was built from change.  Error behaviour follows the reference:
 // the conditions it aborts on (nnr != NNR, lm_end > 4) abort here too, and
 // factorisation failures in ortho_qr print and return.  The filter precision of
-// chebyfilt comes from PWH_PRECISION=fp64|fp32|bf16x3 (default fp64).
+// chebyfilt comes from PWH_PRECISION=fp64|fp32|bf16x3|bf16x6 (default fp64).
 //
 // lanczos clears T[0 .. k*k) and fills T as the reference does; eig_zheevd prints
 // "ERROR calling zheevd_ : info = %d" and returns on a failed factorisation, as the
@@ -88,6 +88,7 @@ int filter_precision() {
   if (!p) return PWH_FP64;
   if (!strcmp(p, "fp32")) return PWH_FP32;
   if (!strcmp(p, "bf16x3")) return PWH_BF16X3;
+  if (!strcmp(p, "bf16x6")) return PWH_BF16X6;
   return PWH_FP64;
 }
 

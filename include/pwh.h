@@ -32,7 +32,9 @@ extern "C" {
 
 #define PWH_FP64 0    /* FP64 FFT, FP64 MFMA GEMMs */
 #define PWH_FP32 1    /* FP32 FFT, FP32 MFMA GEMMs */
-#define PWH_BF16X3 2  /* FP32 FFT, BF16 hi/lo split GEMMs with FP32 accumulation */
+#define PWH_BF16X3 2  /* FP32 FFT, GEMMs on two-term (hi/lo) bf16 splits, FP32 accumulation */
+#define PWH_BF16X6 3  /* FP32 FFT, GEMMs on three-term bf16 splits (six products, FP32-level
+                         accuracy), FP32 accumulation */
 
 typedef struct pwh_system pwh_system;
 
@@ -252,7 +254,7 @@ int pwh_eig_zheevd_noalloc(double *A_device, int ldA, double *lambda_device, int
 typedef struct {
   int degree;            /* CHEBYPOL_DEGREE */
   int niter;             /* outer iterations */
-  int precision;         /* filter precision (PWH_FP64 / PWH_FP32 / PWH_BF16X3) */
+  int precision;         /* filter precision (PWH_FP64 / PWH_FP32 / PWH_BF16X3 / PWH_BF16X6) */
   int lanczos_k;         /* steps of the FP64 Lanczos bound estimate */
   int bounds_mode;       /* 0: ub = max Ritz + ||f|| once, lb = largest RR value after each RR
                             1: the reference's rules (ChebySCFLoop.cpp:176-196,355-368) */

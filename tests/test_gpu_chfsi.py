@@ -21,7 +21,7 @@ def test_chebyfilt_fp64_matches_oracle(si8, gpu_si8, oracle):
     assert relerr(out, ref) < 1e-11
 
 
-@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16x3", 1e-4)])
+@pytest.mark.parametrize("prec,tol", [("fp32", 1e-5), ("bf16x3", 1e-4), ("bf16x6", 1e-5)])
 def test_chebyfilt_mixed_error(si8, gpu_si8, oracle, prec, tol):
     X = si8.psi(16, seed=12)
     ref = oracle.chebyfilt(oracle_globals(si8), X, 6, 0.3, 9.0, si8.vloc, si8.ngw)

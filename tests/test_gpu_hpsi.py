@@ -1,7 +1,8 @@
 """GPU parity of H|psi> (ApplyHam_block / ApplyHam) against the FP64 oracle.
 
 Tolerances: FP64 path 1e-12 relative (per band, 2-norm) as the reference-parity bar;
-FP32 path 2e-6 and BF16x3 path 2e-5 relative (single-precision FFT / GEMM rounding).
+FP32 path 2e-6 and BF16x3 path 2e-5 relative (single-precision FFT / GEMM rounding);
+BF16x6 (three-term bf16 splits) at the FP32 bound, 2e-6.
 """
 import numpy as np
 import pytest
@@ -36,7 +37,7 @@ def test_apply_ham_single_band(si8, gpu_si8, oracle):
     assert relerr(out, ref) < 1e-12
 
 
-@pytest.mark.parametrize("prec,tol", [("fp32", 2e-6), ("bf16x3", 2e-5)])
+@pytest.mark.parametrize("prec,tol", [("fp32", 2e-6), ("bf16x3", 2e-5), ("bf16x6", 2e-6)])
 def test_apply_ham_block_mixed_error(si8, gpu_si8, oracle, prec, tol):
     psi = si8.psi(16, seed=3)
     ref = oracle.apply_ham_block(oracle_globals(si8), psi, si8.vloc, si8.ngw)
@@ -116,9 +117,9 @@ def test_si64_fp64_subset_matches_oracle(oracle):
     assert band_relerr(out[idx], ref) < 1e-12
     T = psi.conj() @ out.T          # psi^H H psi, 128 x 128
     assert np.abs(T - T.conj().T).max() < 1e-12 * np.abs(T).max()
-    for prec, tol in (("fp32", 2e-6), ("bf16x3", 2e-5)):
+    for prec, tol in (("fp32", 2e-6), ("bf16x3", 2e-5), ("bf16x6", 2e-6)):
         o2 = g.apply_ham_block(psi, prec)
-        assert band_relerr(o2[idx], ref) < tol
+        assert band_relerr(o2[idx], ref) < tol, prec
     g.close()
 
 
@@ -145,7 +146,7 @@ def test_autotune_targets_keep_results(oracle):
     idx = [0, 77, 127]
     ref = oracle.apply_ham_block(oracle_globals(s), psi[idx], s.vloc, s.ngw, nthreads=8)
     assert band_relerr(g.apply_ham_block(psi)[idx], ref) < 1e-12
-    for bad in (dict(nstates=129), dict(nstates=128, precision=3), dict(nstates=0, target=32)):
+    for bad in (dict(nstates=129), dict(nstates=128, precision=4), dict(nstates=0, target=32)):
         with pytest.raises(pwh.PwhError) as e:
             g.autotune(**bad)
         assert e.value.code == pwh.PWH_ERR_ARG
