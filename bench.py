@@ -2,9 +2,11 @@
 128 bands, 256 projectors), mixed precision vs FP64, plus ChFSI s/iter.
 
 A step is one H|psi> over the whole 128-band block, resident in HBM, in the mixed path
-(FP32 FFT + FP32 MFMA GEMMs).  The FP64 path and the BF16x3 path are timed the same
+(FP32 FFT + FP32 MFMA GEMMs).  The FP64, BF16x3 and BF16x6 paths are timed the same
 way; ChFSI (degree 10, 10 outer iterations, 20-step FP64 Lanczos bounds, FP64
-CholeskyQR2 and Rayleigh-Ritz on the GPU) is timed once per filter precision.
+CholeskyQR2 and Rayleigh-Ritz on the GPU) is timed once per filter precision.  The CPU
+legs time the oracle's FP64 H|psi> on one thread and on every usable thread, and a
+batched scipy.fft + BLAS schedule, on bounded band samples, and report the host CPU.
 
 Prints ONE JSON line.  --gpus above 1 exits with a message: this build runs on one GPU.
 """
@@ -25,6 +27,7 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0
 
 
 def log(msg):
@@ -98,6 +101,36 @@ def gemm_flops(s, nb):
     return 8.0 * nproj * s.ngw * nb      # one complex GEMM: 8 flops per complex MAC
 
 
+# MFMA flops issued per complex multiply-add of the projector GEMMs, and the peak they
+# issue against: FP64 four real products, FP32 three (3M), BF16X3 / BF16X6 four real
+# products of three / six bf16 split products each
+_MFMA_ISSUE = {"fp64": (8, FP64_PEAK_TFLOPS), "fp32": (6, FP32_PEAK_TFLOPS),
+               "bf16x3": (24, BF16_PEAK_TFLOPS), "bf16x6": (48, BF16_PEAK_TFLOPS)}
+
+
+def gemm_util(s, nb, prec, ms):
+    """Complex-GEMM flop rate of one projector GEMM launch and its MFMA issue rate against
+    the peak of the instruction it runs on."""
+    cplx = gemm_flops(s, nb) / (ms * 1e-3) / 1e12
+    per, peak = _MFMA_ISSUE[prec]
+    issue = cplx * per / 8.0
+    return {"avg_ms": round(ms, 5), "TFLOP/s": round(cplx, 2), "mfma_issue_TFLOP/s": round(issue, 2),
+            "mfma_peak": peak, "mfma_frac": round(issue / peak, 4)}
+
+
+def pass_rates(s, info, nb, prec, prof):
+    """Per-pass HBM rates (algorithmic bytes / event-timed duration) and the GEMMs' MFMA
+    utilisation for one precision."""
+    kb = kernel_bytes(s, info, nb, 8 if prec == "fp64" else 4)
+    out = {}
+    for k in ("zinv", "plane", "zfwd"):
+        gbs = kb[k] / (prof[k] * 1e-3) / 1e9
+        out[k] = {"avg_ms": round(prof[k], 5), "GB/s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    for k in ("proj", "expand"):
+        out[k] = gemm_util(s, nb, prec, prof[k])
+    return out
+
+
 def time_steps(fn, steps, warmup):
     import pwh
     for _ in range(warmup):
@@ -147,6 +180,120 @@ def cpu_baseline(s, psi, seconds_target=15.0):
             "sample": f"{nband} bands of the Si64 block through the oracle's FP64 ApplyHam_block "
                       f"(C restatement, own mixed-radix FFT, per-band loop as ApplyHam_block.cpp:28-167), "
                       f"1 thread, {dt:.1f} s"}
+
+
+def cpu_info():
+    """The host CPU as lscpu reports it (model, sockets, cores), and the threads this
+    process may use (its affinity mask, capped by OMP_NUM_THREADS when that is set)."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)",
+                             "NUMA node(s)", "CPU max MHz"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError) as e:
+        info["lscpu_error"] = str(e)
+    info["usable_threads"] = cpu_threads()
+    return info
+
+
+def cpu_threads():
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return n
+
+
+class ScipyHpsi:
+    """A batched FP64 CPU schedule of ApplyHam_block (ApplyHam_block.cpp:28-167), as an
+    FFTW-class best effort: the bands' sphere coefficients are scattered into 3D grids,
+    transformed with scipy.fft (pocketfft, `workers` threads), multiplied by V_loc and
+    transformed back, and the nonlocal term is two complex GEMMs against the projectors
+    built once (numpy BLAS).  Conventions are the reference's: the G->r transform is
+    unnormalised with exponent +1, the r->G transform has exponent -1 and scale 1/nnr, and
+    the grid index is i1 + nr1 (i2 + nr2 i3)."""
+
+    def __init__(self, s, workers):
+        g = s.globals()
+        self.nr = s.nr
+        self.nnr = s.nr[0] * s.nr[1] * s.nr[2]
+        self.ngw = s.ngw
+        self.idx = np.asarray(s.gv["n123"][:s.ngw], np.int64) - 1
+        self.kin = 0.5 * np.asarray(s.gv["xkg"][:s.ngw]) * s.tpiba2
+        self.vloc = np.asarray(s.vloc, np.float64).reshape(s.nr[2], s.nr[1], s.nr[0])
+        nax, ngwx, nlm = int(g.nax), int(g.ngwx), 4
+        eigr = np.asarray(g.eigr).reshape(nax, ngwx)[:, :s.ngw]
+        pkg = np.asarray(g.pkg_a).reshape(ngwx, nlm)[:s.ngw].T
+        # beta[(a, l)][ig] = EIGR[a][ig] PKG_A[l][ig]; D = WNL(l)
+        self.beta = (eigr[:, None, :] * pkg[None, :, :]).reshape(nax * nlm, s.ngw)
+        self.d = np.tile(np.asarray(g.wnl[:nlm], np.float64), nax)
+        self.workers = workers
+
+    def apply(self, psi, chunk=16):
+        import scipy.fft
+        out = np.empty_like(psi)
+        n3, n2, n1 = self.nr[2], self.nr[1], self.nr[0]
+        for b0 in range(0, psi.shape[0], chunk):
+            p = psi[b0:b0 + chunk]
+            grid = np.zeros((p.shape[0], self.nnr), np.complex128)
+            grid[:, self.idx] = p
+            r = scipy.fft.ifftn(grid.reshape(-1, n3, n2, n1), axes=(1, 2, 3), norm="forward",
+                                workers=self.workers)
+            r *= self.vloc
+            f = scipy.fft.fftn(r, axes=(1, 2, 3), norm="forward", workers=self.workers)
+            out[b0:b0 + chunk] = self.kin * p + f.reshape(p.shape[0], -1)[:, self.idx]
+        proj = np.conj(self.beta) @ psi.T                 # <beta|psi>, (P, nb)
+        out += (self.beta.T @ (self.d[:, None] * proj)).T
+        return out
+
+
+def cpu_baselines(s, psi, label, seconds=8.0):
+    """Three CPU schedules of the same FP64 H|psi> on bounded band samples: the oracle (C
+    restatement, per-band loop as ApplyHam_block.cpp:28-167) on one thread and on every
+    usable thread (bands split over OpenMP threads), and the batched scipy.fft + BLAS
+    schedule on every usable thread."""
+    from oracle import oracle as O
+    g = s.globals()
+    og = O.Globals(nr=s.nr, ngwx=g.ngwx, n123=s.gv["n123"], xkg=s.gv["xkg"], tpiba2=g.tpiba2,
+                   na=g.na, nax=g.nax, l_max=g.l_max, l_loc=g.l_loc, nlmax=g.nlmax, wnl=g.wnl,
+                   eigr=g.eigr, pkg_a=g.pkg_a)
+    nt = cpu_threads()
+    out = {"threads": nt}
+
+    def sample(fn, nmax, per_band):
+        # about `seconds` of work: a band sample, or the whole block repeated
+        nband = int(max(1, min(nmax, seconds / max(per_band, 1e-4))))
+        reps = int(max(1, min(20, seconds / max(per_band * nband, 1e-4))))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn(nband)
+        dt = time.perf_counter() - t0
+        return nband * reps, dt
+
+    t0 = time.perf_counter()
+    O.apply_ham_block(og, psi[:1], s.vloc, s.ngw)
+    t1 = time.perf_counter() - t0
+    nb, dt = sample(lambda n: O.apply_ham_block(og, psi[:n], s.vloc, s.ngw), psi.shape[0], t1)
+    out["oracle_1_thread"] = {"value": round(nb / dt, 2), "unit": "band-applies/s", "cores": 1,
+                              "sample": f"{nb} band-applies over {label}, {dt:.1f} s"}
+    nb, dt = sample(lambda n: O.apply_ham_block(og, psi[:n], s.vloc, s.ngw, nthreads=nt), psi.shape[0],
+                    t1 / nt)
+    out["oracle_all_threads"] = {"value": round(nb / dt, 2), "unit": "band-applies/s", "cores": nt,
+                                 "sample": f"{nb} band-applies over {label}, {dt:.1f} s"}
+    sh = ScipyHpsi(s, nt)
+    sh.apply(psi[:min(16, psi.shape[0])])           # first call: plans and thread pool
+    t0 = time.perf_counter()
+    sh.apply(psi[:min(16, psi.shape[0])])
+    t1 = (time.perf_counter() - t0) / min(16, psi.shape[0])
+    nb, dt = sample(lambda n: sh.apply(psi[:n]), psi.shape[0], t1)
+    out["scipy_fft_all_threads"] = {"value": round(nb / dt, 2), "unit": "band-applies/s", "cores": nt,
+                                    "sample": f"{nb} band-applies over {label} batched 16 at a time (scipy.fft "
+                                              f"workers={nt}, numpy BLAS GEMMs), {dt:.1f} s"}
+    return out
 
 
 def measured_traffic(kernel, path):
@@ -464,7 +611,7 @@ def c3_batch(steps, degree=10):
     return out
 
 
-def c4_rates(steps):
+def c4_rates(steps, cpu=True):
     """Si216 cell (96^3 grid, 43867 plane waves, 432 bands, 864 projectors)."""
     import pwh
     from pwh import synth
@@ -478,10 +625,12 @@ def c4_rates(steps):
     tuned = {p: x.autotune(s.nbands, p)[0] for p in ("fp64", "fp32", "bf16x3", "bf16x6")}
     out = {"workload": f"si216: a={s.a} bohr, {s.natoms} atoms, grid {s.nr[0]}^3, {s.ngw} PW, "
                        f"{s.nbands} bands, {4 * s.natoms} projectors"}
+    info = x.info()
     for p in ("fp64", "fp32", "bf16x3", "bf16x6"):
         dt = time_steps(lambda: x.resident_apply(s.nbands, p), steps, 1)
+        prof = kernel_profile(x, s.nbands, p, reps=3)
         out[p] = {"ms_per_block": round(1e3 * dt / steps, 3), "band_applies_per_s": round(s.nbands * steps / dt, 1),
-                  "gemm_target_blocks": tuned[p]}
+                  "gemm_target_blocks": tuned[p], "passes": pass_rates(s, info, s.nbands, p, prof)}
     # ChFSI at degree 12 (FP64 orthonormalisation and Rayleigh-Ritz; rocSOLVER's Cholesky
     # and zheevd at n = 432): five outer iterations per filter precision from the same
     # orthonormal random start, 20-step FP64 Lanczos bounds.  The eigenvalue difference is
@@ -504,6 +653,8 @@ def c4_rates(steps):
     out["chfsi_fp32_vs_fp64_after_5_iterations"] = {"max_abs_eig_diff_Ha": float(d.max()),
                                                     "lower_half_max_abs_eig_diff_Ha": float(d[:nx // 2].max())}
     x.close()
+    if cpu:
+        out["cpu_baselines"] = cpu_baselines(s, psi, "the si216 block", seconds=6.0)
     return out
 
 
@@ -539,6 +690,7 @@ def main():
               f"(multi-GPU sharding is described in DESIGN.md, not implemented).", file=sys.stderr)
         sys.exit(2)
 
+    hostcpu = cpu_info()          # before the GPU is initialised (lscpu runs as a child)
     import pwh
     from pwh import synth
 
@@ -598,6 +750,10 @@ def main():
         roof = {"kernel": dom, "bound": "mfma", "achieved": round(achieved, 2), "peak": peak,
                 "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
                 "algorithmic_flops": fl, "avg_ms": round(prof[dom], 5)}
+    passes = {}
+    for p in paths:
+        passes[p] = pass_rates(s, info, nb, p, prof if p == main_path else
+                               kernel_profile(sysm, nb, p, reps=max(5, min(args.steps, 20))))
     per_kernel = {}
     for k, v in prof.items():
         e = {"avg_ms": round(v, 5)}
@@ -690,13 +846,15 @@ def main():
     if not args.no_extra and not args.only:
         extra["c3_batched_uneven"] = c3_batch(steps=5)
         log("bench: c3 " + json.dumps(extra["c3_batched_uneven"]))
-        extra["c4_si216"] = c4_rates(steps=5)
+        extra["c4_si216"] = c4_rates(steps=5, cpu=not args.no_cpu)
         log("bench: c4 " + json.dumps(extra["c4_si216"]))
 
-    cpu = None
+    cpu, cpu_more = None, None
     if not args.no_cpu:
         cpu = cpu_baseline(s, psi)
         log(f"bench: cpu baseline {cpu['value']:.2f} band-applies/s ({cpu['sample']})")
+        cpu_more = cpu_baselines(s, psi, f"the {args.config} block")
+        log("bench: cpu baselines " + json.dumps(cpu_more))
 
     value = rates[main_path]
     line = {
@@ -722,6 +880,7 @@ def main():
         "mixed_vs_fp64": round(rates.get("fp32", 0) / rates["fp64"], 3) if "fp64" in rates and "fp32" in rates else None,
         "roofline": roof,
         "kernels": per_kernel,
+        "passes_by_path": passes,
         "chfsi": chfsi or None,
         "eval_rho_psi": rho,
         "block_solvers": solvers,
@@ -731,6 +890,8 @@ def main():
         "c3_batched_uneven": extra.get("c3_batched_uneven"),
         "c4_si216": extra.get("c4_si216"),
         "cpu_baseline": cpu,
+        "cpu_baselines": cpu_more,
+        "host_cpu": hostcpu,
     }
     print(json.dumps(line), flush=True)
     sysm.close()
