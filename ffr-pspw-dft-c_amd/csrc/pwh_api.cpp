@@ -2918,9 +2918,14 @@ struct pwh_batch {
   hipEvent_t fork = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int g_degree = -1, g_prec = -1;
+  int g_degree = -1, g_prec = -1, g_mode = -1;
   unsigned long long g_gen = 0;   // layout generation the graph was captured at
   std::vector<double> g_lb, g_ub;
+  // grouped filter (mode 2): the launch descriptors in device memory and the partial sums
+  // of every projection problem of one H|psi>
+  Buf<unsigned char> gdesc;
+  Buf<cx<double>> gpart64;
+  Buf<cx<float>> gpart32;
 };
 
 namespace {
@@ -2930,6 +2935,324 @@ void batch_drop_graph(pwh_batch* b) {
   b->exec = nullptr;
   b->graph = nullptr;
   b->g_degree = -1;
+}
+
+// ---------------------------------------------------------------------------
+// Grouped Chebyshev filter of a batch: the filter of chebyfilt_dev / filter_on_stream for
+// every system at once, each pass of each H|psi> one launch over all systems.  Every
+// system keeps its own scalars (lb, ub), buffers and split-K shapes, and the passes run the
+// runtime-plan FFT bodies and the one-system GEMM bodies, so a system's result equals its
+// one-at-a-time filter on the runtime FFT path bit for bit.
+// ---------------------------------------------------------------------------
+struct GroupLaunch {
+  int kind;          // 0 convert 64->32, 1 convert 32->64, 2 copy 64, 3 proj, 4 expand, 5 fft
+  int fp64;          // precision of a proj / expand / fft launch
+  int pass;          // fft pass (0 zinv, 1 plane, 2 zfwd)
+  size_t off;        // byte offset of the descriptor array in the device buffer
+  int n, blocks, rblocks;
+  size_t smem;
+  int key = 0;       // fft: the kernel the systems of the launch share (group_fft_key)
+};
+
+struct GroupBuilder {
+  std::vector<unsigned char> host;
+  std::vector<GroupLaunch> launches;
+  template <typename D>
+  size_t put(const std::vector<D>& v) {
+    size_t off = (host.size() + 255) / 256 * 256;
+    host.resize(off + sizeof(D) * v.size());
+    memcpy(host.data() + off, v.data(), sizeof(D) * v.size());
+    return off;
+  }
+};
+
+// One step out = alpha H X + beta X + gamma Z for every system (per-system pointers and
+// scalars in the vectors), in precision T.
+template <typename T>
+int group_hpsi(pwh_batch* b, GroupBuilder& gb, const std::vector<const cx<T>*>& X, const std::vector<cx<T>*>& out,
+               const std::vector<const cx<T>*>& Z, const std::vector<double>& alpha, const std::vector<double>& beta,
+               const std::vector<double>& gamma, cx<T>* part, size_t part_cap) {
+  const int n = (int)b->sys.size();
+  const bool fp64 = std::is_same<T, double>::value;
+  const int prec = fp64 ? PREC_FP64 : PREC_FP32;
+  std::vector<GroupProj<T>> pj;
+  std::vector<std::vector<GroupExpand<T>>> ex;
+  std::vector<GroupHpsi<T>> fh(n);
+  std::vector<bool> have_nl(n, false);
+  int pblk = 0, rblk = 0;
+  size_t poff = 0;
+  for (int i = 0; i < n; ++i) {
+    pwh_system* s = b->sys[i];
+    if (!s->vset) return fail(PWH_ERR_ARG, "local potential not set (pwh_set_vloc)");
+    const int nb = b->nb[i], ld = s->ngw_pad;
+    for (int sp = 0; sp < s->nsp; ++sp) {
+      if (s->na[sp] == 0 || s->lm[sp] == 0) continue;
+      GroupProj<T> p{};
+      p.M = s->na[sp] * s->lm[sp];
+      p.N = nb;
+      p.K = s->ngw;
+      int blocks = 0;
+      if (!proj_split_shape<T>(p.M, p.N, p.K, s->tune[prec], &p.S, &p.kchunk, &blocks))
+        return fail(PWH_ERR_UNSUPPORTED, "grouped filter: the GEMM stage depth is not the default");
+      p.A = Sel<T>::Bt(s) + (size_t)s->poff[sp] * ld;
+      p.lda = ld;
+      p.B = X[i];
+      p.ldb = ld;
+      p.part = part + poff;
+      poff += (size_t)p.S * p.M * p.N;
+      if (poff > part_cap) return fail(PWH_ERR_ARG, "grouped filter: partial-sum buffer too small");
+      p.C = Sel<T>::Q(s).p + (size_t)s->poff[sp] * nb;
+      p.ldc = nb;
+      p.wl = Sel<T>::wl(s) + 4 * sp;
+      p.lm = s->lm[sp];
+      p.blk0 = pblk;
+      p.rblk0 = rblk;
+      pblk += blocks;
+      rblk += (p.M * p.N + 255) / 256;
+      pj.push_back(p);
+      if ((int)ex.size() <= sp) ex.resize(sp + 1);
+      GroupExpand<T> e{};
+      e.A = p.A;
+      e.lda = ld;
+      e.Q = p.C;
+      e.ldq = nb;
+      e.out = Sel<T>::Hnl(s).p;
+      e.ldo = ld;
+      e.J = p.M;
+      e.N = nb;
+      e.K = s->ngw;
+      e.accumulate = have_nl[i] ? 1 : 0;
+      e.blk0 = 0;   // set per launch below
+      ex[sp].push_back(e);
+      have_nl[i] = true;
+    }
+    GroupHpsi<T>& f = fh[i];
+    f.m = s->maps;
+    f.tw1 = static_cast<const cx<T>*>(Sel<T>::tw(s, 0));
+    f.tw2 = static_cast<const cx<T>*>(Sel<T>::tw(s, 1));
+    f.tw3 = static_cast<const cx<T>*>(Sel<T>::tw(s, 2));
+    f.V = Sel<T>::V(s);
+    f.kin = Sel<T>::kin(s);
+    f.X = X[i];
+    f.W = Sel<T>::W(s).p;
+    f.Hnl = have_nl[i] ? Sel<T>::Hnl(s).p : nullptr;
+    f.Z = Z[i];
+    f.out = out[i];
+    f.ld = ld;
+    f.nb = nb;
+    f.inv_nnr = (T)(1.0 / s->nnr);
+    f.alpha = (T)alpha[i];
+    f.beta = (T)beta[i];
+    f.gamma = (T)gamma[i];
+  }
+  if (!pj.empty()) {
+    GroupLaunch l{3, fp64, 0, gb.put(pj), (int)pj.size(), pblk, rblk, 0};
+    gb.launches.push_back(l);
+    for (auto& v : ex) {
+      if (v.empty()) continue;
+      int eb = 0;
+      for (auto& e : v) {
+        e.blk0 = eb;
+        eb += expand_grid_blocks(e.K, e.N);
+      }
+      GroupLaunch le{4, fp64, 0, gb.put(v), (int)v.size(), eb, 0, 0};
+      gb.launches.push_back(le);
+    }
+  }
+  // each FFT pass: one launch per kernel the systems run (systems of one grid share one)
+  for (int pass = 0; pass < 3; ++pass) {
+    std::map<int, std::vector<int>> bykey;
+    std::vector<int> blocks(n), cts(n);
+    std::vector<size_t> sm(n);
+    for (int i = 0; i < n; ++i) {
+      const int key = group_fft_key<T>(pass, b->sys[i]->maps, b->nb[i], &blocks[i], &sm[i], &cts[i]);
+      if (key < 0)
+        return fail(PWH_ERR_UNSUPPORTED, "grouped filter: a system's plane pass needs the split path");
+      bykey[key].push_back(i);
+    }
+    for (const auto& kv : bykey) {
+      std::vector<GroupHpsi<T>> v;
+      int blk = 0;
+      size_t smx = 0;
+      for (int i : kv.second) {
+        GroupHpsi<T> f = fh[i];
+        f.ct = cts[i];
+        f.blk0 = blk;
+        blk += blocks[i];
+        smx = std::max(smx, sm[i]);
+        v.push_back(f);
+      }
+      GroupLaunch l{5, fp64, pass, gb.put(v), (int)v.size(), blk, 0, smx, kv.first};
+      gb.launches.push_back(l);
+    }
+  }
+  return PWH_OK;
+}
+
+// kind 0/1/2 conversion of one array per system
+int group_convert(pwh_batch* b, GroupBuilder& gb, int kind, const std::vector<const void*>& in,
+                  const std::vector<void*>& out) {
+  std::vector<GroupConv> v(b->sys.size());
+  int blk = 0;
+  for (size_t i = 0; i < v.size(); ++i) {
+    const size_t n = (size_t)b->nb[i] * b->sys[i]->ngw_pad;
+    v[i] = GroupConv{in[i], out[i], n, blk};
+    blk += (int)((n + 255) / 256);
+  }
+  GroupLaunch l{kind, 0, 0, gb.put(v), (int)v.size(), blk, 0, 0};
+  gb.launches.push_back(l);
+  return PWH_OK;
+}
+
+// Partial-sum elements of the projections of one H|psi> over the batch in precision T.
+template <typename T>
+size_t group_part_elems(pwh_batch* b) {
+  const int prec = std::is_same<T, double>::value ? PREC_FP64 : PREC_FP32;
+  size_t tot = 0;
+  for (size_t i = 0; i < b->sys.size(); ++i) {
+    pwh_system* s = b->sys[i];
+    for (int sp = 0; sp < s->nsp; ++sp) {
+      if (s->na[sp] == 0 || s->lm[sp] == 0) continue;
+      const int M = s->na[sp] * s->lm[sp];
+      tot += proj_part_elems(M, b->nb[i], s->ngw, s->tune[prec]);
+    }
+  }
+  return std::max<size_t>(tot, 1);
+}
+
+// Builds the launches of the grouped filter (chebyfilt_dev's recurrence per system; FP32
+// filters end with the FP64 step of cheby_last_step_fp64 unless that is switched off).
+int group_filter_build(pwh_batch* b, GroupBuilder& gb, int degree, const double* lb, const double* ub,
+                       int precision) {
+  const int n = (int)b->sys.size();
+  for (int i = 0; i < n; ++i)
+    if (b->nb[i] < 2) return fail(PWH_ERR_UNSUPPORTED, "grouped filter: a system with one band");
+  std::vector<double> e(n), c(n), sig(n), sig1(n);
+  for (int i = 0; i < n; ++i) {
+    e[i] = (ub[i] - lb[i]) / 2.0;
+    c[i] = (ub[i] + lb[i]) / 2.0;
+    sig[i] = sig1[i] = e[i] / (lb[i] - ub[i]);
+  }
+  std::vector<double> al(n), be(n), ga(n);
+  if (precision == PWH_FP64) {
+    CHK(b->gpart64.ensure(group_part_elems<double>(b)));
+    std::vector<cx<double>*> B0(n), B1(n), B2(n);
+    for (int i = 0; i < n; ++i) {
+      B0[i] = b->sys[i]->X.p;
+      B1[i] = b->sys[i]->A64.p;
+      B2[i] = b->sys[i]->B64.p;
+    }
+    std::vector<const cx<double>*> in(B0.begin(), B0.end()), Z(n, nullptr);
+    for (int i = 0; i < n; ++i) { al[i] = sig1[i] / e[i]; be[i] = -c[i] * sig1[i] / e[i]; ga[i] = 0.0; }
+    CHK(group_hpsi<double>(b, gb, in, B1, Z, al, be, ga, b->gpart64.p, b->gpart64.cap));
+    for (int id = 2; id <= degree; ++id) {
+      std::vector<const cx<double>*> yin(B1.begin(), B1.end()), zz(B0.begin(), B0.end());
+      for (int i = 0; i < n; ++i) {
+        const double s2 = 1.0 / (2.0 / sig1[i] - sig[i]);
+        al[i] = 2.0 * s2 / e[i];
+        be[i] = -c[i] * 2.0 * s2 / e[i];
+        ga[i] = -sig[i] * s2;
+        sig[i] = s2;
+      }
+      CHK(group_hpsi<double>(b, gb, yin, B2, zz, al, be, ga, b->gpart64.p, b->gpart64.cap));
+      std::swap(B0, B1);   // X <- Y, Y <- Y1, Y1 <- X
+      std::swap(B1, B2);
+    }
+    if (B1[0] != b->sys[0]->X.p) {   // the result is in B1 for every system alike
+      std::vector<const void*> src(B1.begin(), B1.end());
+      std::vector<void*> dst(n);
+      for (int i = 0; i < n; ++i) dst[i] = b->sys[i]->X.p;
+      CHK(group_convert(b, gb, 2, src, dst));
+    }
+    return PWH_OK;
+  }
+  if (precision != PWH_FP32)
+    return fail(PWH_ERR_UNSUPPORTED, "grouped filter: FP64 and FP32 filters only");
+  const bool last64 = degree >= 2 && filter_last_fp64();
+  CHK(b->gpart32.ensure(group_part_elems<float>(b)));
+  if (last64) CHK(b->gpart64.ensure(group_part_elems<double>(b)));
+  std::vector<cx<float>*> B0(n), B1(n), B2(n);
+  std::vector<const void*> src(n);
+  std::vector<void*> dst(n);
+  for (int i = 0; i < n; ++i) {
+    B0[i] = b->sys[i]->X32.p;
+    B1[i] = b->sys[i]->Y32.p;
+    B2[i] = b->sys[i]->Y1_32.p;
+    src[i] = b->sys[i]->X.p;
+    dst[i] = B0[i];
+  }
+  CHK(group_convert(b, gb, 0, src, dst));
+  const int d32 = last64 ? degree - 1 : degree;
+  {
+    std::vector<const cx<float>*> in(B0.begin(), B0.end()), Z(n, nullptr);
+    for (int i = 0; i < n; ++i) { al[i] = sig1[i] / e[i]; be[i] = -c[i] * sig1[i] / e[i]; ga[i] = 0.0; }
+    CHK(group_hpsi<float>(b, gb, in, B1, Z, al, be, ga, b->gpart32.p, b->gpart32.cap));
+  }
+  for (int id = 2; id <= d32; ++id) {
+    std::vector<const cx<float>*> yin(B1.begin(), B1.end()), zz(B0.begin(), B0.end());
+    for (int i = 0; i < n; ++i) {
+      const double s2 = 1.0 / (2.0 / sig1[i] - sig[i]);
+      al[i] = 2.0 * s2 / e[i];
+      be[i] = -c[i] * 2.0 * s2 / e[i];
+      ga[i] = -sig[i] * s2;
+      sig[i] = s2;
+    }
+    CHK(group_hpsi<float>(b, gb, yin, B2, zz, al, be, ga, b->gpart32.p, b->gpart32.cap));
+    std::swap(B0, B1);
+    std::swap(B1, B2);
+  }
+  if (!last64) {
+    for (int i = 0; i < n; ++i) { src[i] = B1[i]; dst[i] = b->sys[i]->X.p; }
+    return group_convert(b, gb, 1, src, dst);
+  }
+  // prev = B0, cur = B1 -> A64, B64; FP64 step into X
+  for (int i = 0; i < n; ++i) { src[i] = B0[i]; dst[i] = b->sys[i]->A64.p; }
+  CHK(group_convert(b, gb, 1, src, dst));
+  for (int i = 0; i < n; ++i) { src[i] = B1[i]; dst[i] = b->sys[i]->B64.p; }
+  CHK(group_convert(b, gb, 1, src, dst));
+  std::vector<const cx<double>*> in(n), Z(n);
+  std::vector<cx<double>*> o(n);
+  for (int i = 0; i < n; ++i) {
+    const double s2 = 1.0 / (2.0 / sig1[i] - sig[i]);
+    al[i] = 2.0 * s2 / e[i];
+    be[i] = -c[i] * 2.0 * s2 / e[i];
+    ga[i] = -sig[i] * s2;
+    in[i] = b->sys[i]->B64.p;
+    Z[i] = b->sys[i]->A64.p;
+    o[i] = b->sys[i]->X.p;
+  }
+  return group_hpsi<double>(b, gb, in, o, Z, al, be, ga, b->gpart64.p, b->gpart64.cap);
+}
+
+int group_filter_issue(pwh_batch* b, const std::vector<GroupLaunch>& L, hipStream_t st) {
+  unsigned char* base = b->gdesc.p;
+  for (const GroupLaunch& l : L) {
+    const void* d = base + l.off;
+    switch (l.kind) {
+      case 0: HIPCHK((launch_group_convert<double, float>(static_cast<const GroupConv*>(d), l.n, l.blocks, st))); break;
+      case 1: HIPCHK((launch_group_convert<float, double>(static_cast<const GroupConv*>(d), l.n, l.blocks, st))); break;
+      case 2: HIPCHK((launch_group_convert<double, double>(static_cast<const GroupConv*>(d), l.n, l.blocks, st))); break;
+      case 3:
+        if (l.fp64) HIPCHK(launch_group_proj<double>(static_cast<const GroupProj<double>*>(d), l.n, l.blocks, l.rblocks, st));
+        else HIPCHK(launch_group_proj<float>(static_cast<const GroupProj<float>*>(d), l.n, l.blocks, l.rblocks, st));
+        break;
+      case 4:
+        if (l.fp64) HIPCHK(launch_group_expand<double>(static_cast<const GroupExpand<double>*>(d), l.n, l.blocks, st));
+        else HIPCHK(launch_group_expand<float>(static_cast<const GroupExpand<float>*>(d), l.n, l.blocks, st));
+        break;
+      case 5:
+        if (l.fp64)
+          HIPCHK(launch_group_fft<double>(l.pass, l.key, static_cast<const GroupHpsi<double>*>(d), l.n, l.blocks,
+                                          l.smem, st));
+        else
+          HIPCHK(launch_group_fft<float>(l.pass, l.key, static_cast<const GroupHpsi<float>*>(d), l.n, l.blocks,
+                                         l.smem, st));
+        break;
+      default:
+        return fail(PWH_ERR_ARG, "grouped filter: bad launch kind");
+    }
+  }
+  return PWH_OK;
 }
 }  // namespace
 
@@ -2997,20 +3320,33 @@ int pwh_batch_chebyfilt(pwh_batch* b, int degree, const double* lb, const double
       CHK(filter_on_stream(b->sys[i], b->nb[i], degree, lb[i], ub[i], precision, g.stream));
     return PWH_OK;
   }
-  const bool same = b->exec && b->g_degree == degree && b->g_prec == precision && b->g_gen == g.layout_gen &&
-                    std::equal(lb, lb + n, b->g_lb.begin()) && std::equal(ub, ub + n, b->g_ub.begin());
+  if (mode != 1 && mode != 2) return fail(PWH_ERR_ARG, "bad batch filter mode");
+  const bool same = b->exec && b->g_mode == mode && b->g_degree == degree && b->g_prec == precision &&
+                    b->g_gen == g.layout_gen && std::equal(lb, lb + n, b->g_lb.begin()) &&
+                    std::equal(ub, ub + n, b->g_ub.begin());
   if (!same) {
     batch_drop_graph(b);
+    GroupBuilder gb;
+    if (mode == 2) {
+      CHK(group_filter_build(b, gb, degree, lb, ub, precision));
+      CHK(b->gdesc.ensure(gb.host.size()));
+      HIPCHK(hipMemcpyAsync(b->gdesc.p, gb.host.data(), gb.host.size(), hipMemcpyHostToDevice, g.stream));
+      HIPCHK(hipStreamSynchronize(g.stream));
+    }
     const bool timing = g.timing;
     g.timing = false;
     int rc = PWH_OK;
     HIPCHK(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
-    hipEventRecord(b->fork, g.stream);
-    for (int i = 0; i < n && rc == PWH_OK; ++i) {
-      hipStreamWaitEvent(b->streams[i], b->fork, 0);
-      rc = filter_on_stream(b->sys[i], b->nb[i], degree, lb[i], ub[i], precision, b->streams[i]);
-      hipEventRecord(b->done[i], b->streams[i]);
-      hipStreamWaitEvent(g.stream, b->done[i], 0);
+    if (mode == 2) {
+      rc = group_filter_issue(b, gb.launches, g.stream);
+    } else {
+      hipEventRecord(b->fork, g.stream);
+      for (int i = 0; i < n && rc == PWH_OK; ++i) {
+        hipStreamWaitEvent(b->streams[i], b->fork, 0);
+        rc = filter_on_stream(b->sys[i], b->nb[i], degree, lb[i], ub[i], precision, b->streams[i]);
+        hipEventRecord(b->done[i], b->streams[i]);
+        hipStreamWaitEvent(g.stream, b->done[i], 0);
+      }
     }
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(g.stream, &graph);
@@ -3021,6 +3357,7 @@ int pwh_batch_chebyfilt(pwh_batch* b, int degree, const double* lb, const double
     HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     b->g_degree = degree;
     b->g_prec = precision;
+    b->g_mode = mode;
     b->g_gen = g.layout_gen;
     b->g_lb.assign(lb, lb + n);
     b->g_ub.assign(ub, ub + n);

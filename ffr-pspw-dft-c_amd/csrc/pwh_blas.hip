@@ -22,6 +22,35 @@ hipError_t convert_cx(const cx<Tin>* in, cx<Tout>* out, size_t n, hipStream_t s)
 template hipError_t convert_cx<double, float>(const cx<double>*, cx<float>*, size_t, hipStream_t);
 template hipError_t convert_cx<float, double>(const cx<float>*, cx<double>*, size_t, hipStream_t);
 
+// Grouped conversion (or copy, Tin = Tout) of one array per system: block L belongs to the
+// array whose first block is the largest one <= L, 256 elements per block.
+template <typename Tin, typename Tout>
+__global__ void k_convert_group(const GroupConv* __restrict__ g, int ng) {
+  const int L = blockIdx.x;
+  int lo = 0, hi = ng - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (g[mid].blk0 <= L) lo = mid;
+    else hi = mid - 1;
+  }
+  const GroupConv& p = g[lo];
+  const size_t i = (size_t)(L - p.blk0) * 256 + threadIdx.x;
+  if (i < p.n) {
+    const cx<Tin> v = static_cast<const cx<Tin>*>(p.in)[i];
+    static_cast<cx<Tout>*>(p.out)[i] = mk<Tout>((Tout)v.x, (Tout)v.y);
+  }
+}
+
+template <typename Tin, typename Tout>
+hipError_t launch_group_convert(const GroupConv* g, int n, int blocks, hipStream_t s) {
+  if (n < 1 || blocks < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_convert_group<Tin, Tout>), dim3(blocks), dim3(256), 0, s, g, n);
+  return hipGetLastError();
+}
+template hipError_t launch_group_convert<double, float>(const GroupConv*, int, int, hipStream_t);
+template hipError_t launch_group_convert<float, double>(const GroupConv*, int, int, hipStream_t);
+template hipError_t launch_group_convert<double, double>(const GroupConv*, int, int, hipStream_t);
+
 __global__ void k_perm_in(const cx<double>* __restrict__ ref, int ldr, const int* __restrict__ perm,
                           int ngw, int ngw_pad, cx<double>* __restrict__ out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;

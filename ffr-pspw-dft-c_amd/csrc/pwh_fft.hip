@@ -134,15 +134,15 @@ constexpr int kPlaneThreads = 512;
 // than a chain of them.  Entries beyond PE * NT in a tile (PE is picked from the largest
 // tile) go through a plain loop.  The last tile may be short: cn = min(CT, ncol_pad - c0).
 template <typename T, int NZ, int CT, int NT, int PE>
-__global__ __launch_bounds__(NT) void k_zinv(SphereMaps m, const cx<T>* __restrict__ tw3g,
+__device__ __forceinline__ void zinv_body(SphereMaps m, const cx<T>* __restrict__ tw3g,
                                              const cx<T>* __restrict__ X, int ldx,
-                                             cx<T>* __restrict__ W) {
+                                             cx<T>* __restrict__ W, const int bx, const int by) {
   __shared__ cx<T> tile[NZ * CT];
   __shared__ cx<T> tw[NZ];
   constexpr int TWP = (NZ + NT - 1) / NT;
   const int tid = threadIdx.x;
-  const int c0 = blockIdx.x * CT;
-  const int b = blockIdx.y;
+  const int c0 = bx * CT;
+  const int b = by;
   const int cn = min(CT, m.ncol_pad - c0);
   const int kb = m.colstart[c0], ke = m.colstart[c0 + cn];
   const cx<T>* xb = X + (size_t)b * ldx;
@@ -184,6 +184,13 @@ __global__ __launch_bounds__(NT) void k_zinv(SphereMaps m, const cx<T>* __restri
   }
 }
 
+template <typename T, int NZ, int CT, int NT, int PE>
+__global__ __launch_bounds__(NT) void k_zinv(SphereMaps m, const cx<T>* __restrict__ tw3g,
+                                             const cx<T>* __restrict__ X, int ldx,
+                                             cx<T>* __restrict__ W) {
+  zinv_body<T, NZ, CT, NT, PE>(m, tw3g, X, ldx, W, blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // Pass 2: (i1,i2) planes: inverse y, inverse x, *V, forward x, forward y
 // ---------------------------------------------------------------------------
@@ -199,9 +206,9 @@ constexpr size_t plane_smem() {
 }
 
 template <typename T, int N1, int N2, int NT>
-__global__ __launch_bounds__(NT) void k_plane(SphereMaps m, const cx<T>* __restrict__ tw1g,
+__device__ __forceinline__ void plane_body(SphereMaps m, const cx<T>* __restrict__ tw1g,
                                               const cx<T>* __restrict__ tw2g,
-                                              const T* __restrict__ V, cx<T>* __restrict__ W) {
+                                              const T* __restrict__ V, cx<T>* __restrict__ W, const int bx, const int by) {
   constexpr int S1 = plane_stride<N1>();
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
@@ -210,7 +217,7 @@ __global__ __launch_bounds__(NT) void k_plane(SphereMaps m, const cx<T>* __restr
   int* xa = reinterpret_cast<int*>(tw2 + N2);
   unsigned char* xact = reinterpret_cast<unsigned char*>(xa + N1);
   const int tid = threadIdx.x;
-  const int z = blockIdx.x, b = blockIdx.y;
+  const int z = bx, b = by;
   const int nxa = m.nxa;
   for (int i = tid; i < N1; i += NT) { tw1[i] = tw1g[i]; xact[i] = 0; }
   for (int i = tid; i < N2; i += NT) tw2[i] = tw2g[i];
@@ -238,6 +245,13 @@ __global__ __launch_bounds__(NT) void k_plane(SphereMaps m, const cx<T>* __restr
     const int xy = m.colxy[c];
     wrow[c] = tile[(xy / N1) * S1 + xy % N1];
   }
+}
+
+template <typename T, int N1, int N2, int NT>
+__global__ __launch_bounds__(NT) void k_plane(SphereMaps m, const cx<T>* __restrict__ tw1g,
+                                              const cx<T>* __restrict__ tw2g,
+                                              const T* __restrict__ V, cx<T>* __restrict__ W) {
+  plane_body<T, N1, N2, NT>(m, tw1g, tw2g, V, W, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -416,15 +430,15 @@ constexpr size_t band_plane_smem() {
 }
 
 template <typename T, int N, int NT>
-__global__ __launch_bounds__(NT) void k_plane_band(SphereMaps m, const cx<T>* __restrict__ twg,
-                                                   const T* __restrict__ V, cx<T>* __restrict__ W) {
+__device__ __forceinline__ void plane_band_body(SphereMaps m, const cx<T>* __restrict__ twg,
+                                                   const T* __restrict__ V, cx<T>* __restrict__ W, const int bx, const int by) {
   constexpr int S1 = N + 1, Q = N / 4, NBC = N / 2;
   static_assert(N % 4 == 0, "band pruning needs n % 4 == 0");
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
   cx<T>* tw = tile + N * S1;
   const int tid = threadIdx.x;
-  const int z = blockIdx.x, b = blockIdx.y;
+  const int z = bx, b = by;
 
   // V for the outputs this lane produces in the last inverse row stage, loaded first so
   // the latency hides behind the column transforms
@@ -469,6 +483,12 @@ __global__ __launch_bounds__(NT) void k_plane_band(SphereMaps m, const cx<T>* __
     const int xy = m.colxy[c];
     wrow[c] = tile[(xy / N) * S1 + xy % N];
   }
+}
+
+template <typename T, int N, int NT>
+__global__ __launch_bounds__(NT) void k_plane_band(SphereMaps m, const cx<T>* __restrict__ twg,
+                                                   const T* __restrict__ V, cx<T>* __restrict__ W) {
+  plane_band_body<T, N, NT>(m, twg, V, W, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -524,13 +544,13 @@ __device__ __forceinline__ void w64_line(cx<T> (&v)[8], cx<T>* line, int es, int
 // FP32: 77 VGPRs (6 waves per SIMD); capping it at 64 for the 8 waves the LDS would allow
 // spilled and measured slower (83.9 against 77.1 us at 64^3 x 128 bands)
 template <typename T>
-__global__ __launch_bounds__(512) void k_plane_w64(SphereMaps m, const cx<T>* __restrict__ twg,
-                                                   const T* __restrict__ V, cx<T>* __restrict__ W) {
+__device__ __forceinline__ void plane_w64_body(SphereMaps m, const cx<T>* __restrict__ twg,
+                                                   const T* __restrict__ V, cx<T>* __restrict__ W, const int bx, const int by) {
   constexpr int N = 64, S = kW64Stride, Q = N / 4, NBC = N / 2;
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
-  const int z = blockIdx.x, b = blockIdx.y;
+  const int z = bx, b = by;
   const int jc = l >> 3, cl = l & 7;   // column passes: column cl of the wave, j = jc
   const int jr = l & 7, yl = l >> 3;   // row pass: row yl of the wave, j = jr
   // V at the row pass's points (row 8 wave + yl, x = jr + 8t), loaded first
@@ -602,6 +622,12 @@ __global__ __launch_bounds__(512) void k_plane_w64(SphereMaps m, const cx<T>* __
     const int xy = m.colxy[c];
     wrow[c] = tile[(xy >> 6) * S + (xy & 63)];
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(512) void k_plane_w64(SphereMaps m, const cx<T>* __restrict__ twg,
+                                                   const T* __restrict__ V, cx<T>* __restrict__ W) {
+  plane_w64_body<T>(m, twg, V, W, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -742,21 +768,21 @@ __global__ void k_sum_groups(const double* __restrict__ part, int groups, size_t
 // Pass 3: forward transform along i3, gather the sphere, fused epilogue
 // ---------------------------------------------------------------------------
 template <typename T, int NZ, int CT, int NT, int PE>
-__global__ __launch_bounds__(NT) void k_zfwd(SphereMaps m, const cx<T>* __restrict__ tw3g,
+__device__ __forceinline__ void zfwd_body(SphereMaps m, const cx<T>* __restrict__ tw3g,
                                              const cx<T>* __restrict__ W,
                                              const T* __restrict__ kin,
                                              const cx<T>* __restrict__ X,
                                              const cx<T>* __restrict__ Hnl,
                                              const cx<T>* __restrict__ Z,
                                              cx<T>* __restrict__ out, int ld, T inv_nnr,
-                                             T alpha, T beta, T gamma) {
+                                             T alpha, T beta, T gamma, const int bx, const int by) {
   __shared__ cx<T> tile[NZ * CT];
   __shared__ cx<T> tw[NZ];
   constexpr int TWP = (NZ + NT - 1) / NT;
   constexpr int WP = (NZ * CT + NT - 1) / NT;
   const int tid = threadIdx.x;
-  const int c0 = blockIdx.x * CT;
-  const int b = blockIdx.y;
+  const int c0 = bx * CT;
+  const int b = by;
   const int cn = min(CT, m.ncol_pad - c0);
   const int kb = m.colstart[c0], ke = m.colstart[c0 + cn];
   const size_t ob = (size_t)b * ld;
@@ -831,6 +857,18 @@ __global__ __launch_bounds__(NT) void k_zfwd(SphereMaps m, const cx<T>* __restri
     if (Z) o = o + gamma * Z[ob + k];
     out[ob + k] = o;
   }
+}
+
+template <typename T, int NZ, int CT, int NT, int PE>
+__global__ __launch_bounds__(NT) void k_zfwd(SphereMaps m, const cx<T>* __restrict__ tw3g,
+                                             const cx<T>* __restrict__ W,
+                                             const T* __restrict__ kin,
+                                             const cx<T>* __restrict__ X,
+                                             const cx<T>* __restrict__ Hnl,
+                                             const cx<T>* __restrict__ Z,
+                                             cx<T>* __restrict__ out, int ld, T inv_nnr,
+                                             T alpha, T beta, T gamma) {
+  zfwd_body<T, NZ, CT, NT, PE>(m, tw3g, W, kin, X, Hnl, Z, out, ld, inv_nnr, alpha, beta, gamma, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -925,13 +963,25 @@ static hipError_t launch_plane_lds(const SphereMaps& m, const void* tw1, const v
 // Plane pass dispatch: the band-pruned kernel when the sphere's columns lie in the band,
 // else the LDS Stockham kernel with kPlaneThreads (512) lanes (measured faster than 256
 // at Si64).
-template <typename T, int N>
-static hipError_t launch_plane_n(const SphereMaps& m, const void* tw1, const void* tw2, const T* V,
-                                 cx<T>* W, int nb, hipStream_t s) {
-  static const bool w64_on = [] {
+static bool w64_enabled() {
+  static const bool on = [] {
     const char* e = getenv("PWH_PLANE_W64");
     return !(e && e[0] == '0');
   }();
+  return on;
+}
+static bool band_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("PWH_PLANE_BAND");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+template <typename T, int N>
+static hipError_t launch_plane_n(const SphereMaps& m, const void* tw1, const void* tw2, const T* V,
+                                 cx<T>* W, int nb, hipStream_t s) {
+  const bool w64_on = w64_enabled();
   if constexpr (N == 64) {
     if (m.band_xy && w64_on) {
       constexpr size_t sm = sizeof(cx<T>) * (size_t)N * kW64Stride;
@@ -944,10 +994,7 @@ static hipError_t launch_plane_n(const SphereMaps& m, const void* tw1, const voi
       return hipGetLastError();
     }
   }
-  static const bool band_on = [] {
-    const char* e = getenv("PWH_PLANE_BAND");
-    return !(e && e[0] == '0');
-  }();
+  const bool band_on = band_enabled();
   if constexpr (N % 4 == 0 && band_plane_smem<T, N>() <= 160 * 1024) {
     if (m.band_xy && band_on) {
       constexpr int NT = band_plane_threads<N>();
@@ -1091,6 +1138,189 @@ hipError_t launch_zfwd_epilogue(const SphereMaps& m, const void* tw3, const cx<T
     default: return hipErrorInvalidValue;
   }
 }
+
+// ---------------------------------------------------------------------------
+// Grouped passes over a batch.  The systems whose pass runs the same compiled kernel (the
+// same length and plane variant) share one launch; block L belongs to the system whose
+// first block blk0 is the largest one <= L and runs the one-system body with its local
+// block.  The column passes prefetch PE = 4 entries per lane for every system: PE only
+// decides which entries are loaded before the first barrier, so results do not change.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ const GroupHpsi<T>& group_sys(const GroupHpsi<T>* __restrict__ g, int n, int L) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (g[mid].blk0 <= L) lo = mid;
+    else hi = mid - 1;
+  }
+  return g[lo];
+}
+
+template <typename T, int NZ, int CT, int NT>
+__global__ __launch_bounds__(NT) void k_zinv_gc(const GroupHpsi<T>* __restrict__ g, int n) {
+  const GroupHpsi<T>& p = group_sys(g, n, blockIdx.x);
+  const int local = blockIdx.x - p.blk0, ncb = (p.m.ncol_pad + CT - 1) / CT;
+  zinv_body<T, NZ, CT, NT, 4>(p.m, p.tw3, p.X, p.ld, p.W, local % ncb, local / ncb);
+}
+
+template <typename T, int NZ, int CT, int NT>
+__global__ __launch_bounds__(NT) void k_zfwd_gc(const GroupHpsi<T>* __restrict__ g, int n) {
+  const GroupHpsi<T>& p = group_sys(g, n, blockIdx.x);
+  const int local = blockIdx.x - p.blk0, ncb = (p.m.ncol_pad + CT - 1) / CT;
+  zfwd_body<T, NZ, CT, NT, 4>(p.m, p.tw3, p.W, p.kin, p.X, p.Hnl, p.Z, p.out, p.ld, p.inv_nnr, p.alpha, p.beta,
+                              p.gamma, local % ncb, local / ncb);
+}
+
+template <typename T, int N, int NT>
+__global__ __launch_bounds__(NT) void k_plane_gc(const GroupHpsi<T>* __restrict__ g, int n) {
+  const GroupHpsi<T>& p = group_sys(g, n, blockIdx.x);
+  const int local = blockIdx.x - p.blk0;
+  plane_body<T, N, N, NT>(p.m, p.tw1, p.tw2, p.V, p.W, local % p.m.nr3, local / p.m.nr3);
+}
+
+template <typename T, int N, int NT>
+__global__ __launch_bounds__(NT) void k_plane_band_gc(const GroupHpsi<T>* __restrict__ g, int n) {
+  const GroupHpsi<T>& p = group_sys(g, n, blockIdx.x);
+  const int local = blockIdx.x - p.blk0;
+  plane_band_body<T, N, NT>(p.m, p.tw1, p.V, p.W, local % p.m.nr3, local / p.m.nr3);
+}
+
+template <typename T>
+__global__ __launch_bounds__(512) void k_plane_w64_gc(const GroupHpsi<T>* __restrict__ g, int n) {
+  const GroupHpsi<T>& p = group_sys(g, n, blockIdx.x);
+  const int local = blockIdx.x - p.blk0;
+  plane_w64_body<T>(p.m, p.tw1, p.V, p.W, local % p.m.nr3, local / p.m.nr3);
+}
+
+enum GroupKind { GK_RT = 0, GK_COL = 1, GK_W64 = 2, GK_BAND = 3, GK_LDS = 4 };
+constexpr int kGroupKeyN = 4096;   // key = kind * kGroupKeyN + length
+
+template <typename T, int N>
+static int plane_group_kind(const SphereMaps& m, size_t* smem) {
+  if constexpr (N == 64) {
+    if (m.band_xy && w64_enabled()) {
+      *smem = sizeof(cx<T>) * (size_t)N * kW64Stride;
+      return GK_W64;
+    }
+  }
+  if constexpr (N % 4 == 0 && band_plane_smem<T, N>() <= 160 * 1024) {
+    if (m.band_xy && band_enabled()) {
+      *smem = band_plane_smem<T, N>();
+      return GK_BAND;
+    }
+  }
+  if constexpr (plane_smem<T, N, N>() > 160 * 1024) {
+    return -1;
+  } else {
+    *smem = plane_smem<T, N, N>();
+    return GK_LDS;
+  }
+}
+
+template <typename T>
+int group_fft_key(int pass, const SphereMaps& m, int nb, int* blocks, size_t* smem, int* ct) {
+  const bool fp64 = std::is_same<T, double>::value;
+  *smem = 0;
+  *ct = 0;
+  if (pass == 0 || pass == 2) {
+    if (m.path == FFT_COMPILED && fft_size_supported(m.nr3)) {
+      const int c = pass == 0 ? 8 : ((size_t)m.nr3 * 32 * sizeof(cx<T>) <= 32 * 1024 ? 32 : 16);
+      *blocks = (m.ncol_pad + c - 1) / c * nb;
+      return GK_COL * kGroupKeyN + m.nr3;
+    }
+  } else if (pass == 1) {
+    if (m.path == FFT_COMPILED && plane_compiled_fits(m, fp64)) {
+      int kind = -1;
+      switch (m.nr1) {
+#define PWH_CASE(N) \
+  case N: kind = plane_group_kind<T, N>(m, smem); break;
+        PWH_FFT_SIZES(PWH_CASE)
+#undef PWH_CASE
+        default: break;
+      }
+      if (kind < 0) return -1;
+      *blocks = m.nr3 * nb;
+      return kind * kGroupKeyN + m.nr1;
+    }
+  } else {
+    return -1;
+  }
+  int zb, pb;
+  size_t zs, ps;
+  if (!group_fft_shape<T>(m, nb, ct, &zb, &pb, &zs, &ps)) return -1;
+  *blocks = pass == 1 ? pb : zb;
+  *smem = pass == 1 ? ps : zs;
+  return GK_RT * kGroupKeyN;
+}
+
+template <typename K>
+static hipError_t group_attr(K kern, size_t smem) {
+  if (smem <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)smem);
+}
+
+template <typename T, int N>
+static hipError_t launch_group_n(int pass, int kind, const GroupHpsi<T>* g, int n, int blocks, hipStream_t s) {
+  if (pass == 0) {
+    hipLaunchKernelGGL((k_zinv_gc<T, N, 8, 64>), dim3(blocks), dim3(64), 0, s, g, n);
+    return hipGetLastError();
+  }
+  if (pass == 2) {
+    if constexpr ((size_t)N * 32 * sizeof(cx<T>) <= 32 * 1024)
+      hipLaunchKernelGGL((k_zfwd_gc<T, N, 32, 256>), dim3(blocks), dim3(256), 0, s, g, n);
+    else
+      hipLaunchKernelGGL((k_zfwd_gc<T, N, 16, 128>), dim3(blocks), dim3(128), 0, s, g, n);
+    return hipGetLastError();
+  }
+  hipError_t e = hipSuccess;
+  if constexpr (N == 64) {
+    if (kind == GK_W64) {
+      constexpr size_t sm = sizeof(cx<T>) * (size_t)N * kW64Stride;
+      if ((e = group_attr(&k_plane_w64_gc<T>, sm)) != hipSuccess) return e;
+      hipLaunchKernelGGL((k_plane_w64_gc<T>), dim3(blocks), dim3(512), sm, s, g, n);
+      return hipGetLastError();
+    }
+  }
+  if constexpr (N % 4 == 0 && band_plane_smem<T, N>() <= 160 * 1024) {
+    if (kind == GK_BAND) {
+      constexpr int NT = band_plane_threads<N>();
+      constexpr size_t sm = band_plane_smem<T, N>();
+      if ((e = group_attr(&k_plane_band_gc<T, N, NT>, sm)) != hipSuccess) return e;
+      hipLaunchKernelGGL((k_plane_band_gc<T, N, NT>), dim3(blocks), dim3(NT), sm, s, g, n);
+      return hipGetLastError();
+    }
+  }
+  if constexpr (plane_smem<T, N, N>() <= 160 * 1024) {
+    if (kind == GK_LDS) {
+      constexpr size_t sm = plane_smem<T, N, N>();
+      if ((e = group_attr(&k_plane_gc<T, N, kPlaneThreads>, sm)) != hipSuccess) return e;
+      hipLaunchKernelGGL((k_plane_gc<T, N, kPlaneThreads>), dim3(blocks), dim3(kPlaneThreads), sm, s, g, n);
+      return hipGetLastError();
+    }
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t launch_group_fft(int pass, int key, const GroupHpsi<T>* g, int n, int blocks, size_t smem, hipStream_t s) {
+  if (n < 1 || blocks < 1) return hipErrorInvalidValue;
+  const int kind = key / kGroupKeyN, len = key % kGroupKeyN;
+  if (kind == GK_RT) return launch_group_fft_rt<T>(pass, g, n, blocks, smem, s);
+  switch (len) {
+#define PWH_CASE(N) \
+  case N: return launch_group_n<T, N>(pass, kind, g, n, blocks, s);
+    PWH_FFT_SIZES(PWH_CASE)
+#undef PWH_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template int group_fft_key<double>(int, const SphereMaps&, int, int*, size_t*, int*);
+template int group_fft_key<float>(int, const SphereMaps&, int, int*, size_t*, int*);
+template hipError_t launch_group_fft<double>(int, int, const GroupHpsi<double>*, int, int, size_t, hipStream_t);
+template hipError_t launch_group_fft<float>(int, int, const GroupHpsi<float>*, int, int, size_t, hipStream_t);
 
 template <int N>
 static hipError_t launch_plane_rho_n(const SphereMaps& m, const void* tw1, const void* tw2, const cx<double>* W,

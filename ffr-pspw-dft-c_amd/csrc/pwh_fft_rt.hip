@@ -208,15 +208,18 @@ __host__ __device__ inline int odd_stride(int n) { return (n & 1) ? n : n + 1; }
 // ---------------------------------------------------------------------------
 constexpr int kRtColThreads = 256;
 
+// Block (cblk, b) of the i3 passes: columns cblk*CT .. +CT-1 of band b.  The bodies take
+// the block coordinates as arguments so the grouped kernels below run the same code.
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void k_zinv_rt(SphereMaps m, int CT, const cx<T>* __restrict__ tw3g,
-                                                const cx<T>* __restrict__ X, int ldx, cx<T>* __restrict__ W) {
+__device__ __forceinline__ void zinv_rt_block(const SphereMaps& m, int CT, const cx<T>* __restrict__ tw3g,
+                                              const cx<T>* __restrict__ X, int ldx, cx<T>* __restrict__ W,
+                                              int cblk, int b) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int NZ = m.nr3;
   cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
   cx<T>* tw = tile + NZ * CT;
   const int tid = threadIdx.x;
-  const int c0 = blockIdx.x * CT, b = blockIdx.y;
+  const int c0 = cblk * CT;
   const int cn = min(CT, m.ncol_pad - c0);
   const int kb = m.colstart[c0], ke = m.colstart[c0 + cn];
   const cx<T>* xb = X + (size_t)b * ldx;
@@ -233,19 +236,25 @@ __global__ __launch_bounds__(NT) void k_zinv_rt(SphereMaps m, int CT, const cx<T
   }
 }
 
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_zinv_rt(SphereMaps m, int CT, const cx<T>* __restrict__ tw3g,
+                                                const cx<T>* __restrict__ X, int ldx, cx<T>* __restrict__ W) {
+  zinv_rt_block<T, NT>(m, CT, tw3g, X, ldx, W, blockIdx.x, blockIdx.y);
+}
+
 // out = alpha (FFT/nnr + kin X + Hnl) + beta X + gamma Z on the sphere entries
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void k_zfwd_rt(SphereMaps m, int CT, const cx<T>* __restrict__ tw3g,
-                                                const cx<T>* __restrict__ W, const T* __restrict__ kin,
-                                                const cx<T>* __restrict__ X, const cx<T>* __restrict__ Hnl,
-                                                const cx<T>* __restrict__ Z, cx<T>* __restrict__ out, int ld,
-                                                T inv_nnr, T alpha, T beta, T gamma) {
+__device__ __forceinline__ void zfwd_rt_block(const SphereMaps& m, int CT, const cx<T>* __restrict__ tw3g,
+                                              const cx<T>* __restrict__ W, const T* __restrict__ kin,
+                                              const cx<T>* __restrict__ X, const cx<T>* __restrict__ Hnl,
+                                              const cx<T>* __restrict__ Z, cx<T>* __restrict__ out, int ld,
+                                              T inv_nnr, T alpha, T beta, T gamma, int cblk, int b) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int NZ = m.nr3;
   cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
   cx<T>* tw = tile + NZ * CT;
   const int tid = threadIdx.x;
-  const int c0 = blockIdx.x * CT, b = blockIdx.y;
+  const int c0 = cblk * CT;
   const int cn = min(CT, m.ncol_pad - c0);
   const int kb = m.colstart[c0], ke = m.colstart[c0 + cn];
   const cx<T>* wb = W + (size_t)b * NZ * m.ncol_pad + c0;
@@ -269,6 +278,16 @@ __global__ __launch_bounds__(NT) void k_zfwd_rt(SphereMaps m, int CT, const cx<T
   }
 }
 
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_zfwd_rt(SphereMaps m, int CT, const cx<T>* __restrict__ tw3g,
+                                                const cx<T>* __restrict__ W, const T* __restrict__ kin,
+                                                const cx<T>* __restrict__ X, const cx<T>* __restrict__ Hnl,
+                                                const cx<T>* __restrict__ Z, cx<T>* __restrict__ out, int ld,
+                                                T inv_nnr, T alpha, T beta, T gamma) {
+  zfwd_rt_block<T, NT>(m, CT, tw3g, W, kin, X, Hnl, Z, out, ld, inv_nnr, alpha, beta, gamma, blockIdx.x,
+                       blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // Plane pass in LDS (any n1 x n2): inverse i2 on the active i1 columns, inverse i1 on
 // every row times V, forward i1 keeping the active columns, forward i2
@@ -279,9 +298,9 @@ static size_t plane_rt_smem(int n1, int n2) {
 }
 
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void k_plane_rt(SphereMaps m, const cx<T>* __restrict__ tw1g,
-                                                 const cx<T>* __restrict__ tw2g, const T* __restrict__ V,
-                                                 cx<T>* __restrict__ W) {
+__device__ __forceinline__ void plane_rt_block(const SphereMaps& m, const cx<T>* __restrict__ tw1g,
+                                               const cx<T>* __restrict__ tw2g, const T* __restrict__ V,
+                                               cx<T>* __restrict__ W, int z, int b) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int N1 = m.nr1, N2 = m.nr2, S1 = odd_stride(N1);
   cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
@@ -290,7 +309,6 @@ __global__ __launch_bounds__(NT) void k_plane_rt(SphereMaps m, const cx<T>* __re
   int* xa = reinterpret_cast<int*>(tw2 + N2);
   unsigned char* xact = reinterpret_cast<unsigned char*>(xa + N1);
   const int tid = threadIdx.x;
-  const int z = blockIdx.x, b = blockIdx.y;
   const int nxa = m.nxa;
   for (int i = tid; i < N1; i += NT) { tw1[i] = tw1g[i]; xact[i] = 0; }
   for (int i = tid; i < N2; i += NT) tw2[i] = tw2g[i];
@@ -319,6 +337,56 @@ __global__ __launch_bounds__(NT) void k_plane_rt(SphereMaps m, const cx<T>* __re
     const int xy = m.colxy[c];
     wrow[c] = tile[(xy / N1) * S1 + xy % N1];
   }
+}
+
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_plane_rt(SphereMaps m, const cx<T>* __restrict__ tw1g,
+                                                 const cx<T>* __restrict__ tw2g, const T* __restrict__ V,
+                                                 cx<T>* __restrict__ W) {
+  plane_rt_block<T, NT>(m, tw1g, tw2g, V, W, blockIdx.x, blockIdx.y);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped passes: one launch covers the H|psi> pass of every system of a batch that runs
+// on runtime plans.  Block L of the 1D grid belongs to the system whose first block blk0 is
+// the largest one <= L; the system's descriptor carries its maps, plans, constants,
+// buffers and epilogue scalars, and the block runs the same body as the one-system kernels
+// above.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ int group_find(const GroupHpsi<T>* __restrict__ g, int n, int L) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (g[mid].blk0 <= L) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_zinv_group(const GroupHpsi<T>* __restrict__ g, int n) {
+  const int L = blockIdx.x;
+  const GroupHpsi<T>& p = g[group_find(g, n, L)];
+  const int local = L - p.blk0, ncb = p.m.ncol_pad / p.ct;
+  zinv_rt_block<T, NT>(p.m, p.ct, p.tw3, p.X, p.ld, p.W, local % ncb, local / ncb);
+}
+
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_zfwd_group(const GroupHpsi<T>* __restrict__ g, int n) {
+  const int L = blockIdx.x;
+  const GroupHpsi<T>& p = g[group_find(g, n, L)];
+  const int local = L - p.blk0, ncb = p.m.ncol_pad / p.ct;
+  zfwd_rt_block<T, NT>(p.m, p.ct, p.tw3, p.W, p.kin, p.X, p.Hnl, p.Z, p.out, p.ld, p.inv_nnr, p.alpha, p.beta,
+                       p.gamma, local % ncb, local / ncb);
+}
+
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_plane_group(const GroupHpsi<T>* __restrict__ g, int n) {
+  const int L = blockIdx.x;
+  const GroupHpsi<T>& p = g[group_find(g, n, L)];
+  const int local = L - p.blk0;
+  plane_rt_block<T, NT>(p.m, p.tw1, p.tw2, p.V, p.W, local % p.m.nr3, local / p.m.nr3);
 }
 
 // ---------------------------------------------------------------------------
@@ -706,6 +774,45 @@ hipError_t launch_plane_rt(const SphereMaps& m, const void* tw1, const void* tw2
   return hipSuccess;
 }
 
+template <typename T>
+bool group_fft_shape(const SphereMaps& m, int nb, int* ct, int* zblocks, int* pblocks, size_t* zsmem,
+                     size_t* psmem) {
+  constexpr int NT = kRtColThreads;
+  if (m.plan[0].n != m.nr1 || m.plan[1].n != m.nr2 || m.plan[2].n != m.nr3) return false;
+  if (use_split<T>(m) || plane_rt_threads(m) != NT) return false;
+  *ct = rt_lines_per_block(m.plan[2], NT);
+  if (!*ct) return false;
+  *zblocks = m.ncol_pad / *ct * nb;
+  *pblocks = m.nr3 * nb;
+  *zsmem = sizeof(cx<T>) * ((size_t)m.nr3 * *ct + m.nr3);
+  *psmem = plane_rt_smem<T>(m.nr1, m.nr2);
+  return *zsmem <= kLdsMax && *psmem <= kLdsMax;
+}
+
+template <typename T>
+hipError_t launch_group_fft_rt(int pass, const GroupHpsi<T>* g, int n, int blocks, size_t smem, hipStream_t s) {
+  constexpr int NT = kRtColThreads;
+  if (n < 1 || blocks < 1) return hipErrorInvalidValue;
+  hipError_t e;
+  switch (pass) {
+    case 0:
+      if ((e = allow_lds(&k_zinv_group<T, NT>, smem)) != hipSuccess) return e;
+      hipLaunchKernelGGL((k_zinv_group<T, NT>), dim3(blocks), dim3(NT), smem, s, g, n);
+      break;
+    case 1:
+      if ((e = allow_lds(&k_plane_group<T, NT>, smem)) != hipSuccess) return e;
+      hipLaunchKernelGGL((k_plane_group<T, NT>), dim3(blocks), dim3(NT), smem, s, g, n);
+      break;
+    case 2:
+      if ((e = allow_lds(&k_zfwd_group<T, NT>, smem)) != hipSuccess) return e;
+      hipLaunchKernelGGL((k_zfwd_group<T, NT>), dim3(blocks), dim3(NT), smem, s, g, n);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 __global__ void k_sum_groups_rt(const double* __restrict__ part, int groups, size_t n, double* __restrict__ rho) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -780,6 +887,10 @@ hipError_t launch_fft_lines_rt(cx<double>* data, const FftPlan& p, int inner, in
   return hipGetLastError();
 }
 
+template bool group_fft_shape<double>(const SphereMaps&, int, int*, int*, int*, size_t*, size_t*);
+template bool group_fft_shape<float>(const SphereMaps&, int, int*, int*, int*, size_t*, size_t*);
+template hipError_t launch_group_fft_rt<double>(int, const GroupHpsi<double>*, int, int, size_t, hipStream_t);
+template hipError_t launch_group_fft_rt<float>(int, const GroupHpsi<float>*, int, int, size_t, hipStream_t);
 template hipError_t launch_zinv_rt<double>(const SphereMaps&, const void*, const cx<double>*, int, cx<double>*, int,
                                            hipStream_t);
 template hipError_t launch_zinv_rt<float>(const SphereMaps&, const void*, const cx<float>*, int, cx<float>*, int,

@@ -80,10 +80,11 @@ struct RowLoader {
 template <typename T, int KS>
 constexpr size_t gemm_smem() { return 4 * sizeof(T) * (size_t)KS * (BT + LPAD); }
 
+// Block L of the projection grid (the kernel body, also run by the grouped kernel)
 template <typename T, bool BETA, int KS>
-__global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const cx<T>* __restrict__ B,
-                                                  int ldb, int M, int N, int K, int kchunk,
-                                                  cx<T>* __restrict__ part) {
+__device__ __forceinline__ void gemm_proj_block(RowLoader<T, BETA> ld, const cx<T>* __restrict__ B, int ldb,
+                                                int M, int N, int K, int kchunk, cx<T>* __restrict__ part,
+                                                const int L) {
   using acc_t = typename AccT<T>::type;
   static_assert(KS % 16 == 0, "reduction depth per stage: multiple of 16");
   extern __shared__ __align__(16) unsigned char gsm[];
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
   // The tiles of one K-split read the same chunk of both operands, so they get the same
   // L % 8 and neighbouring L: the re-reads hit that XCD's L2.  Splits past the last exit.
   // With a split count that is not a multiple of 8 the plain order (tile fastest) is used.
-  const int L = blockIdx.x, q = L >> 3;
+  const int q = L >> 3;
   const bool xcd = ((K + kchunk - 1) / kchunk) % 8 == 0;
   const int tile = xcd ? q % tiles : L % tiles;
   const int split = xcd ? (L & 7) + 8 * (q / tiles) : L / tiles;
@@ -226,15 +227,22 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
       }
 }
 
+template <typename T, bool BETA, int KS>
+__global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const cx<T>* __restrict__ B,
+                                                  int ldb, int M, int N, int K, int kchunk,
+                                                  cx<T>* __restrict__ part) {
+  gemm_proj_block<T, BETA, KS>(ld, B, ldb, M, N, K, kchunk, part, blockIdx.x);
+}
+
 // C = scale(m) * sum_s part[s][m][n], stored C[m*ldc + n] (row-major) or C[m + n*ldc]
 // (column-major, the LAPACK layout handed to rocSOLVER); scale(m) = wl[m % lm] when wl != null.
 // One output per thread: eight independent partial sums (split s goes to sum s % 8) keep
 // eight loads in flight, then a fixed tree adds them (deterministic).
 template <typename T, typename TO>
-__global__ __launch_bounds__(256) void k_reduce_parts(const cx<T>* __restrict__ part, int S, int M,
-                                                      int N, cx<TO>* __restrict__ C, int ldc,
-                                                      const T* __restrict__ wl, int lm, int colmajor) {
-  const int idx = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void reduce_parts_block(const cx<T>* __restrict__ part, int S, int M, int N,
+                                                   cx<TO>* __restrict__ C, int ldc, const T* __restrict__ wl,
+                                                   int lm, int colmajor, const int blk) {
+  const int idx = blk * 256 + threadIdx.x;
   const size_t MN = (size_t)M * N;
   if (idx >= M * N) return;
   TO ar[8], ai[8];
@@ -260,13 +268,20 @@ __global__ __launch_bounds__(256) void k_reduce_parts(const cx<T>* __restrict__ 
   else C[(size_t)m * ldc + n] = mk<TO>(tr, ti);
 }
 
+template <typename T, typename TO>
+__global__ __launch_bounds__(256) void k_reduce_parts(const cx<T>* __restrict__ part, int S, int M,
+                                                      int N, cx<TO>* __restrict__ C, int ldc,
+                                                      const T* __restrict__ wl, int lm, int colmajor) {
+  reduce_parts_block<T, TO>(part, S, M, N, C, ldc, wl, lm, colmajor, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // expand: out[n][k] (+)= sum_j A[j][k] Q[j][n]
 // ---------------------------------------------------------------------------
 template <typename T, bool BETA, bool QCM, int KS>
-__global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const cx<T>* __restrict__ Q,
-                                                    int ldq, int J, int N, int K,
-                                                    cx<T>* __restrict__ out, int ldo, int accumulate) {
+__device__ __forceinline__ void gemm_expand_block(RowLoader<T, BETA> ld, const cx<T>* __restrict__ Q, int ldq,
+                                                  int J, int N, int K, cx<T>* __restrict__ out, int ldo,
+                                                  int accumulate, const int L) {
   using acc_t = typename AccT<T>::type;
   static_assert(KS % 16 == 0, "reduction depth per stage: multiple of 16");
   extern __shared__ __align__(16) unsigned char gsm[];
@@ -279,7 +294,7 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
   // XCD-aware order (as in k_gemm_proj): the band tiles of one plane-wave tile read the
   // same rows of A, so they share L % 8 and run close together; tiles past K exit.
   const int tiles_n = (N + BT - 1) / BT;
-  const int L = blockIdx.x, q = L >> 3;
+  const int q = L >> 3;
   const int k0 = ((L & 7) + 8 * (q / tiles_n)) * BT, n0 = (q % tiles_n) * BT;
   if (k0 >= K) return;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -396,6 +411,51 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
           *o = v;
         }
       }
+}
+
+template <typename T, bool BETA, bool QCM, int KS>
+__global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const cx<T>* __restrict__ Q,
+                                                    int ldq, int J, int N, int K,
+                                                    cx<T>* __restrict__ out, int ldo, int accumulate) {
+  gemm_expand_block<T, BETA, QCM, KS>(ld, Q, ldq, J, N, K, out, ldo, accumulate, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped projector GEMMs: block L of a launch belongs to the problem whose first block is
+// the largest one <= L, and runs the one-problem body with its local index.
+// ---------------------------------------------------------------------------
+template <typename P, typename F>
+__device__ __forceinline__ int group_problem(const P* __restrict__ g, int n, int L, F first) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first(g[mid]) <= L) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename T, int KS>
+__global__ __launch_bounds__(GT) void k_gemm_proj_group(const GroupProj<T>* __restrict__ g, int n) {
+  const int L = blockIdx.x;
+  const GroupProj<T>& p = g[group_problem(g, n, L, [](const GroupProj<T>& x) { return x.blk0; })];
+  RowLoader<T, false> ld{p.A, p.lda};
+  gemm_proj_block<T, false, KS>(ld, p.B, p.ldb, p.M, p.N, p.K, p.kchunk, p.part, L - p.blk0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_reduce_group(const GroupProj<T>* __restrict__ g, int n) {
+  const int L = blockIdx.x;
+  const GroupProj<T>& p = g[group_problem(g, n, L, [](const GroupProj<T>& x) { return x.rblk0; })];
+  reduce_parts_block<T, T>(p.part, p.S, p.M, p.N, p.C, p.ldc, p.wl, p.lm, 0, L - p.rblk0);
+}
+
+template <typename T, int KS>
+__global__ __launch_bounds__(GT) void k_gemm_expand_group(const GroupExpand<T>* __restrict__ g, int n) {
+  const int L = blockIdx.x;
+  const GroupExpand<T>& p = g[group_problem(g, n, L, [](const GroupExpand<T>& x) { return x.blk0; })];
+  RowLoader<T, false> ld{p.A, p.lda};
+  gemm_expand_block<T, false, false, KS>(ld, p.Q, p.ldq, p.J, p.N, p.K, p.out, p.ldo, p.accumulate, L - p.blk0);
 }
 
 // ---------------------------------------------------------------------------
@@ -875,6 +935,57 @@ static hipError_t launch_expand_bf(dim3 grid, hipStream_t s, const cx<float>* A,
   return hipGetLastError();
 }
 
+// Split-K shape: S chunks of kchunk plane waves (a multiple of the stage depth ks), and
+// the 1D grid (tiles times S rounded up to 8, decoded XCD-aware in the kernel)
+static void split_shape(int M, int N, int K, size_t part_elems, int target, int ks, int* S_out, int* kchunk_out,
+                        int* blocks) {
+  const int tiles = ((M + BT - 1) / BT) * ((N + BT - 1) / BT);
+  int S = choose_split(tiles, K, part_elems, (size_t)M * N, target);
+  int kchunk = (K + S - 1) / S;
+  kchunk = ((kchunk + ks - 1) / ks) * ks;
+  S = (K + kchunk - 1) / kchunk;
+  *S_out = S;
+  *kchunk_out = kchunk;
+  *blocks = tiles * ((S + 7) / 8) * 8;
+}
+
+template <typename T>
+int proj_split_shape(int M, int N, int K, int target, int* S, int* kchunk, int* blocks) {
+  if (gemm_ks<T>() != 16) return 0;
+  split_shape(M, N, K, proj_part_elems(M, N, K, target), target, 16, S, kchunk, blocks);
+  return 1;
+}
+
+int expand_grid_blocks(int K, int N) { return expand_blocks(K, N); }
+
+template <typename T>
+hipError_t launch_group_proj(const GroupProj<T>* g, int n, int blocks, int rblocks, hipStream_t s) {
+  if (n < 1 || blocks < 1 || rblocks < 1) return hipErrorInvalidValue;
+  constexpr size_t sm = gemm_smem<T, 16>();
+  static hipError_t attr = allow_smem(&k_gemm_proj_group<T, 16>, sm);
+  if (attr != hipSuccess) return attr;
+  hipLaunchKernelGGL((k_gemm_proj_group<T, 16>), dim3(blocks), dim3(GT), sm, s, g, n);
+  hipLaunchKernelGGL((k_reduce_group<T>), dim3(rblocks), dim3(256), 0, s, g, n);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_group_expand(const GroupExpand<T>* g, int n, int blocks, hipStream_t s) {
+  if (n < 1 || blocks < 1) return hipErrorInvalidValue;
+  constexpr size_t sm = gemm_smem<T, 16>();
+  static hipError_t attr = allow_smem(&k_gemm_expand_group<T, 16>, sm);
+  if (attr != hipSuccess) return attr;
+  hipLaunchKernelGGL((k_gemm_expand_group<T, 16>), dim3(blocks), dim3(GT), sm, s, g, n);
+  return hipGetLastError();
+}
+
+template int proj_split_shape<double>(int, int, int, int, int*, int*, int*);
+template int proj_split_shape<float>(int, int, int, int, int*, int*, int*);
+template hipError_t launch_group_proj<double>(const GroupProj<double>*, int, int, int, hipStream_t);
+template hipError_t launch_group_proj<float>(const GroupProj<float>*, int, int, int, hipStream_t);
+template hipError_t launch_group_expand<double>(const GroupExpand<double>*, int, int, hipStream_t);
+template hipError_t launch_group_expand<float>(const GroupExpand<float>*, int, int, hipStream_t);
+
 // split-K projection: S chunks of kchunk plane waves (a multiple of the stage depth)
 template <typename T, bool BETA>
 static hipError_t run_proj(RowLoader<T, BETA> ld, const cx<T>* B, int ldb, int M, int N, int K,
@@ -882,13 +993,10 @@ static hipError_t run_proj(RowLoader<T, BETA> ld, const cx<T>* B, int ldb, int M
                            int target = 0) {
   const int nsplit = std::is_same<T, float>::value ? bf_terms(mode) : 0;
   const int ks = nsplit ? BKS : gemm_ks<T>();
-  const int tiles = ((M + BT - 1) / BT) * ((N + BT - 1) / BT);
-  int S = choose_split(tiles, K, part_elems, (size_t)M * N, target);
-  int kchunk = (K + S - 1) / S;
-  kchunk = ((kchunk + ks - 1) / ks) * ks;
-  S = (K + kchunk - 1) / kchunk;
+  int S, kchunk, blocks;
+  split_shape(M, N, K, part_elems, target, ks, &S, &kchunk, &blocks);
   if ((size_t)S * M * N > part_elems) return hipErrorInvalidValue;
-  dim3 grid(tiles * ((S + 7) / 8) * 8);   // 1D, decoded XCD-aware in the kernel
+  dim3 grid(blocks);   // 1D, decoded XCD-aware in the kernel
   hipError_t e;
   if constexpr (std::is_same<T, float>::value) {
     if (nsplit == 3) e = launch_proj_bf<3>(grid, s, ld.A, ld.lda, B, ldb, M, N, K, kchunk, part);

@@ -172,6 +172,41 @@ hipError_t launch_plane_rho_rt(const SphereMaps& m, const void* tw1, const void*
 hipError_t launch_fft_lines_rt(cx<double>* data, const FftPlan& p, int inner, int nlines, const cx<double>* tw,
                                int sign, double scale, hipStream_t s);
 
+// Grouped FFT passes of H|psi> over the systems of a batch (pwh_fft_rt.hip): one launch per
+// pass, one descriptor per system (device memory), systems in order of their first blocks.
+template <typename T>
+struct GroupHpsi {
+  SphereMaps m;
+  const cx<T>* tw1;
+  const cx<T>* tw2;
+  const cx<T>* tw3;
+  const T* V;
+  const T* kin;
+  const cx<T>* X;     // the block H acts on (ld), also the beta X term of the epilogue
+  cx<T>* W;           // the system's column workspace
+  const cx<T>* Hnl;   // nonlocal part (null: none)
+  const cx<T>* Z;     // gamma Z term (null: none)
+  cx<T>* out;
+  int ld, nb, ct;      // ct: column tile of the runtime-plan i3 passes
+  T inv_nnr, alpha, beta, gamma;
+  int blk0;           // first block of the system in its launch's grid
+};
+// The runtime-plan shape of a system in the grouped passes: false when its plane needs
+// the split pass or more than 256 lanes.
+template <typename T>
+bool group_fft_shape(const SphereMaps& m, int nb, int* ct, int* zblocks, int* pblocks, size_t* zsmem,
+                     size_t* psmem);
+// pass 0: i3 inverse, 1: plane, 2: i3 forward with the epilogue.  group_fft_key: the kernel
+// a system's pass runs (systems with equal keys share a launch), its blocks and LDS, and
+// the runtime column tile; -1 when the pass cannot be grouped.  launch_group_fft: one
+// launch of a key over n systems (blocks: their total, smem: their largest).
+template <typename T>
+int group_fft_key(int pass, const SphereMaps& m, int nb, int* blocks, size_t* smem, int* ct);
+template <typename T>
+hipError_t launch_group_fft(int pass, int key, const GroupHpsi<T>* g, int n, int blocks, size_t smem, hipStream_t s);
+template <typename T>
+hipError_t launch_group_fft_rt(int pass, const GroupHpsi<T>* g, int n, int blocks, size_t smem, hipStream_t s);
+
 // GEMM launchers (pwh_gemm.hip)
 struct BetaGen {
   // the projectors of one species, beta[(a,l)][k] = EIGR(k,a) * PKG_A(l,k), stored
@@ -196,9 +231,51 @@ hipError_t gemm_nn_f64(const cx<double>* A, int lda, int J, const cx<double>* Q,
                        int K, cx<double>* out, int ldo, hipStream_t s);
 hipError_t mfma_selftest(int kind, double* max_err);
 
+// Grouped projector GEMMs over the systems (and species) of a batch (pwh_gemm.hip), one
+// launch per kind.  Each problem keeps the split-K shape the one-system path gives it
+// (proj_split_shape with the system's target), so the sums are the same.
+template <typename T>
+struct GroupProj {
+  const cx<T>* A;     // beta rows of the species, [M][lda]
+  const cx<T>* B;     // the block, [N][ldb]
+  cx<T>* part;        // S * M * N partial sums
+  cx<T>* C;           // Q rows of the species, [M][ldc]
+  const T* wl;        // WNL(l) of the species
+  int lda, ldb, ldc, M, N, K, kchunk, S, lm;
+  int blk0;           // first block in the projection grid
+  int rblk0;          // first block in the reduction grid
+};
+template <typename T>
+struct GroupExpand {
+  const cx<T>* A;     // beta rows, [J][lda]
+  const cx<T>* Q;     // [J][ldq]
+  cx<T>* out;         // [N][ldo]
+  int lda, ldq, ldo, J, N, K, accumulate;
+  int blk0;           // first block in the expansion grid
+};
+// Split-K shape of the projection C[M][N] over K in precision T (the one-system rule):
+// S partial sums over chunks of kchunk; blocks of its grid.  0 when the grouped kernels
+// cannot reproduce the one-system launch (another stage depth is configured).
+template <typename T>
+int proj_split_shape(int M, int N, int K, int target, int* S, int* kchunk, int* blocks);
+int expand_grid_blocks(int K, int N);
+template <typename T>
+hipError_t launch_group_proj(const GroupProj<T>* g, int n, int blocks, int rblocks, hipStream_t s);
+template <typename T>
+hipError_t launch_group_expand(const GroupExpand<T>* g, int n, int blocks, hipStream_t s);
+
 // Elementwise / small kernels (pwh_blas.hip)
 template <typename Tin, typename Tout>
 hipError_t convert_cx(const cx<Tin>* in, cx<Tout>* out, size_t n, hipStream_t s);
+// one array of a grouped conversion: n complex values, from block blk0 on (256 per block)
+struct GroupConv {
+  const void* in;
+  void* out;
+  size_t n;
+  int blk0;
+};
+template <typename Tin, typename Tout>
+hipError_t launch_group_convert(const GroupConv* g, int n, int blocks, hipStream_t s);
 hipError_t permute_in_f64(const cx<double>* ref, int ldr, const int* perm, int ngw, int ngw_pad,
                           cx<double>* out, int nb, hipStream_t s);
 hipError_t permute_out_f64(const cx<double>* in, int ngw_pad, const int* perm, int ngw,

@@ -649,7 +649,7 @@ class System:
 class Batch:
     """Independent systems whose resident blocks are Chebyshev-filtered together
     (pwh_batch_*): mode 0 one after the other, mode 1 one hipGraph with a branch per
-    system."""
+    system, mode 2 grouped (each pass of each H|psi> one launch over all systems)."""
 
     def __init__(self, systems, nstates):
         self.systems = list(systems)

@@ -346,7 +346,14 @@ int pwh_batch_destroy(pwh_batch *batch);
 int pwh_batch_set(pwh_batch *batch, int isys, const double *X);
 int pwh_batch_get(pwh_batch *batch, int isys, double *X);
 /* chebyfilt.cpp:6-74 on every resident block; lb/ub [nsys].  mode 0: systems one after
- * the other on the library stream; mode 1: one hipGraph with a branch per system. */
+ * the other on the library stream; mode 1: one hipGraph with a branch per system; mode 2:
+ * grouped, every pass of every H|psi> of the filter is one launch over all systems that
+ * run the same kernel (the systems of one grid share the compiled per-length kernel;
+ * systems on runtime plans share one launch), with one descriptor per system for its
+ * maps, plans, buffers and scalars; the launches are captured into one hipGraph.  Mode 2
+ * takes PWH_FP64 and PWH_FP32 and systems with at least two states whose plane pass does
+ * not need the split path, and gives each system its one-at-a-time result bit for bit;
+ * anything else returns PWH_ERR_UNSUPPORTED. */
 int pwh_batch_chebyfilt(pwh_batch *batch, int degree, const double *lb, const double *ub,
                         int precision, int mode);
 

@@ -12,7 +12,9 @@
     residual norms of a band subset recomputed with the oracle's H|psi>
     (ChebySCFLoop.cpp:271-311,415-433).
   * 24 uneven cells (grids 24^3 to 60^3, 8 to 96 bands) through one captured filter:
-    bitwise equal to the cells filtered one at a time, two cells against the oracle.
+    bitwise equal to the cells filtered one at a time, two cells against the oracle; the
+    grouped filter (one launch per pass and kernel over all cells) bitwise equal to the
+    cells filtered one at a time.
 """
 import numpy as np
 import pytest
@@ -171,6 +173,61 @@ def test_c3_24_cells_graph_equals_one_at_a_time(cells24, oracle):
                 c = cells[i]
                 ref = oracle.chebyfilt(oracle_globals(c), X0[i], 5, lb[i], ub[i], c.vloc, c.ngw)
                 assert relerr(seq[i], ref) < 1e-11, i
+    B.close()
+    for s in systems:
+        s.close()
+
+
+def test_c3_grouped_launches_equal_one_at_a_time(cells24, oracle):
+    """Mode 2 (each pass of each H|psi> one launch per kernel over the 24 systems: the
+    systems of one grid share a launch) gives every system its one-at-a-time filter bit for
+    bit, on replays of the captured graph too, on the compiled and on the runtime FFT path;
+    two cells match the oracle's chebyfilt."""
+    import pwh
+    cells = cells24
+    pwh.init(0)
+    systems = []
+    for c in cells:
+        s = pwh.System(c.globals())
+        s.set_vloc(c.vloc)
+        systems.append(s)
+    X0 = [c.psi(c.nbands, seed=40 + i) for i, c in enumerate(cells)]
+    lb = np.array([-0.5 + 0.03 * i for i in range(len(cells))])
+    ub = np.array([40.0 + 0.7 * i for i in range(len(cells))])
+    B = pwh.Batch(systems, [c.nbands for c in cells])
+
+    def run(prec, mode, degree):
+        for i, x in enumerate(X0):
+            B.set(i, x)
+        B.chebyfilt(degree, lb, ub, prec, mode)
+        pwh.synchronize()
+        return [B.get(i) for i in range(len(cells))]
+
+    for path, cases in (("auto", (("fp64", 5), ("fp32", 5), ("fp32", 2), ("fp64", 1), ("fp32", 1))),
+                        ("runtime", (("fp64", 3), ("fp32", 3)))):
+        for s in systems:
+            assert s.fft_path(path) in ("compiled", "runtime")
+        for prec, degree in cases:
+            seq = run(prec, 0, degree)
+            for rep in range(2):
+                grp = run(prec, 2, degree)
+                for i in range(len(cells)):
+                    assert np.array_equal(grp[i], seq[i]), (path, prec, degree, rep, i)
+    for s in systems:
+        s.fft_path("auto")
+    grp = run("fp64", 2, 5)
+    small = min(range(len(cells)), key=lambda i: cells[i].ngw * cells[i].nbands)
+    big = max(range(len(cells)), key=lambda i: cells[i].ngw)
+    for i in (small, big):
+        c = cells[i]
+        ref = oracle.chebyfilt(oracle_globals(c), X0[i], 5, lb[i], ub[i], c.vloc, c.ngw)
+        assert relerr(grp[i], ref) < 1e-11, i
+    with pytest.raises(pwh.PwhError) as e:
+        B.chebyfilt(3, lb, ub, "bf16x3", 2)
+    assert e.value.code == pwh.PWH_ERR_UNSUPPORTED
+    with pytest.raises(pwh.PwhError) as e:
+        B.chebyfilt(3, lb, ub, "fp32", 3)
+    assert e.value.code == pwh.PWH_ERR_ARG
     B.close()
     for s in systems:
         s.close()
