@@ -590,7 +590,9 @@ int zgemm_nn_small(int n, const cx<double>* A, const cx<double>* B, cx<double>* 
 
 // CholeskyQR2 (FP64) of the nb columns of X [nb][ld] (nrows valid rows), using tmp [nb][ld];
 // dinfo: 2 ints of device scratch.  For nb <= 128 the factor inverse comes from the
-// one-workgroup kernel and both passes are checked with one synchronisation at the end.
+// one-workgroup kernel and both passes are checked with one synchronisation at the end;
+// above, from rocSOLVER's zpotrf + ztrtri on the upper triangle of the Gram matrix (the
+// only half computed), with X R^-1 reducing over the nonzero half of R^-1 only.
 // span_only (a caller that needs an orthonormal basis of span(X), not the R of a QR: the
 // ChFSI loop, whose Rayleigh-Ritz step rotates the basis anyway), with lw (2 nb^2) and
 // dmax (1) given: after the first pass G = X^H X = I + E is nearly the identity, and when
@@ -656,10 +658,11 @@ int cholqr2(cx<double>* X, int ld, int nrows, int nb, cx<double>* tmp, cx<double
                                         std::to_string(hinfo[pass]) + ")");
     return PWH_OK;
   }
-  for (int pass = 0; pass < 2; ++pass) {
-    cx<double>* src = (pass == 0) ? X : tmp;
-    cx<double>* dst = (pass == 0) ? tmp : X;
-    HIPCHK(gemm_cn_f64(src, ld, nb, src, ld, nb, nrows, S, nb, part, part_elems, g.stream));
+  // nb > 128: rocSOLVER's zpotrf and ztrtri on the Gram matrix, of which only the upper
+  // triangle is computed (all zpotrf reads); X R^-1 skips the zero half of R^-1.  With
+  // span_only the second pass is the series above when the first left X^H X close to I.
+  auto lapack_pass = [&](cx<double>* src, cx<double>* dst, bool have_gram) -> int {
+    if (!have_gram) HIPCHK(gemm_gram_upper_f64(src, ld, nb, nrows, S, part, part_elems, g.stream));
     RBCHK(rocsolver_zpotrf(g.rb, rocblas_fill_upper, nb, reinterpret_cast<rocblas_double_complex*>(S),
                            nb, dinfo));
     rocblas_int hinfo = 0;
@@ -671,9 +674,34 @@ int cholqr2(cx<double>* X, int ld, int nrows, int nb, cx<double>* tmp, cx<double
     RBCHK(rocsolver_ztrtri(g.rb, rocblas_fill_upper, rocblas_diagonal_non_unit, nb,
                            reinterpret_cast<rocblas_double_complex*>(S), nb, dinfo));
     HIPCHK(zero_lower(S, nb, nb, g.stream));
-    HIPCHK(gemm_nn_f64(src, ld, nb, S, nb, nb, nrows, dst, ld, g.stream));
+    HIPCHK(gemm_nn_tri_f64(src, ld, nb, S, nb, nb, nrows, dst, ld, g.stream));
+    return PWH_OK;
+  };
+  CHK(lapack_pass(X, tmp, false));
+  if (lw && dmax && cholqr_series_tol() > 0.0) {
+    HIPCHK(gemm_gram_upper_f64(tmp, ld, nb, nrows, S, part, part_elems, g.stream));
+    HIPCHK(herm_fill_lower(S, nb, nb, g.stream));
+    HIPCHK(eye_residual(S, nb, dmax, g.stream));
+    double emax = 0.0;
+    HIPCHK(hipMemcpyAsync(&emax, dmax, sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if ((double)nb * emax <= cholqr_series_tol()) {
+      cx<double>* A = lw;
+      cx<double>* B = lw + (size_t)nb * nb;
+      HIPCHK(axpy_eye(A, S, 35.0 / 128.0, -5.0 / 16.0, nb, g.stream));
+      CHK(zgemm_nn_small(nb, S, A, B));
+      HIPCHK(axpy_eye(B, B, 1.0, 3.0 / 8.0, nb, g.stream));
+      CHK(zgemm_nn_small(nb, S, B, A));
+      HIPCHK(axpy_eye(A, A, 1.0, -0.5, nb, g.stream));
+      CHK(zgemm_nn_small(nb, S, A, B));
+      HIPCHK(axpy_eye(B, B, 1.0, 1.0, nb, g.stream));
+      HIPCHK(gemm_nn_f64(tmp, ld, nb, B, nb, nb, nrows, X, ld, g.stream));
+      return PWH_OK;
+    }
+    HIPCHK(axpy_eye(S, S, 1.0, 1.0, nb, g.stream));   // G = E + I
+    return lapack_pass(tmp, X, true);
   }
-  return PWH_OK;
+  return lapack_pass(tmp, X, false);
 }
 
 int sys_cholqr2(pwh_system* s, cx<double>* X, int nb, bool span_only = false) {

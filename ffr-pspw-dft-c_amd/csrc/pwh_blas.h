@@ -23,6 +23,8 @@ hipError_t multi_dotc_split(const cx<double>* V, int ld, int j, const cx<double>
 // p[0:n] = 0 by a kernel (a memset node does not replay reliably inside a captured graph)
 hipError_t fill_zero(double* p, size_t n, hipStream_t s);
 hipError_t symmetrize(cx<double>* A, int n, int ld, hipStream_t s);
+// column-major n x n: the strict lower triangle <- the conjugate of the upper one
+hipError_t herm_fill_lower(cx<double>* A, int n, int ld, hipStream_t s);
 hipError_t zero_lower(cx<double>* A, int n, int ld, hipStream_t s);
 // Column-major n x n, ld = n, one workgroup: G <- G - I, maxabs[0] = max |Re|, |Im| of it
 hipError_t eye_residual(cx<double>* G, int n, double* maxabs, hipStream_t s);

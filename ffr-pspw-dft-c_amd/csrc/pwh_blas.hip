@@ -243,6 +243,18 @@ hipError_t symmetrize(cx<double>* A, int n, int ld, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Column-major: fill the strict lower triangle with the conjugate of the upper one
+__global__ void k_herm_fill_lower(cx<double>* A, int n, int ld) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= n || i <= j) return;
+  A[i + (size_t)j * ld] = conjg(A[j + (size_t)i * ld]);
+}
+hipError_t herm_fill_lower(cx<double>* A, int n, int ld, hipStream_t s) {
+  hipLaunchKernelGGL(k_herm_fill_lower, dim3((n + 63) / 64, n), dim3(64), 0, s, A, n, ld);
+  return hipGetLastError();
+}
+
 // Column-major: zero the strict lower triangle
 __global__ void k_zero_lower(cx<double>* A, int n, int ld) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -80,8 +80,10 @@ struct RowLoader {
 template <typename T, int KS>
 constexpr size_t gemm_smem() { return 4 * sizeof(T) * (size_t)KS * (BT + LPAD); }
 
-// Block L of the projection grid (the kernel body, also run by the grouped kernel)
-template <typename T, bool BETA, int KS>
+// Block L of the projection grid (the kernel body, also run by the grouped kernel).
+// UPPER: only the tiles that reach the upper triangle (m <= n) are computed; the others
+// exit (a Hermitian product whose lower triangle is not read).
+template <typename T, bool BETA, int KS, bool UPPER = false>
 __device__ __forceinline__ void gemm_proj_block(RowLoader<T, BETA> ld, const cx<T>* __restrict__ B, int ldb,
                                                 int M, int N, int K, int kchunk, cx<T>* __restrict__ part,
                                                 const int L) {
@@ -105,6 +107,7 @@ __device__ __forceinline__ void gemm_proj_block(RowLoader<T, BETA> ld, const cx<
   const int split = xcd ? (L & 7) + 8 * (q / tiles) : L / tiles;
   if (split * kchunk >= K) return;
   const int m0 = (tile / tiles_n) * BT, n0 = (tile % tiles_n) * BT;
+  if (UPPER && m0 >= n0 + BT) return;
   const int ks = split * kchunk, ke = min(K, ks + kchunk);
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int li = lane & 15, lq = lane >> 4;
@@ -234,6 +237,13 @@ __global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const c
   gemm_proj_block<T, BETA, KS>(ld, B, ldb, M, N, K, kchunk, part, blockIdx.x);
 }
 
+template <int KS>
+__global__ __launch_bounds__(GT) void k_gemm_gram_upper(RowLoader<double, false> ld, const cx<double>* __restrict__ B,
+                                                        int ldb, int M, int N, int K, int kchunk,
+                                                        cx<double>* __restrict__ part) {
+  gemm_proj_block<double, false, KS, true>(ld, B, ldb, M, N, K, kchunk, part, blockIdx.x);
+}
+
 // C = scale(m) * sum_s part[s][m][n], stored C[m*ldc + n] (row-major) or C[m + n*ldc]
 // (column-major, the LAPACK layout handed to rocSOLVER); scale(m) = wl[m % lm] when wl != null.
 // One output per thread: eight independent partial sums (split s goes to sum s % 8) keep
@@ -278,7 +288,9 @@ __global__ __launch_bounds__(256) void k_reduce_parts(const cx<T>* __restrict__ 
 // ---------------------------------------------------------------------------
 // expand: out[n][k] (+)= sum_j A[j][k] Q[j][n]
 // ---------------------------------------------------------------------------
-template <typename T, bool BETA, bool QCM, int KS>
+// TRI: Q is upper triangular (Q[j][n] = 0 for j > n), so the tile of columns n0.. reduces
+// over j < n0 + BT only (the skipped products are exact zeros: the result is unchanged)
+template <typename T, bool BETA, bool QCM, int KS, bool TRI = false>
 __device__ __forceinline__ void gemm_expand_block(RowLoader<T, BETA> ld, const cx<T>* __restrict__ Q, int ldq,
                                                   int J, int N, int K, cx<T>* __restrict__ out, int ldo,
                                                   int accumulate, const int L) {
@@ -297,6 +309,7 @@ __device__ __forceinline__ void gemm_expand_block(RowLoader<T, BETA> ld, const c
   const int q = L >> 3;
   const int k0 = ((L & 7) + 8 * (q / tiles_n)) * BT, n0 = (q % tiles_n) * BT;
   if (k0 >= K) return;
+  if (TRI) J = min(J, n0 + BT);
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int li = lane & 15, lq = lane >> 4;
   // FP32 uses three real products per complex one (3M): cr, c2, ci accumulate
@@ -418,6 +431,13 @@ __global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const
                                                     int ldq, int J, int N, int K,
                                                     cx<T>* __restrict__ out, int ldo, int accumulate) {
   gemm_expand_block<T, BETA, QCM, KS>(ld, Q, ldq, J, N, K, out, ldo, accumulate, blockIdx.x);
+}
+
+template <int KS>
+__global__ __launch_bounds__(GT) void k_gemm_nn_tri(RowLoader<double, false> ld, const cx<double>* __restrict__ Q,
+                                                    int ldq, int J, int N, int K, cx<double>* __restrict__ out,
+                                                    int ldo) {
+  gemm_expand_block<double, false, true, KS, true>(ld, Q, ldq, J, N, K, out, ldo, 0, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1062,6 +1082,29 @@ hipError_t gemm_cn_f64(const cx<double>* A, int lda, int M, const cx<double>* B,
   const int tot = M * N;
   hipLaunchKernelGGL((k_reduce_parts<double, double>), dim3((tot + 255) / 256), dim3(256), 0, s, part,
                      S, M, N, C, ldc, static_cast<const double*>(nullptr), 1, 1);
+  return hipGetLastError();
+}
+
+hipError_t gemm_gram_upper_f64(const cx<double>* A, int lda, int N, int K, cx<double>* C, cx<double>* part,
+                               size_t part_elems, hipStream_t s) {
+  int S, kchunk, blocks;
+  split_shape(N, N, K, part_elems, 0, 16, &S, &kchunk, &blocks);
+  if ((size_t)S * N * N > part_elems) return hipErrorInvalidValue;
+  RowLoader<double, false> ld{A, lda};
+  constexpr size_t sm = gemm_smem<double, 16>();
+  hipLaunchKernelGGL((k_gemm_gram_upper<16>), dim3(blocks), dim3(GT), sm, s, ld, A, lda, N, N, K, kchunk, part);
+  const int tot = N * N;
+  hipLaunchKernelGGL((k_reduce_parts<double, double>), dim3((tot + 255) / 256), dim3(256), 0, s, part, S, N, N, C,
+                     N, static_cast<const double*>(nullptr), 1, 1);
+  return hipGetLastError();
+}
+
+hipError_t gemm_nn_tri_f64(const cx<double>* A, int lda, int J, const cx<double>* Q, int ldq, int N, int K,
+                           cx<double>* out, int ldo, hipStream_t s) {
+  RowLoader<double, false> ld{A, lda};
+  constexpr size_t sm = gemm_smem<double, 16>();
+  hipLaunchKernelGGL((k_gemm_nn_tri<16>), dim3(expand_blocks(K, N)), dim3(GT), sm, s, ld, Q, ldq, J, N, K, out,
+                     ldo);
   return hipGetLastError();
 }
 

@@ -229,6 +229,14 @@ hipError_t gemm_cn_f64(const cx<double>* A, int lda, int M, const cx<double>* B,
 // out[n][k] = sum_j A[j][k] Q[j][n]  (A: [J][lda], Q: [J][ldq] row j, out: [N][ldo])
 hipError_t gemm_nn_f64(const cx<double>* A, int lda, int J, const cx<double>* Q, int ldq, int N,
                        int K, cx<double>* out, int ldo, hipStream_t s);
+// The Gram matrix C = A^H A (A: [N][lda] rows = vectors), column-major N x N; only its
+// upper triangle (m <= n) is computed, the strict lower triangle is left undefined.
+hipError_t gemm_gram_upper_f64(const cx<double>* A, int lda, int N, int K, cx<double>* C, cx<double>* part,
+                               size_t part_elems, hipStream_t s);
+// gemm_nn_f64 with Q upper triangular in column-major storage (Q(j, n) = Q[j + n ldq] = 0
+// for j > n): each column tile reduces over the rows up to its last column only.
+hipError_t gemm_nn_tri_f64(const cx<double>* A, int lda, int J, const cx<double>* Q, int ldq, int N, int K,
+                           cx<double>* out, int ldo, hipStream_t s);
 hipError_t mfma_selftest(int kind, double* max_err);
 
 // Grouped projector GEMMs over the systems (and species) of a batch (pwh_gemm.hip), one
