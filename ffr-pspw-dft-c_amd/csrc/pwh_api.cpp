@@ -224,6 +224,7 @@ struct pwh_system {
   // next filter runs on the library stream (fork/join events)
   hipStream_t eig_stream = nullptr;
   hipEvent_t ev_eig_fork = nullptr, ev_eig_done = nullptr;
+  rocblas_handle rb_eig = nullptr;   // rocSOLVER calls of the eigenvector half, on eig_stream
 };
 
 namespace {
@@ -745,6 +746,37 @@ bool small_eigensolver(int n) {
   return n <= kHeevSmallMax && (!e || !strcmp(e, "own"));
 }
 
+// Scratch of the mid-size path: the reduction's work, the reflectors, tau, d and e
+Buf<cx<double>>& trd_ws() {
+  static Buf<cx<double>>* b = new Buf<cx<double>>();
+  return *b;
+}
+Buf<double>& trd_de() {
+  static Buf<double>* b = new Buf<double>();
+  return *b;
+}
+
+// 128 < n <= trd_max_n(): the multi-workgroup Householder reduction and back-transformation
+// (pwh_trd.hip) around rocSOLVER's zstedc on the tridiagonal matrix.  A: Hermitian, upper
+// triangle authoritative.  (rocSOLVER's zunmtr returned non-finite vectors for reflectors
+// with tau = 0, as from a diagonal A; the back-transformation here skips those.)
+int hermitian_eig_mid(cx<double>* A, int n, double* w, rocblas_int* info) {
+  const size_t wk = trd_work_elems(n);
+  CHK(trd_ws().ensure(wk + n));
+  CHK(trd_de().ensure(2 * (size_t)n));
+  cx<double>* work = trd_ws().p;
+  cx<double>* tau = work + wk;
+  double* d = trd_de().p;
+  double* e = d + n;
+  HIPCHK(herm_fill_lower(A, n, n, g.stream));
+  HIPCHK(tridiag_householder(A, n, work, d, e, tau, g.stream));
+  RBCHK(rocsolver_zstedc(g.rb, rocblas_evect_tridiagonal, n, d, e, reinterpret_cast<rocblas_double_complex*>(A), n,
+                         info));
+  HIPCHK(tridiag_backtransform(work, tau, A, n, g.stream));
+  HIPCHK(hipMemcpyAsync(w, d, sizeof(double) * n, hipMemcpyDeviceToDevice, g.stream));
+  return PWH_OK;
+}
+
 int hermitian_eig_impl(cx<double>* A, int n, double* w, double* work, rocblas_int* info) {
   const int mode = [] {
     const char* e = getenv("PWH_EIGSOLVER");   // read per call: a caller may switch solvers
@@ -770,6 +802,7 @@ int hermitian_eig_impl(cx<double>* A, int n, double* w, double* work, rocblas_in
     RBCHK(rocsolver_zheevd(g.rb, rocblas_evect_original, rocblas_fill_upper, n, a, n, w, work, info));
     return PWH_OK;
   }
+  if (mode == 0 && n <= trd_max_n()) return hermitian_eig_mid(A, n, w, info);
   if (mode == 2) {
     RBCHK(rocsolver_zheevj(g.rb, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, n,
                            a, n, 0.0, work, 100, info + 1, w, info));
@@ -820,7 +853,14 @@ int eig_overlap_prepare(pwh_system* s) {
   HIPCHK(hipStreamCreateWithFlags(&s->eig_stream, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&s->ev_eig_fork, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&s->ev_eig_done, hipEventDisableTiming));
+  RBCHK(rocblas_create_handle(&s->rb_eig));
+  RBCHK(rocblas_set_stream(s->rb_eig, s->eig_stream));
   return PWH_OK;
+}
+
+bool mid_eigensolver(int n) {
+  const char* e = getenv("PWH_EIGSOLVER");
+  return n > kHeevSmallMax && n <= trd_max_n() && (!e || !strcmp(e, "own"));
 }
 
 bool chfsi_overlap_on(int nb) {
@@ -828,7 +868,49 @@ bool chfsi_overlap_on(int nb) {
     const char* e = getenv("PWH_CHFSI_OVERLAP");
     return !(e && e[0] == '0');
   }();
-  return on && small_eigensolver(nb);
+  return on && (small_eigensolver(nb) || mid_eigensolver(nb));
+}
+
+// The two halves of the mid-size solver: the values half reduces S to tridiagonal form
+// (mid_values_half) and, once the vectors half is launched, takes the eigenvalues by
+// multisection (mid_values_sterf: into evals; info[1] cleared); the vectors half, on the
+// third stream, runs zstedc on copies of the tridiagonal matrix into S (info into info[0])
+// and the back-transformation.  (rocSOLVER's dsterf took ~60 ms at n = 432.)
+int mid_values_half(pwh_system* s, int nb) {
+  const size_t wk = trd_work_elems(nb);
+  CHK(trd_ws().ensure(wk + nb));
+  CHK(trd_de().ensure(4 * (size_t)nb));
+  cx<double>* work = trd_ws().p;
+  cx<double>* tau = work + wk;
+  double* d = trd_de().p;
+  double* e = d + nb;
+  HIPCHK(tridiag_householder(s->S.p, nb, work, d, e, tau, g.stream));
+  double* d2 = e + nb;
+  double* e2 = d2 + nb;
+  HIPCHK(hipMemcpyAsync(s->evals.p, d, sizeof(double) * nb, hipMemcpyDeviceToDevice, g.stream));
+  HIPCHK(hipMemcpyAsync(e2, e, sizeof(double) * nb, hipMemcpyDeviceToDevice, g.stream));
+  HIPCHK(hipMemcpyAsync(d2, d, sizeof(double) * nb, hipMemcpyDeviceToDevice, g.stream));
+  return PWH_OK;
+}
+
+int mid_values_sterf(pwh_system* s, int nb) {
+  const double* d = trd_de().p;
+  const double* e2 = trd_de().p + 3 * (size_t)nb;
+  HIPCHK(tridiag_eigenvalues(d, e2, nb, s->evals.p, g.stream));
+  HIPCHK(hipMemsetAsync(s->info.p + 1, 0, sizeof(rocblas_int), g.stream));
+  return PWH_OK;
+}
+
+int mid_vectors_half(pwh_system* s, int nb) {
+  const size_t wk = trd_work_elems(nb);
+  cx<double>* work = trd_ws().p;
+  cx<double>* tau = work + wk;
+  double* d2 = trd_de().p + 2 * (size_t)nb;
+  double* e = trd_de().p + nb;
+  RBCHK(rocsolver_zstedc(s->rb_eig, rocblas_evect_tridiagonal, nb, d2, e,
+                         reinterpret_cast<rocblas_double_complex*>(s->S.p), nb, s->info.p));
+  HIPCHK(tridiag_backtransform(work, tau, s->S.p, nb, s->eig_stream));
+  return PWH_OK;
 }
 
 // First half (ChebySCFLoop.cpp:292-300): HX = H X, S = sym(X^H HX), the eigenvalues of S
@@ -853,13 +935,30 @@ int rr_begin(pwh_system* s, cx<double>* X, int nb, double* eig_host) {
   HIPCHK(symmetrize(s->S.p, nb, nb, g.stream));
   HIPCHK(hipMemcpyAsync(heev_save().p, s->S.p, sizeof(cx<double>) * nb * nb, hipMemcpyDeviceToDevice, g.stream));
   int* info = reinterpret_cast<int*>(s->info.p);
-  HIPCHK(launch_heev_small_values(s->S.p, nb, nb, s->evals.p, info, heev_ws().p, g.stream));
+  const bool mid = nb > kHeevSmallMax;
+  if (mid) CHK(mid_values_half(s, nb));
+  else HIPCHK(launch_heev_small_values(s->S.p, nb, nb, s->evals.p, info, heev_ws().p, g.stream));
   HIPCHK(hipEventRecord(s->ev_eig_fork, g.stream));
   HIPCHK(hipStreamWaitEvent(s->eig_stream, s->ev_eig_fork, 0));
-  HIPCHK(launch_heev_small_vectors(s->S.p, nb, nb, s->evals.p, info, heev_ws().p, s->eig_stream));
+  if (mid) CHK(mid_vectors_half(s, nb));
+  else HIPCHK(launch_heev_small_vectors(s->S.p, nb, nb, s->evals.p, info, heev_ws().p, s->eig_stream));
   HIPCHK(hipEventRecord(s->ev_eig_done, s->eig_stream));
+  if (mid) CHK(mid_values_sterf(s, nb));
   HIPCHK(hipMemcpyAsync(eig_host, s->evals.p, sizeof(double) * nb, hipMemcpyDeviceToHost, g.stream));
+  rocblas_int vinfo = 0;
+  if (mid) HIPCHK(hipMemcpyAsync(&vinfo, s->info.p + 1, sizeof(vinfo), hipMemcpyDeviceToHost, g.stream));
   HIPCHK(hipStreamSynchronize(g.stream));
+  if (vinfo != 0) {
+    // dsterf did not converge: join the vectors half and solve the saved matrix with zheevd
+    HIPCHK(hipStreamSynchronize(s->eig_stream));
+    HIPCHK(hipMemcpyAsync(s->S.p, heev_save().p, sizeof(cx<double>) * nb * nb, hipMemcpyDeviceToDevice, g.stream));
+    RBCHK(rocsolver_zheevd(g.rb, rocblas_evect_original, rocblas_fill_upper, nb,
+                           reinterpret_cast<rocblas_double_complex*>(s->S.p), nb, s->evals.p, s->rwork.p,
+                           s->info.p));
+    HIPCHK(hipMemcpyAsync(eig_host, s->evals.p, sizeof(double) * nb, hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipEventRecord(s->ev_eig_done, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+  }
   return PWH_OK;
 }
 
@@ -1273,6 +1372,7 @@ int pwh_system_destroy(pwh_system* s) {
   if (s->ev_fork) hipEventDestroy(s->ev_fork);
   if (s->ev_join) hipEventDestroy(s->ev_join);
   if (s->eig_stream) { hipStreamSynchronize(s->eig_stream); hipStreamDestroy(s->eig_stream); }
+  if (s->rb_eig) rocblas_destroy_handle(s->rb_eig);
   if (s->ev_eig_fork) hipEventDestroy(s->ev_eig_fork);
   if (s->ev_eig_done) hipEventDestroy(s->ev_eig_done);
   delete s;

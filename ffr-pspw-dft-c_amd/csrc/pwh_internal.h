@@ -133,6 +133,17 @@ constexpr int kCholSmallMax = 128;
 hipError_t launch_chol_inv_small(cx<double>* S, int lda, int n, int* info, hipStream_t s);
 hipError_t launch_heev_small(cx<double>* A, int lda, int n, double* w, int* info, double* scratch,
                              hipStream_t s);
+// Householder reduction of the Hermitian n x n matrix A (column-major, both triangles read)
+// to real tridiagonal T = Q^H A Q (pwh_trd.hip), 3 <= n <= trd_max_n(): d, e (n, n-1), tau
+// (n-1), the reflectors kept in work (trd_work_elems(n) complex values).  A is not
+// modified.  tridiag_backtransform: Y (n x n, column-major) <- Q Y with that work and tau.
+size_t trd_work_elems(int n);
+int trd_max_n();
+hipError_t tridiag_householder(const cx<double>* A, int n, cx<double>* work, double* d, double* e, cx<double>* tau,
+                               hipStream_t s);
+hipError_t tridiag_backtransform(const cx<double>* work, const cx<double>* tau, cx<double>* Y, int n, hipStream_t s);
+// the eigenvalues of the symmetric tridiagonal matrix (d, e), ascending, by multisection
+hipError_t tridiag_eigenvalues(const double* d, const double* e, int n, double* w, hipStream_t s);
 // Address, on the current device, of the solver scratch held in a device-resident array
 // (heev_small_scratch_doubles() doubles): no allocation, shared by every call on the device.
 hipError_t heev_small_static_scratch(double** ptr);

@@ -183,6 +183,37 @@ def test_eig_zheevd_hard_spectra(case):
     _check_eig(H, got, V, tol=1e-13)
 
 
+@pytest.mark.parametrize("n,case", [(129, "random"), (200, "4-fold"), (256, "diagonal"), (300, "near-diagonal"),
+                                    (384, "zero"), (512, "wide"), (640, "random")])
+def test_eig_zheevd_mid_sizes(n, case):
+    """128 < n: the multi-workgroup Householder reduction, then rocSOLVER's zstedc and
+    zunmtr.  Eigenvalues against numpy to 1e-12 of the largest entry, A V = V Lambda and
+    V^H V = I to 1e-13 n, on random, degenerate, diagonal, near-diagonal, zero and
+    Rayleigh-Ritz-like spectra."""
+    import pwh
+    Q = _unitary(n, n)
+    rng = np.random.default_rng(n + 1)
+    if case == "4-fold":
+        lam = np.repeat(np.linspace(-1, 2, n // 4), 4)
+    elif case == "wide":
+        lam = np.sort(np.concatenate([np.linspace(-0.3, 0.5, n - 100), rng.uniform(1, 60, 100)]))
+    elif case == "zero":
+        lam = np.zeros(n)
+    else:
+        lam = np.sort(rng.standard_normal(n))
+    if case == "diagonal":
+        H = np.diag(lam).astype(np.complex128)
+    elif case == "near-diagonal":
+        H = np.diag(np.linspace(-2, 3, n)) + 1e-9 * (Q + Q.conj().T)
+    else:
+        H = (Q * lam) @ Q.conj().T
+    H = 0.5 * (H + H.conj().T)
+    got, V = pwh.eig_zheevd(H)
+    s = max(np.abs(H).max(), 1.0)
+    assert np.abs(got - np.linalg.eigvalsh(H)).max() < 1e-12 * s
+    _check_eig(H, got, V, tol=1e-13)
+
+
 def test_shim_eig_zheevd_vectors_only(shim_lib):
     """eig_zheevd.cpp:37-69: the two-argument overload leaves the eigenvectors in A."""
     sym = "_Z10eig_zheevdPCdii"
