@@ -632,6 +632,13 @@ def c4_rates(steps, cpu=True):
         prof = kernel_profile(x, s.nbands, p, reps=3)
         out[p] = {"ms_per_block": round(1e3 * dt / steps, 3), "band_applies_per_s": round(s.nbands * steps / dt, 1),
                   "gemm_target_blocks": tuned[p], "passes": pass_rates(s, info, s.nbands, p, prof)}
+    # projector storage: the FP32 H|psi> with beta generated on the fly from the factorised
+    # phases (pwh_projector_mode 1) against the stored rows
+    x.projector_mode(1, s.A, s.tau)
+    dt = time_steps(lambda: x.resident_apply(s.nbands, "fp32"), steps, 1)
+    out["fp32_projectors_on_the_fly"] = {"ms_per_block": round(1e3 * dt / steps, 3),
+                                         "stored_ms_per_block": out["fp32"]["ms_per_block"]}
+    x.projector_mode(0)
     # ChFSI at degree 12 (FP64 orthonormalisation and Rayleigh-Ritz; rocSOLVER's Cholesky
     # and zheevd at n = 432): five outer iterations per filter precision from the same
     # orthonormal random start, 20-step FP64 Lanczos bounds.  The eigenvalue difference is

@@ -225,6 +225,14 @@ struct pwh_system {
   hipStream_t eig_stream = nullptr;
   hipEvent_t ev_eig_fork = nullptr, ev_eig_done = nullptr;
   rocblas_handle rb_eig = nullptr;   // rocSOLVER calls of the eigenvector half, on eig_stream
+  // projectors generated on the fly in the FP32 GEMMs (pwh_projector_mode): per-plane-wave
+  // packed table indices, PKG_A in sphere order [sum lm][ngw_pad], the FP32 phase tables
+  int beta_fly = 0, nax = 0, ldt = 0;
+  std::vector<int> pko;              // first PKG_A row of each species
+  Buf<int> mi;
+  Buf<float> pkg32;
+  Buf<cx<float>> ei32;
+  std::vector<float> pkg_host;       // kept from creation until the mode is set
 };
 
 namespace {
@@ -311,6 +319,17 @@ int hpsi_prepare(pwh_system* s, int nb) {
   return PWH_OK;
 }
 
+// the on-the-fly projector fields of species sp (FP32 GEMMs only read them)
+void beta_fly_fields(const pwh_system* s, int sp, BetaGen& bg) {
+  if (!s->beta_fly) return;
+  bg.fly = 1;
+  bg.ei = s->ei32.p;
+  bg.mi = s->mi.p;
+  bg.pkg = s->pkg32.p + (size_t)s->pko[sp] * s->ngw_pad;
+  bg.ldt = s->ldt;
+  bg.slot0 = sp * s->nax;
+}
+
 template <typename T>
 int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double beta,
          double gamma, const cx<T>* Z, int mode, hipStream_t st = nullptr) {
@@ -344,6 +363,7 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
     const size_t pe = proj_part_elems(M, nb, s->ngw, target);
     CHK(Sel<T>::part(s).ensure(pe));
     BetaGen bg{Sel<T>::Bt(s) + (size_t)s->poff[sp] * ld, M, s->lm[sp], ld, target};
+    beta_fly_fields(s, sp, bg);
     hipStream_t sp_st = sn;
     {
       // <beta|psi> scaled by WNL straight into Q by the split-K reduction
@@ -357,6 +377,7 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
     if (s->na[sp] == 0 || s->lm[sp] == 0) continue;
     const int M = s->na[sp] * s->lm[sp];
     BetaGen bg{Sel<T>::Bt(s) + (size_t)s->poff[sp] * ld, M, s->lm[sp], ld};
+    beta_fly_fields(s, sp, bg);
     {
       Timed t(TM_EXPAND, sn);
       HIPCHK(gemm_expand<T>(bg, Sel<T>::Q(s).p + (size_t)s->poff[sp] * nb, nb, s->ngw, Sel<T>::Hnl(s).p, ld,
@@ -1328,6 +1349,35 @@ int pwh_system_create(const pwh_system_desc* d, pwh_system** out) {
         }
       }
   for (int k = 0; k < ngw; ++k) kin[k] = 0.5 * d->xkg[s->perm[k]] * d->tpiba2;
+  // PKG_A in sphere order and the packed phase-table indices of each plane wave, for the
+  // projectors generated on the fly (pwh_projector_mode); the tables are built there
+  {
+    s->nax = d->nax;
+    s->pko.assign(s->nsp, 0);
+    int rows = 0;
+    for (int sp = 0; sp < s->nsp; ++sp) { s->pko[sp] = rows; rows += s->lm[sp]; }
+    s->pkg_host.assign((size_t)std::max(rows, 1) * ld, 0.f);
+    for (int sp = 0; sp < s->nsp; ++sp)
+      for (int l = 0; l < s->lm[sp]; ++l)
+        for (int k = 0; k < ngw; ++k)
+          s->pkg_host[(size_t)(s->pko[sp] + l) * ld + k] =
+              (float)d->pkg_a[l + (size_t)d->nlmax * (s->perm[k] + (size_t)d->ngwx * sp)];
+    s->ldt = std::max(nr1, std::max(nr2, d->nr3)) + 2;
+    std::vector<int> mi(ld, 0);
+    const int nrv[3] = {nr1, nr2, d->nr3};
+    for (int k = 0; k < ngw; ++k) {
+      unsigned g3 = d->n123[s->perm[k]] - 1;
+      int packed = 0;
+      for (int dd = 0; dd < 3; ++dd) {
+        const int i = (int)(g3 % (unsigned)nrv[dd]);
+        g3 /= (unsigned)nrv[dd];
+        const int m = 2 * i > nrv[dd] ? i - nrv[dd] : i;   // the signed Miller index
+        packed |= (m + (nrv[dd] + 1) / 2) << (10 * dd);
+      }
+      mi[k] = packed;
+    }
+    if ((rc = upload(s->mi, mi.data(), mi.size()))) return bail(rc);
+  }
   std::vector<cx<float>> Bt32(Bt.size());
   std::vector<float> kin32(kin.size()), wl32(s->wl.size());
   for (size_t i = 0; i < Bt.size(); ++i) Bt32[i] = mk<float>((float)Bt[i].x, (float)Bt[i].y);
@@ -2768,6 +2818,28 @@ int pseudo_check(const pwh_pseudo_desc* ps) {
 }  // namespace
 
 extern "C" {
+
+int pwh_projector_mode(pwh_system* s, int mode, const double A[9], const double* tau) {
+  if (!s || (mode != 0 && mode != 1)) return fail(PWH_ERR_ARG, "bad projector mode");
+  CHK(ensure_init());
+  if (mode == 0) {
+    s->beta_fly = 0;
+    return PWH_OK;
+  }
+  if (!A || !tau) return fail(PWH_ERR_ARG, "the on-the-fly projectors need the cell and the atoms");
+  if (s->ldt > 1024 + 2) return fail(PWH_ERR_UNSUPPORTED, "grid too large for the packed table indices");
+  EiTables t;
+  CHK(ei_build(t, A, s->nr, s->nsp, s->na.data(), s->nax, tau));
+  const size_t n = (size_t)s->nsp * s->nax * 3 * t.ldt;
+  if (t.ldt != s->ldt) return fail(PWH_ERR_ARG, "phase table length mismatch");
+  CHK(s->ei32.ensure(n));
+  HIPCHK((convert_cx<double, float>(t.ei.p, s->ei32.p, n, g.stream)));
+  CHK(upload(s->pkg32, s->pkg_host.data(), s->pkg_host.size()));
+  HIPCHK(hipStreamSynchronize(g.stream));
+  s->beta_fly = 1;
+  ++g.layout_gen;
+  return PWH_OK;
+}
 
 int pwh_phase_factor_gpu(const double A[9], const int nr[3], int nsp, const int* na, int nax, const double* tau,
                          int nkpt, const int* ngw, int ngwx, const int* igk, int ngx, const int* in123,

@@ -73,6 +73,26 @@ struct RowLoader {
   __device__ __forceinline__ cx<T> operator()(int m, int k) const { return A[(size_t)m * lda + k]; }
 };
 
+// beta[(a, l)][k] generated from the factorised phases (PhaseFactor.cpp:41-77 forms EIGR
+// as the same product) and PKG_A: three table reads, two complex products and a scale in
+// place of one read of the stored row
+template <>
+struct RowLoader<float, true> {
+  const cx<float>* A; int lda;          // unused (the stored rows)
+  const cx<float>* ei;
+  const int* mi;
+  const float* pkg;
+  int ldt, lm, ldp, slot0;
+  __device__ __forceinline__ cx<float> operator()(int m, int k) const {
+    const int a = m / lm, l = m - a * lm;
+    const int p = mi[k];
+    const cx<float>* t = ei + (size_t)(slot0 + a) * 3 * ldt;
+    const cx<float> e = (t[p & 1023] * t[ldt + ((p >> 10) & 1023)]) * t[2 * ldt + (p >> 20)];
+    const float s = pkg[(size_t)l * ldp + k];
+    return mk<float>(e.x * s, e.y * s);
+  }
+};
+
 // ---------------------------------------------------------------------------
 // proj: part[s][m][n] = sum_{k in chunk s} conj(A[m][k]) B[n][k]
 // ---------------------------------------------------------------------------
@@ -1041,7 +1061,18 @@ hipError_t gemm_proj(const BetaGen& bg, const cx<T>* X, int ldx, int nb, int K, 
     return hipGetLastError();
   }
   int S = 0;
-  hipError_t e = run_proj<T, false>(ld, X, ldx, M, nb, K, part, part_elems, mode, s, &S, bg.target);
+  hipError_t e;
+  if constexpr (std::is_same<T, float>::value) {
+    if (bg.fly && mode == PREC_FP32) {
+      RowLoader<float, true> lf{nullptr, 0, static_cast<const cx<float>*>(bg.ei), bg.mi, bg.pkg, bg.ldt, bg.lm,
+                                bg.ld, bg.slot0};
+      e = run_proj<float, true>(lf, X, ldx, M, nb, K, part, part_elems, mode, s, &S, bg.target);
+    } else {
+      e = run_proj<T, false>(ld, X, ldx, M, nb, K, part, part_elems, mode, s, &S, bg.target);
+    }
+  } else {
+    e = run_proj<T, false>(ld, X, ldx, M, nb, K, part, part_elems, mode, s, &S, bg.target);
+  }
   if (e != hipSuccess) return e;
   const int tot = M * nb;
   hipLaunchKernelGGL((k_reduce_parts<T, T>), dim3((tot + 255) / 256), dim3(256), 0, s, part, S, M,
@@ -1067,6 +1098,11 @@ hipError_t gemm_expand(const BetaGen& bg, const cx<T>* Q, int nb, int K, cx<T>* 
     const int nsplit = bf_terms(mode);
     if (nsplit == 3) return launch_expand_bf<3>(grid, s, ld.A, ld.lda, Q, nb, J, nb, K, out, ldo, acc);
     if (nsplit == 2) return launch_expand_bf<2>(grid, s, ld.A, ld.lda, Q, nb, J, nb, K, out, ldo, acc);
+    if (bg.fly && mode == PREC_FP32) {
+      RowLoader<float, true> lf{nullptr, 0, static_cast<const cx<float>*>(bg.ei), bg.mi, bg.pkg, bg.ldt, bg.lm,
+                                bg.ld, bg.slot0};
+      return launch_expand<float, true, false>(gemm_ks<float>(), grid, s, lf, Q, nb, J, nb, K, out, ldo, acc);
+    }
   }
   return launch_expand<T, false, false>(gemm_ks<T>(), grid, s, ld, Q, nb, J, nb, K, out, ldo, acc);
 }

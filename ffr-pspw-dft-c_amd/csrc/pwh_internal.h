@@ -225,6 +225,15 @@ struct BetaGen {
   const void* B;   // cx<T>
   int rows, lm, ld;
   int target = 0;  // split-K block target of the projection (0: the library default)
+  // FP32 projections and expansions with beta generated on the fly from factorised phases
+  // instead of read from B (fly != 0): EIGR(k, a) = EI1(m1) EI2(m2) EI3(m3) of atom slot
+  // slot0 + a, tables ei [slot][3][ldt] (cx<float>); mi[k]: the three table indices of
+  // plane wave k packed 10 bits each; pkg [lm][ld] (float) of the species
+  int fly = 0;
+  const void* ei = nullptr;
+  const int* mi = nullptr;
+  const float* pkg = nullptr;
+  int ldt = 0, slot0 = 0;
 };
 // C[(a,l)][b] = wl[l] * <beta_(a,l)|X_b> (wl null: unscaled), row-major [rows][nb]
 template <typename T>

@@ -46,7 +46,7 @@ EXPORTS = [
     "pwh_kerker_prec", "pwh_mix_anderson", "pwh_mix_pulay", "pwh_mix_broyden", "pwh_eval_enl",
     "pwh_phase_factor_gpu", "pwh_structure_factor", "pwh_nl_form_factor", "pwh_form_factor",
     "pwh_resident_chfsi", "pwh_resident_eval_rho_psi", "pwh_resident_eval_enl",
-    "pwh_eig_zheevd_noalloc", "pwh_fft_path",
+    "pwh_eig_zheevd_noalloc", "pwh_fft_path", "pwh_projector_mode",
 ]
 
 
@@ -152,6 +152,7 @@ def lib():
         "pwh_resident_get": [vp, vp, C.c_int],
         "pwh_resident_apply": [vp, C.c_int, C.c_int],
         "pwh_autotune": [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)],
+        "pwh_projector_mode": [vp, C.c_int, C.c_void_p, C.c_void_p],
         "pwh_resident_chebyfilt": [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int],
         "pwh_synchronize": [],
         "pwh_timing_enable": [C.c_int],
@@ -601,6 +602,16 @@ class System:
 
     def resident_apply(self, nstates, precision=FP64):
         _check(lib().pwh_resident_apply(self.h, int(nstates), _prec(precision)))
+
+    def projector_mode(self, mode, A=None, tau=None):
+        """pwh_projector_mode: 0 stored projector rows, 1 generated on the fly in the FP32
+        GEMMs from the factorised phases (A: lattice, tau: cartesian positions)."""
+        if mode == 0:
+            _check(lib().pwh_projector_mode(self.h, 0, None, None))
+        else:
+            a = _lat(A)
+            tp = np.ascontiguousarray(np.asarray(tau, np.float64).reshape(-1))
+            _check(lib().pwh_projector_mode(self.h, int(mode), a.ctypes.data, tp.ctypes.data))
 
     def autotune(self, nstates, precision=FP64, target=0):
         """Picks the projection GEMMs' split-K block target for H|psi> of the first nstates

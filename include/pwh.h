@@ -66,6 +66,14 @@ int pwh_phase_factor(const double A[9], const int nr[3], int nsp, const int *na,
                      const double *tau, int nkpt, const int *ngw, int ngwx, const int *igk,
                      int ngx, const int *in123, double *eigr /* complex */);
 
+/* No counterpart in the reference (it stores EIGR and forms the projector rows per call,
+ * ApplyHam_block.cpp:88-95): where the projectors of the FP32 H|psi> GEMMs come from.
+ * mode 0 (default): the beta rows stored at system creation; mode 1: generated on the fly
+ * inside the GEMMs from the factorised phases EI1 EI2 EI3 (PhaseFactor.cpp:41-77, built
+ * from A and tau as pwh_phase_factor_gpu does: tau [3*nax*nsp] cartesian) and PKG_A.
+ * FP64 and BF16 GEMMs, ortho and the grouped batch filter keep the stored rows. */
+int pwh_projector_mode(pwh_system *sys, int mode, const double A[9], const double *tau);
+
 /* ---- set-up on the GPU: phases, structure factor, form factors (FP64) ------------ */
 /* PhaseFactor.cpp:5-80 computed on the GPU: same arguments and output as
  * pwh_phase_factor (agrees to rounding: the device sin/cos differ from libm by an ulp). */

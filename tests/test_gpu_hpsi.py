@@ -151,3 +151,29 @@ def test_autotune_targets_keep_results(oracle):
             g.autotune(**bad)
         assert e.value.code == pwh.PWH_ERR_ARG
     g.close()
+
+
+def test_projectors_on_the_fly_match_stored(si8, oracle):
+    """pwh_projector_mode 1: the FP32 GEMMs generate beta from the factorised phases and
+    PKG_A; H|psi> agrees with the stored-projector result to FP32 rounding and with the
+    oracle to the FP32 bound; FP64 keeps the stored rows (bitwise unchanged); mode 0
+    restores the stored path bit for bit."""
+    import pwh
+    pwh.init(0)
+    g = pwh.System(si8.globals())
+    g.set_vloc(si8.vloc)
+    psi = si8.psi(16, seed=21)
+    ref = oracle.apply_ham_block(oracle_globals(si8), psi, si8.vloc, si8.ngw)
+    st32 = g.apply_ham_block(psi, "fp32")
+    st64 = g.apply_ham_block(psi, "fp64")
+    g.projector_mode(1, si8.A, si8.tau)
+    fly32 = g.apply_ham_block(psi, "fp32")
+    assert band_relerr(fly32, st32) < 1e-6
+    assert band_relerr(fly32, ref) < 2e-6
+    assert np.array_equal(g.apply_ham_block(psi, "fp64"), st64)
+    g.projector_mode(0)
+    assert np.array_equal(g.apply_ham_block(psi, "fp32"), st32)
+    with pytest.raises(pwh.PwhError) as e:
+        g.projector_mode(2, si8.A, si8.tau)
+    assert e.value.code == pwh.PWH_ERR_ARG
+    g.close()
