@@ -631,7 +631,7 @@ def c4_rates(steps, cpu=True):
         dt = time_steps(lambda: x.resident_apply(s.nbands, p), steps, 1)
         prof = kernel_profile(x, s.nbands, p, reps=3)
         out[p] = {"ms_per_block": round(1e3 * dt / steps, 3), "band_applies_per_s": round(s.nbands * steps / dt, 1),
-                  "gemm_target_blocks": tuned[p], "passes": pass_rates(s, info, s.nbands, p, prof)}
+                  "tuning": x.tuning(p), "passes": pass_rates(s, info, s.nbands, p, prof)}
     # projector storage: the FP32 H|psi> with beta generated on the fly from the factorised
     # phases (pwh_projector_mode 1) against the stored rows
     x.projector_mode(1, s.A, s.tau)
@@ -723,7 +723,8 @@ def main():
             tuned[p] = sysm.autotune(nb, p, target=fixed[p])[0]
         else:
             tuned[p] = 512 if args.no_tune else sysm.autotune(nb, p)[0]
-    log("bench: gemm split-K block targets " + json.dumps(tuned))
+    tuning = {p: sysm.tuning(p) for p in paths}
+    log("bench: tuned settings " + json.dumps(tuning))
     rates, ms_step = {}, {}
     for p in paths:
         dt = time_steps(lambda: sysm.resident_apply(nb, p), args.steps, args.warmup)
@@ -887,6 +888,7 @@ def main():
         "band_applies_per_s": {k: round(v, 1) for k, v in rates.items()},
         "ms_per_block": {k: round(v, 4) for k, v in ms_step.items()},
         "gemm_target_blocks": tuned,
+        "tuning": tuning,
         "mixed_vs_fp64": round(rates.get("fp32", 0) / rates["fp64"], 3) if "fp64" in rates and "fp32" in rates else None,
         "roofline": roof,
         "kernels": per_kernel,

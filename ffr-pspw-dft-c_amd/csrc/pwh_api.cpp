@@ -217,6 +217,10 @@ struct pwh_system {
   // split-K block target of the projections per precision (PREC_FP64/FP32/BF16X3/BF16X6; 0: the
   // library default), set by pwh_autotune
   int tune[4] = {0, 0, 0, 0};
+  // LDS stage depth of the projection [p][0] and expansion [p][1] GEMMs per precision (0:
+  // the library default), and the radix-order variant of the runtime FFT plans
+  int gks[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  int fftv = 0;
   // second stream for the nonlocal GEMMs, which overlap the FFT passes (fork/join events)
   hipStream_t nl_stream = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -364,6 +368,7 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
     CHK(Sel<T>::part(s).ensure(pe));
     BetaGen bg{Sel<T>::Bt(s) + (size_t)s->poff[sp] * ld, M, s->lm[sp], ld, target};
     beta_fly_fields(s, sp, bg);
+    bg.ks = s->gks[std::is_same<T, double>::value ? PREC_FP64 : mode][0];
     hipStream_t sp_st = sn;
     {
       // <beta|psi> scaled by WNL straight into Q by the split-K reduction
@@ -378,6 +383,7 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
     const int M = s->na[sp] * s->lm[sp];
     BetaGen bg{Sel<T>::Bt(s) + (size_t)s->poff[sp] * ld, M, s->lm[sp], ld};
     beta_fly_fields(s, sp, bg);
+    bg.ks = s->gks[std::is_same<T, double>::value ? PREC_FP64 : mode][1];
     {
       Timed t(TM_EXPAND, sn);
       HIPCHK(gemm_expand<T>(bg, Sel<T>::Q(s).p + (size_t)s->poff[sp] * nb, nb, s->ngw, Sel<T>::Hnl(s).p, ld,
@@ -3051,52 +3057,99 @@ int pwh_autotune(pwh_system* s, int nstates, int precision, int* target_blocks, 
   CHK(check_ld(s, nstates));
   if (nstates > s->resident_nb) return fail(PWH_ERR_ARG, "more states than resident");
   if (!s->vset) return fail(PWH_ERR_ARG, "local potential not set (pwh_set_vloc)");
-  // split-K block targets of the projection GEMMs: their best value moves with the band
-  // count, the projector count and the precision (one to eight blocks per CU)
-  static const int cand[] = {256, 512, 1024, 2048};
-  const int nc = (int)(sizeof(cand) / sizeof(cand[0]));
   const bool timing = g.timing;
   g.timing = false;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = PWH_OK;
-  double ms[4] = {1e300, 1e300, 1e300, 1e300};
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     rc = fail(PWH_ERR_HIP, "hipEventCreate");
-  // two passes over the candidates, the smaller time of each kept, so a slow box moment
-  // does not pick the target
-  for (int pass = 0; pass < 2 && rc == PWH_OK; ++pass) {
-    for (int c = 0; c < nc && rc == PWH_OK; ++c) {
-      s->tune[precision] = cand[c];
+  // ms per H|psi> of the current settings: 2 warm calls, then 5 timed, the smaller of
+  // two such passes (a slow box moment does not decide)
+  auto time_apply = [&](double* out) -> int {
+    double best = 1e300;
+    for (int pass = 0; pass < 2; ++pass) {
       const int reps = 5;
-      for (int r = 0; r < 2 && rc == PWH_OK; ++r) rc = pwh_resident_apply(s, nstates, precision);
-      if (rc != PWH_OK) break;
-      if (hipEventRecord(e0, g.stream) != hipSuccess) { rc = fail(PWH_ERR_HIP, "hipEventRecord"); break; }
-      for (int r = 0; r < reps && rc == PWH_OK; ++r) rc = pwh_resident_apply(s, nstates, precision);
-      if (rc != PWH_OK) break;
+      for (int r = 0; r < 2; ++r) CHK(pwh_resident_apply(s, nstates, precision));
+      if (hipEventRecord(e0, g.stream) != hipSuccess) return fail(PWH_ERR_HIP, "hipEventRecord");
+      for (int r = 0; r < reps; ++r) CHK(pwh_resident_apply(s, nstates, precision));
       float el = 0.f;
       if (hipEventRecord(e1, g.stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-          hipEventElapsedTime(&el, e0, e1) != hipSuccess) {
-        rc = fail(PWH_ERR_HIP, "event timing");
-        break;
-      }
-      ms[c] = std::min(ms[c], (double)el / reps);
+          hipEventElapsedTime(&el, e0, e1) != hipSuccess)
+        return fail(PWH_ERR_HIP, "event timing");
+      best = std::min(best, (double)el / reps);
     }
+    *out = best;
+    return PWH_OK;
+  };
+  // one knob over its candidates; the default (index 0 of cand) stays unless another is
+  // faster by more than 1 %; every change bumps the layout generation
+  auto search = [&](int* knob, const std::vector<int>& cand, double* ms_best) -> int {
+    std::vector<double> ms(cand.size(), 1e300);
+    for (size_t c = 0; c < cand.size(); ++c) {
+      *knob = cand[c];
+      ++g.layout_gen;
+      CHK(time_apply(&ms[c]));
+    }
+    size_t best = 0;
+    for (size_t c = 1; c < cand.size(); ++c)
+      if (ms[c] < 0.99 * ms[best]) best = c;
+    *knob = cand[best];
+    ++g.layout_gen;
+    *ms_best = ms[best];
+    return PWH_OK;
+  };
+  double ms = 0.0;
+  // 1. split-K block target of the projections (one to eight blocks per CU): its best
+  //    value moves with the band count, the projector count and the precision
+  if (rc == PWH_OK) rc = search(&s->tune[precision], {512, 256, 1024, 2048}, &ms);
+  // 2. the LDS stage depth of the projection and of the expansion GEMM (FP64 / FP32;
+  //    the BF16 splits have one)
+  if (rc == PWH_OK && precision <= PWH_FP32) {
+    const std::vector<int> ks = precision == PWH_FP32 ? std::vector<int>{16, 32, 64} : std::vector<int>{16, 32};
+    for (int kind = 0; kind < 2 && rc == PWH_OK; ++kind) rc = search(&s->gks[precision][kind], ks, &ms);
+  }
+  // 3. the radix order of the runtime FFT plans, for grids off the compiled kernels
+  if (rc == PWH_OK && s->maps.path != FFT_COMPILED) {
+    int v = s->fftv;
+    std::vector<double> vms(3, 1e300);
+    for (int c = 0; c < 3 && rc == PWH_OK; ++c) {
+      FftPlan p[3];
+      bool ok = true;
+      for (int d = 0; d < 3; ++d) ok = ok && fft_plan_variant(s->nr[d], c, &p[d]);
+      if (!ok) continue;
+      for (int d = 0; d < 3; ++d) s->maps.plan[d] = p[d];
+      ++g.layout_gen;
+      rc = time_apply(&vms[c]);
+    }
+    int best = 0;
+    for (int c = 1; c < 3; ++c)
+      if (vms[c] < 0.99 * vms[best]) best = c;
+    v = best;
+    for (int d = 0; d < 3; ++d) fft_plan_variant(s->nr[d], v, &s->maps.plan[d]);
+    s->fftv = v;
+    ++g.layout_gen;
+    if (rc == PWH_OK) ms = vms[best];
   }
   if (e0) hipEventDestroy(e0);
   if (e1) hipEventDestroy(e1);
   g.timing = timing;
   if (rc != PWH_OK) {
     s->tune[precision] = 0;
+    s->gks[precision][0] = s->gks[precision][1] = 0;
     return rc;
   }
-  // the library default (512) unless another target is faster by more than 1%
-  int best = 1;
-  for (int c = 0; c < nc; ++c)
-    if (ms[c] < 0.99 * ms[best]) best = c;
-  s->tune[precision] = cand[best];
-  ++g.layout_gen;
-  if (target_blocks) *target_blocks = cand[best];
-  if (best_ms) *best_ms = ms[best];
+  if (target_blocks) *target_blocks = s->tune[precision];
+  if (best_ms) *best_ms = ms;
+  return PWH_OK;
+}
+
+int pwh_tuning_get(pwh_system* s, int precision, int* out) {
+  if (!s || !out || precision < PWH_FP64 || precision > PWH_BF16X6) return fail(PWH_ERR_ARG, "bad tuning query");
+  out[0] = s->tune[precision] ? s->tune[precision] : 512;
+  const int dflt = precision >= PWH_BF16X3 ? 32 : 16;   // the BF16 split kernels stage 32 deep
+  out[1] = s->gks[precision][0] ? s->gks[precision][0] : dflt;
+  out[2] = s->gks[precision][1] ? s->gks[precision][1] : dflt;
+  out[3] = s->maps.path == FFT_COMPILED ? -1 : s->fftv;
   return PWH_OK;
 }
 
@@ -3184,6 +3237,9 @@ int group_hpsi(pwh_batch* b, GroupBuilder& gb, const std::vector<const cx<T>*>& 
   for (int i = 0; i < n; ++i) {
     pwh_system* s = b->sys[i];
     if (!s->vset) return fail(PWH_ERR_ARG, "local potential not set (pwh_set_vloc)");
+    for (int kind = 0; kind < 2; ++kind)
+      if (s->gks[prec][kind] != 0 && s->gks[prec][kind] != 16)
+        return fail(PWH_ERR_UNSUPPORTED, "grouped filter: a GEMM stage depth tuned away from 16");
     const int nb = b->nb[i], ld = s->ngw_pad;
     for (int sp = 0; sp < s->nsp; ++sp) {
       if (s->na[sp] == 0 || s->lm[sp] == 0) continue;

@@ -17,6 +17,8 @@
 //     i1 rows with V between (k_xsplit_rt), and back (k_ysplit_rt), in chunks of bands;
 //   * the density pass (k_plane_rho_rt, or k_ysplit_rt + k_xrho_split) and dense lines
 //     (k_lines_rt) for pwh_fft3d and the density grid.
+#include <utility>
+
 #include "pwh_dft.h"
 
 namespace pwh {
@@ -38,6 +40,35 @@ bool fft_plan(int n, FftPlan* p) {
     else if (rem % 7 == 0) r = 7;
     else {
       r = 11;                      // rem has no factor 2, 3, 5 or 7 left
+      while (rem % r != 0) r += 2;
+    }
+    if (p->nst >= kFftMaxStages) return false;
+    p->radix[p->nst++] = r;
+    rem /= r;
+  }
+  return true;
+}
+
+bool fft_plan_variant(int n, int variant, FftPlan* p) {
+  if (variant == 0 || variant == 2) {
+    if (!fft_plan(n, p)) return false;
+    if (variant == 2)
+      for (int i = 0, j = p->nst - 1; i < j; ++i, --j) std::swap(p->radix[i], p->radix[j]);
+    return true;
+  }
+  if (variant != 1 || !p || n < 2 || n > kFftMaxLen) return false;
+  p->n = n;
+  p->nst = 0;
+  int rem = n;
+  while (rem > 1) {
+    int r;
+    if (rem % 4 == 0) r = 4;
+    else if (rem % 2 == 0) r = 2;
+    else if (rem % 3 == 0) r = 3;
+    else if (rem % 5 == 0) r = 5;
+    else if (rem % 7 == 0) r = 7;
+    else {
+      r = 11;
       while (rem % r != 0) r += 2;
     }
     if (p->nst >= kFftMaxStages) return false;

@@ -1030,9 +1030,9 @@ template hipError_t launch_group_expand<float>(const GroupExpand<float>*, int, i
 template <typename T, bool BETA>
 static hipError_t run_proj(RowLoader<T, BETA> ld, const cx<T>* B, int ldb, int M, int N, int K,
                            cx<T>* part, size_t part_elems, int mode, hipStream_t s, int* S_out,
-                           int target = 0) {
+                           int target = 0, int ks_set = 0) {
   const int nsplit = std::is_same<T, float>::value ? bf_terms(mode) : 0;
-  const int ks = nsplit ? BKS : gemm_ks<T>();
+  const int ks = nsplit ? BKS : (ks_set ? ks_set : gemm_ks<T>());
   int S, kchunk, blocks;
   split_shape(M, N, K, part_elems, target, ks, &S, &kchunk, &blocks);
   if ((size_t)S * M * N > part_elems) return hipErrorInvalidValue;
@@ -1066,12 +1066,12 @@ hipError_t gemm_proj(const BetaGen& bg, const cx<T>* X, int ldx, int nb, int K, 
     if (bg.fly && mode == PREC_FP32) {
       RowLoader<float, true> lf{nullptr, 0, static_cast<const cx<float>*>(bg.ei), bg.mi, bg.pkg, bg.ldt, bg.lm,
                                 bg.ld, bg.slot0};
-      e = run_proj<float, true>(lf, X, ldx, M, nb, K, part, part_elems, mode, s, &S, bg.target);
+      e = run_proj<float, true>(lf, X, ldx, M, nb, K, part, part_elems, mode, s, &S, bg.target, bg.ks);
     } else {
-      e = run_proj<T, false>(ld, X, ldx, M, nb, K, part, part_elems, mode, s, &S, bg.target);
+      e = run_proj<T, false>(ld, X, ldx, M, nb, K, part, part_elems, mode, s, &S, bg.target, bg.ks);
     }
   } else {
-    e = run_proj<T, false>(ld, X, ldx, M, nb, K, part, part_elems, mode, s, &S, bg.target);
+    e = run_proj<T, false>(ld, X, ldx, M, nb, K, part, part_elems, mode, s, &S, bg.target, bg.ks);
   }
   if (e != hipSuccess) return e;
   const int tot = M * nb;
@@ -1101,10 +1101,11 @@ hipError_t gemm_expand(const BetaGen& bg, const cx<T>* Q, int nb, int K, cx<T>* 
     if (bg.fly && mode == PREC_FP32) {
       RowLoader<float, true> lf{nullptr, 0, static_cast<const cx<float>*>(bg.ei), bg.mi, bg.pkg, bg.ldt, bg.lm,
                                 bg.ld, bg.slot0};
-      return launch_expand<float, true, false>(gemm_ks<float>(), grid, s, lf, Q, nb, J, nb, K, out, ldo, acc);
+      return launch_expand<float, true, false>(bg.ks ? bg.ks : gemm_ks<float>(), grid, s, lf, Q, nb, J, nb, K, out,
+                                               ldo, acc);
     }
   }
-  return launch_expand<T, false, false>(gemm_ks<T>(), grid, s, ld, Q, nb, J, nb, K, out, ldo, acc);
+  return launch_expand<T, false, false>(bg.ks ? bg.ks : gemm_ks<T>(), grid, s, ld, Q, nb, J, nb, K, out, ldo, acc);
 }
 
 hipError_t gemm_cn_f64(const cx<double>* A, int lda, int M, const cx<double>* B, int ldb, int N,

@@ -59,6 +59,9 @@ struct FftPlan {
 // Factorises n (1 <= n <= kFftMaxLen): radix 8 while 8 divides the rest (4, 4 for 16), then
 // 4, 2, 3, 5, 7 and the remaining primes in increasing order.  False when n is out of range.
 bool fft_plan(int n, FftPlan* p);
+// Alternative radix orders for the tuner: 0 as fft_plan; 1 radix 4 preferred over 8; 2 the
+// stages of order 0 reversed.
+bool fft_plan_variant(int n, int variant, FftPlan* p);
 
 // Which implementation the FFT passes of a system use.
 enum FftPath {
@@ -229,6 +232,7 @@ struct BetaGen {
   // instead of read from B (fly != 0): EIGR(k, a) = EI1(m1) EI2(m2) EI3(m3) of atom slot
   // slot0 + a, tables ei [slot][3][ldt] (cx<float>); mi[k]: the three table indices of
   // plane wave k packed 10 bits each; pkg [lm][ld] (float) of the species
+  int ks = 0;       // LDS stage depth of the GEMM (16, 32, or 64 for FP32; 0: the library default)
   int fly = 0;
   const void* ei = nullptr;
   const int* mi = nullptr;

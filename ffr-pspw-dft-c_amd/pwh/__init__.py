@@ -46,7 +46,7 @@ EXPORTS = [
     "pwh_kerker_prec", "pwh_mix_anderson", "pwh_mix_pulay", "pwh_mix_broyden", "pwh_eval_enl",
     "pwh_phase_factor_gpu", "pwh_structure_factor", "pwh_nl_form_factor", "pwh_form_factor",
     "pwh_resident_chfsi", "pwh_resident_eval_rho_psi", "pwh_resident_eval_enl",
-    "pwh_eig_zheevd_noalloc", "pwh_fft_path", "pwh_projector_mode",
+    "pwh_eig_zheevd_noalloc", "pwh_fft_path", "pwh_projector_mode", "pwh_tuning_get",
 ]
 
 
@@ -153,6 +153,7 @@ def lib():
         "pwh_resident_apply": [vp, C.c_int, C.c_int],
         "pwh_autotune": [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)],
         "pwh_projector_mode": [vp, C.c_int, C.c_void_p, C.c_void_p],
+        "pwh_tuning_get": [vp, C.c_int, C.c_void_p],
         "pwh_resident_chebyfilt": [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int],
         "pwh_synchronize": [],
         "pwh_timing_enable": [C.c_int],
@@ -613,10 +614,18 @@ class System:
             tp = np.ascontiguousarray(np.asarray(tau, np.float64).reshape(-1))
             _check(lib().pwh_projector_mode(self.h, int(mode), a.ctypes.data, tp.ctypes.data))
 
+    def tuning(self, precision=FP64):
+        """pwh_tuning_get: the settings in use for a precision."""
+        out = np.zeros(4, np.int32)
+        _check(lib().pwh_tuning_get(self.h, _prec(precision), out.ctypes.data))
+        return {"target_blocks": int(out[0]), "proj_stage_depth": int(out[1]), "expand_stage_depth": int(out[2]),
+                "fft_plan_variant": int(out[3])}
+
     def autotune(self, nstates, precision=FP64, target=0):
-        """Picks the projection GEMMs' split-K block target for H|psi> of the first nstates
-        resident states in this precision (pwh_autotune); returns (target, ms per H|psi>).
-        target > 0 sets that target without timing."""
+        """Tunes H|psi> of the first nstates resident states in this precision (pwh_autotune:
+        the projections' split-K block target, the GEMMs' LDS stage depths, the runtime FFT
+        plans' radix order); returns (target, ms per H|psi>).  target > 0 sets that target
+        without timing."""
         t = C.c_int(int(target))
         ms = C.c_double(0.0)
         _check(lib().pwh_autotune(self.h, int(nstates), _prec(precision), C.byref(t), C.byref(ms)))

@@ -315,16 +315,24 @@ int pwh_resident_set(pwh_system *sys, const double *X, int nstates);
 int pwh_resident_get(pwh_system *sys, double *X, int nstates);
 /* One H|psi> over the resident block (nb bands) in the given precision. */
 int pwh_resident_apply(pwh_system *sys, int nstates, int precision);
-/* Split-K block target of the projection GEMMs for this system and precision.  With
- * *target_blocks == 0 (or a null pointer) it times H|psi> of the resident block (nstates
- * bands) for targets 256, 512, 1024 and 2048 and keeps the fastest (512, the default,
- * unless another is faster by more than 1%), returning it in *target_blocks and its time
- * per H|psi> in *best_ms (both optional).  With *target_blocks in [64, 65536] it sets that
- * target without timing (nstates unused), e.g. one found by an earlier run.  Every later
- * H|psi> of the system in that precision uses the target.  No counterpart in the
- * reference: it tunes a launch shape of this library (the reference's GEMMs are cuBLAS
+/* Launch shapes of H|psi> for this system and precision.  With *target_blocks == 0 (or a
+ * null pointer) it times H|psi> of the resident block (nstates bands) and keeps, knob by
+ * knob, the fastest setting (the default unless another is faster by more than 1%): the
+ * projections' split-K block target (256, 512, 1024, 2048; default 512), the LDS stage depth
+ * of the projection and of the expansion GEMM (16, 32, and 64 for FP32; default 16; FP64 and
+ * FP32 only), and, for grids off the compiled kernels, the radix order of the runtime FFT
+ * plans (pwh_tuning_get reports all of them).  It returns the target in *target_blocks and
+ * the time per H|psi> in *best_ms (both optional).  With *target_blocks in [64, 65536] it
+ * sets that target without timing (nstates unused), e.g. one found by an earlier run.
+ * Every later H|psi> of the system in that precision uses the settings.  No counterpart in
+ * the reference: it tunes launch shapes of this library (the reference's GEMMs are cuBLAS
  * calls, ApplyHam.cpp). */
 int pwh_autotune(pwh_system *sys, int nstates, int precision, int *target_blocks, double *best_ms);
+/* No counterpart in the reference: the settings pwh_autotune chose (or the defaults) for a
+ * precision: out[0] the projections' split-K block target, out[1] / out[2] the LDS stage
+ * depth of the projection / expansion GEMMs, out[3] the radix-order variant of the runtime
+ * FFT plans (-1: the grid runs on the compiled per-length kernels). */
+int pwh_tuning_get(pwh_system *sys, int precision, int out[4]);
 /* Chebyshev filter of the resident block (FP32 modes convert in and out once). */
 int pwh_resident_chebyfilt(pwh_system *sys, int nstates, int degree, double lb, double ub,
                            int precision);

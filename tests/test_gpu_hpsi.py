@@ -177,3 +177,31 @@ def test_projectors_on_the_fly_match_stored(si8, oracle):
         g.projector_mode(2, si8.A, si8.tau)
     assert e.value.code == pwh.PWH_ERR_ARG
     g.close()
+
+
+def test_autotune_full_search_keeps_results(si8, oracle):
+    """pwh_autotune searches the split-K target, both GEMMs' LDS stage depths and (on the
+    runtime FFT path) the plans' radix order; whatever it keeps, H|psi> still matches the
+    oracle (FP64 1e-12, FP32 2e-6), and pwh_tuning_get reports settings from the
+    candidate sets."""
+    import pwh
+    pwh.init(0)
+    g = pwh.System(si8.globals())
+    g.set_vloc(si8.vloc)
+    psi = si8.psi(16, seed=22)
+    ref = oracle.apply_ham_block(oracle_globals(si8), psi, si8.vloc, si8.ngw)
+    g.resident_set(psi)
+    assert g.tuning("fp64") == {"target_blocks": 512, "proj_stage_depth": 16, "expand_stage_depth": 16,
+                                "fft_plan_variant": -1}
+    assert g.fft_path("runtime") == "runtime"
+    for p, tol, depths in (("fp64", 1e-12, (16, 32)), ("fp32", 2e-6, (16, 32, 64))):
+        t, ms = g.autotune(16, p)
+        tu = g.tuning(p)
+        assert tu["target_blocks"] == t and t in (256, 512, 1024, 2048) and ms > 0
+        assert tu["proj_stage_depth"] in depths and tu["expand_stage_depth"] in depths
+        assert tu["fft_plan_variant"] in (0, 1, 2)
+        assert band_relerr(g.apply_ham_block(psi, p), ref) < tol, (p, tu)
+    assert g.fft_path("auto") == "compiled"
+    assert g.tuning("fp64")["fft_plan_variant"] == -1
+    assert band_relerr(g.apply_ham_block(psi, "fp64"), ref) < 1e-12
+    g.close()
