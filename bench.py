@@ -683,6 +683,9 @@ def main():
     ap.add_argument("--targets", default="",
                     help="p=N[,p=N]: set these split-K targets without the search (e.g. a profiled "
                          "run repeating a tuned one, fp64=2048)")
+    ap.add_argument("--section", default="", choices=["", "c3", "c4"],
+                    help="run one section only (the C3 batch or the C4 Si216 rates and ChFSI) and print "
+                         "its JSON object (profiling)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -700,6 +703,12 @@ def main():
               f"(multi-GPU sharding is described in DESIGN.md, not implemented).", file=sys.stderr)
         sys.exit(2)
 
+    if args.section:
+        import pwh
+        pwh.init(0)
+        sec = c3_batch(steps=5) if args.section == "c3" else c4_rates(steps=5, cpu=not args.no_cpu)
+        print(json.dumps({args.section: sec}), flush=True)
+        return
     hostcpu = cpu_info()          # before the GPU is initialised (lscpu runs as a child)
     import pwh
     from pwh import synth
