@@ -182,7 +182,7 @@ struct pwh_system {
   double tpiba2 = 0;
   SphereMaps maps{};
   std::vector<int> perm;
-  Buf<int> d_perm, d_colstart, d_kcol, d_kz, d_colxy, d_xa, d_colxa, d_xcols, d_xcstart;
+  Buf<int> d_perm, d_colstart, d_kcol, d_kz, d_colxy, d_xa, d_colxa, d_xcols, d_xcstart, d_bandmap;
   int fft_request = FFT_AUTO;   // the FftPath asked for (pwh_fft_path)
   int nsp = 0, nat = 0;
   std::vector<int> na, lm, aoff, poff;   // poff: first projector row of each species
@@ -1320,6 +1320,16 @@ int pwh_system_create(const pwh_system_desc* d, pwh_system** out) {
     for (int c = 0; band && c < ncol; ++c)
       band = in_band(colxy[c] % nr1, nr1) && in_band(colxy[c] / nr1, nr2);
     s->maps.band_xy = band ? 1 : 0;
+    s->maps.bandmap = nullptr;
+    if (band) {
+      const int h = nr1 / 2, q = nr1 / 4;
+      std::vector<int> bmap((size_t)h * h, -1);
+      auto band_index = [&](int i) { return i < q ? i : i - nr1 / 2; };
+      for (int c = 0; c < ncol; ++c)
+        bmap[(size_t)band_index(colxy[c] / nr1) * h + band_index(colxy[c] % nr1)] = c;
+      if ((rc = upload(s->d_bandmap, bmap.data(), bmap.size()))) return bail(rc);
+      s->maps.bandmap = s->d_bandmap.p;
+    }
     for (int t = 0; t < 4; ++t) {
       const int ct = 8 << t;
       int mx = 0;
@@ -1398,6 +1408,11 @@ int pwh_system_create(const pwh_system_desc* d, pwh_system** out) {
     return bail(rc);
   for (int dd = 0; dd < 3; ++dd) {
     auto t = twiddles(s->nr[dd]);
+    if (dd == 0 && s->nr[0] == 64) {
+      // the 64 x 64 plane pass's per-lane table [j][r] = exp(+2 pi i j r / 64) follows
+      for (int j = 0; j < 8; ++j)
+        for (int r = 0; r < 8; ++r) t.push_back(t[(j * r) % 64]);
+    }
     std::vector<cx<float>> t32(t.size());
     for (size_t i = 0; i < t.size(); ++i) t32[i] = mk<float>((float)t[i].x, (float)t[i].y);
     if ((rc = upload(s->tw64[dd], t.data(), t.size())) || (rc = upload(s->tw32[dd], t32.data(), t32.size())))
@@ -1412,7 +1427,7 @@ int pwh_system_create(const pwh_system_desc* d, pwh_system** out) {
 int pwh_system_destroy(pwh_system* s) {
   if (!s) return PWH_OK;
   if (g.stream) hipStreamSynchronize(g.stream);
-  for (auto* b : {&s->d_perm, &s->d_colstart, &s->d_kcol, &s->d_kz, &s->d_colxy, &s->d_xa}) b->release();
+  for (auto* b : {&s->d_perm, &s->d_colstart, &s->d_kcol, &s->d_kz, &s->d_colxy, &s->d_xa, &s->d_bandmap}) b->release();
   for (auto* b : {&s->Bt64, &s->X, &s->HX, &s->A64, &s->B64, &s->C64, &s->W64, &s->Hnl64,
                   &s->Q64, &s->part64, &s->stage, &s->S, &s->Vl})
     b->release();

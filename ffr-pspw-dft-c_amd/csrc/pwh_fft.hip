@@ -499,14 +499,22 @@ __global__ __launch_bounds__(NT) void k_plane_band(SphereMaps m, const cx<T>* __
 // X[s + 8t] = DFT8_j(y_j[s])[t] in lane s.  The transpose between the two DFT8 steps goes
 // through LDS within the wave (wave-level ordering, no workgroup barrier), and lane s
 // ends up holding X[s + 8t] -- exactly the inputs of the next line transform in the same
-// direction, so inverse i1, x V and forward i1 chain in registers.  A plane needs five
-// workgroup barriers instead of the eighteen of the Stockham stages.
+// direction, so inverse i1, x V and forward i1 chain in registers.
+//
+// The column passes read the plane's sphere columns from W and write them back through
+// the band map (band row, band column) -> column index, so the plane needs no zero fill,
+// scatter or gather phase and two workgroup barriers in all.
 //
 // Layout: LDS serves 16 consecutive lanes of 8-byte words (8 of 16-byte words) per cycle,
-// so consecutive lanes must fall on distinct banks.  Row stride 72 complex (8 mod 16) and
-// a skewed exchange slot 8s + ((j + s) & 7) give that for every access below: the column
-// passes use waves 0-3 (8 band columns per wave, lane l: column l % 8, j = l / 8), the row
-// pass all 8 waves (8 rows per wave, lane l: j = l % 8, row l / 8).
+// so consecutive lanes must fall on distinct banks.  The tile has 64 rows of 72 complex
+// (8 mod 16).  Every LDS address below is a per-lane base plus a compile-time offset:
+//   - column passes, waves 0-3 (lane l: band column 8 wave + l % 8, j = l / 8): the data at
+//     rows j + 8r of the band column; the exchange in the wave's own rows
+//     [16 wave, 16 wave + 16) of the non-band columns 16..47, y_j[s] at row
+//     16 wave + 8 (s >> 2) + j, column 16 + l % 8 + 8 (s & 3);
+//   - row pass, all 8 waves (lane l: j = l % 8, row 8 wave + l / 8): the row itself, band
+//     entries at j + 8r, y_j[s] at slot 9s + j (9 is odd and a row is 8 mod 16, so the
+//     two rows of 8 lanes in a group of 16 hit distinct banks).
 constexpr int kW64Stride = 72;
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -515,77 +523,80 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Exchange slot of value s from lane j.  Columns (element stride 72, 64 elements): skewed
-// within blocks of 8, 8s + ((j + s) & 7).  Rows (element stride 1, 72 elements of room):
-// 9s + j, whose offsets are compile-time constants from a per-lane base (9 is odd and a
-// row is 8 mod 16, so 16 consecutive lanes -- two rows of 8 -- still hit distinct banks).
+// Exchange offsets: lane j writes y_j[s] at wr[w64_woff(s)], lane s reads y_i[s] at
+// rd[w64_roff(i)].
 template <bool ROW>
-__device__ __forceinline__ int w64_slot(int s, int j) { return ROW ? 9 * s + j : 8 * s + ((j + s) & 7); }
+__device__ __forceinline__ constexpr int w64_woff(int s) {
+  return ROW ? 9 * s : 8 * (s >> 2) * kW64Stride + 8 * (s & 3);
+}
+template <bool ROW>
+__device__ __forceinline__ constexpr int w64_roff(int i) { return ROW ? i : i * kW64Stride; }
 
 // The four-step line transform on 8 lanes: lane j passes x[j + 8r] in v (only r in
 // {0, 1, 6, 7} when PIN: the band inputs), gets X[j + 8t] back in v (only t in
-// {0, 1, 6, 7} when POUT).  Element e of the line is at line[e * es].  tw[r] =
-// exp(+2 pi i j r / 64) (r = 1..7) for this lane; SIGN < 0 uses its conjugate.
+// {0, 1, 6, 7} when POUT).  tw[r] = exp(+2 pi i j r / 64) (r = 1..7) for this lane;
+// SIGN < 0 uses its conjugate.
 template <typename T, int SIGN, bool PIN, bool POUT, bool ROW>
-__device__ __forceinline__ void w64_line(cx<T> (&v)[8], cx<T>* line, int es, int j, const cx<T> (&tw)[8]) {
+__device__ __forceinline__ void w64_line(cx<T> (&v)[8], cx<T>* wr, const cx<T>* rd, const cx<T> (&tw)[8]) {
   if constexpr (PIN) dft8_in0167<T, SIGN>(v);
   else dft<T, 8, SIGN>(v);
 #pragma unroll
   for (int r = 1; r < 8; ++r) v[r] = v[r] * (SIGN > 0 ? tw[r] : conjg(tw[r]));
 #pragma unroll
-  for (int r = 0; r < 8; ++r) line[w64_slot<ROW>(r, j) * es] = v[r];
+  for (int r = 0; r < 8; ++r) wr[w64_woff<ROW>(r)] = v[r];
   wave_lds_sync();
 #pragma unroll
-  for (int i = 0; i < 8; ++i) v[i] = line[w64_slot<ROW>(j, i) * es];
+  for (int i = 0; i < 8; ++i) v[i] = rd[w64_roff<ROW>(i)];
   if constexpr (POUT) dft8_out0167<T, SIGN>(v);
   else dft<T, 8, SIGN>(v);
 }
 
-// FP32: 77 VGPRs (6 waves per SIMD); capping it at 64 for the 8 waves the LDS would allow
-// spilled and measured slower (83.9 against 77.1 us at 64^3 x 128 bands)
+// twg: the length-64 twiddle table exp(+2 pi i k / 64), k < 64, followed by the 8 x 8
+// table [j][r] = exp(+2 pi i j r / 64).  bandmap[yb * 32 + xb]: the column index of band
+// row yb and band column xb (band index b is position b for b < 16, b + 32 above), or -1.
 template <typename T>
 __device__ __forceinline__ void plane_w64_body(SphereMaps m, const cx<T>* __restrict__ twg,
                                                    const T* __restrict__ V, cx<T>* __restrict__ W, const int bx, const int by) {
-  constexpr int N = 64, S = kW64Stride, Q = N / 4, NBC = N / 2;
+  constexpr int N = 64, S = kW64Stride;
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* tile = reinterpret_cast<cx<T>*>(smem);
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
   const int z = bx, b = by;
-  const int jc = l >> 3, cl = l & 7;   // column passes: column cl of the wave, j = jc
-  const int jr = l & 7, yl = l >> 3;   // row pass: row yl of the wave, j = jr
+  const int jc = l >> 3, cl = l & 7;   // column passes: band column 8 wave + cl, j = jc
+  const int jr = l & 7, yl = l >> 3;   // row pass: row 8 wave + yl, j = jr
   // V at the row pass's points (row 8 wave + yl, x = jr + 8t), loaded first
   const int y = 8 * wave + yl;
   T vr[8];
   const T* Vrow = V + ((size_t)z * N + y) * N;
 #pragma unroll
   for (int t = 0; t < 8; ++t) vr[t] = Vrow[jr + 8 * t];
-  // this lane's twiddles exp(+2 pi i j r / 64), r = 1..7, for the j of a pass
+  const cx<T>* tw8 = twg + N;
   cx<T> twi[8];
   auto load_tw = [&](int j) {
     twi[0] = mk<T>(1, 0);
 #pragma unroll
-    for (int r = 1; r < 8; ++r) twi[r] = twg[j * r];
+    for (int r = 1; r < 8; ++r) twi[r] = tw8[8 * j + r];
   };
-  auto band_x = [](int t) { return t < Q ? t : t + NBC; };
-  for (int i = tid; i < NBC * NBC; i += 512) {
-    const int yy = i / NBC, xx = i - yy * NBC;
-    tile[band_x(yy) * S + band_x(xx)] = mk<T>(0, 0);
-  }
-  __syncthreads();
   cx<T>* wrow = W + ((size_t)b * m.nr3 + z) * m.ncol_pad;
-  for (int c = tid; c < m.ncol; c += 512) {
-    const int xy = m.colxy[c];
-    tile[(xy >> 6) * S + (xy & 63)] = wrow[c];
-  }
-  __syncthreads();
+  const int xb = 8 * (wave & 3) + cl;
+  const int* bm = m.bandmap + jc * 32 + xb;   // band rows jc + 8q, q = 0..3: + 256 q
+  cx<T>* col = tile + (xb < 16 ? xb : xb + 32);
+  cx<T>* xwr = tile + (16 * wave + jc) * S + 16 + cl;
+  const cx<T>* xrd = tile + (16 * wave + 8 * (jc >> 2)) * S + 16 + cl + 8 * (jc & 3);
   cx<T> v[8];
-  // inverse along i2 on the 32 band columns (inputs in the band rows)
+  // inverse along i2 on the 32 band columns (inputs in the band rows 0, 1, 6, 7 of 8)
   if (wave < 4) {
-    load_tw(jc);
-    cx<T>* col = tile + band_x(8 * wave + cl);
+    int c[4];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = (r < 2 || r >= 6) ? col[(jc + 8 * r) * S] : mk<T>(0, 0);
-    w64_line<T, +1, true, false, false>(v, col, S, jc, twi);
+    for (int q = 0; q < 4; ++q) c[q] = bm[256 * q];
+    load_tw(jc);
+    cx<T> e[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) e[q] = wrow[c[q] < 0 ? 0 : c[q]];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q < 2 ? q : q + 4] = c[q] < 0 ? mk<T>(0, 0) : e[q];
+    v[2] = v[3] = v[4] = v[5] = mk<T>(0, 0);
+    w64_line<T, +1, true, false, false>(v, xwr, xrd, twi);
 #pragma unroll
     for (int t = 0; t < 8; ++t) col[(jc + 8 * t) * S] = v[t];
   }
@@ -597,30 +608,27 @@ __device__ __forceinline__ void plane_w64_body(SphereMaps m, const cx<T>* __rest
     cx<T>* row = tile + y * S;
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = (r < 2 || r >= 6) ? row[jr + 8 * r] : mk<T>(0, 0);
-    w64_line<T, +1, true, false, true>(v, row, 1, jr, twi);
+    w64_line<T, +1, true, false, true>(v, row + jr, row + 9 * jr, twi);
 #pragma unroll
     for (int t = 0; t < 8; ++t) v[t] = vr[t] * v[t];
-    w64_line<T, -1, false, true, true>(v, row, 1, jr, twi);
+    w64_line<T, -1, false, true, true>(v, row + jr, row + 9 * jr, twi);
 #pragma unroll
     for (int t = 0; t < 8; ++t)
       if (t < 2 || t >= 6) row[jr + 8 * t] = v[t];
   }
   __syncthreads();
-  // forward along i2 on the band columns, keeping the band rows
+  // forward along i2 on the band columns, keeping the band rows, written to W
   if (wave < 4) {
+    int c[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) c[q] = bm[256 * q];
     load_tw(jc);
-    cx<T>* col = tile + band_x(8 * wave + cl);
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = col[(jc + 8 * r) * S];
-    w64_line<T, -1, false, true, false>(v, col, S, jc, twi);
+    w64_line<T, -1, false, true, false>(v, xwr, xrd, twi);
 #pragma unroll
-    for (int t = 0; t < 8; ++t)
-      if (t < 2 || t >= 6) col[(jc + 8 * t) * S] = v[t];
-  }
-  __syncthreads();
-  for (int c = tid; c < m.ncol; c += 512) {
-    const int xy = m.colxy[c];
-    wrow[c] = tile[(xy >> 6) * S + (xy & 63)];
+    for (int q = 0; q < 4; ++q)
+      if (c[q] >= 0) wrow[c[q]] = v[q < 2 ? q : q + 4];
   }
 }
 

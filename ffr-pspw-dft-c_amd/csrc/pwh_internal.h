@@ -95,6 +95,9 @@ struct SphereMaps {
   int* colxa;           // [ncol] index in xa of each column's i1
   int* xcols;           // [ncol] the columns ordered by (i1, i2)
   int* xcstart;         // [nxa + 1] first position in xcols of each active i1
+  // band_xy grids: [(nr1/2) x (nr1/2)] column index of (band row, band column), -1 where no
+  // column; band index b is position b for b < n/4 and b + n/2 above
+  int* bandmap;
 };
 
 // Launchers (pwh_fft.hip)
