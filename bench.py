@@ -489,9 +489,11 @@ def scf_loop(s, tuned=None):
     two ChFSI iterations per SCF step.  FP64 throughout; FP32 throughout (the converged
     energy's median distance to FP64 over the last 40 of 100 steps); FP32 with the last
     steps in FP64 (switch at |dE| <= 1e-6).  The block stays resident in HBM for the whole
-    loop (pwh_resident_chfsi / _eval_rho_psi / _eval_enl); per step the density and the
-    potential (2 MB each) cross to the host for the mixer and EvalEnergy calls, whose entry
-    points take host arrays as the reference's do.  Times are wall times per step."""
+    loop (pwh_resident_chfsi / _eval_enl), and so do the density, V_eff and the mixing
+    history (pwh_scf_state_*): a step moves only scalars between host and device.  The same
+    loop with the host-array entry points (the density and potential, 2 MB each, crossing
+    to the host every step, as the reference's routines take them) is timed beside it over
+    10 FP64 steps.  Times are wall times per step."""
     import pwh
     from pwh import synth
     from pwh.scf import cheby_scf
@@ -529,6 +531,12 @@ def scf_loop(s, tuned=None):
                    "max_abs_occ_eig_diff_Ha": float(np.abs(r32.eig[:nocc] - r64.eig[:nocc]).max()),
                    "s_per_step": round(r32.t_total_s / r32.iterations, 5),
                    "chfsi_s_per_step": round(r32.t_chfsi_s / r32.iterations, 5)}
+    steps = {}
+    for dev in (False, True):
+        r = cheby_scf(sysm, grid, X0, nocc, focc, omega, rho0, precision="fp64", max_iter=10, conv=-1.0,
+                      device_state=dev, **kw)
+        steps["device_state" if dev else "host_arrays"] = round(r.t_total_s / r.iterations, 5)
+    out["fp64_10_steps_s_per_step"] = steps
     sysm.close()
     grid.close()
     return out

@@ -1627,9 +1627,10 @@ int pwh_eig_zheevd_noalloc(double* A_device, int ldA, double* lambda_device, int
 
 // Density of nb states (EvalRhoPsi.cpp:83-157) held in sphere order in Xw, which the
 // call modifies (the last plane wave is zeroed).  stage: the states in reference order to
-// permute into Xw first, or null when Xw already holds them.
+// permute into Xw first, or null when Xw already holds them.  The density is added to the
+// host array rhoe, or, with rho_dev, written to that device vector (overwritten).
 static int eval_rho_core(pwh_system* s, const cx<double>* stage, cx<double>* Xw, int nb, const double* w,
-                         double omega, double* rhoe, double* ekin, double* charge) {
+                         double omega, double* rhoe, double* ekin, double* charge, double* rho_dev = nullptr) {
   const int ngw = s->ngw, ldp = s->ngw_pad;
   const size_t nnr = (size_t)s->nnr;
   CHK(s->rho_q.ensure((size_t)nb * kBandSlices));
@@ -1664,12 +1665,16 @@ static int eval_rho_core(pwh_system* s, const cx<double>* stage, cx<double>* Xw,
                             s->rho_part.p, s->rho.p, s->Ysp64.p, chunk, g.stream));
   }
   const size_t nsl = (size_t)nb * kBandSlices;
-  std::vector<double> rho(nnr), qs(nsl), ks(nsl);
-  HIPCHK(hipMemcpyAsync(rho.data(), s->rho.p, sizeof(double) * nnr, hipMemcpyDeviceToHost, g.stream));
+  std::vector<double> rho(rho_dev ? 0 : nnr), qs(nsl), ks(nsl);
+  if (rho_dev)
+    HIPCHK(hipMemcpyAsync(rho_dev, s->rho.p, sizeof(double) * nnr, hipMemcpyDeviceToDevice, g.stream));
+  else
+    HIPCHK(hipMemcpyAsync(rho.data(), s->rho.p, sizeof(double) * nnr, hipMemcpyDeviceToHost, g.stream));
   HIPCHK(hipMemcpyAsync(qs.data(), s->rho_q.p, sizeof(double) * nsl, hipMemcpyDeviceToHost, g.stream));
   HIPCHK(hipMemcpyAsync(ks.data(), s->rho_k.p, sizeof(double) * nsl, hipMemcpyDeviceToHost, g.stream));
   HIPCHK(hipStreamSynchronize(g.stream));
-  for (size_t i = 0; i < nnr; ++i) rhoe[i] += rho[i];
+  if (!rho_dev)
+    for (size_t i = 0; i < nnr; ++i) rhoe[i] += rho[i];
   // sum_b w_b/2 sum_G XKG |c|^2 TPIBA2 = sum_b w_b sum_G kin |c|^2 (kin = XKG TPIBA2 / 2)
   double e = 0.0;
   for (int b = 0; b < nb; ++b) {
@@ -2442,12 +2447,15 @@ int pwh_xc_pz(const double* rho, double* vxc, double* exc, int nnr, int* n_negat
   return PWH_OK;
 }
 
-int pwh_eval_energy(pwh_grid* gr, double* rhoe, pwh_energy* e) {
-  if (!gr || !rhoe || !e) return fail(PWH_ERR_ARG, "null argument");
+}  // extern "C"
+
+namespace {
+
+// EvalEnergy.cpp:5-163 on the device density rho (read only): V_eff into veff (device), the
+// energy terms into e (ekin, enl, esr, eself in)
+int eval_energy_core(pwh_grid* gr, const double* rho, double* veff, pwh_energy* e) {
   ScfWork& w = scfw();
   const size_t n = gr->nnr;
-  CHK(scf_up(w.v[0], rhoe, n));
-  CHK(w.v[1].ensure(n));
   CHK(w.vxc.ensure(n));
   CHK(w.eps.ensure(n));
   CHK(w.neg.ensure(n));
@@ -2457,7 +2465,7 @@ int pwh_eval_energy(pwh_grid* gr, double* rhoe, pwh_energy* e) {
   std::vector<double> hp(4 * kScfBlocks);
   {
     Timed t(TM_ENERGY, g.stream);
-    HIPCHK(real_to_cx(w.v[0].p, nullptr, n, gr->c.p, g.stream));
+    HIPCHK(real_to_cx(rho, nullptr, n, gr->c.p, g.stream));
     CHK(grid_fft(gr, gr->c.p, false));
     HIPCHK(energy_g(gr->c.p, gr->gidx.p, gr->g2.p, gr->vp.p, gr->rp.p, gr->ng, gr->tpiba2, w.part.p, g.stream));
     HIPCHK(hipMemcpyAsync(hp.data(), w.part.p, sizeof(double) * 4 * kScfBlocks, hipMemcpyDeviceToHost,
@@ -2471,15 +2479,13 @@ int pwh_eval_energy(pwh_grid* gr, double* rhoe, pwh_energy* e) {
   // xc_PZ and V_eff = Re(V_es) + V_xc (:124-135)
   {
     Timed t(TM_ENERGY, g.stream);
-    HIPCHK(xc_pz(w.v[0].p, n, w.vxc.p, w.eps.p, w.neg.p, gr->c.p, w.v[1].p, w.part.p, g.stream));
+    HIPCHK(xc_pz(rho, n, w.vxc.p, w.eps.p, w.neg.p, gr->c.p, veff, w.part.p, g.stream));
   }
   std::vector<double> h(kScfBlocks);
   const double cnt = sum_partials(w.part.p, kScfBlocks, h.data());
-  if (cnt > 0) HIPCHK(xc_pz_fix(w.v[0].p, n, w.neg.p, w.vxc.p, w.eps.p, gr->c.p, w.v[1].p, g.stream));
+  if (cnt > 0) HIPCHK(xc_pz_fix(rho, n, w.neg.p, w.vxc.p, w.eps.p, gr->c.p, veff, g.stream));
   double exc = 0;
-  CHK(dot_dev(w.v[0].p, w.eps.p, n, &exc));
-  CHK(scf_down(rhoe, w.v[1].p, n));
-  CHK(scf_sync());
+  CHK(dot_dev(rho, w.eps.p, n, &exc));
   const double om = gr->omega;
   e->exc = exc * om / (double)n;
   e->eht = sums[0] * om + e->esr - e->eself;
@@ -2489,6 +2495,21 @@ int pwh_eval_energy(pwh_grid* gr, double* rhoe, pwh_energy* e) {
   e->etot = e->ekin + e->eht + e->epseu + e->enl + e->exc;
   e->n_negative = (int)cnt;
   return PWH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pwh_eval_energy(pwh_grid* gr, double* rhoe, pwh_energy* e) {
+  if (!gr || !rhoe || !e) return fail(PWH_ERR_ARG, "null argument");
+  ScfWork& w = scfw();
+  const size_t n = gr->nnr;
+  CHK(scf_up(w.v[0], rhoe, n));
+  CHK(w.v[1].ensure(n));
+  CHK(eval_energy_core(gr, w.v[0].p, w.v[1].p, e));
+  CHK(scf_down(rhoe, w.v[1].p, n));
+  return scf_sync();
 }
 
 int pwh_normalize_rho(double* rho, int nnr, double omega, double nel, double* diff) {
@@ -2501,6 +2522,200 @@ int pwh_normalize_rho(double* rho, int nnr, double omega, double nel, double* di
   return scf_sync();
 }
 
+}  // extern "C"
+
+namespace {
+
+// The mixers' device bodies.  R is the reference's rhoe (the new density in, the mixed one
+// out); O, F, G, DF, U are the columns of its history arrays (rho_old, f, g, df, u), each a
+// device vector of n values.
+struct MixCols {
+  double* R = nullptr;
+  double* O[8] = {};
+  double* F[8] = {};
+  double* G[2] = {};
+  double* DF[8] = {};
+  double* U[8] = {};
+};
+
+// SimpleRhoMix.cpp:5-20
+int mix_simple_core(double beta, const MixCols& c, size_t n, double* d) {
+  double sq = 0;
+  CHK(comb_dev({{beta, c.R}, {1.0 - beta, c.O[0]}}, c.R, n, c.O[0], c.R, c.O[0], &sq));
+  *d = std::sqrt(sq / (double)n);
+  return PWH_OK;
+}
+
+// KerkerRhoMix.cpp:5-55 (G[0]: a work vector)
+int mix_kerker_core(pwh_grid* gr, double beta, double q0, const MixCols& c, double* d) {
+  const size_t n = gr->nnr;
+  CHK(comb_dev({{1.0, c.R}, {-1.0, c.O[0]}}, c.G[0], n));
+  CHK(kerker_dev(gr, c.G[0], beta, q0));
+  double sq = 0;
+  CHK(comb_dev({{1.0, c.O[0]}, {1.0, c.G[0]}}, c.R, n, c.O[0], c.R, c.O[0], &sq));
+  *d = std::sqrt(sq / (double)n);
+  return PWH_OK;
+}
+
+// AndersonRhoMix.cpp:9-85 (rho_old, f: 3 columns; g: 2)
+int mix_anderson_core(int iter, double beta, const MixCols& c, size_t n, double* d) {
+  double* R = c.R;
+  auto O = [&](int j) { return c.O[j]; };
+  auto F = [&](int j) { return c.F[j]; };
+  auto G = [&](int j) { return c.G[j]; };
+  if (iter == 1) {
+    // AndersonRhoMix.cpp:18-23
+    CHK(comb_dev({{1.0, R}}, O(0), n));
+    CHK(comb_dev({}, F(0), n));
+    *d = 1.0;
+    return PWH_OK;
+  }
+  const int j0 = (iter - 1) % 3, j1 = (iter - 2) % 3, j2 = (iter - 3) % 3;
+  double sq = 0;
+  CHK(comb_dev({{1.0, R}, {-1.0, O(j1)}}, F(j0), n, nullptr, R, O(j1), &sq));
+  *d = sq;
+  if (iter == 2) {
+    CHK(comb_dev({{beta, R}, {1.0 - beta, O(j1)}}, R, n, O(j0)));
+    return PWH_OK;
+  }
+  CHK(comb_dev({{1.0, F(j1)}, {-1.0, F(j0)}}, G(0), n));
+  double d11 = 0, r1 = 0;
+  CHK(dot_dev(G(0), G(0), n, &d11));
+  CHK(dot_dev(F(j0), G(0), n, &r1));
+  if (iter == 3) {
+    const double t1 = -r1 / d11;
+    CHK(comb_dev({{1.0 - t1, O(j1)}, {t1, O(j2)}, {beta, F(j0)}, {beta * t1, G(0)}}, R, n, O(j0)));
+    return PWH_OK;
+  }
+  CHK(comb_dev({{1.0, F(j2)}, {-1.0, F(j0)}}, G(1), n));
+  double d22 = 0, d12 = 0, r2 = 0;
+  CHK(dot_dev(G(1), G(1), n, &d22));
+  CHK(dot_dev(G(0), G(1), n, &d12));
+  CHK(dot_dev(F(j0), G(1), n, &r2));
+  double t1 = d11 * d22, t2 = t1 - d12 * d12;
+  if (std::fabs(t2) > std::fabs(t1) * 1.0e-8) {
+    t1 = (-r1 * d22 + r2 * d12) / t2;
+    t2 = (r1 * d12 - r2 * d11) / t2;
+  } else {
+    t1 = 0.0;
+    t2 = 0.0;
+  }
+  // the reference adds t2 g(:,1), not t2 g(:,2) (AndersonRhoMix.cpp:76-78)
+  CHK(comb_dev({{1.0 - t1 - t2, O(j1)}, {t1, O(j2)}, {t2, O(j0)}, {beta, F(j0)}, {beta * t1, G(0)},
+                {beta * t2, G(0)}},
+               R, n, O(j0)));
+  return PWH_OK;
+}
+
+// PulayRhoMix.cpp:5-91 (rho_old, f: mixdim columns)
+int mix_pulay_core(pwh_grid* gr, int iter, const MixCols& c, int mixdim, double q0, double nel, double* d,
+                   double* nel_diff) {
+  const size_t n = gr->nnr;
+  const double beta = 0.25;
+  double* R = c.R;
+  auto O = [&](int j) { return c.O[j]; };
+  auto F = [&](int j) { return c.F[j]; };
+  double sq = 0;
+  if (iter < mixdim) {
+    // PulayRhoMix.cpp:22-35
+    CHK(comb_dev({{beta, R}, {1.0 - beta, O(iter - 1)}}, R, n));
+    CHK(comb_dev({{1.0, R}, {-1.0, O(iter - 1)}}, F(iter), n));
+    CHK(kerker_dev(gr, F(iter), 1.0, q0));
+    CHK(comb_dev({{1.0, R}}, O(iter), n));
+    CHK(dot_dev(F(iter), F(iter), n, &sq));
+    *d = std::sqrt(sq / (double)n);
+    return PWH_OK;
+  }
+  const int jc = iter % mixdim, jn = (iter + 1) % mixdim;
+  const int m = std::min(iter + 1, mixdim) + 1;
+  CHK(comb_dev({{1.0, R}, {-1.0, O(jc)}}, F(jc), n));
+  CHK(kerker_dev(gr, F(jc), 1.0, q0));
+  CHK(dot_dev(F(jc), F(jc), n, &sq));
+  *d = std::sqrt(sq / (double)n);
+  // the bordered Gram system (:54-71), symmetric from its upper triangle
+  std::vector<double> A((size_t)m * m, 0.0), b(m, 0.0);
+  for (int i = 0; i < m - 1; ++i) {
+    for (int j = i; j < m - 1; ++j) {
+      double v = 0;
+      CHK(dot_dev(F(i), F(j), n, &v));
+      A[i + (size_t)j * m] = v;
+      A[j + (size_t)i * m] = v;
+    }
+    A[i + (size_t)(m - 1) * m] = 1.0;
+    A[(m - 1) + (size_t)i * m] = 1.0;
+  }
+  b[m - 1] = 1.0;
+  CHK(lu_solve_dev(A, b, m));
+  // rhoe = sum_j alpha_j (rho_old_j + f_j), three histories per pass
+  CHK(comb_dev({}, R, n));
+  for (int j0 = 0; j0 < m - 1; j0 += 2) {
+    if (j0 + 1 < m - 1)
+      CHK(comb_dev({{1.0, R}, {b[j0], O(j0)}, {b[j0], F(j0)}, {b[j0 + 1], O(j0 + 1)}, {b[j0 + 1], F(j0 + 1)}},
+                   R, n));
+    else
+      CHK(comb_dev({{1.0, R}, {b[j0], O(j0)}, {b[j0], F(j0)}}, R, n));
+  }
+  CHK(normalize_dev(R, n, gr->omega, nel, nel_diff));
+  CHK(comb_dev({{1.0, R}}, O(jn), n));
+  return PWH_OK;
+}
+
+// BroydenRhoMix.cpp:33-103 (rho_old, f: 2 columns; df, u: mixdim; a [mixdim x mixdim] host)
+int mix_broyden_core(int iter, double alpha, double w0, const MixCols& c, double* a, int mixdim, size_t n,
+                     double* d) {
+  double* R = c.R;
+  auto O = [&](int j) { return c.O[j]; };
+  auto F = [&](int j) { return c.F[j]; };
+  auto DF = [&](int j) { return c.DF[j]; };
+  auto U = [&](int j) { return c.U[j]; };
+  const int m = std::min(iter + 1, mixdim);
+  const int jc = iter % m, kp = (iter - 1) % 2, kc = iter % 2;
+  double sq = 0;
+  CHK(comb_dev({{1.0, R}, {-1.0, O(kp)}}, F(kc), n, nullptr, R, O(kp), &sq));
+  *d = std::sqrt(sq / (double)n);
+  CHK(comb_dev({{1.0, F(kc)}, {-1.0, F(kp)}}, DF(jc), n));
+  double t1 = 0;
+  CHK(dot_dev(DF(jc), DF(jc), n, &t1));
+  t1 = std::sqrt(t1);
+  if (t1 > 1.e-8) t1 = 1.0 / t1;
+  CHK(comb_dev({{t1, DF(jc)}}, DF(jc), n));
+  CHK(comb_dev({{alpha, DF(jc)}, {t1, O(kp)}, {-t1, O(kc)}}, U(jc), n));
+  std::vector<double> cv(m);
+  for (int k = 0; k < m; ++k) CHK(dot_dev(DF(k), F(kc), n, &cv[k]));
+  for (int k = 0; k < m; ++k) {
+    double v = 0;
+    CHK(dot_dev(DF(jc), DF(k), n, &v));
+    a[k + (size_t)jc * mixdim] = v;
+    a[jc + (size_t)k * mixdim] = v;
+  }
+  std::vector<double> beta(a, a + (size_t)mixdim * mixdim);
+  for (int k = 0; k < m; ++k) beta[k + (size_t)k * mixdim] += w0 * w0;
+  CHK(lu_inverse_dev(beta, m, mixdim));
+  std::vector<double> gamma(m, 0.0);
+  for (int l = 0; l < m; ++l)
+    for (int k = 0; k < m; ++k) gamma[l] = gamma[l] + cv[k] * beta[k + (size_t)l * mixdim];
+  CHK(comb_dev({{1.0, O(kp)}, {alpha, F(kc)}}, R, n));
+  for (int l = 0; l < m; ++l) CHK(comb_dev({{1.0, R}, {-gamma[l], U(l)}}, R, n));
+  CHK(comb_dev({{1.0, R}}, O(kc), n));
+  return PWH_OK;
+}
+
+// host columns of a reference history array <-> device work vectors v[first ...]
+int cols_up(int first, const double* h, int ncols, size_t n) {
+  ScfWork& w = scfw();
+  for (int k = 0; k < ncols; ++k) CHK(scf_up(w.v[first + k], h + k * n, n));
+  return PWH_OK;
+}
+int cols_down(double* h, int first, int ncols, size_t n) {
+  ScfWork& w = scfw();
+  for (int k = 0; k < ncols; ++k) CHK(scf_down(h + k * n, w.v[first + k].p, n));
+  return PWH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int pwh_mix_simple(double beta, double* rhoe, double* rho_old, int nnr, double* d) {
   if (!rhoe || !rho_old || !d || nnr < 1) return fail(PWH_ERR_ARG, "bad mix arguments");
   CHK(ensure_init());
@@ -2508,9 +2723,10 @@ int pwh_mix_simple(double beta, double* rhoe, double* rho_old, int nnr, double* 
   const size_t n = nnr;
   CHK(scf_up(w.v[0], rhoe, n));
   CHK(scf_up(w.v[1], rho_old, n));
-  double sq = 0;
-  CHK(comb_dev({{beta, w.v[0].p}, {1.0 - beta, w.v[1].p}}, w.v[0].p, n, w.v[1].p, w.v[0].p, w.v[1].p, &sq));
-  *d = std::sqrt(sq / (double)n);
+  MixCols c;
+  c.R = w.v[0].p;
+  c.O[0] = w.v[1].p;
+  CHK(mix_simple_core(beta, c, n, d));
   CHK(scf_down(rhoe, w.v[0].p, n));
   CHK(scf_down(rho_old, w.v[1].p, n));
   return scf_sync();
@@ -2523,11 +2739,11 @@ int pwh_mix_kerker(pwh_grid* gr, double beta, double q0, double* rhoe, double* r
   CHK(scf_up(w.v[0], rhoe, n));
   CHK(scf_up(w.v[1], rho_old, n));
   CHK(w.v[2].ensure(n));
-  CHK(comb_dev({{1.0, w.v[0].p}, {-1.0, w.v[1].p}}, w.v[2].p, n));
-  CHK(kerker_dev(gr, w.v[2].p, beta, q0));
-  double sq = 0;
-  CHK(comb_dev({{1.0, w.v[1].p}, {1.0, w.v[2].p}}, w.v[0].p, n, w.v[1].p, w.v[0].p, w.v[1].p, &sq));
-  *d = std::sqrt(sq / (double)n);
+  MixCols c;
+  c.R = w.v[0].p;
+  c.O[0] = w.v[1].p;
+  c.G[0] = w.v[2].p;
+  CHK(mix_kerker_core(gr, beta, q0, c, d));
   CHK(scf_down(rhoe, w.v[0].p, n));
   CHK(scf_down(rho_old, w.v[1].p, n));
   return scf_sync();
@@ -2552,58 +2768,18 @@ int pwh_mix_anderson(int iter, double beta, double* rhoe, double* rho_old, doubl
   const size_t n = nnr;
   // device: 0 rhoe, 1..3 rho_old columns, 4..6 f columns, 7..8 g columns
   CHK(scf_up(w.v[0], rhoe, n));
-  for (int k = 0; k < 3; ++k) CHK(scf_up(w.v[1 + k], rho_old + k * n, n));
-  for (int k = 0; k < 3; ++k) CHK(scf_up(w.v[4 + k], f + k * n, n));
-  for (int k = 0; k < 2; ++k) CHK(scf_up(w.v[7 + k], gv + k * n, n));
-  double* R = w.v[0].p;
-  auto O = [&](int j) { return w.v[1 + j].p; };
-  auto F = [&](int j) { return w.v[4 + j].p; };
-  auto G = [&](int j) { return w.v[7 + j].p; };
-  if (iter == 1) {
-    // AndersonRhoMix.cpp:18-23
-    CHK(comb_dev({{1.0, R}}, O(0), n));
-    CHK(comb_dev({}, F(0), n));
-    *d = 1.0;
-  } else {
-    const int j0 = (iter - 1) % 3, j1 = (iter - 2) % 3, j2 = (iter - 3) % 3;
-    double sq = 0;
-    CHK(comb_dev({{1.0, R}, {-1.0, O(j1)}}, F(j0), n, nullptr, R, O(j1), &sq));
-    *d = sq;
-    if (iter == 2) {
-      CHK(comb_dev({{beta, R}, {1.0 - beta, O(j1)}}, R, n, O(j0)));
-    } else {
-      CHK(comb_dev({{1.0, F(j1)}, {-1.0, F(j0)}}, G(0), n));
-      double d11 = 0, r1 = 0;
-      CHK(dot_dev(G(0), G(0), n, &d11));
-      CHK(dot_dev(F(j0), G(0), n, &r1));
-      if (iter == 3) {
-        const double t1 = -r1 / d11;
-        CHK(comb_dev({{1.0 - t1, O(j1)}, {t1, O(j2)}, {beta, F(j0)}, {beta * t1, G(0)}}, R, n, O(j0)));
-      } else {
-        CHK(comb_dev({{1.0, F(j2)}, {-1.0, F(j0)}}, G(1), n));
-        double d22 = 0, d12 = 0, r2 = 0;
-        CHK(dot_dev(G(1), G(1), n, &d22));
-        CHK(dot_dev(G(0), G(1), n, &d12));
-        CHK(dot_dev(F(j0), G(1), n, &r2));
-        double t1 = d11 * d22, t2 = t1 - d12 * d12;
-        if (std::fabs(t2) > std::fabs(t1) * 1.0e-8) {
-          t1 = (-r1 * d22 + r2 * d12) / t2;
-          t2 = (r1 * d12 - r2 * d11) / t2;
-        } else {
-          t1 = 0.0;
-          t2 = 0.0;
-        }
-        // the reference adds t2 g(:,1), not t2 g(:,2) (AndersonRhoMix.cpp:76-78)
-        CHK(comb_dev({{1.0 - t1 - t2, O(j1)}, {t1, O(j2)}, {t2, O(j0)}, {beta, F(j0)}, {beta * t1, G(0)},
-                      {beta * t2, G(0)}},
-                     R, n, O(j0)));
-      }
-    }
-  }
-  CHK(scf_down(rhoe, R, n));
-  for (int k = 0; k < 3; ++k) CHK(scf_down(rho_old + k * n, O(k), n));
-  for (int k = 0; k < 3; ++k) CHK(scf_down(f + k * n, F(k), n));
-  for (int k = 0; k < 2; ++k) CHK(scf_down(gv + k * n, G(k), n));
+  CHK(cols_up(1, rho_old, 3, n));
+  CHK(cols_up(4, f, 3, n));
+  CHK(cols_up(7, gv, 2, n));
+  MixCols c;
+  c.R = w.v[0].p;
+  for (int k = 0; k < 3; ++k) { c.O[k] = w.v[1 + k].p; c.F[k] = w.v[4 + k].p; }
+  for (int k = 0; k < 2; ++k) c.G[k] = w.v[7 + k].p;
+  CHK(mix_anderson_core(iter, beta, c, n, d));
+  CHK(scf_down(rhoe, c.R, n));
+  CHK(cols_down(rho_old, 1, 3, n));
+  CHK(cols_down(f, 4, 3, n));
+  CHK(cols_down(gv, 7, 2, n));
   return scf_sync();
 }
 
@@ -2613,59 +2789,17 @@ int pwh_mix_pulay(pwh_grid* gr, int iter, double* rhoe, double* rho_old, double*
   if (mixdim < 2 || mixdim > 7) return fail(PWH_ERR_ARG, "PulayRhoMix: mixdim must be in 2..7");
   ScfWork& w = scfw();
   const size_t n = gr->nnr;
-  const double beta = 0.25;
   // device: 0 rhoe, 1..mixdim rho_old columns, 8..7+mixdim f columns
   CHK(scf_up(w.v[0], rhoe, n));
-  for (int k = 0; k < mixdim; ++k) CHK(scf_up(w.v[1 + k], rho_old + k * n, n));
-  for (int k = 0; k < mixdim; ++k) CHK(scf_up(w.v[8 + k], f + k * n, n));
-  double* R = w.v[0].p;
-  auto O = [&](int j) { return w.v[1 + j].p; };
-  auto F = [&](int j) { return w.v[8 + j].p; };
-  double sq = 0;
-  if (iter < mixdim) {
-    // PulayRhoMix.cpp:22-35
-    CHK(comb_dev({{beta, R}, {1.0 - beta, O(iter - 1)}}, R, n));
-    CHK(comb_dev({{1.0, R}, {-1.0, O(iter - 1)}}, F(iter), n));
-    CHK(kerker_dev(gr, F(iter), 1.0, q0));
-    CHK(comb_dev({{1.0, R}}, O(iter), n));
-    CHK(dot_dev(F(iter), F(iter), n, &sq));
-    *d = std::sqrt(sq / (double)n);
-  } else {
-    const int jc = iter % mixdim, jn = (iter + 1) % mixdim;
-    const int m = std::min(iter + 1, mixdim) + 1;
-    CHK(comb_dev({{1.0, R}, {-1.0, O(jc)}}, F(jc), n));
-    CHK(kerker_dev(gr, F(jc), 1.0, q0));
-    CHK(dot_dev(F(jc), F(jc), n, &sq));
-    *d = std::sqrt(sq / (double)n);
-    // the bordered Gram system (:54-71), symmetric from its upper triangle
-    std::vector<double> A((size_t)m * m, 0.0), b(m, 0.0);
-    for (int i = 0; i < m - 1; ++i) {
-      for (int j = i; j < m - 1; ++j) {
-        double v = 0;
-        CHK(dot_dev(F(i), F(j), n, &v));
-        A[i + (size_t)j * m] = v;
-        A[j + (size_t)i * m] = v;
-      }
-      A[i + (size_t)(m - 1) * m] = 1.0;
-      A[(m - 1) + (size_t)i * m] = 1.0;
-    }
-    b[m - 1] = 1.0;
-    CHK(lu_solve_dev(A, b, m));
-    // rhoe = sum_j alpha_j (rho_old_j + f_j), three histories per pass
-    CHK(comb_dev({}, R, n));
-    for (int j0 = 0; j0 < m - 1; j0 += 2) {
-      if (j0 + 1 < m - 1)
-        CHK(comb_dev({{1.0, R}, {b[j0], O(j0)}, {b[j0], F(j0)}, {b[j0 + 1], O(j0 + 1)}, {b[j0 + 1], F(j0 + 1)}},
-                     R, n));
-      else
-        CHK(comb_dev({{1.0, R}, {b[j0], O(j0)}, {b[j0], F(j0)}}, R, n));
-    }
-    CHK(normalize_dev(R, n, gr->omega, nel, nel_diff));
-    CHK(comb_dev({{1.0, R}}, O(jn), n));
-  }
-  CHK(scf_down(rhoe, R, n));
-  for (int k = 0; k < mixdim; ++k) CHK(scf_down(rho_old + k * n, O(k), n));
-  for (int k = 0; k < mixdim; ++k) CHK(scf_down(f + k * n, F(k), n));
+  CHK(cols_up(1, rho_old, mixdim, n));
+  CHK(cols_up(8, f, mixdim, n));
+  MixCols c;
+  c.R = w.v[0].p;
+  for (int k = 0; k < mixdim; ++k) { c.O[k] = w.v[1 + k].p; c.F[k] = w.v[8 + k].p; }
+  CHK(mix_pulay_core(gr, iter, c, mixdim, q0, nel, d, nel_diff));
+  CHK(scf_down(rhoe, c.R, n));
+  CHK(cols_down(rho_old, 1, mixdim, n));
+  CHK(cols_down(f, 8, mixdim, n));
   return scf_sync();
 }
 
@@ -2677,53 +2811,143 @@ int pwh_mix_broyden(int iter, double alpha, double w0, double* rho, double* rho_
   CHK(ensure_init());
   ScfWork& w = scfw();
   const size_t n = nnr;
-  // device: 0 rho, 1..2 rho_old, 3..4 f, 5..4+mixdim df, 11..10+mixdim u
+  // device: 0 rho, 1..2 rho_old, 3..4 f, 5..4+mixdim df, 13..12+mixdim u
   CHK(scf_up(w.v[0], rho, n));
-  for (int k = 0; k < 2; ++k) CHK(scf_up(w.v[1 + k], rho_old + k * n, n));
-  for (int k = 0; k < 2; ++k) CHK(scf_up(w.v[3 + k], f + k * n, n));
-  for (int k = 0; k < mixdim; ++k) CHK(scf_up(w.v[5 + k], df + k * n, n));
-  for (int k = 0; k < mixdim; ++k) CHK(scf_up(w.v[11 + k], u + k * n, n));
-  double* R = w.v[0].p;
-  auto O = [&](int j) { return w.v[1 + j].p; };
-  auto F = [&](int j) { return w.v[3 + j].p; };
-  auto DF = [&](int j) { return w.v[5 + j].p; };
-  auto U = [&](int j) { return w.v[11 + j].p; };
-  // BroydenRhoMix.cpp:33-103
-  const int m = std::min(iter + 1, mixdim);
-  const int jc = iter % m, kp = (iter - 1) % 2, kc = iter % 2;
-  double sq = 0;
-  CHK(comb_dev({{1.0, R}, {-1.0, O(kp)}}, F(kc), n, nullptr, R, O(kp), &sq));
-  *d = std::sqrt(sq / (double)n);
-  CHK(comb_dev({{1.0, F(kc)}, {-1.0, F(kp)}}, DF(jc), n));
-  double t1 = 0;
-  CHK(dot_dev(DF(jc), DF(jc), n, &t1));
-  t1 = std::sqrt(t1);
-  if (t1 > 1.e-8) t1 = 1.0 / t1;
-  CHK(comb_dev({{t1, DF(jc)}}, DF(jc), n));
-  CHK(comb_dev({{alpha, DF(jc)}, {t1, O(kp)}, {-t1, O(kc)}}, U(jc), n));
-  std::vector<double> c(m);
-  for (int k = 0; k < m; ++k) CHK(dot_dev(DF(k), F(kc), n, &c[k]));
-  for (int k = 0; k < m; ++k) {
-    double v = 0;
-    CHK(dot_dev(DF(jc), DF(k), n, &v));
-    a[k + (size_t)jc * mixdim] = v;
-    a[jc + (size_t)k * mixdim] = v;
-  }
-  std::vector<double> beta(a, a + (size_t)mixdim * mixdim);
-  for (int k = 0; k < m; ++k) beta[k + (size_t)k * mixdim] += w0 * w0;
-  CHK(lu_inverse_dev(beta, m, mixdim));
-  std::vector<double> gamma(m, 0.0);
-  for (int l = 0; l < m; ++l)
-    for (int k = 0; k < m; ++k) gamma[l] = gamma[l] + c[k] * beta[k + (size_t)l * mixdim];
-  CHK(comb_dev({{1.0, O(kp)}, {alpha, F(kc)}}, R, n));
-  for (int l = 0; l < m; ++l) CHK(comb_dev({{1.0, R}, {-gamma[l], U(l)}}, R, n));
-  CHK(comb_dev({{1.0, R}}, O(kc), n));
-  CHK(scf_down(rho, R, n));
-  for (int k = 0; k < 2; ++k) CHK(scf_down(rho_old + k * n, O(k), n));
-  for (int k = 0; k < 2; ++k) CHK(scf_down(f + k * n, F(k), n));
-  for (int k = 0; k < mixdim; ++k) CHK(scf_down(df + k * n, DF(k), n));
-  for (int k = 0; k < mixdim; ++k) CHK(scf_down(u + k * n, U(k), n));
+  CHK(cols_up(1, rho_old, 2, n));
+  CHK(cols_up(3, f, 2, n));
+  CHK(cols_up(5, df, mixdim, n));
+  CHK(cols_up(13, u, mixdim, n));
+  MixCols c;
+  c.R = w.v[0].p;
+  for (int k = 0; k < 2; ++k) { c.O[k] = w.v[1 + k].p; c.F[k] = w.v[3 + k].p; }
+  for (int k = 0; k < mixdim; ++k) { c.DF[k] = w.v[5 + k].p; c.U[k] = w.v[13 + k].p; }
+  CHK(mix_broyden_core(iter, alpha, w0, c, a, mixdim, n, d));
+  CHK(scf_down(rho, c.R, n));
+  CHK(cols_down(rho_old, 1, 2, n));
+  CHK(cols_down(f, 3, 2, n));
+  CHK(cols_down(df, 5, mixdim, n));
+  CHK(cols_down(u, 13, mixdim, n));
   return scf_sync();
+}
+
+// ---------------------------------------------------------------------------
+// The density side of an SCF loop held in HBM (density, V_eff, mixing history)
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct pwh_scf_state {
+  pwh_grid* grid = nullptr;
+  int mixer = 0, mixdim = 0, ncols = 0;
+  Buf<double> rho, veff, cols;   // cols: the history columns, nnr each
+  std::vector<double> a;         // Broyden's a [mixdim x mixdim]
+  MixCols mc;                    // R = rho, the history columns in cols
+};
+
+extern "C" {
+
+int pwh_scf_state_create(pwh_grid* gr, int mixer, int mixdim, pwh_scf_state** out) {
+  if (!gr || !out) return fail(PWH_ERR_ARG, "null argument");
+  if (mixer != 0 && mixer != 1 && mixer != 2 && mixer != 4)
+    return fail(PWH_ERR_UNSUPPORTED, "scf state mixers: 0 simple, 1 Kerker, 2 Anderson, 4 Broyden");
+  if (mixer == 4 && (mixdim < 2 || mixdim > 8)) return fail(PWH_ERR_ARG, "BroydenRhoMix: mixdim must be in 2..8");
+  CHK(ensure_init());
+  std::unique_ptr<pwh_scf_state> st(new pwh_scf_state());
+  st->grid = gr;
+  st->mixer = mixer;
+  st->mixdim = mixer == 4 ? mixdim : 0;
+  st->ncols = mixer == 0 ? 1 : mixer == 1 ? 2 : mixer == 2 ? 8 : 4 + 2 * mixdim;
+  const size_t n = gr->nnr;
+  CHK(st->rho.ensure(n));
+  CHK(st->veff.ensure(n));
+  CHK(st->cols.ensure(n * st->ncols));
+  MixCols& c = st->mc;
+  c.R = st->rho.p;
+  auto col = [&](int k) { return st->cols.p + (size_t)k * n; };
+  if (mixer == 0) {
+    c.O[0] = col(0);
+  } else if (mixer == 1) {
+    c.O[0] = col(0);
+    c.G[0] = col(1);
+  } else if (mixer == 2) {
+    for (int k = 0; k < 3; ++k) { c.O[k] = col(k); c.F[k] = col(3 + k); }
+    for (int k = 0; k < 2; ++k) c.G[k] = col(6 + k);
+  } else {
+    for (int k = 0; k < 2; ++k) { c.O[k] = col(k); c.F[k] = col(2 + k); }
+    for (int k = 0; k < mixdim; ++k) { c.DF[k] = col(4 + k); c.U[k] = col(4 + mixdim + k); }
+    st->a.assign((size_t)mixdim * mixdim, 0.0);
+  }
+  *out = st.release();
+  return PWH_OK;
+}
+
+int pwh_scf_state_destroy(pwh_scf_state* st) {
+  if (!st) return PWH_OK;
+  if (g.stream) hipStreamSynchronize(g.stream);
+  delete st;
+  return PWH_OK;
+}
+
+int pwh_scf_state_set_rho(pwh_scf_state* st, const double* rho) {
+  if (!st || !rho) return fail(PWH_ERR_ARG, "null argument");
+  const size_t n = st->grid->nnr;
+  CHK(upload(st->rho, rho, n));
+  HIPCHK(fill_zero(st->cols.p, n * st->ncols, g.stream));
+  // the reference's history arrays before the first mixing step (ChebySCFLoop.cpp:121-168):
+  // rho_old = the start density for the simple and Kerker mixers; the Anderson and Broyden
+  // histories start at zero
+  if (st->mixer == 0 || st->mixer == 1)
+    HIPCHK(hipMemcpyAsync(st->mc.O[0], st->rho.p, sizeof(double) * n, hipMemcpyDeviceToDevice, g.stream));
+  std::fill(st->a.begin(), st->a.end(), 0.0);
+  return scf_sync();
+}
+
+int pwh_scf_state_get(pwh_scf_state* st, double* rho, double* veff) {
+  if (!st) return fail(PWH_ERR_ARG, "null argument");
+  const size_t n = st->grid->nnr;
+  if (rho) CHK(scf_down(rho, st->rho.p, n));
+  if (veff) CHK(scf_down(veff, st->veff.p, n));
+  return scf_sync();
+}
+
+int pwh_scf_state_mix(pwh_scf_state* st, int iter, double beta, double q0, double alpha, double w0, double* d) {
+  if (!st || !d || iter < 1) return fail(PWH_ERR_ARG, "bad mix arguments");
+  const size_t n = st->grid->nnr;
+  switch (st->mixer) {
+    case 0: CHK(mix_simple_core(beta, st->mc, n, d)); break;
+    case 1: CHK(mix_kerker_core(st->grid, beta, q0, st->mc, d)); break;
+    case 2: CHK(mix_anderson_core(iter, beta, st->mc, n, d)); break;
+    default: CHK(mix_broyden_core(iter, alpha, w0, st->mc, st->a.data(), st->mixdim, n, d)); break;
+  }
+  return scf_sync();
+}
+
+int pwh_scf_state_energy(pwh_scf_state* st, pwh_energy* e) {
+  if (!st || !e) return fail(PWH_ERR_ARG, "null argument");
+  return eval_energy_core(st->grid, st->rho.p, st->veff.p, e);
+}
+
+int pwh_scf_state_density(pwh_system* s, pwh_scf_state* st, int nstates, const double* w, double omega,
+                          double* ekin, double* charge) {
+  CHK(check_ld(s, nstates));
+  if (!st || !w || !ekin || !charge || !(omega > 0.0)) return fail(PWH_ERR_ARG, "bad scf_state_density arguments");
+  if ((size_t)s->nnr != st->grid->nnr) return fail(PWH_ERR_ARG, "the grid and the system differ in NNR");
+  if (nstates > s->resident_nb) return fail(PWH_ERR_ARG, "more states than resident");
+  const size_t n = (size_t)nstates * s->ngw_pad;
+  CHK(s->dtmp.ensure(n));
+  HIPCHK(hipMemcpyAsync(s->dtmp.p, s->X.p, sizeof(cx<double>) * n, hipMemcpyDeviceToDevice, g.stream));
+  return eval_rho_core(s, nullptr, s->dtmp.p, nstates, w, omega, nullptr, ekin, charge, st->rho.p);
+}
+
+int pwh_scf_state_set_vloc(pwh_system* s, pwh_scf_state* st) {
+  if (!s || !st) return fail(PWH_ERR_ARG, "null argument");
+  if ((size_t)s->nnr != st->grid->nnr) return fail(PWH_ERR_ARG, "the grid and the system differ in NNR");
+  const size_t n = (size_t)s->nnr;
+  CHK(s->V64.ensure(n));
+  CHK(s->V32.ensure(n));
+  HIPCHK(hipMemcpyAsync(s->V64.p, st->veff.p, sizeof(double) * n, hipMemcpyDeviceToDevice, g.stream));
+  HIPCHK(to_float(st->veff.p, n, s->V32.p, g.stream));
+  s->vset = true;
+  return PWH_OK;
 }
 
 // ENL (EvalENL.cpp:5-49) of nb states held in sphere order in Xs (read only)

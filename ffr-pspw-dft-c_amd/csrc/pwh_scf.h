@@ -35,6 +35,7 @@ hipError_t cx_real(const cx<double>* c, size_t n, double* out, hipStream_t s);
 hipError_t comb(const Comb& cb, size_t n, double* part, hipStream_t s);
 hipError_t clip_sum(double* rho, size_t n, double* part, hipStream_t s);
 hipError_t scale_vec(double* x, size_t n, double num, double den, hipStream_t s);
+hipError_t to_float(const double* x, size_t n, float* y, hipStream_t s);
 hipError_t enl_rows(const cx<double>* C, int ld, int rows, int nb, const double* focc, double* out,
                     hipStream_t s);
 

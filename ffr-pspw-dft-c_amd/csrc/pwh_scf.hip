@@ -214,6 +214,12 @@ __global__ void k_scale(double* __restrict__ x, size_t n, double num, double den
     x[i] = x[i] * num / den;
 }
 
+// the FP32 copy of the local potential (rounded to nearest, as the host conversion)
+__global__ void k_to_float(const double* __restrict__ x, size_t n, float* __restrict__ y) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    y[i] = (float)x[i];
+}
+
 // EvalENL.cpp:36-40: row r of C [rows][ld]: sum_ist focc[ist] |C[r][ist]|^2
 __global__ __launch_bounds__(kScfThreads) void k_enl_rows(const cx<double>* __restrict__ C, int ld, int nb,
                                                           const double* __restrict__ focc,
@@ -280,6 +286,10 @@ hipError_t clip_sum(double* rho, size_t n, double* part, hipStream_t s) {
 }
 hipError_t scale_vec(double* x, size_t n, double num, double den, hipStream_t s) {
   hipLaunchKernelGGL(k_scale, stride_grid(n), dim3(256), 0, s, x, n, num, den);
+  return hipGetLastError();
+}
+hipError_t to_float(const double* x, size_t n, float* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_float, stride_grid(n), dim3(256), 0, s, x, n, y);
   return hipGetLastError();
 }
 hipError_t enl_rows(const cx<double>* C, int ld, int rows, int nb, const double* focc, double* out,

@@ -47,6 +47,8 @@ EXPORTS = [
     "pwh_phase_factor_gpu", "pwh_structure_factor", "pwh_nl_form_factor", "pwh_form_factor",
     "pwh_resident_chfsi", "pwh_resident_eval_rho_psi", "pwh_resident_eval_enl",
     "pwh_eig_zheevd_noalloc", "pwh_fft_path", "pwh_projector_mode", "pwh_tuning_get",
+    "pwh_scf_state_create", "pwh_scf_state_destroy", "pwh_scf_state_set_rho", "pwh_scf_state_get",
+    "pwh_scf_state_density", "pwh_scf_state_mix", "pwh_scf_state_energy", "pwh_scf_state_set_vloc",
 ]
 
 
@@ -190,6 +192,14 @@ def lib():
         "pwh_form_factor": [C.POINTER(PseudoDesc), _ip, C.c_double, C.c_double, C.c_int, _dp, _dp, _dp,
                             C.POINTER(C.c_double)],
         "pwh_lobpcg": [vp, vp, C.c_int, _dp, C.POINTER(DiagOpts), _dp, _dp],
+        "pwh_scf_state_create": [vp, C.c_int, C.c_int, C.POINTER(vp)],
+        "pwh_scf_state_destroy": [vp],
+        "pwh_scf_state_set_rho": [vp, _dp],
+        "pwh_scf_state_get": [vp, vp, vp],
+        "pwh_scf_state_density": [vp, vp, C.c_int, _dp, C.c_double, C.POINTER(C.c_double), _dp],
+        "pwh_scf_state_mix": [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)],
+        "pwh_scf_state_energy": [vp, C.POINTER(Energy)],
+        "pwh_scf_state_set_vloc": [vp, vp],
     }
     for name, args in proto.items():
         f = getattr(L, name)
@@ -762,6 +772,63 @@ class Grid:
         _check(lib().pwh_mix_pulay(self.h, it, rhoe, rho_old.reshape(-1), f.reshape(-1), mixdim, q0, nel,
                                    C.byref(d), C.byref(nd)))
         return d.value
+
+
+MIXERS = {"simple": 0, "kerker": 1, "anderson": 2, "broyden": 4}
+
+
+class ScfState:
+    """The density, V_eff and mixing history of an SCF loop held in HBM
+    (pwh_scf_state_*): each step moves only scalars between host and device."""
+
+    def __init__(self, grid, mixer, mixdim=4):
+        init()
+        h = C.c_void_p()
+        _check(lib().pwh_scf_state_create(grid.h, MIXERS[mixer] if isinstance(mixer, str) else int(mixer),
+                                          int(mixdim), C.byref(h)))
+        self.h = h
+        self.nnr = grid.nnr
+
+    def close(self):
+        if self.h:
+            lib().pwh_scf_state_destroy(self.h)
+            self.h = None
+
+    def set_rho(self, rho):
+        r = np.ascontiguousarray(rho, np.float64).reshape(-1)
+        if r.size != self.nnr:
+            raise ValueError("rho does not match the grid")
+        _check(lib().pwh_scf_state_set_rho(self.h, r))
+
+    def get(self):
+        """(density, V_eff) as host arrays."""
+        rho = np.zeros(self.nnr)
+        veff = np.zeros(self.nnr)
+        _check(lib().pwh_scf_state_get(self.h, rho.ctypes.data_as(C.c_void_p), veff.ctypes.data_as(C.c_void_p)))
+        return rho, veff
+
+    def density(self, sysm, nstates, w, omega):
+        """EvalRhoPsi of the first nstates resident states into the held density: returns
+        (ekin, charges)."""
+        w = np.ascontiguousarray(w, np.float64)
+        ek = C.c_double()
+        q = np.zeros(nstates)
+        _check(lib().pwh_scf_state_density(sysm.h, self.h, int(nstates), w, float(omega), C.byref(ek), q))
+        return ek.value, q
+
+    def mix(self, it, beta=0.3, q0=1.0, alpha=0.25, w0=0.01):
+        d = C.c_double()
+        _check(lib().pwh_scf_state_mix(self.h, int(it), float(beta), float(q0), float(alpha), float(w0),
+                                       C.byref(d)))
+        return d.value
+
+    def energy(self, ekin=0.0, enl=0.0, esr=0.0, eself=0.0):
+        e = Energy(ekin=ekin, enl=enl, esr=esr, eself=eself)
+        _check(lib().pwh_scf_state_energy(self.h, C.byref(e)))
+        return {f: getattr(e, f) for f, _ in Energy._fields_}
+
+    def set_vloc(self, sysm):
+        _check(lib().pwh_scf_state_set_vloc(sysm.h, self.h))
 
 
 def xc_pz(rho):

@@ -47,7 +47,7 @@ class ScfResult:
 def cheby_scf(sysm, grid, X0, nocc, focc, omega, rho0, degree=10, precision=FP64, mixer="anderson",
               beta=0.3, max_iter=60, conv=1e-10, lanczos_k=20, v0=None, esr=0.0, eself=0.0,
               mixdim=4, kerker_q0=1.0, chfsi_iters=1, seed=1, fp64_refine=None, resident=True,
-              autotune=False):
+              autotune=False, device_state=True):
     """Runs the SCF loop from start vectors X0 (nstates, ngw) and density rho0 [nnr].
     nocc: occupied states (the first nocc Ritz vectors, ascending); focc [nocc]: their
     occupations times the k-point weight.  mixer: simple, kerker, anderson or broyden
@@ -64,7 +64,10 @@ def cheby_scf(sysm, grid, X0, nocc, focc, omega, rho0, degree=10, precision=FP64
     otherwise every step passes the host block in and out.  autotune: before the first
     ChFSI step, pick the projection GEMMs' split-K targets for this block (pwh_autotune) for
     FP64 (Rayleigh-Ritz, the filter's last step) and the filter precision; needs
-    resident=True.  Stops when
+    resident=True.  device_state (with resident=True): hold the density, V_eff and the
+    mixing history in HBM as well (pwh_scf_state_*), so a step moves only scalars between
+    host and device; the steps and their results are those of the host-array entry points.
+    Stops when
     |ETOT - ETOT_previous| <= conv (SCF_CONV_CRIT) or after max_iter iterations."""
     import time
     t0 = time.perf_counter()
@@ -75,17 +78,27 @@ def cheby_scf(sysm, grid, X0, nocc, focc, omega, rho0, degree=10, precision=FP64
     focc = np.ascontiguousarray(focc, np.float64)
     if focc.size != nocc:
         raise ValueError("focc needs nocc entries")
-    if mixer == "anderson":
+    use_state = resident and device_state
+    if mixer not in ("anderson", "broyden", "simple", "kerker"):
+        raise ValueError("mixer: simple, kerker, anderson or broyden")
+    if use_state:
+        from . import ScfState
+        st = ScfState(grid, mixer, mixdim)
+        st.set_rho(rho)
+        state = None
+    elif mixer == "anderson":
         state = [np.zeros((3, nnr)), np.zeros((3, nnr)), np.zeros((2, nnr))]
     elif mixer == "broyden":
+        # every history array starts at zero, as the reference allocates them
+        # (ChebySCFLoop.cpp:137-149); only simple and Kerker mixing start from rho_old = rho0
         state = [np.zeros((2, nnr)), np.zeros((2, nnr)), np.zeros((mixdim, nnr)), np.zeros((mixdim, nnr)),
                  np.zeros((mixdim, mixdim))]
-        state[0][:] = rho
-    elif mixer in ("simple", "kerker"):
-        state = [rho.copy()]
     else:
-        raise ValueError("mixer: simple, kerker, anderson or broyden")
-    veff, e = grid.eval_energy(rho, 0.0, 0.0, esr, eself)
+        state = [rho.copy()]
+    if use_state:
+        e = st.energy(0.0, 0.0, esr, eself)
+    else:
+        veff, e = grid.eval_energy(rho, 0.0, 0.0, esr, eself)
     etot_old = e["etot"]
     X = np.array(X0, np.complex128)
     if v0 is None:
@@ -101,7 +114,10 @@ def cheby_scf(sysm, grid, X0, nocc, focc, omega, rho0, degree=10, precision=FP64
     if resident:
         sysm.resident_set(X)
     for it in range(1, max_iter + 1):
-        sysm.set_vloc(veff)
+        if use_state:
+            st.set_vloc(sysm)
+        else:
+            sysm.set_vloc(veff)
         if autotune and it == 1:
             for p in sorted({FP64, prec}):
                 sysm.autotune(nst, p)
@@ -109,27 +125,33 @@ def cheby_scf(sysm, grid, X0, nocc, focc, omega, rho0, degree=10, precision=FP64
         ck = dict(degree=degree, niter=chfsi_iters, precision=prec, lanczos_k=lanczos_k if it == 1 else 0,
                   bounds_mode=0, v0=v0, lb=lb, ub=ub)
         if resident:
-            eig, _, st = sysm.resident_chfsi(nst, **ck)
+            eig, _, stc = sysm.resident_chfsi(nst, **ck)
         else:
-            eig, X, _, st = sysm.chfsi(X, **ck)
+            eig, X, _, stc = sysm.chfsi(X, **ck)
         t_chfsi += time.perf_counter() - ta
-        lb, ub = st["lb"], st["ub"]
-        if resident:
-            rnew, ekin, _ = sysm.resident_eval_rho_psi(nocc, focc, omega)
+        lb, ub = stc["lb"], stc["ub"]
+        if use_state:
+            ekin, _ = st.density(sysm, nocc, focc, omega)
             enl, _ = sysm.resident_eval_enl(nocc, focc, 1.0)
+            d = st.mix(it, beta=beta, q0=kerker_q0, alpha=0.25, w0=0.01)
+            e = st.energy(ekin, enl, esr, eself)
         else:
-            rnew, ekin, _ = sysm.eval_rho_psi(X[:nocc], focc, omega)
-            enl, _ = sysm.eval_enl(X[:nocc], focc, 1.0)
-        rho = rnew
-        if mixer == "anderson":
-            d = mix_anderson(it, beta, rho, *state)
-        elif mixer == "broyden":
-            d = mix_broyden(it, 0.25, 0.01, rho, *state, mixdim)
-        elif mixer == "kerker":
-            d = grid.mix_kerker(beta, kerker_q0, rho, state[0])
-        else:
-            d = mix_simple(beta, rho, state[0])
-        veff, e = grid.eval_energy(rho, ekin, enl, esr, eself)
+            if resident:
+                rnew, ekin, _ = sysm.resident_eval_rho_psi(nocc, focc, omega)
+                enl, _ = sysm.resident_eval_enl(nocc, focc, 1.0)
+            else:
+                rnew, ekin, _ = sysm.eval_rho_psi(X[:nocc], focc, omega)
+                enl, _ = sysm.eval_enl(X[:nocc], focc, 1.0)
+            rho = rnew
+            if mixer == "anderson":
+                d = mix_anderson(it, beta, rho, *state)
+            elif mixer == "broyden":
+                d = mix_broyden(it, 0.25, 0.01, rho, *state, mixdim)
+            elif mixer == "kerker":
+                d = grid.mix_kerker(beta, kerker_q0, rho, state[0])
+            else:
+                d = mix_simple(beta, rho, state[0])
+            veff, e = grid.eval_energy(rho, ekin, enl, esr, eself)
         detot = abs(e["etot"] - etot_old)
         etot_old = e["etot"]
         hist.append((it, e["etot"], detot, d, lb, ub, prec))
@@ -140,5 +162,8 @@ def cheby_scf(sysm, grid, X0, nocc, focc, omega, rho0, degree=10, precision=FP64
             prec = FP64
     if resident:
         X = sysm.resident_get(nst)
+    if use_state:
+        rho, _ = st.get()
+        st.close()
     return ScfResult(etot=e["etot"], energies=e, eig=eig, X=X, rho=rho, iterations=it, converged=converged,
                      history=hist, t_chfsi_s=t_chfsi, t_total_s=time.perf_counter() - t0)

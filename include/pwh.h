@@ -228,6 +228,33 @@ int pwh_mix_pulay(pwh_grid *grid, int iter, double *rhoe, double *rho_old, doubl
                   double nel, double *d, double *nel_diff);
 int pwh_mix_broyden(int iter, double alpha, double w0, double *rho, double *rho_old, double *f, double *df,
                     double *u, double *a, int mixdim, int nnr, double *d);
+/* No counterpart in the reference, whose SCF drivers keep the density, the potential and
+ * the mixing history in host arrays (ChebySCFLoop.cpp:171-372): the same state held in HBM,
+ * so an SCF step around pwh_resident_chfsi moves only scalars between host and device.
+ * The steps are the host-array entry points' (same kernels, same order, same results):
+ *   pwh_scf_state_density   EvalRhoPsi of the first nstates resident states -> the density
+ *                           (EvalRhoPsi.cpp:83-157; w, omega, ekin, charge as in
+ *                           pwh_eval_rho_psi; the density is overwritten, not added to)
+ *   pwh_scf_state_mix       the density mixed in place with the held history (mixer 0:
+ *                           SimpleRhoMix (beta), 1: KerkerRhoMix (beta, q0), 2: AndersonRhoMix
+ *                           (beta), 4: BroydenRhoMix (alpha, w0; mixdim 2..8))
+ *   pwh_scf_state_energy    EvalEnergy of the density: V_eff held, energy terms out
+ *   pwh_scf_state_set_vloc  the held V_eff as the system's local potential
+ * pwh_scf_state_set_rho sets the start density and resets the history as the reference's
+ * arrays start (ChebySCFLoop.cpp:121-168: rho_old = the start density for the simple and
+ * Kerker mixers; the Anderson and Broyden histories zero).  pwh_scf_state_get
+ * copies the density and/or V_eff out (either may be null).  The grid and the systems
+ * must share NNR. */
+typedef struct pwh_scf_state pwh_scf_state;
+int pwh_scf_state_create(pwh_grid *grid, int mixer, int mixdim, pwh_scf_state **state);
+int pwh_scf_state_destroy(pwh_scf_state *state);
+int pwh_scf_state_set_rho(pwh_scf_state *state, const double *rho);
+int pwh_scf_state_get(pwh_scf_state *state, double *rho, double *veff);
+int pwh_scf_state_density(pwh_system *sys, pwh_scf_state *state, int nstates, const double *w, double omega,
+                          double *ekin, double *charge);
+int pwh_scf_state_mix(pwh_scf_state *state, int iter, double beta, double q0, double alpha, double w0, double *d);
+int pwh_scf_state_energy(pwh_scf_state *state, pwh_energy *e);
+int pwh_scf_state_set_vloc(pwh_system *sys, pwh_scf_state *state);
 /* EvalENL.cpp:5-49 for one k-point: fnl [nproj x nstates] complex (row = projector in
  * species, atom, lm order; may be null) = beta^H C0, enl = WKPT sum FOCC |fnl|^2 WNL.
  * c0 [ld x nstates] in reference order, ld >= ngw. */

@@ -134,6 +134,62 @@ def test_scf_resident_and_host_paths_agree(scf_setup):
     assert np.abs(a.eig - b.eig).max() < 1e-10
 
 
+@pytest.mark.parametrize("mixer", ["simple", "kerker", "anderson", "broyden"])
+def test_scf_device_state_matches_host_arrays(scf_setup, mixer):
+    """Density, V_eff and mixing history held in HBM (pwh_scf_state_*) against the same loop
+    with the host-array entry points: the same kernels in the same order, so ETOT of every
+    step, the eigenvalues and the final density are equal bit for bit."""
+    kw = dict(max_iter=5, conv=0.0, resident=True)
+    from pwh.scf import cheby_scf
+    sysm, grid, X0, rho0, omega = scf_setup
+    runs = []
+    for dev in (True, False):
+        runs.append(cheby_scf(sysm, grid, X0, NOCC, np.full(NOCC, 2.0), omega, rho0, degree=DEGREE,
+                              precision="fp64", mixer=mixer, beta=BETA, chfsi_iters=CHFSI_ITERS,
+                              v0=_v0(X0.shape[1]), device_state=dev, **kw))
+    a, b = runs
+    ea, eb = np.array([h[1] for h in a.history]), np.array([h[1] for h in b.history])
+    assert np.array_equal(ea, eb), (mixer, ea - eb)
+    assert np.array_equal([h[3] for h in a.history], [h[3] for h in b.history])
+    assert np.array_equal(a.eig, b.eig)
+    assert np.array_equal(a.rho, b.rho)
+
+
+def test_scf_state_steps_match_host_entry_points(scf_setup):
+    """Each pwh_scf_state step against its host-array entry point on the same inputs:
+    EvalEnergy (energies and V_eff), the local potential it sets (H|psi> unchanged) and the
+    density of the resident states (the mixing steps are checked through the loop above).
+    Pulay is not among the state's mixers."""
+    import pwh
+    sysm, grid, X0, rho0, omega = scf_setup
+    focc = np.full(NOCC, 2.0)
+    rng = np.random.default_rng(3)
+    rho = rho0 * (1.0 + 0.1 * rng.random(rho0.size))
+    st = pwh.ScfState(grid, "anderson")
+    try:
+        st.set_rho(rho)
+        e1 = st.energy(1.5, 0.25, 0.1, 0.05)
+        v2, e2 = grid.eval_energy(rho, 1.5, 0.25, 0.1, 0.05)
+        r1, v1 = st.get()
+        assert np.array_equal(r1, rho) and np.array_equal(v1, v2) and e1 == e2
+        st.set_vloc(sysm)
+        h1 = sysm.apply_ham_block(X0[:4], "fp64")
+        h1s = sysm.apply_ham_block(X0[:4], "fp32")
+        sysm.set_vloc(v2)
+        assert np.array_equal(h1, sysm.apply_ham_block(X0[:4], "fp64"))
+        assert np.array_equal(h1s, sysm.apply_ham_block(X0[:4], "fp32"))
+        sysm.resident_set(X0)
+        k1, c1 = st.density(sysm, NOCC, focc, omega)
+        r2, k2, c2 = sysm.resident_eval_rho_psi(NOCC, focc, omega)
+        assert np.array_equal(st.get()[0], r2) and k1 == k2 and np.array_equal(c1, c2)
+        with pytest.raises(pwh.PwhError):
+            st.density(sysm, NST + 1, np.full(NST + 1, 2.0), omega)
+    finally:
+        st.close()
+    with pytest.raises(pwh.PwhError):
+        pwh.ScfState(grid, 3)
+
+
 def test_scf_autotuned_targets_keep_the_trajectory(scf_setup):
     """With the split-K targets tuned first the steps only change by the order of the
     split-K sums: ETOT per step to 1e-10 relative."""
