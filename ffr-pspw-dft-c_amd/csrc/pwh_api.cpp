@@ -1413,6 +1413,11 @@ int pwh_system_create(const pwh_system_desc* d, pwh_system** out) {
       for (int j = 0; j < 8; ++j)
         for (int r = 0; r < 8; ++r) t.push_back(t[(j * r) % 64]);
     }
+    if (dd == 0 && s->nr[0] == 96) {
+      // the 96 x 96 plane pass's per-lane table [j][s] = exp(+2 pi i j s / 96) follows
+      for (int j = 0; j < 8; ++j)
+        for (int q = 0; q < 12; ++q) t.push_back(t[(j * q) % 96]);
+    }
     std::vector<cx<float>> t32(t.size());
     for (size_t i = 0; i < t.size(); ++i) t32[i] = mk<float>((float)t[i].x, (float)t[i].y);
     if ((rc = upload(s->tw64[dd], t.data(), t.size())) || (rc = upload(s->tw32[dd], t32.data(), t32.size())))

@@ -231,3 +231,36 @@ def test_c3_grouped_launches_equal_one_at_a_time(cells24, oracle):
     B.close()
     for s in systems:
         s.close()
+
+
+def test_si216_grouped_filter_equals_one_at_a_time(si216):
+    """The 96^3 grid in a grouped batch (mode 2): two copies of the Si216 system with 16
+    states each, FP32 filter (the 96 x 96 four-step plane kernel, grouped form) and FP64
+    (the Stockham kernel), every system bit for bit equal to its one-at-a-time filter."""
+    import pwh
+    s = si216
+    pwh.init(0)
+    systems = []
+    for _ in range(2):
+        g = pwh.System(s.globals())
+        g.set_vloc(s.vloc)
+        systems.append(g)
+    X0 = [s.psi(16, seed=70 + i) for i in range(2)]
+    lb, ub = np.array([-0.5, -0.4]), np.array([40.0, 41.0])
+    B = pwh.Batch(systems, [16, 16])
+
+    def run(prec, mode):
+        for i, x in enumerate(X0):
+            B.set(i, x)
+        B.chebyfilt(3, lb, ub, prec, mode)
+        pwh.synchronize()
+        return [B.get(i) for i in range(2)]
+
+    for prec in ("fp32", "fp64"):
+        seq = run(prec, 0)
+        grp = run(prec, 2)
+        for i in range(2):
+            assert np.array_equal(grp[i], seq[i]), (prec, i)
+    B.close()
+    for g in systems:
+        g.close()
