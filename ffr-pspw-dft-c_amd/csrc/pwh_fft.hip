@@ -717,6 +717,11 @@ __device__ __forceinline__ void dft12(cx<T> (&v)[12]) {
 template <bool ROW> __device__ __forceinline__ constexpr int w96_h1() { return ROW ? 52 : 48; }
 template <bool ROW> __device__ __forceinline__ constexpr int w96_b2() { return ROW ? 25 : 24; }
 
+// c ? a : b component by component (a select of the whole struct can go through scratch
+// memory for FP64)
+template <typename T>
+__device__ __forceinline__ cx<T> sel_cx(bool c, cx<T> a, cx<T> b) { return mk<T>(c ? a.x : b.x, c ? a.y : b.y); }
+
 // w8^b (SIGN) for b = 0..3 at compile time
 template <typename T, int SIGN, int B>
 __device__ __forceinline__ cx<T> w8_mul(cx<T> a) {
@@ -730,10 +735,10 @@ __device__ __forceinline__ cx<T> w8_mul(cx<T> a) {
 // Inverse (SIGN > 0) line transform: lane l passes x[l + 8r] in v[r] (PIN: only r in
 // {0, 1, 2, 9, 10, 11}); gets X[w96_k(l, u, d)] back in v[4u + d].  line: the line's
 // exchange storage (element e at line[e * ES]); tw96[12 j + s] = w96^(js).
-template <typename T, int SIGN, bool PIN, bool ROW>
+template <typename T, int SIGN, bool PIN, bool ROW, int CS = kW96Stride>
 __device__ __forceinline__ void w96_inv(cx<T> (&v)[12], cx<T>* line, int l, const cx<T>* __restrict__ tw96,
                                         const cx<T>* __restrict__ twg) {
-  constexpr int ES = ROW ? 1 : kW96Stride, H1 = w96_h1<ROW>(), B2 = w96_b2<ROW>();
+  constexpr int ES = ROW ? 1 : CS, H1 = w96_h1<ROW>(), B2 = w96_b2<ROW>();
   // every phase that hands data between lanes is fenced: LDS serves a wave's instructions
   // in order, and the fence keeps the compiler from moving a lane's write above another
   // lane's earlier read of the same slot
@@ -783,9 +788,9 @@ __device__ __forceinline__ void w96_inv(cx<T> (&v)[12], cx<T>* line, int l, cons
 // Forward (SIGN < 0) line transform, the steps of w96_inv in reverse: lane l passes
 // Y[w96_k(l, u, d)] in v[4u + d]; gets F[l + 8r] back in v[r] (POUT: only r in
 // {0, 1, 2, 9, 10, 11} are formed).
-template <typename T, int SIGN, bool POUT, bool ROW>
+template <typename T, int SIGN, bool POUT, bool ROW, int CS = kW96Stride>
 __device__ __forceinline__ void w96_fwd(cx<T> (&v)[12], cx<T>* line, int l, const cx<T>* __restrict__ tw96) {
-  constexpr int ES = ROW ? 1 : kW96Stride, H1 = w96_h1<ROW>(), B2 = w96_b2<ROW>();
+  constexpr int ES = ROW ? 1 : CS, H1 = w96_h1<ROW>(), B2 = w96_b2<ROW>();
   const int b = l & 3, hh = l >> 2;
   wave_lds_sync();
   {
@@ -798,9 +803,9 @@ __device__ __forceinline__ void w96_fwd(cx<T> (&v)[12], cx<T>* line, int l, cons
       const bool c1 = ((b + u) & 1) != 0;
       wr[(0 + 8 * u) * ES] = t[0];
       const cx<T> t1 = w8_mul<T, SIGN, 1>(t[1]), t2 = w8_mul<T, SIGN, 2>(t[2]), t3 = w8_mul<T, SIGN, 3>(t[3]);
-      wr[(1 + 8 * u) * ES] = c1 ? t1 : t[1];
-      wr[(2 + 8 * u) * ES] = c1 ? t2 : t[2];
-      wr[(3 + 8 * u) * ES] = c1 ? t3 : t[3];
+      wr[(1 + 8 * u) * ES] = sel_cx(c1, t1, t[1]);
+      wr[(2 + 8 * u) * ES] = sel_cx(c1, t2, t[2]);
+      wr[(3 + 8 * u) * ES] = sel_cx(c1, t3, t[3]);
     }
   }
   wave_lds_sync();
@@ -920,6 +925,191 @@ template <typename T>
 __global__ __launch_bounds__(768) void k_plane_w96(SphereMaps m, const cx<T>* __restrict__ twg,
                                                    const T* __restrict__ V, cx<T>* __restrict__ W) {
   plane_w96_body<T>(m, twg, V, W, blockIdx.x, blockIdx.y);
+}
+
+// FP64 96 x 96 planes.  An FP64 plane with full rows would fill a CU's LDS, so the tile
+// holds only the 48 band columns of each row (52 complex = 104 eight-byte words per row,
+// 78 KB per plane: two planes per CU), and the row transforms exchange the real parts and
+// then the imaginary parts through 100 of the row's words with the row layouts above
+// (the row stride, 104 words, is 8 mod 16, as the FP32 rows').  Columns exchange complex
+// values in place, as in FP32.  A workgroup is 6 waves: the 48 band columns, then the 96
+// rows in two rounds of 48.
+constexpr int kW96dStride = 52;
+
+// The value unchanged, but opaque to the optimiser: per-lane table reads made through it
+// are not merged with the same reads of an earlier pass, whose results would otherwise be
+// kept live (and spilled) across the passes in between.
+__device__ __forceinline__ int w96d_opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+// The 12-value exchange of one row in two halves, in place: lane writes part of v[i] at
+// wr[WO(i)], then reads part of v[i] from rd[RO(i)]; the real parts first (the imaginary
+// parts are still the inputs when they are written).
+template <typename WO, typename RO>
+__device__ __forceinline__ void w96d_xchg(cx<double> (&v)[12], double* wr, const double* rd, WO wo, RO ro) {
+#pragma unroll
+  for (int part = 0; part < 2; ++part) {
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < 12; ++i) wr[wo(i)] = part ? v[i].y : v[i].x;
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < 12; ++i) {
+      const double x = rd[ro(i)];
+      if (part) v[i].y = x;
+      else v[i].x = x;
+    }
+  }
+}
+
+template <int SIGN, bool PIN>
+__device__ __forceinline__ void w96d_inv_row(cx<double> (&v)[12], double* row, int l,
+                                             const cx<double>* __restrict__ tw96, const cx<double>* __restrict__ twg) {
+  constexpr int H1 = 52, B2 = 25;
+  dft12<double, SIGN, PIN, false>(v);
+  const cx<double>* tl = tw96 + 12 * l;
+#pragma unroll
+  for (int s = 1; s < 12; ++s) v[s] = v[s] * (SIGN > 0 ? tl[s] : conjg(tl[s]));
+  const int b = l & 3, hh = l >> 2;
+  // step 1 -> 2: B_j[s] at j + 8 (s % 6) + H1 (s / 6); lane (h, b) reads B_b (even i) and
+  // B_(b+4) (odd i) of s = 6h + i / 2
+  w96d_xchg(v, row + l, row + b + H1 * hh, [](int s) { return 8 * (s % 6) + H1 * (s / 6); },
+            [](int i) { return (i & 1) * 4 + 8 * (i >> 1); });
+  const cx<double> w8b = SIGN > 0 ? twg[12 * b] : conjg(twg[12 * b]);
+#pragma unroll
+  for (int sg = 0; sg < 6; ++sg) {
+    const cx<double> x0 = v[2 * sg], x1 = v[2 * sg + 1];
+    v[2 * sg] = x0 + x1;
+    v[2 * sg + 1] = (x0 - x1) * w8b;
+  }
+  // step 2 -> 3: (h, b, q) at 4h + b + 8 (q % 3) + B2 (q / 3); lane (h, b') reads b = i % 4
+  // of the pair q = 3b' + i / 4
+  w96d_xchg(v, row + 4 * hh + b, row + 4 * hh + B2 * b, [](int q) { return 8 * (q % 3) + B2 * (q / 3); },
+            [](int i) { return (i & 3) + 8 * (i >> 2); });
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    cx<double> q4[4] = {v[4 * u], v[4 * u + 1], v[4 * u + 2], v[4 * u + 3]};
+    dft<double, 4, SIGN>(q4);
+#pragma unroll
+    for (int d = 0; d < 4; ++d) v[4 * u + d] = q4[d];
+  }
+}
+
+template <int SIGN, bool POUT>
+__device__ __forceinline__ void w96d_fwd_row(cx<double> (&v)[12], double* row, int l,
+                                             const cx<double>* __restrict__ tw96) {
+  constexpr int H1 = 52, B2 = 25;
+  const int b = l & 3, hh = l >> 2;
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    cx<double> q4[4] = {v[4 * u], v[4 * u + 1], v[4 * u + 2], v[4 * u + 3]};
+    dft<double, 4, SIGN>(q4);
+    const bool c1 = ((b + u) & 1) != 0;
+    const cx<double> t1 = w8_mul<double, SIGN, 1>(q4[1]), t2 = w8_mul<double, SIGN, 2>(q4[2]),
+                     t3 = w8_mul<double, SIGN, 3>(q4[3]);
+    v[4 * u] = q4[0];
+    v[4 * u + 1] = sel_cx(c1, t1, q4[1]);
+    v[4 * u + 2] = sel_cx(c1, t2, q4[2]);
+    v[4 * u + 3] = sel_cx(c1, t3, q4[3]);
+  }
+  // step 3 -> 2 (the inverse's read pattern as writes, and back)
+  w96d_xchg(v, row + 4 * hh + B2 * b, row + 4 * hh + b, [](int i) { return (i & 3) + 8 * (i >> 2); },
+            [](int q) { return 8 * (q % 3) + B2 * (q / 3); });
+#pragma unroll
+  for (int sg = 0; sg < 6; ++sg) {
+    const cx<double> z0 = v[2 * sg], z1 = v[2 * sg + 1];
+    v[2 * sg] = z0 + z1;        // U_b[6h + sg]
+    v[2 * sg + 1] = z0 - z1;    // U_(b+4)[6h + sg]
+  }
+  // step 2 -> 1
+  w96d_xchg(v, row + b + H1 * hh, row + l, [](int i) { return (i & 1) * 4 + 8 * (i >> 1); },
+            [](int s) { return 8 * (s % 6) + H1 * (s / 6); });
+  const cx<double>* tl = tw96 + 12 * l;
+#pragma unroll
+  for (int s = 1; s < 12; ++s) v[s] = v[s] * (SIGN > 0 ? tl[s] : conjg(tl[s]));
+  dft12<double, SIGN, false, POUT>(v);
+}
+
+__device__ __forceinline__ void plane_w96d_body(SphereMaps m, const cx<double>* __restrict__ twg,
+                                                const double* __restrict__ V, cx<double>* __restrict__ W,
+                                                const int bx, const int by) {
+  constexpr int N = 96, S = kW96dStride, NB = 48;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<double>* tile = reinterpret_cast<cx<double>*>(smem);
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+  const int z = bx, b = by;
+  const cx<double>* tw96 = twg + N;
+  cx<double>* wrow = W + ((size_t)b * m.nr3 + z) * m.ncol_pad;
+  const int jc = l >> 3, xb = 8 * wave + (l & 7);
+  cx<double>* col = tile + xb;
+  const int* bm = m.bandmap + jc * NB + xb;
+  cx<double> v[12];
+  {
+    int c[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) c[i] = bm[8 * NB * i];
+    cx<double> e[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) e[i] = wrow[c[i] < 0 ? 0 : c[i]];
+#pragma unroll
+    for (int r = 0; r < 12; ++r) v[r] = mk<double>(0, 0);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) v[i < 3 ? i : i + 6] = c[i] < 0 ? mk<double>(0, 0) : e[i];
+    w96_inv<double, +1, true, false, S>(v, col, jc, tw96, twg);
+    wave_lds_sync();
+#pragma unroll
+    for (int u = 0; u < 3; ++u)
+#pragma unroll
+      for (int d = 0; d < 4; ++d) col[w96_k(jc, u, d) * S] = v[4 * u + d];
+  }
+  __syncthreads();
+#pragma unroll 1
+  for (int rd = 0; rd < 2; ++rd) {
+    const int jr = w96d_opaque(l & 7);
+    const int y = 8 * wave + (l >> 3) + 48 * rd;
+    double vr[12];
+    const double* Vrow = V + ((size_t)z * N + y) * N;
+#pragma unroll
+    for (int u = 0; u < 3; ++u)
+#pragma unroll
+      for (int d = 0; d < 4; ++d) vr[4 * u + d] = Vrow[w96_k(jr, u, d)];
+    cx<double>* row = tile + y * S;
+    // band column jr + 8r of the row is at position jr + 8r' (r' = r below 3, r - 6 above 8)
+#pragma unroll
+    for (int r = 0; r < 12; ++r) v[r] = (r < 3 || r >= 9) ? row[jr + 8 * (r < 3 ? r : r - 6)] : mk<double>(0, 0);
+    wave_lds_sync();
+    w96d_inv_row<+1, true>(v, reinterpret_cast<double*>(row), jr, tw96, twg);
+#pragma unroll
+    for (int i = 0; i < 12; ++i) v[i] = vr[i] * v[i];
+    w96d_fwd_row<-1, true>(v, reinterpret_cast<double*>(row), jr, tw96);
+    wave_lds_sync();
+#pragma unroll
+    for (int r = 0; r < 12; ++r)
+      if (r < 3 || r >= 9) row[jr + 8 * (r < 3 ? r : r - 6)] = v[r];
+  }
+  __syncthreads();
+  {
+    const int jf = w96d_opaque(jc);
+    const int* bmf = m.bandmap + jf * NB + xb;
+    int c[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) c[i] = bmf[8 * NB * i];
+#pragma unroll
+    for (int u = 0; u < 3; ++u)
+#pragma unroll
+      for (int d = 0; d < 4; ++d) v[4 * u + d] = col[w96_k(jf, u, d) * S];
+    w96_fwd<double, -1, true, false, S>(v, col, jf, tw96);
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+      if (c[i] >= 0) wrow[c[i]] = v[i < 3 ? i : i + 6];
+  }
+}
+
+__global__ __launch_bounds__(384, 3) void k_plane_w96d(SphereMaps m, const cx<double>* __restrict__ twg,
+                                                    const double* __restrict__ V, cx<double>* __restrict__ W) {
+  plane_w96d_body(m, twg, V, W, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -1293,9 +1483,6 @@ static hipError_t launch_plane_n(const SphereMaps& m, const void* tw1, const voi
       return hipGetLastError();
     }
   }
-  // FP32 only: the FP64 form needs 148 VGPRs and the whole 160 KB of LDS, one 12-wave
-  // workgroup per CU, and measured slower than the Stockham kernel (4.34 against 3.34 ms
-  // at 96^3 x 432 bands)
   if constexpr (N == 96 && sizeof(T) == 4) {
     if (m.band_xy && w96_enabled()) {
       constexpr size_t sm = sizeof(cx<T>) * (size_t)N * kW96Stride;
@@ -1304,6 +1491,18 @@ static hipError_t launch_plane_n(const SphereMaps& m, const void* tw1, const voi
       if (attr != hipSuccess) return attr;
       hipLaunchKernelGGL((k_plane_w96<T>), dim3(m.nr3, nb), dim3(768), sm, s, m, static_cast<const cx<T>*>(tw1),
                          V, W);
+      return hipGetLastError();
+    }
+  }
+  if constexpr (N == 96 && sizeof(T) == 8) {
+    if (m.band_xy && w96_enabled()) {
+      constexpr size_t sm = sizeof(cx<double>) * (size_t)N * kW96dStride;
+      static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_w96d),
+                                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+      if (attr != hipSuccess) return attr;
+      hipLaunchKernelGGL(k_plane_w96d, dim3(m.nr3, nb), dim3(384), sm, s, m,
+                         reinterpret_cast<const cx<double>*>(tw1), reinterpret_cast<const double*>(V),
+                         reinterpret_cast<cx<double>*>(W));
       return hipGetLastError();
     }
   }
@@ -1513,6 +1712,12 @@ __global__ __launch_bounds__(768) void k_plane_w96_gc(const GroupHpsi<T>* __rest
   plane_w96_body<T>(p.m, p.tw1, p.V, p.W, local % p.m.nr3, local / p.m.nr3);
 }
 
+__global__ __launch_bounds__(384, 3) void k_plane_w96d_gc(const GroupHpsi<double>* __restrict__ g, int n) {
+  const GroupHpsi<double>& p = group_sys(g, n, blockIdx.x);
+  const int local = blockIdx.x - p.blk0;
+  plane_w96d_body(p.m, p.tw1, p.V, p.W, local % p.m.nr3, local / p.m.nr3);
+}
+
 enum GroupKind { GK_RT = 0, GK_COL = 1, GK_W64 = 2, GK_BAND = 3, GK_LDS = 4, GK_W96 = 5 };
 constexpr int kGroupKeyN = 4096;   // key = kind * kGroupKeyN + length
 
@@ -1524,9 +1729,9 @@ static int plane_group_kind(const SphereMaps& m, size_t* smem) {
       return GK_W64;
     }
   }
-  if constexpr (N == 96 && sizeof(T) == 4) {
+  if constexpr (N == 96) {
     if (m.band_xy && w96_enabled()) {
-      *smem = sizeof(cx<T>) * (size_t)N * kW96Stride;
+      *smem = sizeof(cx<T>) * (size_t)N * (sizeof(T) == 4 ? kW96Stride : kW96dStride);
       return GK_W96;
     }
   }
@@ -1614,6 +1819,15 @@ static hipError_t launch_group_n(int pass, int kind, const GroupHpsi<T>* g, int 
       constexpr size_t sm = sizeof(cx<T>) * (size_t)N * kW96Stride;
       if ((e = group_attr(&k_plane_w96_gc<T>, sm)) != hipSuccess) return e;
       hipLaunchKernelGGL((k_plane_w96_gc<T>), dim3(blocks), dim3(768), sm, s, g, n);
+      return hipGetLastError();
+    }
+  }
+  if constexpr (N == 96 && sizeof(T) == 8) {
+    if (kind == GK_W96) {
+      constexpr size_t sm = sizeof(cx<double>) * (size_t)N * kW96dStride;
+      if ((e = group_attr(&k_plane_w96d_gc, sm)) != hipSuccess) return e;
+      hipLaunchKernelGGL(k_plane_w96d_gc, dim3(blocks), dim3(384), sm, s,
+                         reinterpret_cast<const GroupHpsi<double>*>(g), n);
       return hipGetLastError();
     }
   }
