@@ -1112,6 +1112,142 @@ __global__ __launch_bounds__(384, 3) void k_plane_w96d(SphereMaps m, const cx<do
   plane_w96d_body(m, twg, V, W, blockIdx.x, blockIdx.y);
 }
 
+// FP64 64 x 64 planes.  The FP32 layout in FP64 takes 74 KB per plane (two per CU, the
+// kernel LDS-bound); this one keeps only the 32 band columns of each row (36 complex = 72
+// eight-byte words per row, 37 KB per plane: four per CU).  The rows exchange the real
+// parts and then the imaginary parts through their own words, slot 9s + j (the row
+// stride, 72 words, is 8 mod 16, so the two rows of a 16-lane group use different bank
+// halves); the columns exchange complex values in place, slot j + 8s of the column's 64
+// rows (a bank cycle of 16-byte words serves the 8 consecutive columns of a wave).  A
+// workgroup is 4 waves: the 32 band columns, then the 64 rows in two rounds.
+constexpr int kW64dStride = 36;
+
+template <int SIGN, bool PIN, bool POUT>
+__device__ __forceinline__ void w64d_col(cx<double> (&v)[8], cx<double>* col, int j, const cx<double> (&tw)[8]) {
+  constexpr int S = kW64dStride;
+  if constexpr (PIN) dft8_in0167<double, SIGN>(v);
+  else dft<double, 8, SIGN>(v);
+#pragma unroll
+  for (int r = 1; r < 8; ++r) v[r] = v[r] * (SIGN > 0 ? tw[r] : conjg(tw[r]));
+  wave_lds_sync();
+#pragma unroll
+  for (int r = 0; r < 8; ++r) col[(j + 8 * r) * S] = v[r];
+  wave_lds_sync();
+#pragma unroll
+  for (int i = 0; i < 8; ++i) v[i] = col[(8 * j + i) * S];
+  if constexpr (POUT) dft8_out0167<double, SIGN>(v);
+  else dft<double, 8, SIGN>(v);
+}
+
+template <int SIGN, bool PIN, bool POUT>
+__device__ __forceinline__ void w64d_row(cx<double> (&v)[8], double* row, int j, const cx<double> (&tw)[8]) {
+  if constexpr (PIN) dft8_in0167<double, SIGN>(v);
+  else dft<double, 8, SIGN>(v);
+#pragma unroll
+  for (int r = 1; r < 8; ++r) v[r] = v[r] * (SIGN > 0 ? tw[r] : conjg(tw[r]));
+#pragma unroll
+  for (int part = 0; part < 2; ++part) {
+    wave_lds_sync();
+#pragma unroll
+    for (int r = 0; r < 8; ++r) row[9 * r + j] = part ? v[r].y : v[r].x;
+    wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const double x = row[9 * j + i];
+      if (part) v[i].y = x;
+      else v[i].x = x;
+    }
+  }
+  if constexpr (POUT) dft8_out0167<double, SIGN>(v);
+  else dft<double, 8, SIGN>(v);
+}
+
+__device__ __forceinline__ void plane_w64d_body(SphereMaps m, const cx<double>* __restrict__ twg,
+                                                const double* __restrict__ V, cx<double>* __restrict__ W,
+                                                const int bx, const int by) {
+  constexpr int N = 64, S = kW64dStride, NB = 32;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<double>* tile = reinterpret_cast<cx<double>*>(smem);
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+  const int z = bx, b = by;
+  const cx<double>* tw8 = twg + N;
+  cx<double>* wrow = W + ((size_t)b * m.nr3 + z) * m.ncol_pad;
+  const int jc = l >> 3, xb = 8 * wave + (l & 7);
+  cx<double>* col = tile + xb;
+  cx<double> v[8], tw[8];
+  auto load_tw = [&](int j) {
+    tw[0] = mk<double>(1, 0);
+#pragma unroll
+    for (int r = 1; r < 8; ++r) tw[r] = tw8[8 * j + r];
+  };
+  // inverse along i2 on the 32 band columns (inputs in the band rows jc + 8q: band index
+  // jc + 8q, q = 0..3, at rows jc, jc + 8, jc + 48, jc + 56)
+  {
+    const int* bm = m.bandmap + jc * NB + xb;
+    int c[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) c[q] = bm[8 * NB * q];
+    load_tw(jc);
+    cx<double> e[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) e[q] = wrow[c[q] < 0 ? 0 : c[q]];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q < 2 ? q : q + 4] = c[q] < 0 ? mk<double>(0, 0) : e[q];
+    v[2] = v[3] = v[4] = v[5] = mk<double>(0, 0);
+    w64d_col<+1, true, false>(v, col, jc, tw);
+    wave_lds_sync();
+#pragma unroll
+    for (int t = 0; t < 8; ++t) col[(jc + 8 * t) * S] = v[t];
+  }
+  __syncthreads();
+  // rows in two rounds: inverse along i1, times V, forward along i1 keeping the band
+  // columns (band column jr + 8r of the row is at position jr + 8r', r' = r below 2,
+  // r - 4 above 5)
+#pragma unroll 1
+  for (int rd = 0; rd < 2; ++rd) {
+    const int jr = w96d_opaque(l & 7);
+    const int y = 8 * wave + (l >> 3) + 32 * rd;
+    double vr[8];
+    const double* Vrow = V + ((size_t)z * N + y) * N;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) vr[t] = Vrow[jr + 8 * t];
+    load_tw(jr);
+    cx<double>* row = tile + y * S;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = (r < 2 || r >= 6) ? row[jr + 8 * (r < 2 ? r : r - 4)] : mk<double>(0, 0);
+    wave_lds_sync();
+    w64d_row<+1, true, false>(v, reinterpret_cast<double*>(row), jr, tw);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = vr[t] * v[t];
+    w64d_row<-1, false, true>(v, reinterpret_cast<double*>(row), jr, tw);
+    wave_lds_sync();
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+      if (t < 2 || t >= 6) row[jr + 8 * (t < 2 ? t : t - 4)] = v[t];
+  }
+  __syncthreads();
+  // forward along i2 on the band columns, keeping the band rows, written to W
+  {
+    const int jf = w96d_opaque(jc);
+    const int* bm = m.bandmap + jf * NB + xb;
+    int c[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) c[q] = bm[8 * NB * q];
+    load_tw(jf);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = col[(jf + 8 * r) * S];
+    w64d_col<-1, false, true>(v, col, jf, tw);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (c[q] >= 0) wrow[c[q]] = v[q < 2 ? q : q + 4];
+  }
+}
+
+__global__ __launch_bounds__(256, 4) void k_plane_w64d(SphereMaps m, const cx<double>* __restrict__ twg,
+                                                       const double* __restrict__ V, cx<double>* __restrict__ W) {
+  plane_w64d_body(m, twg, V, W, blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // Density pass (EvalRhoPsi.cpp:138-150): per (i3, band group) plane, inverse FFT along i2
 // on the active columns and along i1, then sum w_b |psi_b(r)|^2 over the group's bands
@@ -1452,6 +1588,13 @@ static bool w64_enabled() {
   }();
   return on;
 }
+static bool w64d_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("PWH_PLANE_W64D");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
 static bool w96_enabled() {
   static const bool on = [] {
     const char* e = getenv("PWH_PLANE_W96");
@@ -1471,6 +1614,15 @@ template <typename T, int N>
 static hipError_t launch_plane_n(const SphereMaps& m, const void* tw1, const void* tw2, const T* V,
                                  cx<T>* W, int nb, hipStream_t s) {
   const bool w64_on = w64_enabled();
+  if constexpr (N == 64 && sizeof(T) == 8) {
+    if (m.band_xy && w64_on && w64d_enabled()) {
+      constexpr size_t sm = sizeof(cx<double>) * (size_t)N * kW64dStride;
+      hipLaunchKernelGGL(k_plane_w64d, dim3(m.nr3, nb), dim3(256), sm, s, m,
+                         reinterpret_cast<const cx<double>*>(tw1), reinterpret_cast<const double*>(V),
+                         reinterpret_cast<cx<double>*>(W));
+      return hipGetLastError();
+    }
+  }
   if constexpr (N == 64) {
     if (m.band_xy && w64_on) {
       constexpr size_t sm = sizeof(cx<T>) * (size_t)N * kW64Stride;
@@ -1712,6 +1864,12 @@ __global__ __launch_bounds__(768) void k_plane_w96_gc(const GroupHpsi<T>* __rest
   plane_w96_body<T>(p.m, p.tw1, p.V, p.W, local % p.m.nr3, local / p.m.nr3);
 }
 
+__global__ __launch_bounds__(256, 4) void k_plane_w64d_gc(const GroupHpsi<double>* __restrict__ g, int n) {
+  const GroupHpsi<double>& p = group_sys(g, n, blockIdx.x);
+  const int local = blockIdx.x - p.blk0;
+  plane_w64d_body(p.m, p.tw1, p.V, p.W, local % p.m.nr3, local / p.m.nr3);
+}
+
 __global__ __launch_bounds__(384, 3) void k_plane_w96d_gc(const GroupHpsi<double>* __restrict__ g, int n) {
   const GroupHpsi<double>& p = group_sys(g, n, blockIdx.x);
   const int local = blockIdx.x - p.blk0;
@@ -1725,7 +1883,7 @@ template <typename T, int N>
 static int plane_group_kind(const SphereMaps& m, size_t* smem) {
   if constexpr (N == 64) {
     if (m.band_xy && w64_enabled()) {
-      *smem = sizeof(cx<T>) * (size_t)N * kW64Stride;
+      *smem = sizeof(cx<T>) * (size_t)N * (sizeof(T) == 8 && w64d_enabled() ? kW64dStride : kW64Stride);
       return GK_W64;
     }
   }
@@ -1806,6 +1964,14 @@ static hipError_t launch_group_n(int pass, int kind, const GroupHpsi<T>* g, int 
     return hipGetLastError();
   }
   hipError_t e = hipSuccess;
+  if constexpr (N == 64 && sizeof(T) == 8) {
+    if (kind == GK_W64 && w64d_enabled()) {
+      constexpr size_t sm = sizeof(cx<double>) * (size_t)N * kW64dStride;
+      hipLaunchKernelGGL(k_plane_w64d_gc, dim3(blocks), dim3(256), sm, s,
+                         reinterpret_cast<const GroupHpsi<double>*>(g), n);
+      return hipGetLastError();
+    }
+  }
   if constexpr (N == 64) {
     if (kind == GK_W64) {
       constexpr size_t sm = sizeof(cx<T>) * (size_t)N * kW64Stride;
