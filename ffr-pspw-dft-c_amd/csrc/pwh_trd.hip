@@ -9,47 +9,66 @@
 // block B.  Q = H_0 ... H_{n-2}, A = Q T Q^H; rocSOLVER's zstedc solves T, and the
 // back-transformation below applies Q to T's eigenvectors.
 //
-// GPU form: one launch per step; workgroup g (one wave) owns kTrdCols consecutive columns
-// of the full Hermitian matrix (both triangles kept up to date).  A launch for step k
-//   * rebuilds w_{k-1} from p_{k-1}, v_{k-1}, tau_{k-1} and the workgroups' partial sums
-//     of p_{k-1}^H v_{k-1} (every workgroup does this redundantly: no grid-wide sync);
+// GPU form: one launch per step; wave j of a step owns column k + 1 + j of the full
+// Hermitian matrix (both triangles kept up to date), so only the trailing columns are
+// launched.  A launch for step k, in every wave:
+//   * rebuilds w_{k-1} from p_{k-1}, v_{k-1}, tau_{k-1} and the waves' partial sums of
+//     p_{k-1}^H v_{k-1} (redundantly in every wave: no grid-wide sync);
 //   * rebuilds column k with step k-1's update applied and forms the reflector of step k
-//     from it (redundantly again); workgroup 0 stores v_k, tau_k, d_k, e_k;
-//   * applies step k-1's update to its own columns j > k, then forms their entries of
-//     p_k = tau_k B v_k as p_k(j) = tau_k sum_r conj(A(r, j)) v_k(r) (B is Hermitian) and
-//     its partial of p_k^H v_k.
-// Column k is read by every workgroup and written by none in that launch; p, the partial
-// sums and the reflectors are double-buffered by step parity, so launches order all
-// cross-workgroup traffic.
+//     from it (redundantly again); the first wave stores v_k, tau_k, d_k, e_k;
+//   * applies step k-1's update to its own column j > k, then forms p_k(j) =
+//     tau_k sum_r conj(A(r, j)) v_k(r) (B is Hermitian) and its term of p_k^H v_k.
+// Column k is read by every wave and written by none in that launch; p, the partial sums
+// and the reflectors are double-buffered by step parity, so launches order all
+// cross-workgroup traffic.  Wave sums run as DPP row sums plus four lane reads, and
+// values at a wave-uniform row are read with v_readlane (no LDS round trips).
 #include <cfloat>
+#include <type_traits>
 
 #include "pwh_internal.h"
 
 namespace pwh {
 
-constexpr int kTrdCols = 4;          // columns per workgroup
+constexpr int kTrdWaves = 4;         // columns (one per wave) per workgroup of a step
 constexpr int kTrdMaxN = 1024;
 
 struct TrdWork {
   cx<double>* A;      // [n][n] column-major working copy (full Hermitian)
   cx<double>* V;      // [n][n] reflector k at V[k n + r], rows r > k
   cx<double>* P;      // [2][n] p of the last step by parity
-  cx<double>* PS;     // [2][groups] partial sums of p^H v by parity
+  cx<double>* PS;     // [2][n] terms of p^H v of the last step by parity, by column
   cx<double>* tau;    // [n]
   double* d;          // [n]
   double* e;          // [n]
-  int n, groups;
+  int n;
 };
 
+// Cross-lane move of a double through DPP (CTRL: quad_perm / row_ror patterns, every
+// source lane valid)
+template <int CTRL>
+__device__ __forceinline__ double trd_dpp(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+// value of lane l (wave-uniform l)
+__device__ __forceinline__ double read_lane(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+// sum over the wave, the same value in every lane: row sums of 16 by DPP, then the four
+// rows' sums read from lanes 0, 16, 32 and 48
 __device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-  return x;
+  x += trd_dpp<0xB1>(x);    // quad_perm [1,0,3,2]
+  x += trd_dpp<0x4E>(x);    // quad_perm [2,3,0,1]
+  x += trd_dpp<0x124>(x);   // row_ror:4
+  x += trd_dpp<0x128>(x);   // row_ror:8
+  return (read_lane(x, 0) + read_lane(x, 16)) + (read_lane(x, 32) + read_lane(x, 48));
 }
 
 // Element i of a lane's register vector (rows lane + 64 i) at a wave-uniform index:
-// selected with compile-time indices (no dynamic register indexing), then broadcast from
-// the owning lane.
+// selected with compile-time indices (no dynamic register indexing), then read from the
+// owning lane.  Called with every lane of the wave active (the selection runs per lane).
 template <int R>
 __device__ __forceinline__ cx<double> row_value(const cx<double> (&a)[R], int r) {
   const int i = r >> 6, src = r & 63;
@@ -57,23 +76,28 @@ __device__ __forceinline__ cx<double> row_value(const cx<double> (&a)[R], int r)
 #pragma unroll
   for (int q = 0; q < R; ++q)
     if (q == i) s = a[q];
-  return mk<double>(__shfl(s.x, src), __shfl(s.y, src));
+  return mk<double>(read_lane(s.x, src), read_lane(s.y, src));
 }
 
-// One Householder step on one wave per workgroup: the vectors of the step live in
+// One Householder step, one wave per trailing column: the vectors of the step live in
 // registers (lane l holds rows l + 64 q, q < R, so n <= 64 R), every reduction is a wave
 // reduction, and all global loads are issued before the first of them.
 template <int R>
-__global__ __launch_bounds__(64) void k_trd_step(TrdWork t, int k) {
-  const int n = t.n, lane = threadIdx.x, g = blockIdx.x;
+__global__ __launch_bounds__(64 * kTrdWaves) void k_trd_step(TrdWork t, int k) {
+  const int n = t.n, lane = threadIdx.x & 63;
+  const int j = k + 1 + blockIdx.x * kTrdWaves + (threadIdx.x >> 6);   // the wave's column
+  if (j >= n) return;   // the whole wave; the first wave (j = k + 1 < n) always stays
+  const bool lead = j == k + 1;
   const int prev = (k - 1) & 1, cur = k & 1;
-  const int j0 = g * kTrdCols;
-  // loads: step k-1's partial sums, v and p; column k; the own columns
-  cx<double> ps = mk<double>(0, 0);
-  if (k > 0)
-    for (int i = lane; i < t.groups; i += 64) ps = ps + t.PS[prev * t.groups + i];
+  // loads: step k-1's partial sums (columns k .. n-1), v and p; column k; the own column
+  cx<double> psq[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int i = k + lane + 64 * q;
+    psq[q] = (k > 0 && i < n) ? t.PS[prev * n + i] : mk<double>(0, 0);
+  }
   const cx<double> tau_p = k > 0 ? t.tau[k - 1] : mk<double>(0, 0);
-  cx<double> vp[R], wp[R], ck[R], own[kTrdCols][R];
+  cx<double> vp[R], wp[R], ck[R], own[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const int r = lane + 64 * q;
@@ -82,14 +106,13 @@ __global__ __launch_bounds__(64) void k_trd_step(TrdWork t, int k) {
     vp[q] = (k > 0 && r > k && in) ? t.V[(size_t)(k - 1) * n + rc] : mk<double>(r == k ? 1.0 : 0.0, 0.0);
     wp[q] = (k > 0 && in) ? t.P[prev * n + rc] : mk<double>(0, 0);
     ck[q] = in ? t.A[(size_t)k * n + rc] : mk<double>(0, 0);
-#pragma unroll
-    for (int c = 0; c < kTrdCols; ++c) {
-      const int j = j0 + c;
-      own[c][q] = (j > k && j < n && r > k && r < n) ? t.A[(size_t)j * n + r] : mk<double>(0, 0);
-    }
+    own[q] = (r > k && r < n) ? t.A[(size_t)j * n + r] : mk<double>(0, 0);
   }
   // w_{k-1} = p_{k-1} - 1/2 tau_{k-1} (p^H v) v_{k-1}
   if (k > 0) {
+    cx<double> ps = psq[0];
+#pragma unroll
+    for (int q = 1; q < R; ++q) ps = ps + psq[q];
     ps.x = wave_sum(ps.x);
     ps.y = wave_sum(ps.y);
     const cx<double> al = (-0.5) * (tau_p * ps);
@@ -118,14 +141,14 @@ __global__ __launch_bounds__(64) void k_trd_step(TrdWork t, int k) {
     const double ar = alpha.x - beta, ai = alpha.y, den = ar * ar + ai * ai;
     scale = mk<double>(ar / den, -ai / den);
   }
-  const double dk = row_value(ck, k).x;
   cx<double> vk[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const int r = lane + 64 * q;
     vk[q] = r == k + 1 ? mk<double>(1, 0) : (r >= k + 2 && r < n) ? scale * ck[q] : mk<double>(0, 0);
   }
-  if (g == 0) {
+  if (lead) {
+    const double dk = row_value(ck, k).x;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int r = lane + 64 * q;
@@ -137,29 +160,25 @@ __global__ __launch_bounds__(64) void k_trd_step(TrdWork t, int k) {
       t.e[k] = beta;
     }
   }
-  // own columns j > k: step k-1's update, then p_k(j) = tau sum_r conj(A(r, j)) v_k(r)
-  cx<double> part = mk<double>(0, 0);
+  // own column j > k: step k-1's update, then p_k(j) = tau sum_r conj(A(r, j)) v_k(r)
+  const cx<double> wj = conjg(row_value(wp, j)), vj = conjg(row_value(vp, j));
+  cx<double> dot = mk<double>(0, 0);
 #pragma unroll
-  for (int c = 0; c < kTrdCols; ++c) {
-    const int j = j0 + c;
-    if (j <= k || j >= n) continue;   // wave-uniform
-    const cx<double> wj = conjg(row_value(wp, j)), vj = conjg(row_value(vp, j));
-    cx<double> dot = mk<double>(0, 0);
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int r = lane + 64 * q;
-      cx<double> a = own[c][q];
-      if (k > 0) a = a - vp[q] * wj - wp[q] * vj;
-      if (r > k && r < n) t.A[(size_t)j * n + r] = a;
-      dot = dot + conjg(a) * vk[q];
-    }
-    dot.x = wave_sum(dot.x);
-    dot.y = wave_sum(dot.y);
-    const cx<double> pj = tau * dot;
-    if (lane == 0) t.P[cur * n + j] = pj;
-    part = part + conjg(pj) * row_value(vk, j);
+  for (int q = 0; q < R; ++q) {
+    const int r = lane + 64 * q;
+    cx<double> a = own[q];
+    if (k > 0) a = a - vp[q] * wj - wp[q] * vj;
+    if (r > k && r < n) t.A[(size_t)j * n + r] = a;
+    dot = dot + conjg(a) * vk[q];
   }
-  if (lane == 0) t.PS[cur * t.groups + g] = part;
+  dot.x = wave_sum(dot.x);
+  dot.y = wave_sum(dot.y);
+  const cx<double> pj = tau * dot;
+  const cx<double> vkj = row_value(vk, j);   // (lane reads: every lane active)
+  if (lane == 0) {
+    t.P[cur * n + j] = pj;
+    t.PS[cur * n + j] = conjg(pj) * vkj;
+  }
 }
 
 // d(n-1) after the last step's update
@@ -167,7 +186,7 @@ __global__ void k_trd_last(TrdWork t) {
   if (threadIdx.x != 0) return;
   const int n = t.n, k = n - 2, cur = k & 1;
   cx<double> s = mk<double>(0, 0);
-  for (int i = 0; i < t.groups; ++i) s = s + t.PS[cur * t.groups + i];
+  for (int i = k + 1; i < n; ++i) s = s + t.PS[cur * n + i];
   const cx<double> tau = t.tau[k];
   const cx<double> al = (-0.5) * (tau * s);
   const cx<double> v = (k + 1 == n - 1) ? mk<double>(1, 0) : t.V[(size_t)k * n + n - 1];
@@ -179,43 +198,97 @@ __global__ void k_trd_last(TrdWork t) {
 // Back-transformation Y <- Q Y with Q = H_0 H_1 ... H_{n-2}: the reflectors from the last to
 // the first, y <- y - tau_k v_k (v_k^H y).  Columns of Y are independent: one wave per
 // column, the column in registers (lane l holds rows l + 64 i), the dot product a wave
-// reduction, so no workgroup barrier is needed.  Reflectors with tau = 0 are skipped.
-constexpr int kBtRows = kTrdMaxN / 64;
-__global__ __launch_bounds__(256) void k_trd_backtr(const cx<double>* __restrict__ V, const cx<double>* __restrict__ tau,
-                                                    cx<double>* __restrict__ Y, int n) {
-  const int lane = threadIdx.x & 63;
-  const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (col >= n) return;   // the whole wave
-  cx<double>* y = Y + (size_t)col * n;
-  cx<double> yr[kBtRows];
+// reduction.  The workgroup's 4 waves share the reflectors: blocks of kb of them (zero
+// above row k + 1, one at k + 1) are staged in LDS, the next block's loads in flight while
+// the waves apply the current one.  Reflectors with tau = 0 are skipped.
+constexpr int kBtWaves = 4;
+constexpr int kBtPf = 16;   // staged values per lane: kb n <= kBtPf 64 kBtWaves
+static int backtr_kb(int n) { return n <= 512 ? 8 : 4; }
+template <int R>
+__global__ __launch_bounds__(64 * kBtWaves) void k_trd_backtr(const cx<double>* __restrict__ V,
+                                                              const cx<double>* __restrict__ tau,
+                                                              cx<double>* __restrict__ Y, int n, int kb) {
+  extern __shared__ __align__(16) unsigned char trd_sm[];
+  cx<double>* sV = reinterpret_cast<cx<double>*>(trd_sm);   // [kb][n]
+  __shared__ cx<double> sT[8];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int col = blockIdx.x * kBtWaves + (tid >> 6);
+  const bool act = col < n;   // waves past the last column still stage
+  cx<double>* y = Y + (size_t)(act ? col : 0) * n;
+  cx<double> yr[R];
 #pragma unroll
-  for (int i = 0; i < kBtRows; ++i) {
+  for (int i = 0; i < R; ++i) {
     const int r = lane + 64 * i;
-    yr[i] = r < n ? y[r] : mk<double>(0, 0);
+    yr[i] = (act && r < n) ? y[r] : mk<double>(0, 0);
   }
-  for (int k = n - 2; k >= 0; --k) {
-    const cx<double> tk = tau[k];
-    if (tk.x == 0.0 && tk.y == 0.0) continue;
-    const cx<double>* vk = V + (size_t)k * n;
-    cx<double> vr[kBtRows];
-    cx<double> s = mk<double>(0, 0);
+  const int per = kb * n;
+  cx<double> pf[kBtPf];
+  // block of reflectors k = hi, hi - 1, ..., hi - kb + 1 (slot kk = hi - k)
+  auto fetch = [&](int hi) {
 #pragma unroll
-    for (int i = 0; i < kBtRows; ++i) {
-      const int r = lane + 64 * i;
-      vr[i] = r == k + 1 ? mk<double>(1, 0) : (r > k + 1 && r < n) ? vk[r] : mk<double>(0, 0);
-      s = s + conjg(vr[i]) * yr[i];
+    for (int q = 0; q < kBtPf; ++q) {
+      const int e = tid + 64 * kBtWaves * q;
+      cx<double> v = mk<double>(0, 0);
+      if (e < per) {
+        const int kk = e / n, r = e - kk * n, k = hi - kk;
+        if (k >= 0) v = r == k + 1 ? mk<double>(1, 0) : r > k + 1 ? V[(size_t)k * n + r] : mk<double>(0, 0);
+      }
+      pf[q] = v;
     }
-    s.x = wave_sum(s.x);
-    s.y = wave_sum(s.y);
-    const cx<double> f = tk * s;
+  };
+  fetch(n - 2);
+  for (int hi = n - 2; hi >= 0; hi -= kb) {
 #pragma unroll
-    for (int i = 0; i < kBtRows; ++i) yr[i] = yr[i] - vr[i] * f;
-  }
+    for (int q = 0; q < kBtPf; ++q) {
+      const int e = tid + 64 * kBtWaves * q;
+      if (e < per) sV[e] = pf[q];
+    }
+    if (tid < kb) sT[tid] = hi - tid >= 0 ? tau[hi - tid] : mk<double>(0, 0);
+    __syncthreads();
+    if (hi - kb >= 0) fetch(hi - kb);
+    if (act) {
+      for (int kk = 0; kk < kb && hi - kk >= 0; ++kk) {
+        const cx<double> tk = sT[kk];
+        if (tk.x == 0.0 && tk.y == 0.0) continue;
+        cx<double> vr[R];
+        cx<double> s = mk<double>(0, 0);
 #pragma unroll
-  for (int i = 0; i < kBtRows; ++i) {
-    const int r = lane + 64 * i;
-    if (r < n) y[r] = yr[i];
+        for (int i = 0; i < R; ++i) {
+          const int r = lane + 64 * i;
+          vr[i] = r < n ? sV[kk * n + r] : mk<double>(0, 0);
+          s = s + conjg(vr[i]) * yr[i];
+        }
+        s.x = wave_sum(s.x);
+        s.y = wave_sum(s.y);
+        const cx<double> f = tk * s;
+#pragma unroll
+        for (int i = 0; i < R; ++i) yr[i] = yr[i] - vr[i] * f;
+      }
+    }
+    __syncthreads();
   }
+  if (act) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int r = lane + 64 * i;
+      if (r < n) y[r] = yr[i];
+    }
+  }
+}
+
+// the row-register count R of a kernel for n: ceil(n / 64) rounded up to an instantiated size
+template <typename F>
+static void with_rows(int n, F&& f) {
+  const int rows = (n + 63) / 64;
+  if (rows <= 3) f(std::integral_constant<int, 3>{});
+  else if (rows <= 4) f(std::integral_constant<int, 4>{});
+  else if (rows <= 5) f(std::integral_constant<int, 5>{});
+  else if (rows <= 6) f(std::integral_constant<int, 6>{});
+  else if (rows <= 7) f(std::integral_constant<int, 7>{});
+  else if (rows <= 8) f(std::integral_constant<int, 8>{});
+  else if (rows <= 10) f(std::integral_constant<int, 10>{});
+  else if (rows <= 12) f(std::integral_constant<int, 12>{});
+  else f(std::integral_constant<int, 16>{});
 }
 
 // Eigenvalues of the symmetric tridiagonal (d, e) by multisection on Sturm counts: a
@@ -292,17 +365,19 @@ hipError_t tridiag_eigenvalues(const double* d, const double* e, int n, double* 
   return hipGetLastError();
 }
 
-size_t trd_work_elems(int n) {
-  const int groups = (n + kTrdCols - 1) / kTrdCols;
-  return (size_t)2 * n * n + 2 * (size_t)n + 2 * (size_t)groups;
-}
+size_t trd_work_elems(int n) { return (size_t)2 * n * n + 4 * (size_t)n; }
 
 int trd_max_n() { return kTrdMaxN; }
 
 hipError_t tridiag_backtransform(const cx<double>* work, const cx<double>* tau, cx<double>* Y, int n, hipStream_t s) {
   if (n < 3 || n > kTrdMaxN) return hipErrorInvalidValue;
   const cx<double>* V = work + (size_t)n * n;
-  hipLaunchKernelGGL(k_trd_backtr, dim3((n + 3) / 4), dim3(256), 0, s, V, tau, Y, n);
+  const int kb = backtr_kb(n);
+  const size_t sm = sizeof(cx<double>) * (size_t)kb * n;   // <= 64 KB
+  with_rows(n, [&](auto rc) {
+    hipLaunchKernelGGL(k_trd_backtr<decltype(rc)::value>, dim3((n + kBtWaves - 1) / kBtWaves), dim3(64 * kBtWaves),
+                       sm, s, V, tau, Y, n, kb);
+  });
   return hipGetLastError();
 }
 
@@ -311,7 +386,6 @@ hipError_t tridiag_householder(const cx<double>* A, int n, cx<double>* work, dou
   if (n < 3 || n > kTrdMaxN) return hipErrorInvalidValue;
   TrdWork t;
   t.n = n;
-  t.groups = (n + kTrdCols - 1) / kTrdCols;
   t.A = work;
   t.V = t.A + (size_t)n * n;
   t.P = t.V + (size_t)n * n;
@@ -321,10 +395,12 @@ hipError_t tridiag_householder(const cx<double>* A, int n, cx<double>* work, dou
   t.e = e;
   hipError_t err = hipMemcpyAsync(t.A, A, sizeof(cx<double>) * n * n, hipMemcpyDeviceToDevice, s);
   if (err != hipSuccess) return err;
-  for (int k = 0; k <= n - 2; ++k) {
-    if (n <= 512) hipLaunchKernelGGL(k_trd_step<8>, dim3(t.groups), dim3(64), 0, s, t, k);
-    else hipLaunchKernelGGL(k_trd_step<16>, dim3(t.groups), dim3(64), 0, s, t, k);
-  }
+  with_rows(n, [&](auto rc) {
+    for (int k = 0; k <= n - 2; ++k) {
+      const int blocks = (n - k - 1 + kTrdWaves - 1) / kTrdWaves;
+      hipLaunchKernelGGL(k_trd_step<decltype(rc)::value>, dim3(blocks), dim3(64 * kTrdWaves), 0, s, t, k);
+    }
+  });
   hipLaunchKernelGGL(k_trd_last, dim3(1), dim3(64), 0, s, t);
   return hipGetLastError();
 }
