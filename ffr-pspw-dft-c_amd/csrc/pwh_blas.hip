@@ -243,12 +243,14 @@ hipError_t symmetrize(cx<double>* A, int n, int ld, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Column-major: fill the strict lower triangle with the conjugate of the upper one
+// Column-major: fill the strict lower triangle with the conjugate of the upper one, and
+// keep only the real part of the diagonal (as LAPACK's Hermitian routines read it)
 __global__ void k_herm_fill_lower(cx<double>* A, int n, int ld) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
-  if (i >= n || i <= j) return;
-  A[i + (size_t)j * ld] = conjg(A[j + (size_t)i * ld]);
+  if (i >= n || i < j) return;
+  if (i == j) A[i + (size_t)i * ld].y = 0.0;
+  else A[i + (size_t)j * ld] = conjg(A[j + (size_t)i * ld]);
 }
 hipError_t herm_fill_lower(cx<double>* A, int n, int ld, hipStream_t s) {
   hipLaunchKernelGGL(k_herm_fill_lower, dim3((n + 63) / 64, n), dim3(64), 0, s, A, n, ld);

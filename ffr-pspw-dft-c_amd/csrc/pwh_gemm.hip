@@ -15,7 +15,7 @@
 //          - rotation:  X <- X Q (ChebySCFLoop.cpp:304-306, CholeskyQR's X R^-1)
 //
 // Precisions:
-//   FP64   v_mfma_f64_16x16x4_f64, four real products per complex one
+//   FP64   v_mfma_f64_16x16x4_f64, three real products per complex one (3M)
 //   FP32   v_mfma_f32_16x16x4_f32, three real products per complex one (3M)
 //   BF16X3 / BF16X6  v_mfma_f32_16x16x32_bf16 on two- / three-term bf16 splits of the FP32
 //          operands (3 / 6 split products per real product), FP32 accumulation
@@ -60,8 +60,9 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
 }
 
 constexpr int BT = 64;    // block tile (both output dims)
-// FP32 complex products take three real MFMAs (3M) instead of four
-template <typename T> constexpr bool THREE_M = std::is_same<T, float>::value;
+// workgroups per CU the register allocation must allow: the FP64 kernels hold three FP64
+// accumulator sets (3M) and would otherwise fall to one wave per SIMD
+template <typename T> constexpr int gemm_min_blocks() { return sizeof(T) == 8 ? 2 : 1; }
 constexpr int LPAD = 4;
 constexpr int GT = 256;   // threads per block
 
@@ -131,9 +132,8 @@ __device__ __forceinline__ void gemm_proj_block(RowLoader<T, BETA> ld, const cx<
   const int ks = split * kchunk, ke = min(K, ks + kchunk);
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int li = lane & 15, lq = lane >> 4;
-  // FP32 uses three real products per complex one (3M): cr, c2, ci accumulate
-  // S1 = sum ar br, S2 = sum ai bi and S3 = sum of the (sum/difference) product; FP64 and
-  // BF16x3 accumulate Re and Im directly in cr, ci
+  // three real products per complex one (3M): cr, c2, ci accumulate S1 = sum ar br,
+  // S2 = sum ai bi and S3 = sum of the (sum/difference) product
   acc_t cr[2][2], ci[2][2], c2[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -188,56 +188,33 @@ __device__ __forceinline__ void gemm_proj_block(RowLoader<T, BETA> ld, const cx<
           ar[t] = sAr[kr][wm + t * 16 + li]; ai[t] = sAi[kr][wm + t * 16 + li];
           br[t] = sBr[kr][wn + t * 16 + li]; bi[t] = sBi[kr][wn + t * 16 + li];
         }
-        if constexpr (THREE_M<T>) {
-          // conj(a) b: S1 = ar br, S2 = ai bi, S3 = (ar - ai)(br + bi); Re = S1 + S2,
-          // Im = S3 - S1 + S2.  Consecutive MFMAs never feed each other.
-          T as[2], bs[2];
+        // conj(a) b: S1 = ar br, S2 = ai bi, S3 = (ar - ai)(br + bi); Re = S1 + S2,
+        // Im = S3 - S1 + S2.  Consecutive MFMAs never feed each other.
+        T as[2], bs[2];
 #pragma unroll
-          for (int t = 0; t < 2; ++t) { as[t] = ar[t] - ai[t]; bs[t] = br[t] + bi[t]; }
-#pragma unroll
-          for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-              cr[a][b] = mfma4(ar[a], br[b], cr[a][b]);
-              c2[a][b] = mfma4(ai[a], bi[b], c2[a][b]);
-              ci[a][b] = mfma4(as[a], bs[b], ci[a][b]);
-            }
-        } else {
-        // first halves of all 8 accumulators, then the second halves: consecutive
-        // MFMAs never feed each other
+        for (int t = 0; t < 2; ++t) { as[t] = ar[t] - ai[t]; bs[t] = br[t] + bi[t]; }
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
           for (int b = 0; b < 2; ++b) {
             cr[a][b] = mfma4(ar[a], br[b], cr[a][b]);
-            ci[a][b] = mfma4(ar[a], bi[b], ci[a][b]);
+            c2[a][b] = mfma4(ai[a], bi[b], c2[a][b]);
+            ci[a][b] = mfma4(as[a], bs[b], ci[a][b]);
           }
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-          for (int b = 0; b < 2; ++b) {
-            cr[a][b] = mfma4(ai[a], bi[b], cr[a][b]);
-            ci[a][b] = mfma4(-ai[a], br[b], ci[a][b]);
-          }
-        }
       }
     }
     __syncthreads();
   }
-  if constexpr (THREE_M<T>) {
-    {
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < 2; ++b)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const T s1 = cr[a][b][r], s2 = c2[a][b][r], s3 = ci[a][b][r];
-            cr[a][b][r] = s1 + s2;
-            ci[a][b][r] = (s3 - s1) + s2;
-          }
-    }
-  }
+      for (int r = 0; r < 4; ++r) {
+        const T s1 = cr[a][b][r], s2 = c2[a][b][r], s3 = ci[a][b][r];
+        cr[a][b][r] = s1 + s2;
+        ci[a][b][r] = (s3 - s1) + s2;
+      }
   cx<T>* P = part + (size_t)split * M * N;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -251,14 +228,14 @@ __device__ __forceinline__ void gemm_proj_block(RowLoader<T, BETA> ld, const cx<
 }
 
 template <typename T, bool BETA, int KS>
-__global__ __launch_bounds__(GT) void k_gemm_proj(RowLoader<T, BETA> ld, const cx<T>* __restrict__ B,
+__global__ __launch_bounds__(GT, gemm_min_blocks<T>()) void k_gemm_proj(RowLoader<T, BETA> ld, const cx<T>* __restrict__ B,
                                                   int ldb, int M, int N, int K, int kchunk,
                                                   cx<T>* __restrict__ part) {
   gemm_proj_block<T, BETA, KS>(ld, B, ldb, M, N, K, kchunk, part, blockIdx.x);
 }
 
 template <int KS>
-__global__ __launch_bounds__(GT) void k_gemm_gram_upper(RowLoader<double, false> ld, const cx<double>* __restrict__ B,
+__global__ __launch_bounds__(GT, gemm_min_blocks<double>()) void k_gemm_gram_upper(RowLoader<double, false> ld, const cx<double>* __restrict__ B,
                                                         int ldb, int M, int N, int K, int kchunk,
                                                         cx<double>* __restrict__ part) {
   gemm_proj_block<double, false, KS, true>(ld, B, ldb, M, N, K, kchunk, part, blockIdx.x);
@@ -332,9 +309,8 @@ __device__ __forceinline__ void gemm_expand_block(RowLoader<T, BETA> ld, const c
   if (TRI) J = min(J, n0 + BT);
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int li = lane & 15, lq = lane >> 4;
-  // FP32 uses three real products per complex one (3M): cr, c2, ci accumulate
-  // S1 = sum ar br, S2 = sum ai bi and S3 = sum of the (sum/difference) product; FP64 and
-  // BF16x3 accumulate Re and Im directly in cr, ci
+  // three real products per complex one (3M): cr, c2, ci accumulate S1 = sum ar br,
+  // S2 = sum ai bi and S3 = sum of the (sum/difference) product
   acc_t cr[2][2], ci[2][2], c2[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -382,54 +358,33 @@ __device__ __forceinline__ void gemm_expand_block(RowLoader<T, BETA> ld, const c
           ar[t] = sAr[jr][wm + t * 16 + li]; ai[t] = sAi[jr][wm + t * 16 + li];
           br[t] = sQr[jr][wn + t * 16 + li]; bi[t] = sQi[jr][wn + t * 16 + li];
         }
-        if constexpr (THREE_M<T>) {
-          // a q: S1 = ar qr, S2 = ai qi, S3 = (ar + ai)(qr + qi); Re = S1 - S2,
-          // Im = S3 - S1 - S2
-          T as[2], bs[2];
+        // a q: S1 = ar qr, S2 = ai qi, S3 = (ar + ai)(qr + qi); Re = S1 - S2,
+        // Im = S3 - S1 - S2
+        T as[2], bs[2];
 #pragma unroll
-          for (int t = 0; t < 2; ++t) { as[t] = ar[t] + ai[t]; bs[t] = br[t] + bi[t]; }
-#pragma unroll
-          for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int b = 0; b < 2; ++b) {
-              cr[a][b] = mfma4(ar[a], br[b], cr[a][b]);
-              c2[a][b] = mfma4(ai[a], bi[b], c2[a][b]);
-              ci[a][b] = mfma4(as[a], bs[b], ci[a][b]);
-            }
-        } else {
+        for (int t = 0; t < 2; ++t) { as[t] = ar[t] + ai[t]; bs[t] = br[t] + bi[t]; }
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
           for (int b = 0; b < 2; ++b) {
             cr[a][b] = mfma4(ar[a], br[b], cr[a][b]);
-            ci[a][b] = mfma4(ar[a], bi[b], ci[a][b]);
+            c2[a][b] = mfma4(ai[a], bi[b], c2[a][b]);
+            ci[a][b] = mfma4(as[a], bs[b], ci[a][b]);
           }
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-          for (int b = 0; b < 2; ++b) {
-            cr[a][b] = mfma4(-ai[a], bi[b], cr[a][b]);
-            ci[a][b] = mfma4(ai[a], br[b], ci[a][b]);
-          }
-        }
       }
     }
     __syncthreads();
   }
-  if constexpr (THREE_M<T>) {
-    {
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+    for (int b = 0; b < 2; ++b)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const T s1 = cr[a][b][r], s2 = c2[a][b][r], s3 = ci[a][b][r];
-            cr[a][b][r] = s1 - s2;
-            ci[a][b][r] = (s3 - s1) - s2;
-          }
-    }
-  }
+      for (int r = 0; r < 4; ++r) {
+        const T s1 = cr[a][b][r], s2 = c2[a][b][r], s3 = ci[a][b][r];
+        cr[a][b][r] = s1 - s2;
+        ci[a][b][r] = (s3 - s1) - s2;
+      }
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -447,14 +402,14 @@ __device__ __forceinline__ void gemm_expand_block(RowLoader<T, BETA> ld, const c
 }
 
 template <typename T, bool BETA, bool QCM, int KS>
-__global__ __launch_bounds__(GT) void k_gemm_expand(RowLoader<T, BETA> ld, const cx<T>* __restrict__ Q,
+__global__ __launch_bounds__(GT, gemm_min_blocks<T>()) void k_gemm_expand(RowLoader<T, BETA> ld, const cx<T>* __restrict__ Q,
                                                     int ldq, int J, int N, int K,
                                                     cx<T>* __restrict__ out, int ldo, int accumulate) {
   gemm_expand_block<T, BETA, QCM, KS>(ld, Q, ldq, J, N, K, out, ldo, accumulate, blockIdx.x);
 }
 
 template <int KS>
-__global__ __launch_bounds__(GT) void k_gemm_nn_tri(RowLoader<double, false> ld, const cx<double>* __restrict__ Q,
+__global__ __launch_bounds__(GT, gemm_min_blocks<double>()) void k_gemm_nn_tri(RowLoader<double, false> ld, const cx<double>* __restrict__ Q,
                                                     int ldq, int J, int N, int K, cx<double>* __restrict__ out,
                                                     int ldo) {
   gemm_expand_block<double, false, true, KS, true>(ld, Q, ldq, J, N, K, out, ldo, 0, blockIdx.x);
@@ -476,7 +431,7 @@ __device__ __forceinline__ int group_problem(const P* __restrict__ g, int n, int
 }
 
 template <typename T, int KS>
-__global__ __launch_bounds__(GT) void k_gemm_proj_group(const GroupProj<T>* __restrict__ g, int n) {
+__global__ __launch_bounds__(GT, gemm_min_blocks<T>()) void k_gemm_proj_group(const GroupProj<T>* __restrict__ g, int n) {
   const int L = blockIdx.x;
   const GroupProj<T>& p = g[group_problem(g, n, L, [](const GroupProj<T>& x) { return x.blk0; })];
   RowLoader<T, false> ld{p.A, p.lda};
@@ -491,7 +446,7 @@ __global__ __launch_bounds__(256) void k_reduce_group(const GroupProj<T>* __rest
 }
 
 template <typename T, int KS>
-__global__ __launch_bounds__(GT) void k_gemm_expand_group(const GroupExpand<T>* __restrict__ g, int n) {
+__global__ __launch_bounds__(GT, gemm_min_blocks<T>()) void k_gemm_expand_group(const GroupExpand<T>* __restrict__ g, int n) {
   const int L = blockIdx.x;
   const GroupExpand<T>& p = g[group_problem(g, n, L, [](const GroupExpand<T>& x) { return x.blk0; })];
   RowLoader<T, false> ld{p.A, p.lda};
