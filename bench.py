@@ -503,8 +503,8 @@ def scf_loop(s, tuned=None):
     grid = pwh.Grid(s.nr, gsd["n1"], gsd["n2"], gsd["n3"], gsd["g2"], gsd["sfac"], gsd["vps"], gsd["rhops"],
                     gsd["tpiba2"], gsd["omega"], n123=s.gv["n123"][:ngw], xkg=s.gv["xkg"][:ngw])
     sysm = pwh.System(s.globals())
-    for p, t in (tuned or {}).items():      # the split-K targets tuned on the same block shape
-        sysm.autotune(0, p, target=t)
+    for p, t in (tuned or {}).items():      # the launch settings tuned on the same block shape
+        sysm.set_tuning(p, t)
     rng = np.random.default_rng(5)
     X0 = pwh.ortho_qr(rng.standard_normal((nst, ngw)) + 1j * rng.standard_normal((nst, ngw)))
     omega = gsd["omega"]
@@ -691,6 +691,10 @@ def main():
     ap.add_argument("--targets", default="",
                     help="p=N[,p=N]: set these split-K targets without the search (e.g. a profiled "
                          "run repeating a tuned one, fp64=2048)")
+    ap.add_argument("--set-tuning", default="",
+                    help="p=T:P:E:C[,...]: set the split-K target, the projection and expansion stage "
+                         "depths and the plane workgroups per CU without the search (a profiled run "
+                         "repeating a tuned one, e.g. fp32=512:16:16:3)")
     ap.add_argument("--section", default="", choices=["", "c3", "c4"],
                     help="run one section only (the C3 batch or the C4 Si216 rates and ChFSI) and print "
                          "its JSON object (profiling)")
@@ -734,13 +738,22 @@ def main():
     paths = ["fp64", "fp32", "bf16x3", "bf16x6"] if not args.only else [args.only]
     # set-up, outside the timed region: the projection GEMMs' split-K target per precision
     fixed = {k: int(v) for k, v in (kv.split("=") for kv in args.targets.split(",") if kv)}
+    preset = {}
+    for kv in (x for x in args.set_tuning.split(",") if x):
+        p, v = kv.split("=")
+        t, pk, ek, cap = (int(u) for u in v.split(":"))
+        preset[p] = {"target_blocks": t, "proj_stage_depth": pk, "expand_stage_depth": ek,
+                     "plane_blocks_per_cu": cap}
     tuned = {}
     for p in paths:
-        if p in fixed:
-            tuned[p] = sysm.autotune(nb, p, target=fixed[p])[0]
-        else:
-            tuned[p] = 512 if args.no_tune else sysm.autotune(nb, p)[0]
+        if p in preset:
+            sysm.set_tuning(p, preset[p])
+        elif p in fixed:
+            sysm.autotune(nb, p, target=fixed[p])
+        elif not args.no_tune:
+            sysm.autotune(nb, p)
     tuning = {p: sysm.tuning(p) for p in paths}
+    tuned = {p: dict(tuning[p]) for p in paths}
     log("bench: tuned settings " + json.dumps(tuning))
     rates, ms_step = {}, {}
     for p in paths:
@@ -904,7 +917,7 @@ def main():
                    "path": main_path, "global_batch": nb, "parallelism": "single GPU"},
         "band_applies_per_s": {k: round(v, 1) for k, v in rates.items()},
         "ms_per_block": {k: round(v, 4) for k, v in ms_step.items()},
-        "gemm_target_blocks": tuned,
+        "gemm_target_blocks": {p: t["target_blocks"] for p, t in tuned.items()},
         "tuning": tuning,
         "mixed_vs_fp64": round(rates.get("fp32", 0) / rates["fp64"], 3) if "fp64" in rates and "fp32" in rates else None,
         "roofline": roof,

@@ -221,6 +221,9 @@ struct pwh_system {
   // the library default), and the radix-order variant of the runtime FFT plans
   int gks[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
   int fftv = 0;
+  // plane-pass workgroups per CU while the nonlocal GEMMs overlap it, per precision (0: no
+  // cap), set by pwh_autotune
+  int pcap[4] = {0, 0, 0, 0};
   // second stream for the nonlocal GEMMs, which overlap the FFT passes (fork/join events)
   hipStream_t nl_stream = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -398,8 +401,9 @@ int hpsi(pwh_system* s, const cx<T>* X, cx<T>* out, int nb, double alpha, double
   }
   {
     Timed t(TM_PLANE, st);
+    const int cap = overlap ? s->pcap[std::is_same<T, double>::value ? PREC_FP64 : mode] : 0;
     HIPCHK(launch_plane<T>(s->maps, Sel<T>::tw(s, 0), Sel<T>::tw(s, 1), Sel<T>::V(s), Sel<T>::W(s).p,
-                           nb, Sel<T>::Ysp(s).p, chunk, st));
+                           nb, Sel<T>::Ysp(s).p, chunk, st, cap));
   }
   if (overlap) HIPCHK(hipStreamWaitEvent(st, s->ev_join, 0));
   {
@@ -3352,7 +3356,10 @@ int pwh_autotune(pwh_system* s, int nstates, int precision, int* target_blocks, 
     const std::vector<int> ks = precision == PWH_FP32 ? std::vector<int>{16, 32, 64} : std::vector<int>{16, 32};
     for (int kind = 0; kind < 2 && rc == PWH_OK; ++kind) rc = search(&s->gks[precision][kind], ks, &ms);
   }
-  // 3. the radix order of the runtime FFT plans, for grids off the compiled kernels
+  // 3. plane-pass workgroups per CU beside the overlapping nonlocal GEMMs (no cap, 3, 2, 1):
+  //    fewer plane workgroups leave LDS and wave slots to GEMM workgroups on the same CU
+  if (rc == PWH_OK) rc = search(&s->pcap[precision], {0, 3, 2, 1}, &ms);
+  // 4. the radix order of the runtime FFT plans, for grids off the compiled kernels
   if (rc == PWH_OK && s->maps.path != FFT_COMPILED) {
     int v = s->fftv;
     std::vector<double> vms(3, 1e300);
@@ -3380,6 +3387,7 @@ int pwh_autotune(pwh_system* s, int nstates, int precision, int* target_blocks, 
   if (rc != PWH_OK) {
     s->tune[precision] = 0;
     s->gks[precision][0] = s->gks[precision][1] = 0;
+    s->pcap[precision] = 0;
     return rc;
   }
   if (target_blocks) *target_blocks = s->tune[precision];
@@ -3394,6 +3402,37 @@ int pwh_tuning_get(pwh_system* s, int precision, int* out) {
   out[1] = s->gks[precision][0] ? s->gks[precision][0] : dflt;
   out[2] = s->gks[precision][1] ? s->gks[precision][1] : dflt;
   out[3] = s->maps.path == FFT_COMPILED ? -1 : s->fftv;
+  out[4] = s->pcap[precision];
+  return PWH_OK;
+}
+
+int pwh_tuning_set(pwh_system* s, int precision, const int* in) {
+  if (!s || !in || precision < PWH_FP64 || precision > PWH_BF16X6) return fail(PWH_ERR_ARG, "bad tuning setting");
+  if (in[0] != 0 && (in[0] < 64 || in[0] > 65536)) return fail(PWH_ERR_ARG, "target outside [64, 65536]");
+  // stage depths: the FP64 / FP32 kernels are compiled for 16 and 32 (and 64 in FP32); the
+  // BF16 split kernels have their one depth (32)
+  auto depth_ok = [&](int d) {
+    if (d == 0) return true;
+    if (precision >= PWH_BF16X3) return d == 32;
+    return d == 16 || d == 32 || (d == 64 && precision == PWH_FP32);
+  };
+  if (!depth_ok(in[1]) || !depth_ok(in[2])) return fail(PWH_ERR_ARG, "stage depth not compiled for this precision");
+  if (in[4] < 0 || in[4] > 3) return fail(PWH_ERR_ARG, "plane workgroups per CU outside [0, 3]");
+  if (s->maps.path != FFT_COMPILED && in[3] != s->fftv) {
+    if (in[3] < 0 || in[3] > 2) return fail(PWH_ERR_ARG, "FFT plan variant outside [0, 2]");
+    FftPlan p[3];
+    for (int d = 0; d < 3; ++d)
+      if (!fft_plan_variant(s->nr[d], in[3], &p[d])) return fail(PWH_ERR_ARG, "FFT plan variant not available");
+    for (int d = 0; d < 3; ++d) s->maps.plan[d] = p[d];
+    s->fftv = in[3];
+  }
+  s->tune[precision] = in[0];
+  if (precision <= PWH_FP32) {
+    s->gks[precision][0] = in[1];
+    s->gks[precision][1] = in[2];
+  }
+  s->pcap[precision] = in[4];
+  ++g.layout_gen;
   return PWH_OK;
 }
 

@@ -1610,13 +1610,36 @@ static bool band_enabled() {
   return on;
 }
 
+// Dynamic LDS of one plane workgroup when at most `cap` of them may share a CU (cap 0: no
+// cap, the kernel's own LDS decides).  The padding leaves LDS and wave slots to the
+// nonlocal GEMM workgroups that run beside the plane pass on the second stream: the plane
+// pass is VALU-bound and the GEMMs MFMA-bound, so a CU holding both issues from both.  A cap
+// the kernel's own LDS already meets leaves it unchanged.
+static size_t plane_lds_capped(size_t base, int cap) {
+  constexpr size_t lds = 160 * 1024;
+  if (cap <= 0) return base;
+  const size_t want = (lds / (size_t)(cap + 1) / 1024 + 1) * 1024;
+  const size_t sm = base > want ? base : want;
+  return (size_t)cap * sm <= lds ? sm : base;
+}
+
+// one-time raise of a kernel's dynamic LDS limit to the whole CU (the capped sizes); one
+// instance, and so one flag, per kernel
+template <auto KERN>
+static hipError_t plane_lds_attr() {
+  static const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(KERN),
+                                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  return e;
+}
+
 template <typename T, int N>
 static hipError_t launch_plane_n(const SphereMaps& m, const void* tw1, const void* tw2, const T* V,
-                                 cx<T>* W, int nb, hipStream_t s) {
+                                 cx<T>* W, int nb, hipStream_t s, int cap) {
   const bool w64_on = w64_enabled();
   if constexpr (N == 64 && sizeof(T) == 8) {
     if (m.band_xy && w64_on && w64d_enabled()) {
-      constexpr size_t sm = sizeof(cx<double>) * (size_t)N * kW64dStride;
+      const size_t sm = plane_lds_capped(sizeof(cx<double>) * (size_t)N * kW64dStride, cap);
+      if (hipError_t e = plane_lds_attr<&k_plane_w64d>(); e != hipSuccess) return e;
       hipLaunchKernelGGL(k_plane_w64d, dim3(m.nr3, nb), dim3(256), sm, s, m,
                          reinterpret_cast<const cx<double>*>(tw1), reinterpret_cast<const double*>(V),
                          reinterpret_cast<cx<double>*>(W));
@@ -1625,11 +1648,8 @@ static hipError_t launch_plane_n(const SphereMaps& m, const void* tw1, const voi
   }
   if constexpr (N == 64) {
     if (m.band_xy && w64_on) {
-      constexpr size_t sm = sizeof(cx<T>) * (size_t)N * kW64Stride;
-      static const hipError_t attr = sm > 65536 ? hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_w64<T>),
-                                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm)
-                                                : hipSuccess;
-      if (attr != hipSuccess) return attr;
+      const size_t sm = plane_lds_capped(sizeof(cx<T>) * (size_t)N * kW64Stride, cap);
+      if (hipError_t e = plane_lds_attr<&k_plane_w64<T>>(); e != hipSuccess) return e;
       hipLaunchKernelGGL((k_plane_w64<T>), dim3(m.nr3, nb), dim3(512), sm, s, m, static_cast<const cx<T>*>(tw1),
                          V, W);
       return hipGetLastError();
@@ -1637,10 +1657,8 @@ static hipError_t launch_plane_n(const SphereMaps& m, const void* tw1, const voi
   }
   if constexpr (N == 96 && sizeof(T) == 4) {
     if (m.band_xy && w96_enabled()) {
-      constexpr size_t sm = sizeof(cx<T>) * (size_t)N * kW96Stride;
-      static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_w96<T>),
-                                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-      if (attr != hipSuccess) return attr;
+      const size_t sm = plane_lds_capped(sizeof(cx<T>) * (size_t)N * kW96Stride, cap);
+      if (hipError_t e = plane_lds_attr<&k_plane_w96<T>>(); e != hipSuccess) return e;
       hipLaunchKernelGGL((k_plane_w96<T>), dim3(m.nr3, nb), dim3(768), sm, s, m, static_cast<const cx<T>*>(tw1),
                          V, W);
       return hipGetLastError();
@@ -1648,10 +1666,8 @@ static hipError_t launch_plane_n(const SphereMaps& m, const void* tw1, const voi
   }
   if constexpr (N == 96 && sizeof(T) == 8) {
     if (m.band_xy && w96_enabled()) {
-      constexpr size_t sm = sizeof(cx<double>) * (size_t)N * kW96dStride;
-      static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_w96d),
-                                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
-      if (attr != hipSuccess) return attr;
+      const size_t sm = plane_lds_capped(sizeof(cx<double>) * (size_t)N * kW96dStride, cap);
+      if (hipError_t e = plane_lds_attr<&k_plane_w96d>(); e != hipSuccess) return e;
       hipLaunchKernelGGL(k_plane_w96d, dim3(m.nr3, nb), dim3(384), sm, s, m,
                          reinterpret_cast<const cx<double>*>(tw1), reinterpret_cast<const double*>(V),
                          reinterpret_cast<cx<double>*>(W));
@@ -1662,14 +1678,8 @@ static hipError_t launch_plane_n(const SphereMaps& m, const void* tw1, const voi
   if constexpr (N % 4 == 0 && band_plane_smem<T, N>() <= 160 * 1024) {
     if (m.band_xy && band_on) {
       constexpr int NT = band_plane_threads<N>();
-      constexpr size_t bsm = band_plane_smem<T, N>();
-      static bool battr = false;
-      if (!battr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_plane_band<T, N, NT>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bsm);
-        if (e != hipSuccess) return e;
-        battr = true;
-      }
+      const size_t bsm = plane_lds_capped(band_plane_smem<T, N>(), cap);
+      if (hipError_t e = plane_lds_attr<&k_plane_band<T, N, NT>>(); e != hipSuccess) return e;
       dim3 grid(m.nr3, nb);
       hipLaunchKernelGGL((k_plane_band<T, N, NT>), grid, dim3(NT), bsm, s, m,
                          static_cast<const cx<T>*>(tw1), V, W);
@@ -1735,12 +1745,12 @@ hipError_t launch_zinv(const SphereMaps& m, const void* tw3, const cx<T>* X, int
 
 template <typename T>
 hipError_t launch_plane(const SphereMaps& m, const void* tw1, const void* tw2, const T* V, cx<T>* W,
-                        int nb, cx<T>* Y, int chunk, hipStream_t s) {
+                        int nb, cx<T>* Y, int chunk, hipStream_t s, int cap) {
   if (m.path != FFT_COMPILED || !plane_compiled_fits(m, sizeof(T) == sizeof(double)))
     return launch_plane_rt<T>(m, tw1, tw2, V, W, nb, Y, chunk, s);
   switch (m.nr1) {
 #define PWH_CASE(N) \
-  case N: return launch_plane_n<T, N>(m, tw1, tw2, V, W, nb, s);
+  case N: return launch_plane_n<T, N>(m, tw1, tw2, V, W, nb, s, cap);
     PWH_FFT_SIZES(PWH_CASE)
 #undef PWH_CASE
     default: return hipErrorInvalidValue;
@@ -2127,9 +2137,9 @@ template hipError_t launch_zinv<double>(const SphereMaps&, const void*, const cx
 template hipError_t launch_zinv<float>(const SphereMaps&, const void*, const cx<float>*, int,
                                        cx<float>*, int, hipStream_t);
 template hipError_t launch_plane<double>(const SphereMaps&, const void*, const void*, const double*,
-                                         cx<double>*, int, cx<double>*, int, hipStream_t);
+                                         cx<double>*, int, cx<double>*, int, hipStream_t, int);
 template hipError_t launch_plane<float>(const SphereMaps&, const void*, const void*, const float*,
-                                        cx<float>*, int, cx<float>*, int, hipStream_t);
+                                        cx<float>*, int, cx<float>*, int, hipStream_t, int);
 template hipError_t launch_zfwd_epilogue<double>(const SphereMaps&, const void*, const cx<double>*,
                                                  const double*, const cx<double>*,
                                                  const cx<double>*, const cx<double>*, cx<double>*,

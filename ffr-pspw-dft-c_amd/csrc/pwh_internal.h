@@ -105,10 +105,12 @@ template <typename T>
 hipError_t launch_zinv(const SphereMaps& m, const void* tw3, const cx<T>* X, int ldx, cx<T>* W,
                        int nb, hipStream_t s);
 // Y: the split pass's intermediate (plane_split_elems(m, chunk) elements, chunk =
-// plane_split_chunk(...)); unused by the other paths (may be null there).
+// plane_split_chunk(...)); unused by the other paths (may be null there).  cap: at most that
+// many workgroups of a compiled plane kernel per CU (0: no cap), so the nonlocal GEMMs of
+// the second stream find room beside them.
 template <typename T>
 hipError_t launch_plane(const SphereMaps& m, const void* tw1, const void* tw2, const T* V, cx<T>* W,
-                        int nb, cx<T>* Y, int chunk, hipStream_t s);
+                        int nb, cx<T>* Y, int chunk, hipStream_t s, int cap = 0);
 template <typename T>
 hipError_t launch_zfwd_epilogue(const SphereMaps& m, const void* tw3, const cx<T>* W, const T* kin,
                                 const cx<T>* X, const cx<T>* Hnl, const cx<T>* Z, cx<T>* out,

@@ -46,7 +46,7 @@ EXPORTS = [
     "pwh_kerker_prec", "pwh_mix_anderson", "pwh_mix_pulay", "pwh_mix_broyden", "pwh_eval_enl",
     "pwh_phase_factor_gpu", "pwh_structure_factor", "pwh_nl_form_factor", "pwh_form_factor",
     "pwh_resident_chfsi", "pwh_resident_eval_rho_psi", "pwh_resident_eval_enl",
-    "pwh_eig_zheevd_noalloc", "pwh_fft_path", "pwh_projector_mode", "pwh_tuning_get",
+    "pwh_eig_zheevd_noalloc", "pwh_fft_path", "pwh_projector_mode", "pwh_tuning_get", "pwh_tuning_set",
     "pwh_scf_state_create", "pwh_scf_state_destroy", "pwh_scf_state_set_rho", "pwh_scf_state_get",
     "pwh_scf_state_density", "pwh_scf_state_mix", "pwh_scf_state_energy", "pwh_scf_state_set_vloc",
 ]
@@ -156,6 +156,7 @@ def lib():
         "pwh_autotune": [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)],
         "pwh_projector_mode": [vp, C.c_int, C.c_void_p, C.c_void_p],
         "pwh_tuning_get": [vp, C.c_int, C.c_void_p],
+        "pwh_tuning_set": [vp, C.c_int, C.c_void_p],
         "pwh_resident_chebyfilt": [vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int],
         "pwh_synchronize": [],
         "pwh_timing_enable": [C.c_int],
@@ -626,10 +627,20 @@ class System:
 
     def tuning(self, precision=FP64):
         """pwh_tuning_get: the settings in use for a precision."""
-        out = np.zeros(4, np.int32)
+        out = np.zeros(5, np.int32)
         _check(lib().pwh_tuning_get(self.h, _prec(precision), out.ctypes.data))
         return {"target_blocks": int(out[0]), "proj_stage_depth": int(out[1]), "expand_stage_depth": int(out[2]),
-                "fft_plan_variant": int(out[3])}
+                "fft_plan_variant": int(out[3]), "plane_blocks_per_cu": int(out[4])}
+
+    def set_tuning(self, precision, settings):
+        """pwh_tuning_set: sets the settings tuning() reports (a dict with its keys; missing
+        keys keep the value in use), without timing."""
+        cur = self.tuning(precision)
+        cur.update(settings)
+        keys = ("target_blocks", "proj_stage_depth", "expand_stage_depth", "fft_plan_variant",
+                "plane_blocks_per_cu")
+        a = np.array([cur[k] for k in keys], np.int32)
+        _check(lib().pwh_tuning_set(self.h, _prec(precision), a.ctypes.data))
 
     def autotune(self, nstates, precision=FP64, target=0):
         """Tunes H|psi> of the first nstates resident states in this precision (pwh_autotune:

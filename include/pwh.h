@@ -347,8 +347,9 @@ int pwh_resident_apply(pwh_system *sys, int nstates, int precision);
  * knob, the fastest setting (the default unless another is faster by more than 1%): the
  * projections' split-K block target (256, 512, 1024, 2048; default 512), the LDS stage depth
  * of the projection and of the expansion GEMM (16, 32, and 64 for FP32; default 16; FP64 and
- * FP32 only), and, for grids off the compiled kernels, the radix order of the runtime FFT
- * plans (pwh_tuning_get reports all of them).  It returns the target in *target_blocks and
+ * FP32 only), the plane-pass workgroups per CU while the nonlocal GEMMs overlap the FFT
+ * passes (no cap, 3, 2, 1; default no cap), and, for grids off the compiled kernels, the
+ * radix order of the runtime FFT plans (pwh_tuning_get reports all of them).  It returns the target in *target_blocks and
  * the time per H|psi> in *best_ms (both optional).  With *target_blocks in [64, 65536] it
  * sets that target without timing (nstates unused), e.g. one found by an earlier run.
  * Every later H|psi> of the system in that precision uses the settings.  No counterpart in
@@ -358,8 +359,13 @@ int pwh_autotune(pwh_system *sys, int nstates, int precision, int *target_blocks
 /* No counterpart in the reference: the settings pwh_autotune chose (or the defaults) for a
  * precision: out[0] the projections' split-K block target, out[1] / out[2] the LDS stage
  * depth of the projection / expansion GEMMs, out[3] the radix-order variant of the runtime
- * FFT plans (-1: the grid runs on the compiled per-length kernels). */
-int pwh_tuning_get(pwh_system *sys, int precision, int out[4]);
+ * FFT plans (-1: the grid runs on the compiled per-length kernels), out[4] the cap on
+ * plane-pass workgroups per CU beside the overlapping GEMMs (0: none). */
+int pwh_tuning_get(pwh_system *sys, int precision, int out[5]);
+/* No counterpart in the reference: sets the five settings pwh_tuning_get reports, without
+ * timing (e.g. those of an earlier pwh_autotune run, so a profiled run repeats a tuned one).
+ * 0 selects a knob's default; out-of-range values return PWH_ERR_ARG and change nothing. */
+int pwh_tuning_set(pwh_system *sys, int precision, const int in[5]);
 /* Chebyshev filter of the resident block (FP32 modes convert in and out once). */
 int pwh_resident_chebyfilt(pwh_system *sys, int nstates, int degree, double lb, double ub,
                            int precision);

@@ -153,6 +153,44 @@ def test_autotune_targets_keep_results(oracle):
     g.close()
 
 
+def test_plane_cap_and_tuning_set(oracle):
+    """Capping the plane-pass workgroups per CU (pwh_tuning_set, the fifth knob of
+    pwh_autotune) only pads their LDS: at the Si64 shape, where the nonlocal GEMMs overlap the
+    FFT passes, H|psi> is bitwise equal for every cap in FP64 and FP32, and still matches the
+    oracle; pwh_tuning_get reads back what was set; out-of-range settings are refused and
+    change nothing."""
+    import pwh
+    from pwh import synth
+    s = synth.config("si64")
+    pwh.init(0)
+    g = pwh.System(s.globals())
+    g.set_vloc(s.vloc)
+    psi = s.psi(128, seed=13)
+    base = {p: g.apply_ham_block(psi, p) for p in ("fp64", "fp32")}
+    for cap in (3, 2, 1):
+        for p in ("fp64", "fp32"):
+            g.set_tuning(p, {"plane_blocks_per_cu": cap})
+            assert g.tuning(p)["plane_blocks_per_cu"] == cap
+            assert np.array_equal(g.apply_ham_block(psi, p), base[p]), (cap, p)
+    idx = [0, 127]
+    ref = oracle.apply_ham_block(oracle_globals(s), psi[idx], s.vloc, s.ngw, nthreads=8)
+    assert band_relerr(base["fp64"][idx], ref) < 1e-12
+    g.set_tuning("fp32", {"target_blocks": 1024, "proj_stage_depth": 32, "expand_stage_depth": 64,
+                          "plane_blocks_per_cu": 3})
+    t = g.tuning("fp32")
+    assert (t["target_blocks"], t["proj_stage_depth"], t["expand_stage_depth"], t["plane_blocks_per_cu"]) == \
+        (1024, 32, 64, 3)
+    assert band_relerr(g.apply_ham_block(psi, "fp32")[idx], ref) < 2e-6
+    for p, bad in (("fp32", {"plane_blocks_per_cu": 4}), ("fp64", {"expand_stage_depth": 64}),
+                   ("fp32", {"target_blocks": 32}), ("bf16x3", {"proj_stage_depth": 16})):
+        before = g.tuning(p)
+        with pytest.raises(pwh.PwhError) as e:
+            g.set_tuning(p, bad)
+        assert e.value.code == pwh.PWH_ERR_ARG
+        assert g.tuning(p) == before
+    g.close()
+
+
 def test_projectors_on_the_fly_match_stored(si8, oracle):
     """pwh_projector_mode 1: the FP32 GEMMs generate beta from the factorised phases and
     PKG_A; H|psi> agrees with the stored-projector result to FP32 rounding and with the
@@ -192,7 +230,7 @@ def test_autotune_full_search_keeps_results(si8, oracle):
     ref = oracle.apply_ham_block(oracle_globals(si8), psi, si8.vloc, si8.ngw)
     g.resident_set(psi)
     assert g.tuning("fp64") == {"target_blocks": 512, "proj_stage_depth": 16, "expand_stage_depth": 16,
-                                "fft_plan_variant": -1}
+                                "fft_plan_variant": -1, "plane_blocks_per_cu": 0}
     assert g.fft_path("runtime") == "runtime"
     for p, tol, depths in (("fp64", 1e-12, (16, 32)), ("fp32", 2e-6, (16, 32, 64))):
         t, ms = g.autotune(16, p)
@@ -200,6 +238,7 @@ def test_autotune_full_search_keeps_results(si8, oracle):
         assert tu["target_blocks"] == t and t in (256, 512, 1024, 2048) and ms > 0
         assert tu["proj_stage_depth"] in depths and tu["expand_stage_depth"] in depths
         assert tu["fft_plan_variant"] in (0, 1, 2)
+        assert tu["plane_blocks_per_cu"] in (0, 1, 2, 3)
         assert band_relerr(g.apply_ham_block(psi, p), ref) < tol, (p, tu)
     assert g.fft_path("auto") == "compiled"
     assert g.tuning("fp64")["fft_plan_variant"] == -1
