@@ -102,9 +102,9 @@ def gemm_flops(s, nb):
 
 
 # MFMA flops issued per complex multiply-add of the projector GEMMs, and the peak they
-# issue against: FP64 four real products, FP32 three (3M), BF16X3 / BF16X6 four real
+# issue against: FP64 and FP32 three real products (3M), BF16X3 / BF16X6 four real
 # products of three / six bf16 split products each
-_MFMA_ISSUE = {"fp64": (8, FP64_PEAK_TFLOPS), "fp32": (6, FP32_PEAK_TFLOPS),
+_MFMA_ISSUE = {"fp64": (6, FP64_PEAK_TFLOPS), "fp32": (6, FP32_PEAK_TFLOPS),
                "bf16x3": (24, BF16_PEAK_TFLOPS), "bf16x6": (48, BF16_PEAK_TFLOPS)}
 
 
