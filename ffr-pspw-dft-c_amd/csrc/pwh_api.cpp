@@ -740,7 +740,7 @@ int sys_cholqr2(pwh_system* s, cx<double>* X, int nb, bool span_only = false) {
   const size_t n = (size_t)nb * s->ngw_pad;
   CHK(s->A64.ensure(n));
   CHK(s->S.ensure((size_t)nb * nb));
-  const size_t pe = proj_part_elems(nb, nb, s->ngw);
+  const size_t pe = std::max(proj_part_elems(nb, nb, s->ngw), gram_part_elems(nb, s->ngw));
   CHK(s->part64.ensure(pe));
   CHK(s->info.ensure(2));
   const bool series = span_only && cholqr_series_on();

@@ -50,4 +50,6 @@ hipError_t resid_norms(const cx<double>* H, const cx<double>* X, const double* l
                        int nb, double* res, hipStream_t s);
 // split-K partial buffer (elements) of a projection; target: block target (0: default)
 size_t proj_part_elems(int M, int N, int K, int target = 0);
+// partial-sum elements the upper-triangle Gram product of N columns splits into
+size_t gram_part_elems(int N, int K);
 }  // namespace pwh

@@ -269,17 +269,21 @@ hipError_t zero_lower(cx<double>* A, int n, int ld, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Column-major n x n, ld = n, one workgroup: G <- G - I in place, maxabs[0] = the largest
-// |Re| or |Im| of the result (infinite when an entry is not a number)
+// Column-major n x n, ld = n: G <- G - I in place, maxabs[0] = the largest |e_ij| of the
+// result (infinite when an entry is not a number).  One workgroup per column; the column
+// maxima meet in an atomic max on the bit patterns (non-negative doubles order as their
+// bits, and infinity above every finite value), so the result does not depend on the order.
+__global__ void k_zero_max(double* maxabs) { maxabs[0] = 0.0; }
+
 __global__ __launch_bounds__(256) void k_eye_residual(cx<double>* G, int n, double* maxabs) {
   __shared__ double red[256];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, j = blockIdx.x;
   double m = 0.0;
-  for (int e = tid; e < n * n; e += 256) {
-    cx<double> v = G[e];
-    if (e % (n + 1) == 0) {
+  for (int i = tid; i < n; i += 256) {
+    cx<double> v = G[i + (size_t)j * n];
+    if (i == j) {
       v.x -= 1.0;
-      G[e] = v;
+      G[i + (size_t)j * n] = v;
     }
     // |e_ij| (the modulus, not the larger component), so n max|e_ij| bounds ||E||_2
     const double a = hypot(v.x, v.y);
@@ -291,10 +295,12 @@ __global__ __launch_bounds__(256) void k_eye_residual(cx<double>* G, int n, doub
     if (tid < h) red[tid] = fmax(red[tid], red[tid + h]);
     __syncthreads();
   }
-  if (tid == 0) maxabs[0] = red[0];
+  if (tid == 0)
+    atomicMax(reinterpret_cast<unsigned long long*>(maxabs), (unsigned long long)__double_as_longlong(red[0]));
 }
 hipError_t eye_residual(cx<double>* G, int n, double* maxabs, hipStream_t s) {
-  hipLaunchKernelGGL(k_eye_residual, dim3(1), dim3(256), 0, s, G, n, maxabs);
+  hipLaunchKernelGGL(k_zero_max, dim3(1), dim3(1), 0, s, maxabs);
+  hipLaunchKernelGGL(k_eye_residual, dim3(n), dim3(256), 0, s, G, n, maxabs);
   return hipGetLastError();
 }
 

@@ -117,7 +117,9 @@ __device__ __forceinline__ void gemm_proj_block(RowLoader<T, BETA> ld, const cx<
   Row* sBr = sAi + KS;
   Row* sBi = sBr + KS;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int tiles_n = (N + BT - 1) / BT, tiles = ((M + BT - 1) / BT) * tiles_n;
+  // UPPER (M == N): the grid enumerates only the tiles on or above the diagonal
+  const int tiles_n = (N + BT - 1) / BT;
+  const int tiles = UPPER ? tiles_n * (tiles_n + 1) / 2 : ((M + BT - 1) / BT) * tiles_n;
   // XCD-aware order: workgroup L of the 1D grid runs on XCD L % 8 (round-robin dispatch).
   // The tiles of one K-split read the same chunk of both operands, so they get the same
   // L % 8 and neighbouring L: the re-reads hit that XCD's L2.  Splits past the last exit.
@@ -127,8 +129,18 @@ __device__ __forceinline__ void gemm_proj_block(RowLoader<T, BETA> ld, const cx<
   const int tile = xcd ? q % tiles : L % tiles;
   const int split = xcd ? (L & 7) + 8 * (q / tiles) : L / tiles;
   if (split * kchunk >= K) return;
-  const int m0 = (tile / tiles_n) * BT, n0 = (tile % tiles_n) * BT;
-  if (UPPER && m0 >= n0 + BT) return;
+  int mt = tile / tiles_n, nt = tile % tiles_n;
+  if (UPPER) {
+    // upper tile t, rows of tiles in order: row r holds tiles_n - r tiles (r .. tiles_n - 1)
+    int r = 0, t = tile;
+    while (t >= tiles_n - r) {
+      t -= tiles_n - r;
+      ++r;
+    }
+    mt = r;
+    nt = r + t;
+  }
+  const int m0 = mt * BT, n0 = nt * BT;
   const int ks = split * kchunk, ke = min(K, ks + kchunk);
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int li = lane & 15, lq = lane >> 4;
@@ -1077,10 +1089,29 @@ hipError_t gemm_cn_f64(const cx<double>* A, int lda, int M, const cx<double>* B,
   return hipGetLastError();
 }
 
+// Split of the upper-triangle Gram product: the split count comes from the tiles that
+// compute (the upper ones), not from the full square of tiles.
+static void gram_split_shape(int N, int K, size_t part_elems, int* S_out, int* kchunk_out, int* blocks) {
+  const int tn = (N + BT - 1) / BT, tiles = tn * (tn + 1) / 2;
+  int S = choose_split(tiles, K, part_elems, (size_t)N * N, 0);
+  int kchunk = (K + S - 1) / S;
+  kchunk = ((kchunk + 15) / 16) * 16;
+  S = (K + kchunk - 1) / kchunk;
+  *S_out = S;
+  *kchunk_out = kchunk;
+  *blocks = tiles * ((S + 7) / 8) * 8;
+}
+
+size_t gram_part_elems(int N, int K) {
+  int S, kchunk, blocks;
+  gram_split_shape(N, K, (size_t)-1, &S, &kchunk, &blocks);
+  return (size_t)S * N * N;
+}
+
 hipError_t gemm_gram_upper_f64(const cx<double>* A, int lda, int N, int K, cx<double>* C, cx<double>* part,
                                size_t part_elems, hipStream_t s) {
   int S, kchunk, blocks;
-  split_shape(N, N, K, part_elems, 0, 16, &S, &kchunk, &blocks);
+  gram_split_shape(N, K, part_elems, &S, &kchunk, &blocks);
   if ((size_t)S * N * N > part_elems) return hipErrorInvalidValue;
   RowLoader<double, false> ld{A, lda};
   constexpr size_t sm = gemm_smem<double, 16>();
