@@ -1721,13 +1721,18 @@ static hipError_t zinv_shape(const SphereMaps& m, const void* tw3, const cx<T>* 
   }
 }
 
-// Launch shapes measured at the 64^3 Si64 block (FP32 and FP64): the gather-and-write
-// pass runs best as one-wave blocks of 8 columns, the transform-and-epilogue pass as
-// 256-lane blocks of 32 columns (16 x 128 when that tile would exceed 32 KB of LDS).
+// Launch shapes measured at the 64^3 Si64 and 96^3 Si216 blocks (FP32 and FP64): the
+// gather-and-write pass runs best as one-wave blocks of 8 columns up to 64-point columns
+// and as two-wave blocks of 16 columns above (96: FP32 0.187 -> 0.119 ms, FP64 0.243 ->
+// 0.219 ms; the FP32 rows of 16 columns are whole 128-byte lines of W); the
+// transform-and-epilogue pass as 256-lane blocks of 32 columns (16 x 128 when that tile
+// would exceed 32 KB of LDS).  The column transforms are the same arithmetic in either
+// shape, so the result does not depend on it.
 template <typename T, int N>
 static hipError_t zinv_n(const SphereMaps& m, const void* tw3, const cx<T>* X, int ldx, cx<T>* W, int nb,
                          hipStream_t s) {
-  return zinv_shape<T, N, 8, 64>(m, tw3, X, ldx, W, nb, s);
+  if constexpr (N > 64) return zinv_shape<T, N, 16, 128>(m, tw3, X, ldx, W, nb, s);
+  else return zinv_shape<T, N, 8, 64>(m, tw3, X, ldx, W, nb, s);
 }
 
 template <typename T>
