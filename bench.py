@@ -647,16 +647,17 @@ def c4_rates(steps, cpu=True):
     out["fp32_projectors_on_the_fly"] = {"ms_per_block": round(1e3 * dt / steps, 3),
                                          "stored_ms_per_block": out["fp32"]["ms_per_block"]}
     x.projector_mode(0)
-    # ChFSI at degree 12 (FP64 orthonormalisation and Rayleigh-Ritz; rocSOLVER's Cholesky
-    # and zheevd at n = 432): five outer iterations per filter precision from the same
-    # orthonormal random start, 20-step FP64 Lanczos bounds.  The eigenvalue difference is
-    # given for all states and for the lower half (the upper states near lb are still far
-    # from converged after five iterations)
+    # ChFSI at degree 12 (FP64 orthonormalisation with rocSOLVER's Cholesky factor at n = 432,
+    # FP64 Rayleigh-Ritz with the library's Householder reduction and rocSOLVER's zstedc):
+    # five outer iterations per filter precision (FP64, FP32 and the FP32-accurate BF16X6
+    # split) from the same orthonormal random start, 20-step FP64 Lanczos bounds.  The
+    # eigenvalue difference is given for all states and for the lower half (the upper states
+    # near lb are still far from converged after five iterations)
     X0 = pwh.ortho_qr(psi)
     v0 = rng.random(s.ngw) + 1j * rng.random(s.ngw)
     x.chfsi(X0, degree=2, niter=1, precision="fp32", lanczos_k=4, v0=v0)   # first-call set-up
     eigs = {}
-    for p in ("fp64", "fp32"):
+    for p in ("fp64", "fp32", "bf16x6"):
         x.resident_set(X0)          # the block is in HBM before the timed loop starts
         eig, res, st = x.resident_chfsi(s.nbands, degree=12, niter=5, precision=p, lanczos_k=20, v0=v0)
         eigs[p] = eig
@@ -668,6 +669,9 @@ def c4_rates(steps, cpu=True):
     d = np.abs(eigs["fp32"][:nx] - eigs["fp64"][:nx])
     out["chfsi_fp32_vs_fp64_after_5_iterations"] = {"max_abs_eig_diff_Ha": float(d.max()),
                                                     "lower_half_max_abs_eig_diff_Ha": float(d[:nx // 2].max())}
+    d = np.abs(eigs["bf16x6"][:nx] - eigs["fp64"][:nx])
+    out["chfsi_bf16x6_vs_fp64_after_5_iterations"] = {"max_abs_eig_diff_Ha": float(d.max()),
+                                                      "lower_half_max_abs_eig_diff_Ha": float(d[:nx // 2].max())}
     x.close()
     if cpu:
         out["cpu_baselines"] = cpu_baselines(s, psi, "the si216 block", seconds=6.0)
