@@ -24,7 +24,6 @@
 // values at a wave-uniform row are read with v_readlane (no LDS round trips).
 #include <cfloat>
 #include <type_traits>
-#include <vector>
 
 #include "pwh_internal.h"
 
@@ -382,37 +381,6 @@ hipError_t tridiag_backtransform(const cx<double>* work, const cx<double>* tau, 
   return hipGetLastError();
 }
 
-// The n - 1 step launches of a reduction, captured once per (device, operands) into a HIP
-// graph and replayed: a replay dispatches the same kernels with less host work per launch
-// than n - 1 stream launches.  PWH_TRD_GRAPH=0 launches them on the stream instead.
-static bool trd_graph_on() {
-  static const bool on = [] {
-    const char* e = getenv("PWH_TRD_GRAPH");
-    return !(e && e[0] == '0');
-  }();
-  return on;
-}
-
-struct TrdGraph {
-  int dev, n;
-  const void *A, *work, *d, *e, *tau;
-  hipGraphExec_t exec;
-};
-
-static hipError_t trd_record(const TrdWork& t, const cx<double>* A, hipStream_t s) {
-  const int n = t.n;
-  hipError_t err = hipMemcpyAsync(t.A, A, sizeof(cx<double>) * n * n, hipMemcpyDeviceToDevice, s);
-  if (err != hipSuccess) return err;
-  with_rows(n, [&](auto rc) {
-    for (int k = 0; k <= n - 2; ++k) {
-      const int blocks = (n - k - 1 + kTrdWaves - 1) / kTrdWaves;
-      hipLaunchKernelGGL(k_trd_step<decltype(rc)::value>, dim3(blocks), dim3(64 * kTrdWaves), 0, s, t, k);
-    }
-  });
-  hipLaunchKernelGGL(k_trd_last, dim3(1), dim3(64), 0, s, t);
-  return hipGetLastError();
-}
-
 hipError_t tridiag_householder(const cx<double>* A, int n, cx<double>* work, double* d, double* e, cx<double>* tau,
                                hipStream_t s) {
   if (n < 3 || n > kTrdMaxN) return hipErrorInvalidValue;
@@ -425,35 +393,16 @@ hipError_t tridiag_householder(const cx<double>* A, int n, cx<double>* work, dou
   t.tau = tau;
   t.d = d;
   t.e = e;
-  if (!trd_graph_on()) return trd_record(t, A, s);
-  // a few graphs (one per system buffer set), the oldest dropped first
-  static std::vector<TrdGraph> cache;
-  static hipStream_t cap = nullptr;
-  int dev = 0;
-  hipError_t err = hipGetDevice(&dev);
+  hipError_t err = hipMemcpyAsync(t.A, A, sizeof(cx<double>) * n * n, hipMemcpyDeviceToDevice, s);
   if (err != hipSuccess) return err;
-  for (const TrdGraph& gr : cache)
-    if (gr.dev == dev && gr.n == n && gr.A == A && gr.work == work && gr.d == d && gr.e == e && gr.tau == tau)
-      return hipGraphLaunch(gr.exec, s);
-  if (!cap && (err = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking)) != hipSuccess) return err;
-  if ((err = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal)) != hipSuccess) return err;
-  const hipError_t rec = trd_record(t, A, cap);
-  hipGraph_t graph = nullptr;
-  err = hipStreamEndCapture(cap, &graph);
-  if (rec != hipSuccess || err != hipSuccess) {
-    if (graph) hipGraphDestroy(graph);
-    return rec != hipSuccess ? rec : err;
-  }
-  hipGraphExec_t exec = nullptr;
-  err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  hipGraphDestroy(graph);
-  if (err != hipSuccess) return err;
-  if (cache.size() >= 8) {
-    hipGraphExecDestroy(cache.front().exec);
-    cache.erase(cache.begin());
-  }
-  cache.push_back({dev, n, A, work, d, e, tau, exec});
-  return hipGraphLaunch(exec, s);
+  with_rows(n, [&](auto rc) {
+    for (int k = 0; k <= n - 2; ++k) {
+      const int blocks = (n - k - 1 + kTrdWaves - 1) / kTrdWaves;
+      hipLaunchKernelGGL(k_trd_step<decltype(rc)::value>, dim3(blocks), dim3(64 * kTrdWaves), 0, s, t, k);
+    }
+  });
+  hipLaunchKernelGGL(k_trd_last, dim3(1), dim3(64), 0, s, t);
+  return hipGetLastError();
 }
 
 }  // namespace pwh
