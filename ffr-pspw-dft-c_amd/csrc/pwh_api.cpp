@@ -3350,15 +3350,58 @@ int pwh_autotune(pwh_system* s, int nstates, int precision, int* target_blocks, 
   // 1. split-K block target of the projections (one to eight blocks per CU): its best
   //    value moves with the band count, the projector count and the precision
   if (rc == PWH_OK) rc = search(&s->tune[precision], {512, 256, 1024, 2048}, &ms);
-  // 2. the LDS stage depth of the projection and of the expansion GEMM (FP64 / FP32;
-  //    the BF16 splits have one)
-  if (rc == PWH_OK && precision <= PWH_FP32) {
-    const std::vector<int> ks = precision == PWH_FP32 ? std::vector<int>{16, 32, 64} : std::vector<int>{16, 32};
-    for (int kind = 0; kind < 2 && rc == PWH_OK; ++kind) rc = search(&s->gks[precision][kind], ks, &ms);
+  // 2. jointly: the LDS stage depth of the projection and of the expansion GEMM (FP64 /
+  //    FP32; the BF16 splits have one) and the plane-pass workgroups per CU beside the
+  //    overlapping nonlocal GEMMs (no cap, 3, 2, 1; fewer plane workgroups leave LDS and wave
+  //    slots to GEMM workgroups).  The two interact (a deeper stage takes more of the LDS the
+  //    cap frees), so every combination is timed; the default combination stays unless
+  //    another is faster by more than 1 %
+  if (rc == PWH_OK) {
+    std::vector<int> ks = {16};
+    if (precision == PWH_FP32) ks = {16, 32, 64};
+    else if (precision == PWH_FP64) ks = {16, 32};
+    const std::vector<int> caps = {0, 3, 2, 1};
+    int* kp = &s->gks[precision][0];
+    int* ke = &s->gks[precision][1];
+    int* cp = &s->pcap[precision];
+    const bool depths = precision <= PWH_FP32;
+    double best = 1e300;
+    int bk = 0, be = 0, bc = 0;
+    for (size_t ia = 0; ia < ks.size() && rc == PWH_OK; ++ia)
+      for (size_t ib = 0; ib < ks.size() && rc == PWH_OK; ++ib)
+        for (size_t ic = 0; ic < caps.size() && rc == PWH_OK; ++ic) {
+          const int a = ks[ia], b = ks[ib], c = caps[ic];
+          if (depths) {
+            *kp = a;
+            *ke = b;
+          }
+          *cp = c;
+          ++g.layout_gen;
+          double t = 1e300;
+          if ((rc = time_apply(&t)) != PWH_OK) break;
+          const bool first = best == 1e300;
+          if (first || t < 0.99 * best) {
+            best = t;
+            bk = a;
+            be = b;
+            bc = c;
+          }
+        }
+    if (depths) {
+      *kp = bk;
+      *ke = be;
+    }
+    *cp = bc;
+    ++g.layout_gen;
+    ms = best;
+    // 3. the split-K target once more, at the chosen depths and cap (the current value first)
+    if (rc == PWH_OK) {
+      std::vector<int> cand = {s->tune[precision]};
+      for (int t : {256, 512, 1024, 2048})
+        if (t != cand[0]) cand.push_back(t);
+      rc = search(&s->tune[precision], cand, &ms);
+    }
   }
-  // 3. plane-pass workgroups per CU beside the overlapping nonlocal GEMMs (no cap, 3, 2, 1):
-  //    fewer plane workgroups leave LDS and wave slots to GEMM workgroups on the same CU
-  if (rc == PWH_OK) rc = search(&s->pcap[precision], {0, 3, 2, 1}, &ms);
   // 4. the radix order of the runtime FFT plans, for grids off the compiled kernels
   if (rc == PWH_OK && s->maps.path != FFT_COMPILED) {
     int v = s->fftv;

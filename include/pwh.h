@@ -343,13 +343,14 @@ int pwh_resident_get(pwh_system *sys, double *X, int nstates);
 /* One H|psi> over the resident block (nb bands) in the given precision. */
 int pwh_resident_apply(pwh_system *sys, int nstates, int precision);
 /* Launch shapes of H|psi> for this system and precision.  With *target_blocks == 0 (or a
- * null pointer) it times H|psi> of the resident block (nstates bands) and keeps, knob by
- * knob, the fastest setting (the default unless another is faster by more than 1%): the
- * projections' split-K block target (256, 512, 1024, 2048; default 512), the LDS stage depth
- * of the projection and of the expansion GEMM (16, 32, and 64 for FP32; default 16; FP64 and
- * FP32 only), the plane-pass workgroups per CU while the nonlocal GEMMs overlap the FFT
- * passes (no cap, 3, 2, 1; default no cap), and, for grids off the compiled kernels, the
- * radix order of the runtime FFT plans (pwh_tuning_get reports all of them).  It returns the target in *target_blocks and
+ * null pointer) it times H|psi> of the resident block (nstates bands) and keeps the fastest
+ * setting (the default unless another is faster by more than 1%) of: the projections'
+ * split-K block target (256, 512, 1024, 2048; default 512); then, every combination timed,
+ * the LDS stage depth of the projection and of the expansion GEMM (16, 32, and 64 for FP32;
+ * default 16; FP64 and FP32 only) with the plane-pass workgroups per CU while the nonlocal
+ * GEMMs overlap the FFT passes (no cap, 3, 2, 1; default no cap); the split-K target once
+ * more at those; and, for grids off the compiled kernels, the radix order of the runtime FFT
+ * plans (pwh_tuning_get reports all of them).  It returns the target in *target_blocks and
  * the time per H|psi> in *best_ms (both optional).  With *target_blocks in [64, 65536] it
  * sets that target without timing (nstates unused), e.g. one found by an earlier run.
  * Every later H|psi> of the system in that precision uses the settings.  No counterpart in
