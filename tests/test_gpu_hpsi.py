@@ -218,8 +218,9 @@ def test_projectors_on_the_fly_match_stored(si8, oracle):
 
 
 def test_autotune_full_search_keeps_results(si8, oracle):
-    """pwh_autotune searches the split-K target, both GEMMs' LDS stage depths and (on the
-    runtime FFT path) the plans' radix order; whatever it keeps, H|psi> still matches the
+    """pwh_autotune searches the split-K target, both GEMMs' LDS stage depths jointly with
+    the plane-pass cap, the target again and (on the runtime FFT path) the plans' radix
+    order; whatever it keeps, H|psi> still matches the
     oracle (FP64 1e-12, FP32 2e-6), and pwh_tuning_get reports settings from the
     candidate sets."""
     import pwh
