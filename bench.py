@@ -717,8 +717,8 @@ def main():
         print(json.dumps({"copy_calib_bytes_per_launch_read": 1 << 30, "write": 1 << 30}))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(multi-GPU sharding is described in DESIGN.md, not implemented).", file=sys.stderr)
+        print(f"bench.py: this library runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(sharding bands and G-vectors over GPUs is designed, not implemented).", file=sys.stderr)
         sys.exit(2)
 
     if args.section:
